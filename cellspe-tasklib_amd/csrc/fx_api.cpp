@@ -1,0 +1,808 @@
+// fx_api.cpp -- host side of the F-X engine C-ABI (include/cellspe_fx.h).
+//
+// Owns the device context (stream, FP64-derived twiddle table, window
+// tables), argument checking, chunking of the correlator time axis and the
+// phase-cal extractor state.  No compute runs on the host: every entry point
+// enqueues the gfx950 kernels of fx_fft.hip, fx_xengine.hip and fx_misc.hip.
+#include "fx_internal.h"
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <vector>
+
+// ---------------------------------------------------------------------------
+// error state
+// ---------------------------------------------------------------------------
+static thread_local char g_err[512] = "";
+
+void fx_set_error(const char* fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+}
+
+extern "C" const char* fx_last_error(void) { return g_err; }
+
+#define FX_CHECK_LAUNCH(rc)                                                     \
+    do {                                                                        \
+        int _rc = (rc);                                                         \
+        if (_rc != FX_OK) {                                                     \
+            if (_rc == FX_ERR_HIP)                                              \
+                fx_set_error("%s:%d: kernel launch failed: %s", __FILE__,       \
+                             __LINE__, hipGetErrorString(hipGetLastError()));   \
+            else if (!g_err[0])                                                 \
+                fx_set_error("%s:%d: unsupported size", __FILE__, __LINE__);    \
+            return _rc;                                                         \
+        }                                                                       \
+    } while (0)
+
+// ---------------------------------------------------------------------------
+// context
+// ---------------------------------------------------------------------------
+struct fx_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    float2* tw = nullptr;                       // FX_TW_N entries
+    std::map<std::pair<int, int>, float*> windows;
+    float* scratch = nullptr;                   // spectrometer partials
+    size_t scratch_bytes = 0;
+    std::mutex mu;
+};
+
+static int ctx_device(fx_ctx* ctx)
+{
+    FX_REQUIRE(ctx != nullptr, FX_ERR_ARG, "null context");
+    FX_HIP_TRY(hipSetDevice(ctx->device));
+    return FX_OK;
+}
+
+extern "C" int fx_device_count(void)
+{
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+extern "C" int fx_device_name(int device, char* buf, int len)
+{
+    hipDeviceProp_t p;
+    FX_HIP_TRY(hipGetDeviceProperties(&p, device));
+    snprintf(buf, (size_t)len, "%s (%s)", p.name, p.gcnArchName);
+    return FX_OK;
+}
+
+extern "C" int fx_ctx_create(int device, fx_ctx** out)
+{
+    FX_REQUIRE(out != nullptr, FX_ERR_ARG, "fx_ctx_create: null output");
+    *out = nullptr;
+    int ndev = 0;
+    FX_HIP_TRY(hipGetDeviceCount(&ndev));
+    FX_REQUIRE(device >= 0 && device < ndev, FX_ERR_STATE,
+               "fx_ctx_create: device %d not present (%d HIP devices)", device, ndev);
+    FX_HIP_TRY(hipSetDevice(device));
+    fx_ctx* c = new fx_ctx;
+    c->device = device;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        fx_set_error("fx_ctx_create: stream creation failed");
+        return FX_ERR_HIP;
+    }
+    // twiddle table derived in FP64, rounded once to FP32
+    std::vector<float2> h(FX_TW_N);
+    for (int e = 0; e < FX_TW_N; e++) {
+        double a = -2.0 * M_PI * (double)e / (double)FX_TW_N;
+        h[e] = make_float2((float)cos(a), (float)sin(a));
+    }
+    if (hipMalloc(&c->tw, sizeof(float2) * FX_TW_N) != hipSuccess ||
+        hipMemcpy(c->tw, h.data(), sizeof(float2) * FX_TW_N, hipMemcpyHostToDevice) != hipSuccess) {
+        hipStreamDestroy(c->stream);
+        delete c;
+        fx_set_error("fx_ctx_create: twiddle table upload failed");
+        return FX_ERR_NOMEM;
+    }
+    *out = c;
+    return FX_OK;
+}
+
+extern "C" int fx_ctx_destroy(fx_ctx* ctx)
+{
+    if (!ctx) return FX_OK;
+    hipSetDevice(ctx->device);
+    hipStreamSynchronize(ctx->stream);
+    for (auto& kv : ctx->windows) hipFree(kv.second);
+    if (ctx->scratch) hipFree(ctx->scratch);
+    if (ctx->tw) hipFree(ctx->tw);
+    hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return FX_OK;
+}
+
+extern "C" int fx_ctx_sync(fx_ctx* ctx)
+{
+    int rc = ctx_device(ctx);
+    if (rc) return rc;
+    FX_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return FX_OK;
+}
+
+extern "C" void* fx_ctx_stream(fx_ctx* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
+
+const float2* fx_ctx_twiddles(fx_ctx* ctx) { return ctx->tw; }
+hipStream_t fx_ctx_hip_stream(fx_ctx* ctx) { return ctx->stream; }
+
+// ---------------------------------------------------------------------------
+// memory / events
+// ---------------------------------------------------------------------------
+extern "C" int fx_malloc(fx_ctx* ctx, size_t bytes, void** dptr)
+{
+    int rc = ctx_device(ctx);
+    if (rc) return rc;
+    FX_REQUIRE(dptr != nullptr, FX_ERR_ARG, "fx_malloc: null output");
+    *dptr = nullptr;
+    if (bytes == 0) return FX_OK;
+    if (hipMalloc(dptr, bytes) != hipSuccess) {
+        hipGetLastError();
+        fx_set_error("fx_malloc: %zu bytes failed", bytes);
+        return FX_ERR_NOMEM;
+    }
+    return FX_OK;
+}
+
+extern "C" int fx_free(fx_ctx* ctx, void* dptr)
+{
+    int rc = ctx_device(ctx);
+    if (rc) return rc;
+    if (dptr) FX_HIP_TRY(hipFree(dptr));
+    return FX_OK;
+}
+
+extern "C" int fx_memcpy_h2d(fx_ctx* ctx, void* dst, const void* src, size_t bytes)
+{
+    int rc = ctx_device(ctx);
+    if (rc) return rc;
+    if (!bytes) return FX_OK;
+    FX_HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+    FX_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return FX_OK;
+}
+
+extern "C" int fx_memcpy_d2h(fx_ctx* ctx, void* dst, const void* src, size_t bytes)
+{
+    int rc = ctx_device(ctx);
+    if (rc) return rc;
+    if (!bytes) return FX_OK;
+    FX_HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    FX_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return FX_OK;
+}
+
+extern "C" int fx_memcpy_d2d(fx_ctx* ctx, void* dst, const void* src, size_t bytes)
+{
+    int rc = ctx_device(ctx);
+    if (rc) return rc;
+    if (!bytes) return FX_OK;
+    FX_HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+    return FX_OK;
+}
+
+extern "C" int fx_memset(fx_ctx* ctx, void* dst, int value, size_t bytes)
+{
+    int rc = ctx_device(ctx);
+    if (rc) return rc;
+    if (!bytes) return FX_OK;
+    FX_HIP_TRY(hipMemsetAsync(dst, value, bytes, ctx->stream));
+    return FX_OK;
+}
+
+extern "C" int fx_event_create(fx_ctx* ctx, void** ev)
+{
+    int rc = ctx_device(ctx);
+    if (rc) return rc;
+    hipEvent_t e;
+    FX_HIP_TRY(hipEventCreate(&e));
+    *ev = (void*)e;
+    return FX_OK;
+}
+
+extern "C" int fx_event_record(fx_ctx* ctx, void* ev)
+{
+    int rc = ctx_device(ctx);
+    if (rc) return rc;
+    FX_HIP_TRY(hipEventRecord((hipEvent_t)ev, ctx->stream));
+    return FX_OK;
+}
+
+extern "C" int fx_event_elapsed_ms(void* a, void* b, float* ms)
+{
+    FX_HIP_TRY(hipEventSynchronize((hipEvent_t)b));
+    FX_HIP_TRY(hipEventElapsedTime(ms, (hipEvent_t)a, (hipEvent_t)b));
+    return FX_OK;
+}
+
+extern "C" int fx_event_destroy(void* ev)
+{
+    if (ev) FX_HIP_TRY(hipEventDestroy((hipEvent_t)ev));
+    return FX_OK;
+}
+
+// ---------------------------------------------------------------------------
+// synthetic generators
+// ---------------------------------------------------------------------------
+extern "C" int fx_synth_station(fx_ctx* ctx, uint64_t seed, int station, double rho, long delay,
+                                double tone_amp, double tone_freq_norm, double thresh,
+                                size_t nsamp, uint32_t* d_words)
+{
+    int rc = ctx_device(ctx);
+    if (rc) return rc;
+    FX_REQUIRE(delay >= 0 && delay <= 64, FX_ERR_ARG, "fx_synth_station: delay %ld not in [0,64]",
+               delay);
+    FX_CHECK_LAUNCH(fxk_synth_station(ctx->stream, seed, station, rho, delay, tone_amp,
+                                      tone_freq_norm, thresh, nsamp, d_words));
+    return FX_OK;
+}
+
+static bool valid_nchan(int n) { return n == 1 || n == 2 || n == 4 || n == 8 || n == 16; }
+
+extern "C" int fx_synth_multichannel(fx_ctx* ctx, uint64_t seed, int nchan, const double* amp,
+                                     const double* freq, double thresh, size_t ntime,
+                                     uint32_t* d_words)
+{
+    int rc = ctx_device(ctx);
+    if (rc) return rc;
+    FX_REQUIRE(valid_nchan(nchan), FX_ERR_ARG, "fx_synth_multichannel: nchan %d", nchan);
+    FX_CHECK_LAUNCH(fxk_synth_multichannel(ctx->stream, seed, nchan, amp, freq, thresh, ntime, d_words));
+    return FX_OK;
+}
+
+extern "C" int fx_synth_channelized(fx_ctx* ctx, uint64_t seed, int nchan, long ntime, int ninp,
+                                    double rho, double thresh, uint8_t* d_codes)
+{
+    int rc = ctx_device(ctx);
+    if (rc) return rc;
+    FX_CHECK_LAUNCH(fxk_synth_channelized(ctx->stream, seed, nchan, ntime, ninp, rho, thresh, d_codes));
+    return FX_OK;
+}
+
+// ---------------------------------------------------------------------------
+// primitives
+// ---------------------------------------------------------------------------
+static bool is_pow2(long n) { return n > 0 && (n & (n - 1)) == 0; }
+
+extern "C" int fx_unpack2b(fx_ctx* ctx, const uint32_t* d_words, size_t ntime, int nchan, float* d_out)
+{
+    int rc = ctx_device(ctx);
+    if (rc) return rc;
+    FX_REQUIRE(valid_nchan(nchan), FX_ERR_ARG, "fx_unpack2b: nchan %d not in {1,2,4,8,16}", nchan);
+    FX_REQUIRE(nchan != 1 || ((uintptr_t)d_out % 16) == 0, FX_ERR_ARG,
+               "fx_unpack2b: output must be 16-byte aligned");
+    FX_CHECK_LAUNCH(fxk_unpack2b(ctx->stream, d_words, ntime, nchan, d_out));
+    return FX_OK;
+}
+
+extern "C" int fx_fft_c2c(fx_ctx* ctx, const fx_cf32* d_in, fx_cf32* d_out, int n, long batch)
+{
+    int rc = ctx_device(ctx);
+    if (rc) return rc;
+    FX_REQUIRE(is_pow2(n) && n >= 2 && n <= 16384, FX_ERR_ARG, "fx_fft_c2c: n=%d", n);
+    FX_REQUIRE(batch >= 0, FX_ERR_ARG, "fx_fft_c2c: batch=%ld", batch);
+    if (batch == 0) return FX_OK;
+    FX_CHECK_LAUNCH(fxk_fft_c2c(ctx->stream, ctx->tw, (const float2*)d_in, (float2*)d_out, n, batch));
+    return FX_OK;
+}
+
+extern "C" int fx_fft_r2c(fx_ctx* ctx, const float* d_in, fx_cf32* d_out, int n, long batch,
+                          int keep_nyquist)
+{
+    int rc = ctx_device(ctx);
+    if (rc) return rc;
+    FX_REQUIRE(is_pow2(n) && n >= 4 && n <= FX_MAX_N, FX_ERR_ARG, "fx_fft_r2c: n=%d", n);
+    FX_REQUIRE(d_in != (const float*)d_out, FX_ERR_ARG, "fx_fft_r2c: in-place not supported");
+    if (batch <= 0) return batch == 0 ? FX_OK : FX_ERR_ARG;
+    FX_CHECK_LAUNCH(fxk_fft_r2c(ctx->stream, ctx->tw, d_in, (float2*)d_out, n, batch, keep_nyquist));
+    return FX_OK;
+}
+
+extern "C" int fx_complex_mac(fx_ctx* ctx, const fx_cf32* a, const fx_cf32* b, fx_cf32* acc,
+                              long n, int conj)
+{
+    int rc = ctx_device(ctx);
+    if (rc) return rc;
+    FX_CHECK_LAUNCH(fxk_complex_mac(ctx->stream, (const float2*)a, (const float2*)b, (float2*)acc, n, conj));
+    return FX_OK;
+}
+
+extern "C" int fx_complex_mul(fx_ctx* ctx, const fx_cf32* a, const fx_cf32* b, fx_cf32* out, long n)
+{
+    int rc = ctx_device(ctx);
+    if (rc) return rc;
+    FX_CHECK_LAUNCH(fxk_complex_mul(ctx->stream, (const float2*)a, (const float2*)b, (float2*)out, n));
+    return FX_OK;
+}
+
+extern "C" int fx_sincos(fx_ctx* ctx, const float* arg, fx_cf32* out, long n, int normalized)
+{
+    int rc = ctx_device(ctx);
+    if (rc) return rc;
+    FX_CHECK_LAUNCH(fxk_sincos(ctx->stream, arg, (float2*)out, n, normalized));
+    return FX_OK;
+}
+
+extern "C" int fx_sincos_generator(fx_ctx* ctx, double start, double inc, fx_cf32* out, long n)
+{
+    int rc = ctx_device(ctx);
+    if (rc) return rc;
+    FX_CHECK_LAUNCH(fxk_sincos_gen(ctx->stream, start, inc, (float2*)out, n));
+    return FX_OK;
+}
+
+// ---------------------------------------------------------------------------
+// spectrometer
+// ---------------------------------------------------------------------------
+static float* get_window(fx_ctx* ctx, int kind, int n)
+{
+    auto key = std::make_pair(kind, n);
+    auto it = ctx->windows.find(key);
+    if (it != ctx->windows.end()) return it->second;
+    std::vector<float> w((size_t)n);
+    for (int i = 0; i < n; i++) {
+        double v = 1.0;
+        if (kind == FX_WIN_SIN2) {
+            double s = sin(M_PI * (double)i / (double)(n - 1));
+            v = s * s;
+        } else if (kind == FX_WIN_COSINE) {
+            v = cos(M_PI * ((double)i - 0.5 * (double)n + 0.5) / (double)n);
+        } else if (kind == FX_WIN_COSINE2) {
+            double c = cos(M_PI * ((double)i - 0.5 * (double)n + 0.5) / (double)n);
+            v = c * c;
+        }
+        w[(size_t)i] = (float)v;
+    }
+    float* d = nullptr;
+    if (hipMalloc(&d, sizeof(float) * (size_t)n) != hipSuccess) return nullptr;
+    if (hipMemcpy(d, w.data(), sizeof(float) * (size_t)n, hipMemcpyHostToDevice) != hipSuccess) {
+        hipFree(d);
+        return nullptr;
+    }
+    ctx->windows[key] = d;
+    return d;
+}
+
+static int ensure_scratch(fx_ctx* ctx, size_t bytes)
+{
+    if (ctx->scratch_bytes >= bytes) return FX_OK;
+    FX_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    if (ctx->scratch) hipFree(ctx->scratch);
+    ctx->scratch = nullptr;
+    ctx->scratch_bytes = 0;
+    if (hipMalloc(&ctx->scratch, bytes) != hipSuccess) {
+        hipGetLastError();
+        fx_set_error("scratch allocation of %zu bytes failed", bytes);
+        return FX_ERR_NOMEM;
+    }
+    ctx->scratch_bytes = bytes;
+    return FX_OK;
+}
+
+extern "C" int fx_spectrometer(fx_ctx* ctx, const uint32_t* d_words, size_t ntime, int nchan, int n,
+                               int integrate_ffts, int window, float* d_out)
+{
+    int rc = ctx_device(ctx);
+    if (rc) return rc;
+    FX_REQUIRE(valid_nchan(nchan), FX_ERR_ARG, "fx_spectrometer: nchan %d", nchan);
+    FX_REQUIRE(is_pow2(n) && n >= 256 && n <= FX_MAX_N, FX_ERR_ARG, "fx_spectrometer: n=%d", n);
+    FX_REQUIRE(integrate_ffts >= 1, FX_ERR_ARG, "fx_spectrometer: integrate_ffts=%d", integrate_ffts);
+    FX_REQUIRE(window >= FX_WIN_NONE && window <= FX_WIN_COSINE2, FX_ERR_ARG,
+               "fx_spectrometer: window %d", window);
+    long nfft = (long)(ntime / (size_t)n);
+    long nspec = nfft / integrate_ffts;
+    if (nspec == 0) return FX_OK;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    float* win = get_window(ctx, window, n);
+    FX_REQUIRE(win != nullptr, FX_ERR_NOMEM, "fx_spectrometer: window table");
+    // split each spectrum's FFTs over `groups` workgroups so the grid holds
+    // at least ~512 workgroups; each writes one partial spectrum tile
+    const int M = n / 2;
+    const int FM = (M >= 16384) ? 16384 : 8192;
+    const int F = FM / M;
+    long per_spec_units = (integrate_ffts + F - 1) / F;
+    long want = 512 / (nspec * nchan) + 1;
+    int groups = (int)std::min<long>(std::max<long>(want, 1), per_spec_units);
+    size_t part_bytes = (size_t)nspec * nchan * groups * (size_t)FM * sizeof(float);
+    rc = ensure_scratch(ctx, part_bytes);
+    if (rc) return rc;
+    FX_CHECK_LAUNCH(fxk_spectrometer(ctx->stream, ctx->tw, win, d_words, ntime, nchan, n,
+                                     integrate_ffts, (int)nspec, ctx->scratch, groups, d_out));
+    return FX_OK;
+}
+
+// ---------------------------------------------------------------------------
+// phase-cal
+// ---------------------------------------------------------------------------
+struct fx_pcal {
+    fx_ctx* ctx;
+    int kind;             // 0 trivial, 1 implicit shift, 2 shifting
+    double fs, spacing, offset;
+    int nbins, ntones, rotatorlen, nfold, step, offset_bin;
+    double dphi;
+    double* acc = nullptr;   // nfold FP64
+    double* tones = nullptr; // 2*ntones FP64
+    size_t position = 0;
+    uint64_t samplecount = 0;
+    bool finalized = false;
+};
+
+static long long gcd_ll(long a, long b)
+{
+    for (;;) {
+        a = a % b;
+        if (a == 0) return b;
+        b = b % a;
+        if (b == 0) return a;
+    }
+}
+
+extern "C" int fx_pcal_create(fx_ctx* ctx, double bandwidth_hz, double spacing_hz, int offset_hz,
+                              size_t sampleoffset, fx_pcal** out)
+{
+    int rc = ctx_device(ctx);
+    if (rc) return rc;
+    FX_REQUIRE(out && bandwidth_hz > 0 && spacing_hz > 0 && offset_hz >= 0, FX_ERR_ARG,
+               "fx_pcal_create: invalid parameters");
+    fx_pcal* p = new fx_pcal;
+    p->ctx = ctx;
+    double fs = 2.0 * bandwidth_hz;
+    p->fs = fs;
+    p->spacing = spacing_hz;
+    p->offset = offset_hz;
+    p->rotatorlen = 0;
+    if (offset_hz == 0) {
+        p->kind = 0;
+        p->nbins = (int)(fs / (double)gcd_ll((long)spacing_hz, (long)fs));
+        p->ntones = (int)floor(bandwidth_hz / spacing_hz);
+    } else {
+        int No = (int)(fs / (double)gcd_ll(offset_hz, (long)fs));
+        int Np = (int)(fs / (double)gcd_ll((long)spacing_hz, (long)fs));
+        p->ntones = (int)floor((bandwidth_hz - offset_hz) / spacing_hz) + 1;
+        if (No % Np == 0) {
+            p->kind = 1;
+            p->nbins = No;
+        } else {
+            p->kind = 2;
+            p->nbins = Np;
+            p->rotatorlen = No;
+        }
+    }
+    p->step = (int)floor((double)p->nbins * (spacing_hz / fs));
+    p->offset_bin = (p->kind == 1) ? (int)floor((double)p->nbins * (double)offset_hz / fs) : 0;
+    p->nfold = p->kind == 2 ? (int)((long long)p->rotatorlen / gcd_ll(p->rotatorlen, p->nbins) * p->nbins)
+                            : p->nbins;
+    p->dphi = 2.0 * M_PI * (-(double)offset_hz / fs);
+    if (hipMalloc(&p->acc, sizeof(double) * (size_t)p->nfold) != hipSuccess ||
+        hipMalloc(&p->tones, sizeof(double) * 2 * (size_t)std::max(p->ntones, 1)) != hipSuccess) {
+        hipGetLastError();
+        if (p->acc) hipFree(p->acc);
+        delete p;
+        fx_set_error("fx_pcal_create: allocation failed");
+        return FX_ERR_NOMEM;
+    }
+    p->position = sampleoffset % (size_t)p->nfold;
+    if (hipMemsetAsync(p->acc, 0, sizeof(double) * (size_t)p->nfold, ctx->stream) != hipSuccess) {
+        hipFree(p->acc);
+        hipFree(p->tones);
+        delete p;
+        fx_set_error("fx_pcal_create: clear failed");
+        return FX_ERR_HIP;
+    }
+    *out = p;
+    return FX_OK;
+}
+
+extern "C" int fx_pcal_destroy(fx_pcal* p)
+{
+    if (!p) return FX_OK;
+    hipSetDevice(p->ctx->device);
+    hipStreamSynchronize(p->ctx->stream);
+    hipFree(p->acc);
+    hipFree(p->tones);
+    delete p;
+    return FX_OK;
+}
+
+extern "C" int fx_pcal_clear(fx_pcal* p)
+{
+    FX_REQUIRE(p, FX_ERR_ARG, "null pcal");
+    int rc = ctx_device(p->ctx);
+    if (rc) return rc;
+    FX_HIP_TRY(hipMemsetAsync(p->acc, 0, sizeof(double) * (size_t)p->nfold, p->ctx->stream));
+    p->samplecount = 0;
+    p->finalized = false;
+    return FX_OK;
+}
+
+extern "C" int fx_pcal_adjust_sample_offset(fx_pcal* p, size_t sampleoffset)
+{
+    FX_REQUIRE(p, FX_ERR_ARG, "null pcal");
+    p->position = sampleoffset % (size_t)p->nfold;
+    return FX_OK;
+}
+
+extern "C" int fx_pcal_extract_2bit(fx_pcal* p, const uint32_t* d_words, size_t nsamp)
+{
+    FX_REQUIRE(p, FX_ERR_ARG, "null pcal");
+    FX_REQUIRE(!p->finalized, FX_ERR_STATE, "fx_pcal_extract_2bit: extractor finalized, clear first");
+    int rc = ctx_device(p->ctx);
+    if (rc) return rc;
+    FX_CHECK_LAUNCH(fxk_pcal_fold(p->ctx->stream, d_words, nsamp, p->position, p->nfold, p->acc));
+    p->position = (p->position + nsamp) % (size_t)p->nfold;
+    p->samplecount += nsamp;
+    return FX_OK;
+}
+
+extern "C" int fx_pcal_final(fx_pcal* p, fx_cf64* tones, uint64_t* nsamples)
+{
+    FX_REQUIRE(p, FX_ERR_ARG, "null pcal");
+    int rc = ctx_device(p->ctx);
+    if (rc) return rc;
+    if (!p->finalized) {
+        FX_CHECK_LAUNCH(fxk_pcal_final(p->ctx->stream, p->acc, p->nfold, p->nbins, p->ntones,
+                                       p->offset_bin, p->step, p->rotatorlen, p->dphi, p->tones));
+        p->finalized = true;
+    }
+    if (tones && p->ntones > 0) {
+        FX_HIP_TRY(hipMemcpyAsync(tones, p->tones, sizeof(double) * 2 * (size_t)p->ntones,
+                                  hipMemcpyDeviceToHost, p->ctx->stream));
+    }
+    FX_HIP_TRY(hipStreamSynchronize(p->ctx->stream));
+    if (nsamples) *nsamples = p->samplecount;
+    return FX_OK;
+}
+
+extern "C" int fx_pcal_info(fx_pcal* p, int* kind, int* nbins, int* ntones)
+{
+    FX_REQUIRE(p, FX_ERR_ARG, "null pcal");
+    if (kind) *kind = p->kind;
+    if (nbins) *nbins = p->nbins;
+    if (ntones) *ntones = p->ntones;
+    return FX_OK;
+}
+
+// ---------------------------------------------------------------------------
+// correlator
+// ---------------------------------------------------------------------------
+struct fx_corr {
+    fx_ctx* ctx;
+    fx_corr_config cfg;
+    int nchan;
+    size_t elem;            // bytes per channelized complex sample
+    void* chan = nullptr;   // [c][t][i], chunk_spectra deep
+    float* acc = nullptr;   // engine-layout accumulators
+    size_t acc_floats = 0;
+    fx_phase_model* pm = nullptr;
+    const float2* mult = nullptr;
+    float* sc_chan = nullptr;
+    float* sc_inp = nullptr;
+    int2* pij = nullptr;
+    long spectra_done = 0;
+};
+
+static size_t prec_elem(int prec)
+{
+    switch (prec) {
+    case FX_PREC_F32: return 8;
+    case FX_PREC_F16: return 4;
+    case FX_PREC_FP8: return 2;
+    case FX_PREC_FP8_2BIT: return 1;
+    default: return 0;
+    }
+}
+
+extern "C" int fx_corr_create(fx_ctx* ctx, const fx_corr_config* cfg, fx_corr** out)
+{
+    int rc = ctx_device(ctx);
+    if (rc) return rc;
+    FX_REQUIRE(cfg && out, FX_ERR_ARG, "fx_corr_create: null argument");
+    FX_REQUIRE(cfg->ninp >= 1 && cfg->ninp <= 4096, FX_ERR_ARG, "fx_corr_create: ninp=%d", cfg->ninp);
+    FX_REQUIRE(is_pow2(cfg->nfft) && cfg->nfft >= 256 && cfg->nfft <= 16384, FX_ERR_ARG,
+               "fx_corr_create: nfft=%d not a power of two in [256,16384]", cfg->nfft);
+    FX_REQUIRE(cfg->prec == FX_PREC_F32 || cfg->prec == FX_PREC_F16, FX_ERR_UNSUPPORTED,
+               "fx_corr_create: precision %d not built", cfg->prec);
+    FX_REQUIRE(cfg->chunk_spectra >= 8 && cfg->chunk_spectra % 8 == 0, FX_ERR_ARG,
+               "fx_corr_create: chunk_spectra=%ld must be a positive multiple of 8",
+               cfg->chunk_spectra);
+    fx_corr* c = new fx_corr;
+    c->ctx = ctx;
+    c->cfg = *cfg;
+    c->nchan = cfg->nfft / 2;
+    c->elem = prec_elem(cfg->prec);
+    size_t chan_bytes = (size_t)c->nchan * (size_t)cfg->chunk_spectra * (size_t)cfg->ninp * c->elem;
+    c->acc_floats = fxk_xengine_acc_floats(c->nchan, cfg->ninp, 1);
+    long P = (long)cfg->ninp * (cfg->ninp + 1) / 2;
+    std::vector<int2> hp((size_t)P);
+    for (int i = 0, p = 0; i < cfg->ninp; i++)
+        for (int j = i; j < cfg->ninp; j++, p++) hp[(size_t)p] = make_int2(i, j);
+    bool ok = hipMalloc(&c->chan, chan_bytes) == hipSuccess &&
+              hipMalloc(&c->acc, c->acc_floats * sizeof(float)) == hipSuccess &&
+              hipMalloc(&c->pm, sizeof(fx_phase_model) * (size_t)cfg->ninp) == hipSuccess &&
+              hipMalloc(&c->pij, sizeof(int2) * (size_t)P) == hipSuccess;
+    if (ok) ok = hipMemcpy(c->pij, hp.data(), sizeof(int2) * (size_t)P, hipMemcpyHostToDevice) == hipSuccess;
+    if (ok) ok = hipMemsetAsync(c->acc, 0, c->acc_floats * sizeof(float), ctx->stream) == hipSuccess;
+    if (ok) ok = hipMemsetAsync(c->pm, 0, sizeof(fx_phase_model) * (size_t)cfg->ninp, ctx->stream) == hipSuccess;
+    if (ok && cfg->prec != FX_PREC_F32) {
+        // default power scaling: a 2-bit stream through the level table has
+        // E[x^2] = 0.6726 + 0.3274*H^2 ~ 4.316, so the FFT output has
+        // sigma ~ sqrt(N * 4.316); map that to 1 before the cast
+        std::vector<float> sc((size_t)c->nchan, (float)(1.0 / sqrt((double)cfg->nfft * 4.316)));
+        std::vector<float> si((size_t)cfg->ninp, 1.0f);
+        ok = hipMalloc(&c->sc_chan, sizeof(float) * sc.size()) == hipSuccess &&
+             hipMalloc(&c->sc_inp, sizeof(float) * si.size()) == hipSuccess &&
+             hipMemcpy(c->sc_chan, sc.data(), sizeof(float) * sc.size(), hipMemcpyHostToDevice) == hipSuccess &&
+             hipMemcpy(c->sc_inp, si.data(), sizeof(float) * si.size(), hipMemcpyHostToDevice) == hipSuccess;
+    }
+    if (!ok) {
+        hipGetLastError();
+        fx_corr_destroy(c);
+        fx_set_error("fx_corr_create: device allocation failed (channel buffer %zu bytes)", chan_bytes);
+        return FX_ERR_NOMEM;
+    }
+    *out = c;
+    return FX_OK;
+}
+
+extern "C" int fx_corr_destroy(fx_corr* c)
+{
+    if (!c) return FX_OK;
+    hipSetDevice(c->ctx->device);
+    hipStreamSynchronize(c->ctx->stream);
+    if (c->chan) hipFree(c->chan);
+    if (c->acc) hipFree(c->acc);
+    if (c->pm) hipFree(c->pm);
+    if (c->pij) hipFree(c->pij);
+    if (c->sc_chan) hipFree(c->sc_chan);
+    if (c->sc_inp) hipFree(c->sc_inp);
+    delete c;
+    return FX_OK;
+}
+
+extern "C" int fx_corr_set_phase_model(fx_corr* c, const fx_phase_model* pm)
+{
+    FX_REQUIRE(c && pm, FX_ERR_ARG, "fx_corr_set_phase_model: null argument");
+    int rc = ctx_device(c->ctx);
+    if (rc) return rc;
+    FX_HIP_TRY(hipMemcpyAsync(c->pm, pm, sizeof(fx_phase_model) * (size_t)c->cfg.ninp,
+                              hipMemcpyHostToDevice, c->ctx->stream));
+    FX_HIP_TRY(hipStreamSynchronize(c->ctx->stream));
+    return FX_OK;
+}
+
+extern "C" int fx_corr_set_multipliers(fx_corr* c, const fx_cf32* d_mult)
+{
+    FX_REQUIRE(c, FX_ERR_ARG, "null correlator");
+    c->mult = (const float2*)d_mult;
+    return FX_OK;
+}
+
+extern "C" int fx_corr_set_scales(fx_corr* c, const float* chan_scale, const float* inp_scale)
+{
+    FX_REQUIRE(c, FX_ERR_ARG, "null correlator");
+    int rc = ctx_device(c->ctx);
+    if (rc) return rc;
+    if (!c->sc_chan) {
+        if (hipMalloc(&c->sc_chan, sizeof(float) * (size_t)c->nchan) != hipSuccess ||
+            hipMalloc(&c->sc_inp, sizeof(float) * (size_t)c->cfg.ninp) != hipSuccess) {
+            hipGetLastError();
+            fx_set_error("fx_corr_set_scales: allocation failed");
+            return FX_ERR_NOMEM;
+        }
+    }
+    std::vector<float> sc((size_t)c->nchan, 1.0f), si((size_t)c->cfg.ninp, 1.0f);
+    if (chan_scale) memcpy(sc.data(), chan_scale, sizeof(float) * sc.size());
+    if (inp_scale) memcpy(si.data(), inp_scale, sizeof(float) * si.size());
+    for (float v : sc) FX_REQUIRE(v > 0.f && std::isfinite(v), FX_ERR_ARG, "scales must be positive");
+    for (float v : si) FX_REQUIRE(v > 0.f && std::isfinite(v), FX_ERR_ARG, "scales must be positive");
+    FX_HIP_TRY(hipMemcpy(c->sc_chan, sc.data(), sizeof(float) * sc.size(), hipMemcpyHostToDevice));
+    FX_HIP_TRY(hipMemcpy(c->sc_inp, si.data(), sizeof(float) * si.size(), hipMemcpyHostToDevice));
+    return FX_OK;
+}
+
+extern "C" int fx_corr_fstage(fx_corr* c, const uint32_t* d_words, size_t in_stride_words,
+                              long t_first, long nspectra, void* d_chan, int out_prec)
+{
+    FX_REQUIRE(c && d_words && d_chan, FX_ERR_ARG, "fx_corr_fstage: null argument");
+    int rc = ctx_device(c->ctx);
+    if (rc) return rc;
+    FX_REQUIRE(in_stride_words % 2 == 0 && ((uintptr_t)d_words % 8) == 0, FX_ERR_ARG,
+               "fx_corr_fstage: input stride must be even and the base 8-byte aligned");
+    FX_REQUIRE(t_first >= 0 && nspectra >= 0, FX_ERR_ARG, "fx_corr_fstage: negative range");
+    if (nspectra == 0) return FX_OK;
+    const bool scaled = out_prec != FX_PREC_F32;
+    FX_CHECK_LAUNCH(fxk_fstage(c->ctx->stream, c->ctx->tw, d_words, in_stride_words, c->cfg.ninp,
+                               c->cfg.nfft, t_first, nspectra, c->cfg.use_phase ? c->pm : nullptr,
+                               c->mult, scaled ? c->sc_chan : nullptr, scaled ? c->sc_inp : nullptr,
+                               out_prec, d_chan));
+    return FX_OK;
+}
+
+extern "C" int fx_corr_process(fx_corr* c, const uint32_t* d_words, size_t in_stride_words,
+                               long t_first, long nspectra)
+{
+    FX_REQUIRE(c && d_words, FX_ERR_ARG, "fx_corr_process: null argument");
+    int rc = ctx_device(c->ctx);
+    if (rc) return rc;
+    for (long done = 0; done < nspectra;) {
+        long nt = std::min(c->cfg.chunk_spectra, nspectra - done);
+        rc = fx_corr_fstage(c, d_words, in_stride_words, t_first + done, nt, c->chan, c->cfg.prec);
+        if (rc) return rc;
+        FX_CHECK_LAUNCH(fxk_xengine(c->ctx->stream, c->cfg.prec, c->chan, c->nchan, nt, c->cfg.ninp, c->acc));
+        done += nt;
+    }
+    c->spectra_done += nspectra;
+    return FX_OK;
+}
+
+extern "C" int fx_corr_xengine(fx_corr* c, const void* d_chan, long nspectra)
+{
+    FX_REQUIRE(c && d_chan, FX_ERR_ARG, "fx_corr_xengine: null argument");
+    int rc = ctx_device(c->ctx);
+    if (rc) return rc;
+    if (nspectra <= 0) return nspectra == 0 ? FX_OK : FX_ERR_ARG;
+    FX_CHECK_LAUNCH(fxk_xengine(c->ctx->stream, c->cfg.prec, d_chan, c->nchan, nspectra, c->cfg.ninp, c->acc));
+    c->spectra_done += nspectra;
+    return FX_OK;
+}
+
+extern "C" int fx_corr_dump(fx_corr* c, fx_cf32* d_vis)
+{
+    FX_REQUIRE(c && d_vis, FX_ERR_ARG, "fx_corr_dump: null argument");
+    int rc = ctx_device(c->ctx);
+    if (rc) return rc;
+    const bool scaled = c->cfg.prec != FX_PREC_F32;
+    FX_CHECK_LAUNCH(fxk_xengine_dump_pij(c->ctx->stream, c->acc, c->nchan, c->cfg.ninp, 1, nullptr,
+                                         scaled ? c->sc_chan : nullptr, scaled ? c->sc_inp : nullptr,
+                                         c->pij, 0, (float2*)d_vis));
+    return FX_OK;
+}
+
+extern "C" int fx_corr_reset(fx_corr* c)
+{
+    FX_REQUIRE(c, FX_ERR_ARG, "null correlator");
+    int rc = ctx_device(c->ctx);
+    if (rc) return rc;
+    FX_HIP_TRY(hipMemsetAsync(c->acc, 0, c->acc_floats * sizeof(float), c->ctx->stream));
+    c->spectra_done = 0;
+    return FX_OK;
+}
+
+extern "C" int fx_corr_info(fx_corr* c, long* chunk, size_t* chan_bytes, size_t* acc_bytes)
+{
+    FX_REQUIRE(c, FX_ERR_ARG, "null correlator");
+    if (chunk) *chunk = c->cfg.chunk_spectra;
+    if (chan_bytes) *chan_bytes = (size_t)c->nchan * (size_t)c->cfg.chunk_spectra * (size_t)c->cfg.ninp * c->elem;
+    if (acc_bytes) *acc_bytes = c->acc_floats * sizeof(float);
+    return FX_OK;
+}
+
+// ---------------------------------------------------------------------------
+// diagnostics
+// ---------------------------------------------------------------------------
+extern "C" int fx_mfma_probe(fx_ctx* ctx, int which, float* host_out)
+{
+    int rc = ctx_device(ctx);
+    if (rc) return rc;
+    FX_REQUIRE(which >= 0 && which <= 2 && host_out, FX_ERR_ARG, "fx_mfma_probe: bad argument");
+    float* d = nullptr;
+    FX_HIP_TRY(hipMalloc(&d, 32 * 32 * sizeof(float)));
+    FX_HIP_TRY(hipMemsetAsync(d, 0, 32 * 32 * sizeof(float), ctx->stream));
+    rc = fxk_mfma_probe(ctx->stream, which, d);
+    if (rc == FX_OK) {
+        hipMemcpyAsync(host_out, d, 32 * 32 * sizeof(float), hipMemcpyDeviceToHost, ctx->stream);
+        hipStreamSynchronize(ctx->stream);
+    }
+    hipFree(d);
+    return rc;
+}

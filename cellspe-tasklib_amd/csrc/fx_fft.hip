@@ -1,0 +1,482 @@
+// fx_fft.hip -- LDS-staged FFT kernels for gfx950 and the fused front ends.
+//
+//   k_fft_c2c        forward complex FFT (speFFT_exec / CELLDSP_TASK_FFT_EXEC,
+//                    cellspe-tasklib-dsp.cpp:565-602)
+//   k_fft_r2c        real-input FFT (genericFFTexec_R, architecture.h:563-578)
+//   k_spectrometer   2-bit VDIF multi-channel unpack -> window -> real FFT ->
+//                    |X|^2 accumulate (spectrometer_spu.c:97-224,253-349 with
+//                    the IPP mean-power convention, TaskCoreIPP.cpp:437-511)
+//   k_fstage         2-bit unpack -> real FFT -> FP64 phase model rotation ->
+//                    channelized [c][t][i] store in FP32/FP16/FP8 (the F half
+//                    of minicorrelator_spu.c:326-410 / exec_difxprocess,
+//                    cellspe-tasklib-dsp.cpp:609-698)
+//
+// Every transform is a Stockham radix-16 FFT over a workgroup-resident LDS
+// tile: F transforms of complex length M per workgroup with F*M = FM complex
+// (8192 or 16384), NT = FM/16 threads, so each thread holds exactly 16 complex
+// values per pass.  Real transforms of length N = 2M pack even/odd samples
+// into one complex M-point FFT and split the result (rsplit).
+#include "fx_internal.h"
+
+#include <hip/hip_fp8.h>
+
+namespace {
+
+template <int M>
+struct FftShape {
+    static constexpr int FM = (M >= 16384) ? 16384 : 8192;
+    static constexpr int F = FM / M;
+    static constexpr int NT = FM / 16;
+    static constexpr int LDS = lds_complex(FM);
+};
+
+// ---------------------------------------------------------------------------
+// complex -> complex
+// ---------------------------------------------------------------------------
+template <int M>
+__global__ __launch_bounds__(FftShape<M>::NT) void k_fft_c2c(const float2* __restrict__ tw,
+                                                             const float2* __restrict__ in,
+                                                             float2* __restrict__ out, long batch)
+{
+    using S = FftShape<M>;
+    __shared__ float2 sh[S::LDS];
+    const int tid = threadIdx.x;
+    const long f0 = (long)blockIdx.x * S::F;
+    const long nvalid = (batch - f0 < S::F ? batch - f0 : S::F) * M;
+#pragma unroll
+    for (int s = 0; s < 16; s++) {
+        int e = tid + s * S::NT;
+        sh[lpad(e)] = (e < nvalid) ? in[f0 * M + e] : make_float2(0.f, 0.f);
+    }
+    __syncthreads();
+    fft_passes<M, 1, S::NT, S::F>(sh, tw, tid);
+#pragma unroll
+    for (int s = 0; s < 16; s++) {
+        int e = tid + s * S::NT;
+        if (e < nvalid) out[f0 * M + e] = sh[lpad(e)];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// real -> complex (N = 2M real samples per transform)
+// ---------------------------------------------------------------------------
+template <int M>
+__global__ __launch_bounds__(FftShape<M>::NT) void k_fft_r2c(const float2* __restrict__ tw,
+                                                             const float2* __restrict__ in,
+                                                             float2* __restrict__ out, long batch,
+                                                             int keep_nyq)
+{
+    using S = FftShape<M>;
+    __shared__ float2 sh[S::LDS];
+    const int tid = threadIdx.x;
+    const long f0 = (long)blockIdx.x * S::F;
+    const long nf = (batch - f0 < S::F ? batch - f0 : S::F);
+    const long nvalid = nf * M;
+#pragma unroll
+    for (int s = 0; s < 16; s++) {
+        int e = tid + s * S::NT;
+        sh[lpad(e)] = (e < nvalid) ? in[f0 * M + e] : make_float2(0.f, 0.f);
+    }
+    __syncthreads();
+    fft_passes<M, 1, S::NT, S::F>(sh, tw, tid);
+    const int ostride = M + (keep_nyq ? 1 : 0);
+    constexpr int H = (M / 2 > 0) ? M / 2 : 1;
+    for (int q = tid; q < S::F * H; q += S::NT) {
+        int f = q / H;
+        int kp = q - f * H;
+        if (f >= nf) continue;
+        float2 xk, xm;
+        rsplit<M>(sh, f * M, kp, tw, xk, xm);
+        float2* o = out + (f0 + f) * ostride;
+        o[kp] = xk;
+        if (kp == 0) {
+            if (keep_nyq) o[M] = xm;
+        } else {
+            o[M - kp] = xm;
+        }
+    }
+    if (tid < S::F && tid < nf && M >= 2) {
+        float2 xk, xm;
+        rsplit<M>(sh, tid * M, M / 2, tw, xk, xm);
+        out[(f0 + tid) * ostride + M / 2] = xk;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// spectrometer: nchan-interleaved 2-bit words, one time sample per 2*nchan
+// bits.  Workgroup (c, s, g) integrates FFTs [g*per, (g+1)*per) of spectrum s
+// of channel c, F transforms at a time, and writes its partial power
+// partial[wg][f][k].
+// ---------------------------------------------------------------------------
+template <int M>
+__global__ __launch_bounds__(FftShape<M>::NT) void k_spectrometer(
+    const float2* __restrict__ tw, const float* __restrict__ win, const uint32_t* __restrict__ words,
+    int nchan, int integrate_ffts, int groups, int per_group, float* __restrict__ partial)
+{
+    using S = FftShape<M>;
+    constexpr int N = 2 * M;
+    __shared__ float2 sh[S::LDS];
+    const int tid = threadIdx.x;
+    const int wg = blockIdx.x;
+    const int g = wg % groups;
+    const int cs = wg / groups;
+    const int c = cs % nchan;
+    const long s = cs / nchan;
+    const int spw = 16 / nchan;          // time samples per word
+    const int shift0 = 2 * c;
+    const int bits_per_t = 2 * nchan;
+
+    float acc[8][2];
+#pragma unroll
+    for (int i = 0; i < 8; i++) acc[i][0] = acc[i][1] = 0.f;
+    float accm = 0.f;
+
+    const long fft_first = s * integrate_ffts + (long)g * per_group;
+    int fft_count = per_group;
+    if ((long)g * per_group + fft_count > integrate_ffts) fft_count = integrate_ffts - g * per_group;
+
+    for (int it = 0; it < fft_count; it += S::F) {
+        // load F transforms: F*N samples of channel c; thread handles 2*16 = 32
+        // consecutive-in-pairs samples: sample pair e -> z[e] = (x[2e], x[2e+1])
+#pragma unroll
+        for (int q = 0; q < 16; q++) {
+            int e = tid + q * S::NT;      // complex index within the WG tile
+            int f = e / M;
+            int n2 = 2 * (e - f * M);     // sample index within transform f
+            float2 z = make_float2(0.f, 0.f);
+            if (it + f < fft_count) {
+                long t = (fft_first + it + f) * (long)N + n2;
+                uint32_t w0 = words[t / spw];
+                uint32_t w1 = words[(t + 1) / spw];
+                int p0 = (int)(t % spw) * bits_per_t + shift0;
+                int p1 = (int)((t + 1) % spw) * bits_per_t + shift0;
+                z.x = lut2((w0 >> p0) & 3u) * win[n2];
+                z.y = lut2((w1 >> p1) & 3u) * win[n2 + 1];
+            }
+            sh[lpad(e)] = z;
+        }
+        __syncthreads();
+        fft_passes<M, 1, S::NT, S::F>(sh, tw, tid);
+        // power of the thread's 8 (kp, f) items, kp-major so the same thread
+        // owns the same bins on every iteration
+#pragma unroll
+        for (int i = 0; i < 8; i++) {
+            int q = tid * 8 + i;
+            int kp = q / S::F;
+            int f = q - kp * S::F;
+            if (it + f < fft_count) {
+                float2 xk, xm;
+                rsplit<M>(sh, f * M, kp, tw, xk, xm);
+                acc[i][0] += xk.x * xk.x + xk.y * xk.y;
+                acc[i][1] += xm.x * xm.x + xm.y * xm.y;
+            }
+        }
+        if (tid < S::F && it + tid < fft_count) {
+            float2 xk, xm;
+            rsplit<M>(sh, tid * M, M / 2, tw, xk, xm);
+            accm += xk.x * xk.x + xk.y * xk.y;
+        }
+        __syncthreads();
+    }
+    float* pw = partial + (long)wg * S::F * M;
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+        int q = tid * 8 + i;
+        int kp = q / S::F;
+        int f = q - kp * S::F;
+        pw[f * M + kp] = acc[i][0];
+        if (kp != 0) pw[f * M + (M - kp)] = acc[i][1];
+    }
+    if (tid < S::F) pw[tid * M + M / 2] = accm;
+}
+
+// out[s][c][k] = (1/K) sum_g sum_f partial[(s*nchan+c)*groups+g][f][k]
+__global__ void k_spec_reduce(const float* __restrict__ partial, int M, int F, int groups,
+                              int nchan, long nspec, float scale, float* __restrict__ out)
+{
+    long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    long total = nspec * nchan * (long)M;
+    if (idx >= total) return;
+    int k = (int)(idx % M);
+    long cs = idx / M;
+    const float* p = partial + cs * (long)groups * F * M;
+    float sum = 0.f;
+    for (int g = 0; g < groups; g++)
+        for (int f = 0; f < F; f++) sum += p[((long)g * F + f) * M + k];
+    out[idx] = sum * scale;
+}
+
+// ---------------------------------------------------------------------------
+// F-stage.  Workgroup b covers F consecutive items g = b*F + f of the
+// row-major (t, i) grid of the chunk; item (t, i) is spectrum t_first + t of
+// input i.  Output element (c, t, i) lives at out[c*(nt*I) + t*I + i], so the
+// F results of one channel are contiguous.
+// ---------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ void store_chan(T* out, long idx, float2 v);
+template <>
+__device__ __forceinline__ void store_chan<float2>(float2* out, long idx, float2 v) { out[idx] = v; }
+template <>
+__device__ __forceinline__ void store_chan<__half2>(__half2* out, long idx, float2 v)
+{
+    out[idx] = __floats2half2_rn(v.x, v.y);
+}
+template <>
+__device__ __forceinline__ void store_chan<uint16_t>(uint16_t* out, long idx, float2 v)
+{
+    __hip_fp8_e4m3 r(v.x), i(v.y);
+    out[idx] = (uint16_t)((uint16_t)r.__x | ((uint16_t)i.__x << 8));
+}
+
+template <int M, typename T>
+__global__ __launch_bounds__(FftShape<M>::NT) void k_fstage(
+    const float2* __restrict__ tw, const uint32_t* __restrict__ words, size_t in_stride, int ninp,
+    long t_first, long nt, const fx_phase_model* __restrict__ pm, const float2* __restrict__ mult,
+    const float* __restrict__ sc_chan, const float* __restrict__ sc_inp, T* __restrict__ out)
+{
+    using S = FftShape<M>;
+    constexpr int N = 2 * M;
+    constexpr int WPT = N / 16;     // packed words per transform
+    __shared__ float2 sh[S::LDS];
+    const int tid = threadIdx.x;
+    const long g0 = (long)blockIdx.x * S::F;
+    const long nitems = nt * ninp;
+
+    // ---- load + unpack: the tile is F*N samples = 2*NT words; thread takes
+    // two consecutive words (32 samples -> 16 complex z values)
+    {
+        int wl = 2 * tid;                 // word index within the tile
+        int f = wl / WPT;
+        int wof = wl - f * WPT;
+        long g = g0 + f;
+        uint32_t w0 = 0, w1 = 0;
+        if (g < nitems) {
+            long t = g / ninp;
+            int i = (int)(g - t * ninp);
+            const uint32_t* src = words + (size_t)i * in_stride + (size_t)(t_first + t) * WPT + wof;
+            uint2 ww = *reinterpret_cast<const uint2*>(src);
+            w0 = ww.x;
+            w1 = ww.y;
+        }
+        int zb = f * M + wof * 8;
+#pragma unroll
+        for (int m = 0; m < 8; m++)
+            sh[lpad(zb + m)] = make_float2(lut2((w0 >> (4 * m)) & 3u), lut2((w0 >> (4 * m + 2)) & 3u));
+#pragma unroll
+        for (int m = 0; m < 8; m++)
+            sh[lpad(zb + 8 + m)] = make_float2(lut2((w1 >> (4 * m)) & 3u), lut2((w1 >> (4 * m + 2)) & 3u));
+    }
+    __syncthreads();
+    fft_passes<M, 1, S::NT, S::F>(sh, tw, tid);
+
+    const long cstride = nitems;   // distance between channels in the output
+#pragma unroll
+    for (int it = 0; it < 9; it++) {
+        int kp, f;
+        if (it < 8) {
+            int q = tid * 8 + it;
+            kp = q / S::F;
+            f = q - kp * S::F;
+        } else {
+            if (tid >= S::F) break;
+            kp = M / 2;
+            f = tid;
+        }
+        long g = g0 + f;
+        if (g >= nitems) continue;
+        long t = g / ninp;
+        int i = (int)(g - t * ninp);
+        float2 xk, xm;
+        rsplit<M>(sh, f * M, kp, tw, xk, xm);
+        const int k2 = M - kp;
+        const bool two = (kp != 0) && (kp != M / 2);
+        if (pm) {
+            const fx_phase_model p = pm[i];
+            const double tg = (double)(t_first + t);
+            const double tau = p.delay0 + p.delay_rate * tg;
+            const double fr = (p.fringe0 + p.fringe_rate * tg) * 0.15915494309189535;
+            {
+                double cyc = (double)kp * tau / (double)N + fr;
+                cyc -= rint(cyc);
+                float ph = (float)(6.283185307179586 * cyc);
+                float sn, cs;
+                sincosf(ph, &sn, &cs);
+                xk = cmul(xk, make_float2(cs, sn));
+            }
+            if (two) {
+                double cyc = (double)k2 * tau / (double)N + fr;
+                cyc -= rint(cyc);
+                float ph = (float)(6.283185307179586 * cyc);
+                float sn, cs;
+                sincosf(ph, &sn, &cs);
+                xm = cmul(xm, make_float2(cs, sn));
+            }
+        }
+        if (mult) {
+            xk = cmul(xk, mult[(size_t)i * M + kp]);
+            if (two) xm = cmul(xm, mult[(size_t)i * M + k2]);
+        }
+        if (sc_chan) {
+            float si = sc_inp[i];
+            float a = sc_chan[kp] * si;
+            xk.x *= a;
+            xk.y *= a;
+            if (two) {
+                float b = sc_chan[k2] * si;
+                xm.x *= b;
+                xm.y *= b;
+            }
+        }
+        store_chan<T>(out, (long)kp * cstride + g, xk);
+        if (two) store_chan<T>(out, (long)k2 * cstride + g, xm);
+    }
+}
+
+template <int M>
+int launch_c2c(hipStream_t s, const float2* tw, const float2* in, float2* out, long batch)
+{
+    using S = FftShape<M>;
+    long nblk = (batch + S::F - 1) / S::F;
+    hipLaunchKernelGGL(k_fft_c2c<M>, dim3((unsigned)nblk), dim3(S::NT), 0, s, tw, in, out, batch);
+    return hipGetLastError() == hipSuccess ? FX_OK : FX_ERR_HIP;
+}
+
+template <int M>
+int launch_r2c(hipStream_t s, const float2* tw, const float* in, float2* out, long batch, int keep)
+{
+    using S = FftShape<M>;
+    long nblk = (batch + S::F - 1) / S::F;
+    hipLaunchKernelGGL(k_fft_r2c<M>, dim3((unsigned)nblk), dim3(S::NT), 0, s, tw,
+                       reinterpret_cast<const float2*>(in), out, batch, keep);
+    return hipGetLastError() == hipSuccess ? FX_OK : FX_ERR_HIP;
+}
+
+template <int M>
+int launch_spec(hipStream_t s, const float2* tw, const float* win, const uint32_t* words,
+                int nchan, int integrate_ffts, int nspec, int groups, float* partial, float* out)
+{
+    using S = FftShape<M>;
+    int per_group = (integrate_ffts + groups - 1) / groups;
+    per_group = ((per_group + S::F - 1) / S::F) * S::F;
+    long nblk = (long)nspec * nchan * groups;
+    hipLaunchKernelGGL(k_spectrometer<M>, dim3((unsigned)nblk), dim3(S::NT), 0, s, tw, win, words,
+                       nchan, integrate_ffts, groups, per_group, partial);
+    if (hipGetLastError() != hipSuccess) return FX_ERR_HIP;
+    long total = (long)nspec * nchan * M;
+    hipLaunchKernelGGL(k_spec_reduce, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s,
+                       partial, M, S::F, groups, nchan, (long)nspec, 1.0f / (float)integrate_ffts,
+                       out);
+    return hipGetLastError() == hipSuccess ? FX_OK : FX_ERR_HIP;
+}
+
+template <int M, typename T>
+int launch_fstage_t(hipStream_t s, const float2* tw, const uint32_t* words, size_t stride,
+                    int ninp, long t_first, long nt, const fx_phase_model* pm, const float2* mult,
+                    const float* sc, const float* si, void* out)
+{
+    using S = FftShape<M>;
+    long nitems = nt * ninp;
+    long nblk = (nitems + S::F - 1) / S::F;
+    hipLaunchKernelGGL((k_fstage<M, T>), dim3((unsigned)nblk), dim3(S::NT), 0, s, tw, words, stride,
+                       ninp, t_first, nt, pm, mult, sc, si, reinterpret_cast<T*>(out));
+    return hipGetLastError() == hipSuccess ? FX_OK : FX_ERR_HIP;
+}
+
+template <int M>
+int launch_fstage(hipStream_t s, const float2* tw, const uint32_t* words, size_t stride, int ninp,
+                  long t_first, long nt, const fx_phase_model* pm, const float2* mult,
+                  const float* sc, const float* si, int out_type, void* out)
+{
+    switch (out_type) {
+    case FX_PREC_F32:
+        return launch_fstage_t<M, float2>(s, tw, words, stride, ninp, t_first, nt, pm, mult, sc, si, out);
+    case FX_PREC_F16:
+        return launch_fstage_t<M, __half2>(s, tw, words, stride, ninp, t_first, nt, pm, mult, sc, si, out);
+    case FX_PREC_FP8:
+        return launch_fstage_t<M, uint16_t>(s, tw, words, stride, ninp, t_first, nt, pm, mult, sc, si, out);
+    default:
+        return FX_ERR_ARG;
+    }
+}
+
+} // namespace
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+int fxk_fft_c2c(hipStream_t s, const float2* tw, const float2* in, float2* out, int m, long batch)
+{
+    switch (m) {
+    case 2: return launch_c2c<2>(s, tw, in, out, batch);
+    case 4: return launch_c2c<4>(s, tw, in, out, batch);
+    case 8: return launch_c2c<8>(s, tw, in, out, batch);
+    case 16: return launch_c2c<16>(s, tw, in, out, batch);
+    case 32: return launch_c2c<32>(s, tw, in, out, batch);
+    case 64: return launch_c2c<64>(s, tw, in, out, batch);
+    case 128: return launch_c2c<128>(s, tw, in, out, batch);
+    case 256: return launch_c2c<256>(s, tw, in, out, batch);
+    case 512: return launch_c2c<512>(s, tw, in, out, batch);
+    case 1024: return launch_c2c<1024>(s, tw, in, out, batch);
+    case 2048: return launch_c2c<2048>(s, tw, in, out, batch);
+    case 4096: return launch_c2c<4096>(s, tw, in, out, batch);
+    case 8192: return launch_c2c<8192>(s, tw, in, out, batch);
+    case 16384: return launch_c2c<16384>(s, tw, in, out, batch);
+    default: return FX_ERR_ARG;
+    }
+}
+
+int fxk_fft_r2c(hipStream_t s, const float2* tw, const float* in, float2* out, int n, long batch,
+                int keep)
+{
+    switch (n) {
+    case 4: return launch_r2c<2>(s, tw, in, out, batch, keep);
+    case 8: return launch_r2c<4>(s, tw, in, out, batch, keep);
+    case 16: return launch_r2c<8>(s, tw, in, out, batch, keep);
+    case 32: return launch_r2c<16>(s, tw, in, out, batch, keep);
+    case 64: return launch_r2c<32>(s, tw, in, out, batch, keep);
+    case 128: return launch_r2c<64>(s, tw, in, out, batch, keep);
+    case 256: return launch_r2c<128>(s, tw, in, out, batch, keep);
+    case 512: return launch_r2c<256>(s, tw, in, out, batch, keep);
+    case 1024: return launch_r2c<512>(s, tw, in, out, batch, keep);
+    case 2048: return launch_r2c<1024>(s, tw, in, out, batch, keep);
+    case 4096: return launch_r2c<2048>(s, tw, in, out, batch, keep);
+    case 8192: return launch_r2c<4096>(s, tw, in, out, batch, keep);
+    case 16384: return launch_r2c<8192>(s, tw, in, out, batch, keep);
+    case 32768: return launch_r2c<16384>(s, tw, in, out, batch, keep);
+    default: return FX_ERR_ARG;
+    }
+}
+
+int fxk_spectrometer(hipStream_t s, const float2* tw, const float* win, const uint32_t* words,
+                     size_t ntime, int nchan, int n, int integrate_ffts, int nspec, float* partial,
+                     int groups, float* out)
+{
+    (void)ntime;
+    switch (n) {
+    case 256: return launch_spec<128>(s, tw, win, words, nchan, integrate_ffts, nspec, groups, partial, out);
+    case 512: return launch_spec<256>(s, tw, win, words, nchan, integrate_ffts, nspec, groups, partial, out);
+    case 1024: return launch_spec<512>(s, tw, win, words, nchan, integrate_ffts, nspec, groups, partial, out);
+    case 2048: return launch_spec<1024>(s, tw, win, words, nchan, integrate_ffts, nspec, groups, partial, out);
+    case 4096: return launch_spec<2048>(s, tw, win, words, nchan, integrate_ffts, nspec, groups, partial, out);
+    case 8192: return launch_spec<4096>(s, tw, win, words, nchan, integrate_ffts, nspec, groups, partial, out);
+    case 16384: return launch_spec<8192>(s, tw, win, words, nchan, integrate_ffts, nspec, groups, partial, out);
+    case 32768: return launch_spec<16384>(s, tw, win, words, nchan, integrate_ffts, nspec, groups, partial, out);
+    default: return FX_ERR_ARG;
+    }
+}
+
+int fxk_fstage(hipStream_t s, const float2* tw, const uint32_t* words, size_t stride, int ninp,
+               int n, long t_first, long nt, const fx_phase_model* pm, const float2* mult,
+               const float* sc, const float* si, int out_type, void* out)
+{
+    switch (n) {
+    case 256: return launch_fstage<128>(s, tw, words, stride, ninp, t_first, nt, pm, mult, sc, si, out_type, out);
+    case 512: return launch_fstage<256>(s, tw, words, stride, ninp, t_first, nt, pm, mult, sc, si, out_type, out);
+    case 1024: return launch_fstage<512>(s, tw, words, stride, ninp, t_first, nt, pm, mult, sc, si, out_type, out);
+    case 2048: return launch_fstage<1024>(s, tw, words, stride, ninp, t_first, nt, pm, mult, sc, si, out_type, out);
+    case 4096: return launch_fstage<2048>(s, tw, words, stride, ninp, t_first, nt, pm, mult, sc, si, out_type, out);
+    case 8192: return launch_fstage<4096>(s, tw, words, stride, ninp, t_first, nt, pm, mult, sc, si, out_type, out);
+    case 16384: return launch_fstage<8192>(s, tw, words, stride, ninp, t_first, nt, pm, mult, sc, si, out_type, out);
+    default: return FX_ERR_ARG;
+    }
+}

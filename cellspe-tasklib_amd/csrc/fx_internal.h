@@ -1,0 +1,224 @@
+// fx_internal.h -- shared definitions of the gfx950 F-X engine (HIP).
+//
+// Device-side helpers (complex arithmetic, LDS padding, small register DFTs,
+// the LDS Stockham FFT) and the host-side launcher declarations every
+// translation unit of libcellspe_fx.so shares.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <hip/hip_fp16.h>
+#include <stdint.h>
+#include <stddef.h>
+
+#include "cellspe_fx.h"
+
+// ---------------------------------------------------------------------------
+// Error handling
+// ---------------------------------------------------------------------------
+#define FX_HIP_TRY(expr)                                                        \
+    do {                                                                        \
+        hipError_t _e = (expr);                                                 \
+        if (_e != hipSuccess) {                                                 \
+            fx_set_error("%s:%d: %s -> %s", __FILE__, __LINE__, #expr,          \
+                         hipGetErrorString(_e));                                \
+            return FX_ERR_HIP;                                                  \
+        }                                                                       \
+    } while (0)
+
+#define FX_REQUIRE(cond, code, ...)                                             \
+    do {                                                                        \
+        if (!(cond)) {                                                          \
+            fx_set_error(__VA_ARGS__);                                          \
+            return (code);                                                      \
+        }                                                                       \
+    } while (0)
+
+void fx_set_error(const char* fmt, ...);
+
+// ---------------------------------------------------------------------------
+// Constants
+// ---------------------------------------------------------------------------
+// Master twiddle table: T[e] = exp(-2 pi i e / FX_TW_N), derived in FP64 and
+// rounded once to FP32.  Every FFT of complex length M <= FX_TW_N/2 reads its
+// twiddles W_M^e = T[e * FX_TW_N / M] and the real-FFT split twiddles
+// W_N^k = T[k * FX_TW_N / N] from it.
+#define FX_TW_N 32768
+#define FX_MAX_N 32768      // largest real FFT length
+#define FX_LUT_HI 3.3359f   // mark5access OPTIMAL_2BIT_HIGH
+
+// ---------------------------------------------------------------------------
+// Device helpers
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float2 cmul(float2 a, float2 b)
+{
+    return make_float2(a.x * b.x - a.y * b.y, a.y * b.x + a.x * b.y);
+}
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ float2 cconj(float2 a) { return make_float2(a.x, -a.y); }
+// multiply by -i
+__device__ __forceinline__ float2 cmul_mi(float2 a) { return make_float2(a.y, -a.x); }
+
+// LDS index padding: one spare complex every 32 keeps the strided Stockham
+// stores of the early passes off a single bank.
+__device__ __forceinline__ int lpad(int i) { return i + (i >> 5); }
+__host__ __device__ constexpr int lds_complex(int n) { return n + (n >> 5); }
+
+// Offset-binary 2-bit level table {-H, -1, +1, +H}.
+__device__ __forceinline__ float lut2(uint32_t code)
+{
+    // branch-free select; exact constants
+    float m = (code == 0u || code == 3u) ? FX_LUT_HI : 1.0f;
+    return (code & 2u) ? m : -m;
+}
+
+// cos/sin(2 pi k/16), k = 0..7 (forward sign: -sin)
+__device__ constexpr float kW16re[8] = {1.0f, 0.9238795325f, 0.7071067812f, 0.3826834324f,
+                                       0.0f, -0.3826834324f, -0.7071067812f, -0.9238795325f};
+__device__ constexpr float kW16im[8] = {-0.0f, -0.3826834324f, -0.7071067812f, -0.9238795325f,
+                                       -1.0f, -0.9238795325f, -0.7071067812f, -0.3826834324f};
+
+// In-register forward DFT of R points (R = 1,2,4,8,16), decimation in time.
+template <int R>
+__device__ __forceinline__ void dft_reg(float2* u)
+{
+    if constexpr (R == 1) {
+        return;
+    } else if constexpr (R == 2) {
+        float2 a = u[0], b = u[1];
+        u[0] = cadd(a, b);
+        u[1] = csub(a, b);
+    } else {
+        constexpr int H = R / 2;
+        float2 e[H], o[H];
+#pragma unroll
+        for (int i = 0; i < H; i++) {
+            e[i] = u[2 * i];
+            o[i] = u[2 * i + 1];
+        }
+        dft_reg<H>(e);
+        dft_reg<H>(o);
+#pragma unroll
+        for (int k = 0; k < H; k++) {
+            float2 t;
+            constexpr int S = 16 / R;
+            if (k == 0) {
+                t = o[k];
+            } else if (2 * k == H) {
+                t = cmul_mi(o[k]);
+            } else {
+                t = cmul(o[k], make_float2(kW16re[k * S], kW16im[k * S]));
+            }
+            u[k] = cadd(e[k], t);
+            u[k + H] = csub(e[k], t);
+        }
+    }
+}
+
+// One radix-R Stockham pass over F complex FFTs of length M held in LDS
+// (padded indexing), NT threads, current sub-transform span P.
+// Butterfly j of FFT f reads x[j + r M/R], multiplies by W_{PR}^{r k}
+// (k = j mod P), does the R-point DFT and stores y[(j-k) R + k + r P].
+template <int M, int R, int P, int NT, int F>
+__device__ __forceinline__ void stockham_pass(float2* sh, const float2* __restrict__ tw, int tid)
+{
+    constexpr int NB = F * M / R;
+    constexpr int BPT = NB / NT;
+    static_assert(NB % NT == 0, "butterflies must divide evenly");
+    float2 u[BPT][R];
+#pragma unroll
+    for (int b = 0; b < BPT; b++) {
+        int g = tid + b * NT;
+        int f = g / (M / R);
+        int j = g - f * (M / R);
+#pragma unroll
+        for (int r = 0; r < R; r++) u[b][r] = sh[lpad(f * M + j + r * (M / R))];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int b = 0; b < BPT; b++) {
+        int g = tid + b * NT;
+        int f = g / (M / R);
+        int j = g - f * (M / R);
+        int k = j & (P - 1);
+        if constexpr (P > 1) {
+#pragma unroll
+            for (int r = 1; r < R; r++) u[b][r] = cmul(u[b][r], tw[(r * k) * (FX_TW_N / (P * R))]);
+        }
+        dft_reg<R>(u[b]);
+        int base = f * M + (j - k) * R + k;
+#pragma unroll
+        for (int r = 0; r < R; r++) sh[lpad(base + r * P)] = u[b][r];
+    }
+    __syncthreads();
+}
+
+template <int M, int P, int NT, int F>
+__device__ __forceinline__ void fft_passes(float2* sh, const float2* __restrict__ tw, int tid)
+{
+    if constexpr (P < M) {
+        constexpr int REM = M / P;
+        constexpr int R = REM >= 16 ? 16 : REM;
+        stockham_pass<M, R, P, NT, F>(sh, tw, tid);
+        fft_passes<M, P * R, NT, F>(sh, tw, tid);
+    }
+}
+
+// Real-FFT split of pair (k, M-k) of one FFT held in LDS:
+//   E = (Z_k + conj Z_{M-k})/2, O = -i (Z_k - conj Z_{M-k})/2,
+//   X_k = E + W_N^k O,  X_{M-k} = conj(E - W_N^k O),  N = 2M.
+// For k = 0 only X_0 = Re Z_0 + Im Z_0 is a channel (Nyquist dropped).
+template <int M>
+__device__ __forceinline__ void rsplit(const float2* sh, int base, int k, const float2* __restrict__ tw,
+                                      float2& xk, float2& xmk)
+{
+    float2 a = sh[lpad(base + k)];
+    float2 b = sh[lpad(base + ((M - k) & (M - 1)))];
+    float2 e = make_float2(0.5f * (a.x + b.x), 0.5f * (a.y - b.y));
+    float2 o = make_float2(0.5f * (a.y + b.y), -0.5f * (a.x - b.x));
+    float2 w = tw[k * (FX_TW_N / (2 * M))];
+    float2 wo = cmul(w, o);
+    xk = cadd(e, wo);
+    xmk = make_float2(e.x - wo.x, -(e.y - wo.y));
+}
+
+// ---------------------------------------------------------------------------
+// Launchers (defined next to their kernels)
+// ---------------------------------------------------------------------------
+int fxk_unpack2b(hipStream_t s, const uint32_t* words, size_t ntime, int nchan, float* out);
+int fxk_synth_station(hipStream_t s, uint64_t seed, int station, double rho, long delay,
+                      double tone_amp, double tone_freq, double thresh, size_t nsamp,
+                      uint32_t* words);
+int fxk_synth_multichannel(hipStream_t s, uint64_t seed, int nchan, const double* tone_amp,
+                           const double* tone_freq, double thresh, size_t ntime, uint32_t* words);
+int fxk_synth_channelized(hipStream_t s, uint64_t seed, int nchan, long ntime, int ninp,
+                          double rho, double thresh, uint8_t* codes);
+
+int fxk_twiddle_init(hipStream_t s, float2* tw);
+int fxk_fft_c2c(hipStream_t s, const float2* tw, const float2* in, float2* out, int m, long batch);
+int fxk_fft_r2c(hipStream_t s, const float2* tw, const float* in, float2* out, int n, long batch,
+                int keep_nyquist);
+int fxk_spectrometer(hipStream_t s, const float2* tw, const float* window, const uint32_t* words,
+                     size_t ntime, int nchan, int n, int integrate_ffts, int nspec, float* partial,
+                     int groups, float* out);
+int fxk_fstage(hipStream_t s, const float2* tw, const uint32_t* words, size_t in_stride_words,
+               int ninp, int n, long t_first, long nt, const fx_phase_model* pm,
+               const float2* mult, const float* scale_chan, const float* scale_inp, int out_type,
+               void* out);
+int fxk_xengine(hipStream_t s, int in_type, const void* X, int nchan, long nt, int ninp,
+                float* acc);
+int fxk_xengine_codes(hipStream_t s, int mode, const uint8_t* codes, int nchan, long nt, int ninp,
+                      float* acc);
+size_t fxk_xengine_acc_floats(int nchan, int ninp, int nsets);
+int fxk_xengine_dump_pij(hipStream_t s, const float* acc, int nchan, int ninp, int nsets,
+                         const float* set_coef, const float* scale_chan, const float* scale_inp,
+                         const int2* pij, int accumulate, float2* vis);
+int fxk_mfma_probe(hipStream_t s, int which, float* out);
+int fxk_pcal_fold(hipStream_t s, const uint32_t* words, size_t nsamp, size_t position,
+                  int nfold, double* acc);
+int fxk_pcal_final(hipStream_t s, const double* acc, int nfold, int nbins, int ntones,
+                   int offset_bin, int step, int rotatorlen, double dphi, double* tones);
+int fxk_complex_mac(hipStream_t s, const float2* a, const float2* b, float2* acc, long n, int conj);
+int fxk_complex_mul(hipStream_t s, const float2* a, const float2* b, float2* out, long n);
+int fxk_sincos(hipStream_t s, const float* arg, float2* out, long n, int normalized);
+int fxk_sincos_gen(hipStream_t s, double start, double inc, float2* out, long n);

@@ -1,0 +1,382 @@
+// fx_misc.hip -- unpack, synthetic generators, phase-cal and elementwise DSP
+// kernels of the task library.
+//
+//   k_unpack2b         2-bit -> float through {-H,-1,+1,+H}
+//                      (more_raws_to_complex, minicorrelator_spu.c:662-692;
+//                      TwoBitUnpacker::extract_samples, DataUnpackers.cpp:174-227;
+//                      VDIF convention format_vdif.c:66-96)
+//   k_pcal_fold        exact FP64 fold of 2-bit samples into the phase-cal
+//                      period (PCal.cpp:265-288, 424-479, 610-644)
+//   k_pcal_final       tone extraction DFT (PCal.cpp:298-314, 489-515, 654-678)
+//   k_cmac / k_cmul    genericAddProduct_32fc[_conj] / genericMul_32fc
+//                      (architecture.h:270-277, 405-434), do_complex_mac
+//                      (cellspe-tasklib-dsp.cpp:454-514)
+//   k_sincos*          exec_sincos / exec_sincos_generator
+//                      (cellspe-tasklib-dsp.cpp:704-804)
+#include "fx_internal.h"
+
+namespace {
+
+// ---------------------------------------------------------------------------
+// unpack: thread per packed word; out[c*ntime + t]
+// ---------------------------------------------------------------------------
+__global__ void k_unpack2b(const uint32_t* __restrict__ words, size_t ntime, int nchan,
+                           float* __restrict__ out)
+{
+    const size_t w = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int spw = 16 / nchan;
+    const size_t t0 = w * (size_t)spw;
+    if (t0 >= ntime) return;
+    const uint32_t v = words[w];
+    if (nchan == 1) {
+        if (t0 + 16 <= ntime) {
+            float4* o = reinterpret_cast<float4*>(out + t0);
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                int b = 8 * q;
+                o[q] = make_float4(lut2((v >> b) & 3u), lut2((v >> (b + 2)) & 3u),
+                                   lut2((v >> (b + 4)) & 3u), lut2((v >> (b + 6)) & 3u));
+            }
+            return;
+        }
+    }
+    for (int s = 0; s < spw; s++) {
+        size_t t = t0 + (size_t)s;
+        if (t >= ntime) break;
+        for (int c = 0; c < nchan; c++)
+            out[(size_t)c * ntime + t] = lut2((v >> (2 * (s * nchan + c))) & 3u);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// synthetic generators: the oracle's counter-based model evaluated per word
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint64_t mix64(uint64_t x)
+{
+    x += 0x9E3779B97F4A7C15ull;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+    return x ^ (x >> 31);
+}
+__device__ __forceinline__ double uniform01(uint64_t key, uint64_t ctr)
+{
+    uint64_t r = mix64(key ^ mix64(ctr));
+    return ((double)(r >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+}
+__device__ __forceinline__ double gauss(uint64_t key, uint64_t ctr)
+{
+    double u1 = uniform01(key, 2 * ctr);
+    double u2 = uniform01(key, 2 * ctr + 1);
+    return sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2);
+}
+__device__ __forceinline__ uint32_t quant2(double x, double t)
+{
+    return x < -t ? 0u : (x < 0.0 ? 1u : (x < t ? 2u : 3u));
+}
+__host__ __device__ inline uint64_t stream_key(uint64_t seed, uint64_t stream)
+{
+    auto m = [](uint64_t x) {
+        x += 0x9E3779B97F4A7C15ull;
+        x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+        x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+        return x ^ (x >> 31);
+    };
+    return m(seed ^ m(stream + 0x51ED270B27A3D1C5ull));
+}
+
+__global__ void k_synth_station(uint64_t kc, uint64_t ko, double rho, long delay, double tone_amp,
+                                double tone_freq, double thresh, size_t nsamp,
+                                uint32_t* __restrict__ words)
+{
+    const size_t w = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (w * 16 >= nsamp) return;
+    const double a = sqrt(1.0 - rho * rho);
+    uint32_t v = 0;
+    for (int j = 0; j < 16; j++) {
+        size_t n = w * 16 + (size_t)j;
+        if (n >= nsamp) break;
+        double x = a * gauss(ko, n);
+        if (rho != 0.0) x += rho * gauss(kc, (uint64_t)((long)n - delay + 64));
+        if (tone_amp != 0.0) {
+            double ph = tone_freq * (double)n;
+            ph -= floor(ph);
+            x += tone_amp * cos(6.283185307179586 * ph);
+        }
+        v |= quant2(x, thresh) << (2 * j);
+    }
+    words[w] = v;
+}
+
+struct ToneSet {
+    double amp[16];
+    double freq[16];
+};
+
+__global__ void k_synth_multichannel(uint64_t seed, int nchan, ToneSet tones, double thresh,
+                                     size_t ntime, uint32_t* __restrict__ words)
+{
+    const size_t w = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int spw = 16 / nchan;
+    if (w * (size_t)spw >= ntime) return;
+    uint32_t v = 0;
+    for (int s = 0; s < spw; s++) {
+        size_t t = w * (size_t)spw + (size_t)s;
+        if (t >= ntime) break;
+        for (int c = 0; c < nchan; c++) {
+            double x = gauss(stream_key(seed, 0x200 + (uint64_t)c), t);
+            if (tones.amp[c] != 0.0) {
+                double ph = tones.freq[c] * (double)t;
+                ph -= floor(ph);
+                x += tones.amp[c] * cos(6.283185307179586 * ph);
+            }
+            v |= quant2(x, thresh) << (2 * (s * nchan + c));
+        }
+    }
+    words[w] = v;
+}
+
+__global__ void k_synth_channelized(uint64_t seed, int nchan, long ntime, int ninp, double rho,
+                                    double thresh, uint8_t* __restrict__ codes)
+{
+    const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t total = (size_t)nchan * (size_t)ntime * (size_t)ninp;
+    if (idx >= total) return;
+    const int i = (int)(idx % (size_t)ninp);
+    const size_t ct = idx / (size_t)ninp;
+    const long t = (long)(ct % (size_t)ntime);
+    const int c = (int)(ct / (size_t)ntime);
+    const double a = sqrt(1.0 - rho * rho);
+    const uint64_t kc = stream_key(seed, 0x10000 + (uint64_t)c);
+    const uint64_t ko = stream_key(seed, 0x100000000ull + (uint64_t)c * 65536u + (uint64_t)i);
+    double xr = rho * gauss(kc, (uint64_t)(2 * t)) + a * gauss(ko, (uint64_t)(2 * t));
+    double xi = rho * gauss(kc, (uint64_t)(2 * t + 1)) + a * gauss(ko, (uint64_t)(2 * t + 1));
+    codes[idx] = (uint8_t)(quant2(xr, thresh) | (quant2(xi, thresh) << 2));
+}
+
+// ---------------------------------------------------------------------------
+// phase-cal fold.  acc[(position + n) mod nfold] += LUT(code_n) in FP64.  The
+// summands are FP32 level values (multiples of 2^-22, |x| < 4), so every FP64
+// partial sum below 2^31 is exact: the result does not depend on the order of
+// the atomic additions and is bit-identical run to run.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_pcal_fold(const uint32_t* __restrict__ words, size_t nsamp,
+                                                   size_t position, int nfold, int use_lds,
+                                                   double* __restrict__ acc)
+{
+    extern __shared__ double lacc[];
+    const size_t w = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (use_lds) {
+        for (int b = threadIdx.x; b < nfold; b += blockDim.x) lacc[b] = 0.0;
+        __syncthreads();
+    }
+    if (w * 16 < nsamp) {
+        const uint32_t v = words[w];
+        size_t n0 = w * 16;
+        int bin = (int)((position + n0) % (size_t)nfold);
+        for (int j = 0; j < 16; j++) {
+            if (n0 + (size_t)j >= nsamp) break;
+            double x = (double)lut2((v >> (2 * j)) & 3u);
+            if (use_lds)
+                atomicAdd(&lacc[bin], x);
+            else
+                atomicAdd(&acc[bin], x);
+            if (++bin == nfold) bin = 0;
+        }
+    }
+    if (use_lds) {
+        __syncthreads();
+        for (int b = threadIdx.x; b < nfold; b += blockDim.x)
+            if (lacc[b] != 0.0) atomicAdd(&acc[b], lacc[b]);
+    }
+}
+
+// tone t at DFT bin k = offset_bin + t*step of the nbins-point fold:
+//   A[b] = sum_{r = b mod nbins} R[r] e^{i dphi (r mod rotatorlen)}
+//          (Shifting extractor, nfold = lcm(rotatorlen, nbins))
+//   A[b] = R[b]  (rotatorlen == 0: Trivial / ImplicitShift, nfold = nbins)
+//   tone = (1/nbins) sum_b A[b] e^{-2 pi i b k/nbins}
+__global__ void k_pcal_final(const double* __restrict__ acc, int nfold, int nbins, int ntones,
+                             int offset_bin, int step, int rotatorlen, double dphi,
+                             double* __restrict__ tones)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ntones) return;
+    const long k = (long)offset_bin + (long)t * step;
+    double sr = 0.0, si = 0.0;
+    if (k < nbins) {
+        for (int r = 0; r < nfold; r++) {
+            double v = acc[r];
+            if (v == 0.0) continue;
+            long e = ((long)(r % nbins) * k) % nbins;
+            double ang = -6.283185307179586 * (double)e / (double)nbins;
+            if (rotatorlen) ang += dphi * (double)(r % rotatorlen);
+            double sn, cs;
+            sincos(ang, &sn, &cs);
+            sr += v * cs;
+            si += v * sn;
+        }
+        sr /= (double)nbins;
+        si /= (double)nbins;
+    }
+    tones[2 * t] = sr;
+    tones[2 * t + 1] = si;
+}
+
+// ---------------------------------------------------------------------------
+// elementwise DSP
+// ---------------------------------------------------------------------------
+__global__ void k_cmac(const float2* __restrict__ a, const float2* __restrict__ b,
+                       float2* __restrict__ acc, long n, int conj)
+{
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float2 x = a[i], y = b[i], s = acc[i];
+    if (conj) {
+        s.x += (x.x * y.x) + (x.y * y.y);
+        s.y += (x.y * y.x) - (x.x * y.y);
+    } else {
+        s.x += (x.x * y.x) - (x.y * y.y);
+        s.y += (x.y * y.x) + (x.x * y.y);
+    }
+    acc[i] = s;
+}
+
+__global__ void k_cmul(const float2* __restrict__ a, const float2* __restrict__ b,
+                       float2* __restrict__ out, long n)
+{
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float2 x = a[i], y = b[i];
+    out[i] = make_float2((x.x * y.x) - (x.y * y.y), (x.y * y.x) + (x.x * y.y));
+}
+
+__global__ void k_sincos(const float* __restrict__ arg, float2* __restrict__ out, long n, int norm)
+{
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float s, c;
+    if (norm) {
+        // turns: reduce in FP64 so large arguments keep their fraction
+        double cyc = (double)arg[i];
+        cyc -= rint(cyc);
+        sincosf((float)(6.283185307179586 * cyc), &s, &c);
+    } else {
+        sincosf(arg[i], &s, &c);
+    }
+    out[i] = make_float2(c, s);
+}
+
+__global__ void k_sincos_gen(double start, double inc, float2* __restrict__ out, long n)
+{
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double ph = start + inc * (double)i;
+    double cyc = ph * 0.15915494309189535;
+    cyc -= rint(cyc);
+    float s, c;
+    sincosf((float)(6.283185307179586 * cyc), &s, &c);
+    out[i] = make_float2(c, s);
+}
+
+inline unsigned nblocks(size_t n, unsigned bs) { return (unsigned)((n + bs - 1) / bs); }
+inline int last_status() { return hipGetLastError() == hipSuccess ? FX_OK : FX_ERR_HIP; }
+
+} // namespace
+
+int fxk_unpack2b(hipStream_t s, const uint32_t* words, size_t ntime, int nchan, float* out)
+{
+    size_t nw = (ntime * (size_t)nchan + 15) / 16;
+    if (nw == 0) return FX_OK;
+    hipLaunchKernelGGL(k_unpack2b, dim3(nblocks(nw, 256)), dim3(256), 0, s, words, ntime, nchan, out);
+    return last_status();
+}
+
+int fxk_synth_station(hipStream_t s, uint64_t seed, int station, double rho, long delay,
+                      double tone_amp, double tone_freq, double thresh, size_t nsamp,
+                      uint32_t* words)
+{
+    size_t nw = (nsamp + 15) / 16;
+    if (nw == 0) return FX_OK;
+    uint64_t kc = stream_key(seed, 0xC0);
+    uint64_t ko = stream_key(seed, 0x100 + (uint64_t)station);
+    hipLaunchKernelGGL(k_synth_station, dim3(nblocks(nw, 256)), dim3(256), 0, s, kc, ko, rho, delay,
+                       tone_amp, tone_freq, thresh, nsamp, words);
+    return last_status();
+}
+
+int fxk_synth_multichannel(hipStream_t s, uint64_t seed, int nchan, const double* tone_amp,
+                           const double* tone_freq, double thresh, size_t ntime, uint32_t* words)
+{
+    ToneSet ts;
+    for (int c = 0; c < 16; c++) {
+        ts.amp[c] = (tone_amp && c < nchan) ? tone_amp[c] : 0.0;
+        ts.freq[c] = (tone_freq && c < nchan) ? tone_freq[c] : 0.0;
+    }
+    int spw = 16 / nchan;
+    size_t nw = (ntime + (size_t)spw - 1) / (size_t)spw;
+    if (nw == 0) return FX_OK;
+    hipLaunchKernelGGL(k_synth_multichannel, dim3(nblocks(nw, 256)), dim3(256), 0, s, seed, nchan,
+                       ts, thresh, ntime, words);
+    return last_status();
+}
+
+int fxk_synth_channelized(hipStream_t s, uint64_t seed, int nchan, long ntime, int ninp,
+                          double rho, double thresh, uint8_t* codes)
+{
+    size_t total = (size_t)nchan * (size_t)ntime * (size_t)ninp;
+    if (total == 0) return FX_OK;
+    hipLaunchKernelGGL(k_synth_channelized, dim3(nblocks(total, 256)), dim3(256), 0, s, seed, nchan,
+                       ntime, ninp, rho, thresh, codes);
+    return last_status();
+}
+
+int fxk_pcal_fold(hipStream_t s, const uint32_t* words, size_t nsamp, size_t position, int nfold,
+                  double* acc)
+{
+    size_t nw = (nsamp + 15) / 16;
+    if (nw == 0) return FX_OK;
+    int use_lds = nfold <= 8192;
+    size_t lds = use_lds ? (size_t)nfold * sizeof(double) : 0;
+    hipLaunchKernelGGL(k_pcal_fold, dim3(nblocks(nw, 256)), dim3(256), lds, s, words, nsamp,
+                       position, nfold, use_lds, acc);
+    return last_status();
+}
+
+int fxk_pcal_final(hipStream_t s, const double* acc, int nfold, int nbins, int ntones,
+                   int offset_bin, int step, int rotatorlen, double dphi, double* tones)
+{
+    if (ntones <= 0) return FX_OK;
+    hipLaunchKernelGGL(k_pcal_final, dim3(nblocks((size_t)ntones, 64)), dim3(64), 0, s, acc, nfold,
+                       nbins, ntones, offset_bin, step, rotatorlen, dphi, tones);
+    return last_status();
+}
+
+int fxk_complex_mac(hipStream_t s, const float2* a, const float2* b, float2* acc, long n, int conj)
+{
+    if (n <= 0) return FX_OK;
+    hipLaunchKernelGGL(k_cmac, dim3(nblocks((size_t)n, 256)), dim3(256), 0, s, a, b, acc, n, conj);
+    return last_status();
+}
+
+int fxk_complex_mul(hipStream_t s, const float2* a, const float2* b, float2* out, long n)
+{
+    if (n <= 0) return FX_OK;
+    hipLaunchKernelGGL(k_cmul, dim3(nblocks((size_t)n, 256)), dim3(256), 0, s, a, b, out, n);
+    return last_status();
+}
+
+int fxk_sincos(hipStream_t s, const float* arg, float2* out, long n, int normalized)
+{
+    if (n <= 0) return FX_OK;
+    hipLaunchKernelGGL(k_sincos, dim3(nblocks((size_t)n, 256)), dim3(256), 0, s, arg, out, n,
+                       normalized);
+    return last_status();
+}
+
+int fxk_sincos_gen(hipStream_t s, double start, double inc, float2* out, long n)
+{
+    if (n <= 0) return FX_OK;
+    hipLaunchKernelGGL(k_sincos_gen, dim3(nblocks((size_t)n, 256)), dim3(256), 0, s, start, inc,
+                       out, n);
+    return last_status();
+}

@@ -1,0 +1,332 @@
+"""ctypes binding of libcellspe_fx.so (include/cellspe_fx.h and the task-library
+headers).
+
+This is the Python-side view of the C-ABI used by the tests and the benchmark.
+It holds no compute of its own: every call goes to the gfx950 kernels through
+the shared library, and importing it fails loudly when the library has not
+been built.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libcellspe_fx.so")
+
+FX_OK = 0
+FX_PREC_F32, FX_PREC_F16, FX_PREC_FP8, FX_PREC_FP8_2BIT = 0, 1, 2, 3
+FX_WIN_NONE, FX_WIN_SIN2, FX_WIN_COSINE, FX_WIN_COSINE2 = 0, 1, 2, 3
+
+
+class FxError(RuntimeError):
+    pass
+
+
+class PhaseModel(C.Structure):
+    _fields_ = [("delay0", C.c_double), ("delay_rate", C.c_double),
+                ("fringe0", C.c_double), ("fringe_rate", C.c_double)]
+
+
+class CorrConfig(C.Structure):
+    _fields_ = [("ninp", C.c_int), ("nfft", C.c_int), ("chunk_spectra", C.c_long),
+                ("prec", C.c_int), ("use_phase", C.c_int)]
+
+
+_lib = None
+
+
+def load():
+    """Load the shared library (raises if it is missing)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise FxError(f"{LIB_PATH} is not built; run __graft_entry__.build() first")
+    lib = C.CDLL(LIB_PATH)
+    vp, sz, i, l, d, u64 = C.c_void_p, C.c_size_t, C.c_int, C.c_long, C.c_double, C.c_uint64
+    sigs = {
+        "fx_last_error": (C.c_char_p, []),
+        "fx_device_count": (i, []),
+        "fx_device_name": (i, [i, C.c_char_p, i]),
+        "fx_ctx_create": (i, [i, C.POINTER(vp)]),
+        "fx_ctx_destroy": (i, [vp]),
+        "fx_ctx_sync": (i, [vp]),
+        "fx_ctx_stream": (vp, [vp]),
+        "fx_malloc": (i, [vp, sz, C.POINTER(vp)]),
+        "fx_free": (i, [vp, vp]),
+        "fx_memcpy_h2d": (i, [vp, vp, vp, sz]),
+        "fx_memcpy_d2h": (i, [vp, vp, vp, sz]),
+        "fx_memcpy_d2d": (i, [vp, vp, vp, sz]),
+        "fx_memset": (i, [vp, vp, i, sz]),
+        "fx_event_create": (i, [vp, C.POINTER(vp)]),
+        "fx_event_record": (i, [vp, vp]),
+        "fx_event_elapsed_ms": (i, [vp, vp, C.POINTER(C.c_float)]),
+        "fx_event_destroy": (i, [vp]),
+        "fx_synth_station": (i, [vp, u64, i, d, l, d, d, d, sz, vp]),
+        "fx_synth_multichannel": (i, [vp, u64, i, vp, vp, d, sz, vp]),
+        "fx_synth_channelized": (i, [vp, u64, i, l, i, d, d, vp]),
+        "fx_unpack2b": (i, [vp, vp, sz, i, vp]),
+        "fx_fft_c2c": (i, [vp, vp, vp, i, l]),
+        "fx_fft_r2c": (i, [vp, vp, vp, i, l, i]),
+        "fx_complex_mac": (i, [vp, vp, vp, vp, l, i]),
+        "fx_complex_mul": (i, [vp, vp, vp, vp, l]),
+        "fx_sincos": (i, [vp, vp, vp, l, i]),
+        "fx_sincos_generator": (i, [vp, d, d, vp, l]),
+        "fx_spectrometer": (i, [vp, vp, sz, i, i, i, i, vp]),
+        "fx_pcal_create": (i, [vp, d, d, i, sz, C.POINTER(vp)]),
+        "fx_pcal_destroy": (i, [vp]),
+        "fx_pcal_clear": (i, [vp]),
+        "fx_pcal_adjust_sample_offset": (i, [vp, sz]),
+        "fx_pcal_extract_2bit": (i, [vp, vp, sz]),
+        "fx_pcal_final": (i, [vp, vp, C.POINTER(u64)]),
+        "fx_pcal_info": (i, [vp, C.POINTER(i), C.POINTER(i), C.POINTER(i)]),
+        "fx_corr_create": (i, [vp, C.POINTER(CorrConfig), C.POINTER(vp)]),
+        "fx_corr_destroy": (i, [vp]),
+        "fx_corr_set_phase_model": (i, [vp, vp]),
+        "fx_corr_set_multipliers": (i, [vp, vp]),
+        "fx_corr_set_scales": (i, [vp, vp, vp]),
+        "fx_corr_process": (i, [vp, vp, sz, l, l]),
+        "fx_corr_xengine": (i, [vp, vp, l]),
+        "fx_corr_dump": (i, [vp, vp]),
+        "fx_corr_reset": (i, [vp]),
+        "fx_corr_fstage": (i, [vp, vp, sz, l, l, vp, i]),
+        "fx_corr_info": (i, [vp, C.POINTER(l), C.POINTER(sz), C.POINTER(sz)]),
+        "fx_mfma_probe": (i, [vp, i, vp]),
+    }
+    for name, (res, args) in sigs.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def check(rc):
+    if rc != FX_OK:
+        msg = load().fx_last_error().decode(errors="replace")
+        raise FxError(f"fx call failed ({rc}): {msg}")
+    return rc
+
+
+def _ptr(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+class DeviceBuffer:
+    """A device allocation owned by a Context."""
+
+    def __init__(self, ctx, nbytes):
+        self.ctx = ctx
+        self.nbytes = int(nbytes)
+        p = C.c_void_p()
+        check(load().fx_malloc(ctx.handle, self.nbytes, C.byref(p)))
+        self.ptr = p
+
+    def free(self):
+        if self.ptr is not None and self.ptr.value:
+            load().fx_free(self.ctx.handle, self.ptr)
+        self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+    def upload(self, arr):
+        arr = np.ascontiguousarray(arr)
+        assert arr.nbytes <= self.nbytes
+        check(load().fx_memcpy_h2d(self.ctx.handle, self.ptr, _ptr(arr), arr.nbytes))
+
+    def download(self, dtype, count=None, shape=None):
+        dtype = np.dtype(dtype)
+        if count is None:
+            count = self.nbytes // dtype.itemsize
+        out = np.empty(count, dtype=dtype)
+        check(load().fx_memcpy_d2h(self.ctx.handle, _ptr(out), self.ptr, out.nbytes))
+        return out.reshape(shape) if shape is not None else out
+
+    def zero(self):
+        check(load().fx_memset(self.ctx.handle, self.ptr, 0, self.nbytes))
+
+
+class Context:
+    def __init__(self, device=0):
+        lib = load()
+        h = C.c_void_p()
+        check(lib.fx_ctx_create(device, C.byref(h)))
+        self.handle = h
+        self.lib = lib
+
+    def close(self):
+        if self.handle is not None:
+            self.lib.fx_ctx_destroy(self.handle)
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def sync(self):
+        check(self.lib.fx_ctx_sync(self.handle))
+
+    def alloc(self, nbytes):
+        return DeviceBuffer(self, max(int(nbytes), 1))
+
+    def to_device(self, arr):
+        arr = np.ascontiguousarray(arr)
+        b = self.alloc(arr.nbytes)
+        b.upload(arr)
+        return b
+
+    # events on the context stream
+    def event(self):
+        e = C.c_void_p()
+        check(self.lib.fx_event_create(self.handle, C.byref(e)))
+        return e
+
+    def record(self, ev):
+        check(self.lib.fx_event_record(self.handle, ev))
+
+    def elapsed_ms(self, a, b):
+        ms = C.c_float()
+        check(self.lib.fx_event_elapsed_ms(a, b, C.byref(ms)))
+        return ms.value
+
+    # ---- primitives ---------------------------------------------------
+    def unpack2b(self, d_words, ntime, nchan, d_out):
+        check(self.lib.fx_unpack2b(self.handle, d_words.ptr, ntime, nchan, d_out.ptr))
+
+    def fft_c2c(self, d_in, d_out, n, batch):
+        check(self.lib.fx_fft_c2c(self.handle, d_in.ptr, d_out.ptr, n, batch))
+
+    def fft_r2c(self, d_in, d_out, n, batch, keep_nyquist=False):
+        check(self.lib.fx_fft_r2c(self.handle, d_in.ptr, d_out.ptr, n, batch, int(keep_nyquist)))
+
+    def spectrometer(self, d_words, ntime, nchan, n, integrate_ffts, window, d_out):
+        check(self.lib.fx_spectrometer(self.handle, d_words.ptr, ntime, nchan, n,
+                                       integrate_ffts, window, d_out.ptr))
+
+    def complex_mac(self, a, b, acc, n, conj):
+        check(self.lib.fx_complex_mac(self.handle, a.ptr, b.ptr, acc.ptr, n, int(conj)))
+
+    def synth_station(self, seed, station, rho, delay, tone_amp, tone_freq, thresh, nsamp, d_words):
+        check(self.lib.fx_synth_station(self.handle, seed, station, rho, delay, tone_amp,
+                                        tone_freq, thresh, nsamp, d_words.ptr))
+
+    def synth_multichannel(self, seed, nchan, amp, freq, thresh, ntime, d_words):
+        a = np.ascontiguousarray(amp, dtype=np.float64)
+        f = np.ascontiguousarray(freq, dtype=np.float64)
+        check(self.lib.fx_synth_multichannel(self.handle, seed, nchan, _ptr(a), _ptr(f), thresh,
+                                             ntime, d_words.ptr))
+
+    def synth_channelized(self, seed, nchan, ntime, ninp, rho, thresh, d_codes):
+        check(self.lib.fx_synth_channelized(self.handle, seed, nchan, ntime, ninp, rho, thresh,
+                                            d_codes.ptr))
+
+    def mfma_probe(self, which):
+        out = np.zeros((32, 32), dtype=np.float32)
+        check(self.lib.fx_mfma_probe(self.handle, which, _ptr(out)))
+        return out
+
+
+class PCal:
+    """Device phase-cal extractor with the reference PCal interface."""
+
+    def __init__(self, ctx, bandwidth_hz, spacing_hz, offset_hz, sampleoffset=0):
+        self.ctx = ctx
+        h = C.c_void_p()
+        check(load().fx_pcal_create(ctx.handle, bandwidth_hz, spacing_hz, int(offset_hz),
+                                    sampleoffset, C.byref(h)))
+        self.handle = h
+        k, nb, nt = C.c_int(), C.c_int(), C.c_int()
+        check(load().fx_pcal_info(h, C.byref(k), C.byref(nb), C.byref(nt)))
+        self.kind, self.nbins, self.ntones = k.value, nb.value, nt.value
+
+    def clear(self):
+        check(load().fx_pcal_clear(self.handle))
+
+    def adjust_sample_offset(self, off):
+        check(load().fx_pcal_adjust_sample_offset(self.handle, off))
+
+    def extract_2bit(self, d_words, nsamp):
+        check(load().fx_pcal_extract_2bit(self.handle, d_words.ptr, nsamp))
+
+    def final(self):
+        out = np.zeros(max(self.ntones, 1), dtype=np.complex128)
+        n = C.c_uint64()
+        check(load().fx_pcal_final(self.handle, _ptr(out), C.byref(n)))
+        return out[: self.ntones], n.value
+
+    def close(self):
+        if self.handle is not None:
+            load().fx_pcal_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Correlator:
+    def __init__(self, ctx, ninp, nfft, chunk_spectra, prec=FX_PREC_F32, use_phase=False):
+        self.ctx = ctx
+        self.cfg = CorrConfig(ninp, nfft, chunk_spectra, prec, int(use_phase))
+        h = C.c_void_p()
+        check(load().fx_corr_create(ctx.handle, C.byref(self.cfg), C.byref(h)))
+        self.handle = h
+        self.ninp, self.nfft, self.nchan = ninp, nfft, nfft // 2
+        self.nprod = ninp * (ninp + 1) // 2
+
+    def set_phase_model(self, models):
+        arr = (PhaseModel * self.ninp)(*[PhaseModel(*m) for m in models])
+        check(load().fx_corr_set_phase_model(self.handle, C.cast(arr, C.c_void_p)))
+
+    def set_scales(self, chan=None, inp=None):
+        c = None if chan is None else np.ascontiguousarray(chan, dtype=np.float32)
+        i = None if inp is None else np.ascontiguousarray(inp, dtype=np.float32)
+        check(load().fx_corr_set_scales(self.handle, None if c is None else _ptr(c),
+                                        None if i is None else _ptr(i)))
+
+    def process(self, d_words, stride_words, t_first, nspectra, offset_bytes=0):
+        ptr = C.c_void_p(d_words.ptr.value + offset_bytes)
+        check(load().fx_corr_process(self.handle, ptr, stride_words, t_first, nspectra))
+
+    def fstage(self, d_words, stride_words, t_first, nspectra, d_chan, out_prec):
+        check(load().fx_corr_fstage(self.handle, d_words.ptr, stride_words, t_first, nspectra,
+                                    d_chan.ptr, out_prec))
+
+    def xengine(self, d_chan, nspectra):
+        check(load().fx_corr_xengine(self.handle, d_chan.ptr, nspectra))
+
+    def dump(self, d_vis):
+        check(load().fx_corr_dump(self.handle, d_vis.ptr))
+
+    def reset(self):
+        check(load().fx_corr_reset(self.handle))
+
+    def visibilities(self):
+        """Dump to a temporary device buffer and return [P][C] complex64."""
+        b = self.ctx.alloc(self.nprod * self.nchan * 8)
+        self.dump(b)
+        v = b.download(np.complex64, shape=(self.nprod, self.nchan))
+        b.free()
+        return v
+
+    def close(self):
+        if self.handle is not None:
+            load().fx_corr_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,178 @@
+/*
+ * cellspe_fx.h -- C-ABI of the MI355X (gfx950) F-X engine.
+ *
+ * Device-pointer API of the hot path: 2-bit unpack, FFT, FP64 phase model
+ * rotation, phase-cal extraction, the X-engine (FP32 / FP16 / FP8 MFMA) and
+ * the spectrometer power accumulation.  The reference's own drop-in surfaces
+ * (celldsp_* task runtime, spe* DSP wrappers, minicorrelator control block,
+ * spectrometer control block) are declared in celldsp_tasklib.h,
+ * celldsp_dsp.h, minicorrelator_fx.h and spectrometer_fx.h and are built on
+ * these calls.
+ *
+ * All calls are plain C: pointers, sizes and integer codes.  Work is enqueued
+ * on the context's HIP stream; calls return once the work is enqueued unless
+ * stated otherwise.  Every call returns FX_OK (0) or a negative FX_ERR_* code
+ * and leaves a message for fx_last_error().
+ */
+#ifndef CELLSPE_FX_H
+#define CELLSPE_FX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status ------------------------------------------------------------ */
+enum {
+    FX_OK = 0,
+    FX_ERR_ARG = -1,         /* invalid argument / unsupported size       */
+    FX_ERR_HIP = -2,         /* HIP runtime error                          */
+    FX_ERR_NOMEM = -3,       /* device allocation failed                   */
+    FX_ERR_UNSUPPORTED = -4, /* valid request this build does not support  */
+    FX_ERR_STATE = -5        /* call out of order (finalized, no device)   */
+};
+const char* fx_last_error(void);
+
+/* ---- types --------------------------------------------------------------- */
+typedef struct { float re, im; } fx_cf32;   /* types_generic.h:44-45 cf32 */
+typedef struct { double re, im; } fx_cf64;
+
+/* FP64 phase model per input: phi(k,t) = 2 pi k tau(t)/N + fringe(t),
+ * tau(t) = delay0 + delay_rate t [samples], fringe(t) = fringe0 + fringe_rate t
+ * [rad], t = spectrum index.  Reduced mod 2 pi in FP64, applied in FP32. */
+typedef struct {
+    double delay0, delay_rate, fringe0, fringe_rate;
+} fx_phase_model;
+
+/* X-engine input precisions */
+enum {
+    FX_PREC_F32 = 0,     /* plain FP32 MFMA (v_mfma_f32_32x32x2_f32)              */
+    FX_PREC_F16 = 1,     /* FP16 in, FP32 accumulate (v_mfma_f32_32x32x16_f16)     */
+    FX_PREC_FP8 = 2,     /* FP8-e4m3 in, FP32 accumulate, per-channel/input scales */
+    FX_PREC_FP8_2BIT = 3 /* exact FP8 decomposition of 2-bit {+-1,+-H} voltages    */
+};
+
+/* spectrometer windows */
+enum { FX_WIN_NONE = 0, FX_WIN_SIN2 = 1, FX_WIN_COSINE = 2, FX_WIN_COSINE2 = 3 };
+
+/* ---- context ------------------------------------------------------------- */
+typedef struct fx_ctx fx_ctx;
+int   fx_ctx_create(int device, fx_ctx** ctx);
+int   fx_ctx_destroy(fx_ctx* ctx);
+int   fx_ctx_sync(fx_ctx* ctx);
+void* fx_ctx_stream(fx_ctx* ctx);           /* hipStream_t of the context */
+int   fx_device_count(void);
+int   fx_device_name(int device, char* buf, int len);
+
+/* ---- memory and timing helpers ------------------------------------------ */
+int fx_malloc(fx_ctx* ctx, size_t bytes, void** dptr);
+int fx_free(fx_ctx* ctx, void* dptr);
+int fx_memcpy_h2d(fx_ctx* ctx, void* dst, const void* src, size_t bytes);   /* synchronous */
+int fx_memcpy_d2h(fx_ctx* ctx, void* dst, const void* src, size_t bytes);   /* synchronous */
+int fx_memcpy_d2d(fx_ctx* ctx, void* dst, const void* src, size_t bytes);   /* async */
+int fx_memset(fx_ctx* ctx, void* dst, int value, size_t bytes);             /* async */
+int fx_event_create(fx_ctx* ctx, void** ev);
+int fx_event_record(fx_ctx* ctx, void* ev);
+int fx_event_elapsed_ms(void* ev_start, void* ev_stop, float* ms);          /* syncs stop */
+int fx_event_destroy(void* ev);
+
+/* ---- synthetic signal generators (benchmarks and tests) ----------------- */
+/* One 2-bit station stream of nsamp samples (see fx_oracle for the model). */
+int fx_synth_station(fx_ctx* ctx, uint64_t seed, int station, double rho, long delay,
+                     double tone_amp, double tone_freq_norm, double thresh, size_t nsamp,
+                     uint32_t* d_words);
+/* nchan-interleaved 2-bit stream (VDIF layout), per-channel tone (host arrays). */
+int fx_synth_multichannel(fx_ctx* ctx, uint64_t seed, int nchan, const double* tone_amp,
+                          const double* tone_freq_norm, double thresh, size_t ntime,
+                          uint32_t* d_words);
+/* 2-bit channelized voltages, codes[c][t][i] = re | im << 2. */
+int fx_synth_channelized(fx_ctx* ctx, uint64_t seed, int nchan, long ntime, int ninp,
+                         double rho, double thresh, uint8_t* d_codes);
+
+/* ---- primitives ------------------------------------------------------------ */
+/* a1: 2-bit unpack through {-3.3359,-1,+1,+3.3359}; nchan in {1,2,4,8,16};
+ * out[c*ntime + t]. */
+int fx_unpack2b(fx_ctx* ctx, const uint32_t* d_words, size_t ntime, int nchan, float* d_out);
+/* a3: forward complex FFT, unnormalized, natural order; n = 2^k, 2..16384. */
+int fx_fft_c2c(fx_ctx* ctx, const fx_cf32* d_in, fx_cf32* d_out, int n, long batch);
+/* a3: real-input FFT, n = 2^k, 4..32768; out has n/2 (or n/2+1 with
+ * keep_nyquist) complex bins per transform. */
+int fx_fft_r2c(fx_ctx* ctx, const float* d_in, fx_cf32* d_out, int n, long batch,
+               int keep_nyquist);
+/* acc += a*b (conj=0, genericAddProduct_32fc) or a*conj(b) (conj=1). */
+int fx_complex_mac(fx_ctx* ctx, const fx_cf32* d_a, const fx_cf32* d_b, fx_cf32* d_acc,
+                   long n, int conj);
+int fx_complex_mul(fx_ctx* ctx, const fx_cf32* d_a, const fx_cf32* d_b, fx_cf32* d_out, long n);
+/* (cos, sin) of arg (radians, or turns when normalized != 0). */
+int fx_sincos(fx_ctx* ctx, const float* d_arg, fx_cf32* d_out, long n, int normalized);
+/* oscillator output (cos, sin)(start + i*inc), phase accumulated in FP64. */
+int fx_sincos_generator(fx_ctx* ctx, double start, double inc, fx_cf32* d_out, long n);
+
+/* ---- a8: spectrometer ------------------------------------------------------ */
+/* nchan interleaved 2-bit channels of ntime samples; n-point real FFT per
+ * channel, window, |X|^2 summed over integrate_ffts FFTs and scaled by
+ * 1/integrate_ffts.  d_out[s][c][n/2], s < ntime/(n*integrate_ffts). */
+int fx_spectrometer(fx_ctx* ctx, const uint32_t* d_words, size_t ntime, int nchan, int n,
+                    int integrate_ffts, int window, float* d_out);
+
+/* ---- a5: phase-cal ------------------------------------------------------- */
+typedef struct fx_pcal fx_pcal;
+/* PCal::getNew semantics: picks the Trivial, ImplicitShift or Shifting
+ * extractor from (bandwidth, spacing, offset). */
+int fx_pcal_create(fx_ctx* ctx, double bandwidth_hz, double spacing_hz, int offset_hz,
+                   size_t sampleoffset, fx_pcal** pc);
+int fx_pcal_destroy(fx_pcal* pc);
+int fx_pcal_clear(fx_pcal* pc);
+int fx_pcal_adjust_sample_offset(fx_pcal* pc, size_t sampleoffset);
+/* fold nsamp 2-bit samples (1-channel packing) starting at sample 0 of d_words */
+int fx_pcal_extract_2bit(fx_pcal* pc, const uint32_t* d_words, size_t nsamp);
+/* host output, ntones complex; returns samples integrated via *nsamples */
+int fx_pcal_final(fx_pcal* pc, fx_cf64* tones, uint64_t* nsamples);
+int fx_pcal_info(fx_pcal* pc, int* kind, int* nbins, int* ntones);
+
+/* ---- a3/a4/a6/a7: correlator -------------------------------------------- */
+typedef struct {
+    int    ninp;          /* inputs = stations x polarizations                  */
+    int    nfft;          /* real FFT length N; C = N/2 channels               */
+    long   chunk_spectra; /* spectra per F->X chunk (channelized buffer depth)  */
+    int    prec;          /* FX_PREC_*                                          */
+    int    use_phase;     /* apply the FP64 phase model                         */
+} fx_corr_config;
+typedef struct fx_corr fx_corr;
+int fx_corr_create(fx_ctx* ctx, const fx_corr_config* cfg, fx_corr** corr);
+int fx_corr_destroy(fx_corr* corr);
+/* host array of ninp phase models (copied) */
+int fx_corr_set_phase_model(fx_corr* corr, const fx_phase_model* pm);
+/* optional per-(channel) multiplier table [ninp][C] (device), applied after
+ * the FFT as genericMul_32fc; NULL disables */
+int fx_corr_set_multipliers(fx_corr* corr, const fx_cf32* d_mult);
+/* host arrays: per-channel [C] and per-input [ninp] power scales applied
+ * before FP16/FP8 conversion and removed in the dump (NULL = defaults) */
+int fx_corr_set_scales(fx_corr* corr, const float* chan_scale, const float* inp_scale);
+/* F + X over spectra [t_first, t_first+nspectra): d_words points at input 0,
+ * input i at d_words + i*in_stride_words; spectrum t starts at sample t*N. */
+int fx_corr_process(fx_corr* corr, const uint32_t* d_words, size_t in_stride_words,
+                    long t_first, long nspectra);
+/* X only over channelized data in the engine layout [c][t][i]
+ * (FP32/FP16 complex) or 2-bit codes (FX_PREC_FP8_2BIT: byte re|im<<2). */
+int fx_corr_xengine(fx_corr* corr, const void* d_chan, long nspectra);
+/* write visibilities [p][c] complex64, p = i*I - i(i-1)/2 + (j-i), i <= j */
+int fx_corr_dump(fx_corr* corr, fx_cf32* d_vis);
+int fx_corr_reset(fx_corr* corr);
+/* F-stage alone into a caller buffer laid out [c][t][i] (FP32 complex) */
+int fx_corr_fstage(fx_corr* corr, const uint32_t* d_words, size_t in_stride_words,
+                   long t_first, long nspectra, void* d_chan, int out_prec);
+int fx_corr_info(fx_corr* corr, long* chunk_spectra, size_t* chan_bytes, size_t* acc_bytes);
+
+/* ---- diagnostics --------------------------------------------------------- */
+/* MFMA lane-layout probe: which = 0 (f32 32x32x2), 1 (f16 32x32x16),
+ * 2 (fp8 32x32x64); writes 32x32 floats D[i][j] for A[i][k] = i+1 when
+ * k == 0 else 0, B[k][j] = j+1 when k == 0 else 0 (so D[i][j] = (i+1)(j+1)). */
+int fx_mfma_probe(fx_ctx* ctx, int which, float* host_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,322 @@
+"""ctypes binding of the CPU oracle (oracle/liboracle_fx.so).
+
+TEST INFRASTRUCTURE ONLY: the checker for the HIP F-X engine.  Imported by
+tests/, by __graft_entry__.smoke() and by bench.py's cpu_baseline leg; never
+by product code.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle_fx.so")
+
+LUT = np.array([-3.3359, -1.0, 1.0, 3.3359], dtype=np.float32)
+WIN_NONE, WIN_SIN2, WIN_COSINE, WIN_COSINE2 = 0, 1, 2, 3
+PCAL_TRIVIAL, PCAL_IMPLICIT, PCAL_SHIFTING = 0, 1, 2
+
+
+class PhaseModel(C.Structure):
+    _fields_ = [("delay0", C.c_double), ("delay_rate", C.c_double),
+                ("fringe0", C.c_double), ("fringe_rate", C.c_double)]
+
+
+class PCalCfg(C.Structure):
+    _fields_ = [("kind", C.c_int), ("fs_hz", C.c_double), ("spacing_hz", C.c_double),
+                ("offset_hz", C.c_double), ("nbins", C.c_int), ("ntones", C.c_int),
+                ("rotatorlen", C.c_int), ("step", C.c_int), ("offset_bin", C.c_int)]
+
+
+_lib = None
+
+
+def build():
+    subprocess.check_call(["make", "-s", "-C", HERE, "liboracle_fx.so"])
+
+
+def load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        build()
+    lib = C.CDLL(LIB_PATH)
+    vp, sz, i, l, d, u64 = C.c_void_p, C.c_size_t, C.c_int, C.c_long, C.c_double, C.c_uint64
+    sigs = {
+        "fxo_mix64": (u64, [u64]),
+        "fxo_uniform": (d, [u64, u64]),
+        "fxo_gauss": (d, [u64, u64]),
+        "fxo_stream_key": (u64, [u64, u64]),
+        "fxo_quantize2": (i, [d, d]),
+        "fxo_pack2b": (None, [vp, sz, vp]),
+        "fxo_gen_station": (None, [u64, i, d, l, d, d, d, sz, vp]),
+        "fxo_gen_multichannel": (None, [u64, i, vp, vp, d, sz, vp]),
+        "fxo_gen_channelized_codes": (None, [u64, i, l, i, d, d, vp]),
+        "fxo_unpack2b_1ch": (None, [vp, sz, vp]),
+        "fxo_unpack2b_nch": (None, [vp, sz, i, vp]),
+        "fxo_twiddle_r2": (None, [i, vp]),
+        "fxo_fft_r2_c2c": (None, [vp, i, vp]),
+        "fxo_rfft_r2": (None, [vp, i, vp, vp, vp]),
+        "fxo_fft_f64": (None, [vp, i]),
+        "fxo_dft_f64": (None, [vp, i, vp]),
+        "fxo_phase": (d, [vp, i, i, l]),
+        "fxo_rotate": (None, [vp, i, i, vp, l]),
+        "fxo_mul": (None, [vp, vp, vp, i]),
+        "fxo_addproduct": (None, [vp, vp, vp, i]),
+        "fxo_addproduct_conj": (None, [vp, vp, vp, i]),
+        "fxo_prod_index": (sz, [i, i, i]),
+        "fxo_nprod": (sz, [i]),
+        "fxo_xengine_f32": (None, [vp, i, l, i, vp]),
+        "fxo_xengine_f64": (None, [vp, i, l, i, vp]),
+        "fxo_corr_f32": (None, [vp, i, i, l, vp, vp, i]),
+        "fxo_corr_f64": (None, [vp, i, i, l, vp, vp, i]),
+        "fxo_fstage_f32": (None, [vp, i, i, l, l, vp, vp]),
+        "fxo_window": (None, [i, i, vp]),
+        "fxo_spectrometer_f32": (None, [vp, sz, i, i, i, i, vp, i]),
+        "fxo_spectrometer_f64": (None, [vp, sz, i, i, i, i, vp]),
+        "fxo_gcd": (C.c_longlong, [l, l]),
+        "fxo_pcal_config": (None, [d, d, i, C.POINTER(PCalCfg)]),
+        "fxo_pcal_extract": (None, [C.POINTER(PCalCfg), vp, sz, sz, vp]),
+        "fxo_pcal_final": (None, [C.POINTER(PCalCfg), vp, vp]),
+        "fxo_pcal_reference": (None, [C.POINTER(PCalCfg), vp, sz, sz, vp]),
+    }
+    for name, (res, args) in sigs.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+# ---- generators -----------------------------------------------------------
+def gen_station(seed, station, rho, delay, tone_amp, tone_freq, thresh, nsamp):
+    w = np.zeros((nsamp + 15) // 16, dtype=np.uint32)
+    load().fxo_gen_station(seed, station, rho, delay, tone_amp, tone_freq, thresh, nsamp, _p(w))
+    return w
+
+
+def gen_multichannel(seed, nchan, amp, freq, thresh, ntime):
+    spw = 16 // nchan
+    w = np.zeros((ntime + spw - 1) // spw, dtype=np.uint32)
+    a = np.ascontiguousarray(amp, dtype=np.float64)
+    f = np.ascontiguousarray(freq, dtype=np.float64)
+    load().fxo_gen_multichannel(seed, nchan, _p(a), _p(f), thresh, ntime, _p(w))
+    return w
+
+
+def gen_channelized_codes(seed, nchan, ntime, ninp, rho, thresh):
+    c = np.zeros(nchan * ntime * ninp, dtype=np.uint8)
+    load().fxo_gen_channelized_codes(seed, nchan, ntime, ninp, rho, thresh, _p(c))
+    return c.reshape(nchan, ntime, ninp)
+
+
+def pack2b(codes):
+    codes = np.ascontiguousarray(codes, dtype=np.uint8)
+    w = np.zeros((codes.size + 15) // 16, dtype=np.uint32)
+    load().fxo_pack2b(_p(codes), codes.size, _p(w))
+    return w
+
+
+# ---- a1 -----------------------------------------------------------------
+def unpack_1ch(words, nsamp):
+    out = np.zeros(nsamp, dtype=np.float32)
+    load().fxo_unpack2b_1ch(_p(np.ascontiguousarray(words, np.uint32)), nsamp, _p(out))
+    return out
+
+
+def unpack_nch(words, ntime, nchan):
+    out = np.zeros(nchan * ntime, dtype=np.float32)
+    load().fxo_unpack2b_nch(_p(np.ascontiguousarray(words, np.uint32)), ntime, nchan, _p(out))
+    return out.reshape(nchan, ntime)
+
+
+# ---- a2/a3 --------------------------------------------------------------
+def twiddles(n):
+    W = np.zeros(n // 4 + 1, dtype=np.complex64)
+    load().fxo_twiddle_r2(n, _p(W))
+    return W[: n // 4]
+
+
+def fft_r2(x):
+    """Forward complex FFT of x (complex64), radix-2 FP32 restatement."""
+    x = np.array(x, dtype=np.complex64, copy=True)
+    n = x.size
+    W = np.zeros(max(n // 4, 1) + 1, dtype=np.complex64)
+    load().fxo_twiddle_r2(n, _p(W))
+    load().fxo_fft_r2_c2c(_p(x), n, _p(W))
+    return x
+
+
+def rfft_r2(x):
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    n = x.size
+    W = np.zeros(n // 4 + 1, dtype=np.complex64)
+    load().fxo_twiddle_r2(n, _p(W))
+    X = np.zeros(n // 2, dtype=np.complex64)
+    work = np.zeros(n, dtype=np.complex64)
+    load().fxo_rfft_r2(_p(x), n, _p(W), _p(X), _p(work))
+    return X
+
+
+def fft_f64(x):
+    x = np.array(x, dtype=np.complex128, copy=True)
+    load().fxo_fft_f64(_p(x), x.size)
+    return x
+
+
+def dft_f64(x):
+    x = np.ascontiguousarray(x, dtype=np.complex128)
+    out = np.zeros_like(x)
+    load().fxo_dft_f64(_p(x), x.size, _p(out))
+    return out
+
+
+# ---- a4 / a6 --------------------------------------------------------------
+def phase_models(models):
+    arr = (PhaseModel * len(models))(*[PhaseModel(*m) for m in models])
+    return arr
+
+
+def phase(model, k, n, t):
+    pm = PhaseModel(*model)
+    return load().fxo_phase(C.byref(pm), k, n, t)
+
+
+def rotate(X, n, model, t):
+    X = np.array(X, dtype=np.complex64, copy=True)
+    pm = PhaseModel(*model)
+    load().fxo_rotate(_p(X), X.size, n, C.byref(pm), t)
+    return X
+
+
+def addproduct(a, b, acc, conj):
+    a = np.ascontiguousarray(a, np.complex64)
+    b = np.ascontiguousarray(b, np.complex64)
+    acc = np.array(acc, dtype=np.complex64, copy=True)
+    fn = load().fxo_addproduct_conj if conj else load().fxo_addproduct
+    fn(_p(a), _p(b), _p(acc), a.size)
+    return acc
+
+
+def mul(a, b):
+    a = np.ascontiguousarray(a, np.complex64)
+    b = np.ascontiguousarray(b, np.complex64)
+    out = np.zeros_like(a)
+    load().fxo_mul(_p(a), _p(b), _p(out), a.size)
+    return out
+
+
+def prod_index(i, j, ninp):
+    return load().fxo_prod_index(i, j, ninp)
+
+
+def xengine_f32(X):
+    """X[c][t][i] complex64 -> vis[p][c] complex64."""
+    X = np.ascontiguousarray(X, np.complex64)
+    C_, T, I = X.shape
+    vis = np.zeros((I * (I + 1) // 2, C_), dtype=np.complex64)
+    load().fxo_xengine_f32(_p(X), C_, T, I, _p(vis))
+    return vis
+
+
+def xengine_f64(X):
+    X = np.ascontiguousarray(X, np.complex64)
+    C_, T, I = X.shape
+    vis = np.zeros((I * (I + 1) // 2, C_), dtype=np.complex128)
+    load().fxo_xengine_f64(_p(X), C_, T, I, _p(vis))
+    return vis
+
+
+def _word_ptrs(streams):
+    arrs = [np.ascontiguousarray(s, np.uint32) for s in streams]
+    ptrs = (C.c_void_p * len(arrs))(*[a.ctypes.data for a in arrs])
+    return arrs, ptrs
+
+
+def corr_f32(streams, n, ntime, models=None, nthreads=1):
+    arrs, ptrs = _word_ptrs(streams)
+    I = len(arrs)
+    vis = np.zeros((I * (I + 1) // 2, n // 2), dtype=np.complex64)
+    pm = phase_models(models) if models is not None else None
+    load().fxo_corr_f32(C.cast(ptrs, C.c_void_p), I, n, ntime,
+                        None if pm is None else C.cast(pm, C.c_void_p), _p(vis), nthreads)
+    return vis
+
+
+def corr_f64(streams, n, ntime, models=None, nthreads=1):
+    arrs, ptrs = _word_ptrs(streams)
+    I = len(arrs)
+    vis = np.zeros((I * (I + 1) // 2, n // 2), dtype=np.complex128)
+    pm = phase_models(models) if models is not None else None
+    load().fxo_corr_f64(C.cast(ptrs, C.c_void_p), I, n, ntime,
+                        None if pm is None else C.cast(pm, C.c_void_p), _p(vis), nthreads)
+    return vis
+
+
+def fstage_f32(streams, n, t0, nt, models=None):
+    arrs, ptrs = _word_ptrs(streams)
+    I = len(arrs)
+    X = np.zeros((n // 2, nt, I), dtype=np.complex64)
+    pm = phase_models(models) if models is not None else None
+    load().fxo_fstage_f32(C.cast(ptrs, C.c_void_p), I, n, t0, nt,
+                          None if pm is None else C.cast(pm, C.c_void_p), _p(X))
+    return X
+
+
+# ---- a8 -----------------------------------------------------------------
+def window(kind, n):
+    w = np.zeros(n, dtype=np.float32)
+    load().fxo_window(kind, n, _p(w))
+    return w
+
+
+def spectrometer_f32(words, ntime, nchan, n, integrate_ffts, window_kind, nthreads=1):
+    nspec = (ntime // n) // integrate_ffts
+    out = np.zeros((nspec, nchan, n // 2), dtype=np.float32)
+    load().fxo_spectrometer_f32(_p(np.ascontiguousarray(words, np.uint32)), ntime, nchan, n,
+                                integrate_ffts, window_kind, _p(out), nthreads)
+    return out
+
+
+def spectrometer_f64(words, ntime, nchan, n, integrate_ffts, window_kind):
+    nspec = (ntime // n) // integrate_ffts
+    out = np.zeros((nspec, nchan, n // 2), dtype=np.float64)
+    load().fxo_spectrometer_f64(_p(np.ascontiguousarray(words, np.uint32)), ntime, nchan, n,
+                                integrate_ffts, window_kind, _p(out))
+    return out
+
+
+# ---- a5 -----------------------------------------------------------------
+def pcal_config(bandwidth_hz, spacing_hz, offset_hz):
+    cfg = PCalCfg()
+    load().fxo_pcal_config(bandwidth_hz, spacing_hz, int(offset_hz), C.byref(cfg))
+    return cfg
+
+
+def pcal_extract(cfg, x, sampleoffset=0, acc=None):
+    x = np.ascontiguousarray(x, np.float32)
+    if acc is None:
+        acc = np.zeros(2 * cfg.nbins, dtype=np.complex128)
+    load().fxo_pcal_extract(C.byref(cfg), _p(x), x.size, sampleoffset, _p(acc))
+    return acc
+
+
+def pcal_final(cfg, acc):
+    out = np.zeros(max(cfg.ntones, 1), dtype=np.complex128)
+    load().fxo_pcal_final(C.byref(cfg), _p(acc), _p(out))
+    return out[: cfg.ntones]
+
+
+def pcal_reference(cfg, x, sampleoffset=0):
+    x = np.ascontiguousarray(x, np.float32)
+    out = np.zeros(max(cfg.ntones, 1), dtype=np.complex128)
+    load().fxo_pcal_reference(C.byref(cfg), _p(x), x.size, sampleoffset, _p(out))
+    return out[: cfg.ntones]
