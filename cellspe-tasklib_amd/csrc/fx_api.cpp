@@ -325,6 +325,24 @@ extern "C" int fx_complex_mul(fx_ctx* ctx, const fx_cf32* a, const fx_cf32* b, f
     return FX_OK;
 }
 
+extern "C" int fx_complex_add(fx_ctx* ctx, const fx_cf32* a, const fx_cf32* b, fx_cf32* out, long n)
+{
+    int rc = ctx_device(ctx);
+    if (rc) return rc;
+    FX_CHECK_LAUNCH(fxk_complex_add(ctx->stream, (const float2*)a, (const float2*)b, (float2*)out, n));
+    return FX_OK;
+}
+
+extern "C" int fx_copy_kernel(fx_ctx* ctx, const void* src, void* dst, size_t bytes)
+{
+    int rc = ctx_device(ctx);
+    if (rc) return rc;
+    FX_REQUIRE(bytes % 16 == 0 && (uintptr_t)src % 16 == 0 && (uintptr_t)dst % 16 == 0, FX_ERR_ARG,
+               "fx_copy_kernel: 16-byte multiples and alignment required");
+    FX_CHECK_LAUNCH(fxk_copy(ctx->stream, src, dst, bytes));
+    return FX_OK;
+}
+
 extern "C" int fx_sincos(fx_ctx* ctx, const float* arg, fx_cf32* out, long n, int normalized)
 {
     int rc = ctx_device(ctx);
@@ -392,15 +410,34 @@ static int ensure_scratch(fx_ctx* ctx, size_t bytes)
 extern "C" int fx_spectrometer(fx_ctx* ctx, const uint32_t* d_words, size_t ntime, int nchan, int n,
                                int integrate_ffts, int window, float* d_out)
 {
+    return fx_spectrometer_ex(ctx, d_words, ntime, nchan, n, 1, integrate_ffts, window, 0, d_out, nullptr);
+}
+
+extern "C" int fx_spectrometer_ex(fx_ctx* ctx, const uint32_t* d_words, size_t ntime, int nchan, int n,
+                                  int overlap, int integrate_ffts, int window, int continuous,
+                                  float* d_out, long* nspec_out)
+{
     int rc = ctx_device(ctx);
     if (rc) return rc;
     FX_REQUIRE(valid_nchan(nchan), FX_ERR_ARG, "fx_spectrometer: nchan %d", nchan);
     FX_REQUIRE(is_pow2(n) && n >= 256 && n <= FX_MAX_N, FX_ERR_ARG, "fx_spectrometer: n=%d", n);
     FX_REQUIRE(integrate_ffts >= 1, FX_ERR_ARG, "fx_spectrometer: integrate_ffts=%d", integrate_ffts);
+    FX_REQUIRE(overlap >= 1 && n % overlap == 0, FX_ERR_ARG, "fx_spectrometer: overlap %d", overlap);
     FX_REQUIRE(window >= FX_WIN_NONE && window <= FX_WIN_COSINE2, FX_ERR_ARG,
                "fx_spectrometer: window %d", window);
-    long nfft = (long)(ntime / (size_t)n);
-    long nspec = nfft / integrate_ffts;
+    const long hop = n / overlap;
+    long nspec, spec_stride;
+    if (continuous) {
+        // one unbroken run of overlapped FFTs (spectrometer_spu.c:171-210)
+        long nfft = ntime >= (size_t)n ? (long)((ntime - (size_t)n) / (size_t)hop) + 1 : 0;
+        nspec = nfft / integrate_ffts;
+        spec_stride = (long)integrate_ffts * hop;
+    } else {
+        // spectra do not share samples (TaskCoreIPP.cpp:495-510)
+        spec_stride = (long)(integrate_ffts - 1) * hop + n;
+        nspec = (long)(ntime / (size_t)spec_stride);
+    }
+    if (nspec_out) *nspec_out = nspec;
     if (nspec == 0) return FX_OK;
     std::lock_guard<std::mutex> lk(ctx->mu);
     float* win = get_window(ctx, window, n);
@@ -416,7 +453,7 @@ extern "C" int fx_spectrometer(fx_ctx* ctx, const uint32_t* d_words, size_t ntim
     size_t part_bytes = (size_t)nspec * nchan * groups * (size_t)FM * sizeof(float);
     rc = ensure_scratch(ctx, part_bytes);
     if (rc) return rc;
-    FX_CHECK_LAUNCH(fxk_spectrometer(ctx->stream, ctx->tw, win, d_words, ntime, nchan, n,
+    FX_CHECK_LAUNCH(fxk_spectrometer(ctx->stream, ctx->tw, win, d_words, hop, spec_stride, nchan, n,
                                      integrate_ffts, (int)nspec, ctx->scratch, groups, d_out));
     return FX_OK;
 }

@@ -111,7 +111,8 @@ __global__ __launch_bounds__(FftShape<M>::NT) void k_fft_r2c(const float2* __res
 template <int M>
 __global__ __launch_bounds__(FftShape<M>::NT) void k_spectrometer(
     const float2* __restrict__ tw, const float* __restrict__ win, const uint32_t* __restrict__ words,
-    int nchan, int integrate_ffts, int groups, int per_group, float* __restrict__ partial)
+    int nchan, int integrate_ffts, int groups, int per_group, long hop, long spec_stride,
+    float* __restrict__ partial)
 {
     using S = FftShape<M>;
     constexpr int N = 2 * M;
@@ -145,7 +146,8 @@ __global__ __launch_bounds__(FftShape<M>::NT) void k_spectrometer(
             int n2 = 2 * (e - f * M);     // sample index within transform f
             float2 z = make_float2(0.f, 0.f);
             if (it + f < fft_count) {
-                long t = (fft_first + it + f) * (long)N + n2;
+                // FFT j of spectrum s starts at s*spec_stride + j*hop (hop = N/overlap)
+                long t = s * spec_stride + ((long)g * per_group + it + f) * hop + n2;
                 uint32_t w0 = words[t / spw];
                 uint32_t w1 = words[(t + 1) / spw];
                 int p0 = (int)(t % spw) * bits_per_t + shift0;
@@ -352,15 +354,15 @@ int launch_r2c(hipStream_t s, const float2* tw, const float* in, float2* out, lo
 }
 
 template <int M>
-int launch_spec(hipStream_t s, const float2* tw, const float* win, const uint32_t* words,
-                int nchan, int integrate_ffts, int nspec, int groups, float* partial, float* out)
+int launch_spec(hipStream_t s, const float2* tw, const float* win, const uint32_t* words, long hop,
+                long spec_stride, int nchan, int integrate_ffts, int nspec, int groups, float* partial, float* out)
 {
     using S = FftShape<M>;
     int per_group = (integrate_ffts + groups - 1) / groups;
     per_group = ((per_group + S::F - 1) / S::F) * S::F;
     long nblk = (long)nspec * nchan * groups;
     hipLaunchKernelGGL(k_spectrometer<M>, dim3((unsigned)nblk), dim3(S::NT), 0, s, tw, win, words,
-                       nchan, integrate_ffts, groups, per_group, partial);
+                       nchan, integrate_ffts, groups, per_group, hop, spec_stride, partial);
     if (hipGetLastError() != hipSuccess) return FX_ERR_HIP;
     long total = (long)nspec * nchan * M;
     hipLaunchKernelGGL(k_spec_reduce, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s,
@@ -448,19 +450,18 @@ int fxk_fft_r2c(hipStream_t s, const float2* tw, const float* in, float2* out, i
 }
 
 int fxk_spectrometer(hipStream_t s, const float2* tw, const float* win, const uint32_t* words,
-                     size_t ntime, int nchan, int n, int integrate_ffts, int nspec, float* partial,
+                     long hop, long spec_stride, int nchan, int n, int integrate_ffts, int nspec, float* partial,
                      int groups, float* out)
 {
-    (void)ntime;
     switch (n) {
-    case 256: return launch_spec<128>(s, tw, win, words, nchan, integrate_ffts, nspec, groups, partial, out);
-    case 512: return launch_spec<256>(s, tw, win, words, nchan, integrate_ffts, nspec, groups, partial, out);
-    case 1024: return launch_spec<512>(s, tw, win, words, nchan, integrate_ffts, nspec, groups, partial, out);
-    case 2048: return launch_spec<1024>(s, tw, win, words, nchan, integrate_ffts, nspec, groups, partial, out);
-    case 4096: return launch_spec<2048>(s, tw, win, words, nchan, integrate_ffts, nspec, groups, partial, out);
-    case 8192: return launch_spec<4096>(s, tw, win, words, nchan, integrate_ffts, nspec, groups, partial, out);
-    case 16384: return launch_spec<8192>(s, tw, win, words, nchan, integrate_ffts, nspec, groups, partial, out);
-    case 32768: return launch_spec<16384>(s, tw, win, words, nchan, integrate_ffts, nspec, groups, partial, out);
+    case 256: return launch_spec<128>(s, tw, win, words, hop, spec_stride, nchan, integrate_ffts, nspec, groups, partial, out);
+    case 512: return launch_spec<256>(s, tw, win, words, hop, spec_stride, nchan, integrate_ffts, nspec, groups, partial, out);
+    case 1024: return launch_spec<512>(s, tw, win, words, hop, spec_stride, nchan, integrate_ffts, nspec, groups, partial, out);
+    case 2048: return launch_spec<1024>(s, tw, win, words, hop, spec_stride, nchan, integrate_ffts, nspec, groups, partial, out);
+    case 4096: return launch_spec<2048>(s, tw, win, words, hop, spec_stride, nchan, integrate_ffts, nspec, groups, partial, out);
+    case 8192: return launch_spec<4096>(s, tw, win, words, hop, spec_stride, nchan, integrate_ffts, nspec, groups, partial, out);
+    case 16384: return launch_spec<8192>(s, tw, win, words, hop, spec_stride, nchan, integrate_ffts, nspec, groups, partial, out);
+    case 32768: return launch_spec<16384>(s, tw, win, words, hop, spec_stride, nchan, integrate_ffts, nspec, groups, partial, out);
     default: return FX_ERR_ARG;
     }
 }

@@ -9,6 +9,7 @@
 #include <hip/hip_fp16.h>
 #include <stdint.h>
 #include <stddef.h>
+#include <algorithm>
 
 #include "cellspe_fx.h"
 
@@ -199,7 +200,7 @@ int fxk_fft_c2c(hipStream_t s, const float2* tw, const float2* in, float2* out, 
 int fxk_fft_r2c(hipStream_t s, const float2* tw, const float* in, float2* out, int n, long batch,
                 int keep_nyquist);
 int fxk_spectrometer(hipStream_t s, const float2* tw, const float* window, const uint32_t* words,
-                     size_t ntime, int nchan, int n, int integrate_ffts, int nspec, float* partial,
+                     long hop, long spec_stride, int nchan, int n, int integrate_ffts, int nspec, float* partial,
                      int groups, float* out);
 int fxk_fstage(hipStream_t s, const float2* tw, const uint32_t* words, size_t in_stride_words,
                int ninp, int n, long t_first, long nt, const fx_phase_model* pm,
@@ -222,3 +223,12 @@ int fxk_complex_mac(hipStream_t s, const float2* a, const float2* b, float2* acc
 int fxk_complex_mul(hipStream_t s, const float2* a, const float2* b, float2* out, long n);
 int fxk_sincos(hipStream_t s, const float* arg, float2* out, long n, int normalized);
 int fxk_sincos_gen(hipStream_t s, double start, double inc, float2* out, long n);
+int fxk_copy(hipStream_t s, const void* src, void* dst, size_t bytes);
+int fxk_complex_add(hipStream_t s, const float2* a, const float2* b, float2* out, long n);
+int fxk_iotest(hipStream_t s, float* out);
+int fxk_difx_post(hipStream_t s, const float2* fft, int nfft, int band0, int n, const float2* mult,
+                  float2* out, float2* conj_out, float2* acc);
+
+// context internals shared with the task runtime and the application shims
+const float2* fx_ctx_twiddles(fx_ctx* ctx);
+hipStream_t fx_ctx_hip_stream(fx_ctx* ctx);

@@ -278,6 +278,48 @@ __global__ void k_sincos_gen(double start, double inc, float2* __restrict__ out,
     out[i] = make_float2(c, s);
 }
 
+// plain 16-byte-per-lane copy (traffic calibration and staging)
+__global__ void k_copy16(const uint4* __restrict__ src, uint4* __restrict__ dst, size_t n16)
+{
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (; i < n16; i += stride) dst[i] = src[i];
+}
+
+__global__ void k_cadd(const float2* __restrict__ a, const float2* __restrict__ b,
+                       float2* __restrict__ out, long n)
+{
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float2 x = a[i], y = b[i];
+    out[i] = make_float2(x.x + y.x, x.y + y.y);
+}
+
+// DiFX-process epilogue: y = band * mult; out = y; conj_out = conj(y);
+// acc.re += |y|^2 (acc.im untouched, autocorrelations are real)
+__global__ void k_difx_post(const float2* __restrict__ fft, int nfft, int band0, int n,
+                            const float2* __restrict__ mult, float2* __restrict__ out,
+                            float2* __restrict__ conj_out, float2* __restrict__ acc)
+{
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    float2 x = fft[(band0 + k) & (nfft - 1)], m = mult[k];
+    float2 y = make_float2((x.x * m.x) - (x.y * m.y), (x.y * m.x) + (x.x * m.y));
+    out[k] = y;
+    conj_out[k] = make_float2(y.x, -y.y);
+    acc[k].x += y.x * y.x + y.y * y.y;
+}
+
+__global__ void k_iotest(float* out)
+{
+    if (threadIdx.x == 0) {
+        out[0] = 0.1f;
+        out[1] = 2.3f;
+        out[2] = 4.5f;
+        out[3] = 6.7f;
+    }
+}
+
 inline unsigned nblocks(size_t n, unsigned bs) { return (unsigned)((n + bs - 1) / bs); }
 inline int last_status() { return hipGetLastError() == hipSuccess ? FX_OK : FX_ERR_HIP; }
 
@@ -378,5 +420,37 @@ int fxk_sincos_gen(hipStream_t s, double start, double inc, float2* out, long n)
     if (n <= 0) return FX_OK;
     hipLaunchKernelGGL(k_sincos_gen, dim3(nblocks((size_t)n, 256)), dim3(256), 0, s, start, inc,
                        out, n);
+    return last_status();
+}
+
+int fxk_copy(hipStream_t s, const void* src, void* dst, size_t bytes)
+{
+    if (bytes % 16 || ((uintptr_t)src % 16) || ((uintptr_t)dst % 16)) return FX_ERR_ARG;
+    size_t n16 = bytes / 16;
+    if (n16 == 0) return FX_OK;
+    unsigned blocks = (unsigned)std::min<size_t>((n16 + 255) / 256, 65536);
+    hipLaunchKernelGGL(k_copy16, dim3(blocks), dim3(256), 0, s, (const uint4*)src, (uint4*)dst, n16);
+    return last_status();
+}
+
+int fxk_complex_add(hipStream_t s, const float2* a, const float2* b, float2* out, long n)
+{
+    if (n <= 0) return FX_OK;
+    hipLaunchKernelGGL(k_cadd, dim3(nblocks((size_t)n, 256)), dim3(256), 0, s, a, b, out, n);
+    return last_status();
+}
+
+int fxk_difx_post(hipStream_t s, const float2* fft, int nfft, int band0, int n, const float2* mult,
+                  float2* out, float2* conj_out, float2* acc)
+{
+    if (n <= 0) return FX_OK;
+    hipLaunchKernelGGL(k_difx_post, dim3(nblocks((size_t)n, 256)), dim3(256), 0, s, fft, nfft, band0, n,
+                       mult, out, conj_out, acc);
+    return last_status();
+}
+
+int fxk_iotest(hipStream_t s, float* out)
+{
+    hipLaunchKernelGGL(k_iotest, dim3(1), dim3(64), 0, s, out);
     return last_status();
 }

@@ -73,6 +73,8 @@ def load():
         "fx_fft_r2c": (i, [vp, vp, vp, i, l, i]),
         "fx_complex_mac": (i, [vp, vp, vp, vp, l, i]),
         "fx_complex_mul": (i, [vp, vp, vp, vp, l]),
+        "fx_complex_add": (i, [vp, vp, vp, vp, l]),
+        "fx_copy_kernel": (i, [vp, vp, vp, sz]),
         "fx_sincos": (i, [vp, vp, vp, l, i]),
         "fx_sincos_generator": (i, [vp, d, d, vp, l]),
         "fx_spectrometer": (i, [vp, vp, sz, i, i, i, i, vp]),
@@ -148,6 +150,14 @@ class DeviceBuffer:
         out = np.empty(count, dtype=dtype)
         check(load().fx_memcpy_d2h(self.ctx.handle, _ptr(out), self.ptr, out.nbytes))
         return out.reshape(shape) if shape is not None else out
+
+    def download_at(self, offset_bytes, dtype, count):
+        dtype = np.dtype(dtype)
+        assert offset_bytes + count * dtype.itemsize <= self.nbytes
+        out = np.empty(count, dtype=dtype)
+        check(load().fx_memcpy_d2h(self.ctx.handle, _ptr(out),
+                                   C.c_void_p(self.ptr.value + offset_bytes), out.nbytes))
+        return out
 
     def zero(self):
         check(load().fx_memset(self.ctx.handle, self.ptr, 0, self.nbytes))

@@ -105,6 +105,10 @@ int fx_fft_r2c(fx_ctx* ctx, const float* d_in, fx_cf32* d_out, int n, long batch
 int fx_complex_mac(fx_ctx* ctx, const fx_cf32* d_a, const fx_cf32* d_b, fx_cf32* d_acc,
                    long n, int conj);
 int fx_complex_mul(fx_ctx* ctx, const fx_cf32* d_a, const fx_cf32* d_b, fx_cf32* d_out, long n);
+int fx_complex_add(fx_ctx* ctx, const fx_cf32* d_a, const fx_cf32* d_b, fx_cf32* d_out, long n);
+/* device copy through a plain load/store kernel (16-byte multiples, aligned);
+ * used to calibrate HBM traffic counters */
+int fx_copy_kernel(fx_ctx* ctx, const void* d_src, void* d_dst, size_t bytes);
 /* (cos, sin) of arg (radians, or turns when normalized != 0). */
 int fx_sincos(fx_ctx* ctx, const float* d_arg, fx_cf32* d_out, long n, int normalized);
 /* oscillator output (cos, sin)(start + i*inc), phase accumulated in FP64. */
@@ -116,6 +120,14 @@ int fx_sincos_generator(fx_ctx* ctx, double start, double inc, fx_cf32* d_out, l
  * 1/integrate_ffts.  d_out[s][c][n/2], s < ntime/(n*integrate_ffts). */
 int fx_spectrometer(fx_ctx* ctx, const uint32_t* d_words, size_t ntime, int nchan, int n,
                     int integrate_ffts, int window, float* d_out);
+/* Overlapped FFTs: hop = n/overlap samples between FFT starts.  continuous=1
+ * runs one unbroken sequence (the SPU kernel's convention: spectrum s uses
+ * FFTs s*K .. s*K+K-1); continuous=0 starts each spectrum after the last
+ * sample of the previous one (the IPP core's convention).  *nspec_out (may be
+ * NULL) receives the number of spectra written. */
+int fx_spectrometer_ex(fx_ctx* ctx, const uint32_t* d_words, size_t ntime, int nchan, int n,
+                       int overlap, int integrate_ffts, int window, int continuous, float* d_out,
+                       long* nspec_out);
 
 /* ---- a5: phase-cal ------------------------------------------------------- */
 typedef struct fx_pcal fx_pcal;

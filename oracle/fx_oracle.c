@@ -604,9 +604,29 @@ void fxo_window(int kind, int n, float* w)
 void fxo_spectrometer_f32(const uint32_t* words, size_t ntime, int nchan, int n,
                           int integrate_ffts, int window_kind, float* out, int nthreads)
 {
+    fxo_spectrometer_ex_f32(words, ntime, nchan, n, 1, integrate_ffts, window_kind, 0, out, nthreads);
+}
+
+/* Overlapped variant: FFT j of spectrum s starts at s*stride + j*hop,
+ * hop = n/overlap; stride = K*hop for one unbroken run (spectrometer_spu.c:
+ * 171-210) or (K-1)*hop + n when spectra do not share samples
+ * (TaskCoreIPP.cpp:495-510). */
+size_t fxo_spectrometer_ex_f32(const uint32_t* words, size_t ntime, int nchan, int n, int overlap,
+                               int integrate_ffts, int window_kind, int continuous, float* out,
+                               int nthreads)
+{
     int C = n / 2;
-    size_t nfft = ntime / (size_t)n;
-    size_t nspec = nfft / (size_t)integrate_ffts;
+    size_t hop = (size_t)(n / overlap);
+    size_t stride, nspec;
+    if (continuous) {
+        size_t nfft = ntime >= (size_t)n ? (ntime - (size_t)n) / hop + 1 : 0;
+        nspec = nfft / (size_t)integrate_ffts;
+        stride = (size_t)integrate_ffts * hop;
+    } else {
+        stride = (size_t)(integrate_ffts - 1) * hop + (size_t)n;
+        nspec = ntime / stride;
+    }
+    if (!out) return nspec;
     float* win = (float*)malloc(sizeof(float) * (size_t)n);
     fxo_window(window_kind, n, win);
     int spw = 16 / nchan;
@@ -625,7 +645,7 @@ void fxo_spectrometer_f32(const uint32_t* words, size_t ntime, int nchan, int n,
         float* acc = (float*)calloc((size_t)C, sizeof(float));
         fxo_twiddle_r2(n, W);
         for (int f = 0; f < integrate_ffts; f++) {
-            size_t t0 = (s * (size_t)integrate_ffts + (size_t)f) * (size_t)n;
+            size_t t0 = s * stride + (size_t)f * hop;
             for (int i = 0; i < n; i++) {
                 size_t t = t0 + (size_t)i;
                 uint32_t wd = words[t / (size_t)spw];
@@ -642,6 +662,7 @@ void fxo_spectrometer_f32(const uint32_t* words, size_t ntime, int nchan, int n,
         free(W); free(xs); free(X); free(work); free(acc);
     }
     free(win);
+    return nspec;
 }
 
 void fxo_spectrometer_f64(const uint32_t* words, size_t ntime, int nchan, int n,

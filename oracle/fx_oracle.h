@@ -121,6 +121,9 @@ void fxo_window(int kind, int n, float* w);
  * 1/integrate_ffts.  out[s][c][n/2] for s in [0, ntime/(n*integrate_ffts)). */
 void fxo_spectrometer_f32(const uint32_t* words, size_t ntime, int nchan, int n,
                           int integrate_ffts, int window_kind, float* out, int nthreads);
+size_t fxo_spectrometer_ex_f32(const uint32_t* words, size_t ntime, int nchan, int n, int overlap,
+                               int integrate_ffts, int window_kind, int continuous, float* out,
+                               int nthreads);
 void fxo_spectrometer_f64(const uint32_t* words, size_t ntime, int nchan, int n,
                           int integrate_ffts, int window_kind, double* out);
 
