@@ -71,6 +71,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-check", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="instead of the benchmark, run the plain copy kernel over MIB MiB "
+                         "(HBM bandwidth check and traffic-counter calibration)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -80,6 +83,26 @@ def main():
         sys.exit(2)
 
     import fxlib
+
+    if args.copy_probe:
+        ctx = fxlib.Context(0)
+        nbytes = args.copy_probe << 20
+        src, dst = ctx.alloc(nbytes), ctx.alloc(nbytes)
+        src.zero()
+        lib = fxlib.load()
+        for _ in range(args.warmup):
+            fxlib.check(lib.fx_copy_kernel(ctx.handle, src.ptr, dst.ptr, nbytes))
+        e0, e1 = ctx.event(), ctx.event()
+        ctx.sync()
+        ctx.record(e0)
+        for _ in range(args.steps):
+            fxlib.check(lib.fx_copy_kernel(ctx.handle, src.ptr, dst.ptr, nbytes))
+        ctx.record(e1)
+        ms = ctx.elapsed_ms(e0, e1) / args.steps
+        print(json.dumps({"copy_probe_bytes": nbytes, "ms_per_copy": ms,
+                          "gbps_read_plus_write": 2 * nbytes / (ms * 1e-3) / 1e9}), flush=True)
+        ctx.close()
+        return
 
     I, N, C, T, tau, models = workload(args)
     P = I * (I + 1) // 2

@@ -618,6 +618,7 @@ struct fx_corr {
     int nchan;
     size_t elem;            // bytes per channelized complex sample
     void* chan = nullptr;   // [c][t][i], chunk_spectra deep
+    void* nat = nullptr;    // F-stage rows [t][i][c] before the corner turn
     float* acc = nullptr;   // engine-layout accumulators
     size_t acc_floats = 0;
     fx_phase_model* pm = nullptr;
@@ -664,6 +665,7 @@ extern "C" int fx_corr_create(fx_ctx* ctx, const fx_corr_config* cfg, fx_corr** 
     for (int i = 0, p = 0; i < cfg->ninp; i++)
         for (int j = i; j < cfg->ninp; j++, p++) hp[(size_t)p] = make_int2(i, j);
     bool ok = hipMalloc(&c->chan, chan_bytes) == hipSuccess &&
+              hipMalloc(&c->nat, chan_bytes) == hipSuccess &&
               hipMalloc(&c->acc, c->acc_floats * sizeof(float)) == hipSuccess &&
               hipMalloc(&c->pm, sizeof(fx_phase_model) * (size_t)cfg->ninp) == hipSuccess &&
               hipMalloc(&c->pij, sizeof(int2) * (size_t)P) == hipSuccess;
@@ -697,6 +699,7 @@ extern "C" int fx_corr_destroy(fx_corr* c)
     hipSetDevice(c->ctx->device);
     hipStreamSynchronize(c->ctx->stream);
     if (c->chan) hipFree(c->chan);
+    if (c->nat) hipFree(c->nat);
     if (c->acc) hipFree(c->acc);
     if (c->pm) hipFree(c->pm);
     if (c->pij) hipFree(c->pij);
@@ -756,12 +759,16 @@ extern "C" int fx_corr_fstage(fx_corr* c, const uint32_t* d_words, size_t in_str
     FX_REQUIRE(in_stride_words % 2 == 0 && ((uintptr_t)d_words % 8) == 0, FX_ERR_ARG,
                "fx_corr_fstage: input stride must be even and the base 8-byte aligned");
     FX_REQUIRE(t_first >= 0 && nspectra >= 0, FX_ERR_ARG, "fx_corr_fstage: negative range");
+    FX_REQUIRE(nspectra <= c->cfg.chunk_spectra, FX_ERR_ARG,
+               "fx_corr_fstage: %ld spectra exceed the %ld-spectrum chunk", nspectra, c->cfg.chunk_spectra);
+    FX_REQUIRE(prec_elem(out_prec) > 0 && prec_elem(out_prec) <= c->elem, FX_ERR_ARG,
+               "fx_corr_fstage: output precision %d wider than the engine's", out_prec);
     if (nspectra == 0) return FX_OK;
     const bool scaled = out_prec != FX_PREC_F32;
     FX_CHECK_LAUNCH(fxk_fstage(c->ctx->stream, c->ctx->tw, d_words, in_stride_words, c->cfg.ninp,
                                c->cfg.nfft, t_first, nspectra, c->cfg.use_phase ? c->pm : nullptr,
                                c->mult, scaled ? c->sc_chan : nullptr, scaled ? c->sc_inp : nullptr,
-                               out_prec, d_chan));
+                               out_prec, c->nat, d_chan));
     return FX_OK;
 }
 

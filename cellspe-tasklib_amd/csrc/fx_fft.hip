@@ -13,8 +13,8 @@
 //
 // Every transform is a Stockham radix-16 FFT over a workgroup-resident LDS
 // tile: F transforms of complex length M per workgroup with F*M = FM complex
-// (8192 or 16384), NT = FM/16 threads, so each thread holds exactly 16 complex
-// values per pass.  Real transforms of length N = 2M pack even/odd samples
+// (8192 or 16384) and 512 threads, so each thread holds 16 (32) complex values
+// per pass.  Real transforms of length N = 2M pack even/odd samples
 // into one complex M-point FFT and split the result (rsplit).
 #include "fx_internal.h"
 
@@ -24,9 +24,11 @@ namespace {
 
 template <int M>
 struct FftShape {
-    static constexpr int FM = (M >= 16384) ? 16384 : 8192;
-    static constexpr int F = FM / M;
-    static constexpr int NT = FM / 16;
+    static constexpr int FM = (M >= 16384) ? 16384 : 8192;   // complex per workgroup tile
+    static constexpr int F = FM / M;                          // transforms per tile
+    static constexpr int NT = 512;                            // threads
+    static constexpr int QPT = FM / NT;                       // complex per thread (16 or 32)
+    static constexpr int SLT = FM / (2 * NT);                 // real-split pairs per thread
     static constexpr int LDS = lds_complex(FM);
 };
 
@@ -34,24 +36,26 @@ struct FftShape {
 // complex -> complex
 // ---------------------------------------------------------------------------
 template <int M>
-__global__ __launch_bounds__(FftShape<M>::NT) void k_fft_c2c(const float2* __restrict__ tw,
+__global__ __launch_bounds__(FftShape<M>::NT) void k_fft_c2c(const float2* __restrict__ twg,
                                                              const float2* __restrict__ in,
                                                              float2* __restrict__ out, long batch)
 {
     using S = FftShape<M>;
     __shared__ float2 sh[S::LDS];
+    __shared__ float2 tw[FX_TWL];
     const int tid = threadIdx.x;
+    twl_load(tw, twg, tid, S::NT);
     const long f0 = (long)blockIdx.x * S::F;
     const long nvalid = (batch - f0 < S::F ? batch - f0 : S::F) * M;
 #pragma unroll
-    for (int s = 0; s < 16; s++) {
+    for (int s = 0; s < S::QPT; s++) {
         int e = tid + s * S::NT;
         sh[lpad(e)] = (e < nvalid) ? in[f0 * M + e] : make_float2(0.f, 0.f);
     }
     __syncthreads();
     fft_passes<M, 1, S::NT, S::F>(sh, tw, tid);
 #pragma unroll
-    for (int s = 0; s < 16; s++) {
+    for (int s = 0; s < S::QPT; s++) {
         int e = tid + s * S::NT;
         if (e < nvalid) out[f0 * M + e] = sh[lpad(e)];
     }
@@ -61,19 +65,21 @@ __global__ __launch_bounds__(FftShape<M>::NT) void k_fft_c2c(const float2* __res
 // real -> complex (N = 2M real samples per transform)
 // ---------------------------------------------------------------------------
 template <int M>
-__global__ __launch_bounds__(FftShape<M>::NT) void k_fft_r2c(const float2* __restrict__ tw,
+__global__ __launch_bounds__(FftShape<M>::NT) void k_fft_r2c(const float2* __restrict__ twg,
                                                              const float2* __restrict__ in,
                                                              float2* __restrict__ out, long batch,
                                                              int keep_nyq)
 {
     using S = FftShape<M>;
     __shared__ float2 sh[S::LDS];
+    __shared__ float2 tw[FX_TWL];
     const int tid = threadIdx.x;
+    twl_load(tw, twg, tid, S::NT);
     const long f0 = (long)blockIdx.x * S::F;
     const long nf = (batch - f0 < S::F ? batch - f0 : S::F);
     const long nvalid = nf * M;
 #pragma unroll
-    for (int s = 0; s < 16; s++) {
+    for (int s = 0; s < S::QPT; s++) {
         int e = tid + s * S::NT;
         sh[lpad(e)] = (e < nvalid) ? in[f0 * M + e] : make_float2(0.f, 0.f);
     }
@@ -110,14 +116,16 @@ __global__ __launch_bounds__(FftShape<M>::NT) void k_fft_r2c(const float2* __res
 // ---------------------------------------------------------------------------
 template <int M>
 __global__ __launch_bounds__(FftShape<M>::NT) void k_spectrometer(
-    const float2* __restrict__ tw, const float* __restrict__ win, const uint32_t* __restrict__ words,
+    const float2* __restrict__ twg, const float* __restrict__ win, const uint32_t* __restrict__ words,
     int nchan, int integrate_ffts, int groups, int per_group, long hop, long spec_stride,
     float* __restrict__ partial)
 {
     using S = FftShape<M>;
     constexpr int N = 2 * M;
     __shared__ float2 sh[S::LDS];
+    __shared__ float2 tw[FX_TWL];
     const int tid = threadIdx.x;
+    twl_load(tw, twg, tid, S::NT);
     const int wg = blockIdx.x;
     const int g = wg % groups;
     const int cs = wg / groups;
@@ -127,9 +135,9 @@ __global__ __launch_bounds__(FftShape<M>::NT) void k_spectrometer(
     const int shift0 = 2 * c;
     const int bits_per_t = 2 * nchan;
 
-    float acc[8][2];
+    float acc[S::SLT][2];
 #pragma unroll
-    for (int i = 0; i < 8; i++) acc[i][0] = acc[i][1] = 0.f;
+    for (int i = 0; i < S::SLT; i++) acc[i][0] = acc[i][1] = 0.f;
     float accm = 0.f;
 
     const long fft_first = s * integrate_ffts + (long)g * per_group;
@@ -139,8 +147,9 @@ __global__ __launch_bounds__(FftShape<M>::NT) void k_spectrometer(
     for (int it = 0; it < fft_count; it += S::F) {
         // load F transforms: F*N samples of channel c; thread handles 2*16 = 32
         // consecutive-in-pairs samples: sample pair e -> z[e] = (x[2e], x[2e+1])
-#pragma unroll
-        for (int q = 0; q < 16; q++) {
+        // (partial unroll bounds the loads in flight to the 1024-thread budget)
+#pragma unroll 4
+        for (int q = 0; q < S::QPT; q++) {
             int e = tid + q * S::NT;      // complex index within the WG tile
             int f = e / M;
             int n2 = 2 * (e - f * M);     // sample index within transform f
@@ -162,8 +171,8 @@ __global__ __launch_bounds__(FftShape<M>::NT) void k_spectrometer(
         // power of the thread's 8 (kp, f) items, kp-major so the same thread
         // owns the same bins on every iteration
 #pragma unroll
-        for (int i = 0; i < 8; i++) {
-            int q = tid * 8 + i;
+        for (int i = 0; i < S::SLT; i++) {
+            int q = tid * S::SLT + i;
             int kp = q / S::F;
             int f = q - kp * S::F;
             if (it + f < fft_count) {
@@ -182,8 +191,8 @@ __global__ __launch_bounds__(FftShape<M>::NT) void k_spectrometer(
     }
     float* pw = partial + (long)wg * S::F * M;
 #pragma unroll
-    for (int i = 0; i < 8; i++) {
-        int q = tid * 8 + i;
+    for (int i = 0; i < S::SLT; i++) {
+        int q = tid * S::SLT + i;
         int kp = q / S::F;
         int f = q - kp * S::F;
         pw[f * M + kp] = acc[i][0];
@@ -209,128 +218,192 @@ __global__ void k_spec_reduce(const float* __restrict__ partial, int M, int F, i
 }
 
 // ---------------------------------------------------------------------------
-// F-stage.  Workgroup b covers F consecutive items g = b*F + f of the
-// row-major (t, i) grid of the chunk; item (t, i) is spectrum t_first + t of
-// input i.  Output element (c, t, i) lives at out[c*(nt*I) + t*I + i], so the
-// F results of one channel are contiguous.
+// F-stage.  Item g = t*I + i of a chunk is spectrum t_first + t of input i.
+// The kernel writes its natural output rows nat[g][c] (each transform's C
+// channels contiguous, every store instruction a 256-byte run), and
+// k_cornerturn re-tiles them through LDS into the X-engine layout
+// out[c][g] ([c][t][i]).  Writing [c][t][i] straight from the transform
+// scatters 4-16 byte fragments that the L2 writes back as partial lines.
 // ---------------------------------------------------------------------------
+template <typename T> struct ChanT;
+template <> struct ChanT<float2> { typedef float2 V; };
+template <> struct ChanT<__half2> { typedef uint32_t V; };
+template <> struct ChanT<uint16_t> { typedef uint16_t V; };
+
 template <typename T>
-__device__ __forceinline__ void store_chan(T* out, long idx, float2 v);
+__device__ __forceinline__ typename ChanT<T>::V to_chan(float2 v);
 template <>
-__device__ __forceinline__ void store_chan<float2>(float2* out, long idx, float2 v) { out[idx] = v; }
+__device__ __forceinline__ float2 to_chan<float2>(float2 v) { return v; }
 template <>
-__device__ __forceinline__ void store_chan<__half2>(__half2* out, long idx, float2 v)
+__device__ __forceinline__ uint32_t to_chan<__half2>(float2 v)
 {
-    out[idx] = __floats2half2_rn(v.x, v.y);
+    __half2 h = __floats2half2_rn(v.x, v.y);
+    return *reinterpret_cast<uint32_t*>(&h);
 }
 template <>
-__device__ __forceinline__ void store_chan<uint16_t>(uint16_t* out, long idx, float2 v)
+__device__ __forceinline__ uint16_t to_chan<uint16_t>(float2 v)
 {
     __hip_fp8_e4m3 r(v.x), i(v.y);
-    out[idx] = (uint16_t)((uint16_t)r.__x | ((uint16_t)i.__x << 8));
+    return (uint16_t)((uint16_t)r.__x | ((uint16_t)i.__x << 8));
 }
 
 template <int M, typename T>
 __global__ __launch_bounds__(FftShape<M>::NT) void k_fstage(
-    const float2* __restrict__ tw, const uint32_t* __restrict__ words, size_t in_stride, int ninp,
+    const float2* __restrict__ twg, const uint32_t* __restrict__ words, size_t in_stride, int ninp,
     long t_first, long nt, const fx_phase_model* __restrict__ pm, const float2* __restrict__ mult,
-    const float* __restrict__ sc_chan, const float* __restrict__ sc_inp, T* __restrict__ out)
+    const float* __restrict__ sc_chan, const float* __restrict__ sc_inp, T* __restrict__ nat)
 {
     using S = FftShape<M>;
+    constexpr int F = S::F;
+    constexpr int NT = S::NT;
     constexpr int N = 2 * M;
-    constexpr int WPT = N / 16;     // packed words per transform
+    constexpr int WPT = N / 16;            // packed words per transform
+    constexpr int WTH = (S::FM / 8) / NT;  // packed words per thread (2)
+    typedef typename ChanT<T>::V V;
     __shared__ float2 sh[S::LDS];
+    __shared__ float2 tw[FX_TWL];
     const int tid = threadIdx.x;
-    const long g0 = (long)blockIdx.x * S::F;
+    twl_load(tw, twg, tid, NT);
     const long nitems = nt * ninp;
+    const long g0 = (long)blockIdx.x * F;
 
-    // ---- load + unpack: the tile is F*N samples = 2*NT words; thread takes
-    // two consecutive words (32 samples -> 16 complex z values)
+    // ---- load + unpack F transforms
     {
-        int wl = 2 * tid;                 // word index within the tile
-        int f = wl / WPT;
-        int wof = wl - f * WPT;
-        long g = g0 + f;
-        uint32_t w0 = 0, w1 = 0;
+        const int wl = WTH * tid;
+        const int f = wl / WPT;
+        const int wof = wl - f * WPT;
+        const long g = g0 + f;
+        uint32_t w[WTH];
+#pragma unroll
+        for (int q = 0; q < WTH; q++) w[q] = 0;
         if (g < nitems) {
-            long t = g / ninp;
-            int i = (int)(g - t * ninp);
+            const long t = g / ninp;
+            const int i = (int)(g - t * ninp);
             const uint32_t* src = words + (size_t)i * in_stride + (size_t)(t_first + t) * WPT + wof;
-            uint2 ww = *reinterpret_cast<const uint2*>(src);
-            w0 = ww.x;
-            w1 = ww.y;
+#pragma unroll
+            for (int q = 0; q < WTH; q += 2) {
+                const uint2 ww = *reinterpret_cast<const uint2*>(src + q);
+                w[q] = ww.x;
+                w[q + 1] = ww.y;
+            }
         }
-        int zb = f * M + wof * 8;
+        const int zb = f * M + wof * 8;
 #pragma unroll
-        for (int m = 0; m < 8; m++)
-            sh[lpad(zb + m)] = make_float2(lut2((w0 >> (4 * m)) & 3u), lut2((w0 >> (4 * m + 2)) & 3u));
+        for (int q = 0; q < WTH; q++)
 #pragma unroll
-        for (int m = 0; m < 8; m++)
-            sh[lpad(zb + 8 + m)] = make_float2(lut2((w1 >> (4 * m)) & 3u), lut2((w1 >> (4 * m + 2)) & 3u));
+            for (int m = 0; m < 8; m++)
+                sh[lpad(zb + 8 * q + m)] =
+                    make_float2(lut2((w[q] >> (4 * m)) & 3u), lut2((w[q] >> (4 * m + 2)) & 3u));
     }
     __syncthreads();
-    fft_passes<M, 1, S::NT, S::F>(sh, tw, tid);
+    fft_passes<M, 1, NT, F>(sh, tw, tid);
 
-    const long cstride = nitems;   // distance between channels in the output
+    // ---- split, rotate, scale, convert; slot q -> (f, kp) with kp fastest so
+    // lanes write consecutive channels of one row
+    V* o = reinterpret_cast<V*>(nat);
+    constexpr int H = M / 2;
 #pragma unroll
-    for (int it = 0; it < 9; it++) {
-        int kp, f;
-        if (it < 8) {
-            int q = tid * 8 + it;
-            kp = q / S::F;
-            f = q - kp * S::F;
-        } else {
-            if (tid >= S::F) break;
-            kp = M / 2;
-            f = tid;
-        }
-        long g = g0 + f;
+    for (int it = 0; it < (F * H) / NT; it++) {
+        const int q = it * NT + tid;
+        const int f = q / H;
+        const int kp = q - f * H;
+        const long g = g0 + f;
         if (g >= nitems) continue;
-        long t = g / ninp;
-        int i = (int)(g - t * ninp);
+        const long t = g / ninp;
+        const int i = (int)(g - t * ninp);
         float2 xk, xm;
         rsplit<M>(sh, f * M, kp, tw, xk, xm);
         const int k2 = M - kp;
-        const bool two = (kp != 0) && (kp != M / 2);
         if (pm) {
-            const fx_phase_model p = pm[i];
+            const fx_phase_model pmi = pm[i];
             const double tg = (double)(t_first + t);
-            const double tau = p.delay0 + p.delay_rate * tg;
-            const double fr = (p.fringe0 + p.fringe_rate * tg) * 0.15915494309189535;
-            {
-                double cyc = (double)kp * tau / (double)N + fr;
-                cyc -= rint(cyc);
-                float ph = (float)(6.283185307179586 * cyc);
-                float sn, cs;
-                sincosf(ph, &sn, &cs);
-                xk = cmul(xk, make_float2(cs, sn));
-            }
-            if (two) {
-                double cyc = (double)k2 * tau / (double)N + fr;
-                cyc -= rint(cyc);
-                float ph = (float)(6.283185307179586 * cyc);
-                float sn, cs;
-                sincosf(ph, &sn, &cs);
-                xm = cmul(xm, make_float2(cs, sn));
-            }
+            const double tau = pmi.delay0 + pmi.delay_rate * tg;
+            const double fr = (pmi.fringe0 + pmi.fringe_rate * tg) * 0.15915494309189535;
+            double c1 = (double)kp * tau / (double)N + fr;
+            double c2 = (double)k2 * tau / (double)N + fr;
+            c1 -= rint(c1);
+            c2 -= rint(c2);
+            float s1, co1, s2, co2;
+            sincosf((float)(6.283185307179586 * c1), &s1, &co1);
+            sincosf((float)(6.283185307179586 * c2), &s2, &co2);
+            xk = cmul(xk, make_float2(co1, s1));
+            xm = cmul(xm, make_float2(co2, s2));
         }
         if (mult) {
             xk = cmul(xk, mult[(size_t)i * M + kp]);
-            if (two) xm = cmul(xm, mult[(size_t)i * M + k2]);
+            xm = cmul(xm, mult[(size_t)i * M + (k2 & (M - 1))]);
         }
         if (sc_chan) {
-            float si = sc_inp[i];
-            float a = sc_chan[kp] * si;
+            const float si = sc_inp[i];
+            const float a = sc_chan[kp] * si;
+            const float b = sc_chan[k2 & (M - 1)] * si;
             xk.x *= a;
             xk.y *= a;
-            if (two) {
-                float b = sc_chan[k2] * si;
-                xm.x *= b;
-                xm.y *= b;
-            }
+            xm.x *= b;
+            xm.y *= b;
         }
-        store_chan<T>(out, (long)kp * cstride + g, xk);
-        if (two) store_chan<T>(out, (long)k2 * cstride + g, xm);
+        V* row = o + (size_t)g * M;
+        row[kp] = to_chan<T>(xk);
+        if (kp != 0) row[k2] = to_chan<T>(xm);
+    }
+    // kp = M/2 of each transform
+    if (tid < F) {
+        const long g = g0 + tid;
+        if (g < nitems) {
+            const long t = g / ninp;
+            const int i = (int)(g - t * ninp);
+            float2 xk, xm;
+            rsplit<M>(sh, tid * M, M / 2, tw, xk, xm);
+            if (pm) {
+                const fx_phase_model pmi = pm[i];
+                const double tg = (double)(t_first + t);
+                const double tau = pmi.delay0 + pmi.delay_rate * tg;
+                const double fr = (pmi.fringe0 + pmi.fringe_rate * tg) * 0.15915494309189535;
+                double c1 = (double)(M / 2) * tau / (double)N + fr;
+                c1 -= rint(c1);
+                float s1, co1;
+                sincosf((float)(6.283185307179586 * c1), &s1, &co1);
+                xk = cmul(xk, make_float2(co1, s1));
+            }
+            if (mult) xk = cmul(xk, mult[(size_t)i * M + M / 2]);
+            if (sc_chan) {
+                const float a = sc_chan[M / 2] * sc_inp[i];
+                xk.x *= a;
+                xk.y *= a;
+            }
+            o[(size_t)g * M + M / 2] = to_chan<T>(xk);
+        }
+    }
+}
+
+// Corner turn nat[g][c] -> out[c][g] for g < nitems, c < C, through a
+// 64 x 64 LDS tile: both the 256-thread reads (rows of nat) and the writes
+// (rows of out) are contiguous runs of 64 elements.  XCD-aware: tiles of
+// neighbouring items share an XCD.
+template <typename V>
+__global__ __launch_bounds__(256) void k_cornerturn(const V* __restrict__ nat, V* __restrict__ out,
+                                                    long nitems, int C)
+{
+    __shared__ V tile[64][65];
+    const long ntg = (nitems + 63) / 64;
+    const long ntc = C / 64;
+    const long lb = xcd_block(ntg * ntc);
+    if (lb >= ntg * ntc) return;
+    const long tg = lb % ntg;
+    const int tc = (int)(lb / ntg);
+    const int lx = threadIdx.x & 63, ly = threadIdx.x >> 6;
+    const long gbase = tg * 64;
+    const int cbase = tc * 64;
+#pragma unroll
+    for (int r = ly; r < 64; r += 4) {
+        long g = gbase + r;
+        if (g < nitems) tile[r][lx] = nat[(size_t)g * C + cbase + lx];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = ly; r < 64; r += 4) {
+        long g = gbase + lx;
+        if (g < nitems) out[(size_t)(cbase + r) * nitems + g] = tile[lx][r];
     }
 }
 
@@ -374,28 +447,33 @@ int launch_spec(hipStream_t s, const float2* tw, const float* win, const uint32_
 template <int M, typename T>
 int launch_fstage_t(hipStream_t s, const float2* tw, const uint32_t* words, size_t stride,
                     int ninp, long t_first, long nt, const fx_phase_model* pm, const float2* mult,
-                    const float* sc, const float* si, void* out)
+                    const float* sc, const float* si, void* nat, void* out)
 {
     using S = FftShape<M>;
+    typedef typename ChanT<T>::V V;
     long nitems = nt * ninp;
     long nblk = (nitems + S::F - 1) / S::F;
     hipLaunchKernelGGL((k_fstage<M, T>), dim3((unsigned)nblk), dim3(S::NT), 0, s, tw, words, stride,
-                       ninp, t_first, nt, pm, mult, sc, si, reinterpret_cast<T*>(out));
+                       ninp, t_first, nt, pm, mult, sc, si, reinterpret_cast<T*>(nat));
+    if (hipGetLastError() != hipSuccess) return FX_ERR_HIP;
+    long ntiles = ((nitems + 63) / 64) * (M / 64);
+    hipLaunchKernelGGL((k_cornerturn<V>), dim3((unsigned)xcd_grid(ntiles)), dim3(256), 0, s,
+                       reinterpret_cast<const V*>(nat), reinterpret_cast<V*>(out), nitems, M);
     return hipGetLastError() == hipSuccess ? FX_OK : FX_ERR_HIP;
 }
 
 template <int M>
 int launch_fstage(hipStream_t s, const float2* tw, const uint32_t* words, size_t stride, int ninp,
                   long t_first, long nt, const fx_phase_model* pm, const float2* mult,
-                  const float* sc, const float* si, int out_type, void* out)
+                  const float* sc, const float* si, int out_type, void* nat, void* out)
 {
     switch (out_type) {
     case FX_PREC_F32:
-        return launch_fstage_t<M, float2>(s, tw, words, stride, ninp, t_first, nt, pm, mult, sc, si, out);
+        return launch_fstage_t<M, float2>(s, tw, words, stride, ninp, t_first, nt, pm, mult, sc, si, nat, out);
     case FX_PREC_F16:
-        return launch_fstage_t<M, __half2>(s, tw, words, stride, ninp, t_first, nt, pm, mult, sc, si, out);
+        return launch_fstage_t<M, __half2>(s, tw, words, stride, ninp, t_first, nt, pm, mult, sc, si, nat, out);
     case FX_PREC_FP8:
-        return launch_fstage_t<M, uint16_t>(s, tw, words, stride, ninp, t_first, nt, pm, mult, sc, si, out);
+        return launch_fstage_t<M, uint16_t>(s, tw, words, stride, ninp, t_first, nt, pm, mult, sc, si, nat, out);
     default:
         return FX_ERR_ARG;
     }
@@ -468,16 +546,16 @@ int fxk_spectrometer(hipStream_t s, const float2* tw, const float* win, const ui
 
 int fxk_fstage(hipStream_t s, const float2* tw, const uint32_t* words, size_t stride, int ninp,
                int n, long t_first, long nt, const fx_phase_model* pm, const float2* mult,
-               const float* sc, const float* si, int out_type, void* out)
+               const float* sc, const float* si, int out_type, void* nat, void* out)
 {
     switch (n) {
-    case 256: return launch_fstage<128>(s, tw, words, stride, ninp, t_first, nt, pm, mult, sc, si, out_type, out);
-    case 512: return launch_fstage<256>(s, tw, words, stride, ninp, t_first, nt, pm, mult, sc, si, out_type, out);
-    case 1024: return launch_fstage<512>(s, tw, words, stride, ninp, t_first, nt, pm, mult, sc, si, out_type, out);
-    case 2048: return launch_fstage<1024>(s, tw, words, stride, ninp, t_first, nt, pm, mult, sc, si, out_type, out);
-    case 4096: return launch_fstage<2048>(s, tw, words, stride, ninp, t_first, nt, pm, mult, sc, si, out_type, out);
-    case 8192: return launch_fstage<4096>(s, tw, words, stride, ninp, t_first, nt, pm, mult, sc, si, out_type, out);
-    case 16384: return launch_fstage<8192>(s, tw, words, stride, ninp, t_first, nt, pm, mult, sc, si, out_type, out);
+    case 256: return launch_fstage<128>(s, tw, words, stride, ninp, t_first, nt, pm, mult, sc, si, out_type, nat, out);
+    case 512: return launch_fstage<256>(s, tw, words, stride, ninp, t_first, nt, pm, mult, sc, si, out_type, nat, out);
+    case 1024: return launch_fstage<512>(s, tw, words, stride, ninp, t_first, nt, pm, mult, sc, si, out_type, nat, out);
+    case 2048: return launch_fstage<1024>(s, tw, words, stride, ninp, t_first, nt, pm, mult, sc, si, out_type, nat, out);
+    case 4096: return launch_fstage<2048>(s, tw, words, stride, ninp, t_first, nt, pm, mult, sc, si, out_type, nat, out);
+    case 8192: return launch_fstage<4096>(s, tw, words, stride, ninp, t_first, nt, pm, mult, sc, si, out_type, nat, out);
+    case 16384: return launch_fstage<8192>(s, tw, words, stride, ninp, t_first, nt, pm, mult, sc, si, out_type, nat, out);
     default: return FX_ERR_ARG;
     }
 }

@@ -116,6 +116,34 @@ __device__ __forceinline__ void dft_reg(float2* u)
     }
 }
 
+// Twiddles come from a two-level LDS table: W^e = T_hi[e >> 8] * T_lo[e & 255]
+// with T_lo[j] = W^j (j < 256) and T_hi[j] = W^{256 j} (j < 128), W =
+// exp(-2 pi i / FX_TW_N).  384 entries (3 KiB) replace gathers from the 256 KiB
+// global table; the product costs one complex multiply (about 1 ulp).
+#define FX_TWL 384
+__device__ __forceinline__ void twl_load(float2* twl, const float2* __restrict__ tw, int tid, int nt)
+{
+    for (int j = tid; j < FX_TWL; j += nt) twl[j] = (j < 256) ? tw[j] : tw[(j - 256) << 8];
+}
+__device__ __forceinline__ float2 twl_get(const float2* twl, int e)
+{
+    float2 lo = twl[e & 255];
+    int hi = e >> 8;
+    return hi ? cmul(twl[256 + hi], lo) : lo;
+}
+
+// XCD-aware block index: the hardware hands block b to XCD b % 8, so logical
+// block (b % 8) * per + b / 8 gives every XCD a contiguous range of logical
+// blocks (neighbouring tiles share that XCD's L2).  The grid is padded to a
+// multiple of 8; logical blocks >= nlogical return.
+__host__ __device__ inline long xcd_grid(long nlogical) { return ((nlogical + 7) / 8) * 8; }
+__device__ __forceinline__ long xcd_block(long nlogical)
+{
+    long per = (nlogical + 7) / 8;
+    long b = blockIdx.x;
+    return (b & 7) * per + (b >> 3);
+}
+
 // One radix-R Stockham pass over F complex FFTs of length M held in LDS
 // (padded indexing), NT threads, current sub-transform span P.
 // Butterfly j of FFT f reads x[j + r M/R], multiplies by W_{PR}^{r k}
@@ -144,7 +172,7 @@ __device__ __forceinline__ void stockham_pass(float2* sh, const float2* __restri
         int k = j & (P - 1);
         if constexpr (P > 1) {
 #pragma unroll
-            for (int r = 1; r < R; r++) u[b][r] = cmul(u[b][r], tw[(r * k) * (FX_TW_N / (P * R))]);
+            for (int r = 1; r < R; r++) u[b][r] = cmul(u[b][r], twl_get(tw, (r * k) * (FX_TW_N / (P * R))));
         }
         dft_reg<R>(u[b]);
         int base = f * M + (j - k) * R + k;
@@ -177,7 +205,7 @@ __device__ __forceinline__ void rsplit(const float2* sh, int base, int k, const 
     float2 b = sh[lpad(base + ((M - k) & (M - 1)))];
     float2 e = make_float2(0.5f * (a.x + b.x), 0.5f * (a.y - b.y));
     float2 o = make_float2(0.5f * (a.y + b.y), -0.5f * (a.x - b.x));
-    float2 w = tw[k * (FX_TW_N / (2 * M))];
+    float2 w = twl_get(tw, k * (FX_TW_N / (2 * M)));
     float2 wo = cmul(w, o);
     xk = cadd(e, wo);
     xmk = make_float2(e.x - wo.x, -(e.y - wo.y));
@@ -205,7 +233,7 @@ int fxk_spectrometer(hipStream_t s, const float2* tw, const float* window, const
 int fxk_fstage(hipStream_t s, const float2* tw, const uint32_t* words, size_t in_stride_words,
                int ninp, int n, long t_first, long nt, const fx_phase_model* pm,
                const float2* mult, const float* scale_chan, const float* scale_inp, int out_type,
-               void* out);
+               void* nat, void* out);
 int fxk_xengine(hipStream_t s, int in_type, const void* X, int nchan, long nt, int ninp,
                 float* acc);
 int fxk_xengine_codes(hipStream_t s, int mode, const uint8_t* codes, int nchan, long nt, int ninp,
