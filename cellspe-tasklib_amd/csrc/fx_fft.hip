@@ -260,12 +260,47 @@ __global__ __launch_bounds__(FftShape<M>::NT) void k_fstage(
     constexpr int WPT = N / 16;            // packed words per transform
     constexpr int WTH = (S::FM / 8) / NT;  // packed words per thread (2)
     typedef typename ChanT<T>::V V;
+    // phasor tables: channel k = 64 hi + lo of item f rotates by
+    // ph_hi[f][hi] * ph_lo[f][lo] = e^{2 pi i (k tau/N + fringe/2pi)}, both
+    // factors evaluated from the FP64 phase model (sincospi in FP64, one
+    // rounding to FP32), one complex multiply per channel
+    constexpr int PLO = M < 64 ? M : 64;
+    constexpr int PHI = M / PLO;
     __shared__ float2 sh[S::LDS];
     __shared__ float2 tw[FX_TWL];
+    __shared__ float2 ph_lo[F][PLO];
+    __shared__ float2 ph_hi[F][PHI];
     const int tid = threadIdx.x;
     twl_load(tw, twg, tid, NT);
     const long nitems = nt * ninp;
     const long g0 = (long)blockIdx.x * F;
+    if (pm) {
+        for (int e = tid; e < F * (PLO + PHI); e += NT) {
+            const int f = e / (PLO + PHI);
+            const int j = e - f * (PLO + PHI);
+            const long g = g0 + f;
+            if (g >= nitems) continue;
+            const long t = g / ninp;
+            const int i = (int)(g - t * ninp);
+            const fx_phase_model pmi = pm[i];
+            const double tg = (double)(t_first + t);
+            const double tau = pmi.delay0 + pmi.delay_rate * tg;
+            double cyc;
+            if (j < PLO) {
+                cyc = (double)j * tau / (double)N;
+            } else {
+                cyc = (double)((j - PLO) * PLO) * tau / (double)N +
+                      (pmi.fringe0 + pmi.fringe_rate * tg) * 0.15915494309189535;
+            }
+            cyc -= rint(cyc);
+            double sn, cs;
+            sincospi(2.0 * cyc, &sn, &cs);
+            if (j < PLO)
+                ph_lo[f][j] = make_float2((float)cs, (float)sn);
+            else
+                ph_hi[f][j - PLO] = make_float2((float)cs, (float)sn);
+        }
+    }
 
     // ---- load + unpack F transforms
     {
@@ -315,19 +350,9 @@ __global__ __launch_bounds__(FftShape<M>::NT) void k_fstage(
         rsplit<M>(sh, f * M, kp, tw, xk, xm);
         const int k2 = M - kp;
         if (pm) {
-            const fx_phase_model pmi = pm[i];
-            const double tg = (double)(t_first + t);
-            const double tau = pmi.delay0 + pmi.delay_rate * tg;
-            const double fr = (pmi.fringe0 + pmi.fringe_rate * tg) * 0.15915494309189535;
-            double c1 = (double)kp * tau / (double)N + fr;
-            double c2 = (double)k2 * tau / (double)N + fr;
-            c1 -= rint(c1);
-            c2 -= rint(c2);
-            float s1, co1, s2, co2;
-            sincosf((float)(6.283185307179586 * c1), &s1, &co1);
-            sincosf((float)(6.283185307179586 * c2), &s2, &co2);
-            xk = cmul(xk, make_float2(co1, s1));
-            xm = cmul(xm, make_float2(co2, s2));
+            xk = cmul(xk, cmul(ph_hi[f][kp / PLO], ph_lo[f][kp % PLO]));
+            const int k2m = k2 & (M - 1);   // kp = 0: k2 = M is the dropped Nyquist bin
+            xm = cmul(xm, cmul(ph_hi[f][k2m / PLO], ph_lo[f][k2m % PLO]));
         }
         if (mult) {
             xk = cmul(xk, mult[(size_t)i * M + kp]);
@@ -354,17 +379,7 @@ __global__ __launch_bounds__(FftShape<M>::NT) void k_fstage(
             const int i = (int)(g - t * ninp);
             float2 xk, xm;
             rsplit<M>(sh, tid * M, M / 2, tw, xk, xm);
-            if (pm) {
-                const fx_phase_model pmi = pm[i];
-                const double tg = (double)(t_first + t);
-                const double tau = pmi.delay0 + pmi.delay_rate * tg;
-                const double fr = (pmi.fringe0 + pmi.fringe_rate * tg) * 0.15915494309189535;
-                double c1 = (double)(M / 2) * tau / (double)N + fr;
-                c1 -= rint(c1);
-                float s1, co1;
-                sincosf((float)(6.283185307179586 * c1), &s1, &co1);
-                xk = cmul(xk, make_float2(co1, s1));
-            }
+            if (pm) xk = cmul(xk, cmul(ph_hi[tid][(M / 2) / PLO], ph_lo[tid][(M / 2) % PLO]));
             if (mult) xk = cmul(xk, mult[(size_t)i * M + M / 2]);
             if (sc_chan) {
                 const float a = sc_chan[M / 2] * sc_inp[i];

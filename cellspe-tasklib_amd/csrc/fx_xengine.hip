@@ -7,34 +7,39 @@
 //
 // Complex arithmetic as two real GEMMs sharing the left operand: with
 // u_i = (re_i(t0), im_i(t0), re_i(t1), ...) along K and the rotated copy
-// v_j = (-im_j(t0), re_j(t0), ...), Re V = U U^T and Im V = U V^T.  Both
-// operands are read straight from the channelized layout X[c][t][i] (input
-// fastest), so each MFMA fragment load is a 128-byte coalesced row segment.
+// v_j = (-im_j(t0), re_j(t0), ...), Re V = U U^T and Im V = U V^T.
 //
-// A workgroup of 4 waves owns one 64x64 input tile (bi64 <= bj64) of one
-// channel; wave w computes the 32x32 block (2*bi64 + w/2, 2*bj64 + w%2) and
-// skips the strictly-lower block of diagonal tiles.  Accumulators live in
-// registers for the chunk and are added into an engine-layout buffer
-// acc[c][tile][wave][reg][lane] (fully coalesced), which the dump kernel
-// converts to the canonical [p][c] complex64 visibilities.
+// Tiling: a workgroup of 4 waves owns one 128 x 128 input tile (ti <= tj) of
+// one channel.  Per stage it copies the KT-sample slabs X[c][t0..t0+KT][rows]
+// of its two row ranges (one for diagonal tiles) from HBM into LDS with
+// coalesced 16-byte loads, prefetching the next stage into registers while the
+// MFMAs run, so every channelized sample is read from HBM once per tile row.
+// Wave w accumulates the 64 x 64 quadrant (w/2, w%2) as 2 x 2 blocks of
+// 32 x 32 (re and im: 128 accumulators); the strictly-lower quadrant of a
+// diagonal tile idles.  Accumulators are added into an engine-layout buffer
+// acc[c][tile][wave][block][reg][lane] (coalesced), converted to the
+// canonical [p][c] complex64 visibilities by k_xengine_dump.
 #include "fx_internal.h"
 
 #include <hip/hip_fp8.h>
 
 typedef _Float16 fx_half8 __attribute__((ext_vector_type(8)));
 typedef float fx_float16 __attribute__((ext_vector_type(16)));
+typedef int fx_int4v __attribute__((ext_vector_type(4)));
 typedef int fx_int8 __attribute__((ext_vector_type(8)));
 
 namespace {
 
-// number of 64x64 upper-triangle tiles for ninp inputs
-__host__ __device__ inline int ntiles64(int ninp)
+constexpr int TS = 128;     // tile rows
+constexpr int XNT = 256;    // threads per workgroup
+
+__host__ __device__ inline int ntiles128(int ninp)
 {
-    int nb = (ninp + 63) / 64;
+    int nb = (ninp + TS - 1) / TS;
     return nb * (nb + 1) / 2;
 }
 
-// tile index -> (bi, bj), row-major over the upper triangle
+// tile index -> (ti, tj), row-major over the upper triangle
 __device__ inline void tile_coords(int tile, int nb, int& bi, int& bj)
 {
     int r = 0, rem = tile;
@@ -46,126 +51,192 @@ __device__ inline void tile_coords(int tile, int nb, int& bi, int& bj)
     bj = r + rem;
 }
 
-__device__ inline void acc_store(float* dst, int lane, const fx_float16& re, const fx_float16& im)
-{
-#pragma unroll
-    for (int r = 0; r < 16; r++) {
-        dst[r * 64 + lane] += re[r];
-        dst[(16 + r) * 64 + lane] += im[r];
-    }
-}
-
-// ---------------------------------------------------------------------------
-// FP32: v_mfma_f32_32x32x2_f32, K = 2 = one complex sample.
-//   lanes 0-31 carry k=0 (real parts), lanes 32-63 k=1 (imaginary parts).
-// ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_xengine_f32(const float2* __restrict__ X, int nchan,
-                                                     long nt, int ninp, float* __restrict__ acc)
-{
-    const int c = blockIdx.y;
-    const int tile = blockIdx.x;
-    const int nb = (ninp + 63) / 64;
-    const int ntl = nb * (nb + 1) / 2;
-    int bi64, bj64;
-    tile_coords(tile, nb, bi64, bj64);
-    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int bi = 2 * bi64 + (w >> 1), bj = 2 * bj64 + (w & 1);
-    if (bi > bj || bi * 32 >= ninp || bj * 32 >= ninp) return;
-    const int ra = bi * 32 + (lane & 31), rb = bj * 32 + (lane & 31);
-    const bool va = ra < ninp, vb = rb < ninp;
-    const bool hi = lane >= 32;
-    const float2* base = X + (size_t)c * (size_t)nt * (size_t)ninp;
-    fx_float16 dre = {}, dim = {};
-    const float2 z = make_float2(0.f, 0.f);
-    long t = 0;
-    for (; t + 4 <= nt; t += 4) {
-        float2 xa[4], xb[4];
-#pragma unroll
-        for (int q = 0; q < 4; q++) {
-            xa[q] = va ? base[(t + q) * ninp + ra] : z;
-            xb[q] = vb ? base[(t + q) * ninp + rb] : z;
-        }
-#pragma unroll
-        for (int q = 0; q < 4; q++) {
-            float a = hi ? xa[q].y : xa[q].x;
-            float b = hi ? xb[q].y : xb[q].x;
-            float br = hi ? xb[q].x : -xb[q].y;
-            dre = __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, dre, 0, 0, 0);
-            dim = __builtin_amdgcn_mfma_f32_32x32x2f32(a, br, dim, 0, 0, 0);
-        }
-    }
-    for (; t < nt; t++) {
-        float2 xa = va ? base[t * ninp + ra] : z;
-        float2 xb = vb ? base[t * ninp + rb] : z;
-        float a = hi ? xa.y : xa.x;
-        float b = hi ? xb.y : xb.x;
-        float br = hi ? xb.x : -xb.y;
-        dre = __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, dre, 0, 0, 0);
-        dim = __builtin_amdgcn_mfma_f32_32x32x2f32(a, br, dim, 0, 0, 0);
-    }
-    float* dst = acc + (((size_t)c * ntl + tile) * 4 + w) * (32 * 64);
-    acc_store(dst, lane, dre, dim);
-}
-
-// ---------------------------------------------------------------------------
-// FP16: v_mfma_f32_32x32x16_f16, K = 16 = eight complex samples; lane half h
-// carries samples t + 4h .. t + 4h + 3 as (re, im) pairs.  nt % 8 == 0.
-// ---------------------------------------------------------------------------
 __device__ __forceinline__ uint32_t rot_h2(uint32_t v)
 {
     // (re, im) -> (-im, re) on a packed half2: swap halves, flip the new low sign
     return ((v >> 16) | (v << 16)) ^ 0x00008000u;
 }
 
-__global__ __launch_bounds__(256) void k_xengine_f16(const uint32_t* __restrict__ X, int nchan,
-                                                     long nt, int ninp, float* __restrict__ acc)
+template <int PREC> struct XTraits;
+template <> struct XTraits<FX_PREC_F32> {
+    typedef float2 E;                // one complex sample
+    static constexpr int KT = 16;    // samples per LDS stage (16 KiB per slab)
+};
+template <> struct XTraits<FX_PREC_F16> {
+    typedef uint32_t E;
+    static constexpr int KT = 32;
+};
+
+template <int PREC>
+__global__ __launch_bounds__(XNT, 2) void k_xengine(const typename XTraits<PREC>::E* __restrict__ X,
+                                                    int nchan, long nt, int ninp,
+                                                    float* __restrict__ acc)
 {
-    const int c = blockIdx.y;
-    const int tile = blockIdx.x;
-    const int nb = (ninp + 63) / 64;
+    typedef typename XTraits<PREC>::E E;
+    constexpr int KT = XTraits<PREC>::KT;
+    constexpr int CH = 16 / sizeof(E);                   // samples per 16-byte chunk
+    constexpr int SLAB_CHUNKS = KT * TS / CH;            // chunks per slab
+    constexpr int LPT = SLAB_CHUNKS / XNT;               // chunks per thread per slab
+    __shared__ E slab[2][KT][TS];
+
+    const int nb = (ninp + TS - 1) / TS;
     const int ntl = nb * (nb + 1) / 2;
-    int bi64, bj64;
-    tile_coords(tile, nb, bi64, bj64);
-    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int bi = 2 * bi64 + (w >> 1), bj = 2 * bj64 + (w & 1);
-    if (bi > bj || bi * 32 >= ninp || bj * 32 >= ninp) return;
-    const int ra = bi * 32 + (lane & 31), rb = bj * 32 + (lane & 31);
-    const bool va = ra < ninp, vb = rb < ninp;
-    const int h = lane >> 5;
-    const uint32_t* base = X + (size_t)c * (size_t)nt * (size_t)ninp;
-    fx_float16 dre = {}, dim = {};
-    for (long t = 0; t < nt; t += 8) {
-        uint32_t ua[4], ub[4];
+    const long lb = xcd_block((long)ntl * nchan);
+    if (lb >= (long)ntl * nchan) return;
+    const int c = (int)(lb / ntl);
+    const int tile = (int)(lb - (long)c * ntl);
+    int ti, tj;
+    tile_coords(tile, nb, ti, tj);
+    const bool diag = ti == tj;
+    const int nslab = diag ? 1 : 2;
+    const int tid = threadIdx.x;
+    const int w = tid >> 6, lane = tid & 63;
+    const int qi = w >> 1, qj = w & 1;
+    const bool active = !(diag && qi > qj);
+    const E* base = X + (size_t)c * (size_t)nt * (size_t)ninp;
+    const int row0[2] = {ti * TS, tj * TS};
+
+    fx_float16 dre[2][2], dim[2][2];
 #pragma unroll
-        for (int q = 0; q < 4; q++) {
-            long tt = t + 4 * h + q;
-            bool vt = tt < nt;
-            ua[q] = (va && vt) ? base[tt * ninp + ra] : 0u;
-            ub[q] = (vb && vt) ? base[tt * ninp + rb] : 0u;
-        }
-        fx_int8 ia, ib, ir;
+    for (int a = 0; a < 2; a++)
 #pragma unroll
-        for (int q = 0; q < 4; q++) {
-            ia[q] = (int)ua[q];
-            ib[q] = (int)ub[q];
-            ir[q] = (int)rot_h2(ub[q]);
+        for (int b = 0; b < 2; b++) {
+            dre[a][b] = fx_float16{};
+            dim[a][b] = fx_float16{};
         }
-        // only the low 4 dwords carry data: repack as 8 halfs
-        fx_half8 a, b, br;
-        {
-            typedef int int4v __attribute__((ext_vector_type(4)));
-            int4v a4 = {ia[0], ia[1], ia[2], ia[3]};
-            int4v b4 = {ib[0], ib[1], ib[2], ib[3]};
-            int4v r4 = {ir[0], ir[1], ir[2], ir[3]};
-            a = __builtin_bit_cast(fx_half8, a4);
-            b = __builtin_bit_cast(fx_half8, b4);
-            br = __builtin_bit_cast(fx_half8, r4);
+
+    // global -> register prefetch of one stage: chunk q of slab s covers
+    // sample t0 + q / (TS/CH), rows (q % (TS/CH)) * CH .. + CH
+    uint4 pre[2][LPT];
+    auto gload = [&](long t0) {
+#pragma unroll
+        for (int s = 0; s < 2; s++) {
+            if (s >= nslab) break;
+#pragma unroll
+            for (int k = 0; k < LPT; k++) {
+                const int q = tid + k * XNT;
+                const int tt = q / (TS / CH);
+                const int rr = (q - tt * (TS / CH)) * CH;
+                const long t = t0 + tt;
+                const int row = row0[s] + rr;
+                uint4 v = make_uint4(0u, 0u, 0u, 0u);
+                if (t < nt) {
+                    if (row + CH <= ninp && (ninp % CH) == 0) {
+                        v = *reinterpret_cast<const uint4*>(base + (size_t)t * ninp + row);
+                    } else {
+                        E tmp[CH];
+#pragma unroll
+                        for (int e = 0; e < CH; e++) {
+                            if (row + e < ninp) tmp[e] = base[(size_t)t * ninp + row + e];
+                            else __builtin_memset(&tmp[e], 0, sizeof(E));
+                        }
+                        v = *reinterpret_cast<uint4*>(tmp);
+                    }
+                }
+                pre[s][k] = v;
+            }
         }
-        dre = __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b, dre, 0, 0, 0);
-        dim = __builtin_amdgcn_mfma_f32_32x32x16_f16(a, br, dim, 0, 0, 0);
+    };
+    auto lstore = [&]() {
+#pragma unroll
+        for (int s = 0; s < 2; s++) {
+            if (s >= nslab) break;
+#pragma unroll
+            for (int k = 0; k < LPT; k++) {
+                const int q = tid + k * XNT;
+                const int tt = q / (TS / CH);
+                const int rr = (q - tt * (TS / CH)) * CH;
+                *reinterpret_cast<uint4*>(&slab[s][tt][rr]) = pre[s][k];
+            }
+        }
+    };
+
+    const int sb = diag ? 0 : 1;          // LDS slab holding the j rows
+    const int r32 = lane & 31, h = lane >> 5;
+    const long nstage = (nt + KT - 1) / KT;
+    gload(0);
+    for (long st = 0; st < nstage; st++) {
+        __syncthreads();
+        lstore();
+        __syncthreads();
+        if (st + 1 < nstage) gload((st + 1) * KT);
+        if (!active) continue;
+        if constexpr (PREC == FX_PREC_F16) {
+#pragma unroll
+            for (int ks = 0; ks < KT / 8; ks++) {
+                fx_half8 fa[2], fb[2], fr[2];
+#pragma unroll
+                for (int a = 0; a < 2; a++) {
+                    fx_int4v ia;
+                    const int row = (2 * qi + a) * 32 + r32;
+#pragma unroll
+                    for (int q = 0; q < 4; q++) ia[q] = (int)slab[0][ks * 8 + 4 * h + q][row];
+                    fa[a] = __builtin_bit_cast(fx_half8, ia);
+                }
+#pragma unroll
+                for (int b = 0; b < 2; b++) {
+                    fx_int4v ib, ir;
+                    const int row = (2 * qj + b) * 32 + r32;
+#pragma unroll
+                    for (int q = 0; q < 4; q++) {
+                        uint32_t v = slab[sb][ks * 8 + 4 * h + q][row];
+                        ib[q] = (int)v;
+                        ir[q] = (int)rot_h2(v);
+                    }
+                    fb[b] = __builtin_bit_cast(fx_half8, ib);
+                    fr[b] = __builtin_bit_cast(fx_half8, ir);
+                }
+#pragma unroll
+                for (int a = 0; a < 2; a++)
+#pragma unroll
+                    for (int b = 0; b < 2; b++) {
+                        if (diag && qi == qj && a > b) continue;
+                        dre[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_f16(fa[a], fb[b], dre[a][b], 0, 0, 0);
+                        dim[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_f16(fa[a], fr[b], dim[a][b], 0, 0, 0);
+                    }
+            }
+        } else {
+            // FP32: K = 2 reals = one complex sample per MFMA; lanes 0-31
+            // carry the real parts, lanes 32-63 the imaginary parts
+#pragma unroll
+            for (int ks = 0; ks < KT; ks++) {
+                float fa[2], fb[2], fr[2];
+#pragma unroll
+                for (int a = 0; a < 2; a++) {
+                    const float2 v = slab[0][ks][(2 * qi + a) * 32 + r32];
+                    fa[a] = h ? v.y : v.x;
+                }
+#pragma unroll
+                for (int b = 0; b < 2; b++) {
+                    const float2 v = slab[sb][ks][(2 * qj + b) * 32 + r32];
+                    fb[b] = h ? v.y : v.x;
+                    fr[b] = h ? v.x : -v.y;
+                }
+#pragma unroll
+                for (int a = 0; a < 2; a++)
+#pragma unroll
+                    for (int b = 0; b < 2; b++) {
+                        if (diag && qi == qj && a > b) continue;
+                        dre[a][b] = __builtin_amdgcn_mfma_f32_32x32x2f32(fa[a], fb[b], dre[a][b], 0, 0, 0);
+                        dim[a][b] = __builtin_amdgcn_mfma_f32_32x32x2f32(fa[a], fr[b], dim[a][b], 0, 0, 0);
+                    }
+            }
+        }
     }
-    float* dst = acc + (((size_t)c * ntl + tile) * 4 + w) * (32 * 64);
-    acc_store(dst, lane, dre, dim);
+    if (!active) return;
+    float* dst = acc + (((size_t)c * ntl + tile) * 16 + w * 4) * (32 * 64);
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int b = 0; b < 2; b++) {
+            if (diag && qi == qj && a > b) continue;
+            float* d = dst + (a * 2 + b) * (32 * 64);
+#pragma unroll
+            for (int r = 0; r < 16; r++) {
+                d[r * 64 + lane] += dre[a][b][r];
+                d[(16 + r) * 64 + lane] += dim[a][b][r];
+            }
+        }
 }
 
 // ---------------------------------------------------------------------------
@@ -185,19 +256,21 @@ __global__ void k_xengine_dump(const float* __restrict__ acc, int nchan, int nin
     const long p = idx / nchan;
     const int2 ij = pij[p];
     const int i = ij.x, j = ij.y;
-    const int nb = (ninp + 63) / 64;
+    const int nb = (ninp + TS - 1) / TS;
     const int ntl = nb * (nb + 1) / 2;
-    const int bi64 = i >> 6, bj64 = j >> 6;
-    const int tile = bi64 * nb - bi64 * (bi64 - 1) / 2 + (bj64 - bi64);
-    const int w = (((i >> 5) & 1) << 1) | ((j >> 5) & 1);
+    const int ti = i / TS, tj = j / TS;
+    const int tile = ti * nb - ti * (ti - 1) / 2 + (tj - ti);
+    const int bi = (i % TS) >> 5, bj = (j % TS) >> 5;
+    const int w = (bi >> 1) * 2 + (bj >> 1);
+    const int s = (bi & 1) * 2 + (bj & 1);
     const int ii = i & 31, jj = j & 31;
     const int r = 4 * (ii >> 3) + (ii & 3);
     const int l = jj + 32 * ((ii >> 2) & 1);
-    const size_t off = (((size_t)c * ntl + tile) * 4 + w) * (32 * 64);
+    const size_t off = (((size_t)c * ntl + tile) * 16 + w * 4 + s) * (32 * 64);
     float re = 0.f, im = 0.f;
-    for (int s = 0; s < nsets; s++) {
-        const float* a = acc + s * set_stride + off;
-        float k = coef ? coef[s] : 1.0f;
+    for (int q = 0; q < nsets; q++) {
+        const float* a = acc + q * set_stride + off;
+        float k = coef ? coef[q] : 1.0f;
         re += k * a[r * 64 + l];
         im += k * a[(16 + r) * 64 + l];
     }
@@ -250,17 +323,17 @@ __global__ void k_mfma_probe(int which, float* out)
 
 size_t fxk_xengine_acc_floats(int nchan, int ninp, int nsets)
 {
-    return (size_t)nsets * (size_t)nchan * (size_t)ntiles64(ninp) * 4u * 32u * 64u;
+    return (size_t)nsets * (size_t)nchan * (size_t)ntiles128(ninp) * 16u * 32u * 64u;
 }
 
 int fxk_xengine(hipStream_t s, int in_type, const void* X, int nchan, long nt, int ninp, float* acc)
 {
-    dim3 grid((unsigned)ntiles64(ninp), (unsigned)nchan);
+    long nblk = xcd_grid((long)ntiles128(ninp) * nchan);
     if (in_type == FX_PREC_F32) {
-        hipLaunchKernelGGL(k_xengine_f32, grid, dim3(256), 0, s,
+        hipLaunchKernelGGL(k_xengine<FX_PREC_F32>, dim3((unsigned)nblk), dim3(XNT), 0, s,
                            reinterpret_cast<const float2*>(X), nchan, nt, ninp, acc);
     } else if (in_type == FX_PREC_F16) {
-        hipLaunchKernelGGL(k_xengine_f16, grid, dim3(256), 0, s,
+        hipLaunchKernelGGL(k_xengine<FX_PREC_F16>, dim3((unsigned)nblk), dim3(XNT), 0, s,
                            reinterpret_cast<const uint32_t*>(X), nchan, nt, ninp, acc);
     } else {
         return FX_ERR_UNSUPPORTED;
