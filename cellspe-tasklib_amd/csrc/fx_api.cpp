@@ -378,6 +378,16 @@ static float* get_window(fx_ctx* ctx, int kind, int n)
         } else if (kind == FX_WIN_COSINE2) {
             double c = cos(M_PI * ((double)i - 0.5 * (double)n + 0.5) / (double)n);
             v = c * c;
+        } else if (kind >= FX_WIN_HAMMING) {
+            // generalized cosine windows over n-1 intervals (TaskCoreIPP.cpp:
+            // 589-602: Hamming, Blackman with alpha = 0.5, Hann)
+            const double x = 2.0 * M_PI * (double)i / (double)(n - 1);
+            if (kind == FX_WIN_HAMMING)
+                v = 0.54 - 0.46 * cos(x);
+            else if (kind == FX_WIN_BLACKMAN)
+                v = 0.75 - 0.5 * cos(x) - 0.25 * cos(2.0 * x);
+            else
+                v = 0.5 - 0.5 * cos(x);
         }
         w[(size_t)i] = (float)v;
     }
@@ -410,12 +420,12 @@ static int ensure_scratch(fx_ctx* ctx, size_t bytes)
 extern "C" int fx_spectrometer(fx_ctx* ctx, const uint32_t* d_words, size_t ntime, int nchan, int n,
                                int integrate_ffts, int window, float* d_out)
 {
-    return fx_spectrometer_ex(ctx, d_words, ntime, nchan, n, 1, integrate_ffts, window, 0, d_out, nullptr);
+    return fx_spectrometer_ex(ctx, d_words, ntime, nchan, n, 1, integrate_ffts, window, 0, 0, d_out, nullptr);
 }
 
 extern "C" int fx_spectrometer_ex(fx_ctx* ctx, const uint32_t* d_words, size_t ntime, int nchan, int n,
                                   int overlap, int integrate_ffts, int window, int continuous,
-                                  float* d_out, long* nspec_out)
+                                  int keep_nyquist, float* d_out, long* nspec_out)
 {
     int rc = ctx_device(ctx);
     if (rc) return rc;
@@ -423,7 +433,7 @@ extern "C" int fx_spectrometer_ex(fx_ctx* ctx, const uint32_t* d_words, size_t n
     FX_REQUIRE(is_pow2(n) && n >= 256 && n <= FX_MAX_N, FX_ERR_ARG, "fx_spectrometer: n=%d", n);
     FX_REQUIRE(integrate_ffts >= 1, FX_ERR_ARG, "fx_spectrometer: integrate_ffts=%d", integrate_ffts);
     FX_REQUIRE(overlap >= 1 && n % overlap == 0, FX_ERR_ARG, "fx_spectrometer: overlap %d", overlap);
-    FX_REQUIRE(window >= FX_WIN_NONE && window <= FX_WIN_COSINE2, FX_ERR_ARG,
+    FX_REQUIRE(window >= FX_WIN_NONE && window <= FX_WIN_HANN, FX_ERR_ARG,
                "fx_spectrometer: window %d", window);
     const long hop = n / overlap;
     long nspec, spec_stride;
@@ -450,11 +460,12 @@ extern "C" int fx_spectrometer_ex(fx_ctx* ctx, const uint32_t* d_words, size_t n
     long per_spec_units = (integrate_ffts + F - 1) / F;
     long want = 512 / (nspec * nchan) + 1;
     int groups = (int)std::min<long>(std::max<long>(want, 1), per_spec_units);
-    size_t part_bytes = (size_t)nspec * nchan * groups * (size_t)FM * sizeof(float);
+    size_t part_bytes = (size_t)nspec * nchan * groups * (size_t)(FM + F) * sizeof(float);
     rc = ensure_scratch(ctx, part_bytes);
     if (rc) return rc;
     FX_CHECK_LAUNCH(fxk_spectrometer(ctx->stream, ctx->tw, win, d_words, hop, spec_stride, nchan, n,
-                                     integrate_ffts, (int)nspec, ctx->scratch, groups, d_out));
+                                     integrate_ffts, (int)nspec, ctx->scratch, groups, keep_nyquist ? 1 : 0,
+                                     d_out));
     return FX_OK;
 }
 

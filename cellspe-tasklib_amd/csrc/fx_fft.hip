@@ -121,7 +121,6 @@ __global__ __launch_bounds__(FftShape<M>::NT) void k_spectrometer(
     float* __restrict__ partial)
 {
     using S = FftShape<M>;
-    constexpr int N = 2 * M;
     __shared__ float2 sh[S::LDS];
     __shared__ float2 tw[FX_TWL];
     const int tid = threadIdx.x;
@@ -140,7 +139,6 @@ __global__ __launch_bounds__(FftShape<M>::NT) void k_spectrometer(
     for (int i = 0; i < S::SLT; i++) acc[i][0] = acc[i][1] = 0.f;
     float accm = 0.f;
 
-    const long fft_first = s * integrate_ffts + (long)g * per_group;
     int fft_count = per_group;
     if ((long)g * per_group + fft_count > integrate_ffts) fft_count = integrate_ffts - g * per_group;
 
@@ -189,31 +187,33 @@ __global__ __launch_bounds__(FftShape<M>::NT) void k_spectrometer(
         }
         __syncthreads();
     }
-    float* pw = partial + (long)wg * S::F * M;
+    // partial rows hold M+1 bins: k = M is the Nyquist bin (the kp = 0 pair's X_{M})
+    float* pw = partial + (long)wg * S::F * (M + 1);
 #pragma unroll
     for (int i = 0; i < S::SLT; i++) {
         int q = tid * S::SLT + i;
         int kp = q / S::F;
         int f = q - kp * S::F;
-        pw[f * M + kp] = acc[i][0];
-        if (kp != 0) pw[f * M + (M - kp)] = acc[i][1];
+        pw[f * (M + 1) + kp] = acc[i][0];
+        pw[f * (M + 1) + (M - kp)] = acc[i][1];
     }
-    if (tid < S::F) pw[tid * M + M / 2] = accm;
+    if (tid < S::F) pw[tid * (M + 1) + M / 2] = accm;
 }
 
 // out[s][c][k] = (1/K) sum_g sum_f partial[(s*nchan+c)*groups+g][f][k]
 __global__ void k_spec_reduce(const float* __restrict__ partial, int M, int F, int groups,
-                              int nchan, long nspec, float scale, float* __restrict__ out)
+                              int nchan, long nspec, float scale, int nyq, float* __restrict__ out)
 {
     long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    long total = nspec * nchan * (long)M;
+    const int MO = M + nyq;   // output bins per spectrum
+    long total = nspec * nchan * (long)MO;
     if (idx >= total) return;
-    int k = (int)(idx % M);
-    long cs = idx / M;
-    const float* p = partial + cs * (long)groups * F * M;
+    int k = (int)(idx % MO);
+    long cs = idx / MO;
+    const float* p = partial + cs * (long)groups * F * (M + 1);
     float sum = 0.f;
     for (int g = 0; g < groups; g++)
-        for (int f = 0; f < F; f++) sum += p[((long)g * F + f) * M + k];
+        for (int f = 0; f < F; f++) sum += p[((long)g * F + f) * (M + 1) + k];
     out[idx] = sum * scale;
 }
 
@@ -443,7 +443,8 @@ int launch_r2c(hipStream_t s, const float2* tw, const float* in, float2* out, lo
 
 template <int M>
 int launch_spec(hipStream_t s, const float2* tw, const float* win, const uint32_t* words, long hop,
-                long spec_stride, int nchan, int integrate_ffts, int nspec, int groups, float* partial, float* out)
+                long spec_stride, int nchan, int integrate_ffts, int nspec, int groups, int nyq, float* partial,
+                float* out)
 {
     using S = FftShape<M>;
     int per_group = (integrate_ffts + groups - 1) / groups;
@@ -452,9 +453,9 @@ int launch_spec(hipStream_t s, const float2* tw, const float* win, const uint32_
     hipLaunchKernelGGL(k_spectrometer<M>, dim3((unsigned)nblk), dim3(S::NT), 0, s, tw, win, words,
                        nchan, integrate_ffts, groups, per_group, hop, spec_stride, partial);
     if (hipGetLastError() != hipSuccess) return FX_ERR_HIP;
-    long total = (long)nspec * nchan * M;
+    long total = (long)nspec * nchan * (M + nyq);
     hipLaunchKernelGGL(k_spec_reduce, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s,
-                       partial, M, S::F, groups, nchan, (long)nspec, 1.0f / (float)integrate_ffts,
+                       partial, M, S::F, groups, nchan, (long)nspec, 1.0f / (float)integrate_ffts, nyq,
                        out);
     return hipGetLastError() == hipSuccess ? FX_OK : FX_ERR_HIP;
 }
@@ -544,17 +545,17 @@ int fxk_fft_r2c(hipStream_t s, const float2* tw, const float* in, float2* out, i
 
 int fxk_spectrometer(hipStream_t s, const float2* tw, const float* win, const uint32_t* words,
                      long hop, long spec_stride, int nchan, int n, int integrate_ffts, int nspec, float* partial,
-                     int groups, float* out)
+                     int groups, int nyq, float* out)
 {
     switch (n) {
-    case 256: return launch_spec<128>(s, tw, win, words, hop, spec_stride, nchan, integrate_ffts, nspec, groups, partial, out);
-    case 512: return launch_spec<256>(s, tw, win, words, hop, spec_stride, nchan, integrate_ffts, nspec, groups, partial, out);
-    case 1024: return launch_spec<512>(s, tw, win, words, hop, spec_stride, nchan, integrate_ffts, nspec, groups, partial, out);
-    case 2048: return launch_spec<1024>(s, tw, win, words, hop, spec_stride, nchan, integrate_ffts, nspec, groups, partial, out);
-    case 4096: return launch_spec<2048>(s, tw, win, words, hop, spec_stride, nchan, integrate_ffts, nspec, groups, partial, out);
-    case 8192: return launch_spec<4096>(s, tw, win, words, hop, spec_stride, nchan, integrate_ffts, nspec, groups, partial, out);
-    case 16384: return launch_spec<8192>(s, tw, win, words, hop, spec_stride, nchan, integrate_ffts, nspec, groups, partial, out);
-    case 32768: return launch_spec<16384>(s, tw, win, words, hop, spec_stride, nchan, integrate_ffts, nspec, groups, partial, out);
+    case 256: return launch_spec<128>(s, tw, win, words, hop, spec_stride, nchan, integrate_ffts, nspec, groups, nyq, partial, out);
+    case 512: return launch_spec<256>(s, tw, win, words, hop, spec_stride, nchan, integrate_ffts, nspec, groups, nyq, partial, out);
+    case 1024: return launch_spec<512>(s, tw, win, words, hop, spec_stride, nchan, integrate_ffts, nspec, groups, nyq, partial, out);
+    case 2048: return launch_spec<1024>(s, tw, win, words, hop, spec_stride, nchan, integrate_ffts, nspec, groups, nyq, partial, out);
+    case 4096: return launch_spec<2048>(s, tw, win, words, hop, spec_stride, nchan, integrate_ffts, nspec, groups, nyq, partial, out);
+    case 8192: return launch_spec<4096>(s, tw, win, words, hop, spec_stride, nchan, integrate_ffts, nspec, groups, nyq, partial, out);
+    case 16384: return launch_spec<8192>(s, tw, win, words, hop, spec_stride, nchan, integrate_ffts, nspec, groups, nyq, partial, out);
+    case 32768: return launch_spec<16384>(s, tw, win, words, hop, spec_stride, nchan, integrate_ffts, nspec, groups, nyq, partial, out);
     default: return FX_ERR_ARG;
     }
 }

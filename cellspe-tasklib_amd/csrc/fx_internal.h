@@ -229,7 +229,7 @@ int fxk_fft_r2c(hipStream_t s, const float2* tw, const float* in, float2* out, i
                 int keep_nyquist);
 int fxk_spectrometer(hipStream_t s, const float2* tw, const float* window, const uint32_t* words,
                      long hop, long spec_stride, int nchan, int n, int integrate_ffts, int nspec, float* partial,
-                     int groups, float* out);
+                     int groups, int nyq, float* out);
 int fxk_fstage(hipStream_t s, const float2* tw, const uint32_t* words, size_t in_stride_words,
                int ninp, int n, long t_first, long nt, const fx_phase_model* pm,
                const float2* mult, const float* scale_chan, const float* scale_inp, int out_type,

@@ -54,8 +54,13 @@ enum {
     FX_PREC_FP8_2BIT = 3 /* exact FP8 decomposition of 2-bit {+-1,+-H} voltages    */
 };
 
-/* spectrometer windows */
-enum { FX_WIN_NONE = 0, FX_WIN_SIN2 = 1, FX_WIN_COSINE = 2, FX_WIN_COSINE2 = 3 };
+/* spectrometer windows: SIN2 sin^2(pi i/(n-1)) (spectrometer_spu.c:297-315);
+ * COSINE, COSINE2, HAMMING, BLACKMAN (alpha 0.5) and HANN as the IPP core's
+ * generate_windowfunction (TaskCoreIPP.cpp:569-605) */
+enum {
+    FX_WIN_NONE = 0, FX_WIN_SIN2 = 1, FX_WIN_COSINE = 2, FX_WIN_COSINE2 = 3,
+    FX_WIN_HAMMING = 4, FX_WIN_BLACKMAN = 5, FX_WIN_HANN = 6
+};
 
 /* ---- context ------------------------------------------------------------- */
 typedef struct fx_ctx fx_ctx;
@@ -123,11 +128,12 @@ int fx_spectrometer(fx_ctx* ctx, const uint32_t* d_words, size_t ntime, int ncha
 /* Overlapped FFTs: hop = n/overlap samples between FFT starts.  continuous=1
  * runs one unbroken sequence (the SPU kernel's convention: spectrum s uses
  * FFTs s*K .. s*K+K-1); continuous=0 starts each spectrum after the last
- * sample of the previous one (the IPP core's convention).  *nspec_out (may be
- * NULL) receives the number of spectra written. */
+ * sample of the previous one (the IPP core's convention).  keep_nyquist=1
+ * writes n/2+1 bins per spectrum (fft_ssb_points, Settings.h:127), else n/2.
+ * *nspec_out (may be NULL) receives the number of spectra written. */
 int fx_spectrometer_ex(fx_ctx* ctx, const uint32_t* d_words, size_t ntime, int nchan, int n,
-                       int overlap, int integrate_ffts, int window, int continuous, float* d_out,
-                       long* nspec_out);
+                       int overlap, int integrate_ffts, int window, int continuous, int keep_nyquist,
+                       float* d_out, long* nspec_out);
 
 /* ---- a5: phase-cal ------------------------------------------------------- */
 typedef struct fx_pcal fx_pcal;

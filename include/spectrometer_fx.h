@@ -1,0 +1,119 @@
+/*
+ * spectrometer_fx.h -- the software spectrometer's compute seams.
+ *
+ * 1. SPU control block (cell_spectrometer.h:104-115): spectrometer_cb_run()
+ *    performs what one mailbox wake-up of spectrometer_spu.c:97-224 does --
+ *    unpack, sin^2 window, N-point FFT, power accumulation over integrate_ffts
+ *    overlapped FFTs, one spectrum written to spectrum_dst per integration --
+ *    and returns the three mailbox words (t_spu, #spectra, #FFTs).
+ *
+ * 2. TaskCore (TaskCore.h:36-89): fx_taskcore_* is the asynchronous
+ *    prepare/run/join/finalize/combineResults core a TaskDispatcher drives;
+ *    run() returns at once and the work proceeds on the core's own HIP
+ *    stream, join() blocks and returns the number of complete spectra.
+ *
+ * Conventions fixed here (deviations from the SPU code are its defects):
+ * raw input is 2-bit offset-binary samples in the VDIF layout
+ * (samples_per_rawbyte = 4; spectrometer_spu.c:329-343 decodes bytes as
+ * 8-bit whatever the setting), power is (1/K) sum |X|^2 with the 1/K applied
+ * after squaring (TaskCoreIPP.cpp:124,493; spectrometer_spu.c:137,269 squares
+ * the pre-scaled value), and the twiddles are complete (spectrometer_spu.c:
+ * 287-290 leaves W[0].im unset).
+ */
+#ifndef SPECTROMETER_FX_H
+#define SPECTROMETER_FX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "cellspe_fx.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#ifndef FX_ADDR64_DEFINED
+#define FX_ADDR64_DEFINED
+typedef union {
+    unsigned long long ull;
+    unsigned int ui[2];
+    void* p;
+} addr64;
+#endif
+
+/* cell_spectrometer.h:104-115 */
+typedef struct control_block_tt {
+    unsigned int rank __attribute__((aligned(128)));
+    unsigned int terminate_flag;      /* nonzero: spectrometer_cb_run returns 0 at once   */
+    unsigned int fft_points;          /* N, power of two, 256..32768                       */
+    unsigned int fft_points_log2;
+    unsigned int overlap_factor;      /* 2 for 50 %                                        */
+    unsigned int integrate_ffts;      /* FFTs summed per spectrum                          */
+    unsigned int samples_per_rawbyte; /* 4: 2-bit samples                                  */
+    addr64 rawif_src;                 /* host raw bytes                                    */
+    unsigned int rawif_bytecount;
+    addr64 spectrum_dst;              /* host: per spectrum N (re=power, im=0) float pairs */
+} control_block_t __attribute__((aligned(128)));
+
+/* the three outbound mailbox words (spectrometer_spu.c:217-220) */
+typedef struct {
+    unsigned int t_spu;          /* device time in CELLDSP_TIMEBASE ticks (10 ns) */
+    unsigned int num_spectra;
+    unsigned int total_ffts;
+} spectrometer_mbox_t;
+
+/* One control-block run.  spectrum_dst receives num_spectra blocks of
+ * fft_points (power, 0) float pairs: bins 0..N/2 computed (Nyquist included),
+ * bins N/2+1..N-1 the mirror |X(N-k)|^2 of a complex transform of real data.
+ * Returns 0, or -1 on invalid settings / device errors (fx_last_error()). */
+int spectrometer_cb_run(const control_block_t* cb, spectrometer_mbox_t* mbox);
+
+/* ---- TaskCore ------------------------------------------------------------ */
+/* the swspect_settings_t fields the compute core reads (Settings.h:59-159) */
+typedef struct {
+    int fft_points;
+    int fft_overlap_factor;
+    int core_overlapped_ffts;   /* FFTs per spectrum (averaged_overlapped_ffts)       */
+    int window;                 /* FX_WIN_*                                           */
+    int bits_per_sample;        /* 2                                                  */
+    int source_channels;        /* interleaved channels per raw word group: 1..16     */
+    int use_channel;            /* 1-based channel to process; 0 = all channels       */
+    int extract_pcal;           /* phase-cal per spectrum (single-channel sources)    */
+    double samplingfreq;        /* Hz */
+    double pcaloffsethz;
+    double pcalharmonicshz;
+} fx_spectrometer_settings;
+
+typedef struct fx_taskcore fx_taskcore;
+/* TaskCore::prepare.  Phase-cal parameters as swspectrometer.cpp:178-196:
+ * tonebins = fs/harmonics, rotatorlen = fs/gcd(fs, offset) (tonebins when the
+ * offset is 0); extraction is switched off (pcal length 0) when rotatorlen
+ * does not divide fft_points, and refused when tonebins does not divide
+ * rotatorlen (the IPP core would drop the remainder samples). */
+int fx_taskcore_create(int rank, const fx_spectrometer_settings* settings, fx_taskcore** core);
+/* TaskCore::run: processes rawbytes of raw data asynchronously.  Spectrum s
+ * integrates core_overlapped_ffts FFTs hopping fft_points/overlap samples and
+ * the next spectrum starts after its last sample (TaskCoreIPP.cpp:495-510).
+ * spectra_out receives per spectrum and output channel fft_points/2+1 floats
+ * (fft_ssb_points, mean power).  pcal_out (may be NULL) receives per spectrum
+ * tonebins complex values: the rotated, folded phase-cal pulse of the
+ * spectrum's non-overlapped FFT inputs (TaskCoreIPP.cpp:413-418,633-693),
+ * accumulated in FP64.  Buffers stay the caller's and must remain valid until
+ * join().  Returns 0 when the work was started. */
+int fx_taskcore_run(fx_taskcore* core, const void* raw, size_t rawbytes, float* spectra_out,
+                    size_t spectra_capacity_floats, fx_cf64* pcal_out, size_t pcal_capacity);
+/* TaskCore::join: -1 on failure, else the number of complete spectra */
+int fx_taskcore_join(fx_taskcore* core);
+/* TaskCore::combineResults (TaskCoreIPP.cpp:311-345): out[k] += this core's
+ * last spectra, pcal[k] += its last phase-cal (pcal may be NULL) */
+int fx_taskcore_combine(fx_taskcore* core, float* out, size_t nfloats, fx_cf64* pcal, size_t npcal);
+/* complex values per spectrum of phase-cal output (0 if not extracting) */
+int fx_taskcore_pcal_length(fx_taskcore* core);
+/* TaskCore::finalize */
+int fx_taskcore_finalize(fx_taskcore* core);
+int fx_taskcore_destroy(fx_taskcore* core);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

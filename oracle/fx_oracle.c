@@ -593,8 +593,21 @@ void fxo_window(int kind, int n, float* w)
         } else if (kind == FXO_WIN_COSINE2) {
             double c = cos(M_PI * ((double)i - 0.5 * (double)n + 0.5) / (double)n);
             v = c * c;
+        } else if (kind == FXO_WIN_HAMMING) {
+            v = 0.54 - 0.46 * cos(2.0 * M_PI * (double)i / (double)(n - 1));
+        } else if (kind == FXO_WIN_HANN) {
+            v = 0.5 - 0.5 * cos(2.0 * M_PI * (double)i / (double)(n - 1));
         }
         w[i] = (float)v;
+    }
+    if (kind == FXO_WIN_BLACKMAN) fxo_window_blackman(0.5, n, w);
+}
+
+void fxo_window_blackman(double alpha, int n, float* w)
+{
+    for (int i = 0; i < n; i++) {
+        double x = 2.0 * M_PI * (double)i / (double)(n - 1);
+        w[i] = (float)((alpha + 1.0) / 2.0 - 0.5 * cos(x) - 0.5 * alpha * cos(2.0 * x));
     }
 }
 
@@ -604,7 +617,7 @@ void fxo_window(int kind, int n, float* w)
 void fxo_spectrometer_f32(const uint32_t* words, size_t ntime, int nchan, int n,
                           int integrate_ffts, int window_kind, float* out, int nthreads)
 {
-    fxo_spectrometer_ex_f32(words, ntime, nchan, n, 1, integrate_ffts, window_kind, 0, out, nthreads);
+    fxo_spectrometer_ex_f32(words, ntime, nchan, n, 1, integrate_ffts, window_kind, 0, 0, out, nthreads);
 }
 
 /* Overlapped variant: FFT j of spectrum s starts at s*stride + j*hop,
@@ -612,10 +625,10 @@ void fxo_spectrometer_f32(const uint32_t* words, size_t ntime, int nchan, int n,
  * 171-210) or (K-1)*hop + n when spectra do not share samples
  * (TaskCoreIPP.cpp:495-510). */
 size_t fxo_spectrometer_ex_f32(const uint32_t* words, size_t ntime, int nchan, int n, int overlap,
-                               int integrate_ffts, int window_kind, int continuous, float* out,
-                               int nthreads)
+                               int integrate_ffts, int window_kind, int continuous, int keep_nyquist,
+                               float* out, int nthreads)
 {
-    int C = n / 2;
+    int C = n / 2 + (keep_nyquist ? 1 : 0);
     size_t hop = (size_t)(n / overlap);
     size_t stride, nspec;
     if (continuous) {
@@ -640,7 +653,7 @@ size_t fxo_spectrometer_ex_f32(const uint32_t* words, size_t ntime, int nchan, i
         int c = (int)(job % nchan);
         fxo_cf32* W = (fxo_cf32*)malloc(sizeof(fxo_cf32) * (size_t)(n / 4 + 1));
         float* xs = (float*)malloc(sizeof(float) * (size_t)n);
-        fxo_cf32* X = (fxo_cf32*)malloc(sizeof(fxo_cf32) * (size_t)C);
+        fxo_cf32* X = (fxo_cf32*)malloc(sizeof(fxo_cf32) * (size_t)(n / 2));
         fxo_cf32* work = (fxo_cf32*)malloc(sizeof(fxo_cf32) * (size_t)n);
         float* acc = (float*)calloc((size_t)C, sizeof(float));
         fxo_twiddle_r2(n, W);
@@ -653,8 +666,9 @@ size_t fxo_spectrometer_ex_f32(const uint32_t* words, size_t ntime, int nchan, i
                 xs[i] = fxo_lut2bit[(wd >> (2 * pos)) & 3u] * win[i];
             }
             fxo_rfft_r2(xs, n, W, X, work);
+            /* work holds the full n-point transform; bin n/2 is the Nyquist */
             for (int k = 0; k < C; k++)
-                acc[k] += X[k].re * X[k].re + X[k].im * X[k].im;
+                acc[k] += work[k].re * work[k].re + work[k].im * work[k].im;
         }
         float scale = 1.0f / (float)integrate_ffts;
         for (int k = 0; k < C; k++)
@@ -823,4 +837,24 @@ void fxo_pcal_reference(const fxo_pcal_cfg* cfg, const float* x, size_t len,
         tones[t].im = si / (double)nb;
     }
     free(acc);
+}
+
+/* TaskCoreIPP::extract_PCal (TaskCoreIPP.cpp:633-693) with the rotator of
+ * TaskCoreIPP.cpp:129-136, accumulated in FP64.  points must be a multiple of
+ * rotatorlen (swspectrometer.cpp:185-190 disables the feature otherwise). */
+void fxo_tcpcal_fold(const float* x, int points, int tonebins, int rotatorlen, double dphi,
+                     fxo_cf64* acc)
+{
+    int pulses = rotatorlen / tonebins;
+    for (int n = 0; n + rotatorlen <= points; n += rotatorlen) {
+        for (int p = 0; p < pulses; p++) {
+            for (int b = 0; b < tonebins; b++) {
+                int r = p * tonebins + b;
+                double arg = dphi * (double)r;
+                double v = (double)x[n + r];
+                acc[b].re += v * cos(arg);
+                acc[b].im += v * sin(arg);
+            }
+        }
+    }
 }

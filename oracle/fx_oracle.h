@@ -114,16 +114,24 @@ void fxo_fstage_f32(const uint32_t* const* words, int ninp, int n, long t0, long
 /* ------------------------------------------------------------------ */
 /* a8: spectrometer                                                      */
 /* ------------------------------------------------------------------ */
-enum { FXO_WIN_NONE = 0, FXO_WIN_SIN2 = 1, FXO_WIN_COSINE = 2, FXO_WIN_COSINE2 = 3 };
+enum {
+    FXO_WIN_NONE = 0, FXO_WIN_SIN2 = 1, FXO_WIN_COSINE = 2, FXO_WIN_COSINE2 = 3,
+    FXO_WIN_HAMMING = 4, FXO_WIN_BLACKMAN = 5, FXO_WIN_HANN = 6
+};
 void fxo_window(int kind, int n, float* w);
+/* generalized Blackman (alpha+1)/2 - cos(x)/2 - (alpha/2) cos(2x),
+ * x = 2 pi i/(n-1); the IPP core calls it with alpha = 0.5
+ * (TaskCoreIPP.cpp:591-594), alpha = -0.16 is the classical Blackman */
+void fxo_window_blackman(double alpha, int n, float* w);
 /* nchan interleaved 2-bit channels (VDIF layout), n-point real FFT per
  * channel, |X|^2 accumulated over integrate_ffts FFTs then scaled by
  * 1/integrate_ffts.  out[s][c][n/2] for s in [0, ntime/(n*integrate_ffts)). */
 void fxo_spectrometer_f32(const uint32_t* words, size_t ntime, int nchan, int n,
                           int integrate_ffts, int window_kind, float* out, int nthreads);
+/* keep_nyquist: rows of n/2+1 bins (fft_ssb_points) instead of n/2 */
 size_t fxo_spectrometer_ex_f32(const uint32_t* words, size_t ntime, int nchan, int n, int overlap,
-                               int integrate_ffts, int window_kind, int continuous, float* out,
-                               int nthreads);
+                               int integrate_ffts, int window_kind, int continuous, int keep_nyquist,
+                               float* out, int nthreads);
 void fxo_spectrometer_f64(const uint32_t* words, size_t ntime, int nchan, int n,
                           int integrate_ffts, int window_kind, double* out);
 
@@ -147,6 +155,15 @@ void fxo_pcal_extract(const fxo_pcal_cfg* cfg, const float* x, size_t len,
 void fxo_pcal_final(const fxo_pcal_cfg* cfg, const fxo_cf64* acc, fxo_cf64* tones);
 void fxo_pcal_reference(const fxo_pcal_cfg* cfg, const float* x, size_t len,
                         size_t sampleoffset, fxo_cf64* tones);
+
+/* The IPP spectrometer core's own phase-cal (TaskCoreIPP.cpp:128-136,
+ * 413-418, 633-693; parameters swspectrometer.cpp:178-196): each FFT input
+ * of `points` samples is multiplied by the rotator e^{i dphi r}, r = n mod
+ * rotatorlen, dphi = -2 pi offset/fs, and folded into tonebins bins:
+ * acc[r mod tonebins] += x[n] e^{i dphi r} for r < pulses*tonebins,
+ * pulses = rotatorlen / tonebins.  FP64 throughout. */
+void fxo_tcpcal_fold(const float* x, int points, int tonebins, int rotatorlen, double dphi,
+                     fxo_cf64* acc);
 
 #ifdef __cplusplus
 }
