@@ -18,7 +18,8 @@ LIB_PATH = os.path.join(HERE, "libcellspe_fx.so")
 
 FX_OK = 0
 FX_PREC_F32, FX_PREC_F16, FX_PREC_FP8, FX_PREC_FP8_2BIT = 0, 1, 2, 3
-FX_WIN_NONE, FX_WIN_SIN2, FX_WIN_COSINE, FX_WIN_COSINE2 = 0, 1, 2, 3
+(FX_WIN_NONE, FX_WIN_SIN2, FX_WIN_COSINE, FX_WIN_COSINE2, FX_WIN_HAMMING, FX_WIN_BLACKMAN,
+ FX_WIN_HANN) = range(7)
 
 
 class FxError(RuntimeError):
@@ -78,6 +79,7 @@ def load():
         "fx_sincos": (i, [vp, vp, vp, l, i]),
         "fx_sincos_generator": (i, [vp, d, d, vp, l]),
         "fx_spectrometer": (i, [vp, vp, sz, i, i, i, i, vp]),
+        "fx_spectrometer_ex": (i, [vp, vp, sz, i, i, i, i, i, i, i, vp, C.POINTER(l)]),
         "fx_pcal_create": (i, [vp, d, d, i, sz, C.POINTER(vp)]),
         "fx_pcal_destroy": (i, [vp]),
         "fx_pcal_clear": (i, [vp]),
@@ -221,6 +223,15 @@ class Context:
     def spectrometer(self, d_words, ntime, nchan, n, integrate_ffts, window, d_out):
         check(self.lib.fx_spectrometer(self.handle, d_words.ptr, ntime, nchan, n,
                                        integrate_ffts, window, d_out.ptr))
+
+    def spectrometer_ex(self, d_words, ntime, nchan, n, overlap, integrate_ffts, window,
+                        continuous, keep_nyquist, d_out):
+        ns = C.c_long()
+        check(self.lib.fx_spectrometer_ex(self.handle, d_words.ptr, ntime, nchan, n, overlap,
+                                          integrate_ffts, window, int(continuous),
+                                          int(keep_nyquist), d_out.ptr if d_out else None,
+                                          C.byref(ns)))
+        return ns.value
 
     def complex_mac(self, a, b, acc, n, conj):
         check(self.lib.fx_complex_mac(self.handle, a.ptr, b.ptr, acc.ptr, n, int(conj)))

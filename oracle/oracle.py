@@ -16,7 +16,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboracle_fx.so")
 
 LUT = np.array([-3.3359, -1.0, 1.0, 3.3359], dtype=np.float32)
-WIN_NONE, WIN_SIN2, WIN_COSINE, WIN_COSINE2 = 0, 1, 2, 3
+WIN_NONE, WIN_SIN2, WIN_COSINE, WIN_COSINE2, WIN_HAMMING, WIN_BLACKMAN, WIN_HANN = range(7)
 PCAL_TRIVIAL, PCAL_IMPLICIT, PCAL_SHIFTING = 0, 1, 2
 
 
@@ -78,6 +78,9 @@ def load():
         "fxo_window": (None, [i, i, vp]),
         "fxo_spectrometer_f32": (None, [vp, sz, i, i, i, i, vp, i]),
         "fxo_spectrometer_f64": (None, [vp, sz, i, i, i, i, vp]),
+        "fxo_spectrometer_ex_f32": (sz, [vp, sz, i, i, i, i, i, i, i, vp, i]),
+        "fxo_window_blackman": (None, [d, i, vp]),
+        "fxo_tcpcal_fold": (None, [vp, i, i, i, d, vp]),
         "fxo_gcd": (C.c_longlong, [l, l]),
         "fxo_pcal_config": (None, [d, d, i, C.POINTER(PCalCfg)]),
         "fxo_pcal_extract": (None, [C.POINTER(PCalCfg), vp, sz, sz, vp]),
@@ -284,6 +287,35 @@ def spectrometer_f32(words, ntime, nchan, n, integrate_ffts, window_kind, nthrea
     load().fxo_spectrometer_f32(_p(np.ascontiguousarray(words, np.uint32)), ntime, nchan, n,
                                 integrate_ffts, window_kind, _p(out), nthreads)
     return out
+
+
+def spectrometer_ex_f32(words, ntime, nchan, n, overlap, integrate_ffts, window_kind, continuous,
+                        keep_nyquist, nthreads=1):
+    """Overlapped spectrometer; returns out[s][c][n/2 (+1)]."""
+    w = np.ascontiguousarray(words, np.uint32)
+    lib = load()
+    nspec = lib.fxo_spectrometer_ex_f32(_p(w), ntime, nchan, n, overlap, integrate_ffts, window_kind,
+                                        int(continuous), int(keep_nyquist), None, nthreads)
+    out = np.zeros((nspec, nchan, n // 2 + int(bool(keep_nyquist))), dtype=np.float32)
+    if nspec:
+        lib.fxo_spectrometer_ex_f32(_p(w), ntime, nchan, n, overlap, integrate_ffts, window_kind,
+                                    int(continuous), int(keep_nyquist), _p(out), nthreads)
+    return out
+
+
+def window_blackman(alpha, n):
+    w = np.zeros(n, dtype=np.float32)
+    load().fxo_window_blackman(float(alpha), n, _p(w))
+    return w
+
+
+def tcpcal_fold(x, tonebins, rotatorlen, dphi, acc=None):
+    """IPP-core phase-cal of one FFT input x (len multiple of rotatorlen)."""
+    x = np.ascontiguousarray(x, np.float32)
+    if acc is None:
+        acc = np.zeros(tonebins, dtype=np.complex128)
+    load().fxo_tcpcal_fold(_p(x), x.size, tonebins, rotatorlen, float(dphi), _p(acc))
+    return acc
 
 
 def spectrometer_f64(words, ntime, nchan, n, integrate_ffts, window_kind):

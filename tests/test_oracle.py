@@ -1,0 +1,381 @@
+"""Pins of the CPU oracle (no GPU): the reference's own known-answer tests,
+hand-derived vectors and the mathematics each oracle function must obey.
+
+Every check is chosen so that a plausible slip in the restatement (a dropped
+term, a wrong sign, a transposed index) fails it; none retypes the oracle's
+formula.  Sources of the known answers:
+
+* unpack: the VDIF 4-level table and bit order (format_vdif.c:66-96, byte b
+  sample i = bits 2i+1:2i) decoded through an independently built byte table;
+* FFT: brute-force FP64 DFT, numpy.fft, a single-tone spectrum, Parseval,
+  linearity;
+* MAC: minitest-cell.cpp:131-161 (acc += a*b with a=(0.5i,-0.5i),
+  b=(0.15i^2,-0.25), acc0=(1,1.5)) and the conjugate closed form;
+* rotation: rotate(tau) then rotate(-tau) is the identity; a delayed copy is
+  re-aligned (cross phase flat across channels);
+* X-engine: numpy products, Hermitian symmetry, real non-negative autos and
+  the 2-bit Van Vleck ratio;
+* phase-cal: PCal.cpp:879-951,969-980 (tones at -90 + 5 t degrees within 0.1
+  degree, with the adjustSampleOffset skip path) for all three extractors;
+* spectrometer: swspectrometer.m:7-48 (carrier 1/37 of the sample rate peaks
+  at bin round(N/37)), mean-power normalisation and the window closed forms.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle as O
+
+H = 3.3359
+
+
+def nrel(a, b):
+    a = np.asarray(a, np.complex128).ravel()
+    b = np.asarray(b, np.complex128).ravel()
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+# ---------------------------------------------------------------- a1 unpack
+def test_unpack_hand_vectors():
+    # 0xE4 = 11 10 01 00: samples (bits 1:0, 3:2, 5:4, 7:6) are codes 0,1,2,3
+    got = O.unpack_1ch(np.array([0xE4E4E4E4], np.uint32), 16)
+    want = np.tile(np.array([-H, -1, 1, H], np.float32), 4)
+    assert np.array_equal(got, want)
+    got = O.unpack_1ch(np.array([0x1B1B1B1B, 0x00000000, 0xFFFFFFFF], np.uint32), 48)
+    want = np.concatenate([np.tile(np.array([H, 1, -1, -H], np.float32), 4),
+                           np.full(16, -H, np.float32), np.full(16, H, np.float32)])
+    assert np.array_equal(got, want)
+    # the constants themselves are the float32 of the table values
+    assert np.array_equal(O.LUT, np.array([-3.3359, -1.0, 1.0, 3.3359], np.float32))
+
+
+def _byte_table():
+    """lut2bit[b][i] = level[(b >> 2i) & 3] built byte-wise as mark5access
+    initialises it (format_vdif.c:83-89)."""
+    level = np.array([-H, -1.0, 1.0, H], np.float32)
+    return np.array([[level[(b >> (2 * i)) & 3] for i in range(4)] for b in range(256)], np.float32)
+
+
+def test_unpack_matches_byte_table():
+    rng = np.random.default_rng(5)
+    words = rng.integers(0, 2**32, 257, dtype=np.uint64).astype(np.uint32)
+    bytes_ = words.view(np.uint8)          # little-endian byte order of the words
+    want = _byte_table()[bytes_].ravel()
+    got = O.unpack_1ch(words, words.size * 16)
+    assert np.array_equal(got, want)
+
+
+@pytest.mark.parametrize("nchan", [2, 4, 8, 16])
+def test_unpack_multichannel_layout(nchan):
+    # channel c of time sample t sits at field t*nchan + c of the word stream
+    spw = 16 // nchan
+    ntime = spw * 37
+    codes = np.zeros(ntime * nchan, np.uint8)
+    rng = np.random.default_rng(nchan)
+    codes[:] = rng.integers(0, 4, codes.size)
+    words = O.pack2b(codes)
+    got = O.unpack_nch(words, ntime, nchan)
+    want = O.LUT[codes.reshape(ntime, nchan).T]
+    assert np.array_equal(got, want)
+    # the 16-channel mark5b-style word: channel c carries code c % 4 everywhere
+    if nchan == 16:
+        w = sum((c % 4) << (2 * c) for c in range(16))
+        got = O.unpack_nch(np.array([w] * 3, np.uint32), 3, 16)
+        assert np.array_equal(got[:, 0], O.LUT[np.arange(16) % 4])
+
+
+def test_pack_roundtrip_and_quantizer():
+    codes = np.arange(64, dtype=np.uint8) % 4
+    assert np.array_equal(O.unpack_1ch(O.pack2b(codes), 64), O.LUT[codes])
+    q = O.load().fxo_quantize2
+    assert [q(x, 0.9816) for x in (-2.0, -0.5, 0.0, 0.5, 0.9816, 3.0)] == [0, 1, 2, 2, 3, 3]
+
+
+def test_generator_statistics():
+    # 2-bit codes at thresholds +-0.9816 sigma: P = {0.163, 0.337, 0.337, 0.163}
+    w = O.gen_station(7, 0, 0.0, 0, 0.0, 0.0, 0.9816, 1 << 18)
+    x = O.unpack_1ch(w, 1 << 18)
+    p = np.array([np.mean(x == v) for v in O.LUT])
+    assert np.allclose(p, [0.1633, 0.3367, 0.3367, 0.1633], atol=0.005)
+
+
+# ---------------------------------------------------------------- a2/a3 FFT
+@pytest.mark.parametrize("n", [4, 8, 16, 64, 256, 1024])
+def test_twiddles_closed_form(n):
+    W = O.twiddles(n)
+    i = np.arange(n // 4)
+    want = np.cos(2 * np.pi * i / n) - 1j * np.sin(2 * np.pi * i / n)
+    assert np.max(np.abs(W - want)) < 2e-7
+
+
+@pytest.mark.parametrize("n", [2, 4, 8, 32, 128, 512, 2048])
+def test_fft_vs_bruteforce_dft(n):
+    rng = np.random.default_rng(n)
+    x = (rng.standard_normal(n) + 1j * rng.standard_normal(n)).astype(np.complex64)
+    ref = O.dft_f64(x)
+    # the brute-force DFT itself against its definition at a few bins
+    k = np.arange(n)
+    for kk in (0, 1, n // 2, n - 1):
+        assert abs(ref[kk] - np.sum(x.astype(np.complex128) * np.exp(-2j * np.pi * kk * k / n))) < 1e-9 * n
+    assert nrel(O.fft_f64(x), ref) < 1e-13
+    assert nrel(O.fft_r2(x), ref) < 2e-6 * math.log2(max(n, 2))
+    assert nrel(np.fft.fft(x.astype(np.complex128)), ref) < 1e-13
+
+
+@pytest.mark.parametrize("n", [1024, 8192, 32768])
+def test_rfft_vs_numpy(n):
+    rng = np.random.default_rng(n + 1)
+    x = O.LUT[rng.integers(0, 4, n)]
+    got = O.rfft_r2(x)
+    ref = np.fft.rfft(x.astype(np.float64))[: n // 2]
+    assert nrel(got, ref) < 1e-6
+
+
+def test_fft_single_tone_and_parseval_linearity():
+    n = 1024
+    t = np.arange(n)
+    x = np.cos(2 * np.pi * 37 * t / n).astype(np.float32)
+    X = O.rfft_r2(x)
+    assert np.argmax(np.abs(X)) == 37
+    assert abs(abs(X[37]) - n / 2) < 1e-3
+    mask = np.ones(n // 2, bool)
+    mask[37] = False
+    assert np.max(np.abs(X[mask])) < 1e-2
+    rng = np.random.default_rng(3)
+    a = (rng.standard_normal(n) + 1j * rng.standard_normal(n)).astype(np.complex64)
+    b = (rng.standard_normal(n) + 1j * rng.standard_normal(n)).astype(np.complex64)
+    A, B = O.fft_r2(a), O.fft_r2(b)
+    assert nrel(O.fft_r2((2 * a + 3j * b).astype(np.complex64)), 2 * A + 3j * B) < 1e-6
+    e_t = np.sum(np.abs(a.astype(np.complex128)) ** 2)
+    e_f = np.sum(np.abs(A.astype(np.complex128)) ** 2) / n
+    assert abs(e_t - e_f) / e_t < 1e-6
+
+
+# ---------------------------------------------------------------- MAC KATs
+def test_mac_known_answer():
+    n = 64
+    i = np.arange(n, dtype=np.float64)
+    a = (0.5 * i - 0.5j * i).astype(np.complex64)
+    b = (0.15 * i * i - 0.25j).astype(np.complex64)
+    acc0 = np.full(n, 1.0 + 1.5j, np.complex64)
+    got = O.addproduct(a, b, acc0, conj=False)
+    want = (1 + 0.075 * i**3 - 0.125 * i) + 1j * (1.5 - 0.075 * i**3 - 0.125 * i)
+    assert np.max(np.abs(got - want) / np.maximum(np.abs(want), 1)) < 1e-6
+    # conjugate variant: acc += a * conj(b)
+    gotc = O.addproduct(a, b, acc0, conj=True)
+    wantc = (1 + 0.075 * i**3 + 0.125 * i) + 1j * (1.5 - 0.075 * i**3 + 0.125 * i)
+    assert np.max(np.abs(gotc - wantc) / np.maximum(np.abs(wantc), 1)) < 1e-6
+    # the GENERIC header's own four-term example, (1+2i)(3+4i)... as probed
+    assert np.allclose(O.mul(np.array([1 + 2j], np.complex64), np.array([3 + 4j], np.complex64)), [-5 + 10j])
+
+
+# ---------------------------------------------------------------- a4 rotation
+def test_phase_reduction_and_inverse():
+    m = (12.375, 1e-3, 0.3, 0.01)
+    for k in (0, 1, 511, 4095):
+        ph = O.phase(m, k, 8192, 17)
+        assert -math.pi <= ph <= math.pi
+        tau = 12.375 + 1e-3 * 17
+        cyc = k * tau / 8192 + (0.3 + 0.01 * 17) / (2 * math.pi)
+        assert abs(math.remainder(ph / (2 * math.pi) - cyc, 1.0)) < 1e-12
+    rng = np.random.default_rng(4)
+    X = (rng.standard_normal(4096) + 1j * rng.standard_normal(4096)).astype(np.complex64)
+    Y = O.rotate(O.rotate(X, 8192, m, 17), 8192, tuple(-v for v in m), 17)
+    assert nrel(Y, X) < 1e-6
+
+
+def test_delay_compensation_flattens_cross_phase():
+    n, ntime = 1024, 1024 * 256
+    d = 5
+    s0 = O.gen_station(21, 0, 0.6, 0, 0.0, 0.0, 0.9816, ntime)
+    s1 = O.gen_station(21, 1, 0.6, d, 0.0, 0.0, 0.9816, ntime)
+    k = np.arange(1, n // 2)
+    raw = O.corr_f32([s0, s1], n, ntime // n)[1][1:]
+    slope = np.polyfit(k, np.unwrap(np.angle(raw)), 1)[0]
+    # station 1 lags by d samples: cross phase slope +2 pi d / n per channel
+    assert abs(slope - 2 * np.pi * d / n) < 0.05 * 2 * np.pi * d / n
+    models = [(0.0, 0.0, 0.0, 0.0), (float(d), 0.0, 0.0, 0.0)]
+    fixed = O.corr_f32([s0, s1], n, ntime // n, models)[1][1:]
+    assert abs(np.polyfit(k, np.unwrap(np.angle(fixed)), 1)[0]) < 2e-4
+    assert abs(np.angle(np.sum(fixed))) < 0.05
+
+
+# ---------------------------------------------------------------- a6 X-engine
+def test_xengine_products_and_symmetry():
+    rng = np.random.default_rng(9)
+    Cn, T, I = 8, 32, 5
+    X = (rng.standard_normal((Cn, T, I)) + 1j * rng.standard_normal((Cn, T, I))).astype(np.complex64)
+    vis = O.xengine_f64(X)
+    p = 0
+    for i in range(I):
+        for j in range(i, I):
+            assert O.prod_index(i, j, I) == p
+            want = np.sum(X[:, :, i].astype(np.complex128) * np.conj(X[:, :, j]), axis=1)
+            assert nrel(vis[p], want) < 1e-12
+            if i == j:
+                assert np.all(np.abs(vis[p].imag) == 0) and np.all(vis[p].real >= 0)
+            p += 1
+    assert nrel(O.xengine_f32(X), vis) < 1e-6
+    # swapping the two inputs conjugates the product
+    Xs = X[:, :, ::-1].copy()
+    v2 = O.xengine_f64(Xs)
+    assert nrel(v2[O.prod_index(0, I - 1, I)], np.conj(vis[O.prod_index(0, I - 1, I)])) < 1e-12
+
+
+def test_van_vleck_two_bit():
+    # 2-bit quantized correlation is ~0.88 rho at the optimal thresholds;
+    # two stations sharing a common part of weight a correlate at rho = a^2
+    n, nt = 256, 8192
+    rho = 0.1
+    s = [O.gen_station(33, k, math.sqrt(rho), 0, 0.0, 0.0, 0.9816, n * nt) for k in range(2)]
+    v = O.corr_f64(s, n, nt, nthreads=4)
+    r = np.sum(v[1].real) / math.sqrt(np.sum(v[0].real) * np.sum(v[2].real))
+    assert abs(r / rho - 0.88) < 0.03
+
+
+def test_corr_f32_matches_f64():
+    n, nt = 512, 64
+    s = [O.gen_station(2, k, 0.05, k, 0.0, 0.0, 0.9816, n * nt) for k in range(3)]
+    m = [(0.5 * k, 0.0, 0.1 * k, 0.0) for k in range(3)]
+    assert nrel(O.corr_f32(s, n, nt, m, nthreads=2), O.corr_f64(s, n, nt, m)) < 1e-5
+
+
+# ---------------------------------------------------------------- a5 phase-cal
+def _pcal_kat(bw, spacing, offset, sampleoffset, nsamp, skip=True):
+    cfg = O.pcal_config(bw, spacing, offset)
+    nt = cfg.ntones
+    n = np.arange(nsamp, dtype=np.float64)
+    data = np.zeros(nsamp)
+    for t in range(nt):
+        data += np.sin(np.pi * (n + sampleoffset) * (offset + t * spacing) / bw + t * np.pi * 5 / 180)
+    data = data.astype(np.float32)
+    acc = O.pcal_extract(cfg, data, sampleoffset)
+    if skip:
+        # PCal.cpp:915-920: skip 3 samples, continue at the matching offset
+        acc = O.pcal_extract(cfg, data[3:], sampleoffset + nsamp + 3, acc)
+    tones = O.pcal_final(cfg, acc)
+    phi = np.degrees(np.angle(tones))
+    want = -90.0 + 5.0 * np.arange(nt)
+    return cfg, tones, np.abs(((phi - want) + 180) % 360 - 180)
+
+
+@pytest.mark.parametrize("bw,spacing,offset,kind", [
+    (16e6, 1e6, 510e3, O.PCAL_IMPLICIT),      # PCal.cpp:868 example invocation
+    (8e6, 1e6, 0, O.PCAL_TRIVIAL),            # PhaseCal/test.cpp:34
+    (8e6, 1e6, 10e3, O.PCAL_IMPLICIT),        # PhaseCal/test.cpp:36 (1600-sample period)
+    (8e6, 1e6, 400e3, O.PCAL_SHIFTING),       # period 40 not a multiple of 16 bins
+])
+def test_pcal_reference_kat(bw, spacing, offset, kind):
+    cfg, tones, err = _pcal_kat(bw, spacing, offset, 0, 32000)
+    assert cfg.kind == kind
+    if offset == 0:
+        # tone 0 is DC, where the KAT's sin(0) carries no signal (only the
+        # leakage of the skipped partial periods)
+        assert abs(tones[0]) < 1e-3 * np.max(np.abs(tones))
+        err = err[1:]
+    assert np.all(err < 0.1), err
+
+
+def test_pcal_config_factory():
+    c = O.pcal_config(16e6, 1e6, 510e3)
+    assert (c.kind, c.nbins, c.ntones) == (O.PCAL_IMPLICIT, 3200, 16)
+    c = O.pcal_config(8e6, 1e6, 10e3)
+    assert (c.kind, c.nbins) == (O.PCAL_IMPLICIT, 1600)
+    c = O.pcal_config(8e6, 1e6, 0)
+    assert (c.kind, c.nbins, c.ntones) == (O.PCAL_TRIVIAL, 16, 8)
+    c = O.pcal_config(8e6, 1e6, 400e3)
+    assert (c.kind, c.nbins, c.rotatorlen) == (O.PCAL_SHIFTING, 16, 40)
+
+
+def test_pcal_fast_equals_direct_method():
+    # extractAndIntegrate vs the direct (quasi-reference) method of
+    # PCal.cpp:139-175 on noise plus tones
+    rng = np.random.default_rng(8)
+    for bw, sp, off in ((8e6, 1e6, 10e3), (8e6, 1e6, 400e3), (8e6, 1e6, 0)):
+        cfg = O.pcal_config(bw, sp, off)
+        x = O.LUT[rng.integers(0, 4, 16000)]
+        fast = O.pcal_final(cfg, O.pcal_extract(cfg, x, 7))
+        direct = O.pcal_reference(cfg, x, 7)
+        # each method scales by its own DFT length: nbins vs 2*ntones
+        assert nrel(fast * (cfg.nbins / (2 * cfg.ntones)), direct) < 1e-9
+
+
+def test_taskcore_pcal_fold_tones():
+    # the IPP core's folded pulse: its tonebins-point DFT gives the tone
+    # phasors (tone t at bin t for a zero-offset-aligned rotator)
+    fs, spacing, offset = 16e6, 1e6, 500e3
+    T = int(fs / spacing)
+    R = int(fs / math.gcd(int(fs), int(offset)))
+    assert (T, R) == (16, 32)
+    npts = 1024
+    n = np.arange(npts)
+    x = np.zeros(npts)
+    for t in range(T // 2):
+        x += np.sin(2 * np.pi * n * (offset + t * spacing) / fs + t * np.pi * 5 / 180)
+    acc = O.tcpcal_fold(x.astype(np.float32), T, R, -2 * np.pi * offset / fs)
+    spec = np.fft.fft(acc) / (npts)
+    phi = np.degrees(np.angle(spec[: T // 2]))
+    want = -90.0 + 5.0 * np.arange(T // 2)
+    assert np.all(np.abs(((phi - want) + 180) % 360 - 180) < 0.1)
+
+
+# ---------------------------------------------------------------- a8 spectrometer
+def test_window_closed_forms():
+    n = 256
+    i = np.arange(n)
+    assert np.max(np.abs(O.window(O.WIN_HANN, n) - np.hanning(n))) < 1e-7
+    assert np.max(np.abs(O.window(O.WIN_HAMMING, n) - np.hamming(n))) < 1e-7
+    assert np.max(np.abs(O.window_blackman(-0.16, n) - np.blackman(n))) < 1e-7
+    assert np.max(np.abs(O.window(O.WIN_NONE, n) - 1)) == 0
+    # SIN2 is symmetric, zero at both ends, one in the middle of an odd grid
+    w = O.window(O.WIN_SIN2, 257)
+    assert w[0] < 1e-12 and w[-1] < 1e-12 and abs(w[128] - 1) < 1e-7
+    # the cosine window peaks between the two middle samples
+    wc = O.window(O.WIN_COSINE, n)
+    assert np.allclose(wc, wc[::-1], atol=1e-7) and abs(wc[0] - math.cos(math.pi * (0.5 - n / 2) / n)) < 1e-7
+    assert np.allclose(O.window(O.WIN_COSINE2, n), wc.astype(np.float64) ** 2, atol=1e-6)
+
+
+def test_spectrometer_carrier_kat():
+    # swspectrometer.m: carrier 1/37 in noise, 1024-point cos-windowed FFTs
+    # with 50 % overlap; the power peak sits at round(1024/37) = 28
+    n = 1024
+    ntime = 1 << 18
+    w = O.gen_multichannel(37, 1, [0.1], [1 / 37], 0.9816, ntime)
+    out = O.spectrometer_ex_f32(w, ntime, 1, n, 2, 511, O.WIN_COSINE, True, True, nthreads=4)
+    assert out.shape == (1, 1, n // 2 + 1)
+    assert int(np.argmax(out[0, 0, 1:])) + 1 == 28
+
+
+def test_spectrometer_mean_power_and_nyquist():
+    n, K = 256, 8
+    ntime = n * K * 2
+    rng = np.random.default_rng(12)
+    codes = rng.integers(0, 4, ntime).astype(np.uint8)
+    w = O.pack2b(codes)
+    out = O.spectrometer_ex_f32(w, ntime, 1, n, 1, K, O.WIN_NONE, False, True)
+    x = O.LUT[codes].astype(np.float64).reshape(2, K, n)
+    P = np.mean(np.abs(np.fft.rfft(x, axis=2)) ** 2, axis=1)
+    assert nrel(out[:, 0, :], P) < 1e-6
+    base = O.spectrometer_f32(w, ntime, 1, n, K, O.WIN_NONE)
+    assert np.array_equal(base[:, 0, :], out[:, 0, : n // 2])
+
+
+def test_spectrometer_overlap_conventions():
+    n, K, ov = 256, 5, 2
+    ntime = 4096
+    rng = np.random.default_rng(13)
+    codes = rng.integers(0, 4, ntime).astype(np.uint8)
+    w = O.pack2b(codes)
+    x = O.LUT[codes].astype(np.float64)
+    hop = n // ov
+    for continuous in (True, False):
+        out = O.spectrometer_ex_f32(w, ntime, 1, n, ov, K, O.WIN_NONE, continuous, False)
+        stride = K * hop if continuous else (K - 1) * hop + n
+        nspec = ((ntime - n) // hop + 1) // K if continuous else ntime // stride
+        assert out.shape[0] == nspec
+        for s in range(nspec):
+            segs = np.stack([x[s * stride + j * hop: s * stride + j * hop + n] for j in range(K)])
+            P = np.mean(np.abs(np.fft.rfft(segs, axis=1)[:, : n // 2]) ** 2, axis=0)
+            assert nrel(out[s, 0], P) < 1e-6
