@@ -84,6 +84,7 @@ class CellDSPTask(C.Structure):
         ("outputvector_lengths", C.c_uint * CELLDSP_MAX_NUM_IOVECTORS),
         ("subcontext", Subcontext),
         ("next_task", Addr64),
+        ("_align128", C.c_ubyte * 8),       # C pads the aligned(128) struct to 256 bytes
     ]
 
 
@@ -101,6 +102,7 @@ class MiniControlBlock(C.Structure):
         ("ffts_to_integrate", C.c_uint),
         ("spe_ls_listOnPPU", Addr64),
         ("spe_sig_listOnPPU", Addr64),
+        ("_align128", C.c_ubyte * 72),      # aligned(128): array stride 256 bytes
     ]
 
 
@@ -121,6 +123,7 @@ class SpecControlBlock(C.Structure):
         ("rawif_src", Addr64),
         ("rawif_bytecount", C.c_uint),
         ("spectrum_dst", Addr64),
+        ("_align128", C.c_ubyte * 72),      # aligned(128): 128 bytes
     ]
 
 

@@ -124,9 +124,11 @@ def test_ctypes_layouts_match_c(tmp_path):
             t, f = key.split(".")
             if t in py:
                 assert getattr(py[t], f).offset == val, key
-    # C pads the 128-byte aligned structs to a multiple of 128
-    assert c["CellDSPTask_t"] % 128 == 0 and C.sizeof(TL.CellDSPTask) <= c["CellDSPTask_t"]
-    assert c["control_block"] % 128 == 0 and C.sizeof(TL.MiniControlBlock) <= c["control_block"]
+    # C pads the 128-byte aligned structs to a multiple of 128; the mirrors
+    # carry the same padding so arrays of them have the C stride
+    assert c["CellDSPTask_t"] % 128 == 0 and C.sizeof(TL.CellDSPTask) == c["CellDSPTask_t"]
+    assert c["control_block"] % 128 == 0 and C.sizeof(TL.MiniControlBlock) == c["control_block"]
+    assert C.sizeof(TL.SpecControlBlock) == c["control_block_t"]
     assert C.sizeof(TL.SpectrometerSettings) == c["fx_spectrometer_settings"]
     assert C.sizeof(TL.SpecMbox) == c["spectrometer_mbox_t"]
     assert C.sizeof(TL.MiniStats) == c["minicorr_stats"]
