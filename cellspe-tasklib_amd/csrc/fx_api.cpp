@@ -261,11 +261,24 @@ extern "C" int fx_synth_multichannel(fx_ctx* ctx, uint64_t seed, int nchan, cons
 }
 
 extern "C" int fx_synth_channelized(fx_ctx* ctx, uint64_t seed, int nchan, long ntime, int ninp,
-                                    double rho, double thresh, uint8_t* d_codes)
+                                    double rho, double thresh, int input_major, uint8_t* d_codes)
 {
     int rc = ctx_device(ctx);
     if (rc) return rc;
-    FX_CHECK_LAUNCH(fxk_synth_channelized(ctx->stream, seed, nchan, ntime, ninp, rho, thresh, d_codes));
+    FX_CHECK_LAUNCH(fxk_synth_channelized(ctx->stream, seed, nchan, ntime, ninp, rho, thresh,
+                                          input_major ? 1 : 0, d_codes));
+    return FX_OK;
+}
+
+extern "C" int fx_codes_to_chan(fx_ctx* ctx, const uint8_t* d_codes, int input_major, int nchan, long ntime,
+                                int ninp, int out_prec, void* d_out)
+{
+    int rc = ctx_device(ctx);
+    if (rc) return rc;
+    FX_REQUIRE(out_prec == FX_PREC_F32 || out_prec == FX_PREC_F16, FX_ERR_ARG,
+               "fx_codes_to_chan: output precision %d (FP32 or FP16)", out_prec);
+    FX_CHECK_LAUNCH(fxk_codes_to_chan(ctx->stream, d_codes, input_major ? 1 : 0, nchan, ntime, ninp, out_prec,
+                                      d_out));
     return FX_OK;
 }
 
@@ -637,8 +650,18 @@ struct fx_corr {
     float* sc_chan = nullptr;
     float* sc_inp = nullptr;
     int2* pij = nullptr;
+    float* coef = nullptr;  // FP8_2BIT: weights of the three accumulator sets
+    int nsets = 1;
     long spectra_done = 0;
 };
+
+// engine dispatch: FP32/FP16 read [c][t][i], the 8-bit engines [c][i][t]
+static int corr_engine(fx_corr* c, const void* d_chan, long nt)
+{
+    if (c->cfg.prec == FX_PREC_F32 || c->cfg.prec == FX_PREC_F16)
+        return fxk_xengine(c->ctx->stream, c->cfg.prec, d_chan, c->nchan, nt, c->cfg.ninp, c->acc);
+    return fxk_xengine8(c->ctx->stream, c->cfg.prec, d_chan, c->nchan, nt, c->cfg.ninp, c->acc);
+}
 
 static size_t prec_elem(int prec)
 {
@@ -659,35 +682,49 @@ extern "C" int fx_corr_create(fx_ctx* ctx, const fx_corr_config* cfg, fx_corr** 
     FX_REQUIRE(cfg->ninp >= 1 && cfg->ninp <= 4096, FX_ERR_ARG, "fx_corr_create: ninp=%d", cfg->ninp);
     FX_REQUIRE(is_pow2(cfg->nfft) && cfg->nfft >= 256 && cfg->nfft <= 16384, FX_ERR_ARG,
                "fx_corr_create: nfft=%d not a power of two in [256,16384]", cfg->nfft);
-    FX_REQUIRE(cfg->prec == FX_PREC_F32 || cfg->prec == FX_PREC_F16, FX_ERR_UNSUPPORTED,
-               "fx_corr_create: precision %d not built", cfg->prec);
-    FX_REQUIRE(cfg->chunk_spectra >= 8 && cfg->chunk_spectra % 8 == 0, FX_ERR_ARG,
-               "fx_corr_create: chunk_spectra=%ld must be a positive multiple of 8",
-               cfg->chunk_spectra);
+    FX_REQUIRE(cfg->prec >= FX_PREC_F32 && cfg->prec <= FX_PREC_FP8_2BIT, FX_ERR_UNSUPPORTED,
+               "fx_corr_create: precision %d", cfg->prec);
+    const bool twobit = cfg->prec == FX_PREC_FP8_2BIT;
+    FX_REQUIRE(cfg->chunk_spectra >= 8 && cfg->chunk_spectra % (twobit ? 16 : 8) == 0, FX_ERR_ARG,
+               "fx_corr_create: chunk_spectra=%ld must be a positive multiple of %d", cfg->chunk_spectra,
+               twobit ? 16 : 8);
     fx_corr* c = new fx_corr;
     c->ctx = ctx;
     c->cfg = *cfg;
     c->nchan = cfg->nfft / 2;
     c->elem = prec_elem(cfg->prec);
-    size_t chan_bytes = (size_t)c->nchan * (size_t)cfg->chunk_spectra * (size_t)cfg->ninp * c->elem;
-    c->acc_floats = fxk_xengine_acc_floats(c->nchan, cfg->ninp, 1);
+    c->nsets = twobit ? 3 : 1;
+    // 2-bit channelized data arrives ready for the engine: no F-stage buffers
+    size_t chan_bytes =
+        twobit ? 0 : (size_t)c->nchan * (size_t)cfg->chunk_spectra * (size_t)cfg->ninp * c->elem;
+    c->acc_floats = fxk_xengine_acc_floats(c->nchan, cfg->ninp, c->nsets);
     long P = (long)cfg->ninp * (cfg->ninp + 1) / 2;
     std::vector<int2> hp((size_t)P);
     for (int i = 0, p = 0; i < cfg->ninp; i++)
         for (int j = i; j < cfg->ninp; j++, p++) hp[(size_t)p] = make_int2(i, j);
-    bool ok = hipMalloc(&c->chan, chan_bytes) == hipSuccess &&
-              hipMalloc(&c->nat, chan_bytes) == hipSuccess &&
-              hipMalloc(&c->acc, c->acc_floats * sizeof(float)) == hipSuccess &&
+    bool ok = hipMalloc(&c->acc, c->acc_floats * sizeof(float)) == hipSuccess &&
               hipMalloc(&c->pm, sizeof(fx_phase_model) * (size_t)cfg->ninp) == hipSuccess &&
               hipMalloc(&c->pij, sizeof(int2) * (size_t)P) == hipSuccess;
+    if (ok && chan_bytes)
+        ok = hipMalloc(&c->chan, chan_bytes) == hipSuccess && hipMalloc(&c->nat, chan_bytes) == hipSuccess;
     if (ok) ok = hipMemcpy(c->pij, hp.data(), sizeof(int2) * (size_t)P, hipMemcpyHostToDevice) == hipSuccess;
     if (ok) ok = hipMemsetAsync(c->acc, 0, c->acc_floats * sizeof(float), ctx->stream) == hipSuccess;
     if (ok) ok = hipMemsetAsync(c->pm, 0, sizeof(fx_phase_model) * (size_t)cfg->ninp, ctx->stream) == hipSuccess;
-    if (ok && cfg->prec != FX_PREC_F32) {
+    if (ok && twobit) {
+        // X X^H = (2-H) S S^H + (H-1) U U^H + (H-1)(H-2) T T^H
+        const double h = (double)FX_LUT_HI;
+        const float w[3] = {(float)(2.0 - h), (float)(h - 1.0), (float)((h - 1.0) * (h - 2.0))};
+        ok = hipMalloc(&c->coef, sizeof(w)) == hipSuccess &&
+             hipMemcpy(c->coef, w, sizeof(w), hipMemcpyHostToDevice) == hipSuccess;
+    }
+    if (ok && (cfg->prec == FX_PREC_F16 || cfg->prec == FX_PREC_FP8)) {
         // default power scaling: a 2-bit stream through the level table has
         // E[x^2] = 0.6726 + 0.3274*H^2 ~ 4.316, so the FFT output has
-        // sigma ~ sqrt(N * 4.316); map that to 1 before the cast
-        std::vector<float> sc((size_t)c->nchan, (float)(1.0 / sqrt((double)cfg->nfft * 4.316)));
+        // sigma ~ sqrt(N * 4.316); map that to 1 before an FP16 cast and to
+        // 16 before an e4m3 cast (normal range 2^-6 .. 448: below 1e-3 sigma
+        // and above 28 sigma the format loses relative precision)
+        const double target = cfg->prec == FX_PREC_FP8 ? 16.0 : 1.0;
+        std::vector<float> sc((size_t)c->nchan, (float)(target / sqrt((double)cfg->nfft * 4.316)));
         std::vector<float> si((size_t)cfg->ninp, 1.0f);
         ok = hipMalloc(&c->sc_chan, sizeof(float) * sc.size()) == hipSuccess &&
              hipMalloc(&c->sc_inp, sizeof(float) * si.size()) == hipSuccess &&
@@ -716,6 +753,7 @@ extern "C" int fx_corr_destroy(fx_corr* c)
     if (c->pij) hipFree(c->pij);
     if (c->sc_chan) hipFree(c->sc_chan);
     if (c->sc_inp) hipFree(c->sc_inp);
+    if (c->coef) hipFree(c->coef);
     delete c;
     return FX_OK;
 }
@@ -772,8 +810,12 @@ extern "C" int fx_corr_fstage(fx_corr* c, const uint32_t* d_words, size_t in_str
     FX_REQUIRE(t_first >= 0 && nspectra >= 0, FX_ERR_ARG, "fx_corr_fstage: negative range");
     FX_REQUIRE(nspectra <= c->cfg.chunk_spectra, FX_ERR_ARG,
                "fx_corr_fstage: %ld spectra exceed the %ld-spectrum chunk", nspectra, c->cfg.chunk_spectra);
+    FX_REQUIRE(c->nat != nullptr && out_prec != FX_PREC_FP8_2BIT, FX_ERR_UNSUPPORTED,
+               "fx_corr_fstage: no F-stage for 2-bit channelized engines");
     FX_REQUIRE(prec_elem(out_prec) > 0 && prec_elem(out_prec) <= c->elem, FX_ERR_ARG,
                "fx_corr_fstage: output precision %d wider than the engine's", out_prec);
+    FX_REQUIRE(out_prec != FX_PREC_FP8 || nspectra % 8 == 0, FX_ERR_ARG,
+               "fx_corr_fstage: FP8 output needs a multiple of 8 spectra");
     if (nspectra == 0) return FX_OK;
     const bool scaled = out_prec != FX_PREC_F32;
     FX_CHECK_LAUNCH(fxk_fstage(c->ctx->stream, c->ctx->tw, d_words, in_stride_words, c->cfg.ninp,
@@ -789,11 +831,15 @@ extern "C" int fx_corr_process(fx_corr* c, const uint32_t* d_words, size_t in_st
     FX_REQUIRE(c && d_words, FX_ERR_ARG, "fx_corr_process: null argument");
     int rc = ctx_device(c->ctx);
     if (rc) return rc;
+    FX_REQUIRE(c->cfg.prec != FX_PREC_FP8_2BIT, FX_ERR_UNSUPPORTED,
+               "fx_corr_process: a 2-bit channelized engine takes fx_corr_xengine input");
+    FX_REQUIRE(c->cfg.prec != FX_PREC_FP8 || nspectra % 8 == 0, FX_ERR_ARG,
+               "fx_corr_process: the FP8 engine needs a multiple of 8 spectra");
     for (long done = 0; done < nspectra;) {
         long nt = std::min(c->cfg.chunk_spectra, nspectra - done);
         rc = fx_corr_fstage(c, d_words, in_stride_words, t_first + done, nt, c->chan, c->cfg.prec);
         if (rc) return rc;
-        FX_CHECK_LAUNCH(fxk_xengine(c->ctx->stream, c->cfg.prec, c->chan, c->nchan, nt, c->cfg.ninp, c->acc));
+        FX_CHECK_LAUNCH(corr_engine(c, c->chan, nt));
         done += nt;
     }
     c->spectra_done += nspectra;
@@ -806,7 +852,12 @@ extern "C" int fx_corr_xengine(fx_corr* c, const void* d_chan, long nspectra)
     int rc = ctx_device(c->ctx);
     if (rc) return rc;
     if (nspectra <= 0) return nspectra == 0 ? FX_OK : FX_ERR_ARG;
-    FX_CHECK_LAUNCH(fxk_xengine(c->ctx->stream, c->cfg.prec, d_chan, c->nchan, nspectra, c->cfg.ninp, c->acc));
+    const int mult = c->cfg.prec == FX_PREC_FP8_2BIT ? 16 : c->cfg.prec == FX_PREC_FP8 ? 8 : 1;
+    FX_REQUIRE(nspectra % mult == 0, FX_ERR_ARG, "fx_corr_xengine: %ld spectra, need a multiple of %d",
+               nspectra, mult);
+    FX_REQUIRE(c->cfg.prec != FX_PREC_FP8_2BIT || c->spectra_done + nspectra <= (1L << 21), FX_ERR_ARG,
+               "fx_corr_xengine: more than 2^21 spectra per dump leave the exact FP32 range");
+    FX_CHECK_LAUNCH(corr_engine(c, d_chan, nspectra));
     c->spectra_done += nspectra;
     return FX_OK;
 }
@@ -816,8 +867,8 @@ extern "C" int fx_corr_dump(fx_corr* c, fx_cf32* d_vis)
     FX_REQUIRE(c && d_vis, FX_ERR_ARG, "fx_corr_dump: null argument");
     int rc = ctx_device(c->ctx);
     if (rc) return rc;
-    const bool scaled = c->cfg.prec != FX_PREC_F32;
-    FX_CHECK_LAUNCH(fxk_xengine_dump_pij(c->ctx->stream, c->acc, c->nchan, c->cfg.ninp, 1, nullptr,
+    const bool scaled = c->cfg.prec == FX_PREC_F16 || c->cfg.prec == FX_PREC_FP8;
+    FX_CHECK_LAUNCH(fxk_xengine_dump_pij(c->ctx->stream, c->acc, c->nchan, c->cfg.ninp, c->nsets, c->coef,
                                          scaled ? c->sc_chan : nullptr, scaled ? c->sc_inp : nullptr,
                                          c->pij, 0, (float2*)d_vis));
     return FX_OK;

@@ -20,6 +20,8 @@
 
 #include <hip/hip_fp8.h>
 
+#include <type_traits>
+
 namespace {
 
 template <int M>
@@ -218,11 +220,11 @@ __global__ void k_spec_reduce(const float* __restrict__ partial, int M, int F, i
 }
 
 // ---------------------------------------------------------------------------
-// F-stage.  Item g = t*I + i of a chunk is spectrum t_first + t of input i.
-// The kernel writes its natural output rows nat[g][c] (each transform's C
-// channels contiguous, every store instruction a 256-byte run), and
-// k_cornerturn re-tiles them through LDS into the X-engine layout
-// out[c][g] ([c][t][i]).  Writing [c][t][i] straight from the transform
+// F-stage.  Item g = t*I + i of a chunk (g = i*nt + t for FP8 output) is
+// spectrum t_first + t of input i.  The kernel writes its natural output rows
+// nat[g][c] (each transform's C channels contiguous, every store instruction
+// a 256-byte run), and k_cornerturn re-tiles them through LDS into the
+// X-engine layout out[c][g] ([c][t][i], or [c][i][t] for FP8).  Writing [c][t][i] straight from the transform
 // scatters 4-16 byte fragments that the L2 writes back as partial lines.
 // ---------------------------------------------------------------------------
 template <typename T> struct ChanT;
@@ -260,6 +262,9 @@ __global__ __launch_bounds__(FftShape<M>::NT) void k_fstage(
     constexpr int WPT = N / 16;            // packed words per transform
     constexpr int WTH = (S::FM / 8) / NT;  // packed words per thread (2)
     typedef typename ChanT<T>::V V;
+    // item order: g = t*I + i (FP32/FP16 engines, [c][t][i] after the corner
+    // turn) or g = i*nt + t (FP8 engine, [c][i][t])
+    constexpr bool IM = std::is_same<T, uint16_t>::value;
     // phasor tables: channel k = 64 hi + lo of item f rotates by
     // ph_hi[f][hi] * ph_lo[f][lo] = e^{2 pi i (k tau/N + fringe/2pi)}, both
     // factors evaluated from the FP64 phase model (sincospi in FP64, one
@@ -280,8 +285,8 @@ __global__ __launch_bounds__(FftShape<M>::NT) void k_fstage(
             const int j = e - f * (PLO + PHI);
             const long g = g0 + f;
             if (g >= nitems) continue;
-            const long t = g / ninp;
-            const int i = (int)(g - t * ninp);
+            const long t = IM ? g % nt : g / ninp;
+            const int i = (int)(IM ? g / nt : g - t * ninp);
             const fx_phase_model pmi = pm[i];
             const double tg = (double)(t_first + t);
             const double tau = pmi.delay0 + pmi.delay_rate * tg;
@@ -312,8 +317,8 @@ __global__ __launch_bounds__(FftShape<M>::NT) void k_fstage(
 #pragma unroll
         for (int q = 0; q < WTH; q++) w[q] = 0;
         if (g < nitems) {
-            const long t = g / ninp;
-            const int i = (int)(g - t * ninp);
+            const long t = IM ? g % nt : g / ninp;
+            const int i = (int)(IM ? g / nt : g - t * ninp);
             const uint32_t* src = words + (size_t)i * in_stride + (size_t)(t_first + t) * WPT + wof;
 #pragma unroll
             for (int q = 0; q < WTH; q += 2) {
@@ -344,8 +349,8 @@ __global__ __launch_bounds__(FftShape<M>::NT) void k_fstage(
         const int kp = q - f * H;
         const long g = g0 + f;
         if (g >= nitems) continue;
-        const long t = g / ninp;
-        const int i = (int)(g - t * ninp);
+        const long t = IM ? g % nt : g / ninp;
+        const int i = (int)(IM ? g / nt : g - t * ninp);
         float2 xk, xm;
         rsplit<M>(sh, f * M, kp, tw, xk, xm);
         const int k2 = M - kp;
@@ -375,8 +380,8 @@ __global__ __launch_bounds__(FftShape<M>::NT) void k_fstage(
     if (tid < F) {
         const long g = g0 + tid;
         if (g < nitems) {
-            const long t = g / ninp;
-            const int i = (int)(g - t * ninp);
+            const long t = IM ? g % nt : g / ninp;
+            const int i = (int)(IM ? g / nt : g - t * ninp);
             float2 xk, xm;
             rsplit<M>(sh, tid * M, M / 2, tw, xk, xm);
             if (pm) xk = cmul(xk, cmul(ph_hi[tid][(M / 2) / PLO], ph_lo[tid][(M / 2) % PLO]));

@@ -221,7 +221,7 @@ int fxk_synth_station(hipStream_t s, uint64_t seed, int station, double rho, lon
 int fxk_synth_multichannel(hipStream_t s, uint64_t seed, int nchan, const double* tone_amp,
                            const double* tone_freq, double thresh, size_t ntime, uint32_t* words);
 int fxk_synth_channelized(hipStream_t s, uint64_t seed, int nchan, long ntime, int ninp,
-                          double rho, double thresh, uint8_t* codes);
+                          double rho, double thresh, int imajor, uint8_t* codes);
 
 int fxk_twiddle_init(hipStream_t s, float2* tw);
 int fxk_fft_c2c(hipStream_t s, const float2* tw, const float2* in, float2* out, int m, long batch);
@@ -236,8 +236,13 @@ int fxk_fstage(hipStream_t s, const float2* tw, const uint32_t* words, size_t in
                void* nat, void* out);
 int fxk_xengine(hipStream_t s, int in_type, const void* X, int nchan, long nt, int ninp,
                 float* acc);
-int fxk_xengine_codes(hipStream_t s, int mode, const uint8_t* codes, int nchan, long nt, int ninp,
-                      float* acc);
+// 8-bit X-engine on input-major channelized data X[c][i][t]: FX_PREC_FP8
+// (e4m3 re|im<<8 pairs, one accumulator set) or FX_PREC_FP8_2BIT (2-bit
+// codes re|im<<2, three accumulator sets of the exact decomposition)
+int fxk_xengine8(hipStream_t s, int prec, const void* X, int nchan, long nt, int ninp, float* acc);
+// 2-bit channelized codes [c][t][i] or [c][i][t] -> [c][t][i] FP32/FP16 samples
+int fxk_codes_to_chan(hipStream_t s, const uint8_t* codes, int in_imajor, int nchan, long nt, int ninp,
+                      int out_prec, void* out);
 size_t fxk_xengine_acc_floats(int nchan, int ninp, int nsets);
 int fxk_xengine_dump_pij(hipStream_t s, const float* acc, int nchan, int ninp, int nsets,
                          const float* set_coef, const float* scale_chan, const float* scale_inp,

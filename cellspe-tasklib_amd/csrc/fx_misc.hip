@@ -136,15 +136,24 @@ __global__ void k_synth_multichannel(uint64_t seed, int nchan, ToneSet tones, do
 }
 
 __global__ void k_synth_channelized(uint64_t seed, int nchan, long ntime, int ninp, double rho,
-                                    double thresh, uint8_t* __restrict__ codes)
+                                    double thresh, int imajor, uint8_t* __restrict__ codes)
 {
     const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     const size_t total = (size_t)nchan * (size_t)ntime * (size_t)ninp;
     if (idx >= total) return;
-    const int i = (int)(idx % (size_t)ninp);
-    const size_t ct = idx / (size_t)ninp;
-    const long t = (long)(ct % (size_t)ntime);
-    const int c = (int)(ct / (size_t)ntime);
+    int i, c;
+    long t;
+    if (imajor) {   // codes[c][i][t]
+        t = (long)(idx % (size_t)ntime);
+        const size_t ci = idx / (size_t)ntime;
+        i = (int)(ci % (size_t)ninp);
+        c = (int)(ci / (size_t)ninp);
+    } else {        // codes[c][t][i]
+        i = (int)(idx % (size_t)ninp);
+        const size_t ct = idx / (size_t)ninp;
+        t = (long)(ct % (size_t)ntime);
+        c = (int)(ct / (size_t)ntime);
+    }
     const double a = sqrt(1.0 - rho * rho);
     const uint64_t kc = stream_key(seed, 0x10000 + (uint64_t)c);
     const uint64_t ko = stream_key(seed, 0x100000000ull + (uint64_t)c * 65536u + (uint64_t)i);
@@ -363,12 +372,51 @@ int fxk_synth_multichannel(hipStream_t s, uint64_t seed, int nchan, const double
 }
 
 int fxk_synth_channelized(hipStream_t s, uint64_t seed, int nchan, long ntime, int ninp,
-                          double rho, double thresh, uint8_t* codes)
+                          double rho, double thresh, int imajor, uint8_t* codes)
 {
     size_t total = (size_t)nchan * (size_t)ntime * (size_t)ninp;
     if (total == 0) return FX_OK;
     hipLaunchKernelGGL(k_synth_channelized, dim3(nblocks(total, 256)), dim3(256), 0, s, seed, nchan,
-                       ntime, ninp, rho, thresh, codes);
+                       ntime, ninp, rho, thresh, imajor, codes);
+    return last_status();
+}
+
+// 2-bit channelized codes -> [c][t][i] FP32 / FP16 samples through the level
+// table (the FP32 and FP16 engines' input for channelized data)
+template <typename V>
+__global__ void k_codes_to_chan(const uint8_t* __restrict__ codes, int in_imajor, int nchan, long nt,
+                                int ninp, V* __restrict__ out)
+{
+    const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t total = (size_t)nchan * (size_t)nt * (size_t)ninp;
+    if (idx >= total) return;
+    const int i = (int)(idx % (size_t)ninp);
+    const size_t ct = idx / (size_t)ninp;
+    const long t = (long)(ct % (size_t)nt);
+    const int c = (int)(ct / (size_t)nt);
+    const uint32_t code = in_imajor ? codes[((size_t)c * ninp + i) * (size_t)nt + t] : codes[idx];
+    const float re = lut2(code & 3u), im = lut2((code >> 2) & 3u);
+    if constexpr (sizeof(V) == 8) {
+        out[idx] = make_float2(re, im);
+    } else {
+        __half2 h = __floats2half2_rn(re, im);
+        out[idx] = *reinterpret_cast<uint32_t*>(&h);
+    }
+}
+
+int fxk_codes_to_chan(hipStream_t s, const uint8_t* codes, int in_imajor, int nchan, long nt, int ninp,
+                      int out_prec, void* out)
+{
+    size_t total = (size_t)nchan * (size_t)nt * (size_t)ninp;
+    if (total == 0) return FX_OK;
+    if (out_prec == FX_PREC_F32)
+        hipLaunchKernelGGL(k_codes_to_chan<float2>, dim3(nblocks(total, 256)), dim3(256), 0, s, codes,
+                           in_imajor, nchan, nt, ninp, reinterpret_cast<float2*>(out));
+    else if (out_prec == FX_PREC_F16)
+        hipLaunchKernelGGL(k_codes_to_chan<uint32_t>, dim3(nblocks(total, 256)), dim3(256), 0, s, codes,
+                           in_imajor, nchan, nt, ninp, reinterpret_cast<uint32_t*>(out));
+    else
+        return FX_ERR_ARG;
     return last_status();
 }
 

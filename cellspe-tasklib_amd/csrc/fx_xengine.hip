@@ -240,6 +240,184 @@ __global__ __launch_bounds__(XNT, 2) void k_xengine(const typename XTraits<PREC>
 }
 
 // ---------------------------------------------------------------------------
+// 8-bit engine: v_mfma_scale_f32_32x32x64_f8f6f4 with unit (E8M0 = 127)
+// scales, K = 64 e4m3 values = 32 complex samples per instruction.
+//
+// Input is input-major, X[c][i][t], so one row of a stage (KT samples of one
+// input) is a contiguous HBM run and lands in LDS as the contiguous K-run a
+// lane feeds to the MFMA: A and B operands are two 16-byte LDS reads per lane.
+// (Which K index each byte carries is irrelevant: A and B are packed alike.)
+//
+// NS = 1: e4m3 (re, im) pairs, one accumulator set.
+// NS = 3: 2-bit codes expanded while staging into the three exact operands of
+// X = S + (H-1) T: S = sign (+-1), U = S + T (0, +-1, +-2) and T (0, +-1),
+// each an e4m3 table lookup per code (v_perm_b32), so every product and every
+// FP32 partial sum is exact.
+//
+// Workgroup: 8 waves own one 128 x 128 tile of one channel; wave w computes
+// rows 32 (w/2) .. +32 against columns 64 (w%2) .. +64 (two 32 x 32 blocks).
+// Re V = A B^T with A = B = (re, im) pairs; Im V = A' B^T with the rotated
+// A' = (im, -re).  Accumulators go to the same engine layout as k_xengine,
+// one set after another (set stride = one engine image).
+// ---------------------------------------------------------------------------
+constexpr int X8NT = 512;                // threads (8 waves)
+constexpr int X8KT = 64;                 // samples per stage
+constexpr int X8RS = 2 * X8KT + 16;      // LDS row stride in bytes (padded)
+
+// e4m3 images indexed by the 2-bit code of {-H, -1, +1, +H} (byte c = code c)
+constexpr uint32_t kTblS = 0x3838B8B8u;  // S:  -1 -1 +1 +1
+constexpr uint32_t kTblU = 0x4038B8C0u;  // U:  -2 -1 +1 +2
+constexpr uint32_t kTblT = 0x380000B8u;  // T:  -1  0  0 +1
+
+// four code bytes (re | im << 2) -> four e4m3 (re, im) pairs
+__device__ __forceinline__ uint2 expand4(uint32_t w, uint32_t tbl)
+{
+    const uint32_t re = __builtin_amdgcn_perm(0u, tbl, w & 0x03030303u);
+    const uint32_t im = __builtin_amdgcn_perm(0u, tbl, (w >> 2) & 0x03030303u);
+    return make_uint2(__builtin_amdgcn_perm(im, re, 0x05010400u), __builtin_amdgcn_perm(im, re, 0x07030602u));
+}
+
+// two e4m3 pairs (re0, im0, re1, im1) -> (im0, -re0, im1, -re1)
+__device__ __forceinline__ int rot_p8(int v)
+{
+    const uint32_t u = (uint32_t)v;
+    return (int)(__builtin_amdgcn_perm(u, u, 0x02030001u) ^ 0x80008000u);
+}
+
+template <int NS>
+__global__ __launch_bounds__(X8NT, (NS == 1 ? 2 : 1)) void k_xengine8(const uint8_t* __restrict__ X, int nchan,
+                                                                      long nt, int ninp, float* __restrict__ acc,
+                                                                      size_t set_stride)
+{
+    constexpr int BPS = NS == 1 ? 2 : 1;          // HBM bytes per complex sample
+    constexpr int CPR = X8KT * BPS / 16;          // 16-byte chunks per row per stage (8 or 4)
+    constexpr int RCH = TS * CPR;                 // chunks per row range
+    constexpr int LPT = 2 * RCH / X8NT;           // chunks per thread (4 or 2)
+    constexpr int SPC = 16 / BPS;                 // samples per chunk (8 or 16)
+    __shared__ __attribute__((aligned(16))) uint8_t op[2][NS][TS][X8RS];
+
+    const int nb = (ninp + TS - 1) / TS;
+    const int ntl = nb * (nb + 1) / 2;
+    const long lb = xcd_block((long)ntl * nchan);
+    if (lb >= (long)ntl * nchan) return;
+    const int c = (int)(lb / ntl);
+    const int tile = (int)(lb - (long)c * ntl);
+    int ti, tj;
+    tile_coords(tile, nb, ti, tj);
+    const bool diag = ti == tj;
+    const int nslab = diag ? 1 : 2;
+    const int tid = threadIdx.x;
+    const int w = tid >> 6, lane = tid & 63;
+    const int qa = w >> 1, qb = w & 1;
+    bool act[2];
+    act[0] = !(diag && qa > 2 * qb);
+    act[1] = !(diag && qa > 2 * qb + 1);
+    const uint8_t* base = X + (size_t)c * (size_t)ninp * (size_t)nt * BPS;
+    const int row0[2] = {ti * TS, tj * TS};
+
+    fx_float16 are[NS][2], aim[NS][2];
+#pragma unroll
+    for (int s = 0; s < NS; s++)
+#pragma unroll
+        for (int b = 0; b < 2; b++) {
+            are[s][b] = fx_float16{};
+            aim[s][b] = fx_float16{};
+        }
+
+    // chunk q = tid + k * X8NT: range q / RCH, row (q % RCH) / CPR, part q % CPR
+    uint4 pre[LPT];
+    bool val[LPT];
+    auto gload = [&](long t0) {
+#pragma unroll
+        for (int k = 0; k < LPT; k++) {
+            const int q = tid + k * X8NT;
+            const int s = q / RCH;
+            const int row = (q - s * RCH) / CPR;
+            const int part = q % CPR;
+            const int r = row0[s] + row;
+            const long t = t0 + (long)part * SPC;
+            val[k] = s < nslab && r < ninp && t < nt;
+            pre[k] = val[k] ? *reinterpret_cast<const uint4*>(base + ((size_t)r * (size_t)nt + (size_t)t) * BPS)
+                            : make_uint4(0u, 0u, 0u, 0u);
+        }
+    };
+    auto lstore = [&]() {
+#pragma unroll
+        for (int k = 0; k < LPT; k++) {
+            const int q = tid + k * X8NT;
+            const int s = q / RCH;
+            if (s >= nslab) continue;
+            const int row = (q - s * RCH) / CPR;
+            const int part = q % CPR;
+            if constexpr (NS == 1) {
+                *reinterpret_cast<uint4*>(&op[s][0][row][part * 16]) = pre[k];
+            } else {
+                const uint32_t wv[4] = {pre[k].x, pre[k].y, pre[k].z, pre[k].w};
+                const uint32_t tbl[3] = {kTblS, kTblU, kTblT};
+#pragma unroll
+                for (int set = 0; set < 3; set++) {
+                    uint2 e[4];
+#pragma unroll
+                    for (int d = 0; d < 4; d++) e[d] = val[k] ? expand4(wv[d], tbl[set]) : make_uint2(0u, 0u);
+                    uint4* dst = reinterpret_cast<uint4*>(&op[s][set][row][part * 32]);
+                    dst[0] = make_uint4(e[0].x, e[0].y, e[1].x, e[1].y);
+                    dst[1] = make_uint4(e[2].x, e[2].y, e[3].x, e[3].y);
+                }
+            }
+        }
+    };
+
+    const int sb = diag ? 0 : 1;
+    const int r32 = lane & 31, h = lane >> 5;
+    const long nstage = (nt + X8KT - 1) / X8KT;
+    gload(0);
+    for (long st = 0; st < nstage; st++) {
+        __syncthreads();
+        lstore();
+        __syncthreads();
+        if (st + 1 < nstage) gload((st + 1) * X8KT);
+        if (!act[0] && !act[1]) continue;
+#pragma unroll
+        for (int ks = 0; ks < X8KT / 32; ks++) {
+            const int kb = ks * 64 + h * 32;      // byte offset of this lane's 16 samples
+#pragma unroll
+            for (int set = 0; set < NS; set++) {
+                const uint4* pa = reinterpret_cast<const uint4*>(&op[0][set][qa * 32 + r32][kb]);
+                const uint4 a0 = pa[0], a1 = pa[1];
+                fx_int8 fa = {(int)a0.x, (int)a0.y, (int)a0.z, (int)a0.w, (int)a1.x, (int)a1.y, (int)a1.z, (int)a1.w};
+                fx_int8 fr;
+#pragma unroll
+                for (int d = 0; d < 8; d++) fr[d] = rot_p8(fa[d]);
+#pragma unroll
+                for (int b = 0; b < 2; b++) {
+                    if (!act[b]) continue;
+                    const uint4* pb = reinterpret_cast<const uint4*>(&op[sb][set][(2 * qb + b) * 32 + r32][kb]);
+                    const uint4 b0 = pb[0], b1 = pb[1];
+                    fx_int8 fb = {(int)b0.x, (int)b0.y, (int)b0.z, (int)b0.w, (int)b1.x, (int)b1.y, (int)b1.z, (int)b1.w};
+                    are[set][b] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(fa, fb, are[set][b], 0, 0, 0, 127, 0, 127);
+                    aim[set][b] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(fr, fb, aim[set][b], 0, 0, 0, 127, 0, 127);
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int b = 0; b < 2; b++) {
+        if (!act[b]) continue;
+        const int bi = qa, bj = 2 * qb + b;
+        const int w4 = (bi >> 1) * 2 + (bj >> 1), s4 = (bi & 1) * 2 + (bj & 1);
+#pragma unroll
+        for (int set = 0; set < NS; set++) {
+            float* d = acc + set * set_stride + (((size_t)c * ntl + tile) * 16 + w4 * 4 + s4) * (32 * 64);
+#pragma unroll
+            for (int r = 0; r < 16; r++) {
+                d[r * 64 + lane] += are[set][b][r];
+                d[(16 + r) * 64 + lane] += aim[set][b][r];
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // dump: engine layout -> vis[p][c] = sum_s coef[s] * acc_s / (sc_c^2 s_i s_j)
 // MFMA 32x32 output layout: lane l, register r holds row 8(r/4) + 4(l/32) + r%4,
 // column l%32.
@@ -335,6 +513,23 @@ int fxk_xengine(hipStream_t s, int in_type, const void* X, int nchan, long nt, i
     } else if (in_type == FX_PREC_F16) {
         hipLaunchKernelGGL(k_xengine<FX_PREC_F16>, dim3((unsigned)nblk), dim3(XNT), 0, s,
                            reinterpret_cast<const uint32_t*>(X), nchan, nt, ninp, acc);
+    } else {
+        return FX_ERR_UNSUPPORTED;
+    }
+    return hipGetLastError() == hipSuccess ? FX_OK : FX_ERR_HIP;
+}
+
+int fxk_xengine8(hipStream_t s, int prec, const void* X, int nchan, long nt, int ninp, float* acc)
+{
+    long nblk = xcd_grid((long)ntiles128(ninp) * nchan);
+    const size_t stride = fxk_xengine_acc_floats(nchan, ninp, 1);
+    const uint8_t* x = reinterpret_cast<const uint8_t*>(X);
+    if (prec == FX_PREC_FP8) {
+        if (nt % 8) return FX_ERR_ARG;
+        hipLaunchKernelGGL(k_xengine8<1>, dim3((unsigned)nblk), dim3(X8NT), 0, s, x, nchan, nt, ninp, acc, stride);
+    } else if (prec == FX_PREC_FP8_2BIT) {
+        if (nt % 16) return FX_ERR_ARG;
+        hipLaunchKernelGGL(k_xengine8<3>, dim3((unsigned)nblk), dim3(X8NT), 0, s, x, nchan, nt, ninp, acc, stride);
     } else {
         return FX_ERR_UNSUPPORTED;
     }

@@ -68,7 +68,8 @@ def load():
         "fx_event_destroy": (i, [vp]),
         "fx_synth_station": (i, [vp, u64, i, d, l, d, d, d, sz, vp]),
         "fx_synth_multichannel": (i, [vp, u64, i, vp, vp, d, sz, vp]),
-        "fx_synth_channelized": (i, [vp, u64, i, l, i, d, d, vp]),
+        "fx_synth_channelized": (i, [vp, u64, i, l, i, d, d, i, vp]),
+        "fx_codes_to_chan": (i, [vp, vp, i, i, l, i, i, vp]),
         "fx_unpack2b": (i, [vp, vp, sz, i, vp]),
         "fx_fft_c2c": (i, [vp, vp, vp, i, l]),
         "fx_fft_r2c": (i, [vp, vp, vp, i, l, i]),
@@ -246,9 +247,13 @@ class Context:
         check(self.lib.fx_synth_multichannel(self.handle, seed, nchan, _ptr(a), _ptr(f), thresh,
                                              ntime, d_words.ptr))
 
-    def synth_channelized(self, seed, nchan, ntime, ninp, rho, thresh, d_codes):
+    def synth_channelized(self, seed, nchan, ntime, ninp, rho, thresh, d_codes, input_major=False):
         check(self.lib.fx_synth_channelized(self.handle, seed, nchan, ntime, ninp, rho, thresh,
-                                            d_codes.ptr))
+                                            int(input_major), d_codes.ptr))
+
+    def codes_to_chan(self, d_codes, input_major, nchan, ntime, ninp, out_prec, d_out):
+        check(self.lib.fx_codes_to_chan(self.handle, d_codes.ptr, int(input_major), nchan, ntime, ninp,
+                                        out_prec, d_out.ptr))
 
     def mfma_probe(self, which):
         out = np.zeros((32, 32), dtype=np.float32)

@@ -92,9 +92,16 @@ int fx_synth_station(fx_ctx* ctx, uint64_t seed, int station, double rho, long d
 int fx_synth_multichannel(fx_ctx* ctx, uint64_t seed, int nchan, const double* tone_amp,
                           const double* tone_freq_norm, double thresh, size_t ntime,
                           uint32_t* d_words);
-/* 2-bit channelized voltages, codes[c][t][i] = re | im << 2. */
+/* 2-bit channelized voltages, code = re | im << 2, stored codes[c][t][i]
+ * (input_major = 0) or codes[c][i][t] (input_major = 1, the FP8_2BIT
+ * engine's layout); the values do not depend on the layout. */
 int fx_synth_channelized(fx_ctx* ctx, uint64_t seed, int nchan, long ntime, int ninp,
-                         double rho, double thresh, uint8_t* d_codes);
+                         double rho, double thresh, int input_major, uint8_t* d_codes);
+/* 2-bit channelized codes (either layout) -> [c][t][i] FP32 (fx_cf32) or
+ * FP16 (half2) samples through the level table: the input of the FP32 and
+ * FP16 engines for channelized data. */
+int fx_codes_to_chan(fx_ctx* ctx, const uint8_t* d_codes, int input_major, int nchan, long ntime,
+                     int ninp, int out_prec, void* d_out);
 
 /* ---- primitives ------------------------------------------------------------ */
 /* a1: 2-bit unpack through {-3.3359,-1,+1,+3.3359}; nchan in {1,2,4,8,16};
@@ -173,13 +180,24 @@ int fx_corr_set_scales(fx_corr* corr, const float* chan_scale, const float* inp_
  * input i at d_words + i*in_stride_words; spectrum t starts at sample t*N. */
 int fx_corr_process(fx_corr* corr, const uint32_t* d_words, size_t in_stride_words,
                     long t_first, long nspectra);
-/* X only over channelized data in the engine layout [c][t][i]
- * (FP32/FP16 complex) or 2-bit codes (FX_PREC_FP8_2BIT: byte re|im<<2). */
+/* X only over channelized data in the engine layout:
+ *   FX_PREC_F32 / F16: [c][t][i] complex (fx_cf32 / half2);
+ *   FX_PREC_FP8:       [c][i][t] e4m3 pairs (byte re | byte im << 8), scaled
+ *                      by chan_scale[c] * inp_scale[i] (removed in the dump),
+ *                      nspectra a multiple of 8;
+ *   FX_PREC_FP8_2BIT:  [c][i][t] 2-bit codes (byte re | im << 2), nspectra a
+ *                      multiple of 16; exact: X = S + (H-1) T with S the signs
+ *                      and T the signs of the |x| = H samples, XX^H =
+ *                      (2-H) SS^H + (H-1) UU^H + (H-1)(H-2) TT^H, U = S + T,
+ *                      three FP8 GEMMs on values in {0, +-1, +-2} with exact
+ *                      FP32 sums (up to 2^21 spectra per dump) combined in
+ *                      FP32 by the dump. */
 int fx_corr_xengine(fx_corr* corr, const void* d_chan, long nspectra);
 /* write visibilities [p][c] complex64, p = i*I - i(i-1)/2 + (j-i), i <= j */
 int fx_corr_dump(fx_corr* corr, fx_cf32* d_vis);
 int fx_corr_reset(fx_corr* corr);
-/* F-stage alone into a caller buffer laid out [c][t][i] (FP32 complex) */
+/* F-stage alone into a caller buffer in the engine layout of out_prec
+ * (FP32/F16 [c][t][i], FP8 [c][i][t]) */
 int fx_corr_fstage(fx_corr* corr, const uint32_t* d_words, size_t in_stride_words,
                    long t_first, long nspectra, void* d_chan, int out_prec);
 int fx_corr_info(fx_corr* corr, long* chunk_spectra, size_t* chan_bytes, size_t* acc_bytes);
