@@ -1,21 +1,28 @@
 #!/usr/bin/env python3
 """Benchmark of the F-X chain on one MI355X.
 
-Workload (the mid-size correlator): 64 stations x 2 polarizations = 128 inputs
-of 2-bit samples with correlated fraction rho = 0.02 and per-station delays of
-0-50 samples, 8192-point real FFT (4096 channels), FP64 delay/fringe phase
-model applied after the FFT, 65536 spectra (2^29 samples per input, 16 GiB of
-packed input resident in HBM), all 8256 products integrated.
+Headline workload (the mid-size correlator): 64 stations x 2 polarizations =
+128 inputs of 2-bit samples with correlated fraction rho = 0.02 and seeded
+per-station delays of 0-50 samples, 8192-point real FFT (4096 channels), FP64
+delay/fringe phase model applied after the FFT, 65536 spectra (2^29 samples
+per input, 16 GiB of packed input resident in HBM), all 8256 products.
 
-One step = reset accumulators -> F (unpack, FFT, rotate) -> X over all 65536
-spectra in time chunks -> dump [p][c] complex64 visibilities.  The headline
-runs the X-engine on FP16 MFMA; the plain FP32 path is timed beside it.
+One step = reset accumulators -> per 4096-spectrum chunk: F kernel (unpack,
+FFT, rotate, scale, convert) -> corner turn -> X-engine -> dump [p][c]
+complex64 visibilities.  HIP events around every launch give the per-kernel
+durations inside the timed region.  The headline X-engine is FP16 MFMA; the
+plain FP32 path and the scaled FP8 path run the same step beside it.
+
+A second section times the large-N X-engine (256 stations x 2 pol = 512
+inputs, 1024 channels, 8192 spectra of 2-bit channelized voltages): the exact
+FP8 decomposition engine against FP16 and plain FP32 MFMA engines.
 
 Prints ONE JSON line.  `value` is whole-step complex MAC/s (P*C*T / step time).
 """
 import argparse
 import json
 import os
+import re
 import sys
 import time
 
@@ -25,9 +32,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "cellspe-tasklib_amd"))
 
 METRIC = ("X-engine complex MAC/s + input GS/s, FP16/FP8 MFMA vs FP32 plain at ≤1e-3 err")
-PEAK_FP16_TFLOPS = 2500.0
-PEAK_FP32_TFLOPS = 157.3
-PEAK_HBM_GBS = 8000.0
+PEAK = {"hbm": 8000.0, "fp16": 2500.0, "fp32": 157.3, "fp8": 5000.0}
 
 
 def log(msg):
@@ -56,6 +61,124 @@ def nrel(a, b):
     return float(np.linalg.norm((a - b).ravel()) / np.linalg.norm(b.ravel()))
 
 
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC
+    summaries (lines 'bytes_per_launch <kernel> <bytes>'), or None."""
+    pdir = os.path.join(ROOT, "profiles")
+    if not os.path.isdir(pdir):
+        return None
+    for fn in sorted(os.listdir(pdir)):
+        if "traffic" not in fn:
+            continue
+        with open(os.path.join(pdir, fn)) as f:
+            for line in f:
+                m = re.match(r"^bytes_per_launch\s+(\S.*\S)\s+([0-9.eE+]+)\s*$", line)
+                if m and m.group(1) == kernel:
+                    return {"bytes": float(m.group(2)), "source": f"profiles/{fn}"}
+    return None
+
+
+def copy_probe(fxlib, args):
+    ctx = fxlib.Context(0)
+    nbytes = args.copy_probe << 20
+    src, dst = ctx.alloc(nbytes), ctx.alloc(nbytes)
+    src.zero()
+    lib = fxlib.load()
+    for _ in range(args.warmup):
+        fxlib.check(lib.fx_copy_kernel(ctx.handle, src.ptr, dst.ptr, nbytes))
+    e0, e1 = ctx.event(), ctx.event()
+    ctx.sync()
+    ctx.record(e0)
+    for _ in range(args.steps):
+        fxlib.check(lib.fx_copy_kernel(ctx.handle, src.ptr, dst.ptr, nbytes))
+    ctx.record(e1)
+    ms = ctx.elapsed_ms(e0, e1) / args.steps
+    print(json.dumps({"copy_probe_bytes": nbytes, "ms_per_copy": ms,
+                      "gbps_read_plus_write": 2 * nbytes / (ms * 1e-3) / 1e9}), flush=True)
+    ctx.close()
+
+
+def large_n(fxlib, ctx, args):
+    """Large-N X-engine: 2-bit channelized voltages, exact FP8 decomposition
+    vs FP16 and plain FP32 engines (X-engine time only)."""
+    I, C, T = args.large_inputs, args.large_channels, args.large_spectra
+    P = I * (I + 1) // 2
+    cmac = float(P) * C * T
+    log(f"[bench] large-N X-engine: I={I} C={C} T={T} P={P}")
+    codes = ctx.alloc(C * I * T)
+    ctx.synth_channelized(args.seed + 3, C, T, I, 0.01, 0.9816, codes, input_major=True)
+    vis = ctx.alloc(P * C * 8)
+    out = {"workload": f"{I} inputs (256 stations x 2 pol), {C} channels, {T} spectra of 2-bit "
+                       f"channelized voltages, rho=0.01", "products": P}
+    sample_ch = [0, C // 3, 2 * C // 3, C - 1]
+    vis_sample = {}
+
+    def timed(corr, dx, reps):
+        e0, e1 = ctx.event(), ctx.event()
+        corr.reset()
+        corr.xengine(dx, T)          # warm-up
+        ctx.sync()
+        corr.reset()
+        ctx.record(e0)
+        for _ in range(reps):
+            corr.xengine(dx, T)
+        ctx.record(e1)
+        ctx.sync()
+        ms = ctx.elapsed_ms(e0, e1) / reps
+        corr.reset()
+        corr.xengine(dx, T)
+        corr.dump(vis)
+        ctx.sync()
+        v = np.stack([vis.download_at(0, np.complex64, P * C).reshape(P, C)[:, c] for c in sample_ch], 1)
+        return ms, v
+
+    res = {}
+    c8 = fxlib.Correlator(ctx, I, 2 * C, T, fxlib.FX_PREC_FP8_2BIT)
+    ms, v = timed(c8, codes, args.steps)
+    c8.close()
+    res["fp8_2bit"], vis_sample["fp8_2bit"] = ms, v
+    log(f"[bench] large-N fp8 (exact 2-bit decomposition): {ms:.2f} ms")
+    for name, prec, esz, reps in (("f16", fxlib.FX_PREC_F16, 4, args.steps),
+                                  ("f32", fxlib.FX_PREC_F32, 8, 1)):
+        if name == "f32" and args.no_fp32:
+            continue
+        dx = ctx.alloc(C * T * I * esz)
+        ctx.codes_to_chan(codes, True, C, T, I, prec, dx)
+        cc = fxlib.Correlator(ctx, I, 2 * C, 8, prec)
+        # the level values go in unscaled: unit power scales in the dump
+        cc.set_scales(np.ones(C, np.float32), np.ones(I, np.float32))
+        ms, v = timed(cc, dx, reps)
+        cc.close()
+        dx.free()
+        res[name], vis_sample[name] = ms, v
+        log(f"[bench] large-N {name}: {ms:.2f} ms")
+    # FP64 reference on the sample channels (numpy, level-table values)
+    lut = np.array([-3.3359, -1.0, 1.0, 3.3359], np.float32).astype(np.float64)
+    iu = np.triu_indices(I)
+    err = {}
+    if not args.no_check:
+        ref = np.zeros((P, len(sample_ch)), np.complex128)
+        for k, c in enumerate(sample_ch):
+            cb = codes.download_at(c * I * T, np.uint8, I * T).reshape(I, T)
+            x = lut[cb & 3] + 1j * lut[(cb >> 2) & 3]
+            ref[:, k] = (x @ x.conj().T)[iu]
+        err = {k: nrel(v, ref) for k, v in vis_sample.items()}
+        err["sample"] = f"channels {sample_ch}, all {P} products, all {T} spectra, vs FP64"
+        log(f"[bench] large-N errors: " + ", ".join(f"{k} {v:.2e}" for k, v in err.items() if k != "sample"))
+    codes.free()
+    vis.free()
+    flop = 8.0 * cmac
+    out["ms"] = res
+    out["complex_mac_per_s"] = {k: cmac / (v * 1e-3) for k, v in res.items()}
+    out["tflops_equiv"] = {k: flop / (v * 1e-3) / 1e12 for k, v in res.items()}
+    out["speedup_vs_fp32"] = ({k: res["f32"] / v for k, v in res.items() if k != "f32"}
+                              if "f32" in res else None)
+    out["err_vs_fp64"] = err
+    out["note"] = ("fp8_2bit runs three e4m3 GEMMs (S, S+T, T of X = S + (H-1)T), so its MFMA work is "
+                   "3x the algorithmic count; tflops_equiv counts 8*P*C*T")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -67,7 +190,12 @@ def main():
     ap.add_argument("--chunk", type=int, default=4096)
     ap.add_argument("--rho", type=float, default=0.02)
     ap.add_argument("--seed", type=int, default=20240229)
+    ap.add_argument("--large-inputs", type=int, default=512)
+    ap.add_argument("--large-channels", type=int, default=1024)
+    ap.add_argument("--large-spectra", type=int, default=8192)
     ap.add_argument("--no-fp32", action="store_true")
+    ap.add_argument("--no-fp8", action="store_true")
+    ap.add_argument("--no-large", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-check", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
@@ -85,29 +213,14 @@ def main():
     import fxlib
 
     if args.copy_probe:
-        ctx = fxlib.Context(0)
-        nbytes = args.copy_probe << 20
-        src, dst = ctx.alloc(nbytes), ctx.alloc(nbytes)
-        src.zero()
-        lib = fxlib.load()
-        for _ in range(args.warmup):
-            fxlib.check(lib.fx_copy_kernel(ctx.handle, src.ptr, dst.ptr, nbytes))
-        e0, e1 = ctx.event(), ctx.event()
-        ctx.sync()
-        ctx.record(e0)
-        for _ in range(args.steps):
-            fxlib.check(lib.fx_copy_kernel(ctx.handle, src.ptr, dst.ptr, nbytes))
-        ctx.record(e1)
-        ms = ctx.elapsed_ms(e0, e1) / args.steps
-        print(json.dumps({"copy_probe_bytes": nbytes, "ms_per_copy": ms,
-                          "gbps_read_plus_write": 2 * nbytes / (ms * 1e-3) / 1e9}), flush=True)
-        ctx.close()
+        copy_probe(fxlib, args)
         return
 
     I, N, C, T, tau, models = workload(args)
     P = I * (I + 1) // 2
     nwords = T * N // 16            # packed words per input
     chunk = min(args.chunk, T)
+    nchunks = (T + chunk - 1) // chunk
     log(f"[bench] workload: I={I} N={N} C={C} T={T} P={P} chunk={chunk}")
 
     ctx = fxlib.Context(0)
@@ -124,64 +237,57 @@ def main():
     log(f"[bench] generated {I * nwords * 4 / 2**30:.1f} GiB of 2-bit input in {time.time() - t0:.1f}s")
     dvis = ctx.alloc(P * C * 8)
 
-    def run_path(prec, steps, warmup):
+    def run_path(prec, esz, steps, warmup):
+        """Whole steps with an event around every launch; returns ms/step and
+        the mean per-chunk durations of the three kernels."""
         corr = fxlib.Correlator(ctx, I, N, chunk, prec, use_phase=True)
         corr.set_phase_model(models)
+        dchan = ctx.alloc(C * chunk * I * esz)
+        ev = [[ctx.event() for _ in range(4)] for _ in range(nchunks)]
 
         def step():
             corr.reset()
-            corr.process(dw, nwords, 0, T)
+            for k in range(nchunks):
+                nt = min(chunk, T - k * chunk)
+                ctx.record(ev[k][0])
+                corr.fstage_rows(dw, nwords, k * chunk, nt, prec)
+                ctx.record(ev[k][1])
+                corr.cornerturn(nt, prec, dchan)
+                ctx.record(ev[k][2])
+                corr.xengine(dchan, nt)
+                ctx.record(ev[k][3])
             corr.dump(dvis)
 
         for _ in range(warmup):
             step()
         ctx.sync()
         e0, e1 = ctx.event(), ctx.event()
-        ctx.sync()
         ctx.record(e0)
         for _ in range(steps):
             step()
         ctx.record(e1)
         ctx.sync()
         ms = ctx.elapsed_ms(e0, e1) / steps
-        return corr, ms
-
-    def kernel_times(corr, prec, reps=3):
-        """Average duration of one F-stage and one X-engine launch over a chunk,
-        timed with HIP events on the context stream."""
-        esz = {fxlib.FX_PREC_F32: 8, fxlib.FX_PREC_F16: 4}[prec]
-        dchan = ctx.alloc(C * chunk * I * esz)
-        e = [ctx.event() for _ in range(4)]
-        tf = tx = 0.0
-        for r in range(reps):
-            ctx.record(e[0])
-            corr.fstage(dw, nwords, r * chunk % T, chunk, dchan, prec)
-            ctx.record(e[1])
-            corr.xengine(dchan, chunk)
-            ctx.record(e[2])
-            ctx.sync()
-            tf += ctx.elapsed_ms(e[0], e[1])
-            tx += ctx.elapsed_ms(e[1], e[2])
+        # per-launch durations of the last timed step (events are re-recorded each step)
+        kt = np.zeros(3)
+        for k in range(nchunks):
+            kt += [ctx.elapsed_ms(ev[k][j], ev[k][j + 1]) for j in range(3)]
+        kt /= nchunks
         dchan.free()
-        corr.reset()
-        return tf / reps, tx / reps
+        corr.close()
+        return ms, {"fstage": kt[0], "cornerturn": kt[1], "xengine": kt[2]}
 
-    result = {}
-    # ---- headline: FP16 MFMA X-engine --------------------------------------
-    corr16, ms16 = run_path(fxlib.FX_PREC_F16, args.steps, args.warmup)
-    log(f"[bench] fp16 path: {ms16:.2f} ms/step")
-    tf16, tx16 = kernel_times(corr16, fxlib.FX_PREC_F16)
-    log(f"[bench] fp16 per-chunk launches: fstage {tf16:.2f} ms, xengine {tx16:.2f} ms")
-    corr16.close()
-
-    # ---- plain FP32 path ------------------------------------------------------
-    ms32 = tf32 = tx32 = None
+    paths = {}
+    ms16, k16 = run_path(fxlib.FX_PREC_F16, 4, args.steps, args.warmup)
+    paths["f16"] = (ms16, k16)
+    log(f"[bench] fp16 path: {ms16:.2f} ms/step; per chunk " +
+        ", ".join(f"{k} {v:.2f} ms" for k, v in k16.items()))
     if not args.no_fp32:
-        corr32, ms32 = run_path(fxlib.FX_PREC_F32, max(1, args.steps - 1), 1)
-        log(f"[bench] fp32 path: {ms32:.2f} ms/step")
-        tf32, tx32 = kernel_times(corr32, fxlib.FX_PREC_F32)
-        log(f"[bench] fp32 per-chunk launches: fstage {tf32:.2f} ms, xengine {tx32:.2f} ms")
-        corr32.close()
+        paths["f32"] = run_path(fxlib.FX_PREC_F32, 8, max(1, args.steps - 1), 1)
+        log(f"[bench] fp32 path: {paths['f32'][0]:.2f} ms/step")
+    if not args.no_fp8:
+        paths["fp8"] = run_path(fxlib.FX_PREC_FP8, 2, max(1, args.steps - 1), 1)
+        log(f"[bench] fp8 path: {paths['fp8'][0]:.2f} ms/step")
 
     cmac = float(P) * C * T
     value = cmac / (ms16 * 1e-3)
@@ -194,21 +300,22 @@ def main():
         import oracle as O
         te = 16
         streams = [dw.download_at(i * nwords * 4, np.uint32, te * N // 16) for i in range(I)]
-        errs = {}
-        for name, prec in (("f16", fxlib.FX_PREC_F16), ("f32", fxlib.FX_PREC_F32)):
+        got = {}
+        for name, prec in (("f16", fxlib.FX_PREC_F16), ("f32", fxlib.FX_PREC_F32), ("fp8", fxlib.FX_PREC_FP8)):
             c = fxlib.Correlator(ctx, I, N, te, prec, use_phase=True)
             c.set_phase_model(models)
             c.process(dw, nwords, 0, te)
-            errs[name] = c.visibilities()
+            got[name] = c.visibilities()
             c.close()
         nth = min(16, os.cpu_count() or 8)
         t0 = time.time()
         ref64 = O.corr_f64(streams, N, te, models, nthreads=nth)
         log(f"[bench] FP64 oracle on {te} spectra in {time.time() - t0:.1f}s")
-        err = {k: nrel(v, ref64) for k, v in errs.items()}
+        err = {k: nrel(v, ref64) for k, v in got.items()}
         err["sample"] = f"first {te} spectra of all {I} inputs, all {P} products x {C} channels"
-        log(f"[bench] normwise error vs FP64 oracle: f16 {err['f16']:.2e}, f32 {err['f32']:.2e}")
-        del ref64, errs
+        log(f"[bench] normwise error vs FP64 oracle: " +
+            ", ".join(f"{k} {v:.2e}" for k, v in err.items() if k != "sample"))
+        del ref64, got
 
     # ---- CPU baseline (oracle FP32 generic-C chain, all host threads) --------
     cpu = None
@@ -230,20 +337,55 @@ def main():
                "sample": f"{ts} spectra x {I} inputs x {C} channels (FP32 generic-C restatement, "
                          f"project radix-2 FFT, not FFTW), {dt:.1f}s"}
         log(f"[bench] CPU baseline: {cpu['value']:.3e} CMAC/s on {nth} threads ({ts} spectra, {dt:.1f}s)")
+    dw.free()
+    dvis.free()
 
-    # ---- roofline of the dominant kernel -------------------------------------
-    nchunks = (T + chunk - 1) // chunk
+    large = None
+    if not args.no_large:
+        large = large_n(fxlib, ctx, args)
+
+    # ---- roofline of the dominant kernel of the headline step ------------------
     x_flop = 8.0 * P * C * chunk
-    f_bytes = I * chunk * N / 4.0 + I * chunk * C * 4.0   # packed in + FP16 channelized out
-    if tx16 >= tf16:
-        roof = {"kernel": "k_xengine_f16", "bound": "mfma", "achieved": x_flop / (tx16 * 1e-3) / 1e12,
-                "peak": PEAK_FP16_TFLOPS, "unit": "TFLOP/s"}
+    f_bytes = I * chunk * N / 4.0 + I * chunk * C * 4.0      # packed in + FP16 channels out
+    ct_bytes = 2.0 * I * chunk * C * 4.0                      # read + write of the FP16 channels
+    dom = max(k16, key=k16.get)
+    if dom == "xengine":
+        roof = {"kernel": "k_xengine<1>", "bound": "mfma",
+                "achieved": x_flop / (k16[dom] * 1e-3) / 1e12, "peak": PEAK["fp16"], "unit": "TFLOP/s",
+                "algorithmic": f"8*P*C*chunk = {x_flop:.3e} flop per launch"}
+    elif dom == "fstage":
+        roof = {"kernel": "k_fstage<4096, __half2>", "bound": "hbm",
+                "achieved": f_bytes / (k16[dom] * 1e-3) / 1e9, "peak": PEAK["hbm"], "unit": "GB/s",
+                "algorithmic": f"per input-spectrum N/4 B packed in + C*4 B FP16 out; {f_bytes:.3e} B per launch",
+                "valu_tflops": 2.5 * N * np.log2(N) * I * chunk / (k16[dom] * 1e-3) / 1e12}
     else:
-        roof = {"kernel": "k_fstage<4096,__half2>", "bound": "hbm",
-                "achieved": f_bytes / (tf16 * 1e-3) / 1e9, "peak": PEAK_HBM_GBS, "unit": "GB/s"}
+        roof = {"kernel": "k_cornerturn<unsigned int>", "bound": "hbm",
+                "achieved": ct_bytes / (k16[dom] * 1e-3) / 1e9, "peak": PEAK["hbm"], "unit": "GB/s",
+                "algorithmic": f"read + write of the FP16 channels, {ct_bytes:.3e} B per launch"}
     roof["frac"] = roof["achieved"] / roof["peak"]
-    roof["traffic"] = None
-    roof["launch_ms"] = tx16 if roof["bound"] == "mfma" else tf16
+    tr = measured_traffic(roof["kernel"])
+    roof["traffic"] = None if tr is None else tr["bytes"]
+    if tr is not None:
+        roof["traffic_source"] = tr["source"]
+    roof["launch_ms"] = k16[dom]
+
+    kern = {}
+    for name, (ms, kt) in paths.items():
+        for k, v in kt.items():
+            kern[f"{k}_{name}"] = v
+    kern["xengine_f16_tflops"] = x_flop / (k16["xengine"] * 1e-3) / 1e12
+    if "f32" in paths:
+        kern["xengine_f32_tflops"] = x_flop / (paths["f32"][1]["xengine"] * 1e-3) / 1e12
+    if "fp8" in paths:
+        kern["xengine_fp8_tflops"] = x_flop / (paths["fp8"][1]["xengine"] * 1e-3) / 1e12
+    kern["chunks_per_step"] = nchunks
+
+    def path_obj(name):
+        if name not in paths:
+            return None
+        ms = paths[name][0]
+        return {"ms_per_step": ms, "value": cmac / (ms * 1e-3),
+                "input_gsps": I * float(N) * T / (ms * 1e-3) / 1e9}
 
     out = {
         "metric": METRIC,
@@ -264,16 +406,12 @@ def main():
                    "inputs": I, "nfft": N, "channels": C, "spectra": T, "products": P,
                    "chunk_spectra": chunk},
         "input_gsps": input_gsps,
-        "fp32_plain": None if ms32 is None else {
-            "ms_per_step": ms32, "value": cmac / (ms32 * 1e-3),
-            "input_gsps": I * float(N) * T / (ms32 * 1e-3) / 1e9},
-        "speedup_fp16_vs_fp32": None if ms32 is None else ms32 / ms16,
-        "kernels_ms_per_chunk": {"fstage_f16": tf16, "xengine_f16": tx16,
-                                 "fstage_f32": tf32, "xengine_f32": tx32,
-                                 "xengine_f16_tflops": x_flop / (tx16 * 1e-3) / 1e12,
-                                 "xengine_f32_tflops": None if tx32 is None else x_flop / (tx32 * 1e-3) / 1e12,
-                                 "chunks_per_step": nchunks},
+        "fp32_plain": path_obj("f32"),
+        "fp8_scaled": path_obj("fp8"),
+        "speedup_fp16_vs_fp32": None if "f32" not in paths else paths["f32"][0] / ms16,
+        "kernels_ms_per_chunk": kern,
         "err_vs_fp64": err,
+        "large_n_xengine": large,
         "roofline": roof,
         "cpu_baseline": cpu,
     }

@@ -825,6 +825,39 @@ extern "C" int fx_corr_fstage(fx_corr* c, const uint32_t* d_words, size_t in_str
     return FX_OK;
 }
 
+extern "C" int fx_corr_fstage_rows(fx_corr* c, const uint32_t* d_words, size_t in_stride_words,
+                                   long t_first, long nspectra, int out_prec)
+{
+    FX_REQUIRE(c && d_words, FX_ERR_ARG, "fx_corr_fstage_rows: null argument");
+    int rc = ctx_device(c->ctx);
+    if (rc) return rc;
+    FX_REQUIRE(c->nat != nullptr && prec_elem(out_prec) > 0 && prec_elem(out_prec) <= c->elem &&
+                   out_prec != FX_PREC_FP8_2BIT,
+               FX_ERR_ARG, "fx_corr_fstage_rows: output precision %d", out_prec);
+    FX_REQUIRE(in_stride_words % 2 == 0 && ((uintptr_t)d_words % 8) == 0 && t_first >= 0 && nspectra >= 0 &&
+                   nspectra <= c->cfg.chunk_spectra,
+               FX_ERR_ARG, "fx_corr_fstage_rows: bad range or alignment");
+    if (nspectra == 0) return FX_OK;
+    const bool scaled = out_prec != FX_PREC_F32;
+    FX_CHECK_LAUNCH(fxk_fstage(c->ctx->stream, c->ctx->tw, d_words, in_stride_words, c->cfg.ninp,
+                               c->cfg.nfft, t_first, nspectra, c->cfg.use_phase ? c->pm : nullptr,
+                               c->mult, scaled ? c->sc_chan : nullptr, scaled ? c->sc_inp : nullptr,
+                               out_prec, c->nat, nullptr));
+    return FX_OK;
+}
+
+extern "C" int fx_corr_cornerturn(fx_corr* c, long nspectra, int out_prec, void* d_chan)
+{
+    FX_REQUIRE(c && d_chan, FX_ERR_ARG, "fx_corr_cornerturn: null argument");
+    int rc = ctx_device(c->ctx);
+    if (rc) return rc;
+    FX_REQUIRE(c->nat != nullptr && nspectra >= 0 && nspectra <= c->cfg.chunk_spectra, FX_ERR_ARG,
+               "fx_corr_cornerturn: bad range");
+    if (nspectra == 0) return FX_OK;
+    FX_CHECK_LAUNCH(fxk_cornerturn(c->ctx->stream, out_prec, c->nat, d_chan, nspectra * c->cfg.ninp, c->nchan));
+    return FX_OK;
+}
+
 extern "C" int fx_corr_process(fx_corr* c, const uint32_t* d_words, size_t in_stride_words,
                                long t_first, long nspectra)
 {

@@ -465,6 +465,16 @@ int launch_spec(hipStream_t s, const float2* tw, const float* win, const uint32_
     return hipGetLastError() == hipSuccess ? FX_OK : FX_ERR_HIP;
 }
 
+template <typename V>
+int launch_cornerturn_t(hipStream_t s, const void* nat, void* out, long nitems, int M)
+{
+    long ntiles = ((nitems + 63) / 64) * (M / 64);
+    hipLaunchKernelGGL((k_cornerturn<V>), dim3((unsigned)xcd_grid(ntiles)), dim3(256), 0, s,
+                       reinterpret_cast<const V*>(nat), reinterpret_cast<V*>(out), nitems, M);
+    return hipGetLastError() == hipSuccess ? FX_OK : FX_ERR_HIP;
+}
+
+// out == nullptr: the F kernel alone (natural rows), no corner turn
 template <int M, typename T>
 int launch_fstage_t(hipStream_t s, const float2* tw, const uint32_t* words, size_t stride,
                     int ninp, long t_first, long nt, const fx_phase_model* pm, const float2* mult,
@@ -477,10 +487,8 @@ int launch_fstage_t(hipStream_t s, const float2* tw, const uint32_t* words, size
     hipLaunchKernelGGL((k_fstage<M, T>), dim3((unsigned)nblk), dim3(S::NT), 0, s, tw, words, stride,
                        ninp, t_first, nt, pm, mult, sc, si, reinterpret_cast<T*>(nat));
     if (hipGetLastError() != hipSuccess) return FX_ERR_HIP;
-    long ntiles = ((nitems + 63) / 64) * (M / 64);
-    hipLaunchKernelGGL((k_cornerturn<V>), dim3((unsigned)xcd_grid(ntiles)), dim3(256), 0, s,
-                       reinterpret_cast<const V*>(nat), reinterpret_cast<V*>(out), nitems, M);
-    return hipGetLastError() == hipSuccess ? FX_OK : FX_ERR_HIP;
+    if (!out) return FX_OK;
+    return launch_cornerturn_t<V>(s, nat, out, nitems, M);
 }
 
 template <int M>
@@ -577,6 +585,17 @@ int fxk_fstage(hipStream_t s, const float2* tw, const uint32_t* words, size_t st
     case 4096: return launch_fstage<2048>(s, tw, words, stride, ninp, t_first, nt, pm, mult, sc, si, out_type, nat, out);
     case 8192: return launch_fstage<4096>(s, tw, words, stride, ninp, t_first, nt, pm, mult, sc, si, out_type, nat, out);
     case 16384: return launch_fstage<8192>(s, tw, words, stride, ninp, t_first, nt, pm, mult, sc, si, out_type, nat, out);
+    default: return FX_ERR_ARG;
+    }
+}
+
+int fxk_cornerturn(hipStream_t s, int out_type, const void* nat, void* out, long nitems, int nchan)
+{
+    if (nchan % 64) return FX_ERR_ARG;
+    switch (out_type) {
+    case FX_PREC_F32: return launch_cornerturn_t<float2>(s, nat, out, nitems, nchan);
+    case FX_PREC_F16: return launch_cornerturn_t<uint32_t>(s, nat, out, nitems, nchan);
+    case FX_PREC_FP8: return launch_cornerturn_t<uint16_t>(s, nat, out, nitems, nchan);
     default: return FX_ERR_ARG;
     }
 }

@@ -234,6 +234,9 @@ int fxk_fstage(hipStream_t s, const float2* tw, const uint32_t* words, size_t in
                int ninp, int n, long t_first, long nt, const fx_phase_model* pm,
                const float2* mult, const float* scale_chan, const float* scale_inp, int out_type,
                void* nat, void* out);
+// corner turn of F-stage rows nat[g][c] -> out[c][g] (fxk_fstage with
+// out == nullptr stops after the F kernel)
+int fxk_cornerturn(hipStream_t s, int out_type, const void* nat, void* out, long nitems, int nchan);
 int fxk_xengine(hipStream_t s, int in_type, const void* X, int nchan, long nt, int ninp,
                 float* acc);
 // 8-bit X-engine on input-major channelized data X[c][i][t]: FX_PREC_FP8

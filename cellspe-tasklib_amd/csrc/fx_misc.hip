@@ -138,28 +138,30 @@ __global__ void k_synth_multichannel(uint64_t seed, int nchan, ToneSet tones, do
 __global__ void k_synth_channelized(uint64_t seed, int nchan, long ntime, int ninp, double rho,
                                     double thresh, int imajor, uint8_t* __restrict__ codes)
 {
-    const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     const size_t total = (size_t)nchan * (size_t)ntime * (size_t)ninp;
-    if (idx >= total) return;
-    int i, c;
-    long t;
-    if (imajor) {   // codes[c][i][t]
-        t = (long)(idx % (size_t)ntime);
-        const size_t ci = idx / (size_t)ntime;
-        i = (int)(ci % (size_t)ninp);
-        c = (int)(ci / (size_t)ninp);
-    } else {        // codes[c][t][i]
-        i = (int)(idx % (size_t)ninp);
-        const size_t ct = idx / (size_t)ninp;
-        t = (long)(ct % (size_t)ntime);
-        c = (int)(ct / (size_t)ntime);
+    // grid-stride: totals beyond 2^32 work items exceed one launch's grid
+    for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+         idx += (size_t)gridDim.x * blockDim.x) {
+        int i, c;
+        long t;
+        if (imajor) {   // codes[c][i][t]
+            t = (long)(idx % (size_t)ntime);
+            const size_t ci = idx / (size_t)ntime;
+            i = (int)(ci % (size_t)ninp);
+            c = (int)(ci / (size_t)ninp);
+        } else {        // codes[c][t][i]
+            i = (int)(idx % (size_t)ninp);
+            const size_t ct = idx / (size_t)ninp;
+            t = (long)(ct % (size_t)ntime);
+            c = (int)(ct / (size_t)ntime);
+        }
+        const double a = sqrt(1.0 - rho * rho);
+        const uint64_t kc = stream_key(seed, 0x10000 + (uint64_t)c);
+        const uint64_t ko = stream_key(seed, 0x100000000ull + (uint64_t)c * 65536u + (uint64_t)i);
+        double xr = rho * gauss(kc, (uint64_t)(2 * t)) + a * gauss(ko, (uint64_t)(2 * t));
+        double xi = rho * gauss(kc, (uint64_t)(2 * t + 1)) + a * gauss(ko, (uint64_t)(2 * t + 1));
+        codes[idx] = (uint8_t)(quant2(xr, thresh) | (quant2(xi, thresh) << 2));
     }
-    const double a = sqrt(1.0 - rho * rho);
-    const uint64_t kc = stream_key(seed, 0x10000 + (uint64_t)c);
-    const uint64_t ko = stream_key(seed, 0x100000000ull + (uint64_t)c * 65536u + (uint64_t)i);
-    double xr = rho * gauss(kc, (uint64_t)(2 * t)) + a * gauss(ko, (uint64_t)(2 * t));
-    double xi = rho * gauss(kc, (uint64_t)(2 * t + 1)) + a * gauss(ko, (uint64_t)(2 * t + 1));
-    codes[idx] = (uint8_t)(quant2(xr, thresh) | (quant2(xi, thresh) << 2));
 }
 
 // ---------------------------------------------------------------------------
@@ -330,6 +332,8 @@ __global__ void k_iotest(float* out)
 }
 
 inline unsigned nblocks(size_t n, unsigned bs) { return (unsigned)((n + bs - 1) / bs); }
+// grid for the grid-stride kernels: enough workgroups to fill the chip
+inline unsigned nblocks_gs(size_t n, unsigned bs) { return (unsigned)std::min<size_t>((n + bs - 1) / bs, 1u << 18); }
 inline int last_status() { return hipGetLastError() == hipSuccess ? FX_OK : FX_ERR_HIP; }
 
 } // namespace
@@ -376,7 +380,7 @@ int fxk_synth_channelized(hipStream_t s, uint64_t seed, int nchan, long ntime, i
 {
     size_t total = (size_t)nchan * (size_t)ntime * (size_t)ninp;
     if (total == 0) return FX_OK;
-    hipLaunchKernelGGL(k_synth_channelized, dim3(nblocks(total, 256)), dim3(256), 0, s, seed, nchan,
+    hipLaunchKernelGGL(k_synth_channelized, dim3(nblocks_gs(total, 256)), dim3(256), 0, s, seed, nchan,
                        ntime, ninp, rho, thresh, imajor, codes);
     return last_status();
 }
@@ -387,20 +391,21 @@ template <typename V>
 __global__ void k_codes_to_chan(const uint8_t* __restrict__ codes, int in_imajor, int nchan, long nt,
                                 int ninp, V* __restrict__ out)
 {
-    const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     const size_t total = (size_t)nchan * (size_t)nt * (size_t)ninp;
-    if (idx >= total) return;
-    const int i = (int)(idx % (size_t)ninp);
-    const size_t ct = idx / (size_t)ninp;
-    const long t = (long)(ct % (size_t)nt);
-    const int c = (int)(ct / (size_t)nt);
-    const uint32_t code = in_imajor ? codes[((size_t)c * ninp + i) * (size_t)nt + t] : codes[idx];
-    const float re = lut2(code & 3u), im = lut2((code >> 2) & 3u);
-    if constexpr (sizeof(V) == 8) {
-        out[idx] = make_float2(re, im);
-    } else {
-        __half2 h = __floats2half2_rn(re, im);
-        out[idx] = *reinterpret_cast<uint32_t*>(&h);
+    for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+         idx += (size_t)gridDim.x * blockDim.x) {
+        const int i = (int)(idx % (size_t)ninp);
+        const size_t ct = idx / (size_t)ninp;
+        const long t = (long)(ct % (size_t)nt);
+        const int c = (int)(ct / (size_t)nt);
+        const uint32_t code = in_imajor ? codes[((size_t)c * ninp + i) * (size_t)nt + t] : codes[idx];
+        const float re = lut2(code & 3u), im = lut2((code >> 2) & 3u);
+        if constexpr (sizeof(V) == 8) {
+            out[idx] = make_float2(re, im);
+        } else {
+            __half2 h = __floats2half2_rn(re, im);
+            out[idx] = *reinterpret_cast<uint32_t*>(&h);
+        }
     }
 }
 
@@ -410,10 +415,10 @@ int fxk_codes_to_chan(hipStream_t s, const uint8_t* codes, int in_imajor, int nc
     size_t total = (size_t)nchan * (size_t)nt * (size_t)ninp;
     if (total == 0) return FX_OK;
     if (out_prec == FX_PREC_F32)
-        hipLaunchKernelGGL(k_codes_to_chan<float2>, dim3(nblocks(total, 256)), dim3(256), 0, s, codes,
+        hipLaunchKernelGGL(k_codes_to_chan<float2>, dim3(nblocks_gs(total, 256)), dim3(256), 0, s, codes,
                            in_imajor, nchan, nt, ninp, reinterpret_cast<float2*>(out));
     else if (out_prec == FX_PREC_F16)
-        hipLaunchKernelGGL(k_codes_to_chan<uint32_t>, dim3(nblocks(total, 256)), dim3(256), 0, s, codes,
+        hipLaunchKernelGGL(k_codes_to_chan<uint32_t>, dim3(nblocks_gs(total, 256)), dim3(256), 0, s, codes,
                            in_imajor, nchan, nt, ninp, reinterpret_cast<uint32_t*>(out));
     else
         return FX_ERR_ARG;

@@ -98,6 +98,8 @@ def load():
         "fx_corr_dump": (i, [vp, vp]),
         "fx_corr_reset": (i, [vp]),
         "fx_corr_fstage": (i, [vp, vp, sz, l, l, vp, i]),
+        "fx_corr_fstage_rows": (i, [vp, vp, sz, l, l, i]),
+        "fx_corr_cornerturn": (i, [vp, l, i, vp]),
         "fx_corr_info": (i, [vp, C.POINTER(l), C.POINTER(sz), C.POINTER(sz)]),
         "fx_mfma_probe": (i, [vp, i, vp]),
     }
@@ -328,6 +330,13 @@ class Correlator:
     def fstage(self, d_words, stride_words, t_first, nspectra, d_chan, out_prec):
         check(load().fx_corr_fstage(self.handle, d_words.ptr, stride_words, t_first, nspectra,
                                     d_chan.ptr, out_prec))
+
+    def fstage_rows(self, d_words, stride_words, t_first, nspectra, out_prec):
+        check(load().fx_corr_fstage_rows(self.handle, d_words.ptr, stride_words, t_first, nspectra,
+                                         out_prec))
+
+    def cornerturn(self, nspectra, out_prec, d_chan):
+        check(load().fx_corr_cornerturn(self.handle, nspectra, out_prec, d_chan.ptr))
 
     def xengine(self, d_chan, nspectra):
         check(load().fx_corr_xengine(self.handle, d_chan.ptr, nspectra))
