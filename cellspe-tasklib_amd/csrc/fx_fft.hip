@@ -132,7 +132,8 @@ __global__ __launch_bounds__(FftShape<M>::NT) void k_spectrometer(
     const int cs = wg / groups;
     const int c = cs % nchan;
     const long s = cs / nchan;
-    const int spw = 16 / nchan;          // time samples per word
+    const int spw = 16 / nchan;          // time samples per word (a power of two)
+    const int lspw = __builtin_ctz(spw);
     const int shift0 = 2 * c;
     const int bits_per_t = 2 * nchan;
 
@@ -157,10 +158,10 @@ __global__ __launch_bounds__(FftShape<M>::NT) void k_spectrometer(
             if (it + f < fft_count) {
                 // FFT j of spectrum s starts at s*spec_stride + j*hop (hop = N/overlap)
                 long t = s * spec_stride + ((long)g * per_group + it + f) * hop + n2;
-                uint32_t w0 = words[t / spw];
-                uint32_t w1 = words[(t + 1) / spw];
-                int p0 = (int)(t % spw) * bits_per_t + shift0;
-                int p1 = (int)((t + 1) % spw) * bits_per_t + shift0;
+                uint32_t w0 = words[t >> lspw];
+                uint32_t w1 = words[(t + 1) >> lspw];
+                int p0 = (int)(t & (spw - 1)) * bits_per_t + shift0;
+                int p1 = (int)((t + 1) & (spw - 1)) * bits_per_t + shift0;
                 z.x = lut2((w0 >> p0) & 3u) * win[n2];
                 z.y = lut2((w1 >> p1) & 3u) * win[n2 + 1];
             }
@@ -275,20 +276,25 @@ __global__ __launch_bounds__(FftShape<M>::NT) void k_fstage(
     __shared__ float2 tw[FX_TWL];
     __shared__ float2 ph_lo[F][PLO];
     __shared__ float2 ph_hi[F][PHI];
+    __shared__ int it_t[F], it_i[F];      // (spectrum, input) of item f; -1 past the end
     const int tid = threadIdx.x;
     twl_load(tw, twg, tid, NT);
     const long nitems = nt * ninp;
     const long g0 = (long)blockIdx.x * F;
+    if (tid < F) {
+        const long g = g0 + tid;
+        const long t = IM ? g % nt : g / ninp;
+        it_t[tid] = g < nitems ? (int)t : -1;
+        it_i[tid] = (int)(IM ? g / nt : g - t * ninp);
+    }
+    __syncthreads();
     if (pm) {
         for (int e = tid; e < F * (PLO + PHI); e += NT) {
             const int f = e / (PLO + PHI);
             const int j = e - f * (PLO + PHI);
-            const long g = g0 + f;
-            if (g >= nitems) continue;
-            const long t = IM ? g % nt : g / ninp;
-            const int i = (int)(IM ? g / nt : g - t * ninp);
-            const fx_phase_model pmi = pm[i];
-            const double tg = (double)(t_first + t);
+            if (it_t[f] < 0) continue;
+            const fx_phase_model pmi = pm[it_i[f]];
+            const double tg = (double)(t_first + it_t[f]);
             const double tau = pmi.delay0 + pmi.delay_rate * tg;
             double cyc;
             if (j < PLO) {
@@ -307,19 +313,18 @@ __global__ __launch_bounds__(FftShape<M>::NT) void k_fstage(
         }
     }
 
-    // ---- load + unpack F transforms
+    // ---- load + unpack F transforms: 16 consecutive complex per thread, one
+    // padded LDS block (zb is a multiple of 16)
     {
         const int wl = WTH * tid;
         const int f = wl / WPT;
         const int wof = wl - f * WPT;
-        const long g = g0 + f;
         uint32_t w[WTH];
 #pragma unroll
         for (int q = 0; q < WTH; q++) w[q] = 0;
-        if (g < nitems) {
-            const long t = IM ? g % nt : g / ninp;
-            const int i = (int)(IM ? g / nt : g - t * ninp);
-            const uint32_t* src = words + (size_t)i * in_stride + (size_t)(t_first + t) * WPT + wof;
+        if (it_t[f] >= 0) {
+            const uint32_t* src =
+                words + (size_t)it_i[f] * in_stride + (size_t)(t_first + it_t[f]) * WPT + wof;
 #pragma unroll
             for (int q = 0; q < WTH; q += 2) {
                 const uint2 ww = *reinterpret_cast<const uint2*>(src + q);
@@ -327,13 +332,12 @@ __global__ __launch_bounds__(FftShape<M>::NT) void k_fstage(
                 w[q + 1] = ww.y;
             }
         }
-        const int zb = f * M + wof * 8;
+        float2* z = sh + lpad(f * M + wof * 8);
 #pragma unroll
         for (int q = 0; q < WTH; q++)
 #pragma unroll
             for (int m = 0; m < 8; m++)
-                sh[lpad(zb + 8 * q + m)] =
-                    make_float2(lut2((w[q] >> (4 * m)) & 3u), lut2((w[q] >> (4 * m + 2)) & 3u));
+                z[8 * q + m] = make_float2(lut2((w[q] >> (4 * m)) & 3u), lut2((w[q] >> (4 * m + 2)) & 3u));
     }
     __syncthreads();
     fft_passes<M, 1, NT, F>(sh, tw, tid);
@@ -344,29 +348,28 @@ __global__ __launch_bounds__(FftShape<M>::NT) void k_fstage(
     constexpr int H = M / 2;
 #pragma unroll
     for (int it = 0; it < (F * H) / NT; it++) {
-        const int q = it * NT + tid;
-        const int f = q / H;
-        const int kp = q - f * H;
+        // H >= NT: item f is fixed per iteration (it * NT / H), no division
+        const int f = H >= NT ? (it * NT) / H : (it * NT + tid) / H;
+        const int kp = H >= NT ? (it * NT) % H + tid : (it * NT + tid) % H;
+        if (it_t[f] < 0) continue;
+        const int i = it_i[f];
         const long g = g0 + f;
-        if (g >= nitems) continue;
-        const long t = IM ? g % nt : g / ninp;
-        const int i = (int)(IM ? g / nt : g - t * ninp);
         float2 xk, xm;
         rsplit<M>(sh, f * M, kp, tw, xk, xm);
         const int k2 = M - kp;
+        const int k2m = k2 & (M - 1);   // kp = 0: k2 = M is the dropped Nyquist bin
         if (pm) {
             xk = cmul(xk, cmul(ph_hi[f][kp / PLO], ph_lo[f][kp % PLO]));
-            const int k2m = k2 & (M - 1);   // kp = 0: k2 = M is the dropped Nyquist bin
             xm = cmul(xm, cmul(ph_hi[f][k2m / PLO], ph_lo[f][k2m % PLO]));
         }
         if (mult) {
             xk = cmul(xk, mult[(size_t)i * M + kp]);
-            xm = cmul(xm, mult[(size_t)i * M + (k2 & (M - 1))]);
+            xm = cmul(xm, mult[(size_t)i * M + k2m]);
         }
         if (sc_chan) {
             const float si = sc_inp[i];
             const float a = sc_chan[kp] * si;
-            const float b = sc_chan[k2 & (M - 1)] * si;
+            const float b = sc_chan[k2m] * si;
             xk.x *= a;
             xk.y *= a;
             xm.x *= b;
@@ -377,22 +380,19 @@ __global__ __launch_bounds__(FftShape<M>::NT) void k_fstage(
         if (kp != 0) row[k2] = to_chan<T>(xm);
     }
     // kp = M/2 of each transform
-    if (tid < F) {
+    if (tid < F && it_t[tid] >= 0) {
         const long g = g0 + tid;
-        if (g < nitems) {
-            const long t = IM ? g % nt : g / ninp;
-            const int i = (int)(IM ? g / nt : g - t * ninp);
-            float2 xk, xm;
-            rsplit<M>(sh, tid * M, M / 2, tw, xk, xm);
-            if (pm) xk = cmul(xk, cmul(ph_hi[tid][(M / 2) / PLO], ph_lo[tid][(M / 2) % PLO]));
-            if (mult) xk = cmul(xk, mult[(size_t)i * M + M / 2]);
-            if (sc_chan) {
-                const float a = sc_chan[M / 2] * sc_inp[i];
-                xk.x *= a;
-                xk.y *= a;
-            }
-            o[(size_t)g * M + M / 2] = to_chan<T>(xk);
+        const int i = it_i[tid];
+        float2 xk, xm;
+        rsplit<M>(sh, tid * M, M / 2, tw, xk, xm);
+        if (pm) xk = cmul(xk, cmul(ph_hi[tid][(M / 2) / PLO], ph_lo[tid][(M / 2) % PLO]));
+        if (mult) xk = cmul(xk, mult[(size_t)i * M + M / 2]);
+        if (sc_chan) {
+            const float a = sc_chan[M / 2] * sc_inp[i];
+            xk.x *= a;
+            xk.y *= a;
         }
+        o[(size_t)g * M + M / 2] = to_chan<T>(xk);
     }
 }
 

@@ -60,10 +60,13 @@ __device__ __forceinline__ float2 cconj(float2 a) { return make_float2(a.x, -a.y
 // multiply by -i
 __device__ __forceinline__ float2 cmul_mi(float2 a) { return make_float2(a.y, -a.x); }
 
-// LDS index padding: one spare complex every 32 keeps the strided Stockham
-// stores of the early passes off a single bank.
-__device__ __forceinline__ int lpad(int i) { return i + (i >> 5); }
-__host__ __device__ constexpr int lds_complex(int n) { return n + (n >> 5); }
+// LDS index padding: one spare complex every 16 keeps the strided Stockham
+// accesses off a single bank, and makes every stride that is a multiple of 16
+// a constant offset: lpad(i + 16 m) = lpad(i) + 17 m.
+__device__ __forceinline__ int lpad(int i) { return i + (i >> 4); }
+__host__ __device__ constexpr int lds_complex(int n) { return n + (n >> 4); }
+// padded distance of a stride that is a multiple of 16 (or 1 inside a block)
+__host__ __device__ constexpr int lstride(int s) { return s + (s >> 4); }
 
 // Offset-binary 2-bit level table {-H, -1, +1, +H}.
 __device__ __forceinline__ float lut2(uint32_t code)
@@ -144,40 +147,88 @@ __device__ __forceinline__ long xcd_block(long nlogical)
     return (b & 7) * per + (b >> 3);
 }
 
+// Twiddles w^1 .. w^15 of one radix-16 butterfly from three table lookups
+// (w, w^4, w^8) and twelve products, at most two products deep from a looked-up
+// value: p[r] = W^{r e}, W = exp(-2 pi i / FX_TW_N).
+__device__ __forceinline__ void tw16_powers(const float2* twl, int e, float2* p)
+{
+    const float2 w1 = twl_get(twl, e & (FX_TW_N - 1));
+    const float2 w4 = twl_get(twl, (4 * e) & (FX_TW_N - 1));
+    const float2 w8 = twl_get(twl, (8 * e) & (FX_TW_N - 1));
+    p[1] = w1;
+    p[2] = cmul(w1, w1);
+    p[3] = cmul(p[2], w1);
+    p[4] = w4;
+    p[5] = cmul(w4, w1);
+    p[6] = cmul(w4, p[2]);
+    p[7] = cmul(w4, p[3]);
+    p[8] = w8;
+    p[12] = cmul(w8, w4);
+#pragma unroll
+    for (int r = 1; r < 4; r++) {
+        p[8 + r] = cmul(w8, p[r]);
+        p[12 + r] = cmul(p[12], p[r]);
+    }
+}
+
 // One radix-R Stockham pass over F complex FFTs of length M held in LDS
 // (padded indexing), NT threads, current sub-transform span P.
 // Butterfly j of FFT f reads x[j + r M/R], multiplies by W_{PR}^{r k}
 // (k = j mod P), does the R-point DFT and stores y[(j-k) R + k + r P].
+// Strides that are multiples of 16 (and the 16 consecutive stores of the
+// first pass) are constant LDS offsets from one padded base.
 template <int M, int R, int P, int NT, int F>
 __device__ __forceinline__ void stockham_pass(float2* sh, const float2* __restrict__ tw, int tid)
 {
     constexpr int NB = F * M / R;
     constexpr int BPT = NB / NT;
     static_assert(NB % NT == 0, "butterflies must divide evenly");
+    constexpr int S = M / R;
+    constexpr bool CL = (S % 16) == 0;
+    constexpr bool CS = (P % 16) == 0 || (P == 1 && R == 16);
     float2 u[BPT][R];
 #pragma unroll
     for (int b = 0; b < BPT; b++) {
         int g = tid + b * NT;
-        int f = g / (M / R);
-        int j = g - f * (M / R);
+        int f = g / S;
+        int j = g - f * S;
+        if constexpr (CL) {
+            const int base = lpad(f * M + j);
 #pragma unroll
-        for (int r = 0; r < R; r++) u[b][r] = sh[lpad(f * M + j + r * (M / R))];
+            for (int r = 0; r < R; r++) u[b][r] = sh[base + r * lstride(S)];
+        } else {
+#pragma unroll
+            for (int r = 0; r < R; r++) u[b][r] = sh[lpad(f * M + j + r * S)];
+        }
     }
     __syncthreads();
 #pragma unroll
     for (int b = 0; b < BPT; b++) {
         int g = tid + b * NT;
-        int f = g / (M / R);
-        int j = g - f * (M / R);
+        int f = g / S;
+        int j = g - f * S;
         int k = j & (P - 1);
         if constexpr (P > 1) {
+            if constexpr (R == 16) {
+                float2 p[16];
+                tw16_powers(tw, k * (FX_TW_N / (P * R)), p);
 #pragma unroll
-            for (int r = 1; r < R; r++) u[b][r] = cmul(u[b][r], twl_get(tw, (r * k) * (FX_TW_N / (P * R))));
+                for (int r = 1; r < R; r++) u[b][r] = cmul(u[b][r], p[r]);
+            } else {
+#pragma unroll
+                for (int r = 1; r < R; r++) u[b][r] = cmul(u[b][r], twl_get(tw, (r * k) * (FX_TW_N / (P * R))));
+            }
         }
         dft_reg<R>(u[b]);
         int base = f * M + (j - k) * R + k;
+        if constexpr (CS) {
+            const int bl = lpad(base);
 #pragma unroll
-        for (int r = 0; r < R; r++) sh[lpad(base + r * P)] = u[b][r];
+            for (int r = 0; r < R; r++) sh[bl + r * lstride(P)] = u[b][r];
+        } else {
+#pragma unroll
+            for (int r = 0; r < R; r++) sh[lpad(base + r * P)] = u[b][r];
+        }
     }
     __syncthreads();
 }
