@@ -642,7 +642,8 @@ struct fx_corr {
     int nchan;
     size_t elem;            // bytes per channelized complex sample
     void* chan = nullptr;   // [c][t][i], chunk_spectra deep
-    void* nat = nullptr;    // F-stage rows [t][i][c] before the corner turn
+    void* nat = nullptr;    // F-stage rows of one sub-chunk before the corner turn
+    long sub = 0;           // spectra per F-stage sub-chunk (nat depth)
     float* acc = nullptr;   // engine-layout accumulators
     size_t acc_floats = 0;
     fx_phase_model* pm = nullptr;
@@ -697,6 +698,14 @@ extern "C" int fx_corr_create(fx_ctx* ctx, const fx_corr_config* cfg, fx_corr** 
     // 2-bit channelized data arrives ready for the engine: no F-stage buffers
     size_t chan_bytes =
         twobit ? 0 : (size_t)c->nchan * (size_t)cfg->chunk_spectra * (size_t)cfg->ninp * c->elem;
+    // The F kernel writes natural rows of a sub-chunk of about 64 MB, which the
+    // corner turn reads back while they are still in the 256 MB Infinity Cache.
+    const size_t row_bytes = (size_t)c->nchan * (size_t)cfg->ninp * c->elem;   // one spectrum of all inputs
+    // (FX_FSTAGE_SUB_MB overrides the 64 MB target, for tuning)
+    const char* sub_env = getenv("FX_FSTAGE_SUB_MB");
+    const size_t sub_target = (size_t)(sub_env ? std::max(1, atoi(sub_env)) : 64) << 20;
+    c->sub = std::max<long>(8, std::min<long>(cfg->chunk_spectra, (long)(sub_target / row_bytes) / 8 * 8));
+    const size_t nat_bytes = twobit ? 0 : row_bytes * (size_t)c->sub;
     c->acc_floats = fxk_xengine_acc_floats(c->nchan, cfg->ninp, c->nsets);
     long P = (long)cfg->ninp * (cfg->ninp + 1) / 2;
     std::vector<int2> hp((size_t)P);
@@ -706,7 +715,7 @@ extern "C" int fx_corr_create(fx_ctx* ctx, const fx_corr_config* cfg, fx_corr** 
               hipMalloc(&c->pm, sizeof(fx_phase_model) * (size_t)cfg->ninp) == hipSuccess &&
               hipMalloc(&c->pij, sizeof(int2) * (size_t)P) == hipSuccess;
     if (ok && chan_bytes)
-        ok = hipMalloc(&c->chan, chan_bytes) == hipSuccess && hipMalloc(&c->nat, chan_bytes) == hipSuccess;
+        ok = hipMalloc(&c->chan, chan_bytes) == hipSuccess && hipMalloc(&c->nat, nat_bytes) == hipSuccess;
     if (ok) ok = hipMemcpy(c->pij, hp.data(), sizeof(int2) * (size_t)P, hipMemcpyHostToDevice) == hipSuccess;
     if (ok) ok = hipMemsetAsync(c->acc, 0, c->acc_floats * sizeof(float), ctx->stream) == hipSuccess;
     if (ok) ok = hipMemsetAsync(c->pm, 0, sizeof(fx_phase_model) * (size_t)cfg->ninp, ctx->stream) == hipSuccess;
@@ -817,13 +826,18 @@ extern "C" int fx_corr_fstage(fx_corr* c, const uint32_t* d_words, size_t in_str
     FX_REQUIRE(out_prec != FX_PREC_FP8 || nspectra % 8 == 0, FX_ERR_ARG,
                "fx_corr_fstage: FP8 output needs a multiple of 8 spectra");
     if (nspectra == 0) return FX_OK;
-    const bool scaled = out_prec != FX_PREC_F32;
-    FX_CHECK_LAUNCH(fxk_fstage(c->ctx->stream, c->ctx->tw, d_words, in_stride_words, c->cfg.ninp,
-                               c->cfg.nfft, t_first, nspectra, c->cfg.use_phase ? c->pm : nullptr,
-                               c->mult, scaled ? c->sc_chan : nullptr, scaled ? c->sc_inp : nullptr,
-                               out_prec, c->nat, d_chan));
+    // sub-chunks: F kernel into the natural-row buffer, corner turn into d_chan
+    for (long s0 = 0; s0 < nspectra; s0 += c->sub) {
+        const long ns = std::min(c->sub, nspectra - s0);
+        rc = fx_corr_fstage_rows(c, d_words, in_stride_words, t_first + s0, ns, out_prec);
+        if (rc) return rc;
+        rc = fx_corr_cornerturn(c, ns, s0, nspectra, out_prec, d_chan);
+        if (rc) return rc;
+    }
     return FX_OK;
 }
+
+extern "C" long fx_corr_fstage_depth(fx_corr* c) { return c ? c->sub : 0; }
 
 extern "C" int fx_corr_fstage_rows(fx_corr* c, const uint32_t* d_words, size_t in_stride_words,
                                    long t_first, long nspectra, int out_prec)
@@ -835,8 +849,8 @@ extern "C" int fx_corr_fstage_rows(fx_corr* c, const uint32_t* d_words, size_t i
                    out_prec != FX_PREC_FP8_2BIT,
                FX_ERR_ARG, "fx_corr_fstage_rows: output precision %d", out_prec);
     FX_REQUIRE(in_stride_words % 2 == 0 && ((uintptr_t)d_words % 8) == 0 && t_first >= 0 && nspectra >= 0 &&
-                   nspectra <= c->cfg.chunk_spectra,
-               FX_ERR_ARG, "fx_corr_fstage_rows: bad range or alignment");
+                   nspectra <= c->sub,
+               FX_ERR_ARG, "fx_corr_fstage_rows: bad range or alignment (at most %ld spectra)", c->sub);
     if (nspectra == 0) return FX_OK;
     const bool scaled = out_prec != FX_PREC_F32;
     FX_CHECK_LAUNCH(fxk_fstage(c->ctx->stream, c->ctx->tw, d_words, in_stride_words, c->cfg.ninp,
@@ -846,15 +860,20 @@ extern "C" int fx_corr_fstage_rows(fx_corr* c, const uint32_t* d_words, size_t i
     return FX_OK;
 }
 
-extern "C" int fx_corr_cornerturn(fx_corr* c, long nspectra, int out_prec, void* d_chan)
+extern "C" int fx_corr_cornerturn(fx_corr* c, long nspectra, long s0, long chunk_spectra, int out_prec,
+                                  void* d_chan)
 {
     FX_REQUIRE(c && d_chan, FX_ERR_ARG, "fx_corr_cornerturn: null argument");
     int rc = ctx_device(c->ctx);
     if (rc) return rc;
-    FX_REQUIRE(c->nat != nullptr && nspectra >= 0 && nspectra <= c->cfg.chunk_spectra, FX_ERR_ARG,
-               "fx_corr_cornerturn: bad range");
+    FX_REQUIRE(c->nat != nullptr && nspectra >= 0 && nspectra <= c->sub && s0 >= 0 &&
+                   s0 + nspectra <= chunk_spectra && chunk_spectra <= c->cfg.chunk_spectra,
+               FX_ERR_ARG, "fx_corr_cornerturn: bad range");
     if (nspectra == 0) return FX_OK;
-    FX_CHECK_LAUNCH(fxk_cornerturn(c->ctx->stream, out_prec, c->nat, d_chan, nspectra * c->cfg.ninp, c->nchan));
+    const long I = c->cfg.ninp;
+    const bool imajor = out_prec == FX_PREC_FP8;
+    FX_CHECK_LAUNCH(fxk_cornerturn(c->ctx->stream, out_prec, c->nat, d_chan, nspectra * I, c->nchan,
+                                   chunk_spectra * I, imajor ? s0 : s0 * I, (int)nspectra, chunk_spectra));
     return FX_OK;
 }
 

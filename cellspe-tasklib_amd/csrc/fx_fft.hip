@@ -396,13 +396,17 @@ __global__ __launch_bounds__(FftShape<M>::NT) void k_fstage(
     }
 }
 
-// Corner turn nat[g][c] -> out[c][g] for g < nitems, c < C, through a
-// 64 x 64 LDS tile: both the 256-thread reads (rows of nat) and the writes
-// (rows of out) are contiguous runs of 64 elements.  XCD-aware: tiles of
-// neighbouring items share an XCD.
+// Corner turn of one sub-chunk of F-stage rows nat[g][c] (g < nitems) into
+// the chunk-wide engine layout out[c][col], through a 64 x 64 LDS tile: both
+// the 256-thread reads (rows of nat) and the writes (rows of out) are
+// contiguous runs of 64 elements.  Row c of out holds ostride columns.
+// t-major items (g = t'*I + i) land at col = col0 + g; input-major items
+// (g = i*ns + t') at col = i*ostride_i + col0 + t' (ostride_i = spectra per
+// chunk).  XCD-aware: tiles of neighbouring items share an XCD.
 template <typename V>
 __global__ __launch_bounds__(256) void k_cornerturn(const V* __restrict__ nat, V* __restrict__ out,
-                                                    long nitems, int C)
+                                                    long nitems, int C, long ostride, long col0,
+                                                    int imajor, int ns, long ostride_i)
 {
     __shared__ V tile[64][65];
     const long ntg = (nitems + 63) / 64;
@@ -420,10 +424,11 @@ __global__ __launch_bounds__(256) void k_cornerturn(const V* __restrict__ nat, V
         if (g < nitems) tile[r][lx] = nat[(size_t)g * C + cbase + lx];
     }
     __syncthreads();
+    const long g = gbase + lx;
+    const long col = imajor ? (g / ns) * ostride_i + col0 + (g % ns) : col0 + g;
 #pragma unroll
     for (int r = ly; r < 64; r += 4) {
-        long g = gbase + lx;
-        if (g < nitems) out[(size_t)(cbase + r) * nitems + g] = tile[lx][r];
+        if (g < nitems) out[(size_t)(cbase + r) * ostride + col] = tile[lx][r];
     }
 }
 
@@ -466,11 +471,13 @@ int launch_spec(hipStream_t s, const float2* tw, const float* win, const uint32_
 }
 
 template <typename V>
-int launch_cornerturn_t(hipStream_t s, const void* nat, void* out, long nitems, int M)
+int launch_cornerturn_t(hipStream_t s, const void* nat, void* out, long nitems, int M, long ostride, long col0,
+                        int imajor, int ns, long ostride_i)
 {
     long ntiles = ((nitems + 63) / 64) * (M / 64);
     hipLaunchKernelGGL((k_cornerturn<V>), dim3((unsigned)xcd_grid(ntiles)), dim3(256), 0, s,
-                       reinterpret_cast<const V*>(nat), reinterpret_cast<V*>(out), nitems, M);
+                       reinterpret_cast<const V*>(nat), reinterpret_cast<V*>(out), nitems, M, ostride, col0,
+                       imajor, ns, ostride_i);
     return hipGetLastError() == hipSuccess ? FX_OK : FX_ERR_HIP;
 }
 
@@ -488,7 +495,8 @@ int launch_fstage_t(hipStream_t s, const float2* tw, const uint32_t* words, size
                        ninp, t_first, nt, pm, mult, sc, si, reinterpret_cast<T*>(nat));
     if (hipGetLastError() != hipSuccess) return FX_ERR_HIP;
     if (!out) return FX_OK;
-    return launch_cornerturn_t<V>(s, nat, out, nitems, M);
+    return launch_cornerturn_t<V>(s, nat, out, nitems, M, nitems, 0, std::is_same<T, uint16_t>::value ? 1 : 0,
+                                  (int)nt, nt);
 }
 
 template <int M>
@@ -589,13 +597,17 @@ int fxk_fstage(hipStream_t s, const float2* tw, const uint32_t* words, size_t st
     }
 }
 
-int fxk_cornerturn(hipStream_t s, int out_type, const void* nat, void* out, long nitems, int nchan)
+int fxk_cornerturn(hipStream_t s, int out_type, const void* nat, void* out, long nitems, int nchan,
+                   long ostride, long col0, int ns, long ostride_i)
 {
     if (nchan % 64) return FX_ERR_ARG;
     switch (out_type) {
-    case FX_PREC_F32: return launch_cornerturn_t<float2>(s, nat, out, nitems, nchan);
-    case FX_PREC_F16: return launch_cornerturn_t<uint32_t>(s, nat, out, nitems, nchan);
-    case FX_PREC_FP8: return launch_cornerturn_t<uint16_t>(s, nat, out, nitems, nchan);
+    case FX_PREC_F32:
+        return launch_cornerturn_t<float2>(s, nat, out, nitems, nchan, ostride, col0, 0, ns, ostride_i);
+    case FX_PREC_F16:
+        return launch_cornerturn_t<uint32_t>(s, nat, out, nitems, nchan, ostride, col0, 0, ns, ostride_i);
+    case FX_PREC_FP8:
+        return launch_cornerturn_t<uint16_t>(s, nat, out, nitems, nchan, ostride, col0, 1, ns, ostride_i);
     default: return FX_ERR_ARG;
     }
 }

@@ -285,9 +285,12 @@ int fxk_fstage(hipStream_t s, const float2* tw, const uint32_t* words, size_t in
                int ninp, int n, long t_first, long nt, const fx_phase_model* pm,
                const float2* mult, const float* scale_chan, const float* scale_inp, int out_type,
                void* nat, void* out);
-// corner turn of F-stage rows nat[g][c] -> out[c][g] (fxk_fstage with
+// corner turn of one sub-chunk of F-stage rows nat[g][c] (nitems items of ns
+// spectra) into the chunk layout: FP32/FP16 out[c][col0 + g] with rows of
+// ostride items, FP8 out[c][i*ostride_i + col0 + t'] (fxk_fstage with
 // out == nullptr stops after the F kernel)
-int fxk_cornerturn(hipStream_t s, int out_type, const void* nat, void* out, long nitems, int nchan);
+int fxk_cornerturn(hipStream_t s, int out_type, const void* nat, void* out, long nitems, int nchan,
+                   long ostride, long col0, int ns, long ostride_i);
 int fxk_xengine(hipStream_t s, int in_type, const void* X, int nchan, long nt, int ninp,
                 float* acc);
 // 8-bit X-engine on input-major channelized data X[c][i][t]: FX_PREC_FP8

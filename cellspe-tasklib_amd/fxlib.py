@@ -99,7 +99,8 @@ def load():
         "fx_corr_reset": (i, [vp]),
         "fx_corr_fstage": (i, [vp, vp, sz, l, l, vp, i]),
         "fx_corr_fstage_rows": (i, [vp, vp, sz, l, l, i]),
-        "fx_corr_cornerturn": (i, [vp, l, i, vp]),
+        "fx_corr_cornerturn": (i, [vp, l, l, l, i, vp]),
+        "fx_corr_fstage_depth": (l, [vp]),
         "fx_corr_info": (i, [vp, C.POINTER(l), C.POINTER(sz), C.POINTER(sz)]),
         "fx_mfma_probe": (i, [vp, i, vp]),
     }
@@ -335,8 +336,11 @@ class Correlator:
         check(load().fx_corr_fstage_rows(self.handle, d_words.ptr, stride_words, t_first, nspectra,
                                          out_prec))
 
-    def cornerturn(self, nspectra, out_prec, d_chan):
-        check(load().fx_corr_cornerturn(self.handle, nspectra, out_prec, d_chan.ptr))
+    def cornerturn(self, nspectra, s0, chunk_spectra, out_prec, d_chan):
+        check(load().fx_corr_cornerturn(self.handle, nspectra, s0, chunk_spectra, out_prec, d_chan.ptr))
+
+    def fstage_depth(self):
+        return load().fx_corr_fstage_depth(self.handle)
 
     def xengine(self, d_chan, nspectra):
         check(load().fx_corr_xengine(self.handle, d_chan.ptr, nspectra))

@@ -200,12 +200,18 @@ int fx_corr_reset(fx_corr* corr);
  * (FP32/F16 [c][t][i], FP8 [c][i][t]) */
 int fx_corr_fstage(fx_corr* corr, const uint32_t* d_words, size_t in_stride_words,
                    long t_first, long nspectra, void* d_chan, int out_prec);
-/* the two launches of fx_corr_fstage one at a time (per-kernel timing): the
- * transform kernel into the correlator's natural-row buffer, then the corner
- * turn of those rows into d_chan */
+/* fx_corr_fstage works in sub-chunks of fx_corr_fstage_depth() spectra (about
+ * 64 MB of rows, so the corner turn reads them back from the Infinity Cache):
+ * per sub-chunk, the transform kernel writes natural rows into the
+ * correlator's row buffer and the corner turn moves them into d_chan.  The two
+ * launches are exposed one at a time for per-kernel timing: rows of spectra
+ * [t_first, t_first + nspectra), nspectra <= depth, then their corner turn to
+ * spectra [s0, s0 + nspectra) of a chunk of chunk_spectra spectra in d_chan. */
+long fx_corr_fstage_depth(fx_corr* corr);
 int fx_corr_fstage_rows(fx_corr* corr, const uint32_t* d_words, size_t in_stride_words,
                         long t_first, long nspectra, int out_prec);
-int fx_corr_cornerturn(fx_corr* corr, long nspectra, int out_prec, void* d_chan);
+int fx_corr_cornerturn(fx_corr* corr, long nspectra, long s0, long chunk_spectra, int out_prec,
+                       void* d_chan);
 int fx_corr_info(fx_corr* corr, long* chunk_spectra, size_t* chan_bytes, size_t* acc_bytes);
 
 /* ---- diagnostics --------------------------------------------------------- */
