@@ -580,9 +580,35 @@ extern "C" int fx_pcal_clear(fx_pcal* p)
     FX_REQUIRE(p, FX_ERR_ARG, "null pcal");
     int rc = ctx_device(p->ctx);
     if (rc) return rc;
-    FX_HIP_TRY(hipMemsetAsync(p->acc, 0, sizeof(double) * (size_t)p->nfold, p->ctx->stream));
+    // fill kernel rather than a memset node (graph-capturable)
+    FX_CHECK_LAUNCH(fxk_fill32(p->ctx->stream, p->acc, 0u, 2 * (size_t)p->nfold));
     p->samplecount = 0;
     p->finalized = false;
+    return FX_OK;
+}
+
+extern "C" int fx_pcal_final_async(fx_pcal* p)
+{
+    FX_REQUIRE(p, FX_ERR_ARG, "null pcal");
+    int rc = ctx_device(p->ctx);
+    if (rc) return rc;
+    FX_CHECK_LAUNCH(fxk_pcal_final(p->ctx->stream, p->acc, p->nfold, p->nbins, p->ntones, p->offset_bin,
+                                   p->step, p->rotatorlen, p->dphi, p->tones));
+    p->finalized = true;
+    return FX_OK;
+}
+
+extern "C" int fx_pcal_tones(fx_pcal* p, fx_cf64* tones, uint64_t* nsamples)
+{
+    FX_REQUIRE(p && tones, FX_ERR_ARG, "fx_pcal_tones: null argument");
+    int rc = ctx_device(p->ctx);
+    if (rc) return rc;
+    FX_REQUIRE(p->finalized, FX_ERR_STATE, "fx_pcal_tones: no tones extracted yet");
+    if (p->ntones > 0)
+        FX_HIP_TRY(hipMemcpyAsync(tones, p->tones, sizeof(double) * 2 * (size_t)p->ntones, hipMemcpyDeviceToHost,
+                                  p->ctx->stream));
+    FX_HIP_TRY(hipStreamSynchronize(p->ctx->stream));
+    if (nsamples) *nsamples = p->samplecount;
     return FX_OK;
 }
 
@@ -653,6 +679,7 @@ struct fx_corr {
     int2* pij = nullptr;
     float* coef = nullptr;  // FP8_2BIT: weights of the three accumulator sets
     int nsets = 1;
+    int xtile = 128;        // X-engine input tile (128, or 64 for FP32/FP16 engines)
     long spectra_done = 0;
 };
 
@@ -660,7 +687,7 @@ struct fx_corr {
 static int corr_engine(fx_corr* c, const void* d_chan, long nt)
 {
     if (c->cfg.prec == FX_PREC_F32 || c->cfg.prec == FX_PREC_F16)
-        return fxk_xengine(c->ctx->stream, c->cfg.prec, d_chan, c->nchan, nt, c->cfg.ninp, c->acc);
+        return fxk_xengine(c->ctx->stream, c->cfg.prec, c->xtile, d_chan, c->nchan, nt, c->cfg.ninp, c->acc);
     return fxk_xengine8(c->ctx->stream, c->cfg.prec, d_chan, c->nchan, nt, c->cfg.ninp, c->acc);
 }
 
@@ -706,7 +733,8 @@ extern "C" int fx_corr_create(fx_ctx* ctx, const fx_corr_config* cfg, fx_corr** 
     const size_t sub_target = (size_t)(sub_env ? std::max(1, atoi(sub_env)) : 64) << 20;
     c->sub = std::max<long>(8, std::min<long>(cfg->chunk_spectra, (long)(sub_target / row_bytes) / 8 * 8));
     const size_t nat_bytes = twobit ? 0 : row_bytes * (size_t)c->sub;
-    c->acc_floats = fxk_xengine_acc_floats(c->nchan, cfg->ninp, c->nsets);
+    c->xtile = 128;
+    c->acc_floats = fxk_xengine_acc_floats(c->nchan, cfg->ninp, c->nsets, c->xtile);
     long P = (long)cfg->ninp * (cfg->ninp + 1) / 2;
     std::vector<int2> hp((size_t)P);
     for (int i = 0, p = 0; i < cfg->ninp; i++)
@@ -920,7 +948,7 @@ extern "C" int fx_corr_dump(fx_corr* c, fx_cf32* d_vis)
     int rc = ctx_device(c->ctx);
     if (rc) return rc;
     const bool scaled = c->cfg.prec == FX_PREC_F16 || c->cfg.prec == FX_PREC_FP8;
-    FX_CHECK_LAUNCH(fxk_xengine_dump_pij(c->ctx->stream, c->acc, c->nchan, c->cfg.ninp, c->nsets, c->coef,
+    FX_CHECK_LAUNCH(fxk_xengine_dump_pij(c->ctx->stream, c->acc, c->nchan, c->cfg.ninp, c->xtile, c->nsets, c->coef,
                                          scaled ? c->sc_chan : nullptr, scaled ? c->sc_inp : nullptr,
                                          c->pij, 0, (float2*)d_vis));
     return FX_OK;
@@ -931,8 +959,51 @@ extern "C" int fx_corr_reset(fx_corr* c)
     FX_REQUIRE(c, FX_ERR_ARG, "null correlator");
     int rc = ctx_device(c->ctx);
     if (rc) return rc;
-    FX_HIP_TRY(hipMemsetAsync(c->acc, 0, c->acc_floats * sizeof(float), c->ctx->stream));
+    // a fill kernel, not a memset node: the reset may be captured in a graph
+    FX_CHECK_LAUNCH(fxk_fill32(c->ctx->stream, c->acc, 0u, c->acc_floats));
     c->spectra_done = 0;
+    return FX_OK;
+}
+
+extern "C" int fx_corr_tune(fx_corr* c, long sub_spectra, int xtile)
+{
+    FX_REQUIRE(c, FX_ERR_ARG, "null correlator");
+    int rc = ctx_device(c->ctx);
+    if (rc) return rc;
+    FX_REQUIRE(xtile == 0 || xtile == 128 || (xtile == 64 && c->nsets == 1 && c->cfg.prec <= FX_PREC_F16),
+               FX_ERR_ARG, "fx_corr_tune: X tile %d (128, or 64 for the FP32/FP16 engines)", xtile);
+    FX_REQUIRE(sub_spectra == 0 || (sub_spectra >= 8 && sub_spectra % 8 == 0), FX_ERR_ARG,
+               "fx_corr_tune: sub-chunk of %ld spectra", sub_spectra);
+    FX_HIP_TRY(hipStreamSynchronize(c->ctx->stream));
+    if (xtile && xtile != c->xtile) {
+        const size_t n = fxk_xengine_acc_floats(c->nchan, c->cfg.ninp, c->nsets, xtile);
+        float* a = nullptr;
+        if (hipMalloc(&a, n * sizeof(float)) != hipSuccess) {
+            hipGetLastError();
+            fx_set_error("fx_corr_tune: accumulator allocation failed");
+            return FX_ERR_NOMEM;
+        }
+        hipFree(c->acc);
+        c->acc = a;
+        c->acc_floats = n;
+        c->xtile = xtile;
+        FX_CHECK_LAUNCH(fxk_fill32(c->ctx->stream, c->acc, 0u, c->acc_floats));
+    }
+    if (sub_spectra && c->nat) {
+        const long sub = std::min(sub_spectra, c->cfg.chunk_spectra);
+        if (sub > c->sub) {
+            void* nat = nullptr;
+            const size_t bytes = (size_t)c->nchan * (size_t)c->cfg.ninp * c->elem * (size_t)sub;
+            if (hipMalloc(&nat, bytes) != hipSuccess) {
+                hipGetLastError();
+                fx_set_error("fx_corr_tune: row buffer allocation failed");
+                return FX_ERR_NOMEM;
+            }
+            hipFree(c->nat);
+            c->nat = nat;
+        }
+        c->sub = sub;
+    }
     return FX_OK;
 }
 

@@ -291,7 +291,8 @@ int fxk_fstage(hipStream_t s, const float2* tw, const uint32_t* words, size_t in
 // out == nullptr stops after the F kernel)
 int fxk_cornerturn(hipStream_t s, int out_type, const void* nat, void* out, long nitems, int nchan,
                    long ostride, long col0, int ns, long ostride_i);
-int fxk_xengine(hipStream_t s, int in_type, const void* X, int nchan, long nt, int ninp,
+// FP32/FP16 engine on [c][t][i], ts x ts input tiles (ts = 128 or 64)
+int fxk_xengine(hipStream_t s, int in_type, int ts, const void* X, int nchan, long nt, int ninp,
                 float* acc);
 // 8-bit X-engine on input-major channelized data X[c][i][t]: FX_PREC_FP8
 // (e4m3 re|im<<8 pairs, one accumulator set) or FX_PREC_FP8_2BIT (2-bit
@@ -300,8 +301,9 @@ int fxk_xengine8(hipStream_t s, int prec, const void* X, int nchan, long nt, int
 // 2-bit channelized codes [c][t][i] or [c][i][t] -> [c][t][i] FP32/FP16 samples
 int fxk_codes_to_chan(hipStream_t s, const uint8_t* codes, int in_imajor, int nchan, long nt, int ninp,
                       int out_prec, void* out);
-size_t fxk_xengine_acc_floats(int nchan, int ninp, int nsets);
-int fxk_xengine_dump_pij(hipStream_t s, const float* acc, int nchan, int ninp, int nsets,
+// engine image size for ts x ts tiles (the 8-bit engines use ts = 128)
+size_t fxk_xengine_acc_floats(int nchan, int ninp, int nsets, int ts);
+int fxk_xengine_dump_pij(hipStream_t s, const float* acc, int nchan, int ninp, int ts, int nsets,
                          const float* set_coef, const float* scale_chan, const float* scale_inp,
                          const int2* pij, int accumulate, float2* vis);
 int fxk_mfma_probe(hipStream_t s, int which, float* out);
@@ -316,6 +318,8 @@ int fxk_sincos_gen(hipStream_t s, double start, double inc, float2* out, long n)
 int fxk_copy(hipStream_t s, const void* src, void* dst, size_t bytes);
 int fxk_complex_add(hipStream_t s, const float2* a, const float2* b, float2* out, long n);
 int fxk_iotest(hipStream_t s, float* out);
+// fill n 32-bit words with value (the graph-capturable clear)
+int fxk_fill32(hipStream_t s, void* p, uint32_t value, size_t n);
 int fxk_difx_post(hipStream_t s, const float2* fft, int nfft, int band0, int n, const float2* mult,
                   float2* out, float2* conj_out, float2* acc);
 

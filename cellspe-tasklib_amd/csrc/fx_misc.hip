@@ -321,6 +321,13 @@ __global__ void k_difx_post(const float2* __restrict__ fft, int nfft, int band0,
     acc[k].x += y.x * y.x + y.y * y.y;
 }
 
+// grid-stride 32-bit fill (clears that may be captured in a graph)
+__global__ void k_fill32(uint32_t* __restrict__ p, uint32_t value, size_t n)
+{
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        p[i] = value;
+}
+
 __global__ void k_iotest(float* out)
 {
     if (threadIdx.x == 0) {
@@ -499,6 +506,14 @@ int fxk_difx_post(hipStream_t s, const float2* fft, int nfft, int band0, int n, 
     if (n <= 0) return FX_OK;
     hipLaunchKernelGGL(k_difx_post, dim3(nblocks((size_t)n, 256)), dim3(256), 0, s, fft, nfft, band0, n,
                        mult, out, conj_out, acc);
+    return last_status();
+}
+
+int fxk_fill32(hipStream_t s, void* p, uint32_t value, size_t n)
+{
+    if (n == 0) return FX_OK;
+    hipLaunchKernelGGL(k_fill32, dim3(nblocks_gs(n, 256)), dim3(256), 0, s, reinterpret_cast<uint32_t*>(p), value,
+                       n);
     return last_status();
 }
 

@@ -30,14 +30,15 @@ typedef int fx_int8 __attribute__((ext_vector_type(8)));
 
 namespace {
 
-constexpr int TS = 128;     // tile rows
+constexpr int TS = 128;     // tile rows of the 8-bit engine (and default of the others)
 constexpr int XNT = 256;    // threads per workgroup
 
-__host__ __device__ inline int ntiles128(int ninp)
+__host__ __device__ inline int ntiles_ts(int ninp, int ts)
 {
-    int nb = (ninp + TS - 1) / TS;
+    int nb = (ninp + ts - 1) / ts;
     return nb * (nb + 1) / 2;
 }
+__host__ __device__ inline int ntiles128(int ninp) { return ntiles_ts(ninp, TS); }
 
 // tile index -> (ti, tj), row-major over the upper triangle
 __device__ inline void tile_coords(int tile, int nb, int& bi, int& bj)
@@ -60,26 +61,32 @@ __device__ __forceinline__ uint32_t rot_h2(uint32_t v)
 template <int PREC> struct XTraits;
 template <> struct XTraits<FX_PREC_F32> {
     typedef float2 E;                // one complex sample
-    static constexpr int KT = 16;    // samples per LDS stage (16 KiB per slab)
+    static constexpr int KT = 16;    // samples per LDS stage (16 KiB per slab at TSZ = 128)
 };
 template <> struct XTraits<FX_PREC_F16> {
     typedef uint32_t E;
     static constexpr int KT = 32;
 };
 
-template <int PREC>
+// TSZ x TSZ input tiles (TSZ = 128 or 64); wave w = (qi, qj) owns the
+// (TSZ/2) x (TSZ/2) quadrant as NBW x NBW blocks of 32 x 32, NBW = TSZ/64.
+// Engine image: acc[c][tile][block][reg][lane] with block = w NBW^2 + a NBW + b.
+template <int PREC, int TSZ>
 __global__ __launch_bounds__(XNT, 2) void k_xengine(const typename XTraits<PREC>::E* __restrict__ X,
                                                     int nchan, long nt, int ninp,
                                                     float* __restrict__ acc)
 {
     typedef typename XTraits<PREC>::E E;
     constexpr int KT = XTraits<PREC>::KT;
-    constexpr int CH = 16 / sizeof(E);                   // samples per 16-byte chunk
-    constexpr int SLAB_CHUNKS = KT * TS / CH;            // chunks per slab
-    constexpr int LPT = SLAB_CHUNKS / XNT;               // chunks per thread per slab
-    __shared__ E slab[2][KT][TS];
+    constexpr int NBW = TSZ / 64;                         // blocks per wave per dimension
+    constexpr int NBT = (TSZ / 32) * (TSZ / 32);          // blocks per tile
+    constexpr int CH = 16 / sizeof(E);                    // samples per 16-byte chunk
+    constexpr int SLAB_CHUNKS = KT * TSZ / CH;            // chunks per slab
+    constexpr int LPT = SLAB_CHUNKS / XNT;                // chunks per thread per slab
+    static_assert(LPT >= 1, "slab smaller than the workgroup");
+    __shared__ E slab[2][KT][TSZ];
 
-    const int nb = (ninp + TS - 1) / TS;
+    const int nb = (ninp + TSZ - 1) / TSZ;
     const int ntl = nb * (nb + 1) / 2;
     const long lb = xcd_block((long)ntl * nchan);
     if (lb >= (long)ntl * nchan) return;
@@ -94,19 +101,19 @@ __global__ __launch_bounds__(XNT, 2) void k_xengine(const typename XTraits<PREC>
     const int qi = w >> 1, qj = w & 1;
     const bool active = !(diag && qi > qj);
     const E* base = X + (size_t)c * (size_t)nt * (size_t)ninp;
-    const int row0[2] = {ti * TS, tj * TS};
+    const int row0[2] = {ti * TSZ, tj * TSZ};
 
-    fx_float16 dre[2][2], dim[2][2];
+    fx_float16 dre[NBW][NBW], dim[NBW][NBW];
 #pragma unroll
-    for (int a = 0; a < 2; a++)
+    for (int a = 0; a < NBW; a++)
 #pragma unroll
-        for (int b = 0; b < 2; b++) {
+        for (int b = 0; b < NBW; b++) {
             dre[a][b] = fx_float16{};
             dim[a][b] = fx_float16{};
         }
 
     // global -> register prefetch of one stage: chunk q of slab s covers
-    // sample t0 + q / (TS/CH), rows (q % (TS/CH)) * CH .. + CH
+    // sample t0 + q / (TSZ/CH), rows (q % (TSZ/CH)) * CH .. + CH
     uint4 pre[2][LPT];
     auto gload = [&](long t0) {
 #pragma unroll
@@ -115,8 +122,8 @@ __global__ __launch_bounds__(XNT, 2) void k_xengine(const typename XTraits<PREC>
 #pragma unroll
             for (int k = 0; k < LPT; k++) {
                 const int q = tid + k * XNT;
-                const int tt = q / (TS / CH);
-                const int rr = (q - tt * (TS / CH)) * CH;
+                const int tt = q / (TSZ / CH);
+                const int rr = (q - tt * (TSZ / CH)) * CH;
                 const long t = t0 + tt;
                 const int row = row0[s] + rr;
                 uint4 v = make_uint4(0u, 0u, 0u, 0u);
@@ -144,8 +151,8 @@ __global__ __launch_bounds__(XNT, 2) void k_xengine(const typename XTraits<PREC>
 #pragma unroll
             for (int k = 0; k < LPT; k++) {
                 const int q = tid + k * XNT;
-                const int tt = q / (TS / CH);
-                const int rr = (q - tt * (TS / CH)) * CH;
+                const int tt = q / (TSZ / CH);
+                const int rr = (q - tt * (TSZ / CH)) * CH;
                 *reinterpret_cast<uint4*>(&slab[s][tt][rr]) = pre[s][k];
             }
         }
@@ -164,19 +171,19 @@ __global__ __launch_bounds__(XNT, 2) void k_xengine(const typename XTraits<PREC>
         if constexpr (PREC == FX_PREC_F16) {
 #pragma unroll
             for (int ks = 0; ks < KT / 8; ks++) {
-                fx_half8 fa[2], fb[2], fr[2];
+                fx_half8 fa[NBW], fb[NBW], fr[NBW];
 #pragma unroll
-                for (int a = 0; a < 2; a++) {
+                for (int a = 0; a < NBW; a++) {
                     fx_int4v ia;
-                    const int row = (2 * qi + a) * 32 + r32;
+                    const int row = (NBW * qi + a) * 32 + r32;
 #pragma unroll
                     for (int q = 0; q < 4; q++) ia[q] = (int)slab[0][ks * 8 + 4 * h + q][row];
                     fa[a] = __builtin_bit_cast(fx_half8, ia);
                 }
 #pragma unroll
-                for (int b = 0; b < 2; b++) {
+                for (int b = 0; b < NBW; b++) {
                     fx_int4v ib, ir;
-                    const int row = (2 * qj + b) * 32 + r32;
+                    const int row = (NBW * qj + b) * 32 + r32;
 #pragma unroll
                     for (int q = 0; q < 4; q++) {
                         uint32_t v = slab[sb][ks * 8 + 4 * h + q][row];
@@ -187,9 +194,9 @@ __global__ __launch_bounds__(XNT, 2) void k_xengine(const typename XTraits<PREC>
                     fr[b] = __builtin_bit_cast(fx_half8, ir);
                 }
 #pragma unroll
-                for (int a = 0; a < 2; a++)
+                for (int a = 0; a < NBW; a++)
 #pragma unroll
-                    for (int b = 0; b < 2; b++) {
+                    for (int b = 0; b < NBW; b++) {
                         if (diag && qi == qj && a > b) continue;
                         dre[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_f16(fa[a], fb[b], dre[a][b], 0, 0, 0);
                         dim[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_f16(fa[a], fr[b], dim[a][b], 0, 0, 0);
@@ -200,22 +207,22 @@ __global__ __launch_bounds__(XNT, 2) void k_xengine(const typename XTraits<PREC>
             // carry the real parts, lanes 32-63 the imaginary parts
 #pragma unroll
             for (int ks = 0; ks < KT; ks++) {
-                float fa[2], fb[2], fr[2];
+                float fa[NBW], fb[NBW], fr[NBW];
 #pragma unroll
-                for (int a = 0; a < 2; a++) {
-                    const float2 v = slab[0][ks][(2 * qi + a) * 32 + r32];
+                for (int a = 0; a < NBW; a++) {
+                    const float2 v = slab[0][ks][(NBW * qi + a) * 32 + r32];
                     fa[a] = h ? v.y : v.x;
                 }
 #pragma unroll
-                for (int b = 0; b < 2; b++) {
-                    const float2 v = slab[sb][ks][(2 * qj + b) * 32 + r32];
+                for (int b = 0; b < NBW; b++) {
+                    const float2 v = slab[sb][ks][(NBW * qj + b) * 32 + r32];
                     fb[b] = h ? v.y : v.x;
                     fr[b] = h ? v.x : -v.y;
                 }
 #pragma unroll
-                for (int a = 0; a < 2; a++)
+                for (int a = 0; a < NBW; a++)
 #pragma unroll
-                    for (int b = 0; b < 2; b++) {
+                    for (int b = 0; b < NBW; b++) {
                         if (diag && qi == qj && a > b) continue;
                         dre[a][b] = __builtin_amdgcn_mfma_f32_32x32x2f32(fa[a], fb[b], dre[a][b], 0, 0, 0);
                         dim[a][b] = __builtin_amdgcn_mfma_f32_32x32x2f32(fa[a], fr[b], dim[a][b], 0, 0, 0);
@@ -224,13 +231,13 @@ __global__ __launch_bounds__(XNT, 2) void k_xengine(const typename XTraits<PREC>
         }
     }
     if (!active) return;
-    float* dst = acc + (((size_t)c * ntl + tile) * 16 + w * 4) * (32 * 64);
+    float* dst = acc + (((size_t)c * ntl + tile) * NBT + w * NBW * NBW) * (32 * 64);
 #pragma unroll
-    for (int a = 0; a < 2; a++)
+    for (int a = 0; a < NBW; a++)
 #pragma unroll
-        for (int b = 0; b < 2; b++) {
+        for (int b = 0; b < NBW; b++) {
             if (diag && qi == qj && a > b) continue;
-            float* d = dst + (a * 2 + b) * (32 * 64);
+            float* d = dst + (a * NBW + b) * (32 * 64);
 #pragma unroll
             for (int r = 0; r < 16; r++) {
                 d[r * 64 + lane] += dre[a][b][r];
@@ -422,7 +429,7 @@ __global__ __launch_bounds__(X8NT, (NS == 1 ? 2 : 1)) void k_xengine8(const uint
 // MFMA 32x32 output layout: lane l, register r holds row 8(r/4) + 4(l/32) + r%4,
 // column l%32.
 // ---------------------------------------------------------------------------
-__global__ void k_xengine_dump(const float* __restrict__ acc, int nchan, int ninp, int nsets,
+__global__ void k_xengine_dump(const float* __restrict__ acc, int nchan, int ninp, int ts, int nsets,
                                size_t set_stride, const float* __restrict__ coef,
                                const float* __restrict__ sc_chan, const float* __restrict__ sc_inp,
                                const int2* __restrict__ pij, long nprod, int accumulate,
@@ -434,17 +441,19 @@ __global__ void k_xengine_dump(const float* __restrict__ acc, int nchan, int nin
     const long p = idx / nchan;
     const int2 ij = pij[p];
     const int i = ij.x, j = ij.y;
-    const int nb = (ninp + TS - 1) / TS;
+    const int nb = (ninp + ts - 1) / ts;
     const int ntl = nb * (nb + 1) / 2;
-    const int ti = i / TS, tj = j / TS;
+    const int ti = i / ts, tj = j / ts;
     const int tile = ti * nb - ti * (ti - 1) / 2 + (tj - ti);
-    const int bi = (i % TS) >> 5, bj = (j % TS) >> 5;
-    const int w = (bi >> 1) * 2 + (bj >> 1);
-    const int s = (bi & 1) * 2 + (bj & 1);
+    // block of (i, j) in the tile: wave (qi, qj) quadrant, sub-block (a, b)
+    const int nbw = ts / 64;
+    const int bi = (i % ts) >> 5, bj = (j % ts) >> 5;
+    const int w = (bi / nbw) * 2 + (bj / nbw);
+    const int blk = w * nbw * nbw + (bi % nbw) * nbw + (bj % nbw);
     const int ii = i & 31, jj = j & 31;
     const int r = 4 * (ii >> 3) + (ii & 3);
     const int l = jj + 32 * ((ii >> 2) & 1);
-    const size_t off = (((size_t)c * ntl + tile) * 16 + w * 4 + s) * (32 * 64);
+    const size_t off = (((size_t)c * ntl + tile) * (size_t)((ts / 32) * (ts / 32)) + blk) * (32 * 64);
     float re = 0.f, im = 0.f;
     for (int q = 0; q < nsets; q++) {
         const float* a = acc + q * set_stride + off;
@@ -499,19 +508,25 @@ __global__ void k_mfma_probe(int which, float* out)
 
 } // namespace
 
-size_t fxk_xengine_acc_floats(int nchan, int ninp, int nsets)
+size_t fxk_xengine_acc_floats(int nchan, int ninp, int nsets, int ts)
 {
-    return (size_t)nsets * (size_t)nchan * (size_t)ntiles128(ninp) * 16u * 32u * 64u;
+    return (size_t)nsets * (size_t)nchan * (size_t)ntiles_ts(ninp, ts) * (size_t)((ts / 32) * (ts / 32)) * 32u * 64u;
 }
 
-int fxk_xengine(hipStream_t s, int in_type, const void* X, int nchan, long nt, int ninp, float* acc)
+int fxk_xengine(hipStream_t s, int in_type, int ts, const void* X, int nchan, long nt, int ninp, float* acc)
 {
-    long nblk = xcd_grid((long)ntiles128(ninp) * nchan);
-    if (in_type == FX_PREC_F32) {
-        hipLaunchKernelGGL(k_xengine<FX_PREC_F32>, dim3((unsigned)nblk), dim3(XNT), 0, s,
+    long nblk = xcd_grid((long)ntiles_ts(ninp, ts) * nchan);
+    if (in_type == FX_PREC_F32 && ts == 128) {
+        hipLaunchKernelGGL((k_xengine<FX_PREC_F32, 128>), dim3((unsigned)nblk), dim3(XNT), 0, s,
                            reinterpret_cast<const float2*>(X), nchan, nt, ninp, acc);
-    } else if (in_type == FX_PREC_F16) {
-        hipLaunchKernelGGL(k_xengine<FX_PREC_F16>, dim3((unsigned)nblk), dim3(XNT), 0, s,
+    } else if (in_type == FX_PREC_F32 && ts == 64) {
+        hipLaunchKernelGGL((k_xengine<FX_PREC_F32, 64>), dim3((unsigned)nblk), dim3(XNT), 0, s,
+                           reinterpret_cast<const float2*>(X), nchan, nt, ninp, acc);
+    } else if (in_type == FX_PREC_F16 && ts == 128) {
+        hipLaunchKernelGGL((k_xengine<FX_PREC_F16, 128>), dim3((unsigned)nblk), dim3(XNT), 0, s,
+                           reinterpret_cast<const uint32_t*>(X), nchan, nt, ninp, acc);
+    } else if (in_type == FX_PREC_F16 && ts == 64) {
+        hipLaunchKernelGGL((k_xengine<FX_PREC_F16, 64>), dim3((unsigned)nblk), dim3(XNT), 0, s,
                            reinterpret_cast<const uint32_t*>(X), nchan, nt, ninp, acc);
     } else {
         return FX_ERR_UNSUPPORTED;
@@ -522,7 +537,7 @@ int fxk_xengine(hipStream_t s, int in_type, const void* X, int nchan, long nt, i
 int fxk_xengine8(hipStream_t s, int prec, const void* X, int nchan, long nt, int ninp, float* acc)
 {
     long nblk = xcd_grid((long)ntiles128(ninp) * nchan);
-    const size_t stride = fxk_xengine_acc_floats(nchan, ninp, 1);
+    const size_t stride = fxk_xengine_acc_floats(nchan, ninp, 1, TS);
     const uint8_t* x = reinterpret_cast<const uint8_t*>(X);
     if (prec == FX_PREC_FP8) {
         if (nt % 8) return FX_ERR_ARG;
@@ -536,15 +551,15 @@ int fxk_xengine8(hipStream_t s, int prec, const void* X, int nchan, long nt, int
     return hipGetLastError() == hipSuccess ? FX_OK : FX_ERR_HIP;
 }
 
-int fxk_xengine_dump_pij(hipStream_t s, const float* acc, int nchan, int ninp, int nsets,
+int fxk_xengine_dump_pij(hipStream_t s, const float* acc, int nchan, int ninp, int ts, int nsets,
                          const float* set_coef, const float* scale_chan, const float* scale_inp,
                          const int2* pij, int accumulate, float2* vis)
 {
     long nprod = (long)ninp * (ninp + 1) / 2;
     long total = nprod * nchan;
-    size_t set_stride = fxk_xengine_acc_floats(nchan, ninp, 1);
+    size_t set_stride = fxk_xengine_acc_floats(nchan, ninp, 1, ts);
     hipLaunchKernelGGL(k_xengine_dump, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, acc,
-                       nchan, ninp, nsets, set_stride, set_coef, scale_chan, scale_inp, pij, nprod,
+                       nchan, ninp, ts, nsets, set_stride, set_coef, scale_chan, scale_inp, pij, nprod,
                        accumulate, vis);
     return hipGetLastError() == hipSuccess ? FX_OK : FX_ERR_HIP;
 }

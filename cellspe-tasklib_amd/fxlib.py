@@ -101,6 +101,17 @@ def load():
         "fx_corr_fstage_rows": (i, [vp, vp, sz, l, l, i]),
         "fx_corr_cornerturn": (i, [vp, l, l, l, i, vp]),
         "fx_corr_fstage_depth": (l, [vp]),
+        "fx_corr_tune": (i, [vp, l, i]),
+        "fx_pcal_final_async": (i, [vp]),
+        "fx_pcal_tones": (i, [vp, vp, C.POINTER(u64)]),
+        "fx_batch_create": (i, [vp, vp, i, i, i, C.c_char_p, C.POINTER(vp)]),
+        "fx_batch_run": (i, [vp]),
+        "fx_batch_sync": (i, [vp]),
+        "fx_batch_vis": (vp, [vp, i]),
+        "fx_batch_pcal": (i, [vp, i, i, vp, C.POINTER(i)]),
+        "fx_batch_tuning": (i, [vp, i, C.POINTER(l), C.POINTER(i), C.POINTER(l)]),
+        "fx_batch_destroy": (i, [vp]),
+        "fx_autotune": (i, [vp, i, i, i, C.c_char_p, C.POINTER(l), C.POINTER(i)]),
         "fx_corr_info": (i, [vp, C.POINTER(l), C.POINTER(sz), C.POINTER(sz)]),
         "fx_mfma_probe": (i, [vp, i, vp]),
     }
@@ -351,6 +362,9 @@ class Correlator:
     def reset(self):
         check(load().fx_corr_reset(self.handle))
 
+    def tune(self, sub_spectra=0, xtile=0):
+        check(load().fx_corr_tune(self.handle, sub_spectra, xtile))
+
     def visibilities(self):
         """Dump to a temporary device buffer and return [P][C] complex64."""
         b = self.ctx.alloc(self.nprod * self.nchan * 8)
@@ -362,6 +376,82 @@ class Correlator:
     def close(self):
         if self.handle is not None:
             load().fx_corr_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class SubbandDesc(C.Structure):
+    _fields_ = [("nfft", C.c_int), ("ninp", C.c_int), ("nspectra", C.c_long), ("d_words", C.c_void_p),
+                ("stride_words", C.c_size_t), ("pm", C.c_void_p), ("pcal", C.c_int),
+                ("bandwidth_hz", C.c_double), ("pcal_spacing_hz", C.c_double), ("pcal_offset_hz", C.c_int)]
+
+
+class Batch:
+    """Batched subbands (fx_batch_*): one correlator per subband, optionally
+    captured into one hipGraph."""
+
+    def __init__(self, ctx, subbands, prec, use_graph=True, tune_file=None):
+        """subbands: list of dicts with nfft, ninp, nspectra, words (DeviceBuffer),
+        stride_words, and optional models (list of 4-tuples) and pcal
+        (bandwidth_hz, spacing_hz, offset_hz)."""
+        self.ctx = ctx
+        n = len(subbands)
+        descs = (SubbandDesc * n)()
+        self._keep = []
+        for k, s in enumerate(subbands):
+            d = descs[k]
+            d.nfft, d.ninp, d.nspectra = s["nfft"], s["ninp"], s["nspectra"]
+            d.d_words = s["words"].ptr.value + s.get("offset_bytes", 0)
+            d.stride_words = s["stride_words"]
+            if s.get("models") is not None:
+                arr = (PhaseModel * s["ninp"])(*[PhaseModel(*m) for m in s["models"]])
+                self._keep.append(arr)
+                d.pm = C.cast(arr, C.c_void_p)
+            if s.get("pcal") is not None:
+                d.pcal = 1
+                d.bandwidth_hz, d.pcal_spacing_hz, d.pcal_offset_hz = s["pcal"][0], s["pcal"][1], int(s["pcal"][2])
+        h = C.c_void_p()
+        tf = None if tune_file is None else tune_file.encode()
+        check(load().fx_batch_create(ctx.handle, C.cast(descs, C.c_void_p), n, prec, int(use_graph), tf,
+                                     C.byref(h)))
+        self.handle = h
+        self.subbands = subbands
+
+    def run(self):
+        check(load().fx_batch_run(self.handle))
+
+    def sync(self):
+        check(load().fx_batch_sync(self.handle))
+
+    def visibilities(self, s):
+        sb = self.subbands[s]
+        P = sb["ninp"] * (sb["ninp"] + 1) // 2
+        C_ = sb["nfft"] // 2
+        out = np.empty((P, C_), np.complex64)
+        ptr = load().fx_batch_vis(self.handle, s)
+        check(load().fx_memcpy_d2h(self.ctx.handle, _ptr(out), C.c_void_p(ptr), out.nbytes))
+        return out
+
+    def pcal(self, s, i):
+        nt = C.c_int()
+        check(load().fx_batch_pcal(self.handle, s, i, None, C.byref(nt)))
+        out = np.zeros(max(nt.value, 1), np.complex128)
+        check(load().fx_batch_pcal(self.handle, s, i, _ptr(out), C.byref(nt)))
+        return out[: nt.value]
+
+    def tuning(self, s):
+        sub, x, ch = C.c_long(), C.c_int(), C.c_long()
+        check(load().fx_batch_tuning(self.handle, s, C.byref(sub), C.byref(x), C.byref(ch)))
+        return sub.value, x.value, ch.value
+
+    def close(self):
+        if self.handle is not None:
+            load().fx_batch_destroy(self.handle)
             self.handle = None
 
     def __del__(self):

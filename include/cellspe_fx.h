@@ -156,6 +156,10 @@ int fx_pcal_extract_2bit(fx_pcal* pc, const uint32_t* d_words, size_t nsamp);
 /* host output, ntones complex; returns samples integrated via *nsamples */
 int fx_pcal_final(fx_pcal* pc, fx_cf64* tones, uint64_t* nsamples);
 int fx_pcal_info(fx_pcal* pc, int* kind, int* nbins, int* ntones);
+/* graph-capturable split of fx_pcal_final: enqueue the tone extraction into
+ * device memory, then (after the stream or graph ran) copy the tones out */
+int fx_pcal_final_async(fx_pcal* pc);
+int fx_pcal_tones(fx_pcal* pc, fx_cf64* tones, uint64_t* nsamples);
 
 /* ---- a3/a4/a6/a7: correlator -------------------------------------------- */
 typedef struct {
@@ -213,6 +217,48 @@ int fx_corr_fstage_rows(fx_corr* corr, const uint32_t* d_words, size_t in_stride
 int fx_corr_cornerturn(fx_corr* corr, long nspectra, long s0, long chunk_spectra, int out_prec,
                        void* d_chan);
 int fx_corr_info(fx_corr* corr, long* chunk_spectra, size_t* chan_bytes, size_t* acc_bytes);
+/* tuning knobs (0 keeps the current value): F-stage sub-chunk depth in
+ * spectra (multiple of 8) and the X-engine input tile, 128 or 64 (64 only for
+ * the FP32/FP16 engines; smaller tiles waste less MFMA work on small input
+ * counts).  Changing the tile clears the accumulators. */
+int fx_corr_tune(fx_corr* corr, long sub_spectra, int xtile);
+
+/* ---- batched subbands (uneven FFT lengths and input counts) -------------- */
+typedef struct {
+    int nfft;                  /* N, 256..16384                                    */
+    int ninp;                  /* inputs (stations x polarizations)                */
+    long nspectra;             /* spectra integrated                               */
+    const uint32_t* d_words;   /* input 0; input i at d_words + i*stride_words     */
+    size_t stride_words;
+    const fx_phase_model* pm;  /* host array of ninp models, or NULL (no rotation) */
+    int pcal;                  /* extract phase-cal tones of every input (FP64)    */
+    double bandwidth_hz, pcal_spacing_hz;
+    int pcal_offset_hz;
+} fx_subband_desc;
+typedef struct fx_batch fx_batch;
+/* One correlator per subband, each on its own stream; with use_graph the
+ * whole batch (per subband: reset, F-stage, X-engine, dump, phase-cal fold and
+ * tone extraction) is captured once into a hipGraph whose subband branches run
+ * concurrently, and fx_batch_run replays it with one launch.  Without a graph
+ * the same work is issued one launch at a time on the first subband's stream.
+ * tune_file: autotuner cache (NULL: defaults); entries missing from it are
+ * tuned on the spot and appended. */
+int fx_batch_create(fx_ctx* ctx, const fx_subband_desc* sb, int nsub, int prec, int use_graph,
+                    const char* tune_file, fx_batch** batch);
+int fx_batch_run(fx_batch* batch);
+int fx_batch_sync(fx_batch* batch);
+/* device visibilities [p][c] of subband s (valid after a run) */
+const fx_cf32* fx_batch_vis(fx_batch* batch, int s);
+/* host tones of input i of subband s (after fx_batch_sync) */
+int fx_batch_pcal(fx_batch* batch, int s, int i, fx_cf64* tones, int* ntones);
+/* chosen tuning of subband s */
+int fx_batch_tuning(fx_batch* batch, int s, long* sub_spectra, int* xtile, long* chunk_spectra);
+int fx_batch_destroy(fx_batch* batch);
+
+/* Autotuner: times the F-stage + X-engine of an (nfft, ninp, prec) subband on
+ * synthetic input for every candidate (sub-chunk depth x X tile) and appends
+ * the fastest to tune_file ("nfft ninp prec sub xtile ms" lines). */
+int fx_autotune(fx_ctx* ctx, int nfft, int ninp, int prec, const char* tune_file, long* sub_spectra, int* xtile);
 
 /* ---- diagnostics --------------------------------------------------------- */
 /* MFMA lane-layout probe: which = 0 (f32 32x32x2), 1 (f16 32x32x16),

@@ -1,0 +1,117 @@
+"""GPU parity of the batched uneven-subband path (fx_batch_*), the 64 x 64
+X-engine tile and the autotuner.
+
+Each subband of a batch with uneven FFT lengths and input counts must match
+the oracle: FP32 within 1e-5 of the FP32 restatement, FP16 within 1e-3 of the
+FP64 oracle; the hipGraph replay must equal the launch-by-launch run exactly
+on every replay (with eager work and synchronisation between replays), and
+the per-input FP64 phase-cal tones must match the oracle's.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import fxlib
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+TOL32 = 1e-5
+FS = 16e6
+
+
+def nrel(a, b):
+    a = np.asarray(a, np.complex128).ravel()
+    b = np.asarray(b, np.complex128).ravel()
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+@pytest.mark.parametrize("prec", [fxlib.FX_PREC_F32, fxlib.FX_PREC_F16])
+@pytest.mark.parametrize("I,C,T", [(20, 128, 24), (100, 128, 16), (64, 256, 8)])
+def test_xengine_tile64(ctx, prec, I, C, T):
+    rng = np.random.default_rng(I + C)
+    X = (rng.standard_normal((C, T, I)) + 1j * rng.standard_normal((C, T, I))).astype(np.complex64)
+    corr = fxlib.Correlator(ctx, I, 2 * C, 8, prec)
+    corr.tune(0, 64)
+    if prec == fxlib.FX_PREC_F16:
+        corr.set_scales(np.ones(C, np.float32), np.ones(I, np.float32))
+        Xh = np.empty((C, T, I, 2), np.float16)
+        Xh[..., 0], Xh[..., 1] = X.real, X.imag
+        dX = ctx.to_device(Xh)
+    else:
+        dX = ctx.to_device(X)
+    corr.xengine(dX, T)
+    got = corr.visibilities()
+    corr.close()
+    if prec == fxlib.FX_PREC_F32:
+        assert nrel(got, O.xengine_f32(X)) < TOL32
+    else:
+        assert nrel(got, O.xengine_f64(X)) < 1e-3
+
+
+def _make_batch_input(ctx):
+    shapes = [(256, 16, 512), (1024, 40, 64), (4096, 24, 16), (16384, 18, 8), (1024, 96, 32)]
+    subs, streams = [], []
+    for k, (n, I, T) in enumerate(shapes):
+        st = [O.gen_station(50 + k, i, 0.03, i % 5, 0.4 if i == 0 else 0.0, (10e3 + 1e6) / FS, 0.9816, n * T)
+              for i in range(I)]
+        w = np.stack(st)
+        rng = np.random.default_rng(k)
+        models = [(float(rng.uniform(0, 20)), 0.0, float(rng.uniform(-3, 3)), 0.0) for _ in range(I)] if k % 2 else None
+        subs.append({"nfft": n, "ninp": I, "nspectra": T, "words": ctx.to_device(w), "stride_words": w.shape[1],
+                     "models": models, "pcal": (FS / 2, 1e6, 10e3) if k < 2 else None})
+        streams.append(st)
+    return subs, streams
+
+
+def test_batch_graph_vs_eager_vs_oracle(ctx, tmp_path):
+    subs, streams = _make_batch_input(ctx)
+    b32 = fxlib.Batch(ctx, subs, fxlib.FX_PREC_F32, use_graph=False)
+    b32.run()
+    b32.sync()
+    tune = str(tmp_path / "tune.txt")
+    g16 = fxlib.Batch(ctx, subs, fxlib.FX_PREC_F16, use_graph=True, tune_file=tune)
+    e16 = fxlib.Batch(ctx, subs, fxlib.FX_PREC_F16, use_graph=False, tune_file=tune)
+    assert os.path.exists(tune)
+    lines = [ln for ln in open(tune) if ln.strip() and not ln.startswith("#")]
+    assert len(lines) == len({(s["nfft"], s["ninp"]) for s in subs})
+    e16.run()
+    e16.sync()
+    ref16 = [e16.visibilities(k) for k in range(len(subs))]
+    for rep in range(3):
+        g16.run()
+        g16.sync()
+        for k in range(len(subs)):
+            assert np.array_equal(g16.visibilities(k), ref16[k]), (rep, k)
+        # eager work between the replays
+        e16.run()
+        e16.sync()
+    for k, s in enumerate(subs):
+        n, I, T = s["nfft"], s["ninp"], s["nspectra"]
+        r32 = O.corr_f32(streams[k], n, T, s["models"], nthreads=8)
+        r64 = O.corr_f64(streams[k], n, T, s["models"], nthreads=8)
+        assert nrel(b32.visibilities(k), r32) < TOL32, k
+        assert nrel(ref16[k], r64) < 1e-3, k
+        sub, xt, chunk = g16.tuning(k)
+        assert xt in (64, 128) and sub % 8 == 0 and chunk >= T
+        if s["pcal"] is not None:
+            cfg = O.pcal_config(*s["pcal"])
+            for i in (0, I - 1):
+                x = O.unpack_1ch(streams[k][i], n * T)
+                ref = O.pcal_final(cfg, O.pcal_extract(cfg, x))
+                for bb in (b32, g16):
+                    assert nrel(bb.pcal(k, i), ref) < 1e-12
+    for bb in (b32, g16, e16):
+        bb.close()
+
+
+def test_autotune_cache(ctx, tmp_path):
+    tune = str(tmp_path / "t.txt")
+    import ctypes as C
+    sub, xt = C.c_long(), C.c_int()
+    fxlib.check(fxlib.load().fx_autotune(ctx.handle, 1024, 40, fxlib.FX_PREC_F16, tune.encode(), C.byref(sub),
+                                         C.byref(xt)))
+    assert sub.value in (64, 256, 1024) and xt.value in (64, 128)
+    f = open(tune).read().split()
+    assert f[:5] == ["1024", "40", str(fxlib.FX_PREC_F16), str(sub.value), str(xt.value)]
