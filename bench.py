@@ -179,6 +179,93 @@ def large_n(fxlib, ctx, args):
     return out
 
 
+def batched(fxlib, ctx, args):
+    """32 subbands with FFT lengths drawn from {256,1024,4096,16384}, station
+    counts from 8..48 (x2 pol), rho = 0.03, 2^22 samples per input, FP64
+    phase-cal of the 10 kHz-offset 1 MHz comb on every input (one comb tooth
+    injected per subband): one hipGraph launch of the autotuned FP16 batch vs
+    the same subbands one launch each in plain FP32."""
+    import tempfile
+    rng = np.random.default_rng(args.seed + 4)
+    nsub = args.batch_subbands
+    nsamp = 1 << 22
+    subs = []
+    bufs = []
+    tot_cmac = 0.0
+    tot_samp = 0.0
+    for k in range(nsub):
+        n = int(rng.choice([256, 1024, 4096, 16384]))
+        S = int(rng.integers(8, 49))
+        I = 2 * S
+        T = nsamp // n
+        nw = nsamp // 16
+        w = ctx.alloc(I * nw * 4)
+        for i in range(I):
+            tone = 0.2 if i == 0 else 0.0
+            fxlib.check(fxlib.load().fx_synth_station(ctx.handle, args.seed + 100 + k, i, 0.03, i % 7, tone,
+                                                      (10e3 + 3e6) / 16e6, 0.9816, nsamp,
+                                                      w.ptr.value + i * nw * 4))
+        bufs.append(w)
+        subs.append({"nfft": n, "ninp": I, "nspectra": T, "words": w, "stride_words": nw,
+                     "models": None, "pcal": (8e6, 1e6, 10e3)})
+        tot_cmac += (I * (I + 1) / 2) * (n / 2) * T
+        tot_samp += I * nsamp
+    ctx.sync()
+    tune = os.path.join(tempfile.gettempdir(), f"fx_tune_{os.getpid()}.txt")
+    t0 = time.time()
+    g16 = fxlib.Batch(ctx, subs, fxlib.FX_PREC_F16, use_graph=True, tune_file=tune)
+    log(f"[bench] batch: {nsub} subbands, autotuned + captured in {time.time() - t0:.1f}s")
+    e32 = fxlib.Batch(ctx, subs, fxlib.FX_PREC_F32, use_graph=False)
+
+    def timed(b, reps):
+        b.run()
+        b.sync()
+        e0, e1 = ctx.event(), ctx.event()
+        ctx.record(e0)
+        for _ in range(reps):
+            b.run()
+        ctx.record(e1)
+        ctx.sync()
+        return ctx.elapsed_ms(e0, e1) / reps
+
+    ms16 = timed(g16, args.steps)
+    ms32 = timed(e32, max(1, args.steps - 1))
+    log(f"[bench] batch: fp16 graph {ms16:.2f} ms, fp32 one launch each {ms32:.2f} ms")
+    err = None
+    if not args.no_check:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+        cost = [s["ninp"] ** 2 * s["nfft"] * s["nspectra"] for s in subs]
+        err = {}
+        for k in sorted(range(nsub), key=lambda q: cost[q])[:2]:
+            s = subs[k]
+            st = [s["words"].download_at(i * s["stride_words"] * 4, np.uint32, s["stride_words"])
+                  for i in range(s["ninp"])]
+            r64 = O.corr_f64(st, s["nfft"], s["nspectra"], None, nthreads=min(16, os.cpu_count() or 8))
+            r32 = O.corr_f32(st, s["nfft"], s["nspectra"], None, nthreads=min(16, os.cpu_count() or 8))
+            err[f"subband{k}_N{s['nfft']}_I{s['ninp']}"] = {"f16_vs_fp64": nrel(g16.visibilities(k), r64),
+                                                            "f32_vs_fp32_oracle": nrel(e32.visibilities(k), r32)}
+        log(f"[bench] batch errors: {err}")
+    tuning = [g16.tuning(k) for k in range(nsub)]
+    g16.close()
+    e32.close()
+    for w in bufs:
+        w.free()
+    try:
+        os.remove(tune)
+    except OSError:
+        pass
+    return {"workload": f"{nsub} subbands, N in {{256,1024,4096,16384}}, 16-96 inputs, rho=0.03, "
+                        f"2^22 samples per input, FP64 phase-cal of every input",
+            "subbands": [{"nfft": s["nfft"], "inputs": s["ninp"], "spectra": s["nspectra"],
+                          "sub_spectra": t[0], "xtile": t[1]} for s, t in zip(subs, tuning)],
+            "ms": {"f16_graph": ms16, "f32_one_launch_each": ms32},
+            "complex_mac_per_s": {"f16_graph": tot_cmac / (ms16 * 1e-3), "f32_one_launch_each": tot_cmac / (ms32 * 1e-3)},
+            "input_gsps": {"f16_graph": tot_samp / (ms16 * 1e-3) / 1e9,
+                           "f32_one_launch_each": tot_samp / (ms32 * 1e-3) / 1e9},
+            "speedup": ms32 / ms16, "complex_macs": tot_cmac, "err": err}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -196,6 +283,8 @@ def main():
     ap.add_argument("--no-fp32", action="store_true")
     ap.add_argument("--no-fp8", action="store_true")
     ap.add_argument("--no-large", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--batch-subbands", type=int, default=32)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-check", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
@@ -358,6 +447,9 @@ def main():
     large = None
     if not args.no_large:
         large = large_n(fxlib, ctx, args)
+    batch = None
+    if not args.no_batch:
+        batch = batched(fxlib, ctx, args)
 
     # ---- roofline of the dominant kernel of the headline step ------------------
     # per launch: the X-engine runs once per chunk, the F kernel and the corner
@@ -434,6 +526,7 @@ def main():
         "kernels_ms_per_chunk": kern,
         "err_vs_fp64": err,
         "large_n_xengine": large,
+        "batched_subbands": batch,
         "roofline": roof,
         "cpu_baseline": cpu,
     }

@@ -175,12 +175,14 @@ __global__ __launch_bounds__(256) void k_pcal_fold(const uint32_t* __restrict__ 
                                                    double* __restrict__ acc)
 {
     extern __shared__ double lacc[];
-    const size_t w = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (use_lds) {
         for (int b = threadIdx.x; b < nfold; b += blockDim.x) lacc[b] = 0.0;
         __syncthreads();
     }
-    if (w * 16 < nsamp) {
+    // grid-stride over the packed words: a workgroup folds many words into its
+    // LDS copy before the nfold global additions of its partial fold
+    const size_t nw = (nsamp + 15) / 16;
+    for (size_t w = (size_t)blockIdx.x * blockDim.x + threadIdx.x; w < nw; w += (size_t)gridDim.x * blockDim.x) {
         const uint32_t v = words[w];
         size_t n0 = w * 16;
         int bin = (int)((position + n0) % (size_t)nfold);
@@ -439,8 +441,11 @@ int fxk_pcal_fold(hipStream_t s, const uint32_t* words, size_t nsamp, size_t pos
     if (nw == 0) return FX_OK;
     int use_lds = nfold <= 8192;
     size_t lds = use_lds ? (size_t)nfold * sizeof(double) : 0;
-    hipLaunchKernelGGL(k_pcal_fold, dim3(nblocks(nw, 256)), dim3(256), lds, s, words, nsamp,
-                       position, nfold, use_lds, acc);
+    // at least 16 words per thread: each workgroup's nfold global additions
+    // are amortized over >= 65536 samples
+    const size_t blocks = std::min<size_t>(nblocks(nw, 256), std::max<size_t>(64, nw / 4096));
+    hipLaunchKernelGGL(k_pcal_fold, dim3((unsigned)blocks), dim3(256), lds, s, words, nsamp, position, nfold,
+                       use_lds, acc);
     return last_status();
 }
 
