@@ -385,7 +385,7 @@ def main():
     ms16, k16 = run_path(fxlib.FX_PREC_F16, 4, args.steps, args.warmup)
     paths["f16"] = (ms16, k16)
     log(f"[bench] fp16 path: {ms16:.2f} ms/step; per chunk " +
-        ", ".join(f"{k} {v:.2f} ms" for k, v in k16.items()))
+        ", ".join(f"{k} {v}" if k == "subchunks_per_chunk" else f"{k} {v:.2f} ms" for k, v in k16.items()))
     if not args.no_fp32:
         paths["f32"] = run_path(fxlib.FX_PREC_F32, 8, max(1, args.steps - 1), 1)
         log(f"[bench] fp32 path: {paths['f32'][0]:.2f} ms/step")

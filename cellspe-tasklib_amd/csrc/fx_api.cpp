@@ -491,8 +491,9 @@ struct fx_pcal {
     double fs, spacing, offset;
     int nbins, ntones, rotatorlen, nfold, step, offset_bin;
     double dphi;
-    double* acc = nullptr;   // nfold FP64
-    double* tones = nullptr; // 2*ntones FP64
+    int ninp = 1;            // inputs sharing this configuration
+    double* acc = nullptr;   // ninp x nfold FP64
+    double* tones = nullptr; // ninp x 2*ntones FP64
     size_t position = 0;
     uint64_t samplecount = 0;
     bool finalized = false;
@@ -511,12 +512,19 @@ static long long gcd_ll(long a, long b)
 extern "C" int fx_pcal_create(fx_ctx* ctx, double bandwidth_hz, double spacing_hz, int offset_hz,
                               size_t sampleoffset, fx_pcal** out)
 {
+    return fx_pcal_create_inputs(ctx, bandwidth_hz, spacing_hz, offset_hz, sampleoffset, 1, out);
+}
+
+extern "C" int fx_pcal_create_inputs(fx_ctx* ctx, double bandwidth_hz, double spacing_hz, int offset_hz,
+                                     size_t sampleoffset, int ninp, fx_pcal** out)
+{
     int rc = ctx_device(ctx);
     if (rc) return rc;
-    FX_REQUIRE(out && bandwidth_hz > 0 && spacing_hz > 0 && offset_hz >= 0, FX_ERR_ARG,
+    FX_REQUIRE(out && bandwidth_hz > 0 && spacing_hz > 0 && offset_hz >= 0 && ninp >= 1, FX_ERR_ARG,
                "fx_pcal_create: invalid parameters");
     fx_pcal* p = new fx_pcal;
     p->ctx = ctx;
+    p->ninp = ninp;
     double fs = 2.0 * bandwidth_hz;
     p->fs = fs;
     p->spacing = spacing_hz;
@@ -544,8 +552,8 @@ extern "C" int fx_pcal_create(fx_ctx* ctx, double bandwidth_hz, double spacing_h
     p->nfold = p->kind == 2 ? (int)((long long)p->rotatorlen / gcd_ll(p->rotatorlen, p->nbins) * p->nbins)
                             : p->nbins;
     p->dphi = 2.0 * M_PI * (-(double)offset_hz / fs);
-    if (hipMalloc(&p->acc, sizeof(double) * (size_t)p->nfold) != hipSuccess ||
-        hipMalloc(&p->tones, sizeof(double) * 2 * (size_t)std::max(p->ntones, 1)) != hipSuccess) {
+    if (hipMalloc(&p->acc, sizeof(double) * (size_t)p->nfold * ninp) != hipSuccess ||
+        hipMalloc(&p->tones, sizeof(double) * 2 * (size_t)std::max(p->ntones, 1) * ninp) != hipSuccess) {
         hipGetLastError();
         if (p->acc) hipFree(p->acc);
         delete p;
@@ -553,7 +561,7 @@ extern "C" int fx_pcal_create(fx_ctx* ctx, double bandwidth_hz, double spacing_h
         return FX_ERR_NOMEM;
     }
     p->position = sampleoffset % (size_t)p->nfold;
-    if (hipMemsetAsync(p->acc, 0, sizeof(double) * (size_t)p->nfold, ctx->stream) != hipSuccess) {
+    if (hipMemsetAsync(p->acc, 0, sizeof(double) * (size_t)p->nfold * ninp, ctx->stream) != hipSuccess) {
         hipFree(p->acc);
         hipFree(p->tones);
         delete p;
@@ -581,7 +589,7 @@ extern "C" int fx_pcal_clear(fx_pcal* p)
     int rc = ctx_device(p->ctx);
     if (rc) return rc;
     // fill kernel rather than a memset node (graph-capturable)
-    FX_CHECK_LAUNCH(fxk_fill32(p->ctx->stream, p->acc, 0u, 2 * (size_t)p->nfold));
+    FX_CHECK_LAUNCH(fxk_fill32(p->ctx->stream, p->acc, 0u, 2 * (size_t)p->nfold * p->ninp));
     p->samplecount = 0;
     p->finalized = false;
     return FX_OK;
@@ -593,22 +601,41 @@ extern "C" int fx_pcal_final_async(fx_pcal* p)
     int rc = ctx_device(p->ctx);
     if (rc) return rc;
     FX_CHECK_LAUNCH(fxk_pcal_final(p->ctx->stream, p->acc, p->nfold, p->nbins, p->ntones, p->offset_bin,
-                                   p->step, p->rotatorlen, p->dphi, p->tones));
+                                   p->step, p->rotatorlen, p->dphi, p->tones, p->ninp));
     p->finalized = true;
+    return FX_OK;
+}
+
+extern "C" int fx_pcal_tones_input(fx_pcal* p, int input, fx_cf64* tones, uint64_t* nsamples)
+{
+    FX_REQUIRE(p && tones && input >= 0 && input < p->ninp, FX_ERR_ARG, "fx_pcal_tones: bad argument");
+    int rc = ctx_device(p->ctx);
+    if (rc) return rc;
+    FX_REQUIRE(p->finalized, FX_ERR_STATE, "fx_pcal_tones: no tones extracted yet");
+    if (p->ntones > 0)
+        FX_HIP_TRY(hipMemcpyAsync(tones, p->tones + (size_t)input * 2 * (size_t)p->ntones,
+                                  sizeof(double) * 2 * (size_t)p->ntones, hipMemcpyDeviceToHost, p->ctx->stream));
+    FX_HIP_TRY(hipStreamSynchronize(p->ctx->stream));
+    if (nsamples) *nsamples = p->samplecount;
     return FX_OK;
 }
 
 extern "C" int fx_pcal_tones(fx_pcal* p, fx_cf64* tones, uint64_t* nsamples)
 {
-    FX_REQUIRE(p && tones, FX_ERR_ARG, "fx_pcal_tones: null argument");
+    return fx_pcal_tones_input(p, 0, tones, nsamples);
+}
+
+extern "C" int fx_pcal_extract_2bit_inputs(fx_pcal* p, const uint32_t* d_words, size_t in_stride_words,
+                                           size_t nsamp)
+{
+    FX_REQUIRE(p && d_words, FX_ERR_ARG, "null argument");
+    FX_REQUIRE(!p->finalized, FX_ERR_STATE, "fx_pcal_extract_2bit: extractor finalized, clear first");
     int rc = ctx_device(p->ctx);
     if (rc) return rc;
-    FX_REQUIRE(p->finalized, FX_ERR_STATE, "fx_pcal_tones: no tones extracted yet");
-    if (p->ntones > 0)
-        FX_HIP_TRY(hipMemcpyAsync(tones, p->tones, sizeof(double) * 2 * (size_t)p->ntones, hipMemcpyDeviceToHost,
-                                  p->ctx->stream));
-    FX_HIP_TRY(hipStreamSynchronize(p->ctx->stream));
-    if (nsamples) *nsamples = p->samplecount;
+    FX_CHECK_LAUNCH(fxk_pcal_fold(p->ctx->stream, d_words, nsamp, p->position, p->nfold, p->acc, p->ninp,
+                                  in_stride_words));
+    p->position = (p->position + nsamp) % (size_t)p->nfold;
+    p->samplecount += nsamp;
     return FX_OK;
 }
 
@@ -622,13 +649,9 @@ extern "C" int fx_pcal_adjust_sample_offset(fx_pcal* p, size_t sampleoffset)
 extern "C" int fx_pcal_extract_2bit(fx_pcal* p, const uint32_t* d_words, size_t nsamp)
 {
     FX_REQUIRE(p, FX_ERR_ARG, "null pcal");
-    FX_REQUIRE(!p->finalized, FX_ERR_STATE, "fx_pcal_extract_2bit: extractor finalized, clear first");
-    int rc = ctx_device(p->ctx);
-    if (rc) return rc;
-    FX_CHECK_LAUNCH(fxk_pcal_fold(p->ctx->stream, d_words, nsamp, p->position, p->nfold, p->acc));
-    p->position = (p->position + nsamp) % (size_t)p->nfold;
-    p->samplecount += nsamp;
-    return FX_OK;
+    FX_REQUIRE(p->ninp == 1, FX_ERR_ARG, "fx_pcal_extract_2bit: %d inputs, use fx_pcal_extract_2bit_inputs",
+               p->ninp);
+    return fx_pcal_extract_2bit_inputs(p, d_words, 0, nsamp);
 }
 
 extern "C" int fx_pcal_final(fx_pcal* p, fx_cf64* tones, uint64_t* nsamples)
@@ -638,11 +661,11 @@ extern "C" int fx_pcal_final(fx_pcal* p, fx_cf64* tones, uint64_t* nsamples)
     if (rc) return rc;
     if (!p->finalized) {
         FX_CHECK_LAUNCH(fxk_pcal_final(p->ctx->stream, p->acc, p->nfold, p->nbins, p->ntones,
-                                       p->offset_bin, p->step, p->rotatorlen, p->dphi, p->tones));
+                                       p->offset_bin, p->step, p->rotatorlen, p->dphi, p->tones, p->ninp));
         p->finalized = true;
     }
     if (tones && p->ntones > 0) {
-        FX_HIP_TRY(hipMemcpyAsync(tones, p->tones, sizeof(double) * 2 * (size_t)p->ntones,
+        FX_HIP_TRY(hipMemcpyAsync(tones, p->tones, sizeof(double) * 2 * (size_t)p->ntones * p->ninp,
                                   hipMemcpyDeviceToHost, p->ctx->stream));
     }
     FX_HIP_TRY(hipStreamSynchronize(p->ctx->stream));

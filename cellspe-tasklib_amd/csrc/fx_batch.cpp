@@ -25,7 +25,7 @@ struct Subband {
     fx_ctx* ctx = nullptr;
     fx_corr* corr = nullptr;
     fx_cf32* vis = nullptr;
-    std::vector<fx_pcal*> pcal;
+    fx_pcal* pcal = nullptr;   // all inputs of the subband, one fold + one final launch
     long chunk = 0, sub = 0;
     int xtile = 128;
 };
@@ -64,12 +64,11 @@ int enqueue(Subband& b)
     int rc = fx_corr_reset(b.corr);
     if (!rc) rc = fx_corr_process(b.corr, d.d_words, d.stride_words, 0, d.nspectra);
     if (!rc) rc = fx_corr_dump(b.corr, b.vis);
-    for (int i = 0; !rc && i < (int)b.pcal.size(); i++) {
-        fx_pcal* p = b.pcal[(size_t)i];
-        rc = fx_pcal_clear(p);
-        if (!rc) rc = fx_pcal_adjust_sample_offset(p, 0);
-        if (!rc) rc = fx_pcal_extract_2bit(p, d.d_words + (size_t)i * d.stride_words, (size_t)d.nspectra * d.nfft);
-        if (!rc) rc = fx_pcal_final_async(p);
+    if (!rc && b.pcal) {
+        rc = fx_pcal_clear(b.pcal);
+        if (!rc) rc = fx_pcal_adjust_sample_offset(b.pcal, 0);
+        if (!rc) rc = fx_pcal_extract_2bit_inputs(b.pcal, d.d_words, d.stride_words, (size_t)d.nspectra * d.nfft);
+        if (!rc) rc = fx_pcal_final_async(b.pcal);
     }
     return rc;
 }
@@ -160,7 +159,7 @@ extern "C" int fx_batch_destroy(fx_batch* b)
     for (hipEvent_t e : b->join)
         if (e) hipEventDestroy(e);
     for (Subband& s : b->sb) {
-        for (fx_pcal* p : s.pcal) fx_pcal_destroy(p);
+        if (s.pcal) fx_pcal_destroy(s.pcal);
         if (s.corr) fx_corr_destroy(s.corr);
         if (s.vis) fx_free(s.ctx, s.vis);
         if (s.ctx) fx_ctx_destroy(s.ctx);
@@ -210,11 +209,9 @@ extern "C" int fx_batch_create(fx_ctx* ctx, const fx_subband_desc* sb, int nsub,
         }
         const size_t P = (size_t)d.ninp * (d.ninp + 1) / 2;
         if (!rc) rc = fx_malloc(r.ctx, P * (size_t)(d.nfft / 2) * sizeof(fx_cf32), (void**)&r.vis);
-        for (int i = 0; !rc && d.pcal && i < d.ninp; i++) {
-            fx_pcal* p = nullptr;
-            rc = fx_pcal_create(r.ctx, d.bandwidth_hz, d.pcal_spacing_hz, d.pcal_offset_hz, 0, &p);
-            if (!rc) r.pcal.push_back(p);
-        }
+        if (!rc && d.pcal)
+            rc = fx_pcal_create_inputs(r.ctx, d.bandwidth_hz, d.pcal_spacing_hz, d.pcal_offset_hz, 0, d.ninp,
+                                       &r.pcal);
     }
     if (!rc && hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) != hipSuccess) rc = FX_ERR_HIP;
     for (int s = 0; s < nsub && !rc; s++) {
@@ -301,12 +298,12 @@ extern "C" int fx_batch_pcal(fx_batch* b, int s, int i, fx_cf64* tones, int* nto
 {
     FX_REQUIRE(b && s >= 0 && s < (int)b->sb.size(), FX_ERR_ARG, "fx_batch_pcal: subband %d", s);
     Subband& r = b->sb[(size_t)s];
-    FX_REQUIRE(i >= 0 && i < (int)r.pcal.size(), FX_ERR_ARG, "fx_batch_pcal: input %d has no phase-cal", i);
+    FX_REQUIRE(r.pcal && i >= 0 && i < r.d.ninp, FX_ERR_ARG, "fx_batch_pcal: input %d has no phase-cal", i);
     int nt = 0;
-    fx_pcal_info(r.pcal[(size_t)i], nullptr, nullptr, &nt);
+    fx_pcal_info(r.pcal, nullptr, nullptr, &nt);
     if (ntones) *ntones = nt;
     if (!tones) return FX_OK;
-    return fx_pcal_tones(r.pcal[(size_t)i], tones, nullptr);
+    return fx_pcal_tones_input(r.pcal, i, tones, nullptr);
 }
 
 extern "C" int fx_batch_tuning(fx_batch* b, int s, long* sub, int* xtile, long* chunk)

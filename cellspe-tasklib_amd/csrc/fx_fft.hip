@@ -346,6 +346,51 @@ __global__ __launch_bounds__(FftShape<M>::NT) void k_fstage(
     // lanes write consecutive channels of one row
     V* o = reinterpret_cast<V*>(nat);
     constexpr int H = M / 2;
+    if constexpr (H >= NT && NT % PLO == 0) {
+        // thread tid owns channels kp = q NT + tid of every item: the split
+        // twiddle W_N^kp advances by W_N^NT per q, and the low phasor factors
+        // of kp and M - kp are the same for every q
+        const float2 wstep = twl_get(tw, NT * (FX_TW_N / N));
+        const float2 w0 = twl_get(tw, tid * (FX_TW_N / N));
+        const int lo_k = tid % PLO, lo_m = (PLO - lo_k) % PLO;
+#pragma unroll
+        for (int f = 0; f < F; f++) {
+            if (it_t[f] < 0) continue;
+            const int i = it_i[f];
+            V* row = o + (size_t)(g0 + f) * M;
+            const float2 plo_k = pm ? ph_lo[f][lo_k] : make_float2(1.f, 0.f);
+            const float2 plo_m = pm ? ph_lo[f][lo_m] : make_float2(1.f, 0.f);
+            const float si = sc_chan ? sc_inp[i] : 1.f;
+            float2 w = w0;
+#pragma unroll
+            for (int q = 0; q < H / NT; q++) {
+                const int kp = q * NT + tid;
+                if (q) w = cmul(w, wstep);
+                float2 xk, xm;
+                rsplit_tw<M>(sh, f * M, kp, w, xk, xm);
+                const int k2 = M - kp;
+                const int k2m = k2 & (M - 1);   // kp = 0: k2 = M is the dropped Nyquist bin
+                if (pm) {
+                    xk = cmul(xk, cmul(ph_hi[f][kp / PLO], plo_k));
+                    xm = cmul(xm, cmul(ph_hi[f][k2m / PLO], plo_m));
+                }
+                if (mult) {
+                    xk = cmul(xk, mult[(size_t)i * M + kp]);
+                    xm = cmul(xm, mult[(size_t)i * M + k2m]);
+                }
+                if (sc_chan) {
+                    const float a = sc_chan[kp] * si;
+                    const float b = sc_chan[k2m] * si;
+                    xk.x *= a;
+                    xk.y *= a;
+                    xm.x *= b;
+                    xm.y *= b;
+                }
+                row[kp] = to_chan<T>(xk);
+                if (kp != 0) row[k2] = to_chan<T>(xm);
+            }
+        }
+    } else {
 #pragma unroll
     for (int it = 0; it < (F * H) / NT; it++) {
         // H >= NT: item f is fixed per iteration (it * NT / H), no division
@@ -378,6 +423,7 @@ __global__ __launch_bounds__(FftShape<M>::NT) void k_fstage(
         V* row = o + (size_t)g * M;
         row[kp] = to_chan<T>(xk);
         if (kp != 0) row[k2] = to_chan<T>(xm);
+    }
     }
     // kp = M/2 of each transform
     if (tid < F && it_t[tid] >= 0) {

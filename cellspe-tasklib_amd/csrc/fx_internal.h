@@ -248,18 +248,24 @@ __device__ __forceinline__ void fft_passes(float2* sh, const float2* __restrict_
 //   E = (Z_k + conj Z_{M-k})/2, O = -i (Z_k - conj Z_{M-k})/2,
 //   X_k = E + W_N^k O,  X_{M-k} = conj(E - W_N^k O),  N = 2M.
 // For k = 0 only X_0 = Re Z_0 + Im Z_0 is a channel (Nyquist dropped).
+// same split with the twiddle w = W_N^k supplied by the caller
 template <int M>
-__device__ __forceinline__ void rsplit(const float2* sh, int base, int k, const float2* __restrict__ tw,
-                                      float2& xk, float2& xmk)
+__device__ __forceinline__ void rsplit_tw(const float2* sh, int base, int k, float2 w, float2& xk, float2& xmk)
 {
     float2 a = sh[lpad(base + k)];
     float2 b = sh[lpad(base + ((M - k) & (M - 1)))];
     float2 e = make_float2(0.5f * (a.x + b.x), 0.5f * (a.y - b.y));
     float2 o = make_float2(0.5f * (a.y + b.y), -0.5f * (a.x - b.x));
-    float2 w = twl_get(tw, k * (FX_TW_N / (2 * M)));
     float2 wo = cmul(w, o);
     xk = cadd(e, wo);
     xmk = make_float2(e.x - wo.x, -(e.y - wo.y));
+}
+
+template <int M>
+__device__ __forceinline__ void rsplit(const float2* sh, int base, int k, const float2* __restrict__ tw,
+                                      float2& xk, float2& xmk)
+{
+    rsplit_tw<M>(sh, base, k, twl_get(tw, k * (FX_TW_N / (2 * M))), xk, xmk);
 }
 
 // ---------------------------------------------------------------------------
@@ -307,10 +313,12 @@ int fxk_xengine_dump_pij(hipStream_t s, const float* acc, int nchan, int ninp, i
                          const float* set_coef, const float* scale_chan, const float* scale_inp,
                          const int2* pij, int accumulate, float2* vis);
 int fxk_mfma_probe(hipStream_t s, int which, float* out);
+// phase-cal of ninp inputs in one launch: input y reads words + y*in_stride_words
+// and owns acc[y*nfold ..] and tones[y*2*ntones ..]
 int fxk_pcal_fold(hipStream_t s, const uint32_t* words, size_t nsamp, size_t position,
-                  int nfold, double* acc);
+                  int nfold, double* acc, int ninp = 1, size_t in_stride_words = 0);
 int fxk_pcal_final(hipStream_t s, const double* acc, int nfold, int nbins, int ntones,
-                   int offset_bin, int step, int rotatorlen, double dphi, double* tones);
+                   int offset_bin, int step, int rotatorlen, double dphi, double* tones, int ninp = 1);
 int fxk_complex_mac(hipStream_t s, const float2* a, const float2* b, float2* acc, long n, int conj);
 int fxk_complex_mul(hipStream_t s, const float2* a, const float2* b, float2* out, long n);
 int fxk_sincos(hipStream_t s, const float* arg, float2* out, long n, int normalized);

@@ -172,9 +172,12 @@ __global__ void k_synth_channelized(uint64_t seed, int nchan, long ntime, int ni
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_pcal_fold(const uint32_t* __restrict__ words, size_t nsamp,
                                                    size_t position, int nfold, int use_lds,
-                                                   double* __restrict__ acc)
+                                                   double* __restrict__ acc, size_t in_stride)
 {
     extern __shared__ double lacc[];
+    // grid y = input: words of input y at words + y*in_stride, its fold at acc + y*nfold
+    words += (size_t)blockIdx.y * in_stride;
+    acc += (size_t)blockIdx.y * (size_t)nfold;
     if (use_lds) {
         for (int b = threadIdx.x; b < nfold; b += blockDim.x) lacc[b] = 0.0;
         __syncthreads();
@@ -208,16 +211,22 @@ __global__ __launch_bounds__(256) void k_pcal_fold(const uint32_t* __restrict__ 
 //          (Shifting extractor, nfold = lcm(rotatorlen, nbins))
 //   A[b] = R[b]  (rotatorlen == 0: Trivial / ImplicitShift, nfold = nbins)
 //   tone = (1/nbins) sum_b A[b] e^{-2 pi i b k/nbins}
-__global__ void k_pcal_final(const double* __restrict__ acc, int nfold, int nbins, int ntones,
-                             int offset_bin, int step, int rotatorlen, double dphi,
-                             double* __restrict__ tones)
+// One workgroup per tone: the threads share the fold bins and their partial
+// sums are reduced through LDS in a fixed order (deterministic).
+__global__ __launch_bounds__(256) void k_pcal_final(const double* __restrict__ acc, int nfold, int nbins,
+                                                    int ntones, int offset_bin, int step, int rotatorlen,
+                                                    double dphi, double* __restrict__ tones)
 {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    __shared__ double red[2][256];
+    const int t = blockIdx.x;
     if (t >= ntones) return;
+    // grid y = input
+    acc += (size_t)blockIdx.y * (size_t)nfold;
+    tones += (size_t)blockIdx.y * 2 * (size_t)ntones;
     const long k = (long)offset_bin + (long)t * step;
     double sr = 0.0, si = 0.0;
     if (k < nbins) {
-        for (int r = 0; r < nfold; r++) {
+        for (int r = threadIdx.x; r < nfold; r += blockDim.x) {
             double v = acc[r];
             if (v == 0.0) continue;
             long e = ((long)(r % nbins) * k) % nbins;
@@ -228,11 +237,21 @@ __global__ void k_pcal_final(const double* __restrict__ acc, int nfold, int nbin
             sr += v * cs;
             si += v * sn;
         }
-        sr /= (double)nbins;
-        si /= (double)nbins;
     }
-    tones[2 * t] = sr;
-    tones[2 * t + 1] = si;
+    red[0][threadIdx.x] = sr;
+    red[1][threadIdx.x] = si;
+    __syncthreads();
+    for (int h = blockDim.x / 2; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h) {
+            red[0][threadIdx.x] += red[0][threadIdx.x + h];
+            red[1][threadIdx.x] += red[1][threadIdx.x + h];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        tones[2 * t] = k < nbins ? red[0][0] / (double)nbins : 0.0;
+        tones[2 * t + 1] = k < nbins ? red[1][0] / (double)nbins : 0.0;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -435,26 +454,26 @@ int fxk_codes_to_chan(hipStream_t s, const uint8_t* codes, int in_imajor, int nc
 }
 
 int fxk_pcal_fold(hipStream_t s, const uint32_t* words, size_t nsamp, size_t position, int nfold,
-                  double* acc)
+                  double* acc, int ninp, size_t in_stride_words)
 {
     size_t nw = (nsamp + 15) / 16;
-    if (nw == 0) return FX_OK;
+    if (nw == 0 || ninp <= 0) return FX_OK;
     int use_lds = nfold <= 8192;
     size_t lds = use_lds ? (size_t)nfold * sizeof(double) : 0;
     // at least 16 words per thread: each workgroup's nfold global additions
     // are amortized over >= 65536 samples
     const size_t blocks = std::min<size_t>(nblocks(nw, 256), std::max<size_t>(64, nw / 4096));
-    hipLaunchKernelGGL(k_pcal_fold, dim3((unsigned)blocks), dim3(256), lds, s, words, nsamp, position, nfold,
-                       use_lds, acc);
+    hipLaunchKernelGGL(k_pcal_fold, dim3((unsigned)blocks, (unsigned)ninp), dim3(256), lds, s, words, nsamp,
+                       position, nfold, use_lds, acc, in_stride_words);
     return last_status();
 }
 
 int fxk_pcal_final(hipStream_t s, const double* acc, int nfold, int nbins, int ntones,
-                   int offset_bin, int step, int rotatorlen, double dphi, double* tones)
+                   int offset_bin, int step, int rotatorlen, double dphi, double* tones, int ninp)
 {
-    if (ntones <= 0) return FX_OK;
-    hipLaunchKernelGGL(k_pcal_final, dim3(nblocks((size_t)ntones, 64)), dim3(64), 0, s, acc, nfold,
-                       nbins, ntones, offset_bin, step, rotatorlen, dphi, tones);
+    if (ntones <= 0 || ninp <= 0) return FX_OK;
+    hipLaunchKernelGGL(k_pcal_final, dim3((unsigned)ntones, (unsigned)ninp), dim3(256), 0, s, acc, nfold, nbins,
+                       ntones, offset_bin, step, rotatorlen, dphi, tones);
     return last_status();
 }
 

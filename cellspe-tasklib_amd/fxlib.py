@@ -104,6 +104,9 @@ def load():
         "fx_corr_tune": (i, [vp, l, i]),
         "fx_pcal_final_async": (i, [vp]),
         "fx_pcal_tones": (i, [vp, vp, C.POINTER(u64)]),
+        "fx_pcal_create_inputs": (i, [vp, d, d, i, sz, i, C.POINTER(vp)]),
+        "fx_pcal_extract_2bit_inputs": (i, [vp, vp, sz, sz]),
+        "fx_pcal_tones_input": (i, [vp, i, vp, C.POINTER(u64)]),
         "fx_batch_create": (i, [vp, vp, i, i, i, C.c_char_p, C.POINTER(vp)]),
         "fx_batch_run": (i, [vp]),
         "fx_batch_sync": (i, [vp]),
@@ -278,11 +281,12 @@ class Context:
 class PCal:
     """Device phase-cal extractor with the reference PCal interface."""
 
-    def __init__(self, ctx, bandwidth_hz, spacing_hz, offset_hz, sampleoffset=0):
+    def __init__(self, ctx, bandwidth_hz, spacing_hz, offset_hz, sampleoffset=0, ninp=1):
         self.ctx = ctx
+        self.ninp = ninp
         h = C.c_void_p()
-        check(load().fx_pcal_create(ctx.handle, bandwidth_hz, spacing_hz, int(offset_hz),
-                                    sampleoffset, C.byref(h)))
+        check(load().fx_pcal_create_inputs(ctx.handle, bandwidth_hz, spacing_hz, int(offset_hz),
+                                           sampleoffset, ninp, C.byref(h)))
         self.handle = h
         k, nb, nt = C.c_int(), C.c_int(), C.c_int()
         check(load().fx_pcal_info(h, C.byref(k), C.byref(nb), C.byref(nt)))
@@ -297,11 +301,18 @@ class PCal:
     def extract_2bit(self, d_words, nsamp):
         check(load().fx_pcal_extract_2bit(self.handle, d_words.ptr, nsamp))
 
+    def extract_2bit_inputs(self, d_words, stride_words, nsamp):
+        """All ninp inputs in one launch; input i starts stride_words after input i-1."""
+        check(load().fx_pcal_extract_2bit_inputs(self.handle, d_words.ptr, stride_words, nsamp))
+
     def final(self):
-        out = np.zeros(max(self.ntones, 1), dtype=np.complex128)
+        """ntones complex tones (ninp x ntones for a multi-input extractor)."""
+        out = np.zeros(max(self.ntones, 1) * self.ninp, dtype=np.complex128)
         n = C.c_uint64()
         check(load().fx_pcal_final(self.handle, _ptr(out), C.byref(n)))
-        return out[: self.ntones], n.value
+        if self.ninp == 1:
+            return out[: self.ntones], n.value
+        return out.reshape(self.ninp, -1)[:, : self.ntones], n.value
 
     def close(self):
         if self.handle is not None:

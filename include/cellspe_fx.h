@@ -153,13 +153,23 @@ int fx_pcal_clear(fx_pcal* pc);
 int fx_pcal_adjust_sample_offset(fx_pcal* pc, size_t sampleoffset);
 /* fold nsamp 2-bit samples (1-channel packing) starting at sample 0 of d_words */
 int fx_pcal_extract_2bit(fx_pcal* pc, const uint32_t* d_words, size_t nsamp);
-/* host output, ntones complex; returns samples integrated via *nsamples */
+/* host output, ntones complex per input (ninp x ntones for a multi-input
+ * extractor); returns samples integrated via *nsamples */
 int fx_pcal_final(fx_pcal* pc, fx_cf64* tones, uint64_t* nsamples);
 int fx_pcal_info(fx_pcal* pc, int* kind, int* nbins, int* ntones);
 /* graph-capturable split of fx_pcal_final: enqueue the tone extraction into
  * device memory, then (after the stream or graph ran) copy the tones out */
 int fx_pcal_final_async(fx_pcal* pc);
 int fx_pcal_tones(fx_pcal* pc, fx_cf64* tones, uint64_t* nsamples);
+/* ninp inputs with one shared tone configuration and sample position: the
+ * fold and the tone extraction are one launch each for all inputs.  Input i
+ * starts at d_words + i*in_stride_words; fx_pcal_tones_input reads input i.
+ * fx_pcal_create(...) is fx_pcal_create_inputs(..., 1, ...). */
+int fx_pcal_create_inputs(fx_ctx* ctx, double bandwidth_hz, double spacing_hz, int offset_hz,
+                          size_t sampleoffset, int ninp, fx_pcal** pc);
+int fx_pcal_extract_2bit_inputs(fx_pcal* pc, const uint32_t* d_words, size_t in_stride_words,
+                                size_t nsamp);
+int fx_pcal_tones_input(fx_pcal* pc, int input, fx_cf64* tones, uint64_t* nsamples);
 
 /* ---- a3/a4/a6/a7: correlator -------------------------------------------- */
 typedef struct {

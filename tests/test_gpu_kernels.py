@@ -237,3 +237,31 @@ def test_pcal_2bit(ctx, bw, spacing, offset):
     ref = O.pcal_final(cfg, acc)
     assert nrel(got, ref) < 1e-12
     pc.close()
+
+
+@pytest.mark.parametrize("bw,spacing,offset", [(8e6, 1e6, 10e3), (8e6, 1e6, 0)])
+def test_pcal_2bit_inputs(ctx, bw, spacing, offset):
+    """One multi-input extractor (one fold + one final launch for all inputs)
+    against the oracle per input, over two extract calls."""
+    import fxlib
+    fs = 2 * bw
+    nsamp, ninp = 1 << 17, 3
+    rows = [O.gen_station(90 + i, 0, 0.0, 0, 0.5, (offset + (i + 1) * spacing) / fs, 0.9816, nsamp)
+            for i in range(ninp)]
+    nw = nsamp // 16
+    stride = nw + 64
+    buf = np.zeros((ninp, stride), dtype=np.uint32)
+    for i in range(ninp):
+        buf[i, :nw] = rows[i]
+    half = nsamp // 2
+    pc = fxlib.PCal(ctx, bw, spacing, offset, ninp=ninp)
+    pc.extract_2bit_inputs(ctx.to_device(buf.ravel()), stride, half)
+    tail = np.ascontiguousarray(buf[:, half // 16:])
+    pc.extract_2bit_inputs(ctx.to_device(tail.ravel()), tail.shape[1], nsamp - half)
+    got, n = pc.final()
+    assert n == nsamp and got.shape == (ninp, pc.ntones)
+    cfg = O.pcal_config(bw, spacing, offset)
+    for i in range(ninp):
+        ref = O.pcal_final(cfg, O.pcal_extract(cfg, O.unpack_1ch(rows[i], nsamp), 0))
+        assert nrel(got[i], ref) < 1e-12, i
+    pc.close()
