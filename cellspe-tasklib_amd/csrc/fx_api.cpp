@@ -6,6 +6,7 @@
 // enqueues the gfx950 kernels of fx_fft.hip, fx_xengine.hip and fx_misc.hip.
 #include "fx_internal.h"
 
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -693,12 +694,15 @@ struct fx_corr {
     void* chan = nullptr;   // [c][t][i], chunk_spectra deep
     void* nat = nullptr;    // F-stage rows of one sub-chunk before the corner turn
     long sub = 0;           // spectra per F-stage sub-chunk (nat depth)
+    bool direct = false;    // F kernel writes the engine layout itself (no corner turn)
     float* acc = nullptr;   // engine-layout accumulators
     size_t acc_floats = 0;
     fx_phase_model* pm = nullptr;
     const float2* mult = nullptr;
     float* sc_chan = nullptr;
     float* sc_inp = nullptr;
+    bool sc_uniform = true;  // every channel scale equals sc_uni (the F kernel folds it per item)
+    float sc_uni = 1.0f;
     int2* pij = nullptr;
     float* coef = nullptr;  // FP8_2BIT: weights of the three accumulator sets
     int nsets = 1;
@@ -755,6 +759,8 @@ extern "C" int fx_corr_create(fx_ctx* ctx, const fx_corr_config* cfg, fx_corr** 
     const char* sub_env = getenv("FX_FSTAGE_SUB_MB");
     const size_t sub_target = (size_t)(sub_env ? std::max(1, atoi(sub_env)) : 64) << 20;
     c->sub = std::max<long>(8, std::min<long>(cfg->chunk_spectra, (long)(sub_target / row_bytes) / 8 * 8));
+    const char* direct_env = getenv("FX_FSTAGE_DIRECT");
+    c->direct = direct_env && atoi(direct_env) != 0;
     const size_t nat_bytes = twobit ? 0 : row_bytes * (size_t)c->sub;
     c->xtile = 128;
     c->acc_floats = fxk_xengine_acc_floats(c->nchan, cfg->ninp, c->nsets, c->xtile);
@@ -785,6 +791,8 @@ extern "C" int fx_corr_create(fx_ctx* ctx, const fx_corr_config* cfg, fx_corr** 
         // and above 28 sigma the format loses relative precision)
         const double target = cfg->prec == FX_PREC_FP8 ? 16.0 : 1.0;
         std::vector<float> sc((size_t)c->nchan, (float)(target / sqrt((double)cfg->nfft * 4.316)));
+        c->sc_uniform = true;
+        c->sc_uni = sc[0];
         std::vector<float> si((size_t)cfg->ninp, 1.0f);
         ok = hipMalloc(&c->sc_chan, sizeof(float) * sc.size()) == hipSuccess &&
              hipMalloc(&c->sc_inp, sizeof(float) * si.size()) == hipSuccess &&
@@ -856,6 +864,8 @@ extern "C" int fx_corr_set_scales(fx_corr* c, const float* chan_scale, const flo
     for (float v : si) FX_REQUIRE(v > 0.f && std::isfinite(v), FX_ERR_ARG, "scales must be positive");
     FX_HIP_TRY(hipMemcpy(c->sc_chan, sc.data(), sizeof(float) * sc.size(), hipMemcpyHostToDevice));
     FX_HIP_TRY(hipMemcpy(c->sc_inp, si.data(), sizeof(float) * si.size(), hipMemcpyHostToDevice));
+    c->sc_uniform = std::all_of(sc.begin(), sc.end(), [&](float v) { return v == sc[0]; });
+    c->sc_uni = c->sc_uniform ? sc[0] : 1.0f;
     return FX_OK;
 }
 
@@ -877,6 +887,17 @@ extern "C" int fx_corr_fstage(fx_corr* c, const uint32_t* d_words, size_t in_str
     FX_REQUIRE(out_prec != FX_PREC_FP8 || nspectra % 8 == 0, FX_ERR_ARG,
                "fx_corr_fstage: FP8 output needs a multiple of 8 spectra");
     if (nspectra == 0) return FX_OK;
+    const long I = c->cfg.ninp;
+    const bool imajor = out_prec == FX_PREC_FP8;
+    if (c->direct && fxk_fstage_direct_ok(c->cfg.nfft, (int)I, nspectra, out_prec, nspectra * I, 0, nspectra)) {
+        const bool scaled = out_prec != FX_PREC_F32;
+        FX_CHECK_LAUNCH(fxk_fstage_direct(c->ctx->stream, c->ctx->tw, d_words, in_stride_words, (int)I,
+                                          c->cfg.nfft, t_first, nspectra, c->cfg.use_phase ? c->pm : nullptr,
+                                          c->mult, scaled && !c->sc_uniform ? c->sc_chan : nullptr,
+                                          scaled && c->sc_uniform ? c->sc_uni : 1.0f, scaled ? c->sc_inp : nullptr,
+                                          out_prec, d_chan, nspectra * I, 0, imajor ? nspectra : 0));
+        return FX_OK;
+    }
     // sub-chunks: F kernel into the natural-row buffer, corner turn into d_chan
     for (long s0 = 0; s0 < nspectra; s0 += c->sub) {
         const long ns = std::min(c->sub, nspectra - s0);
@@ -906,7 +927,8 @@ extern "C" int fx_corr_fstage_rows(fx_corr* c, const uint32_t* d_words, size_t i
     const bool scaled = out_prec != FX_PREC_F32;
     FX_CHECK_LAUNCH(fxk_fstage(c->ctx->stream, c->ctx->tw, d_words, in_stride_words, c->cfg.ninp,
                                c->cfg.nfft, t_first, nspectra, c->cfg.use_phase ? c->pm : nullptr,
-                               c->mult, scaled ? c->sc_chan : nullptr, scaled ? c->sc_inp : nullptr,
+                               c->mult, scaled && !c->sc_uniform ? c->sc_chan : nullptr,
+                               scaled && c->sc_uniform ? c->sc_uni : 1.0f, scaled ? c->sc_inp : nullptr,
                                out_prec, c->nat, nullptr));
     return FX_OK;
 }

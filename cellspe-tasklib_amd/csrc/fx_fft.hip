@@ -250,51 +250,68 @@ __device__ __forceinline__ uint16_t to_chan<uint16_t>(float2 v)
     return (uint16_t)((uint16_t)r.__x | ((uint16_t)i.__x << 8));
 }
 
-template <int M, typename T>
-__global__ __launch_bounds__(FftShape<M>::NT) void k_fstage(
-    const float2* __restrict__ twg, const uint32_t* __restrict__ words, size_t in_stride, int ninp,
-    long t_first, long nt, const fx_phase_model* __restrict__ pm, const float2* __restrict__ mult,
-    const float* __restrict__ sc_chan, const float* __restrict__ sc_inp, T* __restrict__ nat)
+// Phasor tables of the F kernels: channel k = PLO hi + lo of an item rotates
+// by ph_hi[hi] * ph_lo[lo] = e^{2 pi i (k tau/N + fringe/2pi)}, both factors
+// evaluated from the FP64 phase model (sincospi in FP64, one rounding to
+// FP32), one complex multiply per channel.  The item's amplitude factor
+// amp = 1/2 (of the real split) x the uniform channel scale x the input scale
+// rides in ph_lo (or multiplies each channel when there is no phase model).
+template <int M>
+struct PhaseTab {
+    static constexpr int PLO = M < 64 ? M : 64;
+    static constexpr int PHI = M / PLO;
+};
+
+template <int M, int F>
+struct FItems {
+    float2 ph_lo[F][PhaseTab<M>::PLO];
+    float2 ph_hi[F][PhaseTab<M>::PHI];
+    float amp[F];
+    int t[F], i[F];   // (spectrum, input) of item f; t = -1 past the end
+};
+
+// Front of the F kernels for the F items g0 .. g0+F-1: item bookkeeping, the
+// FP64 phasor tables, the LDS FFT.  Item order: g = t*I + i, or g = i*nt + t
+// when IM.  The packed words of the items are staged at the head of sh and
+// the first radix-16 pass reads its 16 points per butterfly straight from
+// them: complex point e of an item is the 4-bit field e%8 of word e/8, one
+// lookup in the 16-entry table lut (both samples of the point).
+template <int M, int F, int NT, bool IM>
+__device__ __forceinline__ void fstage_front(const float2* __restrict__ twg, const uint32_t* __restrict__ words,
+                                             size_t in_stride, int ninp, long t_first, long nt,
+                                             const fx_phase_model* __restrict__ pm, float sc_uni,
+                                             const float* __restrict__ sc_inp, long g0, float2* sh, float2* tw,
+                                             float2* tw2, float2* lut, FItems<M, F>& it, int tid)
 {
-    using S = FftShape<M>;
-    constexpr int F = S::F;
-    constexpr int NT = S::NT;
     constexpr int N = 2 * M;
-    constexpr int WPT = N / 16;            // packed words per transform
-    constexpr int WTH = (S::FM / 8) / NT;  // packed words per thread (2)
-    typedef typename ChanT<T>::V V;
-    // item order: g = t*I + i (FP32/FP16 engines, [c][t][i] after the corner
-    // turn) or g = i*nt + t (FP8 engine, [c][i][t])
-    constexpr bool IM = std::is_same<T, uint16_t>::value;
-    // phasor tables: channel k = 64 hi + lo of item f rotates by
-    // ph_hi[f][hi] * ph_lo[f][lo] = e^{2 pi i (k tau/N + fringe/2pi)}, both
-    // factors evaluated from the FP64 phase model (sincospi in FP64, one
-    // rounding to FP32), one complex multiply per channel
-    constexpr int PLO = M < 64 ? M : 64;
-    constexpr int PHI = M / PLO;
-    __shared__ float2 sh[S::LDS];
-    __shared__ float2 tw[FX_TWL];
-    __shared__ float2 ph_lo[F][PLO];
-    __shared__ float2 ph_hi[F][PHI];
-    __shared__ int it_t[F], it_i[F];      // (spectrum, input) of item f; -1 past the end
-    const int tid = threadIdx.x;
+    constexpr int WPT = N / 16;             // packed words per transform
+    constexpr int WTH = (F * M / 8) / NT;   // packed words per thread
+    constexpr int PLO = PhaseTab<M>::PLO;
+    constexpr int PHI = PhaseTab<M>::PHI;
+    static_assert(WTH % 2 == 0 && M >= 128, "F kernel shape");
     twl_load(tw, twg, tid, NT);
+    if (tid < 16) lut[tid] = make_float2(lut2((uint32_t)tid & 3u), lut2((uint32_t)tid >> 2));
+    if constexpr (M >= 256) {
+        // second-pass twiddles W_256^{r k}, r-major
+        for (int e = tid; e < 256; e += NT) tw2[e] = twg[(e >> 4) * (e & 15) * (FX_TW_N / 256)];
+    }
     const long nitems = nt * ninp;
-    const long g0 = (long)blockIdx.x * F;
     if (tid < F) {
         const long g = g0 + tid;
         const long t = IM ? g % nt : g / ninp;
-        it_t[tid] = g < nitems ? (int)t : -1;
-        it_i[tid] = (int)(IM ? g / nt : g - t * ninp);
+        const int i = (int)(IM ? g / nt : g - t * ninp);
+        it.t[tid] = g < nitems ? (int)t : -1;
+        it.i[tid] = i;
+        it.amp[tid] = 0.5f * sc_uni * ((sc_inp && g < nitems) ? sc_inp[i] : 1.0f);
     }
     __syncthreads();
     if (pm) {
         for (int e = tid; e < F * (PLO + PHI); e += NT) {
             const int f = e / (PLO + PHI);
             const int j = e - f * (PLO + PHI);
-            if (it_t[f] < 0) continue;
-            const fx_phase_model pmi = pm[it_i[f]];
-            const double tg = (double)(t_first + it_t[f]);
+            if (it.t[f] < 0) continue;
+            const fx_phase_model pmi = pm[it.i[f]];
+            const double tg = (double)(t_first + it.t[f]);
             const double tau = pmi.delay0 + pmi.delay_rate * tg;
             double cyc;
             if (j < PLO) {
@@ -306,139 +323,265 @@ __global__ __launch_bounds__(FftShape<M>::NT) void k_fstage(
             cyc -= rint(cyc);
             double sn, cs;
             sincospi(2.0 * cyc, &sn, &cs);
-            if (j < PLO)
-                ph_lo[f][j] = make_float2((float)cs, (float)sn);
-            else
-                ph_hi[f][j - PLO] = make_float2((float)cs, (float)sn);
+            if (j < PLO) {
+                const double a = (double)it.amp[f];
+                it.ph_lo[f][j] = make_float2((float)(cs * a), (float)(sn * a));
+            } else {
+                it.ph_hi[f][j - PLO] = make_float2((float)cs, (float)sn);
+            }
         }
     }
-
-    // ---- load + unpack F transforms: 16 consecutive complex per thread, one
-    // padded LDS block (zb is a multiple of 16)
+    uint32_t* raw = reinterpret_cast<uint32_t*>(sh);
     {
         const int wl = WTH * tid;
         const int f = wl / WPT;
         const int wof = wl - f * WPT;
-        uint32_t w[WTH];
+        const bool valid = it.t[f] >= 0;
+        const uint32_t* src = words + (size_t)it.i[f] * in_stride + (size_t)(t_first + it.t[f]) * WPT + wof;
 #pragma unroll
-        for (int q = 0; q < WTH; q++) w[q] = 0;
-        if (it_t[f] >= 0) {
-            const uint32_t* src =
-                words + (size_t)it_i[f] * in_stride + (size_t)(t_first + it_t[f]) * WPT + wof;
-#pragma unroll
-            for (int q = 0; q < WTH; q += 2) {
-                const uint2 ww = *reinterpret_cast<const uint2*>(src + q);
-                w[q] = ww.x;
-                w[q + 1] = ww.y;
-            }
-        }
-        float2* z = sh + lpad(f * M + wof * 8);
-#pragma unroll
-        for (int q = 0; q < WTH; q++)
-#pragma unroll
-            for (int m = 0; m < 8; m++)
-                z[8 * q + m] = make_float2(lut2((w[q] >> (4 * m)) & 3u), lut2((w[q] >> (4 * m + 2)) & 3u));
+        for (int q = 0; q < WTH; q += 2)
+            *reinterpret_cast<uint2*>(raw + wl + q) =
+                valid ? *reinterpret_cast<const uint2*>(src + q) : make_uint2(0u, 0u);
     }
     __syncthreads();
-    fft_passes<M, 1, NT, F>(sh, tw, tid);
+    {
+        constexpr int S1 = M / 16;
+        constexpr int BPT = (F * M / 16) / NT;
+        static_assert((F * M / 16) % NT == 0, "butterflies must divide evenly");
+        float2 u[BPT][16];
+#pragma unroll
+        for (int b = 0; b < BPT; b++) {
+            const int g = tid + b * NT;
+            const int f = g / S1;
+            const int j = g - f * S1;
+            const uint32_t* rw = raw + f * WPT + (j >> 3);
+            const int s4 = 4 * (j & 7);
+#pragma unroll
+            for (int r = 0; r < 16; r++) u[b][r] = lut[(rw[r * (S1 >> 3)] >> s4) & 15u];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int b = 0; b < BPT; b++) {
+            const int g = tid + b * NT;
+            const int f = g / S1;
+            const int j = g - f * S1;
+            dft_reg<16>(u[b]);
+            float2* o = sh + lpad(f * M + j * 16);
+#pragma unroll
+            for (int r = 0; r < 16; r++) o[r] = u[b][r];
+        }
+        __syncthreads();
+    }
+    fft_passes<M, 16, NT, F, true>(sh, tw, tid, tw2);
+}
 
-    // ---- split, rotate, scale, convert; slot q -> (f, kp) with kp fastest so
-    // lanes write consecutive channels of one row
+// Channel k of item f from twice its split value: rotate by the phasor (which
+// carries the item amplitude) or scale by the amplitude, then the optional
+// per-channel multiplier and per-channel scale, then the engine format.
+// PMO: phase model only (no multiplier, no per-channel scales), decided at
+// compile time so the epilogue is straight-line code.
+template <int M, typename T, int F, bool PMO>
+__device__ __forceinline__ typename ChanT<T>::V fstage_finish(float2 x, const FItems<M, F>& it, int f, int k,
+                                                              float2 plo, bool pm,
+                                                              const float2* __restrict__ mult,
+                                                              const float* __restrict__ sc_chan)
+{
+    constexpr int PLO = PhaseTab<M>::PLO;
+    if constexpr (PMO) return to_chan<T>(cmul(x, cmul(it.ph_hi[f][k / PLO], plo)));
+    if (pm)
+        x = cmul(x, cmul(it.ph_hi[f][k / PLO], plo));
+    else
+        x = make_float2(x.x * it.amp[f], x.y * it.amp[f]);
+    if (mult) x = cmul(x, mult[(size_t)it.i[f] * M + k]);
+    if (sc_chan) {
+        const float a = sc_chan[k];
+        x.x *= a;
+        x.y *= a;
+    }
+    return to_chan<T>(x);
+}
+
+// sc_chan: per-channel scales, or nullptr when every channel has the scale
+// sc_uni; sc_inp: per-input scales or nullptr.  PMO: launched with a phase
+// model, no multiplier and no per-channel scales.
+template <int M, typename T, bool PMO>
+__global__ __launch_bounds__(FftShape<M>::NT) void k_fstage(
+    const float2* __restrict__ twg, const uint32_t* __restrict__ words, size_t in_stride, int ninp,
+    long t_first, long nt, const fx_phase_model* __restrict__ pm, const float2* __restrict__ mult,
+    const float* __restrict__ sc_chan, float sc_uni, const float* __restrict__ sc_inp, T* __restrict__ nat)
+{
+    using S = FftShape<M>;
+    constexpr int F = S::F;
+    constexpr int NT = S::NT;
+    constexpr int N = 2 * M;
+    typedef typename ChanT<T>::V V;
+    // item order: g = t*I + i (FP32/FP16 engines, [c][t][i] after the corner
+    // turn) or g = i*nt + t (FP8 engine, [c][i][t])
+    constexpr bool IM = std::is_same<T, uint16_t>::value;
+    constexpr int PLO = PhaseTab<M>::PLO;
+    __shared__ float2 sh[S::LDS];
+    __shared__ float2 tw[FX_TWL];
+    __shared__ float2 tw2[256];
+    __shared__ float2 lut[16];
+    __shared__ FItems<M, F> it;
+    const int tid = threadIdx.x;
+    const long g0 = (long)blockIdx.x * F;
+    fstage_front<M, F, NT, IM>(twg, words, in_stride, ninp, t_first, nt, pm, sc_uni, sc_inp, g0, sh, tw, tw2,
+                               lut, it, tid);
+    const bool has_pm = PMO || pm != nullptr;
+
+    // ---- split, rotate, scale, convert; lanes write consecutive channels of
+    // one row
     V* o = reinterpret_cast<V*>(nat);
     constexpr int H = M / 2;
     if constexpr (H >= NT && NT % PLO == 0) {
         // thread tid owns channels kp = q NT + tid of every item: the split
-        // twiddle W_N^kp advances by W_N^NT per q, and the low phasor factors
-        // of kp and M - kp are the same for every q
+        // twiddle W_N^kp advances by W_N^NT per q (shared by the items), and
+        // the low phasor factors of kp and M - kp are the same for every q
         const float2 wstep = twl_get(tw, NT * (FX_TW_N / N));
         const float2 w0 = twl_get(tw, tid * (FX_TW_N / N));
         const int lo_k = tid % PLO, lo_m = (PLO - lo_k) % PLO;
+        // blocks whose items all exist (all but the last) skip the per-item test
+        auto body = [&](auto checked) {
+        float2 w = w0;
 #pragma unroll
-        for (int f = 0; f < F; f++) {
-            if (it_t[f] < 0) continue;
-            const int i = it_i[f];
-            V* row = o + (size_t)(g0 + f) * M;
-            const float2 plo_k = pm ? ph_lo[f][lo_k] : make_float2(1.f, 0.f);
-            const float2 plo_m = pm ? ph_lo[f][lo_m] : make_float2(1.f, 0.f);
-            const float si = sc_chan ? sc_inp[i] : 1.f;
-            float2 w = w0;
+        for (int q = 0; q < H / NT; q++) {
+            const int kp = q * NT + tid;
+            if (q) w = cmul(w, wstep);
+            const int k2 = M - kp;
+            const int k2m = k2 & (M - 1);   // kp = 0: k2 = M is the dropped Nyquist bin
 #pragma unroll
-            for (int q = 0; q < H / NT; q++) {
-                const int kp = q * NT + tid;
-                if (q) w = cmul(w, wstep);
+            for (int f = 0; f < F; f++) {
+                if constexpr (decltype(checked)::value) {
+                    if (it.t[f] < 0) continue;
+                }
+                V* row = o + (size_t)(g0 + f) * M;
                 float2 xk, xm;
-                rsplit_tw<M>(sh, f * M, kp, w, xk, xm);
-                const int k2 = M - kp;
-                const int k2m = k2 & (M - 1);   // kp = 0: k2 = M is the dropped Nyquist bin
-                if (pm) {
-                    xk = cmul(xk, cmul(ph_hi[f][kp / PLO], plo_k));
-                    xm = cmul(xm, cmul(ph_hi[f][k2m / PLO], plo_m));
-                }
-                if (mult) {
-                    xk = cmul(xk, mult[(size_t)i * M + kp]);
-                    xm = cmul(xm, mult[(size_t)i * M + k2m]);
-                }
-                if (sc_chan) {
-                    const float a = sc_chan[kp] * si;
-                    const float b = sc_chan[k2m] * si;
-                    xk.x *= a;
-                    xk.y *= a;
-                    xm.x *= b;
-                    xm.y *= b;
-                }
-                row[kp] = to_chan<T>(xk);
-                if (kp != 0) row[k2] = to_chan<T>(xm);
+                rsplit2_tw<M>(sh, f * M, kp, w, xk, xm);
+                row[kp] = fstage_finish<M, T, F, PMO>(xk, it, f, kp, has_pm ? it.ph_lo[f][lo_k] : make_float2(1.f, 0.f),
+                                                 has_pm, mult, sc_chan);
+                if (kp != 0)
+                    row[k2] = fstage_finish<M, T, F, PMO>(xm, it, f, k2m,
+                                                     has_pm ? it.ph_lo[f][lo_m] : make_float2(1.f, 0.f), has_pm,
+                                                     mult, sc_chan);
             }
         }
+        };
+        if (it.t[F - 1] >= 0)
+            body(std::false_type{});
+        else
+            body(std::true_type{});
     } else {
 #pragma unroll
-    for (int it = 0; it < (F * H) / NT; it++) {
-        // H >= NT: item f is fixed per iteration (it * NT / H), no division
-        const int f = H >= NT ? (it * NT) / H : (it * NT + tid) / H;
-        const int kp = H >= NT ? (it * NT) % H + tid : (it * NT + tid) % H;
-        if (it_t[f] < 0) continue;
-        const int i = it_i[f];
-        const long g = g0 + f;
-        float2 xk, xm;
-        rsplit<M>(sh, f * M, kp, tw, xk, xm);
-        const int k2 = M - kp;
-        const int k2m = k2 & (M - 1);   // kp = 0: k2 = M is the dropped Nyquist bin
-        if (pm) {
-            xk = cmul(xk, cmul(ph_hi[f][kp / PLO], ph_lo[f][kp % PLO]));
-            xm = cmul(xm, cmul(ph_hi[f][k2m / PLO], ph_lo[f][k2m % PLO]));
+        for (int q = 0; q < (F * H) / NT; q++) {
+            const int f = H >= NT ? (q * NT) / H : (q * NT + tid) / H;
+            const int kp = H >= NT ? (q * NT) % H + tid : (q * NT + tid) % H;
+            if (it.t[f] < 0) continue;
+            float2 xk, xm;
+            rsplit2_tw<M>(sh, f * M, kp, twl_get(tw, kp * (FX_TW_N / N)), xk, xm);
+            const int k2 = M - kp;
+            const int k2m = k2 & (M - 1);
+            V* row = o + (size_t)(g0 + f) * M;
+            row[kp] = fstage_finish<M, T, F, PMO>(xk, it, f, kp, has_pm ? it.ph_lo[f][kp % PLO] : make_float2(1.f, 0.f),
+                                             has_pm, mult, sc_chan);
+            if (kp != 0)
+                row[k2] = fstage_finish<M, T, F, PMO>(xm, it, f, k2m,
+                                                 has_pm ? it.ph_lo[f][k2m % PLO] : make_float2(1.f, 0.f), has_pm,
+                                                 mult, sc_chan);
         }
-        if (mult) {
-            xk = cmul(xk, mult[(size_t)i * M + kp]);
-            xm = cmul(xm, mult[(size_t)i * M + k2m]);
-        }
-        if (sc_chan) {
-            const float si = sc_inp[i];
-            const float a = sc_chan[kp] * si;
-            const float b = sc_chan[k2m] * si;
-            xk.x *= a;
-            xk.y *= a;
-            xm.x *= b;
-            xm.y *= b;
-        }
-        V* row = o + (size_t)g * M;
-        row[kp] = to_chan<T>(xk);
-        if (kp != 0) row[k2] = to_chan<T>(xm);
-    }
     }
     // kp = M/2 of each transform
-    if (tid < F && it_t[tid] >= 0) {
-        const long g = g0 + tid;
-        const int i = it_i[tid];
+    if (tid < F && it.t[tid] >= 0) {
         float2 xk, xm;
-        rsplit<M>(sh, tid * M, M / 2, tw, xk, xm);
-        if (pm) xk = cmul(xk, cmul(ph_hi[tid][(M / 2) / PLO], ph_lo[tid][(M / 2) % PLO]));
-        if (mult) xk = cmul(xk, mult[(size_t)i * M + M / 2]);
-        if (sc_chan) {
-            const float a = sc_chan[M / 2] * sc_inp[i];
-            xk.x *= a;
-            xk.y *= a;
+        rsplit2_tw<M>(sh, tid * M, M / 2, twl_get(tw, (M / 2) * (FX_TW_N / N)), xk, xm);
+        o[(size_t)(g0 + tid) * M + M / 2] = fstage_finish<M, T, F, PMO>(
+            xk, it, tid, M / 2, has_pm ? it.ph_lo[tid][(M / 2) % PLO] : make_float2(1.f, 0.f), has_pm, mult, sc_chan);
+    }
+}
+
+// Store of F consecutive channelized values as 16-byte (or 8-byte) vectors.
+template <typename V, int F>
+__device__ __forceinline__ void store_run(V* dst, const V (&v)[F])
+{
+    constexpr int B = (int)sizeof(V) * F;
+    if constexpr (B % 16 == 0) {
+#pragma unroll
+        for (int j = 0; j < B / 16; j++)
+            reinterpret_cast<uint4*>(dst)[j] = reinterpret_cast<const uint4*>(v)[j];
+    } else if constexpr (B == 8) {
+        *reinterpret_cast<uint2*>(dst) = *reinterpret_cast<const uint2*>(v);
+    } else {
+#pragma unroll
+        for (int f = 0; f < F; f++) dst[f] = v[f];
+    }
+}
+
+// F kernel with the corner turn folded in: a 1024-thread workgroup transforms
+// F = 16384/M items that are F consecutive columns of the engine layout
+// (F inputs of one spectrum for [c][t][i], or F spectra of one input for the
+// FP8 [c][i][t]), so each lane stores one channel of all F items as one
+// 8-32 byte vector out[k][col .. col+F-1].  The workgroups of neighbouring
+// columns run on one XCD (xcd_block), so the partial lines of a spectrum meet
+// in that XCD's L2 and reach HBM as (mostly) whole lines.  The host
+// guarantees that every workgroup's F items exist and that col, ostride and
+// ostride_i are multiples of F.
+template <int M, typename T, bool PMO>
+__global__ __launch_bounds__(1024) void k_fstage_direct(
+    const float2* __restrict__ twg, const uint32_t* __restrict__ words, size_t in_stride, int ninp,
+    long t_first, long nt, const fx_phase_model* __restrict__ pm, const float2* __restrict__ mult,
+    const float* __restrict__ sc_chan, float sc_uni, const float* __restrict__ sc_inp,
+    typename ChanT<T>::V* __restrict__ out, long ostride, long col0, long ostride_i)
+{
+    constexpr int NT = 1024;
+    constexpr int F = 16384 / M;
+    constexpr int N = 2 * M;
+    constexpr int H = M / 2;
+    static_assert(H % NT == 0 && NT % 64 == 0, "direct F kernel: M in 2048 .. 8192");
+    typedef typename ChanT<T>::V V;
+    constexpr bool IM = std::is_same<T, uint16_t>::value;
+    constexpr int PLO = PhaseTab<M>::PLO;
+    __shared__ float2 sh[lds_complex(F * M)];
+    __shared__ float2 tw[FX_TWL];
+    __shared__ float2 tw2[256];
+    __shared__ float2 lut[16];
+    __shared__ FItems<M, F> it;
+    const int tid = threadIdx.x;
+    const long nblk = nt * ninp / F;
+    const long lb = xcd_block(nblk);
+    if (lb >= nblk) return;
+    const long g0 = lb * F;
+    fstage_front<M, F, NT, IM>(twg, words, in_stride, ninp, t_first, nt, pm, sc_uni, sc_inp, g0, sh, tw, tw2,
+                               lut, it, tid);
+    const bool has_pm = PMO || pm != nullptr;
+    const long col = IM ? (g0 / nt) * ostride_i + col0 + (g0 % nt) : col0 + g0;
+    const float2 wstep = twl_get(tw, NT * (FX_TW_N / N));
+    float2 w = twl_get(tw, tid * (FX_TW_N / N));
+    const int lo_k = tid % PLO, lo_m = (PLO - lo_k) % PLO;
+#pragma unroll
+    for (int q = 0; q < H / NT; q++) {
+        const int kp = q * NT + tid;
+        if (q) w = cmul(w, wstep);
+        const int k2 = M - kp;
+        const int k2m = k2 & (M - 1);   // kp = 0: k2 = M is the dropped Nyquist bin
+        V vk[F], vm[F];
+#pragma unroll
+        for (int f = 0; f < F; f++) {
+            float2 xk, xm;
+            rsplit2_tw<M>(sh, f * M, kp, w, xk, xm);
+            vk[f] = fstage_finish<M, T, F, PMO>(xk, it, f, kp, has_pm ? it.ph_lo[f][lo_k] : make_float2(1.f, 0.f), has_pm,
+                                           mult, sc_chan);
+            vm[f] = fstage_finish<M, T, F, PMO>(xm, it, f, k2m, has_pm ? it.ph_lo[f][lo_m] : make_float2(1.f, 0.f),
+                                           has_pm, mult, sc_chan);
         }
-        o[(size_t)g * M + M / 2] = to_chan<T>(xk);
+        store_run<V, F>(out + (size_t)kp * ostride + col, vk);
+        if (kp != 0) store_run<V, F>(out + (size_t)k2 * ostride + col, vm);
+    }
+    // kp = M/2 of each item
+    if (tid < F) {
+        float2 xk, xm;
+        rsplit2_tw<M>(sh, tid * M, M / 2, twl_get(tw, (M / 2) * (FX_TW_N / N)), xk, xm);
+        out[(size_t)(M / 2) * ostride + col + tid] = fstage_finish<M, T, F, PMO>(
+            xk, it, tid, M / 2, has_pm ? it.ph_lo[tid][(M / 2) % PLO] : make_float2(1.f, 0.f), has_pm, mult, sc_chan);
     }
 }
 
@@ -531,14 +674,18 @@ int launch_cornerturn_t(hipStream_t s, const void* nat, void* out, long nitems, 
 template <int M, typename T>
 int launch_fstage_t(hipStream_t s, const float2* tw, const uint32_t* words, size_t stride,
                     int ninp, long t_first, long nt, const fx_phase_model* pm, const float2* mult,
-                    const float* sc, const float* si, void* nat, void* out)
+                    const float* sc, float su, const float* si, void* nat, void* out)
 {
     using S = FftShape<M>;
     typedef typename ChanT<T>::V V;
     long nitems = nt * ninp;
     long nblk = (nitems + S::F - 1) / S::F;
-    hipLaunchKernelGGL((k_fstage<M, T>), dim3((unsigned)nblk), dim3(S::NT), 0, s, tw, words, stride,
-                       ninp, t_first, nt, pm, mult, sc, si, reinterpret_cast<T*>(nat));
+    if (pm && !mult && !sc)
+        hipLaunchKernelGGL((k_fstage<M, T, true>), dim3((unsigned)nblk), dim3(S::NT), 0, s, tw, words, stride,
+                           ninp, t_first, nt, pm, mult, sc, su, si, reinterpret_cast<T*>(nat));
+    else
+        hipLaunchKernelGGL((k_fstage<M, T, false>), dim3((unsigned)nblk), dim3(S::NT), 0, s, tw, words, stride,
+                           ninp, t_first, nt, pm, mult, sc, su, si, reinterpret_cast<T*>(nat));
     if (hipGetLastError() != hipSuccess) return FX_ERR_HIP;
     if (!out) return FX_OK;
     return launch_cornerturn_t<V>(s, nat, out, nitems, M, nitems, 0, std::is_same<T, uint16_t>::value ? 1 : 0,
@@ -548,15 +695,56 @@ int launch_fstage_t(hipStream_t s, const float2* tw, const uint32_t* words, size
 template <int M>
 int launch_fstage(hipStream_t s, const float2* tw, const uint32_t* words, size_t stride, int ninp,
                   long t_first, long nt, const fx_phase_model* pm, const float2* mult,
-                  const float* sc, const float* si, int out_type, void* nat, void* out)
+                  const float* sc, float su, const float* si, int out_type, void* nat, void* out)
 {
     switch (out_type) {
     case FX_PREC_F32:
-        return launch_fstage_t<M, float2>(s, tw, words, stride, ninp, t_first, nt, pm, mult, sc, si, nat, out);
+        return launch_fstage_t<M, float2>(s, tw, words, stride, ninp, t_first, nt, pm, mult, sc, su, si, nat, out);
     case FX_PREC_F16:
-        return launch_fstage_t<M, __half2>(s, tw, words, stride, ninp, t_first, nt, pm, mult, sc, si, nat, out);
+        return launch_fstage_t<M, __half2>(s, tw, words, stride, ninp, t_first, nt, pm, mult, sc, su, si, nat, out);
     case FX_PREC_FP8:
-        return launch_fstage_t<M, uint16_t>(s, tw, words, stride, ninp, t_first, nt, pm, mult, sc, si, nat, out);
+        return launch_fstage_t<M, uint16_t>(s, tw, words, stride, ninp, t_first, nt, pm, mult, sc, su, si, nat, out);
+    default:
+        return FX_ERR_ARG;
+    }
+}
+
+template <int M, typename T>
+int launch_fstage_direct_t(hipStream_t s, const float2* tw, const uint32_t* words, size_t stride, int ninp,
+                           long t_first, long nt, const fx_phase_model* pm, const float2* mult, const float* sc, float su,
+                           const float* si, void* out, long ostride, long col0, long ostride_i)
+{
+    constexpr int F = 16384 / M;
+    constexpr bool IM = std::is_same<T, uint16_t>::value;
+    if ((IM ? nt : (long)ninp) % F || ostride % F || col0 % F || (IM && ostride_i % F)) return FX_ERR_ARG;
+    const long nblk = nt * ninp / F;
+    if (nblk == 0) return FX_OK;
+    if (pm && !mult && !sc)
+        hipLaunchKernelGGL((k_fstage_direct<M, T, true>), dim3((unsigned)xcd_grid(nblk)), dim3(1024), 0, s, tw,
+                           words, stride, ninp, t_first, nt, pm, mult, sc, su, si,
+                           reinterpret_cast<typename ChanT<T>::V*>(out), ostride, col0, ostride_i);
+    else
+        hipLaunchKernelGGL((k_fstage_direct<M, T, false>), dim3((unsigned)xcd_grid(nblk)), dim3(1024), 0, s, tw,
+                           words, stride, ninp, t_first, nt, pm, mult, sc, su, si,
+                           reinterpret_cast<typename ChanT<T>::V*>(out), ostride, col0, ostride_i);
+    return hipGetLastError() == hipSuccess ? FX_OK : FX_ERR_HIP;
+}
+
+template <int M>
+int launch_fstage_direct(hipStream_t s, const float2* tw, const uint32_t* words, size_t stride, int ninp,
+                         long t_first, long nt, const fx_phase_model* pm, const float2* mult, const float* sc, float su,
+                         const float* si, int out_type, void* out, long ostride, long col0, long ostride_i)
+{
+    switch (out_type) {
+    case FX_PREC_F32:
+        return launch_fstage_direct_t<M, float2>(s, tw, words, stride, ninp, t_first, nt, pm, mult, sc, su, si, out,
+                                                 ostride, col0, ostride_i);
+    case FX_PREC_F16:
+        return launch_fstage_direct_t<M, __half2>(s, tw, words, stride, ninp, t_first, nt, pm, mult, sc, su, si, out,
+                                                  ostride, col0, ostride_i);
+    case FX_PREC_FP8:
+        return launch_fstage_direct_t<M, uint16_t>(s, tw, words, stride, ninp, t_first, nt, pm, mult, sc, su, si,
+                                                   out, ostride, col0, ostride_i);
     default:
         return FX_ERR_ARG;
     }
@@ -629,16 +817,16 @@ int fxk_spectrometer(hipStream_t s, const float2* tw, const float* win, const ui
 
 int fxk_fstage(hipStream_t s, const float2* tw, const uint32_t* words, size_t stride, int ninp,
                int n, long t_first, long nt, const fx_phase_model* pm, const float2* mult,
-               const float* sc, const float* si, int out_type, void* nat, void* out)
+               const float* sc, float su, const float* si, int out_type, void* nat, void* out)
 {
     switch (n) {
-    case 256: return launch_fstage<128>(s, tw, words, stride, ninp, t_first, nt, pm, mult, sc, si, out_type, nat, out);
-    case 512: return launch_fstage<256>(s, tw, words, stride, ninp, t_first, nt, pm, mult, sc, si, out_type, nat, out);
-    case 1024: return launch_fstage<512>(s, tw, words, stride, ninp, t_first, nt, pm, mult, sc, si, out_type, nat, out);
-    case 2048: return launch_fstage<1024>(s, tw, words, stride, ninp, t_first, nt, pm, mult, sc, si, out_type, nat, out);
-    case 4096: return launch_fstage<2048>(s, tw, words, stride, ninp, t_first, nt, pm, mult, sc, si, out_type, nat, out);
-    case 8192: return launch_fstage<4096>(s, tw, words, stride, ninp, t_first, nt, pm, mult, sc, si, out_type, nat, out);
-    case 16384: return launch_fstage<8192>(s, tw, words, stride, ninp, t_first, nt, pm, mult, sc, si, out_type, nat, out);
+    case 256: return launch_fstage<128>(s, tw, words, stride, ninp, t_first, nt, pm, mult, sc, su, si, out_type, nat, out);
+    case 512: return launch_fstage<256>(s, tw, words, stride, ninp, t_first, nt, pm, mult, sc, su, si, out_type, nat, out);
+    case 1024: return launch_fstage<512>(s, tw, words, stride, ninp, t_first, nt, pm, mult, sc, su, si, out_type, nat, out);
+    case 2048: return launch_fstage<1024>(s, tw, words, stride, ninp, t_first, nt, pm, mult, sc, su, si, out_type, nat, out);
+    case 4096: return launch_fstage<2048>(s, tw, words, stride, ninp, t_first, nt, pm, mult, sc, su, si, out_type, nat, out);
+    case 8192: return launch_fstage<4096>(s, tw, words, stride, ninp, t_first, nt, pm, mult, sc, su, si, out_type, nat, out);
+    case 16384: return launch_fstage<8192>(s, tw, words, stride, ninp, t_first, nt, pm, mult, sc, su, si, out_type, nat, out);
     default: return FX_ERR_ARG;
     }
 }
@@ -656,4 +844,25 @@ int fxk_cornerturn(hipStream_t s, int out_type, const void* nat, void* out, long
         return launch_cornerturn_t<uint16_t>(s, nat, out, nitems, nchan, ostride, col0, 1, ns, ostride_i);
     default: return FX_ERR_ARG;
     }
+}
+
+int fxk_fstage_direct(hipStream_t s, const float2* tw, const uint32_t* words, size_t stride, int ninp, int n,
+                      long t_first, long nt, const fx_phase_model* pm, const float2* mult, const float* sc, float su,
+                      const float* si, int out_type, void* out, long ostride, long col0, long ostride_i)
+{
+    switch (n) {
+    case 4096: return launch_fstage_direct<2048>(s, tw, words, stride, ninp, t_first, nt, pm, mult, sc, su, si, out_type, out, ostride, col0, ostride_i);
+    case 8192: return launch_fstage_direct<4096>(s, tw, words, stride, ninp, t_first, nt, pm, mult, sc, su, si, out_type, out, ostride, col0, ostride_i);
+    case 16384: return launch_fstage_direct<8192>(s, tw, words, stride, ninp, t_first, nt, pm, mult, sc, su, si, out_type, out, ostride, col0, ostride_i);
+    default: return FX_ERR_ARG;
+    }
+}
+
+bool fxk_fstage_direct_ok(int n, int ninp, long nt, int out_type, long ostride, long col0, long ostride_i)
+{
+    if (n != 4096 && n != 8192 && n != 16384) return false;
+    const long F = 32768 / n;
+    if (out_type == FX_PREC_FP8) return nt % F == 0 && ostride % F == 0 && col0 % F == 0 && ostride_i % F == 0;
+    if (out_type != FX_PREC_F32 && out_type != FX_PREC_F16) return false;
+    return ninp % F == 0 && ostride % F == 0 && col0 % F == 0;
 }

@@ -177,8 +177,11 @@ __device__ __forceinline__ void tw16_powers(const float2* twl, int e, float2* p)
 // (k = j mod P), does the R-point DFT and stores y[(j-k) R + k + r P].
 // Strides that are multiples of 16 (and the 16 consecutive stores of the
 // first pass) are constant LDS offsets from one padded base.
-template <int M, int R, int P, int NT, int F>
-__device__ __forceinline__ void stockham_pass(float2* sh, const float2* __restrict__ tw, int tid)
+// With T2, the P = 16 radix-16 pass reads its twiddles W_256^{r k} from the
+// table tw2[r*16 + k] instead of deriving them.
+template <int M, int R, int P, int NT, int F, bool T2 = false>
+__device__ __forceinline__ void stockham_pass(float2* sh, const float2* __restrict__ tw, int tid,
+                                              const float2* __restrict__ tw2 = nullptr)
 {
     constexpr int NB = F * M / R;
     constexpr int BPT = NB / NT;
@@ -209,7 +212,10 @@ __device__ __forceinline__ void stockham_pass(float2* sh, const float2* __restri
         int j = g - f * S;
         int k = j & (P - 1);
         if constexpr (P > 1) {
-            if constexpr (R == 16) {
+            if constexpr (R == 16 && P == 16 && T2) {
+#pragma unroll
+                for (int r = 1; r < R; r++) u[b][r] = cmul(u[b][r], tw2[r * 16 + k]);
+            } else if constexpr (R == 16) {
                 float2 p[16];
                 tw16_powers(tw, k * (FX_TW_N / (P * R)), p);
 #pragma unroll
@@ -233,14 +239,15 @@ __device__ __forceinline__ void stockham_pass(float2* sh, const float2* __restri
     __syncthreads();
 }
 
-template <int M, int P, int NT, int F>
-__device__ __forceinline__ void fft_passes(float2* sh, const float2* __restrict__ tw, int tid)
+template <int M, int P, int NT, int F, bool T2 = false>
+__device__ __forceinline__ void fft_passes(float2* sh, const float2* __restrict__ tw, int tid,
+                                           const float2* __restrict__ tw2 = nullptr)
 {
     if constexpr (P < M) {
         constexpr int REM = M / P;
         constexpr int R = REM >= 16 ? 16 : REM;
-        stockham_pass<M, R, P, NT, F>(sh, tw, tid);
-        fft_passes<M, P * R, NT, F>(sh, tw, tid);
+        stockham_pass<M, R, P, NT, F, T2>(sh, tw, tid, tw2);
+        fft_passes<M, P * R, NT, F, T2>(sh, tw, tid, tw2);
     }
 }
 
@@ -259,6 +266,19 @@ __device__ __forceinline__ void rsplit_tw(const float2* sh, int base, int k, flo
     float2 wo = cmul(w, o);
     xk = cadd(e, wo);
     xmk = make_float2(e.x - wo.x, -(e.y - wo.y));
+}
+
+// the same split without its factor 1/2: returns 2 X_k and 2 X_{M-k}
+template <int M>
+__device__ __forceinline__ void rsplit2_tw(const float2* sh, int base, int k, float2 w, float2& xk, float2& xmk)
+{
+    float2 a = sh[lpad(base + k)];
+    float2 b = sh[lpad(base + ((M - k) & (M - 1)))];
+    float2 e = make_float2(a.x + b.x, a.y - b.y);
+    float2 o = make_float2(a.y + b.y, b.x - a.x);
+    float2 wo = cmul(w, o);
+    xk = cadd(e, wo);
+    xmk = make_float2(e.x - wo.x, wo.y - e.y);
 }
 
 template <int M>
@@ -287,10 +307,21 @@ int fxk_fft_r2c(hipStream_t s, const float2* tw, const float* in, float2* out, i
 int fxk_spectrometer(hipStream_t s, const float2* tw, const float* window, const uint32_t* words,
                      long hop, long spec_stride, int nchan, int n, int integrate_ffts, int nspec, float* partial,
                      int groups, int nyq, float* out);
+// scale_chan: per-channel scales, or nullptr when every channel is scaled by
+// scale_uni; scale_inp: per-input scales or nullptr
 int fxk_fstage(hipStream_t s, const float2* tw, const uint32_t* words, size_t in_stride_words,
                int ninp, int n, long t_first, long nt, const fx_phase_model* pm,
-               const float2* mult, const float* scale_chan, const float* scale_inp, int out_type,
-               void* nat, void* out);
+               const float2* mult, const float* scale_chan, float scale_uni, const float* scale_inp,
+               int out_type, void* nat, void* out);
+// F kernel writing the engine layout itself (out[c][col0 + g] with rows of
+// ostride columns; FP8 out[c][i*ostride_i + col0 + t']) in one pass, for real
+// lengths n = 4096, 8192, 16384 with F = 32768/n items per workgroup;
+// fxk_fstage_direct_ok says whether a launch of these shapes is supported
+int fxk_fstage_direct(hipStream_t s, const float2* tw, const uint32_t* words, size_t in_stride_words,
+                      int ninp, int n, long t_first, long nt, const fx_phase_model* pm,
+                      const float2* mult, const float* scale_chan, float scale_uni, const float* scale_inp,
+                      int out_type, void* out, long ostride, long col0, long ostride_i);
+bool fxk_fstage_direct_ok(int n, int ninp, long nt, int out_type, long ostride, long col0, long ostride_i);
 // corner turn of one sub-chunk of F-stage rows nat[g][c] (nitems items of ns
 // spectra) into the chunk layout: FP32/FP16 out[c][col0 + g] with rows of
 // ostride items, FP8 out[c][i*ostride_i + col0 + t'] (fxk_fstage with
