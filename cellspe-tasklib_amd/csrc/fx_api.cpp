@@ -694,7 +694,8 @@ struct fx_corr {
     void* chan = nullptr;   // [c][t][i], chunk_spectra deep
     void* nat = nullptr;    // F-stage rows of one sub-chunk before the corner turn
     long sub = 0;           // spectra per F-stage sub-chunk (nat depth)
-    bool direct = false;    // F kernel writes the engine layout itself (no corner turn)
+    size_t nat_bytes = 0;   // bytes of the row buffer
+    bool direct = true;     // F kernel writes the engine layout itself (no corner turn)
     float* acc = nullptr;   // engine-layout accumulators
     size_t acc_floats = 0;
     fx_phase_model* pm = nullptr;
@@ -760,8 +761,9 @@ extern "C" int fx_corr_create(fx_ctx* ctx, const fx_corr_config* cfg, fx_corr** 
     const size_t sub_target = (size_t)(sub_env ? std::max(1, atoi(sub_env)) : 64) << 20;
     c->sub = std::max<long>(8, std::min<long>(cfg->chunk_spectra, (long)(sub_target / row_bytes) / 8 * 8));
     const char* direct_env = getenv("FX_FSTAGE_DIRECT");
-    c->direct = direct_env && atoi(direct_env) != 0;
-    const size_t nat_bytes = twobit ? 0 : row_bytes * (size_t)c->sub;
+    c->direct = !direct_env || atoi(direct_env) != 0;
+    c->nat_bytes = twobit ? 0 : row_bytes * (size_t)c->sub;
+    const size_t nat_bytes = c->nat_bytes;
     c->xtile = 128;
     c->acc_floats = fxk_xengine_acc_floats(c->nchan, cfg->ninp, c->nsets, c->xtile);
     long P = (long)cfg->ninp * (cfg->ninp + 1) / 2;
@@ -869,6 +871,31 @@ extern "C" int fx_corr_set_scales(fx_corr* c, const float* chan_scale, const flo
     return FX_OK;
 }
 
+// F kernel of nspectra (<= sub) spectra into the row buffer nat_buf, on stream s
+static int fstage_rows_into(fx_corr* c, const uint32_t* d_words, size_t in_stride_words, long t_first,
+                            long nspectra, int out_prec, void* nat_buf, hipStream_t s)
+{
+    const bool scaled = out_prec != FX_PREC_F32;
+    FX_CHECK_LAUNCH(fxk_fstage(s, c->ctx->tw, d_words, in_stride_words, c->cfg.ninp, c->cfg.nfft, t_first,
+                               nspectra, c->cfg.use_phase ? c->pm : nullptr, c->mult,
+                               scaled && !c->sc_uniform ? c->sc_chan : nullptr,
+                               scaled && c->sc_uniform ? c->sc_uni : 1.0f, scaled ? c->sc_inp : nullptr, out_prec,
+                               nat_buf, nullptr));
+    return FX_OK;
+}
+
+// corner turn of the rows in nat_buf (spectra s0 .. s0+nspectra of a
+// chunk_spectra-deep engine buffer d_chan), on stream s
+static int cornerturn_from(fx_corr* c, long nspectra, long s0, long chunk_spectra, int out_prec, void* d_chan,
+                           const void* nat_buf, hipStream_t s)
+{
+    const long I = c->cfg.ninp;
+    const bool imajor = out_prec == FX_PREC_FP8;
+    FX_CHECK_LAUNCH(fxk_cornerturn(s, out_prec, nat_buf, d_chan, nspectra * I, c->nchan, chunk_spectra * I,
+                                   imajor ? s0 : s0 * I, (int)nspectra, chunk_spectra));
+    return FX_OK;
+}
+
 extern "C" int fx_corr_fstage(fx_corr* c, const uint32_t* d_words, size_t in_stride_words,
                               long t_first, long nspectra, void* d_chan, int out_prec)
 {
@@ -901,9 +928,8 @@ extern "C" int fx_corr_fstage(fx_corr* c, const uint32_t* d_words, size_t in_str
     // sub-chunks: F kernel into the natural-row buffer, corner turn into d_chan
     for (long s0 = 0; s0 < nspectra; s0 += c->sub) {
         const long ns = std::min(c->sub, nspectra - s0);
-        rc = fx_corr_fstage_rows(c, d_words, in_stride_words, t_first + s0, ns, out_prec);
-        if (rc) return rc;
-        rc = fx_corr_cornerturn(c, ns, s0, nspectra, out_prec, d_chan);
+        rc = fstage_rows_into(c, d_words, in_stride_words, t_first + s0, ns, out_prec, c->nat, c->ctx->stream);
+        if (!rc) rc = cornerturn_from(c, ns, s0, nspectra, out_prec, d_chan, c->nat, c->ctx->stream);
         if (rc) return rc;
     }
     return FX_OK;
@@ -924,13 +950,7 @@ extern "C" int fx_corr_fstage_rows(fx_corr* c, const uint32_t* d_words, size_t i
                    nspectra <= c->sub,
                FX_ERR_ARG, "fx_corr_fstage_rows: bad range or alignment (at most %ld spectra)", c->sub);
     if (nspectra == 0) return FX_OK;
-    const bool scaled = out_prec != FX_PREC_F32;
-    FX_CHECK_LAUNCH(fxk_fstage(c->ctx->stream, c->ctx->tw, d_words, in_stride_words, c->cfg.ninp,
-                               c->cfg.nfft, t_first, nspectra, c->cfg.use_phase ? c->pm : nullptr,
-                               c->mult, scaled && !c->sc_uniform ? c->sc_chan : nullptr,
-                               scaled && c->sc_uniform ? c->sc_uni : 1.0f, scaled ? c->sc_inp : nullptr,
-                               out_prec, c->nat, nullptr));
-    return FX_OK;
+    return fstage_rows_into(c, d_words, in_stride_words, t_first, nspectra, out_prec, c->nat, c->ctx->stream);
 }
 
 extern "C" int fx_corr_cornerturn(fx_corr* c, long nspectra, long s0, long chunk_spectra, int out_prec,
@@ -943,11 +963,7 @@ extern "C" int fx_corr_cornerturn(fx_corr* c, long nspectra, long s0, long chunk
                    s0 + nspectra <= chunk_spectra && chunk_spectra <= c->cfg.chunk_spectra,
                FX_ERR_ARG, "fx_corr_cornerturn: bad range");
     if (nspectra == 0) return FX_OK;
-    const long I = c->cfg.ninp;
-    const bool imajor = out_prec == FX_PREC_FP8;
-    FX_CHECK_LAUNCH(fxk_cornerturn(c->ctx->stream, out_prec, c->nat, d_chan, nspectra * I, c->nchan,
-                                   chunk_spectra * I, imajor ? s0 : s0 * I, (int)nspectra, chunk_spectra));
-    return FX_OK;
+    return cornerturn_from(c, nspectra, s0, chunk_spectra, out_prec, d_chan, c->nat, c->ctx->stream);
 }
 
 extern "C" int fx_corr_process(fx_corr* c, const uint32_t* d_words, size_t in_stride_words,
@@ -1036,9 +1052,9 @@ extern "C" int fx_corr_tune(fx_corr* c, long sub_spectra, int xtile)
     }
     if (sub_spectra && c->nat) {
         const long sub = std::min(sub_spectra, c->cfg.chunk_spectra);
-        if (sub > c->sub) {
+        const size_t bytes = (size_t)c->nchan * (size_t)c->cfg.ninp * c->elem * (size_t)sub;
+        if (bytes > c->nat_bytes) {
             void* nat = nullptr;
-            const size_t bytes = (size_t)c->nchan * (size_t)c->cfg.ninp * c->elem * (size_t)sub;
             if (hipMalloc(&nat, bytes) != hipSuccess) {
                 hipGetLastError();
                 fx_set_error("fx_corr_tune: row buffer allocation failed");
@@ -1046,6 +1062,7 @@ extern "C" int fx_corr_tune(fx_corr* c, long sub_spectra, int xtile)
             }
             hipFree(c->nat);
             c->nat = nat;
+            c->nat_bytes = bytes;
         }
         c->sub = sub;
     }

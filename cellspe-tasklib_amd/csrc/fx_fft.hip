@@ -270,18 +270,52 @@ struct FItems {
     int t[F], i[F];   // (spectrum, input) of item f; t = -1 past the end
 };
 
+// Tables of the F kernels in LDS: the 16-entry unpack table, the second-pass
+// twiddles tw2[r*16 + k] = W_256^{r k}, for M = 4096 the third-pass tables
+// t3 = {W_4096^k, W_4096^{4k}, W_4096^{8k}} (k < 256), and the two-level
+// table twl only where another pass needs it (the LDS of two workgroups per
+// CU leaves room for t3 only without twl).
+template <int M>
+struct FTabs {
+    static constexpr bool T3 = M == 4096;
+    static constexpr bool TWL = !(M == 256 || M == 4096);
+    float2 lut[16];
+    float2 tw2[M >= 256 ? 256 : 1];
+    float2 t3[T3 ? 768 : 1];
+    float2 twl[TWL ? FX_TWL : 1];
+};
+
+// Fill the tables (visible after the first barrier of fstage_group).
+template <int M, int NT>
+__device__ __forceinline__ void fstage_consts(const float2* __restrict__ twg, FTabs<M>& tb, int tid)
+{
+    if (tid < 16) tb.lut[tid] = make_float2(lut2((uint32_t)tid & 3u), lut2((uint32_t)tid >> 2));
+    if constexpr (M >= 256) {
+        for (int e = tid; e < 256; e += NT) tb.tw2[e] = twg[(e >> 4) * (e & 15) * (FX_TW_N / 256)];
+    }
+    if constexpr (FTabs<M>::T3) {
+        // W_4096^{m k} = T[m k FX_TW_N / 4096], m = 1, 4, 8
+        for (int e = tid; e < 768; e += NT) {
+            const int blk = e >> 8, k = e & 255;
+            tb.t3[e] = twg[(k << (blk == 0 ? 3 : blk == 1 ? 5 : 6)) & (FX_TW_N - 1)];
+        }
+    }
+    if constexpr (FTabs<M>::TWL) twl_load(tb.twl, twg, tid, NT);
+}
+
 // Front of the F kernels for the F items g0 .. g0+F-1: item bookkeeping, the
 // FP64 phasor tables, the LDS FFT.  Item order: g = t*I + i, or g = i*nt + t
 // when IM.  The packed words of the items are staged at the head of sh and
 // the first radix-16 pass reads its 16 points per butterfly straight from
 // them: complex point e of an item is the 4-bit field e%8 of word e/8, one
 // lookup in the 16-entry table lut (both samples of the point).
-template <int M, int F, int NT, bool IM>
-__device__ __forceinline__ void fstage_front(const float2* __restrict__ twg, const uint32_t* __restrict__ words,
-                                             size_t in_stride, int ninp, long t_first, long nt,
-                                             const fx_phase_model* __restrict__ pm, float sc_uni,
-                                             const float* __restrict__ sc_inp, long g0, float2* sh, float2* tw,
-                                             float2* tw2, float2* lut, FItems<M, F>& it, int tid)
+// hook(std::integral_constant<int, p>) runs after FFT pass p
+// (p < fft_npasses<M, 1>()), between the passes' barriers.
+template <int M, int F, int NT, bool IM, typename Hook>
+__device__ __forceinline__ void fstage_group(const uint32_t* __restrict__ words, size_t in_stride, int ninp,
+                                             long t_first, long nt, const fx_phase_model* __restrict__ pm,
+                                             float sc_uni, const float* __restrict__ sc_inp, long g0, float2* sh,
+                                             const FTabs<M>& tb, FItems<M, F>& it, int tid, Hook&& hook)
 {
     constexpr int N = 2 * M;
     constexpr int WPT = N / 16;             // packed words per transform
@@ -289,12 +323,6 @@ __device__ __forceinline__ void fstage_front(const float2* __restrict__ twg, con
     constexpr int PLO = PhaseTab<M>::PLO;
     constexpr int PHI = PhaseTab<M>::PHI;
     static_assert(WTH % 2 == 0 && M >= 128, "F kernel shape");
-    twl_load(tw, twg, tid, NT);
-    if (tid < 16) lut[tid] = make_float2(lut2((uint32_t)tid & 3u), lut2((uint32_t)tid >> 2));
-    if constexpr (M >= 256) {
-        // second-pass twiddles W_256^{r k}, r-major
-        for (int e = tid; e < 256; e += NT) tw2[e] = twg[(e >> 4) * (e & 15) * (FX_TW_N / 256)];
-    }
     const long nitems = nt * ninp;
     if (tid < F) {
         const long g = g0 + tid;
@@ -357,7 +385,7 @@ __device__ __forceinline__ void fstage_front(const float2* __restrict__ twg, con
             const uint32_t* rw = raw + f * WPT + (j >> 3);
             const int s4 = 4 * (j & 7);
 #pragma unroll
-            for (int r = 0; r < 16; r++) u[b][r] = lut[(rw[r * (S1 >> 3)] >> s4) & 15u];
+            for (int r = 0; r < 16; r++) u[b][r] = tb.lut[(rw[r * (S1 >> 3)] >> s4) & 15u];
         }
         __syncthreads();
 #pragma unroll
@@ -372,7 +400,20 @@ __device__ __forceinline__ void fstage_front(const float2* __restrict__ twg, con
         }
         __syncthreads();
     }
-    fft_passes<M, 16, NT, F, true>(sh, tw, tid, tw2);
+    hook(std::integral_constant<int, 0>{});
+    fft_passes_hook<M, 16, NT, F, true, FTabs<M>::T3, 1>(sh, tb.twl, tid, tb.tw2, tb.t3, hook);
+}
+
+template <int M, int F, int NT, bool IM>
+__device__ __forceinline__ void fstage_front(const float2* __restrict__ twg, const uint32_t* __restrict__ words,
+                                             size_t in_stride, int ninp, long t_first, long nt,
+                                             const fx_phase_model* __restrict__ pm, float sc_uni,
+                                             const float* __restrict__ sc_inp, long g0, float2* sh, FTabs<M>& tb,
+                                             FItems<M, F>& it, int tid)
+{
+    fstage_consts<M, NT>(twg, tb, tid);
+    fstage_group<M, F, NT, IM>(words, in_stride, ninp, t_first, nt, pm, sc_uni, sc_inp, g0, sh, tb, it, tid,
+                               [](auto) {});
 }
 
 // Channel k of item f from twice its split value: rotate by the phasor (which
@@ -420,14 +461,11 @@ __global__ __launch_bounds__(FftShape<M>::NT) void k_fstage(
     constexpr bool IM = std::is_same<T, uint16_t>::value;
     constexpr int PLO = PhaseTab<M>::PLO;
     __shared__ float2 sh[S::LDS];
-    __shared__ float2 tw[FX_TWL];
-    __shared__ float2 tw2[256];
-    __shared__ float2 lut[16];
+    __shared__ FTabs<M> tb;
     __shared__ FItems<M, F> it;
     const int tid = threadIdx.x;
     const long g0 = (long)blockIdx.x * F;
-    fstage_front<M, F, NT, IM>(twg, words, in_stride, ninp, t_first, nt, pm, sc_uni, sc_inp, g0, sh, tw, tw2,
-                               lut, it, tid);
+    fstage_front<M, F, NT, IM>(twg, words, in_stride, ninp, t_first, nt, pm, sc_uni, sc_inp, g0, sh, tb, it, tid);
     const bool has_pm = PMO || pm != nullptr;
 
     // ---- split, rotate, scale, convert; lanes write consecutive channels of
@@ -438,8 +476,8 @@ __global__ __launch_bounds__(FftShape<M>::NT) void k_fstage(
         // thread tid owns channels kp = q NT + tid of every item: the split
         // twiddle W_N^kp advances by W_N^NT per q (shared by the items), and
         // the low phasor factors of kp and M - kp are the same for every q
-        const float2 wstep = twl_get(tw, NT * (FX_TW_N / N));
-        const float2 w0 = twl_get(tw, tid * (FX_TW_N / N));
+        const float2 wstep = twg[NT * (FX_TW_N / N)];
+        const float2 w0 = twg[tid * (FX_TW_N / N)];
         const int lo_k = tid % PLO, lo_m = (PLO - lo_k) % PLO;
         // blocks whose items all exist (all but the last) skip the per-item test
         auto body = [&](auto checked) {
@@ -478,7 +516,7 @@ __global__ __launch_bounds__(FftShape<M>::NT) void k_fstage(
             const int kp = H >= NT ? (q * NT) % H + tid : (q * NT + tid) % H;
             if (it.t[f] < 0) continue;
             float2 xk, xm;
-            rsplit2_tw<M>(sh, f * M, kp, twl_get(tw, kp * (FX_TW_N / N)), xk, xm);
+            rsplit2_tw<M>(sh, f * M, kp, twg[kp * (FX_TW_N / N)], xk, xm);
             const int k2 = M - kp;
             const int k2m = k2 & (M - 1);
             V* row = o + (size_t)(g0 + f) * M;
@@ -493,7 +531,7 @@ __global__ __launch_bounds__(FftShape<M>::NT) void k_fstage(
     // kp = M/2 of each transform
     if (tid < F && it.t[tid] >= 0) {
         float2 xk, xm;
-        rsplit2_tw<M>(sh, tid * M, M / 2, twl_get(tw, (M / 2) * (FX_TW_N / N)), xk, xm);
+        rsplit2_tw<M>(sh, tid * M, M / 2, twg[(M / 2) * (FX_TW_N / N)], xk, xm);
         o[(size_t)(g0 + tid) * M + M / 2] = fstage_finish<M, T, F, PMO>(
             xk, it, tid, M / 2, has_pm ? it.ph_lo[tid][(M / 2) % PLO] : make_float2(1.f, 0.f), has_pm, mult, sc_chan);
     }
@@ -541,21 +579,18 @@ __global__ __launch_bounds__(1024) void k_fstage_direct(
     constexpr bool IM = std::is_same<T, uint16_t>::value;
     constexpr int PLO = PhaseTab<M>::PLO;
     __shared__ float2 sh[lds_complex(F * M)];
-    __shared__ float2 tw[FX_TWL];
-    __shared__ float2 tw2[256];
-    __shared__ float2 lut[16];
+    __shared__ FTabs<M> tb;
     __shared__ FItems<M, F> it;
     const int tid = threadIdx.x;
     const long nblk = nt * ninp / F;
     const long lb = xcd_block(nblk);
     if (lb >= nblk) return;
     const long g0 = lb * F;
-    fstage_front<M, F, NT, IM>(twg, words, in_stride, ninp, t_first, nt, pm, sc_uni, sc_inp, g0, sh, tw, tw2,
-                               lut, it, tid);
+    fstage_front<M, F, NT, IM>(twg, words, in_stride, ninp, t_first, nt, pm, sc_uni, sc_inp, g0, sh, tb, it, tid);
     const bool has_pm = PMO || pm != nullptr;
     const long col = IM ? (g0 / nt) * ostride_i + col0 + (g0 % nt) : col0 + g0;
-    const float2 wstep = twl_get(tw, NT * (FX_TW_N / N));
-    float2 w = twl_get(tw, tid * (FX_TW_N / N));
+    const float2 wstep = twg[NT * (FX_TW_N / N)];
+    float2 w = twg[tid * (FX_TW_N / N)];
     const int lo_k = tid % PLO, lo_m = (PLO - lo_k) % PLO;
 #pragma unroll
     for (int q = 0; q < H / NT; q++) {
@@ -579,36 +614,38 @@ __global__ __launch_bounds__(1024) void k_fstage_direct(
     // kp = M/2 of each item
     if (tid < F) {
         float2 xk, xm;
-        rsplit2_tw<M>(sh, tid * M, M / 2, twl_get(tw, (M / 2) * (FX_TW_N / N)), xk, xm);
+        rsplit2_tw<M>(sh, tid * M, M / 2, twg[(M / 2) * (FX_TW_N / N)], xk, xm);
         out[(size_t)(M / 2) * ostride + col + tid] = fstage_finish<M, T, F, PMO>(
             xk, it, tid, M / 2, has_pm ? it.ph_lo[tid][(M / 2) % PLO] : make_float2(1.f, 0.f), has_pm, mult, sc_chan);
     }
 }
 
 // Corner turn of one sub-chunk of F-stage rows nat[g][c] (g < nitems) into
-// the chunk-wide engine layout out[c][col], through a 64 x 64 LDS tile: both
+// the chunk-wide engine layout out[c][col], through a TL x TL LDS tile: both
 // the 256-thread reads (rows of nat) and the writes (rows of out) are
-// contiguous runs of 64 elements.  Row c of out holds ostride columns.
+// contiguous runs of TL elements.  Row c of out holds ostride columns.
 // t-major items (g = t'*I + i) land at col = col0 + g; input-major items
 // (g = i*ns + t') at col = i*ostride_i + col0 + t' (ostride_i = spectra per
-// chunk).  XCD-aware: tiles of neighbouring items share an XCD.
-template <typename V>
+// chunk).  XCD-aware: tiles of neighbouring items share an XCD.  (Used when
+// the direct F kernel does not apply: other FFT lengths or input counts.)
+template <typename V, int TL>
 __global__ __launch_bounds__(256) void k_cornerturn(const V* __restrict__ nat, V* __restrict__ out,
                                                     long nitems, int C, long ostride, long col0,
                                                     int imajor, int ns, long ostride_i)
 {
-    __shared__ V tile[64][65];
-    const long ntg = (nitems + 63) / 64;
-    const long ntc = C / 64;
+    __shared__ V tile[TL][TL + 1];
+    constexpr int RS = 256 / TL;   // rows per thread step
+    const long ntg = (nitems + TL - 1) / TL;
+    const long ntc = C / TL;
     const long lb = xcd_block(ntg * ntc);
     if (lb >= ntg * ntc) return;
     const long tg = lb % ntg;
     const int tc = (int)(lb / ntg);
-    const int lx = threadIdx.x & 63, ly = threadIdx.x >> 6;
-    const long gbase = tg * 64;
-    const int cbase = tc * 64;
+    const int lx = threadIdx.x % TL, ly = threadIdx.x / TL;
+    const long gbase = tg * TL;
+    const int cbase = tc * TL;
 #pragma unroll
-    for (int r = ly; r < 64; r += 4) {
+    for (int r = ly; r < TL; r += RS) {
         long g = gbase + r;
         if (g < nitems) tile[r][lx] = nat[(size_t)g * C + cbase + lx];
     }
@@ -616,7 +653,7 @@ __global__ __launch_bounds__(256) void k_cornerturn(const V* __restrict__ nat, V
     const long g = gbase + lx;
     const long col = imajor ? (g / ns) * ostride_i + col0 + (g % ns) : col0 + g;
 #pragma unroll
-    for (int r = ly; r < 64; r += 4) {
+    for (int r = ly; r < TL; r += RS) {
         if (g < nitems) out[(size_t)(cbase + r) * ostride + col] = tile[lx][r];
     }
 }
@@ -664,7 +701,7 @@ int launch_cornerturn_t(hipStream_t s, const void* nat, void* out, long nitems, 
                         int imajor, int ns, long ostride_i)
 {
     long ntiles = ((nitems + 63) / 64) * (M / 64);
-    hipLaunchKernelGGL((k_cornerturn<V>), dim3((unsigned)xcd_grid(ntiles)), dim3(256), 0, s,
+    hipLaunchKernelGGL((k_cornerturn<V, 64>), dim3((unsigned)xcd_grid(ntiles)), dim3(256), 0, s,
                        reinterpret_cast<const V*>(nat), reinterpret_cast<V*>(out), nitems, M, ostride, col0,
                        imajor, ns, ostride_i);
     return hipGetLastError() == hipSuccess ? FX_OK : FX_ERR_HIP;
@@ -719,12 +756,13 @@ int launch_fstage_direct_t(hipStream_t s, const float2* tw, const uint32_t* word
     if ((IM ? nt : (long)ninp) % F || ostride % F || col0 % F || (IM && ostride_i % F)) return FX_ERR_ARG;
     const long nblk = nt * ninp / F;
     if (nblk == 0) return FX_OK;
+    const long grid = xcd_grid(nblk);
     if (pm && !mult && !sc)
-        hipLaunchKernelGGL((k_fstage_direct<M, T, true>), dim3((unsigned)xcd_grid(nblk)), dim3(1024), 0, s, tw,
+        hipLaunchKernelGGL((k_fstage_direct<M, T, true>), dim3((unsigned)grid), dim3(1024), 0, s, tw,
                            words, stride, ninp, t_first, nt, pm, mult, sc, su, si,
                            reinterpret_cast<typename ChanT<T>::V*>(out), ostride, col0, ostride_i);
     else
-        hipLaunchKernelGGL((k_fstage_direct<M, T, false>), dim3((unsigned)xcd_grid(nblk)), dim3(1024), 0, s, tw,
+        hipLaunchKernelGGL((k_fstage_direct<M, T, false>), dim3((unsigned)grid), dim3(1024), 0, s, tw,
                            words, stride, ninp, t_first, nt, pm, mult, sc, su, si,
                            reinterpret_cast<typename ChanT<T>::V*>(out), ostride, col0, ostride_i);
     return hipGetLastError() == hipSuccess ? FX_OK : FX_ERR_HIP;

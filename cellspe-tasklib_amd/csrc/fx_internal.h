@@ -10,6 +10,7 @@
 #include <stdint.h>
 #include <stddef.h>
 #include <algorithm>
+#include <type_traits>
 
 #include "cellspe_fx.h"
 
@@ -150,11 +151,8 @@ __device__ __forceinline__ long xcd_block(long nlogical)
 // Twiddles w^1 .. w^15 of one radix-16 butterfly from three table lookups
 // (w, w^4, w^8) and twelve products, at most two products deep from a looked-up
 // value: p[r] = W^{r e}, W = exp(-2 pi i / FX_TW_N).
-__device__ __forceinline__ void tw16_powers(const float2* twl, int e, float2* p)
+__device__ __forceinline__ void tw16_from(float2 w1, float2 w4, float2 w8, float2* p)
 {
-    const float2 w1 = twl_get(twl, e & (FX_TW_N - 1));
-    const float2 w4 = twl_get(twl, (4 * e) & (FX_TW_N - 1));
-    const float2 w8 = twl_get(twl, (8 * e) & (FX_TW_N - 1));
     p[1] = w1;
     p[2] = cmul(w1, w1);
     p[3] = cmul(p[2], w1);
@@ -170,6 +168,11 @@ __device__ __forceinline__ void tw16_powers(const float2* twl, int e, float2* p)
         p[12 + r] = cmul(p[12], p[r]);
     }
 }
+__device__ __forceinline__ void tw16_powers(const float2* twl, int e, float2* p)
+{
+    tw16_from(twl_get(twl, e & (FX_TW_N - 1)), twl_get(twl, (4 * e) & (FX_TW_N - 1)),
+              twl_get(twl, (8 * e) & (FX_TW_N - 1)), p);
+}
 
 // One radix-R Stockham pass over F complex FFTs of length M held in LDS
 // (padded indexing), NT threads, current sub-transform span P.
@@ -178,10 +181,13 @@ __device__ __forceinline__ void tw16_powers(const float2* twl, int e, float2* p)
 // Strides that are multiples of 16 (and the 16 consecutive stores of the
 // first pass) are constant LDS offsets from one padded base.
 // With T2, the P = 16 radix-16 pass reads its twiddles W_256^{r k} from the
-// table tw2[r*16 + k] instead of deriving them.
-template <int M, int R, int P, int NT, int F, bool T2 = false>
+// table tw2[r*16 + k] instead of deriving them; with T3 the P = 256 radix-16
+// pass reads w = W_4096^k, w^4 and w^8 from t3[k], t3[256 + k], t3[512 + k]
+// (lanes read consecutive entries: no LDS bank conflicts).
+template <int M, int R, int P, int NT, int F, bool T2 = false, bool T3 = false>
 __device__ __forceinline__ void stockham_pass(float2* sh, const float2* __restrict__ tw, int tid,
-                                              const float2* __restrict__ tw2 = nullptr)
+                                              const float2* __restrict__ tw2 = nullptr,
+                                              const float2* __restrict__ t3 = nullptr)
 {
     constexpr int NB = F * M / R;
     constexpr int BPT = NB / NT;
@@ -215,6 +221,11 @@ __device__ __forceinline__ void stockham_pass(float2* sh, const float2* __restri
             if constexpr (R == 16 && P == 16 && T2) {
 #pragma unroll
                 for (int r = 1; r < R; r++) u[b][r] = cmul(u[b][r], tw2[r * 16 + k]);
+            } else if constexpr (R == 16 && P == 256 && T3) {
+                float2 p[16];
+                tw16_from(t3[k], t3[256 + k], t3[512 + k], p);
+#pragma unroll
+                for (int r = 1; r < R; r++) u[b][r] = cmul(u[b][r], p[r]);
             } else if constexpr (R == 16) {
                 float2 p[16];
                 tw16_powers(tw, k * (FX_TW_N / (P * R)), p);
@@ -248,6 +259,33 @@ __device__ __forceinline__ void fft_passes(float2* sh, const float2* __restrict_
         constexpr int R = REM >= 16 ? 16 : REM;
         stockham_pass<M, R, P, NT, F, T2>(sh, tw, tid, tw2);
         fft_passes<M, P * R, NT, F, T2>(sh, tw, tid, tw2);
+    }
+}
+
+// Number of Stockham passes from span P to M.
+template <int M, int P>
+__host__ __device__ constexpr int fft_npasses()
+{
+    if constexpr (P < M)
+        return 1 + fft_npasses<M, P * (M / P >= 16 ? 16 : M / P)>();
+    else
+        return 0;
+}
+
+// fft_passes with hook(std::integral_constant<int, p>) called after pass p
+// (counted from 0 at span P0), e.g. to issue deferred global stores between
+// passes.
+template <int M, int P, int NT, int F, bool T2, bool T3, int PI = 0, typename Hook>
+__device__ __forceinline__ void fft_passes_hook(float2* sh, const float2* __restrict__ tw, int tid,
+                                                const float2* __restrict__ tw2, const float2* __restrict__ t3,
+                                                Hook&& hook)
+{
+    if constexpr (P < M) {
+        constexpr int REM = M / P;
+        constexpr int R = REM >= 16 ? 16 : REM;
+        stockham_pass<M, R, P, NT, F, T2, T3>(sh, tw, tid, tw2, t3);
+        hook(std::integral_constant<int, PI>{});
+        fft_passes_hook<M, P * R, NT, F, T2, T3, PI + 1>(sh, tw, tid, tw2, t3, hook);
     }
 }
 
