@@ -8,10 +8,11 @@ delay/fringe phase model applied after the FFT, 65536 spectra (2^29 samples
 per input, 16 GiB of packed input resident in HBM), all 8256 products.
 
 One step = reset accumulators -> per 4096-spectrum chunk: F kernel (unpack,
-FFT, rotate, scale, convert) -> corner turn -> X-engine -> dump [p][c]
-complex64 visibilities.  HIP events around every launch give the per-kernel
-durations inside the timed region.  The headline X-engine is FP16 MFMA; the
-plain FP32 path and the scaled FP8 path run the same step beside it.
+FFT, rotate, scale, convert, store in the engine layout) -> X-engine -> dump
+[p][c] complex64 visibilities.  HIP events around the launches of the first
+chunk give the per-kernel durations inside the timed region.  The headline
+X-engine is FP16 MFMA; the plain FP32 path and the scaled FP8 path run the
+same step beside it.
 
 A second section times the large-N X-engine (256 stations x 2 pol = 512
 inputs, 1024 channels, 8192 spectra of 2-bit channelized voltages): the exact
@@ -327,33 +328,25 @@ def main():
     dvis = ctx.alloc(P * C * 8)
 
     def run_path(prec, esz, steps, warmup):
-        """Whole steps with an event around every launch; returns ms/step and
-        the mean per-chunk durations of the three kernels."""
+        """Whole steps with events around the F stage and the X-engine of chunk
+        0; returns ms/step and their durations (the F stage of a chunk is one
+        launch of the direct engine-layout transform kernel where it applies)."""
         corr = fxlib.Correlator(ctx, I, N, chunk, prec, use_phase=True)
         corr.set_phase_model(models)
         dchan = ctx.alloc(C * chunk * I * esz)
-        sub = corr.fstage_depth()
-        nsub = (chunk + sub - 1) // sub
-        # events around every launch of chunk 0 (its F sub-chunks, their corner
-        # turns and the X-engine); the other chunks run the same launches bare
-        evf = [[ctx.event() for _ in range(3)] for _ in range(nsub)]
+        nf = corr.fstage_launches(chunk, prec)
+        evf = [ctx.event(), ctx.event()]
         evx = [ctx.event(), ctx.event()]
 
         def step():
             corr.reset()
             for k in range(nchunks):
                 nt = min(chunk, T - k * chunk)
-                for q in range(0, nt, sub):
-                    ns = min(sub, nt - q)
-                    if k == 0:
-                        ctx.record(evf[q // sub][0])
-                    corr.fstage_rows(dw, nwords, k * chunk + q, ns, prec)
-                    if k == 0:
-                        ctx.record(evf[q // sub][1])
-                    corr.cornerturn(ns, q, nt, prec, dchan)
-                    if k == 0:
-                        ctx.record(evf[q // sub][2])
                 if k == 0:
+                    ctx.record(evf[0])
+                corr.fstage(dw, nwords, k * chunk, nt, dchan, prec)
+                if k == 0:
+                    ctx.record(evf[1])
                     ctx.record(evx[0])
                 corr.xengine(dchan, nt)
                 if k == 0:
@@ -371,21 +364,18 @@ def main():
         ctx.sync()
         ms = ctx.elapsed_ms(e0, e1) / steps
         # per-launch durations in chunk 0 of the last timed step (events are
-        # re-recorded each step); F kernel and corner turn summed over the
-        # chunk's sub-chunks, with the per-launch mean beside them
-        f = sum(ctx.elapsed_ms(e[0], e[1]) for e in evf)
-        t = sum(ctx.elapsed_ms(e[1], e[2]) for e in evf)
+        # re-recorded each step)
+        f = ctx.elapsed_ms(evf[0], evf[1])
         x = ctx.elapsed_ms(evx[0], evx[1])
         dchan.free()
         corr.close()
-        return ms, {"fstage": f, "cornerturn": t, "xengine": x, "fstage_launch": f / nsub,
-                    "cornerturn_launch": t / nsub, "subchunks_per_chunk": nsub}
+        return ms, {"fstage": f, "xengine": x, "fstage_launches": nf}
 
     paths = {}
     ms16, k16 = run_path(fxlib.FX_PREC_F16, 4, args.steps, args.warmup)
     paths["f16"] = (ms16, k16)
     log(f"[bench] fp16 path: {ms16:.2f} ms/step; per chunk " +
-        ", ".join(f"{k} {v}" if k == "subchunks_per_chunk" else f"{k} {v:.2f} ms" for k, v in k16.items()))
+        ", ".join(f"{k} {v}" if k == "fstage_launches" else f"{k} {v:.2f} ms" for k, v in k16.items()))
     if not args.no_fp32:
         paths["f32"] = run_path(fxlib.FX_PREC_F32, 8, max(1, args.steps - 1), 1)
         log(f"[bench] fp32 path: {paths['f32'][0]:.2f} ms/step")
@@ -452,30 +442,25 @@ def main():
         batch = batched(fxlib, ctx, args)
 
     # ---- roofline of the dominant kernel of the headline step ------------------
-    # per launch: the X-engine runs once per chunk, the F kernel and the corner
-    # turn once per sub-chunk of `sub` spectra
-    sub = chunk // k16["subchunks_per_chunk"]
+    # per launch: the X-engine runs once per chunk, and so does the direct F
+    # kernel (transform + engine-layout store of a whole chunk)
     x_flop = 8.0 * P * C * chunk
-    f_bytes = I * sub * N / 4.0 + I * sub * C * 4.0        # packed in + FP16 channels out
-    ct_bytes = 2.0 * I * sub * C * 4.0                      # read + write of the FP16 channels
-    dom = max(("fstage", "cornerturn", "xengine"), key=lambda k: k16[k])
+    f_bytes = I * chunk * N / 4.0 + I * chunk * C * 4.0        # packed in + FP16 channels out
+    dom = max(("fstage", "xengine"), key=lambda k: k16[k])
     if dom == "xengine":
         launch = k16["xengine"]
         roof = {"kernel": "k_xengine<1>", "bound": "mfma",
                 "achieved": x_flop / (launch * 1e-3) / 1e12, "peak": PEAK["fp16"], "unit": "TFLOP/s",
                 "algorithmic": f"8*P*C*chunk = {x_flop:.3e} flop per launch"}
-    elif dom == "fstage":
-        launch = k16["fstage_launch"]
-        roof = {"kernel": "k_fstage<4096, __half2>", "bound": "hbm",
-                "achieved": f_bytes / (launch * 1e-3) / 1e9, "peak": PEAK["hbm"], "unit": "GB/s",
-                "algorithmic": f"per input-spectrum N/4 B packed in + C*4 B FP16 out; {sub} spectra x {I} "
-                               f"inputs = {f_bytes:.3e} B per launch",
-                "valu_tflops": 2.5 * N * np.log2(N) * I * sub / (launch * 1e-3) / 1e12}
     else:
-        launch = k16["cornerturn_launch"]
-        roof = {"kernel": "k_cornerturn<unsigned int>", "bound": "hbm",
-                "achieved": ct_bytes / (launch * 1e-3) / 1e9, "peak": PEAK["hbm"], "unit": "GB/s",
-                "algorithmic": f"read + write of the FP16 channels, {ct_bytes:.3e} B per launch"}
+        if k16["fstage_launches"] != 1:
+            raise SystemExit("bench: the F stage of this shape is not the single direct launch")
+        launch = k16["fstage"]
+        roof = {"kernel": "k_fstage_direct<4096, __half2, true>", "bound": "hbm",
+                "achieved": f_bytes / (launch * 1e-3) / 1e9, "peak": PEAK["hbm"], "unit": "GB/s",
+                "algorithmic": f"per input-spectrum N/4 B packed in + C*4 B FP16 out; {chunk} spectra x {I} "
+                               f"inputs = {f_bytes:.3e} B per launch",
+                "valu_tflops": 2.5 * N * np.log2(N) * I * chunk / (launch * 1e-3) / 1e12}
     roof["frac"] = roof["achieved"] / roof["peak"]
     tr = measured_traffic(roof["kernel"])
     roof["traffic"] = None if tr is None else tr["bytes"]

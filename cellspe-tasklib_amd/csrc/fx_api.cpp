@@ -937,6 +937,15 @@ extern "C" int fx_corr_fstage(fx_corr* c, const uint32_t* d_words, size_t in_str
 
 extern "C" long fx_corr_fstage_depth(fx_corr* c) { return c ? c->sub : 0; }
 
+extern "C" int fx_corr_fstage_launches(fx_corr* c, long nspectra, int out_prec)
+{
+    if (!c || nspectra <= 0) return 0;
+    const long I = c->cfg.ninp;
+    if (c->direct && fxk_fstage_direct_ok(c->cfg.nfft, (int)I, nspectra, out_prec, nspectra * I, 0, nspectra))
+        return 1;
+    return (int)(2 * ((nspectra + c->sub - 1) / c->sub));
+}
+
 extern "C" int fx_corr_fstage_rows(fx_corr* c, const uint32_t* d_words, size_t in_stride_words,
                                    long t_first, long nspectra, int out_prec)
 {

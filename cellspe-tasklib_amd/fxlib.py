@@ -101,6 +101,7 @@ def load():
         "fx_corr_fstage_rows": (i, [vp, vp, sz, l, l, i]),
         "fx_corr_cornerturn": (i, [vp, l, l, l, i, vp]),
         "fx_corr_fstage_depth": (l, [vp]),
+        "fx_corr_fstage_launches": (i, [vp, l, i]),
         "fx_corr_tune": (i, [vp, l, i]),
         "fx_pcal_final_async": (i, [vp]),
         "fx_pcal_tones": (i, [vp, vp, C.POINTER(u64)]),
@@ -363,6 +364,11 @@ class Correlator:
 
     def fstage_depth(self):
         return load().fx_corr_fstage_depth(self.handle)
+
+    def fstage_launches(self, nspectra, out_prec):
+        """Kernel launches of fstage() for nspectra spectra (1: the direct
+        engine-layout transform kernel)."""
+        return load().fx_corr_fstage_launches(self.handle, nspectra, out_prec)
 
     def xengine(self, d_chan, nspectra):
         check(load().fx_corr_xengine(self.handle, d_chan.ptr, nspectra))

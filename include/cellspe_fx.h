@@ -214,13 +214,22 @@ int fx_corr_reset(fx_corr* corr);
  * (FP32/F16 [c][t][i], FP8 [c][i][t]) */
 int fx_corr_fstage(fx_corr* corr, const uint32_t* d_words, size_t in_stride_words,
                    long t_first, long nspectra, void* d_chan, int out_prec);
-/* fx_corr_fstage works in sub-chunks of fx_corr_fstage_depth() spectra (about
- * 64 MB of rows, so the corner turn reads them back from the Infinity Cache):
- * per sub-chunk, the transform kernel writes natural rows into the
- * correlator's row buffer and the corner turn moves them into d_chan.  The two
- * launches are exposed one at a time for per-kernel timing: rows of spectra
- * [t_first, t_first + nspectra), nspectra <= depth, then their corner turn to
- * spectra [s0, s0 + nspectra) of a chunk of chunk_spectra spectra in d_chan. */
+/* For real FFT lengths 4096, 8192 and 16384 with an input count (FP32/FP16)
+ * or spectrum count (FP8) that is a multiple of 32768/n, fx_corr_fstage is one
+ * launch of a transform kernel that writes the engine layout itself (a
+ * workgroup transforms 32768/n neighbouring columns and stores one channel of
+ * all of them per lane; FX_FSTAGE_DIRECT=0 in the environment at
+ * fx_corr_create turns this off).  Otherwise it works in sub-chunks of
+ * fx_corr_fstage_depth() spectra (about 64 MB of rows, so the corner turn
+ * reads them back from the Infinity Cache): per sub-chunk, the transform
+ * kernel writes natural rows into the correlator's row buffer and the corner
+ * turn moves them into d_chan.  fx_corr_fstage_launches says which: the number
+ * of kernel launches fx_corr_fstage makes for nspectra spectra.  The two
+ * sub-chunk launches are exposed one at a time for per-kernel timing: rows of
+ * spectra [t_first, t_first + nspectra), nspectra <= depth, then their corner
+ * turn to spectra [s0, s0 + nspectra) of a chunk of chunk_spectra spectra in
+ * d_chan. */
+int fx_corr_fstage_launches(fx_corr* corr, long nspectra, int out_prec);
 long fx_corr_fstage_depth(fx_corr* corr);
 int fx_corr_fstage_rows(fx_corr* corr, const uint32_t* d_words, size_t in_stride_words,
                         long t_first, long nspectra, int out_prec);
