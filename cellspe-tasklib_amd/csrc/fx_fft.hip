@@ -555,15 +555,18 @@ __device__ __forceinline__ void store_run(V* dst, const V (&v)[F])
 }
 
 // F kernel with the corner turn folded in: a 1024-thread workgroup transforms
-// F = 16384/M items that are F consecutive columns of the engine layout
-// (F inputs of one spectrum for [c][t][i], or F spectra of one input for the
-// FP8 [c][i][t]), so each lane stores one channel of all F items as one
-// 8-32 byte vector out[k][col .. col+F-1].  The workgroups of neighbouring
-// columns run on one XCD (xcd_block), so the partial lines of a spectrum meet
-// in that XCD's L2 and reach HBM as (mostly) whole lines.  The host
-// guarantees that every workgroup's F items exist and that col, ostride and
-// ostride_i are multiples of F.
-template <int M, typename T, bool PMO>
+// F = 16384/M items per round, RD rounds, FC = RD*F consecutive columns of
+// the engine layout (FC inputs of one spectrum for [c][t][i], or FC spectra
+// of one input for the FP8 [c][i][t]).  The values of every round stay in
+// registers and each lane stores one channel of all FC columns as one
+// vector out[k][col .. col+FC-1] (RD = 2 for the 4-byte FP16 and 2-byte FP8
+// samples: 32- and 16-byte runs, so the vector memory unit touches half as
+// many lines).  The workgroups of neighbouring columns run on one XCD
+// (xcd_block), so the partial lines of a spectrum meet in that XCD's L2 and
+// reach HBM as (mostly) whole lines.  The host guarantees that every
+// workgroup's FC items exist and that col0, ostride and ostride_i are
+// multiples of FC.
+template <int M, typename T, bool PMO, int RD>
 __global__ __launch_bounds__(1024) void k_fstage_direct(
     const float2* __restrict__ twg, const uint32_t* __restrict__ words, size_t in_stride, int ninp,
     long t_first, long nt, const fx_phase_model* __restrict__ pm, const float2* __restrict__ mult,
@@ -572,8 +575,10 @@ __global__ __launch_bounds__(1024) void k_fstage_direct(
 {
     constexpr int NT = 1024;
     constexpr int F = 16384 / M;
+    constexpr int FC = F * RD;
     constexpr int N = 2 * M;
     constexpr int H = M / 2;
+    constexpr int QN = H / NT;
     static_assert(H % NT == 0 && NT % 64 == 0, "direct F kernel: M in 2048 .. 8192");
     typedef typename ChanT<T>::V V;
     constexpr bool IM = std::is_same<T, uint16_t>::value;
@@ -582,41 +587,58 @@ __global__ __launch_bounds__(1024) void k_fstage_direct(
     __shared__ FTabs<M> tb;
     __shared__ FItems<M, F> it;
     const int tid = threadIdx.x;
-    const long nblk = nt * ninp / F;
+    const long nblk = nt * ninp / FC;
     const long lb = xcd_block(nblk);
     if (lb >= nblk) return;
-    const long g0 = lb * F;
-    fstage_front<M, F, NT, IM>(twg, words, in_stride, ninp, t_first, nt, pm, sc_uni, sc_inp, g0, sh, tb, it, tid);
+    const long gb = lb * FC;
+    fstage_consts<M, NT>(twg, tb, tid);
     const bool has_pm = PMO || pm != nullptr;
-    const long col = IM ? (g0 / nt) * ostride_i + col0 + (g0 % nt) : col0 + g0;
-    const float2 wstep = twg[NT * (FX_TW_N / N)];
-    float2 w = twg[tid * (FX_TW_N / N)];
+    const long col = IM ? (gb / nt) * ostride_i + col0 + (gb % nt) : col0 + gb;
     const int lo_k = tid % PLO, lo_m = (PLO - lo_k) % PLO;
+    V vk[QN][FC], vm[QN][FC];
+    // tl is tid re-derived every round (an empty asm the compiler cannot see
+    // through), so the rounds' address arithmetic is not shared and kept live
+    // beside the held values
+    int tl = tid;
 #pragma unroll
-    for (int q = 0; q < H / NT; q++) {
-        const int kp = q * NT + tid;
-        if (q) w = cmul(w, wstep);
-        const int k2 = M - kp;
-        const int k2m = k2 & (M - 1);   // kp = 0: k2 = M is the dropped Nyquist bin
-        V vk[F], vm[F];
-#pragma unroll
-        for (int f = 0; f < F; f++) {
-            float2 xk, xm;
-            rsplit2_tw<M>(sh, f * M, kp, w, xk, xm);
-            vk[f] = fstage_finish<M, T, F, PMO>(xk, it, f, kp, has_pm ? it.ph_lo[f][lo_k] : make_float2(1.f, 0.f), has_pm,
-                                           mult, sc_chan);
-            vm[f] = fstage_finish<M, T, F, PMO>(xm, it, f, k2m, has_pm ? it.ph_lo[f][lo_m] : make_float2(1.f, 0.f),
-                                           has_pm, mult, sc_chan);
+    for (int rd = 0; rd < RD; rd++) {
+        if (rd) {
+            __syncthreads();   // the previous round's epilogue has read sh and it
+            asm volatile("" : "+v"(tl));
         }
-        store_run<V, F>(out + (size_t)kp * ostride + col, vk);
-        if (kp != 0) store_run<V, F>(out + (size_t)k2 * ostride + col, vm);
+        fstage_group<M, F, NT, IM>(words, in_stride, ninp, t_first, nt, pm, sc_uni, sc_inp, gb + rd * F, sh, tb, it,
+                                   tl, [](auto) {});
+        const float2 wstep = twg[NT * (FX_TW_N / N)];
+        float2 w = twg[tl * (FX_TW_N / N)];
+#pragma unroll
+        for (int q = 0; q < QN; q++) {
+            const int kp = q * NT + tl;
+            if (q) w = cmul(w, wstep);
+            const int k2m = (M - kp) & (M - 1);   // kp = 0: the dropped Nyquist bin
+#pragma unroll
+            for (int f = 0; f < F; f++) {
+                float2 xk, xm;
+                rsplit2_tw<M>(sh, f * M, kp, w, xk, xm);
+                vk[q][rd * F + f] = fstage_finish<M, T, F, PMO>(
+                    xk, it, f, kp, has_pm ? it.ph_lo[f][lo_k] : make_float2(1.f, 0.f), has_pm, mult, sc_chan);
+                vm[q][rd * F + f] = fstage_finish<M, T, F, PMO>(
+                    xm, it, f, k2m, has_pm ? it.ph_lo[f][lo_m] : make_float2(1.f, 0.f), has_pm, mult, sc_chan);
+            }
+        }
+        // kp = M/2 of each item
+        if (tl < F) {
+            float2 xk, xm;
+            rsplit2_tw<M>(sh, tl * M, M / 2, twg[(M / 2) * (FX_TW_N / N)], xk, xm);
+            out[(size_t)(M / 2) * ostride + col + rd * F + tl] = fstage_finish<M, T, F, PMO>(
+                xk, it, tl, M / 2, has_pm ? it.ph_lo[tl][(M / 2) % PLO] : make_float2(1.f, 0.f), has_pm, mult,
+                sc_chan);
+        }
     }
-    // kp = M/2 of each item
-    if (tid < F) {
-        float2 xk, xm;
-        rsplit2_tw<M>(sh, tid * M, M / 2, twg[(M / 2) * (FX_TW_N / N)], xk, xm);
-        out[(size_t)(M / 2) * ostride + col + tid] = fstage_finish<M, T, F, PMO>(
-            xk, it, tid, M / 2, has_pm ? it.ph_lo[tid][(M / 2) % PLO] : make_float2(1.f, 0.f), has_pm, mult, sc_chan);
+#pragma unroll
+    for (int q = 0; q < QN; q++) {
+        const int kp = q * NT + tid;
+        store_run<V, FC>(out + (size_t)kp * ostride + col, vk[q]);
+        if (kp != 0) store_run<V, FC>(out + (size_t)(M - kp) * ostride + col, vm[q]);
     }
 }
 
@@ -746,23 +768,37 @@ int launch_fstage(hipStream_t s, const float2* tw, const uint32_t* words, size_t
     }
 }
 
+// rounds of the direct F kernel: 2 for the 2-byte FP8 samples (16-byte runs
+// per lane: 7.10 vs 7.31 ms per 4096-spectrum chunk of 128 inputs), 1 for
+// FP16 and FP32 (16- and 32-byte runs; two FP16 rounds measured 9.30 vs
+// 8.63 ms, their held values leave the straight-line epilogue no registers)
+template <typename T>
+constexpr int direct_rounds()
+{
+    return std::is_same<T, uint16_t>::value ? 2 : 1;
+}
+
 template <int M, typename T>
 int launch_fstage_direct_t(hipStream_t s, const float2* tw, const uint32_t* words, size_t stride, int ninp,
                            long t_first, long nt, const fx_phase_model* pm, const float2* mult, const float* sc, float su,
                            const float* si, void* out, long ostride, long col0, long ostride_i)
 {
-    constexpr int F = 16384 / M;
+    constexpr int RD = direct_rounds<T>();
+    constexpr int FC = (16384 / M) * RD;
     constexpr bool IM = std::is_same<T, uint16_t>::value;
-    if ((IM ? nt : (long)ninp) % F || ostride % F || col0 % F || (IM && ostride_i % F)) return FX_ERR_ARG;
-    const long nblk = nt * ninp / F;
+    if ((IM ? nt : (long)ninp) % FC || ostride % FC || col0 % FC || (IM && ostride_i % FC)) return FX_ERR_ARG;
+    const long nblk = nt * ninp / FC;
     if (nblk == 0) return FX_OK;
     const long grid = xcd_grid(nblk);
-    if (pm && !mult && !sc)
-        hipLaunchKernelGGL((k_fstage_direct<M, T, true>), dim3((unsigned)grid), dim3(1024), 0, s, tw,
+    // the straight-line phase-model-only epilogue needs more registers than
+    // two rounds leave (it spills there), so two-round launches take the
+    // general one
+    if (RD == 1 && pm && !mult && !sc)
+        hipLaunchKernelGGL((k_fstage_direct<M, T, RD == 1, RD>), dim3((unsigned)grid), dim3(1024), 0, s, tw,
                            words, stride, ninp, t_first, nt, pm, mult, sc, su, si,
                            reinterpret_cast<typename ChanT<T>::V*>(out), ostride, col0, ostride_i);
     else
-        hipLaunchKernelGGL((k_fstage_direct<M, T, false>), dim3((unsigned)grid), dim3(1024), 0, s, tw,
+        hipLaunchKernelGGL((k_fstage_direct<M, T, false, RD>), dim3((unsigned)grid), dim3(1024), 0, s, tw,
                            words, stride, ninp, t_first, nt, pm, mult, sc, su, si,
                            reinterpret_cast<typename ChanT<T>::V*>(out), ostride, col0, ostride_i);
     return hipGetLastError() == hipSuccess ? FX_OK : FX_ERR_HIP;
@@ -899,7 +935,9 @@ int fxk_fstage_direct(hipStream_t s, const float2* tw, const uint32_t* words, si
 bool fxk_fstage_direct_ok(int n, int ninp, long nt, int out_type, long ostride, long col0, long ostride_i)
 {
     if (n != 4096 && n != 8192 && n != 16384) return false;
-    const long F = 32768 / n;
+    const long F = (32768 / n) * (out_type == FX_PREC_F32   ? direct_rounds<float2>()
+                                  : out_type == FX_PREC_F16 ? direct_rounds<__half2>()
+                                                            : direct_rounds<uint16_t>());
     if (out_type == FX_PREC_FP8) return nt % F == 0 && ostride % F == 0 && col0 % F == 0 && ostride_i % F == 0;
     if (out_type != FX_PREC_F32 && out_type != FX_PREC_F16) return false;
     return ninp % F == 0 && ostride % F == 0 && col0 % F == 0;
