@@ -566,15 +566,15 @@ __device__ __forceinline__ void store_run(V* dst, const V (&v)[F])
 // reach HBM as (mostly) whole lines.  The host guarantees that every
 // workgroup's FC items exist and that col0, ostride and ostride_i are
 // multiples of FC.
-template <int M, typename T, bool PMO, int RD>
-__global__ __launch_bounds__(1024) void k_fstage_direct(
+template <int M, typename T, bool PMO, int RD, int FMX>
+__global__ __launch_bounds__(FMX / 16) void k_fstage_direct(
     const float2* __restrict__ twg, const uint32_t* __restrict__ words, size_t in_stride, int ninp,
     long t_first, long nt, const fx_phase_model* __restrict__ pm, const float2* __restrict__ mult,
     const float* __restrict__ sc_chan, float sc_uni, const float* __restrict__ sc_inp,
     typename ChanT<T>::V* __restrict__ out, long ostride, long col0, long ostride_i)
 {
-    constexpr int NT = 1024;
-    constexpr int F = 16384 / M;
+    constexpr int NT = FMX / 16;
+    constexpr int F = FMX / M;
     constexpr int FC = F * RD;
     constexpr int N = 2 * M;
     constexpr int H = M / 2;
@@ -778,13 +778,24 @@ constexpr int direct_rounds()
     return std::is_same<T, uint16_t>::value ? 2 : 1;
 }
 
+// complex points per workgroup round of the direct F kernel (threads = /16):
+// 16384 (1024 threads, one workgroup per CU).  The store width per lane is
+// what matters: 8192-point rounds (512 threads, two workgroups per CU, 8-byte
+// FP16 runs) measured 13.95 vs 8.63 ms per chunk.
+template <typename T>
+constexpr int direct_fm()
+{
+    return 16384;
+}
+
 template <int M, typename T>
 int launch_fstage_direct_t(hipStream_t s, const float2* tw, const uint32_t* words, size_t stride, int ninp,
                            long t_first, long nt, const fx_phase_model* pm, const float2* mult, const float* sc, float su,
                            const float* si, void* out, long ostride, long col0, long ostride_i)
 {
     constexpr int RD = direct_rounds<T>();
-    constexpr int FC = (16384 / M) * RD;
+    constexpr int FMX = direct_fm<T>();
+    constexpr int FC = (FMX / M) * RD;
     constexpr bool IM = std::is_same<T, uint16_t>::value;
     if ((IM ? nt : (long)ninp) % FC || ostride % FC || col0 % FC || (IM && ostride_i % FC)) return FX_ERR_ARG;
     const long nblk = nt * ninp / FC;
@@ -794,11 +805,11 @@ int launch_fstage_direct_t(hipStream_t s, const float2* tw, const uint32_t* word
     // two rounds leave (it spills there), so two-round launches take the
     // general one
     if (RD == 1 && pm && !mult && !sc)
-        hipLaunchKernelGGL((k_fstage_direct<M, T, RD == 1, RD>), dim3((unsigned)grid), dim3(1024), 0, s, tw,
+        hipLaunchKernelGGL((k_fstage_direct<M, T, RD == 1, RD, FMX>), dim3((unsigned)grid), dim3(FMX / 16), 0, s, tw,
                            words, stride, ninp, t_first, nt, pm, mult, sc, su, si,
                            reinterpret_cast<typename ChanT<T>::V*>(out), ostride, col0, ostride_i);
     else
-        hipLaunchKernelGGL((k_fstage_direct<M, T, false, RD>), dim3((unsigned)grid), dim3(1024), 0, s, tw,
+        hipLaunchKernelGGL((k_fstage_direct<M, T, false, RD, FMX>), dim3((unsigned)grid), dim3(FMX / 16), 0, s, tw,
                            words, stride, ninp, t_first, nt, pm, mult, sc, su, si,
                            reinterpret_cast<typename ChanT<T>::V*>(out), ostride, col0, ostride_i);
     return hipGetLastError() == hipSuccess ? FX_OK : FX_ERR_HIP;
@@ -935,9 +946,10 @@ int fxk_fstage_direct(hipStream_t s, const float2* tw, const uint32_t* words, si
 bool fxk_fstage_direct_ok(int n, int ninp, long nt, int out_type, long ostride, long col0, long ostride_i)
 {
     if (n != 4096 && n != 8192 && n != 16384) return false;
-    const long F = (32768 / n) * (out_type == FX_PREC_F32   ? direct_rounds<float2>()
-                                  : out_type == FX_PREC_F16 ? direct_rounds<__half2>()
-                                                            : direct_rounds<uint16_t>());
+    // columns per workgroup: (points per round / M) x rounds, M = n/2
+    const long F = out_type == FX_PREC_F32   ? 2L * direct_fm<float2>() / n * direct_rounds<float2>()
+                 : out_type == FX_PREC_F16 ? 2L * direct_fm<__half2>() / n * direct_rounds<__half2>()
+                                           : 2L * direct_fm<uint16_t>() / n * direct_rounds<uint16_t>();
     if (out_type == FX_PREC_FP8) return nt % F == 0 && ostride % F == 0 && col0 % F == 0 && ostride_i % F == 0;
     if (out_type != FX_PREC_F32 && out_type != FX_PREC_F16) return false;
     return ninp % F == 0 && ostride % F == 0 && col0 % F == 0;
