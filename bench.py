@@ -180,6 +180,42 @@ def large_n(fxlib, ctx, args):
     return out
 
 
+def products(fxlib, ctx, dvis, P, C, T, args):
+    """Lag spectra and amplitude/phase of the dumped visibilities of the
+    mid-size correlator (zerocorr.c:620-645 downstream products): one batched
+    backward FFT of P rows x C channels, then |.| and arg of the lags."""
+    n = P * C
+    dl = ctx.alloc(n * 8)
+    da, dp = ctx.alloc(n * 4), ctx.alloc(n * 4)
+    scale = 1.0 / T
+    ctx.lags(dvis, P, C, scale, dl)
+    ctx.amp_phase(dl, n, 1.0, da, dp)
+    ctx.sync()
+    reps = max(3, args.steps)
+    e = [ctx.event() for _ in range(3)]
+    ctx.record(e[0])
+    for _ in range(reps):
+        ctx.lags(dvis, P, C, scale, dl)
+    ctx.record(e[1])
+    for _ in range(reps):
+        ctx.amp_phase(dl, n, 1.0, da, dp)
+    ctx.record(e[2])
+    ctx.sync()
+    lag_ms = ctx.elapsed_ms(e[0], e[1]) / reps
+    ap_ms = ctx.elapsed_ms(e[1], e[2]) / reps
+    out = {"workload": f"{P} products x {C} channels: lag spectra (backward FFT, 1/n_spectra) then "
+                       f"amplitude and phase",
+           "lags_ms": lag_ms, "amp_phase_ms": ap_ms,
+           "lags_gbps": 2.0 * n * 8 / (lag_ms * 1e-3) / 1e9,
+           "lags_fft_tflops": 5.0 * C * np.log2(C) * P / (lag_ms * 1e-3) / 1e12,
+           "amp_phase_gbps": n * (8 + 8) / (ap_ms * 1e-3) / 1e9}
+    log(f"[bench] products: lags {lag_ms:.2f} ms ({out['lags_gbps']:.0f} GB/s), "
+        f"amp/phase {ap_ms:.2f} ms ({out['amp_phase_gbps']:.0f} GB/s)")
+    for b in (dl, da, dp):
+        b.free()
+    return out
+
+
 def batched(fxlib, ctx, args):
     """32 subbands with FFT lengths drawn from {256,1024,4096,16384}, station
     counts from 8..48 (x2 pol), rho = 0.03, 2^22 samples per input, FP64
@@ -432,6 +468,7 @@ def main():
                          f"project radix-2 FFT, not FFTW), {dt:.1f}s"}
         log(f"[bench] CPU baseline: {cpu['value']:.3e} CMAC/s on {nth} threads ({ts} spectra, {dt:.1f}s)")
     dw.free()
+    prod = products(fxlib, ctx, dvis, P, C, T, args)
     dvis.free()
 
     large = None
@@ -512,6 +549,7 @@ def main():
         "err_vs_fp64": err,
         "large_n_xengine": large,
         "batched_subbands": batch,
+        "visibility_products": prod,
         "roofline": roof,
         "cpu_baseline": cpu,
     }

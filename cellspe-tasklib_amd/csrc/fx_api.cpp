@@ -310,6 +310,44 @@ extern "C" int fx_fft_c2c(fx_ctx* ctx, const fx_cf32* d_in, fx_cf32* d_out, int 
     return FX_OK;
 }
 
+// ---------------------------------------------------------------------------
+// visibility products (zerocorr.c:620-645, m5pcal.c:94-150)
+// ---------------------------------------------------------------------------
+extern "C" int fx_lags(fx_ctx* ctx, const fx_cf32* d_vis, long nrows, int nchan, float scale, fx_cf32* d_lags)
+{
+    int rc = ctx_device(ctx);
+    if (rc) return rc;
+    FX_REQUIRE(d_vis && d_lags, FX_ERR_ARG, "fx_lags: null argument");
+    FX_REQUIRE(is_pow2(nchan) && nchan >= 2 && nchan <= 16384, FX_ERR_ARG, "fx_lags: nchan=%d", nchan);
+    FX_REQUIRE(nrows >= 0, FX_ERR_ARG, "fx_lags: nrows=%ld", nrows);
+    if (nrows == 0) return FX_OK;
+    FX_CHECK_LAUNCH(fxk_fft_c2c(ctx->stream, ctx->tw, (const float2*)d_vis, (float2*)d_lags, nchan, nrows, 1,
+                                scale));
+    return FX_OK;
+}
+
+extern "C" int fx_amp_phase(fx_ctx* ctx, const fx_cf32* d_z, long n, float scale, float* d_amp, float* d_phase)
+{
+    int rc = ctx_device(ctx);
+    if (rc) return rc;
+    FX_REQUIRE(d_z && (d_amp || d_phase) && n >= 0, FX_ERR_ARG, "fx_amp_phase: bad argument");
+    FX_CHECK_LAUNCH(fxk_amp_phase(ctx->stream, (const float2*)d_z, n, scale, d_amp, d_phase));
+    return FX_OK;
+}
+
+extern "C" int fx_pcal_delay(fx_ctx* ctx, const fx_cf64* d_tones, int ninp, int ntones, double tone0_hz,
+                             double tone_step_hz, double band_center_hz, double band_valid_hz, double* d_delay_ns)
+{
+    int rc = ctx_device(ctx);
+    if (rc) return rc;
+    FX_REQUIRE(d_tones && d_delay_ns && ninp >= 0 && ntones >= 0, FX_ERR_ARG, "fx_pcal_delay: bad argument");
+    FX_REQUIRE(tone_step_hz != 0.0 && std::isfinite(tone_step_hz) && std::isfinite(tone0_hz), FX_ERR_ARG,
+               "fx_pcal_delay: tone spacing %g Hz", tone_step_hz);
+    FX_CHECK_LAUNCH(fxk_pcal_delay(ctx->stream, (const double*)d_tones, ninp, ntones, tone0_hz, tone_step_hz,
+                                   band_center_hz, band_valid_hz, d_delay_ns));
+    return FX_OK;
+}
+
 extern "C" int fx_fft_r2c(fx_ctx* ctx, const float* d_in, fx_cf32* d_out, int n, long batch,
                           int keep_nyquist)
 {

@@ -37,10 +37,14 @@ struct FftShape {
 // ---------------------------------------------------------------------------
 // complex -> complex
 // ---------------------------------------------------------------------------
-template <int M>
+// INV: the unnormalized backward transform (sign +) times scale, as
+// conj(FFT(conj x)) -- the lag spectrum of a visibility row
+// (zerocorr.c:391,633-645: FFTW_BACKWARD over the channels, 1/n_ffts).
+template <int M, bool INV = false>
 __global__ __launch_bounds__(FftShape<M>::NT) void k_fft_c2c(const float2* __restrict__ twg,
                                                              const float2* __restrict__ in,
-                                                             float2* __restrict__ out, long batch)
+                                                             float2* __restrict__ out, long batch,
+                                                             float scale = 1.0f)
 {
     using S = FftShape<M>;
     __shared__ float2 sh[S::LDS];
@@ -52,14 +56,20 @@ __global__ __launch_bounds__(FftShape<M>::NT) void k_fft_c2c(const float2* __res
 #pragma unroll
     for (int s = 0; s < S::QPT; s++) {
         int e = tid + s * S::NT;
-        sh[lpad(e)] = (e < nvalid) ? in[f0 * M + e] : make_float2(0.f, 0.f);
+        float2 v = (e < nvalid) ? in[f0 * M + e] : make_float2(0.f, 0.f);
+        if (INV) v.y = -v.y;
+        sh[lpad(e)] = v;
     }
     __syncthreads();
     fft_passes<M, 1, S::NT, S::F>(sh, tw, tid);
 #pragma unroll
     for (int s = 0; s < S::QPT; s++) {
         int e = tid + s * S::NT;
-        if (e < nvalid) out[f0 * M + e] = sh[lpad(e)];
+        if (e < nvalid) {
+            float2 v = sh[lpad(e)];
+            if (INV) v = make_float2(v.x * scale, -v.y * scale);
+            out[f0 * M + e] = v;
+        }
     }
 }
 
@@ -681,11 +691,17 @@ __global__ __launch_bounds__(256) void k_cornerturn(const V* __restrict__ nat, V
 }
 
 template <int M>
-int launch_c2c(hipStream_t s, const float2* tw, const float2* in, float2* out, long batch)
+int launch_c2c(hipStream_t s, const float2* tw, const float2* in, float2* out, long batch, int inverse = 0,
+               float scale = 1.0f)
 {
     using S = FftShape<M>;
     long nblk = (batch + S::F - 1) / S::F;
-    hipLaunchKernelGGL(k_fft_c2c<M>, dim3((unsigned)nblk), dim3(S::NT), 0, s, tw, in, out, batch);
+    if (inverse)
+        hipLaunchKernelGGL((k_fft_c2c<M, true>), dim3((unsigned)nblk), dim3(S::NT), 0, s, tw, in, out, batch,
+                           scale);
+    else
+        hipLaunchKernelGGL((k_fft_c2c<M, false>), dim3((unsigned)nblk), dim3(S::NT), 0, s, tw, in, out, batch,
+                           1.0f);
     return hipGetLastError() == hipSuccess ? FX_OK : FX_ERR_HIP;
 }
 
@@ -840,23 +856,24 @@ int launch_fstage_direct(hipStream_t s, const float2* tw, const uint32_t* words,
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
-int fxk_fft_c2c(hipStream_t s, const float2* tw, const float2* in, float2* out, int m, long batch)
+int fxk_fft_c2c(hipStream_t s, const float2* tw, const float2* in, float2* out, int m, long batch, int inverse,
+                float scale)
 {
     switch (m) {
-    case 2: return launch_c2c<2>(s, tw, in, out, batch);
-    case 4: return launch_c2c<4>(s, tw, in, out, batch);
-    case 8: return launch_c2c<8>(s, tw, in, out, batch);
-    case 16: return launch_c2c<16>(s, tw, in, out, batch);
-    case 32: return launch_c2c<32>(s, tw, in, out, batch);
-    case 64: return launch_c2c<64>(s, tw, in, out, batch);
-    case 128: return launch_c2c<128>(s, tw, in, out, batch);
-    case 256: return launch_c2c<256>(s, tw, in, out, batch);
-    case 512: return launch_c2c<512>(s, tw, in, out, batch);
-    case 1024: return launch_c2c<1024>(s, tw, in, out, batch);
-    case 2048: return launch_c2c<2048>(s, tw, in, out, batch);
-    case 4096: return launch_c2c<4096>(s, tw, in, out, batch);
-    case 8192: return launch_c2c<8192>(s, tw, in, out, batch);
-    case 16384: return launch_c2c<16384>(s, tw, in, out, batch);
+    case 2: return launch_c2c<2>(s, tw, in, out, batch, inverse, scale);
+    case 4: return launch_c2c<4>(s, tw, in, out, batch, inverse, scale);
+    case 8: return launch_c2c<8>(s, tw, in, out, batch, inverse, scale);
+    case 16: return launch_c2c<16>(s, tw, in, out, batch, inverse, scale);
+    case 32: return launch_c2c<32>(s, tw, in, out, batch, inverse, scale);
+    case 64: return launch_c2c<64>(s, tw, in, out, batch, inverse, scale);
+    case 128: return launch_c2c<128>(s, tw, in, out, batch, inverse, scale);
+    case 256: return launch_c2c<256>(s, tw, in, out, batch, inverse, scale);
+    case 512: return launch_c2c<512>(s, tw, in, out, batch, inverse, scale);
+    case 1024: return launch_c2c<1024>(s, tw, in, out, batch, inverse, scale);
+    case 2048: return launch_c2c<2048>(s, tw, in, out, batch, inverse, scale);
+    case 4096: return launch_c2c<4096>(s, tw, in, out, batch, inverse, scale);
+    case 8192: return launch_c2c<8192>(s, tw, in, out, batch, inverse, scale);
+    case 16384: return launch_c2c<16384>(s, tw, in, out, batch, inverse, scale);
     default: return FX_ERR_ARG;
     }
 }

@@ -339,7 +339,9 @@ int fxk_synth_channelized(hipStream_t s, uint64_t seed, int nchan, long ntime, i
                           double rho, double thresh, int imajor, uint8_t* codes);
 
 int fxk_twiddle_init(hipStream_t s, float2* tw);
-int fxk_fft_c2c(hipStream_t s, const float2* tw, const float2* in, float2* out, int m, long batch);
+// inverse: the unnormalized backward transform times scale
+int fxk_fft_c2c(hipStream_t s, const float2* tw, const float2* in, float2* out, int m, long batch,
+                int inverse = 0, float scale = 1.0f);
 int fxk_fft_r2c(hipStream_t s, const float2* tw, const float* in, float2* out, int n, long batch,
                 int keep_nyquist);
 int fxk_spectrometer(hipStream_t s, const float2* tw, const float* window, const uint32_t* words,
@@ -397,6 +399,12 @@ int fxk_complex_add(hipStream_t s, const float2* a, const float2* b, float2* out
 int fxk_iotest(hipStream_t s, float* out);
 // fill n 32-bit words with value (the graph-capturable clear)
 int fxk_fill32(hipStream_t s, void* p, uint32_t value, size_t n);
+// amplitude |z|*scale and phase atan2(im, re) of n complex values (either output may be null)
+int fxk_amp_phase(hipStream_t s, const float2* z, long n, float scale, float* amp, float* phase);
+// per input: the m5pcal adjacent-tone delay fit (ns) of ntones tones at
+// f0 + t*df Hz, tones[y*ntones + t]
+int fxk_pcal_delay(hipStream_t s, const double* tones, int ninp, int ntones, double f0_hz, double df_hz,
+                   double band_center_hz, double band_valid_hz, double* delay_ns);
 int fxk_difx_post(hipStream_t s, const float2* fft, int nfft, int band0, int n, const float2* mult,
                   float2* out, float2* conj_out, float2* acc);
 

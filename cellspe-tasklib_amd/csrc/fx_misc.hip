@@ -327,6 +327,68 @@ __global__ void k_cadd(const float2* __restrict__ a, const float2* __restrict__ 
     out[i] = make_float2(x.x + y.x, x.y + y.y);
 }
 
+// amplitude and phase of scaled visibilities or lags, in FP64 then rounded
+// (zerocorr.c:628-630,639-643: x, y scaled by 1/n_ffts, sqrt(x^2+y^2),
+// atan2(y, x)); grid-stride
+__global__ void k_amp_phase(const float2* __restrict__ z, long n, float scale, float* __restrict__ amp,
+                            float* __restrict__ phase)
+{
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        const float2 v = z[i];
+        const double x = (double)v.x * (double)scale, y = (double)v.y * (double)scale;
+        if (amp) amp[i] = (float)sqrt(x * x + y * y);
+        if (phase) phase[i] = (float)atan2(y, x);
+    }
+}
+
+// Phase-cal delay of each input (m5pcal.c:94-150, calcDelay): tone phases
+// phi_t = atan2(im, re) of tones at f_t = f0 + t df; reference slope from the
+// middle pair (ntones/2, ntones/2 + 1), or (0, 1) for two tones (the
+// reference reads past the end there); every adjacent pair with both tones
+// within band_valid of band_center contributes (phi_t - phi_{t-1}), folded
+// once by 2 pi toward the reference slope, over (f_t - f_{t-1}) in MHz; the
+// mean slope is turned into ns (1000 / 2 pi) and wrapped into
+// +-500 / (tone spacing in MHz).  One thread per input.
+__global__ void k_pcal_delay(const double* __restrict__ tones, int ninp, int ntones, double f0_mhz,
+                             double df_mhz, double band_center_mhz, double band_valid_mhz,
+                             double* __restrict__ delay_ns)
+{
+    const int y = blockIdx.x * blockDim.x + threadIdx.x;
+    if (y >= ninp) return;
+    const double* z = tones + (size_t)y * 2 * (size_t)ntones;
+    if (ntones < 2) {
+        delay_ns[y] = 0.0;
+        return;
+    }
+    const double pi = 3.141592653589793;
+    const int a = ntones >= 3 ? ntones / 2 : 0;
+    const double ref = atan2(z[2 * a + 3], z[2 * a + 2]) - atan2(z[2 * a + 1], z[2 * a]);
+    double sum = 0.0;
+    int n = 0;
+    double prev = atan2(z[1], z[0]);
+    for (int t = 1; t < ntones; t++) {
+        const double ph = atan2(z[2 * t + 1], z[2 * t]);
+        const double fa = f0_mhz + (t - 1) * df_mhz, fb = f0_mhz + t * df_mhz;
+        if (fabs(fa - band_center_mhz) <= band_valid_mhz && fabs(fb - band_center_mhz) <= band_valid_mhz) {
+            double d = ph - prev;
+            if (d - ref > pi) d -= 2.0 * pi;
+            if (ref - d > pi) d += 2.0 * pi;
+            sum += d / (fb - fa);
+            n++;
+        }
+        prev = ph;
+    }
+    if (n < 1) {
+        delay_ns[y] = 0.0;
+        return;
+    }
+    const double window = fabs(500.0 / df_mhz);
+    double delay = 1000.0 * (sum / n) / (2.0 * pi);
+    while (delay > window) delay -= 2.0 * window;
+    while (delay < -window) delay += 2.0 * window;
+    delay_ns[y] = delay;
+}
+
 // DiFX-process epilogue: y = band * mult; out = y; conj_out = conj(y);
 // acc.re += |y|^2 (acc.im untouched, autocorrelations are real)
 __global__ void k_difx_post(const float2* __restrict__ fft, int nfft, int band0, int n,
@@ -521,6 +583,24 @@ int fxk_complex_add(hipStream_t s, const float2* a, const float2* b, float2* out
 {
     if (n <= 0) return FX_OK;
     hipLaunchKernelGGL(k_cadd, dim3(nblocks((size_t)n, 256)), dim3(256), 0, s, a, b, out, n);
+    return last_status();
+}
+
+int fxk_amp_phase(hipStream_t s, const float2* z, long n, float scale, float* amp, float* phase)
+{
+    if (n <= 0) return FX_OK;
+    const size_t blocks = std::min<size_t>(nblocks((size_t)n, 256), (size_t)1 << 18);
+    hipLaunchKernelGGL(k_amp_phase, dim3((unsigned)blocks), dim3(256), 0, s, z, n, scale, amp, phase);
+    return last_status();
+}
+
+int fxk_pcal_delay(hipStream_t s, const double* tones, int ninp, int ntones, double f0_hz, double df_hz,
+                   double band_center_hz, double band_valid_hz, double* delay_ns)
+{
+    if (ninp <= 0) return FX_OK;
+    hipLaunchKernelGGL(k_pcal_delay, dim3((unsigned)nblocks((size_t)ninp, 64)), dim3(64), 0, s, tones, ninp,
+                       ntones, f0_hz * 1e-6, df_hz * 1e-6, band_center_hz * 1e-6, band_valid_hz * 1e-6,
+                       delay_ns);
     return last_status();
 }
 

@@ -72,6 +72,9 @@ def load():
         "fx_codes_to_chan": (i, [vp, vp, i, i, l, i, i, vp]),
         "fx_unpack2b": (i, [vp, vp, sz, i, vp]),
         "fx_fft_c2c": (i, [vp, vp, vp, i, l]),
+        "fx_lags": (i, [vp, vp, l, i, C.c_float, vp]),
+        "fx_amp_phase": (i, [vp, vp, l, C.c_float, vp, vp]),
+        "fx_pcal_delay": (i, [vp, vp, i, i, d, d, d, d, vp]),
         "fx_fft_r2c": (i, [vp, vp, vp, i, l, i]),
         "fx_complex_mac": (i, [vp, vp, vp, vp, l, i]),
         "fx_complex_mul": (i, [vp, vp, vp, vp, l]),
@@ -235,6 +238,20 @@ class Context:
 
     def fft_c2c(self, d_in, d_out, n, batch):
         check(self.lib.fx_fft_c2c(self.handle, d_in.ptr, d_out.ptr, n, batch))
+
+    def lags(self, d_vis, nrows, nchan, scale, d_lags):
+        """Lag spectra (backward FFT over channels times scale) of nrows rows."""
+        check(self.lib.fx_lags(self.handle, d_vis.ptr, nrows, nchan, scale, d_lags.ptr))
+
+    def amp_phase(self, d_z, n, scale, d_amp=None, d_phase=None):
+        check(self.lib.fx_amp_phase(self.handle, d_z.ptr, n, scale, d_amp.ptr if d_amp else None,
+                                    d_phase.ptr if d_phase else None))
+
+    def pcal_delay(self, d_tones, ninp, ntones, tone0_hz, tone_step_hz, band_center_hz, band_valid_hz,
+                   d_delay):
+        """m5pcal delay fit (ns) of each input's tones into d_delay (ninp doubles)."""
+        check(self.lib.fx_pcal_delay(self.handle, d_tones.ptr, ninp, ntones, tone0_hz, tone_step_hz,
+                                     band_center_hz, band_valid_hz, d_delay.ptr))
 
     def fft_r2c(self, d_in, d_out, n, batch, keep_nyquist=False):
         check(self.lib.fx_fft_r2c(self.handle, d_in.ptr, d_out.ptr, n, batch, int(keep_nyquist)))

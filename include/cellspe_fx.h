@@ -171,6 +171,27 @@ int fx_pcal_extract_2bit_inputs(fx_pcal* pc, const uint32_t* d_words, size_t in_
                                 size_t nsamp);
 int fx_pcal_tones_input(fx_pcal* pc, int input, fx_cf64* tones, uint64_t* nsamples);
 
+/* ---- visibility products downstream of the dump ------------------------- */
+/* Lag spectra of nrows visibility rows of nchan channels (zerocorr.c:391,
+ * 633-645: FFTW_BACKWARD over the channels, unnormalized, then the 1/n_ffts
+ * scale): lags[r][m] = scale * sum_k vis[r][k] e^{+2 pi i k m / nchan},
+ * natural order (lag j < 0 at m = j + nchan); nchan = 2^k, 2..16384.  The
+ * dump's [p][c] layout is nrows = P rows of nchan = C channels. */
+int fx_lags(fx_ctx* ctx, const fx_cf32* d_vis, long nrows, int nchan, float scale, fx_cf32* d_lags);
+/* Amplitude sqrt(x^2 + y^2) and phase atan2(y, x) of (x, y) = scale * z for
+ * n complex values (zerocorr.c:628-630,639-643), computed in FP64; either
+ * output may be NULL. */
+int fx_amp_phase(fx_ctx* ctx, const fx_cf32* d_z, long n, float scale, float* d_amp, float* d_phase);
+/* Phase-cal delay (ns) of each of ninp inputs from its ntones tones
+ * (fx_pcal_final layout, ninp x ntones complex FP64) at frequencies
+ * tone0_hz + t * tone_step_hz: the m5pcal fit (m5pcal.c:94-150) -- mean of
+ * the adjacent-tone phase slopes of the tones within band_valid_hz of
+ * band_center_hz, each folded once by 2 pi toward the slope of the two middle
+ * tones, wrapped into +-500 / (tone spacing in MHz) ns.  Phases are
+ * atan2(im, re).  0 for fewer than two usable tones. */
+int fx_pcal_delay(fx_ctx* ctx, const fx_cf64* d_tones, int ninp, int ntones, double tone0_hz,
+                  double tone_step_hz, double band_center_hz, double band_valid_hz, double* d_delay_ns);
+
 /* ---- a3/a4/a6/a7: correlator -------------------------------------------- */
 typedef struct {
     int    ninp;          /* inputs = stations x polarizations                  */

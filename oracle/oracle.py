@@ -352,3 +352,64 @@ def pcal_reference(cfg, x, sampleoffset=0):
     out = np.zeros(max(cfg.ntones, 1), dtype=np.complex128)
     load().fxo_pcal_reference(C.byref(cfg), _p(x), x.size, sampleoffset, _p(out))
     return out[: cfg.ntones]
+
+
+# ---- visibility products (zerocorr.c, m5pcal.c) ---------------------------
+def lags(vis, scale):
+    """Lag spectra of visibility rows (zerocorr.c:391,633-645): FFTW_BACKWARD
+    over the channels (unnormalized, sign +) times scale, natural order, as
+    conj(FFT(conj v)) with the radix-2 FP32 restatement."""
+    vis = np.atleast_2d(np.asarray(vis, np.complex64))
+    out = np.empty_like(vis)
+    for r in range(vis.shape[0]):
+        out[r] = np.conj(fft_r2(np.conj(vis[r]))) * np.float32(scale)
+    return out
+
+
+def lags_f64(vis, scale):
+    """The same in FP64 (exact-twiddle transform), for pinning."""
+    vis = np.atleast_2d(np.asarray(vis, np.complex128))
+    return np.stack([np.conj(fft_f64(np.conj(v))) * scale for v in vis])
+
+
+def amp_phase(z, scale):
+    """zerocorr.c:628-630,639-643: x, y = scale * (re, im); sqrt(x^2 + y^2),
+    atan2(y, x), in FP64."""
+    z = np.asarray(z, np.complex128) * float(scale)
+    return np.sqrt(z.real ** 2 + z.imag ** 2), np.arctan2(z.imag, z.real)
+
+
+def pcal_delay(tones, tone0_hz, tone_step_hz, band_center_hz, band_valid_hz):
+    """m5pcal calcDelay (m5pcal.c:94-150) on one input's complex tones at
+    tone0 + t*step Hz, phases atan2(im, re).  The reference slope comes from
+    the tones (n/2, n/2+1); with two tones the reference reads past the end
+    of its arrays, here it uses tones (0, 1).  Returns ns."""
+    z = np.asarray(tones, np.complex128)
+    nt = z.size
+    if nt < 2:
+        return 0.0
+    ph = [float(np.arctan2(v.imag, v.real)) for v in z]
+    f = [(tone0_hz + t * tone_step_hz) * 1e-6 for t in range(nt)]
+    bc, bv = band_center_hz * 1e-6, band_valid_hz * 1e-6
+    a = nt // 2 if nt >= 3 else 0
+    ref = ph[a + 1] - ph[a]
+    s, n = 0.0, 0
+    for i in range(1, nt):
+        if abs(f[i - 1] - bc) > bv or abs(f[i] - bc) > bv:
+            continue
+        d = ph[i] - ph[i - 1]
+        if d - ref > np.pi:
+            d -= 2.0 * np.pi
+        if ref - d > np.pi:
+            d += 2.0 * np.pi
+        s += d / (f[i] - f[i - 1])
+        n += 1
+    if n < 1:
+        return 0.0
+    window = abs(500.0 / (tone_step_hz * 1e-6))
+    delay = 1000.0 * (s / n) / (2.0 * np.pi)
+    while delay > window:
+        delay -= 2.0 * window
+    while delay < -window:
+        delay += 2.0 * window
+    return delay

@@ -379,3 +379,67 @@ def test_spectrometer_overlap_conventions():
             segs = np.stack([x[s * stride + j * hop: s * stride + j * hop + n] for j in range(K)])
             P = np.mean(np.abs(np.fft.rfft(segs, axis=1)[:, : n // 2]) ** 2, axis=0)
             assert nrel(out[s, 0], P) < 1e-6
+
+
+# ---- visibility products ---------------------------------------------------
+@pytest.mark.parametrize("n", [8, 64, 512])
+def test_lags_closed_forms(n):
+    """Oracle lags pinned by the backward-DFT definition (brute force FP64),
+    a single-delay KAT and Parseval."""
+    rng = np.random.default_rng(40 + n)
+    v = (rng.standard_normal((3, n)) + 1j * rng.standard_normal((3, n))).astype(np.complex64)
+    scale = 0.25
+    got = O.lags(v, scale)
+    k = np.arange(n)
+    brute = np.stack([[np.sum(row.astype(np.complex128) * np.exp(2j * np.pi * k * m / n)) * scale
+                       for m in range(n)] for row in v])
+    assert nrel(got, brute) < 1e-5
+    assert nrel(O.lags_f64(v, scale), brute) < 1e-12
+    # a visibility with linear phase e^{-2 pi i k m0 / n} is a lag at m0 (a delay of m0 lags)
+    m0 = n // 3
+    one = np.exp(-2j * np.pi * k * m0 / n).astype(np.complex64)[None, :]
+    lag = O.lags(one, 1.0 / n)[0]
+    assert abs(lag[m0] - 1.0) < 1e-5 and np.max(np.abs(np.delete(lag, m0))) < 1e-5
+    # Parseval: sum |lag|^2 = n scale^2 sum |v|^2
+    assert abs(np.sum(np.abs(got) ** 2) / (n * scale ** 2 * np.sum(np.abs(v) ** 2)) - 1) < 1e-5
+
+
+def test_amp_phase_closed_form():
+    z = np.array([3 + 4j, -1 + 0j, 0 - 2j, 1e-3 + 1e-3j], np.complex64)
+    a, p = O.amp_phase(z, 2.0)
+    assert np.allclose(a, [10.0, 2.0, 4.0, 2 * np.sqrt(2) * 1e-3], rtol=1e-6)
+    assert np.allclose(p, [np.arctan2(4, 3), np.pi, -np.pi / 2, np.pi / 4], atol=1e-6)
+
+
+def _comb(ntones, f0_hz, df_hz, tau_ns, phi0=0.3, amp=1.0):
+    f = f0_hz + df_hz * np.arange(ntones)
+    return amp * np.exp(1j * (phi0 + 2 * np.pi * f * tau_ns * 1e-9))
+
+
+@pytest.mark.parametrize("ntones,tau", [(16, 37.0), (16, -120.0), (8, 480.0), (3, 10.0), (2, -250.0)])
+def test_pcal_delay_recovers_linear_phase(ntones, tau):
+    """A comb whose phase grows as 2 pi f tau gives tau back (within the
+    +-500/spacing[MHz] ns window; 1 MHz spacing -> +-500 ns)."""
+    f0, df = 10e3, 1e6
+    z = _comb(ntones, f0, df, tau)
+    bc = f0 + df * (ntones - 1) / 2
+    got = O.pcal_delay(z, f0, df, bc, 1e9)
+    assert abs(got - tau) < 1e-9
+
+
+def test_pcal_delay_wrap_band_and_degenerate():
+    f0, df = 10e3, 1e6
+    # beyond the window the delay wraps by the ambiguity 1000/spacing[MHz] ns
+    z = _comb(16, f0, df, 700.0)
+    assert abs(O.pcal_delay(z, f0, df, f0 + 7.5 * df, 1e9) - (700.0 - 1000.0)) < 1e-9
+    # tones outside the valid band do not count: garbage there changes nothing
+    z = _comb(16, f0, df, 42.0)
+    bc, bv = f0 + 7.5 * df, 4.6e6
+    ref = O.pcal_delay(z, f0, df, bc, bv)
+    zz = z.copy()
+    zz[:2] = np.exp(1j * np.array([2.9, -1.1]))
+    zz[-2:] = np.exp(1j * np.array([0.4, 2.2]))
+    assert abs(O.pcal_delay(zz, f0, df, bc, bv) - ref) < 1e-12 and abs(ref - 42.0) < 1e-9
+    # fewer than two tones, or no pair inside the band
+    assert O.pcal_delay(z[:1], f0, df, bc, bv) == 0.0
+    assert O.pcal_delay(z, f0, df, bc, 0.1e6) == 0.0
