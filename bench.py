@@ -14,6 +14,10 @@ chunk give the per-kernel durations inside the timed region.  The headline
 X-engine is FP16 MFMA; the plain FP32 path and the scaled FP8 path run the
 same step beside it.
 
+Further sections: the visibility products (lag spectra, amplitude/phase),
+the VDIF front end (frame placement of ~1 GB of frames in HBM, and the
+double-buffered file ingest), and the batch of uneven subbands.
+
 A second section times the large-N X-engine (256 stations x 2 pol = 512
 inputs, 1024 channels, 8192 spectra of 2-bit channelized voltages): the exact
 FP8 decomposition engine against FP16 and plain FP32 MFMA engines.
@@ -216,6 +220,96 @@ def products(fxlib, ctx, dvis, P, C, T, args):
     return out
 
 
+def front_end(fxlib, ctx, args):
+    """VDIF front end: 8 threads x 4000 frames/s x 4 s of 8032-byte frames
+    (8000-byte 2-bit payloads, 4 channels; about 1 GB) resident in HBM, placed
+    into per-thread rows by fx_frames_place; then the same bytes streamed from
+    a file through the double-buffered ingest (page-cache reads, pinned
+    buffers, PCIe copies) and placed chunk by chunk."""
+    import tempfile
+    nth, fps, nsec, pb, hb = 8, 4000, 4, 8000, 32
+    fb = pb + hb
+    nf = nth * fps * nsec
+    rng = np.random.default_rng(args.seed + 5)
+    frames = np.empty((nf, fb), np.uint8)
+    frames[:, hb:] = rng.integers(0, 256, (nf, pb), dtype=np.uint8)
+    idx = np.arange(nf, dtype=np.int64)
+    sec0 = 50_000_000
+    w = np.zeros((nf, hb // 4), "<u4")
+    w[:, 0] = sec0 + idx // (fps * nth)
+    w[:, 1] = ((idx // nth) % fps) | (40 << 24)
+    w[:, 2] = (fb // 8) | (2 << 24)
+    w[:, 3] = ((idx % nth) << 16) | (1 << 26)
+    frames[:, :hb] = w.view(np.uint8).reshape(nf, hb)
+    d_raw = ctx.to_device(frames.ravel())
+    nslots, pw = fps * nsec, pb // 4
+    stride = nslots * pw
+    d_map = ctx.to_device(np.arange(nth, dtype=np.int32))
+    d_out = ctx.alloc(4 * nth * stride)
+    d_valid, d_counts = ctx.alloc(8 * nth * nslots), ctx.alloc(32)
+
+    def place(ptr, n, clear):
+        ctx.frames_place(fxlib.FX_FMT_VDIF, ptr, n, fb, hb, d_map, nth, nth, sec0, 0, fps, nslots, d_out, stride,
+                         d_valid, d_counts, clear=clear)
+
+    place(d_raw, nf, True)
+    ctx.sync()
+    counts = d_counts.download(np.uint64)
+    probe = [(3, 777), (7, nslots - 1), (0, 0)]
+    ok = int(counts[0]) == nf and int(counts[1]) == 0 and all(
+        np.array_equal(d_out.download_at(4 * (r * stride + s * pw), np.uint32, pw),
+                       frames[s * nth + r, hb:].view("<u4")) for r, s in probe)
+    reps = max(3, args.steps)
+    e = [ctx.event() for _ in range(2)]
+    ctx.record(e[0])
+    for _ in range(reps):
+        place(d_raw, nf, True)
+    ctx.record(e[1])
+    ctx.sync()
+    ms = ctx.elapsed_ms(e[0], e[1]) / reps
+    alg = nf * (fb + pb + 8.0)
+    out = {"workload": f"VDIF {nth} threads x {fps} frames/s x {nsec} s, {fb}-byte frames ({nf} frames, "
+                       f"{nf * fb / 1e9:.2f} GB) resident in HBM -> per-thread rows",
+           "place_ms": ms, "frames_per_s": nf / (ms * 1e-3),
+           "payload_gbps": nf * pb / (ms * 1e-3) / 1e9,
+           "roofline": {"bound": "hbm", "achieved": alg / (ms * 1e-3) / 1e9, "peak": PEAK["hbm"], "unit": "GB/s",
+                        "frac": alg / (ms * 1e-3) / 1e9 / PEAK["hbm"],
+                        "algorithmic": "per frame: framebytes read + payload written + 8 B range"},
+           "parity_probe_ok": bool(ok)}
+    d_raw.free()
+    # the same bytes from a file through the double-buffered ingest (half the
+    # stream, so the temporary file stays small)
+    nfi = nf // 2
+    chunk_frames = 8192
+    with tempfile.NamedTemporaryFile(prefix="fx_vdif_", suffix=".vdif", dir=os.environ.get("TMPDIR", "/tmp")) as tf:
+        frames[:nfi].tofile(tf.name)
+        with open(tf.name, "rb") as fh:          # warm the page cache
+            while fh.read(64 << 20):
+                pass
+        ctx.sync()
+        t0 = time.time()
+        g = fxlib.Ingest(ctx, tf.name, chunk_frames * fb, nbuf=3)
+        first = True
+        tot = 0
+        for p, n in g:
+            place(p, n // fb, first)
+            first = False
+            tot += n
+        ctx.sync()
+        dt = time.time() - t0
+        g.close()
+    counts = d_counts.download(np.uint64)
+    out["ingest"] = {"bytes": tot, "seconds": dt, "gbps": tot / dt / 1e9, "chunk_bytes": chunk_frames * fb,
+                     "nbuf": 3, "placed": int(counts[0]),
+                     "note": "file in the page cache -> pinned host buffers -> PCIe -> placement; "
+                             "includes the host read"}
+    log(f"[bench] front end: place {ms:.2f} ms for {nf} frames ({out['roofline']['achieved']:.0f} GB/s, "
+        f"frac {out['roofline']['frac']:.2f}), ingest {out['ingest']['gbps']:.1f} GB/s")
+    for b in (d_map, d_out, d_valid, d_counts):
+        b.free()
+    return out
+
+
 def batched(fxlib, ctx, args):
     """32 subbands with FFT lengths drawn from {256,1024,4096,16384}, station
     counts from 8..48 (x2 pol), rho = 0.03, 2^22 samples per input, FP64
@@ -321,6 +415,7 @@ def main():
     ap.add_argument("--no-fp8", action="store_true")
     ap.add_argument("--no-large", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-front-end", action="store_true")
     ap.add_argument("--batch-subbands", type=int, default=32)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-check", action="store_true")
@@ -328,6 +423,8 @@ def main():
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="instead of the benchmark, run the plain copy kernel over MIB MiB "
                          "(HBM bandwidth check and traffic-counter calibration)")
+    ap.add_argument("--front-end-only", action="store_true",
+                    help="instead of the benchmark, run only the VDIF front-end section")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -340,6 +437,11 @@ def main():
 
     if args.copy_probe:
         copy_probe(fxlib, args)
+        return
+    if args.front_end_only:
+        ctx = fxlib.Context(0)
+        print(json.dumps({"front_end": front_end(fxlib, ctx, args)}), flush=True)
+        ctx.close()
         return
 
     I, N, C, T, tau, models = workload(args)
@@ -470,6 +572,7 @@ def main():
     dw.free()
     prod = products(fxlib, ctx, dvis, P, C, T, args)
     dvis.free()
+    fe = None if args.no_front_end else front_end(fxlib, ctx, args)
 
     large = None
     if not args.no_large:
@@ -550,6 +653,7 @@ def main():
         "large_n_xengine": large,
         "batched_subbands": batch,
         "visibility_products": prod,
+        "front_end": fe,
         "roofline": roof,
         "cpu_baseline": cpu,
     }

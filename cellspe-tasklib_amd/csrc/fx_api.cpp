@@ -136,6 +136,7 @@ extern "C" void* fx_ctx_stream(fx_ctx* ctx) { return ctx ? (void*)ctx->stream : 
 
 const float2* fx_ctx_twiddles(fx_ctx* ctx) { return ctx->tw; }
 hipStream_t fx_ctx_hip_stream(fx_ctx* ctx) { return ctx->stream; }
+int fx_ctx_device_of(fx_ctx* ctx) { return ctx->device; }
 
 // ---------------------------------------------------------------------------
 // memory / events

@@ -399,6 +399,15 @@ int fxk_complex_add(hipStream_t s, const float2* a, const float2* b, float2* out
 int fxk_iotest(hipStream_t s, float* out);
 // fill n 32-bit words with value (the graph-capturable clear)
 int fxk_fill32(hipStream_t s, void* p, uint32_t value, size_t n);
+// place nframes VDIF/Mark5B frames of raw (one workgroup per frame) into
+// out rows by (second, frame number, thread); valid[(row*nslots + slot)*2]
+// gets the valid 32-bit word range of the slot's payload; counts: placed,
+// placed but blank, slot out of range, unmapped thread
+int fxk_frames_place(hipStream_t s, int format, const uint8_t* raw, long nframes, int framebytes,
+                     int payload_offset, const int* thread_row, int map_size, int nrows, long sec0, long frame0,
+                     int fps,
+                     long nslots, uint32_t* out, size_t out_stride_words, uint32_t* valid,
+                     unsigned long long* counts, bool exact);
 // amplitude |z|*scale and phase atan2(im, re) of n complex values (either output may be null)
 int fxk_amp_phase(hipStream_t s, const float2* z, long n, float scale, float* amp, float* phase);
 // per input: the m5pcal adjacent-tone delay fit (ns) of ntones tones at
@@ -411,3 +420,4 @@ int fxk_difx_post(hipStream_t s, const float2* fft, int nfft, int band0, int n, 
 // context internals shared with the task runtime and the application shims
 const float2* fx_ctx_twiddles(fx_ctx* ctx);
 hipStream_t fx_ctx_hip_stream(fx_ctx* ctx);
+int fx_ctx_device_of(fx_ctx* ctx);

@@ -36,6 +36,29 @@ class CorrConfig(C.Structure):
                 ("prec", C.c_int), ("use_phase", C.c_int)]
 
 
+FX_FMT_VDIF, FX_FMT_MARK5B = 0, 1
+FX_PLACE_CLEAR, FX_PLACE_EXACT_BLANK = 1, 2
+
+
+class VdifHeader(C.Structure):
+    _fields_ = [("seconds", C.c_uint32), ("legacy", C.c_int), ("invalid", C.c_int),
+                ("frame", C.c_uint32), ("epoch", C.c_int), ("framebytes", C.c_int),
+                ("headerbytes", C.c_int), ("nchan", C.c_uint32), ("version", C.c_int),
+                ("station", C.c_int), ("thread", C.c_int), ("nbits", C.c_int),
+                ("is_complex", C.c_int), ("mjd", C.c_int), ("second_of_day", C.c_int)]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+class Mark5bHeader(C.Structure):
+    _fields_ = [("sync_ok", C.c_int), ("frame", C.c_int), ("tvg", C.c_int), ("day", C.c_int),
+                ("second_of_day", C.c_int), ("frac_ns", C.c_double)]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
 _lib = None
 
 
@@ -72,6 +95,12 @@ def load():
         "fx_codes_to_chan": (i, [vp, vp, i, i, l, i, i, vp]),
         "fx_unpack2b": (i, [vp, vp, sz, i, vp]),
         "fx_fft_c2c": (i, [vp, vp, vp, i, l]),
+        "fx_vdif_decode": (i, [vp, C.POINTER(VdifHeader)]),
+        "fx_mark5b_decode": (i, [vp, C.POINTER(Mark5bHeader)]),
+        "fx_frames_place": (i, [vp, i, vp, sz, i, i, vp, i, i, l, l, i, l, vp, sz, vp, vp, i]),
+        "fx_ingest_open": (i, [vp, C.c_char_p, sz, sz, i, C.POINTER(vp)]),
+        "fx_ingest_next": (i, [vp, C.POINTER(vp), C.POINTER(sz)]),
+        "fx_ingest_close": (i, [vp]),
         "fx_lags": (i, [vp, vp, l, i, C.c_float, vp]),
         "fx_amp_phase": (i, [vp, vp, l, C.c_float, vp, vp]),
         "fx_pcal_delay": (i, [vp, vp, i, i, d, d, d, d, vp]),
@@ -139,6 +168,27 @@ def check(rc):
 
 def _ptr(a):
     return a.ctypes.data_as(C.c_void_p)
+
+
+def _addr(buf):
+    """Device address of a DeviceBuffer or a raw pointer value."""
+    if isinstance(buf, DeviceBuffer):
+        return buf.ptr.value
+    if isinstance(buf, C.c_void_p):
+        return buf.value
+    return int(buf)
+
+
+def vdif_decode(header: bytes):
+    h = VdifHeader()
+    check(load().fx_vdif_decode(C.c_char_p(bytes(header[:16])), C.byref(h)))
+    return h.as_dict()
+
+
+def mark5b_decode(header: bytes):
+    h = Mark5bHeader()
+    check(load().fx_mark5b_decode(C.c_char_p(bytes(header[:16])), C.byref(h)))
+    return h.as_dict()
 
 
 class DeviceBuffer:
@@ -239,6 +289,16 @@ class Context:
     def fft_c2c(self, d_in, d_out, n, batch):
         check(self.lib.fx_fft_c2c(self.handle, d_in.ptr, d_out.ptr, n, batch))
 
+    def frames_place(self, fmt, d_raw, nframes, framebytes, payload_offset, d_thread_row, map_size,
+                     nrows, sec0, frame0, frames_per_second, nslots, d_out, out_stride_words,
+                     d_valid, d_counts, clear=True, exact_blank=False, raw_offset=0):
+        """Validate, strip and place VDIF/Mark5B frames into per-thread rows."""
+        flags = (FX_PLACE_CLEAR if clear else 0) | (FX_PLACE_EXACT_BLANK if exact_blank else 0)
+        check(self.lib.fx_frames_place(self.handle, fmt, C.c_void_p(_addr(d_raw) + raw_offset), nframes,
+                                       framebytes, payload_offset, d_thread_row.ptr, map_size, nrows,
+                                       sec0, frame0, frames_per_second, nslots, d_out.ptr,
+                                       out_stride_words, d_valid.ptr, d_counts.ptr, flags))
+
     def lags(self, d_vis, nrows, nchan, scale, d_lags):
         """Lag spectra (backward FFT over channels times scale) of nrows rows."""
         check(self.lib.fx_lags(self.handle, d_vis.ptr, nrows, nchan, scale, d_lags.ptr))
@@ -294,6 +354,40 @@ class Context:
         out = np.zeros((32, 32), dtype=np.float32)
         check(self.lib.fx_mfma_probe(self.handle, which, _ptr(out)))
         return out
+
+
+class Ingest:
+    """Double-buffered file -> HBM stream (fx_ingest_*): iterate for
+    (device pointer, bytes) chunks; each stays valid until the next one."""
+
+    def __init__(self, ctx, path, chunk_bytes, nbuf=2, offset=0):
+        self.ctx = ctx
+        h = C.c_void_p()
+        check(load().fx_ingest_open(ctx.handle, os.fsencode(path), offset, chunk_bytes, nbuf, C.byref(h)))
+        self.handle = h
+
+    def next(self):
+        p, n = C.c_void_p(), C.c_size_t()
+        check(load().fx_ingest_next(self.handle, C.byref(p), C.byref(n)))
+        return (p.value, n.value) if n.value else (None, 0)
+
+    def __iter__(self):
+        while True:
+            p, n = self.next()
+            if not n:
+                return
+            yield p, n
+
+    def close(self):
+        if self.handle is not None:
+            load().fx_ingest_close(self.handle)
+        self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class PCal:

@@ -54,6 +54,12 @@ enum {
     FX_PREC_FP8_2BIT = 3 /* exact FP8 decomposition of 2-bit {+-1,+-H} voltages    */
 };
 
+/* recorded-data frame formats of the front end */
+enum {
+    FX_FMT_VDIF = 0,   /* VDIF: 32-byte header (16 legacy), payload of 8-byte multiples */
+    FX_FMT_MARK5B = 1  /* Mark5B: 16-byte header, 10000-byte payload, 10016-byte frames */
+};
+
 /* spectrometer windows: SIN2 sin^2(pi i/(n-1)) (spectrometer_spu.c:297-315);
  * COSINE, COSINE2, HAMMING, BLACKMAN (alpha 0.5) and HANN as the IPP core's
  * generate_windowfunction (TaskCoreIPP.cpp:569-605) */
@@ -170,6 +176,87 @@ int fx_pcal_create_inputs(fx_ctx* ctx, double bandwidth_hz, double spacing_hz, i
 int fx_pcal_extract_2bit_inputs(fx_pcal* pc, const uint32_t* d_words, size_t in_stride_words,
                                 size_t nsamp);
 int fx_pcal_tones_input(fx_pcal* pc, int input, fx_cf64* tones, uint64_t* nsamples);
+
+/* ---- real-file front end: VDIF / Mark5B frames -------------------------- */
+/* VDIF header fields (format_vdif.c:135-206,2199-2240; VDIF specification):
+ * word0 bits 0-29 seconds from the reference epoch, bit 30 legacy (16-byte
+ * header), bit 31 invalid; word1 bits 0-23 frame number within the second,
+ * bits 24-29 reference epoch (half-years since 2000); word2 bits 0-23 frame
+ * length in 8-byte units, bits 24-28 log2(channels), bits 29-31 version;
+ * word3 bits 0-15 station, bits 16-25 thread, bits 26-30 bits per sample - 1,
+ * bit 31 complex. */
+typedef struct {
+    uint32_t seconds;
+    int legacy, invalid;
+    uint32_t frame;
+    int epoch;
+    int framebytes, headerbytes;
+    uint32_t nchan;     /* 2^(log2 channels), up to 2^31 */
+    int version;
+    int station, thread, nbits, is_complex;
+    int mjd;            /* epoch start + seconds (format_vdif.c:143-183 table) */
+    int second_of_day;
+} fx_vdif_header;
+int fx_vdif_decode(const void* header, fx_vdif_header* h);
+/* Mark5B header (mark5_format_mark5b.c:43,129-186): word0 sync 0xABADDEED,
+ * word1 bits 0-14 frame number within the second (bit 15 test vector),
+ * word2 BCD JJJSSSSS (MJD mod 1000, second of day), word3 BCD fraction of a
+ * second in its top four digits (0.1 ms units, rounded up to the 156.25 us
+ * frame grid as mark5access "unrounds" it). */
+typedef struct {
+    int sync_ok, frame, tvg;
+    int day;            /* MJD mod 1000 */
+    int second_of_day;
+    double frac_ns;
+} fx_mark5b_header;
+int fx_mark5b_decode(const void* header, fx_mark5b_header* h);
+/* Place nframes frames of framebytes bytes from d_raw (HBM) into per-thread
+ * rows: the payload of a frame of thread th (VDIF word3 thread id; 0 for
+ * Mark5B) at time (second s, frame number f) goes to slot
+ * (s - sec0) * frames_per_second + f - frame0 of row d_thread_row[th]
+ * (map_size entries; an entry outside [0, nrows) or a thread id >= map_size
+ * means not wanted), payload_words = (framebytes -
+ * payload_offset)/4 words per slot, rows out_stride_words apart.  Seconds are
+ * VDIF epoch seconds, or day*86400 + second of day for Mark5B.  Per slot,
+ * d_valid[(row*nslots + slot)*2 + {0,1}] gets the valid 32-bit word range
+ * [start, end) of the payload, as mark5access blanks it: empty for a VDIF
+ * frame with the invalid bit or a Mark5 fill word (0x11223344 twice) at either
+ * end of the payload (format_vdif.c:2337-2343, blanker_mark5.c:282-320), and
+ * for Mark5B the span outside a fill run at one end (blanker_mark5.c:54-200,
+ * one 32 KB zone per frame; empty without the sync word).  The straggler test
+ * at the start of a Mark5B payload reads the first 32-bit word of the first
+ * non-fill 64-bit word; mark5access reads the word 8 bytes earlier, a fill
+ * word, and so starts every such range one word late.  By default the
+ * mark5access searches are reproduced as they are, including that test and
+ * their inexact bisection (it never probes the index its last step lands on,
+ * so a boundary can come up to 36 bytes late); FX_PLACE_EXACT_BLANK bisects
+ * to the exact fill boundary of a payload with one fill run and tests the
+ * right straggler word.  VDIF 2-bit payload
+ * words are copied as they are; Mark5B words get the two bits of each sample
+ * swapped (sign/magnitude codes -H,+1,-1,+H, mark5_format_mark5b.c:59-96, to
+ * this library's -H,-1,+1,+H).  d_counts (4 x u64, accumulated): placed,
+ * placed but empty, slot out of range, unmapped thread.  flags:
+ * FX_PLACE_CLEAR first empties every slot's range and zeroes d_counts;
+ * FX_PLACE_EXACT_BLANK as above.  A slot written by two frames of one call
+ * holds an unspecified mix of them.  Asynchronous on the context stream. */
+enum { FX_PLACE_CLEAR = 1, FX_PLACE_EXACT_BLANK = 2 };
+int fx_frames_place(fx_ctx* ctx, int format, const void* d_raw, size_t nframes, int framebytes,
+                    int payload_offset, const int* d_thread_row, int map_size, int nrows, long sec0,
+                    long frame0, int frames_per_second, long nslots, uint32_t* d_out, size_t out_stride_words,
+                    uint32_t* d_valid, uint64_t* d_counts, int flags);
+/* Double-buffered file ingest into HBM (the TaskDispatcher prefill and double
+ * buffering, TaskDispatcher.cpp:116-137): a reader thread fills nbuf pinned
+ * host buffers of chunk_bytes with pread and copies them to device buffers on
+ * the ingest's own stream.  fx_ingest_next makes the context stream wait for
+ * the next chunk's copy and returns its device pointer and size (0 at end of
+ * file); the chunk stays valid until the next fx_ingest_next call, which
+ * also lets its buffer be refilled once the work the caller queued on the
+ * context stream meanwhile has finished.  offset skips bytes at the start. */
+typedef struct fx_ingest fx_ingest;
+int fx_ingest_open(fx_ctx* ctx, const char* path, size_t offset, size_t chunk_bytes, int nbuf,
+                   fx_ingest** out);
+int fx_ingest_next(fx_ingest* g, const void** d_chunk, size_t* bytes);
+int fx_ingest_close(fx_ingest* g);
 
 /* ---- visibility products downstream of the dump ------------------------- */
 /* Lag spectra of nrows visibility rows of nchan channels (zerocorr.c:391,
