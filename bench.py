@@ -16,7 +16,8 @@ same step beside it.
 
 Further sections: the visibility products (lag spectra, amplitude/phase),
 the VDIF front end (frame placement of ~1 GB of frames in HBM, and the
-double-buffered file ingest), and the batch of uneven subbands.
+double-buffered file ingest), a task-runtime chain per task vs as one
+hipGraph, and the batch of uneven subbands.
 
 A second section times the large-N X-engine (256 stations x 2 pol = 512
 inputs, 1024 channels, 8192 spectra of 2-bit channelized voltages): the exact
@@ -310,6 +311,58 @@ def front_end(fxlib, ctx, args):
     return out
 
 
+def task_chains(args):
+    """Task runtime: a five-task next_task chain (1024-point FFT -> complex
+    multiply -> complex MAC, a 2-channel unpack of 4096 samples, a DiFX
+    process task fed by the FFT output) executed repeatedly through
+    celldsp_executeTask, per task (stream synchronization after each) and as
+    one replayed hipGraph (CDSP_BEHAV_GRAPH).  Host wall time per chain."""
+    import tasklib as TL
+    lib = TL.load()
+    lib.celldsp_start()
+    n, nch = 1024, 256
+    rng = np.random.default_rng(args.seed + 6)
+    x = (rng.standard_normal(n) + 1j * rng.standard_normal(n)).astype(np.complex64)
+    X, Y, acc = (np.zeros(n, np.complex64) for _ in range(3))
+    m = np.exp(1j * rng.uniform(0, 2 * np.pi, n)).astype(np.complex64)
+    words = rng.integers(0, 2**32, 512, dtype=np.uint64).astype(np.uint32)
+    f = np.zeros(8192, np.float32)
+    mult = np.exp(1j * rng.uniform(0, 2 * np.pi, nch + 1)).astype(np.complex64)
+    fo, cfo, acc2 = (np.zeros(nch + 1, np.complex64) for _ in range(3))
+    t = [TL.CellDSPTaskClass(c) for c in (TL.CELLDSP_TASK_FFT_EXEC_DYN, TL.CELLDSP_TASK_MULC,
+                                          TL.CELLDSP_TASK_MAC, TL.CELLDSP_TASK_UNPACK,
+                                          TL.CELLDSP_TASK_DIFXPROCESS)]
+    t[0].setFFTparams(n)
+    t[0].addInVec(x); t[0].addOutVec(X)
+    t[1].addInVec(X); t[1].addInVec(m); t[1].addOutVec(Y)
+    t[2].addInVec(Y); t[2].addInVec(m); t[2].addOutVec(acc)
+    t[3].task.subcontext.Unpack.nchan = 2
+    t[3].addInVec(words); t[3].addOutVec(f)
+    d = t[4].task.subcontext.Difx
+    d.nfft, d.numchannels, d.useUpperOrLower = n, nch, 1
+    t[4].addInVec(X); t[4].addInVec(mult)
+    t[4].addOutVec(fo); t[4].addOutVec(cfo); t[4].addOutVec(acc2)
+    for a, b in zip(t, t[1:]):
+        a.setNext(b)
+    out = {"workload": "5-task chain: FFT 1024 -> MULC -> MAC, UNPACK 2x4096, DIFXPROCESS 1024/256",
+           "unit": "us per chain (host wall time)"}
+    reps = 200
+    for name, beh in (("per_task", TL.CDSP_BEHAV_DEFAULT), ("graph", TL.CDSP_BEHAV_GRAPH)):
+        lib.celldsp_setBehaviour(beh)
+        for _ in range(5):
+            lib.celldsp_executeTask(t[0].getTask())
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            if lib.celldsp_executeTask(t[0].getTask()) != 0:
+                raise SystemExit("bench: task chain failed")
+        out[name] = (time.perf_counter() - t0) / reps * 1e6
+    out["graph_replays"] = int(lib.celldsp_graphReplays())
+    lib.celldsp_setBehaviour(TL.CDSP_BEHAV_DEFAULT)
+    lib.celldsp_stop()
+    log(f"[bench] task chain: per task {out['per_task']:.0f} us, graph {out['graph']:.0f} us")
+    return out
+
+
 def batched(fxlib, ctx, args):
     """32 subbands with FFT lengths drawn from {256,1024,4096,16384}, station
     counts from 8..48 (x2 pol), rho = 0.03, 2^22 samples per input, FP64
@@ -416,6 +469,7 @@ def main():
     ap.add_argument("--no-large", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-front-end", action="store_true")
+    ap.add_argument("--no-tasks", action="store_true")
     ap.add_argument("--batch-subbands", type=int, default=32)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-check", action="store_true")
@@ -424,7 +478,7 @@ def main():
                     help="instead of the benchmark, run the plain copy kernel over MIB MiB "
                          "(HBM bandwidth check and traffic-counter calibration)")
     ap.add_argument("--front-end-only", action="store_true",
-                    help="instead of the benchmark, run only the VDIF front-end section")
+                    help="instead of the benchmark, run only the VDIF front-end and task-chain sections")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -440,7 +494,8 @@ def main():
         return
     if args.front_end_only:
         ctx = fxlib.Context(0)
-        print(json.dumps({"front_end": front_end(fxlib, ctx, args)}), flush=True)
+        print(json.dumps({"front_end": front_end(fxlib, ctx, args), "task_chains": task_chains(args)}),
+              flush=True)
         ctx.close()
         return
 
@@ -573,6 +628,7 @@ def main():
     prod = products(fxlib, ctx, dvis, P, C, T, args)
     dvis.free()
     fe = None if args.no_front_end else front_end(fxlib, ctx, args)
+    chains = None if args.no_tasks else task_chains(args)
 
     large = None
     if not args.no_large:
@@ -654,6 +710,7 @@ def main():
         "batched_subbands": batch,
         "visibility_products": prod,
         "front_end": fe,
+        "task_chains": chains,
         "roofline": roof,
         "cpu_baseline": cpu,
     }

@@ -14,6 +14,7 @@
 #include "celldsp_dsp.h"
 #include "celldsp_tasklib.h"
 
+#include <algorithm>
 #include <condition_variable>
 #include <cstdio>
 #include <cstring>
@@ -22,6 +23,28 @@
 #include <vector>
 
 namespace {
+
+// A repeated next_task chain replayed as one captured hipGraph
+// (CDSP_BEHAV_GRAPH).  The host vectors of the chain are merged into
+// disjoint address regions, each mirrored by a pinned staging region: the
+// graph's copies read and write the staging, so copies between tasks that
+// pass data through a host vector keep their order and aliasing, and the
+// host copies user <-> staging once per execution.
+struct ChainGraph {
+    struct Region {
+        char* user;
+        size_t len, off;
+        bool written;
+    };
+    std::vector<unsigned char> sig;   // commands, vectors, sub-contexts, FFT size
+    int seen = 0;                     // eager executions with this signature
+    bool broken = false;              // capture failed: stay eager
+    hipGraphExec_t exec = nullptr;
+    void* pinned = nullptr;
+    size_t pinned_cap = 0;
+    std::vector<Region> regions;
+    unsigned fft_n_after = 0;         // host FFT state after the chain
+};
 
 struct Unit {
     std::thread th;
@@ -39,6 +62,7 @@ struct Unit {
     size_t cap[2 * CELLDSP_MAX_NUM_IOVECTORS + 2] = {};
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     unsigned int fft_n = 0;   // continuable FFT size (0 = none)
+    ChainGraph cg;
 };
 
 struct State {
@@ -49,6 +73,7 @@ struct State {
     unsigned flags[CELLDSP_MAX_SPE_NR] = {};
     int free_count = CELLDSP_MAX_SPE_NR;
     int waiters = 0;
+    long graph_replays = 0;   // chains run from a captured graph
     Unit units[CELLDSP_MAX_SPE_NR];
 };
 
@@ -302,6 +327,236 @@ int run_task(Unit& u, CellDSPTask_t* t)
     return rc;
 }
 
+// ---------------------------------------------------------------------------
+// chains as graphs
+// ---------------------------------------------------------------------------
+bool graphable(unsigned cmd)
+{
+    switch (cmd) {
+    case CELLDSP_TASK_IOTEST:
+    case CELLDSP_TASK_MAC:
+    case CELLDSP_TASK_MULC:
+    case CELLDSP_TASK_ADDC:
+    case CELLDSP_TASK_FFT_EXEC:
+    case CELLDSP_TASK_FFT_EXEC_DYN:
+    case CELLDSP_TASK_SINCOS:
+    case CELLDSP_TASK_SINCOS_NORM:
+    case CELLDSP_TASK_SINCOS_GENERTR:
+    case CELLDSP_TASK_UNPACK:
+    case CELLDSP_TASK_DIFXPROCESS:
+        return true;
+    default:
+        return false;
+    }
+}
+
+template <typename T>
+void put(std::vector<unsigned char>& v, const T& x)
+{
+    const unsigned char* b = reinterpret_cast<const unsigned char*>(&x);
+    v.insert(v.end(), b, b + sizeof(T));
+}
+
+// Signature of a chain; false if some task cannot be captured.
+bool chain_signature(const Unit& u, CellDSPTask_t* head, std::vector<unsigned char>& sig, int& ntasks)
+{
+    sig.clear();
+    ntasks = 0;
+    // the unit's FFT size matters only to an FFT_EXEC before any FFT_EXEC_DYN
+    bool dyn = false, uses_fft_n = false;
+    for (CellDSPTask_t* t = head; t; t = (CellDSPTask_t*)t->next_task.p) {
+        if (t->command == CELLDSP_TASK_FFT_EXEC && !dyn) uses_fft_n = true;
+        if (t->command == CELLDSP_TASK_FFT_EXEC_DYN) dyn = true;
+    }
+    put(sig, uses_fft_n ? u.fft_n : 0u);
+    for (CellDSPTask_t* t = head; t; t = (CellDSPTask_t*)t->next_task.p) {
+        if (!graphable(t->command) || t->num_input_vectors > CELLDSP_MAX_NUM_IOVECTORS ||
+            t->num_output_vectors > CELLDSP_MAX_NUM_IOVECTORS)
+            return false;
+        put(sig, t->command);
+        put(sig, t->num_input_vectors);
+        put(sig, t->num_output_vectors);
+        for (unsigned i = 0; i < t->num_input_vectors; i++) {
+            put(sig, t->inputvector_ptrs[i].p);
+            put(sig, t->inputvector_lengths[i]);
+        }
+        for (unsigned o = 0; o < t->num_output_vectors; o++) {
+            put(sig, t->outputvector_ptrs[o].p);
+            put(sig, t->outputvector_lengths[o]);
+        }
+        put(sig, t->subcontext);
+        ntasks++;
+    }
+    return ntasks >= 2;
+}
+
+void chain_graph_reset(ChainGraph& cg)
+{
+    if (cg.exec) hipGraphExecDestroy(cg.exec);
+    cg.exec = nullptr;
+    cg.sig.clear();
+    cg.regions.clear();
+    cg.seen = 0;
+    cg.broken = false;
+}
+
+void chain_graph_free(ChainGraph& cg)
+{
+    chain_graph_reset(cg);
+    if (cg.pinned) hipHostFree(cg.pinned);
+    cg.pinned = nullptr;
+    cg.pinned_cap = 0;
+}
+
+// Merge the chain's host vectors into disjoint regions with staging offsets.
+bool chain_regions(ChainGraph& cg, CellDSPTask_t* head)
+{
+    struct Iv {
+        char* p;
+        size_t len;
+        bool out;
+    };
+    std::vector<Iv> iv;
+    for (CellDSPTask_t* t = head; t; t = (CellDSPTask_t*)t->next_task.p) {
+        for (unsigned i = 0; i < t->num_input_vectors; i++)
+            if (t->inputvector_lengths[i]) iv.push_back({(char*)t->inputvector_ptrs[i].p, t->inputvector_lengths[i], false});
+        for (unsigned o = 0; o < t->num_output_vectors; o++)
+            if (t->outputvector_lengths[o]) iv.push_back({(char*)t->outputvector_ptrs[o].p, t->outputvector_lengths[o], true});
+    }
+    for (const Iv& x : iv)
+        if (!x.p) return false;
+    std::sort(iv.begin(), iv.end(), [](const Iv& a, const Iv& b) { return a.p < b.p; });
+    cg.regions.clear();
+    size_t off = 0;
+    for (const Iv& x : iv) {
+        if (!cg.regions.empty()) {
+            ChainGraph::Region& r = cg.regions.back();
+            if (x.p < r.user + r.len) {
+                const size_t end = std::max<size_t>(r.len, (size_t)(x.p - r.user) + x.len);
+                off += end - r.len;
+                r.len = end;
+                r.written = r.written || x.out;
+                continue;
+            }
+            off = (off + 255) & ~(size_t)255;
+        }
+        cg.regions.push_back({x.p, x.len, off, x.out});
+        off += x.len;
+    }
+    if (off > cg.pinned_cap) {
+        if (cg.pinned) hipHostFree(cg.pinned);
+        cg.pinned = nullptr;
+        cg.pinned_cap = 0;
+        if (hipHostMalloc(&cg.pinned, off, hipHostMallocDefault) != hipSuccess) {
+            hipGetLastError();
+            cg.pinned = nullptr;
+            return false;
+        }
+        cg.pinned_cap = off;
+    }
+    return true;
+}
+
+void* staging_of(const ChainGraph& cg, const void* p)
+{
+    const char* c = (const char*)p;
+    for (const ChainGraph::Region& r : cg.regions)
+        if (c >= r.user && c < r.user + r.len) return (char*)cg.pinned + r.off + (c - r.user);
+    return nullptr;
+}
+
+// Capture the chain (its host vectors redirected to the staging) into cg.exec.
+bool chain_capture(Unit& u, CellDSPTask_t* head)
+{
+    ChainGraph& cg = u.cg;
+    if (!chain_regions(cg, head)) return false;
+    std::vector<CellDSPTask_t> shadow;
+    for (CellDSPTask_t* t = head; t; t = (CellDSPTask_t*)t->next_task.p) {
+        CellDSPTask_t c;
+        memcpy((void*)&c, (const void*)t, sizeof(c));
+        for (unsigned i = 0; i < c.num_input_vectors; i++)
+            if (c.inputvector_lengths[i]) c.inputvector_ptrs[i].p = staging_of(cg, t->inputvector_ptrs[i].p);
+        for (unsigned o = 0; o < c.num_output_vectors; o++)
+            if (c.outputvector_lengths[o]) c.outputvector_ptrs[o].p = staging_of(cg, t->outputvector_ptrs[o].p);
+        c.next_task.p = nullptr;
+        shadow.push_back(c);
+    }
+    hipStream_t s = fx_ctx_hip_stream(u.ctx);
+    const unsigned fft_before = u.fft_n;
+    // the slots were sized by the eager run of this chain, so nothing below allocates
+    if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+        hipGetLastError();
+        return false;
+    }
+    bool ok = true;
+    for (CellDSPTask_t& c : shadow)
+        if (run_device_task(u, &c) != CELLDSP_TASK_OK) ok = false;
+    hipGraph_t graph = nullptr;
+    if (hipStreamEndCapture(s, &graph) != hipSuccess) ok = false;
+    cg.fft_n_after = u.fft_n;
+    u.fft_n = fft_before;
+    if (ok && graph) ok = hipGraphInstantiate(&cg.exec, graph, nullptr, nullptr, 0) == hipSuccess;
+    if (graph) hipGraphDestroy(graph);
+    if (!ok) {
+        hipGetLastError();
+        if (cg.exec) hipGraphExecDestroy(cg.exec);
+        cg.exec = nullptr;
+    }
+    return ok;
+}
+
+int chain_replay(Unit& u, CellDSPTask_t* head, int ntasks)
+{
+    ChainGraph& cg = u.cg;
+    for (const ChainGraph::Region& r : cg.regions) memcpy((char*)cg.pinned + r.off, r.user, r.len);
+    hipStream_t s = fx_ctx_hip_stream(u.ctx);
+    int rc = CELLDSP_TASK_OK;
+    hipEventRecord(u.ev0, s);
+    if (hipGraphLaunch(cg.exec, s) != hipSuccess) rc = fail("graph launch failed", head->command);
+    hipEventRecord(u.ev1, s);
+    if (hipStreamSynchronize(s) != hipSuccess) rc = fail("stream error", head->command);
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, u.ev0, u.ev1);
+    if (rc == CELLDSP_TASK_OK) {
+        for (const ChainGraph::Region& r : cg.regions)
+            if (r.written) memcpy(r.user, (char*)cg.pinned + r.off, r.len);
+        u.fft_n = cg.fft_n_after;
+    }
+    // device time of the whole chain, shared evenly among its tasks
+    const unsigned ticks = (unsigned)(ms * 1e-3 * CELLDSP_TIMEBASE / ntasks);
+    for (CellDSPTask_t* t = head; t; t = (CellDSPTask_t*)t->next_task.p) {
+        t->tick_count = ticks;
+        t->completed_flag = (unsigned short)rc;
+    }
+    return rc;
+}
+
+// Runs the chain from its graph when it can; false: run it eagerly.
+bool run_chain_graph(Unit& u, CellDSPTask_t* head, int* rc)
+{
+    std::vector<unsigned char> sig;
+    int ntasks = 0;
+    if (!chain_signature(u, head, sig, ntasks) || !unit_device(u)) return false;
+    ChainGraph& cg = u.cg;
+    if (sig != cg.sig) {
+        chain_graph_reset(cg);
+        cg.sig.swap(sig);
+    }
+    if (!cg.exec) {
+        if (cg.broken || cg.seen++ == 0) return false;     // first run eager: sizes the slots
+        if (!chain_capture(u, head)) {
+            cg.broken = true;
+            return false;
+        }
+    }
+    *rc = chain_replay(u, head, ntasks);
+    {
+        std::lock_guard<std::mutex> lk(g->m);
+        g->graph_replays++;
+    }
+    return true;
+}
+
 void worker(Unit* u)
 {
     for (;;) {
@@ -313,10 +568,17 @@ void worker(Unit* u)
             head = u->posted;
         }
         int rc = CELLDSP_TASK_OK;
-        // follow the streamed chain (cellspe-tasklib.cpp:594-602)
-        for (CellDSPTask_t* t = head; t; t = (CellDSPTask_t*)t->next_task.p) {
-            rc = run_task(*u, t);
-            t->completed_flag = (unsigned short)rc;
+        unsigned behaviour;
+        {
+            std::lock_guard<std::mutex> lk(g->m);
+            behaviour = g->behaviour;
+        }
+        if (!(behaviour & CDSP_BEHAV_GRAPH) || !run_chain_graph(*u, head, &rc)) {
+            // follow the streamed chain (cellspe-tasklib.cpp:594-602)
+            for (CellDSPTask_t* t = head; t; t = (CellDSPTask_t*)t->next_task.p) {
+                rc = run_task(*u, t);
+                t->completed_flag = (unsigned short)rc;
+            }
         }
         {
             std::lock_guard<std::mutex> lk(u->m);
@@ -328,6 +590,7 @@ void worker(Unit* u)
         u->cv.notify_all();
     }
     // release device resources on the thread that created them
+    chain_graph_free(u->cg);
     for (auto& p : u->slot)
         if (p) hipFree(p);
     if (u->ev0) hipEventDestroy(u->ev0);
@@ -512,6 +775,13 @@ extern "C" int celldsp_stopContinuableTask(int* speID)
     celldsp_setFreeSPE(*speID);
     *speID = -1;
     return vecNoErr;
+}
+
+extern "C" long celldsp_graphReplays(void)
+{
+    if (!g) return 0;
+    std::lock_guard<std::mutex> lk(g->m);
+    return g->graph_replays;
 }
 
 extern "C" int celldsp_freeCount(void)

@@ -31,6 +31,7 @@ CELLDSP_MAX_SPE_NR = 8
 CELLDSP_MAX_NUM_IOVECTORS = 8
 CDSP_BEHAV_DEFAULT = 0
 CDSP_BEHAV_BLOCKING = 1
+CDSP_BEHAV_GRAPH = 2
 
 # cellspe-tasklib-dsp.h:27-46
 (CELLDSP_TASK_NOP, CELLDSP_TASK_MAC, CELLDSP_TASK_MULC, CELLDSP_TASK_ADDC, CELLDSP_TASK_IOTEST,
@@ -173,6 +174,7 @@ def load():
         "celldsp_getBehaviour": (u, []),
         "celldsp_freeCount": (i, []),
         "celldsp_clientCount": (i, []),
+        "celldsp_graphReplays": (C.c_long, []),
         "speNOP": (i, []),
         "speIOCheck": (i, [vp]),
         "speAddProduct_32fc": (i, [vp, vp, vp, i]),

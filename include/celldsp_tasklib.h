@@ -91,6 +91,14 @@ typedef struct CellDSPTask_tt {
 #define CDSP_EF_CONTINUABLE 2
 #define CDSP_EF_DEFAULT 0
 #define CDSP_BEHAV_BLOCKING 1
+/* Not in the reference: a next_task chain of at least two device tasks
+ * (IOTEST, MAC, MULC, ADDC, FFT_EXEC[_DYN], SINCOS*, UNPACK, DIFXPROCESS)
+ * that a unit executes again with the same commands, vectors (addresses and
+ * lengths) and sub-contexts is captured once into a hipGraph and replayed
+ * with one launch; host vectors are mirrored through pinned staging, so data
+ * passed between tasks through a host vector keeps its order.  Each task's
+ * tick_count is then the chain's device time divided evenly. */
+#define CDSP_BEHAV_GRAPH 2
 #define CDSP_BEHAV_DEFAULT 0
 
 /* unit sharing and task control (cellspe-tasklib.h:122-133) */
@@ -110,6 +118,8 @@ unsigned int celldsp_getBehaviour(void);
 /* introspection used by tests: number of free units, started client count */
 int celldsp_freeCount(void);
 int celldsp_clientCount(void);
+/* chains executed from a captured graph since celldsp_start (CDSP_BEHAV_GRAPH) */
+long celldsp_graphReplays(void);
 
 #ifdef __cplusplus
 } /* extern "C" */

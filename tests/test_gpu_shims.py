@@ -154,6 +154,83 @@ def test_difx_process(rt, upper):
     assert np.array_equal(acc[: nch + 1].imag, acc0[: nch + 1].imag.astype(np.float32))
 
 
+def _graph_chain():
+    """FFT -> MULC -> MAC, an UNPACK, and a DIFX task fed by the FFT output
+    through its host vector: five tasks, data passed between them on the host."""
+    n, nch = 1024, 256
+    rng = np.random.default_rng(31)
+    v = {
+        "x": np.zeros(n, np.complex64), "X": np.zeros(n, np.complex64),
+        "m": np.exp(1j * rng.uniform(0, 2 * np.pi, n)).astype(np.complex64),
+        "Y": np.zeros(n, np.complex64), "acc": np.zeros(n, np.complex64),
+        "words": np.zeros(512, np.uint32), "f": np.zeros(4096 * 2, np.float32),
+        "mult": np.exp(1j * rng.uniform(0, 2 * np.pi, nch + 1)).astype(np.complex64),
+        "fo": np.zeros(nch + 1, np.complex64), "cfo": np.zeros(nch + 1, np.complex64),
+        "acc2": np.zeros(nch + 1, np.complex64),
+    }
+    t1 = TL.CellDSPTaskClass(TL.CELLDSP_TASK_FFT_EXEC_DYN)
+    t1.setFFTparams(n)
+    t1.addInVec(v["x"]); t1.addOutVec(v["X"])
+    t2 = TL.CellDSPTaskClass(TL.CELLDSP_TASK_MULC)
+    t2.addInVec(v["X"]); t2.addInVec(v["m"]); t2.addOutVec(v["Y"])
+    t3 = TL.CellDSPTaskClass(TL.CELLDSP_TASK_MAC)
+    t3.addInVec(v["Y"]); t3.addInVec(v["m"]); t3.addOutVec(v["acc"])
+    t4 = TL.CellDSPTaskClass(TL.CELLDSP_TASK_UNPACK)
+    t4.task.subcontext.Unpack.nchan = 2
+    t4.addInVec(v["words"]); t4.addOutVec(v["f"])
+    t5 = TL.CellDSPTaskClass(TL.CELLDSP_TASK_DIFXPROCESS)
+    d = t5.task.subcontext.Difx
+    d.nfft, d.numchannels, d.useUpperOrLower = n, nch, 1
+    t5.addInVec(v["X"]); t5.addInVec(v["mult"])
+    t5.addOutVec(v["fo"]); t5.addOutVec(v["cfo"]); t5.addOutVec(v["acc2"])
+    tasks = [t1, t2, t3, t4, t5]
+    for a, b in zip(tasks, tasks[1:]):
+        a.setNext(b)
+    return tasks, v
+
+
+def test_task_chain_graph_replays(rt):
+    """CDSP_BEHAV_GRAPH: the repeated chain runs from its captured graph and
+    every replay equals the eager execution bit for bit (inputs changed in
+    place between replays, eager runs and synchronizations in between)."""
+    tasks, v = _graph_chain()
+    rng = np.random.default_rng(32)
+    outs = ("X", "Y", "acc", "f", "fo", "cfo", "acc2")
+    r0 = rt.celldsp_graphReplays()
+    for rep in range(5):
+        x = (rng.standard_normal(1024) + 1j * rng.standard_normal(1024)).astype(np.complex64)
+        words = rng.integers(0, 2**32, 512, dtype=np.uint64).astype(np.uint32)
+        acc0 = (rng.standard_normal(1024) + 0j).astype(np.complex64)
+        acc20 = (rng.standard_normal(257) + 0j).astype(np.complex64)
+        got = {}
+        for behaviour in (TL.CDSP_BEHAV_GRAPH, TL.CDSP_BEHAV_DEFAULT):
+            v["x"][:] = x; v["words"][:] = words; v["acc"][:] = acc0; v["acc2"][:] = acc20
+            for k in ("X", "Y", "f", "fo", "cfo"):
+                v[k][:] = 0
+            rt.celldsp_setBehaviour(behaviour)
+            assert rt.celldsp_executeTask(tasks[0].getTask()) == 0
+            assert all(t.completed_flag == TL.CELLDSP_TASK_OK for t in tasks)
+            got[behaviour] = {k: v[k].copy() for k in outs}
+            if behaviour == TL.CDSP_BEHAV_GRAPH and rep > 0:
+                # replayed: the chain's device time is shared evenly
+                assert len({t.tick_count for t in tasks}) == 1
+        for k in outs:
+            assert np.array_equal(got[TL.CDSP_BEHAV_GRAPH][k].view(np.uint32),
+                                  got[TL.CDSP_BEHAV_DEFAULT][k].view(np.uint32)), (rep, k)
+        assert nrel(got[TL.CDSP_BEHAV_DEFAULT]["X"], O.fft_r2(x)) < TOL32
+    # the first graph-mode execution runs eagerly and sizes the buffers; the
+    # next four replay
+    assert rt.celldsp_graphReplays() - r0 == 4
+    # a changed chain (another multiplier vector) runs eagerly once, then replays
+    other = v["m"].copy()
+    tasks[1].task.inputvector_ptrs[1].p = other.ctypes.data
+    rt.celldsp_setBehaviour(TL.CDSP_BEHAV_GRAPH)
+    for k in range(2):
+        assert rt.celldsp_executeTask(tasks[0].getTask()) == 0
+    assert rt.celldsp_graphReplays() - r0 == 5
+    rt.celldsp_setBehaviour(TL.CDSP_BEHAV_DEFAULT)
+
+
 def test_task_chain_on_device(rt):
     io_out = np.zeros(4, np.float32)
     t1 = TL.CellDSPTaskClass(TL.CELLDSP_TASK_IOTEST)
