@@ -522,6 +522,70 @@ extern "C" int fx_spectrometer_ex(fx_ctx* ctx, const uint32_t* d_words, size_t n
     return FX_OK;
 }
 
+extern "C" int fx_spectrometer_xpol(fx_ctx* ctx, const uint32_t* d_words_a, int nchan_a, int chan_a,
+                                    const uint32_t* d_words_b, int nchan_b, int chan_b, size_t ntime, int n,
+                                    int overlap, int integrate_ffts, int window, int continuous, fx_cf32* d_out,
+                                    long* nspec_out)
+{
+    int rc = ctx_device(ctx);
+    if (rc) return rc;
+    FX_REQUIRE(d_words_a && d_words_b && d_out, FX_ERR_ARG, "fx_spectrometer_xpol: null argument");
+    FX_REQUIRE(valid_nchan(nchan_a) && valid_nchan(nchan_b) && chan_a >= 0 && chan_a < nchan_a && chan_b >= 0 &&
+                   chan_b < nchan_b,
+               FX_ERR_ARG, "fx_spectrometer_xpol: channels %d/%d, %d/%d", chan_a, nchan_a, chan_b, nchan_b);
+    FX_REQUIRE(is_pow2(n) && n >= 256 && n <= FX_MAX_N, FX_ERR_ARG, "fx_spectrometer_xpol: n=%d", n);
+    FX_REQUIRE(integrate_ffts >= 1 && integrate_ffts <= 65535, FX_ERR_ARG,
+               "fx_spectrometer_xpol: integrate_ffts=%d", integrate_ffts);
+    FX_REQUIRE(overlap >= 1 && n % overlap == 0, FX_ERR_ARG, "fx_spectrometer_xpol: overlap %d", overlap);
+    FX_REQUIRE(window >= FX_WIN_NONE && window <= FX_WIN_HANN, FX_ERR_ARG,
+               "fx_spectrometer_xpol: window %d", window);
+    const long hop = n / overlap;
+    const long K = integrate_ffts;
+    long nspec, spec_stride;
+    if (continuous) {
+        const long nfft = ntime >= (size_t)n ? (long)((ntime - (size_t)n) / (size_t)hop) + 1 : 0;
+        nspec = nfft / K;
+        spec_stride = K * hop;
+    } else {
+        spec_stride = (K - 1) * hop + n;
+        nspec = (long)(ntime / (size_t)spec_stride);
+    }
+    if (nspec_out) *nspec_out = nspec;
+    if (nspec == 0) return FX_OK;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    float* win = get_window(ctx, window, n);
+    FX_REQUIRE(win != nullptr, FX_ERR_NOMEM, "fx_spectrometer_xpol: window table");
+    // whole spectra per pass: real frames and their n/2+1-bin transforms of
+    // both streams in the context scratch, at most 1 GiB and 65535 frames
+    const int c1 = n / 2 + 1;
+    const size_t frame_bytes = 2 * ((size_t)n * sizeof(float) + (size_t)c1 * sizeof(float2));
+    const size_t budget = (size_t)1 << 30;
+    FX_REQUIRE((size_t)K * frame_bytes <= budget, FX_ERR_UNSUPPORTED,
+               "fx_spectrometer_xpol: %ld FFTs of %d points per spectrum exceed one pass", K, n);
+    long sb = std::min<long>(nspec, (long)(budget / ((size_t)K * frame_bytes)));
+    sb = std::max<long>(1, std::min<long>(sb, 65535 / K));
+    const size_t nf = (size_t)(sb * K);
+    rc = ensure_scratch(ctx, nf * frame_bytes);
+    if (rc) return rc;
+    float* fa = static_cast<float*>(ctx->scratch);
+    float* fb = fa + nf * (size_t)n;
+    float2* xa = reinterpret_cast<float2*>(fb + nf * (size_t)n);
+    float2* xb = xa + nf * (size_t)c1;
+    for (long s0 = 0; s0 < nspec; s0 += sb) {
+        const long ns = std::min<long>(sb, nspec - s0);
+        const long frames = ns * K;
+        FX_CHECK_LAUNCH(fxk_spec_frames(ctx->stream, d_words_a, nchan_a, chan_a, hop, spec_stride, (int)K, n,
+                                        s0 * K, frames, win, fa));
+        FX_CHECK_LAUNCH(fxk_spec_frames(ctx->stream, d_words_b, nchan_b, chan_b, hop, spec_stride, (int)K, n,
+                                        s0 * K, frames, win, fb));
+        FX_CHECK_LAUNCH(fxk_fft_r2c(ctx->stream, ctx->tw, fa, xa, n, frames, 1));
+        FX_CHECK_LAUNCH(fxk_fft_r2c(ctx->stream, ctx->tw, fb, xb, n, frames, 1));
+        FX_CHECK_LAUNCH(fxk_xpol_acc(ctx->stream, xa, xb, c1, (int)K, ns, 1.0f / (float)K,
+                                     reinterpret_cast<float2*>(d_out) + (size_t)s0 * c1));
+    }
+    return FX_OK;
+}
+
 // ---------------------------------------------------------------------------
 // phase-cal
 // ---------------------------------------------------------------------------

@@ -11,6 +11,7 @@
 #include <stddef.h>
 #include <algorithm>
 #include <type_traits>
+#include <vector>
 
 #include "cellspe_fx.h"
 
@@ -403,6 +404,16 @@ int fxk_fill32(hipStream_t s, void* p, uint32_t value, size_t n);
 // out rows by (second, frame number, thread); valid[(row*nslots + slot)*2]
 // gets the valid 32-bit word range of the slot's payload; counts: placed,
 // placed but blank, slot out of range, unmapped thread
+// windowed 2-bit FFT inputs of one channel, frames g0..g0+nframes-1 (see k_spec_frames)
+int fxk_spec_frames(hipStream_t s, const uint32_t* words, int nchan, int ch, long hop, long spec_stride, int K,
+                    int n, long g0, long nframes, const float* win, float* out);
+// per spectrum and bin: scale * sum over K FFTs of A conj(B) (DC, Nyquist: re*re)
+int fxk_xpol_acc(hipStream_t s, const float2* A, const float2* B, int c1, int K, long nspec, float scale,
+                 float2* out);
+// phase-cal of nspec TaskCore spectra of L samples at d_words (1 channel),
+// spectrum indices counted from spec0; fold: grown device scratch
+int fx_spec_pcal(fx_ctx* ctx, const uint32_t* d_words, long nspec, long L, long spec0, int N, int ov, int K, int R,
+                 int T, double dphi, void*& fold, size_t& fold_cap, std::vector<fx_cf64>& out);
 int fxk_frames_place(hipStream_t s, int format, const uint8_t* raw, long nframes, int framebytes,
                      int payload_offset, const int* thread_row, int map_size, int nrows, long sec0, long frame0,
                      int fps,

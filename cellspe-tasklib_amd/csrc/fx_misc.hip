@@ -626,3 +626,71 @@ int fxk_iotest(hipStream_t s, float* out)
     hipLaunchKernelGGL(k_iotest, dim3(1), dim3(64), 0, s, out);
     return last_status();
 }
+
+// ---------------------------------------------------------------------------
+// spectrometer cross spectra (TaskCoreIPP.cpp:456-487)
+// ---------------------------------------------------------------------------
+namespace {
+
+// Windowed real FFT inputs of one channel of a packed 2-bit stream: frame
+// g0 + blockIdx.y = (spectrum g / K, FFT g % K) starts at sample
+// (g / K) * spec_stride + (g % K) * hop.
+__global__ __launch_bounds__(256) void k_spec_frames(const uint32_t* __restrict__ words, int nchan, int ch,
+                                                     long hop, long spec_stride, int K, int n, long g0,
+                                                     const float* __restrict__ win, float* __restrict__ out)
+{
+    const long g = g0 + blockIdx.y;
+    const long t0 = (g / K) * spec_stride + (g % K) * hop;
+    float* o = out + (size_t)blockIdx.y * (size_t)n;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const size_t field = (size_t)(t0 + i) * (size_t)nchan + (size_t)ch;
+        const uint32_t code = (words[field >> 4] >> (2u * (uint32_t)(field & 15u))) & 3u;
+        o[i] = lut2(code) * win[i];
+    }
+}
+
+// Mean over K FFTs of A conj(B) per bin of n/2+1; DC and Nyquist are real
+// products of the real parts (the IPP core keeps them apart).
+__global__ __launch_bounds__(256) void k_xpol_acc(const float2* __restrict__ A, const float2* __restrict__ B,
+                                                  int c1, int K, long nspec, float scale, float2* __restrict__ out)
+{
+    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= nspec * (long)c1) return;
+    const long sp = idx / c1;
+    const int k = (int)(idx % c1);
+    const bool edge = k == 0 || k == c1 - 1;
+    float re = 0.f, im = 0.f;
+    for (int j = 0; j < K; j++) {
+        const size_t at = ((size_t)(sp * K + j)) * (size_t)c1 + (size_t)k;
+        const float2 a = A[at], b = B[at];
+        if (edge) {
+            re += a.x * b.x;
+        } else {
+            re += a.x * b.x + a.y * b.y;
+            im += a.y * b.x - a.x * b.y;
+        }
+    }
+    out[idx] = make_float2(re * scale, im * scale);
+}
+
+} // namespace
+
+int fxk_spec_frames(hipStream_t s, const uint32_t* words, int nchan, int ch, long hop, long spec_stride, int K,
+                    int n, long g0, long nframes, const float* win, float* out)
+{
+    if (nframes <= 0) return FX_OK;
+    const int bx = std::min(8, (n + 255) / 256);
+    hipLaunchKernelGGL(k_spec_frames, dim3((unsigned)bx, (unsigned)nframes), dim3(256), 0, s, words, nchan, ch,
+                       hop, spec_stride, K, n, g0, win, out);
+    return last_status();
+}
+
+int fxk_xpol_acc(hipStream_t s, const float2* A, const float2* B, int c1, int K, long nspec, float scale,
+                 float2* out)
+{
+    const long total = nspec * (long)c1;
+    if (total <= 0) return FX_OK;
+    hipLaunchKernelGGL(k_xpol_acc, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, A, B, c1, K, nspec,
+                       scale, out);
+    return last_status();
+}

@@ -99,6 +99,64 @@ extern "C" int spectrometer_cb_run(const control_block_t* cb, spectrometer_mbox_
 }
 
 // ---------------------------------------------------------------------------
+// phase-cal of the IPP core, on device-resident words
+// ---------------------------------------------------------------------------
+// Phase-cal of every spectrum: the FFT inputs whose index (counted from
+// spectrum spec0 * K) is a multiple of the overlap factor (TaskCoreIPP.cpp:
+// 413-418) are contiguous within a spectrum, so each spectrum is one span of
+// whole FFTs.  The span is folded exactly on the device into rotatorlen FP64
+// bins (every 2-bit value is a multiple of 2^-22, so the sums are exact and
+// order-free); the rotator and the fold into tonebins (TaskCoreIPP.cpp:
+// 633-693) are then applied in FP64 on rotatorlen values per spectrum.
+int fx_spec_pcal(fx_ctx* ctx, const uint32_t* d_words, long nspec, long L, long spec0, int N, int ov, int K, int R,
+                 int T, double dphi, void*& fold, size_t& fold_cap, std::vector<fx_cf64>& out)
+{
+    const long hop = N / ov;
+    const size_t bytes = (size_t)nspec * (size_t)R * sizeof(double);
+    if (fold_cap < bytes) {
+        if (fold) fx_free(ctx, fold);
+        fold = nullptr;
+        fold_cap = 0;
+        if (fx_malloc(ctx, bytes, &fold) != FX_OK) return -1;
+        fold_cap = bytes;
+    }
+    if (fx_memset(ctx, fold, 0, bytes)) return -1;
+    hipStream_t st = fx_ctx_hip_stream(ctx);
+    for (long sp = 0; sp < nspec; sp++) {
+        const long g0 = (spec0 + sp) * (long)K;      // index of the spectrum's first FFT
+        const long j0 = (ov - g0 % ov) % ov;         // first selected FFT of this spectrum
+        if (j0 >= K) continue;
+        const long cnt = (K - 1 - j0) / ov + 1;
+        const size_t start = (size_t)(sp * L + j0 * hop);
+        if (fxk_pcal_fold(st, d_words + start / 16, (size_t)(cnt * N), 0, R, (double*)fold + (size_t)sp * R) !=
+            FX_OK) {
+            fx_set_error("phase-cal fold launch failed");
+            return -1;
+        }
+    }
+    std::vector<double> h((size_t)nspec * R);
+    if (fx_memcpy_d2h(ctx, h.data(), fold, bytes)) return -1;
+    std::vector<double> rc((size_t)R), rs((size_t)R);
+    for (int r = 0; r < R; r++) {
+        rc[(size_t)r] = cos(dphi * (double)r);
+        rs[(size_t)r] = sin(dphi * (double)r);
+    }
+    const int pulses = R / T;
+    out.assign((size_t)nspec * T, fx_cf64{0.0, 0.0});
+    for (long sp = 0; sp < nspec; sp++) {
+        const double* f = &h[(size_t)sp * R];
+        fx_cf64* o = &out[(size_t)sp * T];
+        for (int p = 0; p < pulses; p++)
+            for (int b = 0; b < T; b++) {
+                const int r = p * T + b;
+                o[b].re += f[r] * rc[(size_t)r];
+                o[b].im += f[r] * rs[(size_t)r];
+            }
+    }
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
 // TaskCore
 // ---------------------------------------------------------------------------
 struct fx_taskcore {
@@ -135,52 +193,14 @@ int ensure(fx_ctx* ctx, void*& p, size_t& cap, size_t bytes)
     return rc;
 }
 
-// Phase-cal of every spectrum: the FFT inputs whose run-global index is a
-// multiple of the overlap factor (TaskCoreIPP.cpp:413-418) are contiguous
-// within a spectrum, so each spectrum is one span of whole FFTs.  The span is
-// folded exactly on the device into rotatorlen FP64 bins (every 2-bit value
-// is a multiple of 2^-22, so the sums are exact and order-free); the rotator
-// and the fold into tonebins (TaskCoreIPP.cpp:633-693) are then applied in
-// FP64 on rotatorlen values per spectrum.
 int core_pcal(fx_taskcore* c, long nspec, long L, fx_cf64* pcal_out)
 {
     const fx_spectrometer_settings& s = c->s;
-    const int N = s.fft_points, ov = s.fft_overlap_factor, K = s.core_overlapped_ffts;
-    const long hop = N / ov;
-    const int R = c->rotatorlen, T = c->tonebins;
-    const size_t bytes = (size_t)nspec * (size_t)R * sizeof(double);
-    if (ensure(c->ctx, c->fold, c->fold_cap, bytes) || fx_memset(c->ctx, c->fold, 0, bytes)) return -1;
-    hipStream_t st = fx_ctx_hip_stream(c->ctx);
-    for (long sp = 0; sp < nspec; sp++) {
-        const long g0 = sp * (long)K;                // run-global index of the first FFT
-        const long j0 = (ov - g0 % ov) % ov;         // first selected FFT of this spectrum
-        if (j0 >= K) continue;
-        const long cnt = (K - 1 - j0) / ov + 1;
-        const size_t start = (size_t)(sp * L + j0 * hop);
-        const uint32_t* w = (const uint32_t*)c->words + start / 16;
-        if (fxk_pcal_fold(st, w, (size_t)(cnt * N), 0, R, (double*)c->fold + (size_t)sp * R) != FX_OK) {
-            c->err = "phase-cal fold launch failed";
-            return -1;
-        }
-    }
-    std::vector<double> h((size_t)nspec * R);
-    if (fx_memcpy_d2h(c->ctx, h.data(), c->fold, bytes)) return -1;
-    std::vector<double> rc((size_t)R), rs((size_t)R);
-    for (int r = 0; r < R; r++) {
-        rc[(size_t)r] = cos(c->dphi * (double)r);
-        rs[(size_t)r] = sin(c->dphi * (double)r);
-    }
-    const int pulses = R / T;
-    c->last_pcal.assign((size_t)nspec * T, fx_cf64{0.0, 0.0});
-    for (long sp = 0; sp < nspec; sp++) {
-        const double* f = &h[(size_t)sp * R];
-        fx_cf64* o = &c->last_pcal[(size_t)sp * T];
-        for (int p = 0; p < pulses; p++)
-            for (int b = 0; b < T; b++) {
-                const int r = p * T + b;
-                o[b].re += f[r] * rc[(size_t)r];
-                o[b].im += f[r] * rs[(size_t)r];
-            }
+    if (fx_spec_pcal(c->ctx, (const uint32_t*)c->words, nspec, L, 0, s.fft_points, s.fft_overlap_factor,
+                     s.core_overlapped_ffts, c->rotatorlen, c->tonebins, c->dphi, c->fold, c->fold_cap,
+                     c->last_pcal)) {
+        c->err = "phase-cal fold failed";
+        return -1;
     }
     memcpy(pcal_out, c->last_pcal.data(), c->last_pcal.size() * sizeof(fx_cf64));
     return 0;

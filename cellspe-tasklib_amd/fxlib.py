@@ -36,7 +36,7 @@ class CorrConfig(C.Structure):
                 ("prec", C.c_int), ("use_phase", C.c_int)]
 
 
-FX_FMT_VDIF, FX_FMT_MARK5B = 0, 1
+FX_FMT_VDIF, FX_FMT_MARK5B, FX_FMT_RAW = 0, 1, 2
 FX_PLACE_CLEAR, FX_PLACE_EXACT_BLANK = 1, 2
 
 
@@ -101,6 +101,7 @@ def load():
         "fx_ingest_open": (i, [vp, C.c_char_p, sz, sz, i, C.POINTER(vp)]),
         "fx_ingest_next": (i, [vp, C.POINTER(vp), C.POINTER(sz)]),
         "fx_ingest_close": (i, [vp]),
+        "fx_spectrometer_xpol": (i, [vp, vp, i, i, vp, i, i, sz, i, i, i, i, i, vp, C.POINTER(l)]),
         "fx_lags": (i, [vp, vp, l, i, C.c_float, vp]),
         "fx_amp_phase": (i, [vp, vp, l, C.c_float, vp, vp]),
         "fx_pcal_delay": (i, [vp, vp, i, i, d, d, d, d, vp]),
@@ -327,6 +328,15 @@ class Context:
                                           integrate_ffts, window, int(continuous),
                                           int(keep_nyquist), d_out.ptr if d_out else None,
                                           C.byref(ns)))
+        return ns.value
+
+    def spectrometer_xpol(self, d_a, nchan_a, chan_a, d_b, nchan_b, chan_b, ntime, n, overlap, integrate_ffts,
+                          window, continuous, d_out):
+        """Cross spectra of two 2-bit channels; returns the spectrum count."""
+        ns = C.c_long()
+        check(self.lib.fx_spectrometer_xpol(self.handle, d_a.ptr, nchan_a, chan_a, d_b.ptr, nchan_b, chan_b, ntime,
+                                            n, overlap, integrate_ffts, window, int(continuous), d_out.ptr,
+                                            C.byref(ns)))
         return ns.value
 
     def complex_mac(self, a, b, acc, n, conj):

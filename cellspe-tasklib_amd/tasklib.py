@@ -149,6 +149,26 @@ class SpectrometerSettings(C.Structure):
     ]
 
 
+class SwspecSettings(C.Structure):
+    """fx_swspec_settings: one run of the spectrometer application."""
+    _fields_ = [
+        ("core", SpectrometerSettings),
+        ("format", C.c_int),
+        ("framebytes", C.c_int),
+        ("headerbytes", C.c_int),
+        ("frames_per_second", C.c_int),
+        ("use_channel_file", C.c_int * 2),
+        ("calc_xpol", C.c_int),
+        ("spectra_per_chunk", C.c_int),
+        ("basefilename", C.c_char_p * 2),
+    ]
+
+
+class SwspecStats(C.Structure):
+    _fields_ = [("spectra", C.c_long), ("chunks", C.c_long), ("seconds", C.c_double),
+                ("realtime_factor", C.c_double)]
+
+
 _lib = None
 
 
@@ -198,6 +218,7 @@ def load():
         "fx_taskcore_pcal_length": (i, [vp]),
         "fx_taskcore_finalize": (i, [vp]),
         "fx_taskcore_destroy": (i, [vp]),
+        "fx_swspectrometer_run": (i, [C.POINTER(SwspecSettings), C.c_char_p, C.c_char_p, C.POINTER(SwspecStats)]),
     }
     for name, (res, args) in sigs.items():
         fn = getattr(lib, name)

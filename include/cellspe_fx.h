@@ -57,7 +57,8 @@ enum {
 /* recorded-data frame formats of the front end */
 enum {
     FX_FMT_VDIF = 0,   /* VDIF: 32-byte header (16 legacy), payload of 8-byte multiples */
-    FX_FMT_MARK5B = 1  /* Mark5B: 16-byte header, 10000-byte payload, 10016-byte frames */
+    FX_FMT_MARK5B = 1, /* Mark5B: 16-byte header, 10000-byte payload, 10016-byte frames */
+    FX_FMT_RAW = 2     /* headerless packed 2-bit samples (this library's layout) */
 };
 
 /* spectrometer windows: SIN2 sin^2(pi i/(n-1)) (spectrometer_spu.c:297-315);
@@ -147,6 +148,18 @@ int fx_spectrometer(fx_ctx* ctx, const uint32_t* d_words, size_t ntime, int ncha
 int fx_spectrometer_ex(fx_ctx* ctx, const uint32_t* d_words, size_t ntime, int nchan, int n,
                        int overlap, int integrate_ffts, int window, int continuous, int keep_nyquist,
                        float* d_out, long* nspec_out);
+/* Cross spectrum of channel chan_a of stream a and channel chan_b of stream b
+ * (TaskCoreIPP.cpp:456-487, the cross-polarization product of two sources):
+ * the same windowed, overlapped FFT frames and spectrum boundaries as
+ * fx_spectrometer_ex (ntime samples per channel in both streams), per
+ * spectrum n/2+1 complex bins of (1/integrate_ffts) sum X_a conj(X_b) with
+ * FP32 accumulation in FFT order; DC and Nyquist hold the product of the two
+ * real parts with a zero imaginary part, as the IPP core keeps them.  d_out:
+ * nspec x (n/2+1) fx_cf32.  One pass holds at most 1 GiB of frames and
+ * transforms of whole spectra (integrate_ffts <= 65535). */
+int fx_spectrometer_xpol(fx_ctx* ctx, const uint32_t* d_words_a, int nchan_a, int chan_a,
+                         const uint32_t* d_words_b, int nchan_b, int chan_b, size_t ntime, int n, int overlap,
+                         int integrate_ffts, int window, int continuous, fx_cf32* d_out, long* nspec_out);
 
 /* ---- a5: phase-cal ------------------------------------------------------- */
 typedef struct fx_pcal fx_pcal;

@@ -113,6 +113,42 @@ int fx_taskcore_pcal_length(fx_taskcore* core);
 int fx_taskcore_finalize(fx_taskcore* core);
 int fx_taskcore_destroy(fx_taskcore* core);
 
+/* ---- the spectrometer application (TaskDispatcher + sinks) --------------- */
+/* Settings of one run (swspect_settings_t, Settings.h:59-159, and the INI
+ * keys UseFile1Channel/UseFile2Channel, swspectrometer.cpp:140-160). */
+typedef struct {
+    fx_spectrometer_settings core;  /* use_channel is ignored: see use_channel_file */
+    int format;                     /* FX_FMT_RAW or FX_FMT_VDIF (one thread per file, frames in order) */
+    int framebytes, headerbytes;    /* VDIF frame and header sizes */
+    int frames_per_second;          /* VDIF frames per second of the thread */
+    int use_channel_file[2];        /* 1-based channel of input 1 and input 2 */
+    int calc_xpol;                  /* cross spectrum of the two selected channels */
+    int spectra_per_chunk;          /* whole spectra per double-buffered input chunk */
+    const char* basefilename[2];    /* sink names: <base>_swspec.bin, <base>_pcal.bin,
+                                       <base 1>_xpol_swspec.bin; [1] NULL: [0] */
+} fx_swspec_settings;
+
+typedef struct {
+    long spectra;                   /* spectra written per sink */
+    long chunks;                    /* input chunks read */
+    double seconds;                 /* wall time of the processing loop */
+    double realtime_factor;         /* spectra x integrate_ffts x fft_points / fs / seconds */
+} fx_swspec_stats;
+
+/* The TaskDispatcher loop (TaskDispatcher.cpp:38-230) over one or two input
+ * files (input2 may be NULL): the files are streamed into HBM through the
+ * double-buffered ingest (fx_ingest_*), VDIF frames placed by
+ * fx_frames_place, and every whole spectrum -- TaskCore spectra of
+ * (integrate_ffts - 1) * fft_points/overlap + fft_points samples, counted
+ * from the start of the file and independent of the chunking -- written to
+ * the binary sinks: fft_points/2+1 float32 per spectrum of each input's
+ * selected channel, the cross spectrum of the two (fft_points/2+1 complex
+ * float32, fx_spectrometer_xpol) and the phase-cal (tonebins complex
+ * float32).  The run ends when either input ends.  Returns 0 or an FX_ERR_*
+ * code (fx_last_error()). */
+int fx_swspectrometer_run(const fx_swspec_settings* settings, const char* input1, const char* input2,
+                          fx_swspec_stats* stats);
+
 #ifdef __cplusplus
 }
 #endif

@@ -413,3 +413,32 @@ def pcal_delay(tones, tone0_hz, tone_step_hz, band_center_hz, band_valid_hz):
     while delay < -window:
         delay += 2.0 * window
     return delay
+
+
+def spectrometer_xpol_f64(words_a, nchan_a, chan_a, words_b, nchan_b, chan_b, ntime, n, overlap,
+                          integrate_ffts, window_kind, continuous=False):
+    """Cross spectrum of two 2-bit channels (TaskCoreIPP.cpp:456-487) in FP64:
+    the same windowed, overlapped frames as the spectrometer; per spectrum
+    (1/K) sum X_a conj(X_b) over bins 0..n/2, with DC and Nyquist the product
+    of the real parts (imaginary part 0).  Returns [nspec][n/2+1] complex128."""
+    hop = n // overlap
+    K = integrate_ffts
+    if continuous:
+        nfft = (ntime - n) // hop + 1 if ntime >= n else 0
+        nspec, stride = nfft // K, K * hop
+    else:
+        stride = (K - 1) * hop + n
+        nspec = ntime // stride
+    xa = unpack_nch(words_a, ntime, nchan_a)[chan_a].astype(np.float64)
+    xb = unpack_nch(words_b, ntime, nchan_b)[chan_b].astype(np.float64)
+    w = window(window_kind, n).astype(np.float64)
+    out = np.zeros((nspec, n // 2 + 1), dtype=np.complex128)
+    for s in range(nspec):
+        t0 = s * stride + np.arange(K)[:, None] * hop + np.arange(n)[None, :]
+        A = np.fft.rfft(xa[t0] * w, axis=1)
+        B = np.fft.rfft(xb[t0] * w, axis=1)
+        acc = (A * np.conj(B)).sum(axis=0)
+        acc[0] = (A[:, 0].real * B[:, 0].real).sum()
+        acc[-1] = (A[:, -1].real * B[:, -1].real).sum()
+        out[s] = acc / K
+    return out

@@ -101,6 +101,11 @@ int main(void) {
   S(fx_spectrometer_settings) P(fx_spectrometer_settings, samplingfreq)
   P(fx_spectrometer_settings, pcalharmonicshz)
   S(fx_corr_config) P(fx_corr_config, prec) S(fx_phase_model)
+  S(fx_swspec_settings) P(fx_swspec_settings, format) P(fx_swspec_settings, use_channel_file)
+  P(fx_swspec_settings, spectra_per_chunk) P(fx_swspec_settings, basefilename)
+  S(fx_swspec_stats) P(fx_swspec_stats, realtime_factor)
+  S(fx_vdif_header) P(fx_vdif_header, nchan) P(fx_vdif_header, second_of_day)
+  S(fx_mark5b_header) P(fx_mark5b_header, frac_ns)
   return 0;
 }
 """
@@ -117,7 +122,9 @@ def test_ctypes_layouts_match_c(tmp_path):
     py = {
         "CellDSPTask_t": TL.CellDSPTask, "control_block": TL.MiniControlBlock,
         "control_block_t": TL.SpecControlBlock, "fx_spectrometer_settings": TL.SpectrometerSettings,
-        "fx_corr_config": fxlib.CorrConfig,
+        "fx_corr_config": fxlib.CorrConfig, "fx_swspec_settings": TL.SwspecSettings,
+        "fx_swspec_stats": TL.SwspecStats, "fx_vdif_header": fxlib.VdifHeader,
+        "fx_mark5b_header": fxlib.Mark5bHeader,
     }
     for key, val in c.items():
         if "." in key:
@@ -133,6 +140,8 @@ def test_ctypes_layouts_match_c(tmp_path):
     assert C.sizeof(TL.SpecMbox) == c["spectrometer_mbox_t"]
     assert C.sizeof(TL.MiniStats) == c["minicorr_stats"]
     assert C.sizeof(fxlib.PhaseModel) == c["fx_phase_model"]
+    for name in ("fx_swspec_settings", "fx_swspec_stats", "fx_vdif_header", "fx_mark5b_header"):
+        assert C.sizeof(py[name]) == c[name], name
 
 
 # ---------------------------------------------------------------- validation

@@ -443,3 +443,31 @@ def test_pcal_delay_wrap_band_and_degenerate():
     # fewer than two tones, or no pair inside the band
     assert O.pcal_delay(z[:1], f0, df, bc, bv) == 0.0
     assert O.pcal_delay(z, f0, df, bc, 0.1e6) == 0.0
+
+
+@pytest.mark.parametrize("window", [O.WIN_NONE, O.WIN_HANN])
+def test_xpol_oracle_pins(window):
+    """Cross-spectrum oracle: X conj(X) of a channel with itself is the
+    pinned auto spectrum (real), swapping the streams conjugates, and a copy
+    delayed by d samples gives the phase slope +2 pi k d / n (the product is
+    A conj(B), not conj(A) B)."""
+    rng = np.random.default_rng(77)
+    n, ov, K, ntime = 1024, 2, 6, 1024 * 40
+    codes = rng.integers(0, 4, 2 * ntime).astype(np.uint8)
+    wa = O.pack2b(codes)                                  # 2 channels interleaved
+    auto = O.spectrometer_ex_f32(wa, ntime, 2, n, ov, K, window, False, True)
+    x00 = O.spectrometer_xpol_f64(wa, 2, 0, wa, 2, 0, ntime, n, ov, K, window)
+    assert np.max(np.abs(x00.imag)) <= 1e-12 * np.max(x00.real)
+    assert np.max(np.abs(x00.real - auto[:, 0, :])) / np.max(auto[:, 0, :]) < 1e-5
+    x01 = O.spectrometer_xpol_f64(wa, 2, 0, wa, 2, 1, ntime, n, ov, K, window)
+    x10 = O.spectrometer_xpol_f64(wa, 2, 1, wa, 2, 0, ntime, n, ov, K, window)
+    assert np.max(np.abs(x01 - np.conj(x10))) <= 1e-12 * np.max(np.abs(x01))
+    # delayed copy: channel 0 of b is channel 0 of a late by d samples
+    d = 3
+    c0 = codes.reshape(ntime, 2)[:, 0]
+    cb = np.concatenate([rng.integers(0, 4, d).astype(np.uint8), c0[:-d]])
+    wb = O.pack2b(cb)
+    xd = O.spectrometer_xpol_f64(wa, 2, 0, wb, 1, 0, ntime, n, ov, K, window)
+    k = np.arange(20, n // 2 - 20)
+    err = np.angle(xd[:, k].sum(axis=0) * np.exp(-2j * np.pi * k * d / n))
+    assert np.median(np.abs(err)) < 0.05
