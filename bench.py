@@ -17,7 +17,8 @@ same step beside it.
 Further sections: the visibility products (lag spectra, amplitude/phase),
 the VDIF front end (frame placement of ~1 GB of frames in HBM, and the
 double-buffered file ingest), a task-runtime chain per task vs as one
-hipGraph, and the batch of uneven subbands.
+hipGraph, the spectrometer application over two files, and the batch of
+uneven subbands.
 
 A second section times the large-N X-engine (256 stations x 2 pol = 512
 inputs, 1024 channels, 8192 spectra of 2-bit channelized voltages): the exact
@@ -311,6 +312,57 @@ def front_end(fxlib, ctx, args):
     return out
 
 
+def spectrometer_app(fxlib, args):
+    """Spectrometer application: two 256 MiB raw files of single-channel
+    2-bit samples (2^30 samples each, 16 MHz sampling: 67 s of data) through
+    fx_swspectrometer_run -- 1024-point FFTs with 50% overlap (the reference's
+    spectrometer benchmark shape), Hann window, 1 s integrations (31249 FFTs
+    per spectrum), auto spectra of both inputs and their cross spectrum to the
+    binary sinks.  FFT/s counts the overlapped FFTs of one input."""
+    import tempfile
+    import tasklib as TL
+    n, ov, fs = 1024, 2, 16e6
+    K = int(fs / (n // ov)) - 1                 # (K-1)*hop + n samples = 1 s
+    ntime = 1 << 30
+    rng = np.random.default_rng(args.seed + 7)
+    tdir = os.environ.get("TMPDIR", "/tmp")
+    out = {"workload": f"2 raw files x {ntime} single-channel 2-bit samples ({ntime / fs:.0f} s at 16 MHz), "
+                       f"{n}-pt FFT, 50% overlap, Hann, {K} FFTs per 1 s spectrum, autos + cross spectrum"}
+    with tempfile.TemporaryDirectory(dir=tdir) as d:
+        block = rng.integers(0, 2**32, 1 << 22, dtype=np.uint64).astype(np.uint32)   # 16 MiB
+        paths = []
+        for i in range(2):
+            p = os.path.join(d, f"in{i}.raw")
+            with open(p, "wb") as fh:
+                for k in range(ntime // 16 // block.size):
+                    np.roll(block, 4099 * (k + 17 * i)).tofile(fh)
+            paths.append(p)
+        for p in paths:                            # page cache
+            with open(p, "rb") as fh:
+                while fh.read(64 << 20):
+                    pass
+        s = TL.SwspecSettings()
+        c = s.core
+        c.fft_points, c.fft_overlap_factor, c.core_overlapped_ffts = n, ov, K
+        c.window, c.bits_per_sample, c.source_channels, c.samplingfreq = fxlib.FX_WIN_HANN, 2, 1, fs
+        s.format = fxlib.FX_FMT_RAW
+        s.use_channel_file[0] = s.use_channel_file[1] = 1
+        s.calc_xpol, s.spectra_per_chunk = 1, 8
+        s.basefilename[0] = os.path.join(d, "out").encode()
+        st = TL.SwspecStats()
+        rc = TL.load().fx_swspectrometer_run(s, paths[0].encode(), paths[1].encode(), st)
+        if rc != 0:
+            raise SystemExit(f"bench: spectrometer application failed: {fxlib.load().fx_last_error()}")
+    nfft = st.spectra * K
+    out.update({"spectra": st.spectra, "seconds": st.seconds, "ffts_per_s_per_input": nfft / st.seconds,
+                "realtime_factor": st.realtime_factor,
+                "note": "files in the page cache; host reads, PCIe copies, device compute and sink writes "
+                        "all inside the timed loop"})
+    log(f"[bench] spectrometer app: {st.spectra} spectra in {st.seconds:.2f} s "
+        f"({nfft / st.seconds:.3g} FFT/s per input, {st.realtime_factor:.1f}x realtime)")
+    return out
+
+
 def task_chains(args):
     """Task runtime: a five-task next_task chain (1024-point FFT -> complex
     multiply -> complex MAC, a 2-channel unpack of 4096 samples, a DiFX
@@ -478,7 +530,7 @@ def main():
                     help="instead of the benchmark, run the plain copy kernel over MIB MiB "
                          "(HBM bandwidth check and traffic-counter calibration)")
     ap.add_argument("--front-end-only", action="store_true",
-                    help="instead of the benchmark, run only the VDIF front-end and task-chain sections")
+                    help="instead of the benchmark, run only the front-end, task-chain and spectrometer-application sections")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -494,8 +546,8 @@ def main():
         return
     if args.front_end_only:
         ctx = fxlib.Context(0)
-        print(json.dumps({"front_end": front_end(fxlib, ctx, args), "task_chains": task_chains(args)}),
-              flush=True)
+        print(json.dumps({"front_end": front_end(fxlib, ctx, args), "task_chains": task_chains(args),
+                          "spectrometer_app": spectrometer_app(fxlib, args)}), flush=True)
         ctx.close()
         return
 
@@ -629,6 +681,7 @@ def main():
     dvis.free()
     fe = None if args.no_front_end else front_end(fxlib, ctx, args)
     chains = None if args.no_tasks else task_chains(args)
+    app = None if args.no_front_end else spectrometer_app(fxlib, args)
 
     large = None
     if not args.no_large:
@@ -711,6 +764,7 @@ def main():
         "visibility_products": prod,
         "front_end": fe,
         "task_chains": chains,
+        "spectrometer_app": app,
         "roofline": roof,
         "cpu_baseline": cpu,
     }

@@ -296,10 +296,10 @@ extern "C" int fx_swspectrometer_run(const fx_swspec_settings* s, const char* in
         stats->spectra = total;
         stats->chunks = chunks;
         stats->seconds = secs;
-        // integrated time per spectrum over wall time (TaskDispatcher.cpp:183-192)
-        stats->realtime_factor = c.samplingfreq > 0 && secs > 0
-                                     ? (double)total * (double)K * (double)N / c.samplingfreq / secs
-                                     : 0.0;
+        // signal time of the spectra over wall time (TaskDispatcher.cpp:183-192):
+        // a spectrum spans L samples
+        stats->realtime_factor =
+            c.samplingfreq > 0 && secs > 0 ? (double)total * (double)L / c.samplingfreq / secs : 0.0;
     }
     if (rc == FX_ERR_STATE) fx_set_error("fx_swspectrometer_run: sink write failed");
     close_app(a);

@@ -132,7 +132,8 @@ typedef struct {
     long spectra;                   /* spectra written per sink */
     long chunks;                    /* input chunks read */
     double seconds;                 /* wall time of the processing loop */
-    double realtime_factor;         /* spectra x integrate_ffts x fft_points / fs / seconds */
+    double realtime_factor;         /* signal time of the spectra / seconds: spectra x
+                                       ((integrate_ffts-1) x fft_points/overlap + fft_points) / fs / seconds */
 } fx_swspec_stats;
 
 /* The TaskDispatcher loop (TaskDispatcher.cpp:38-230) over one or two input
