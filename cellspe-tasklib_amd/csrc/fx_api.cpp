@@ -565,12 +565,15 @@ extern "C" int fx_spectrometer_xpol(fx_ctx* ctx, const uint32_t* d_words_a, int 
     long sb = std::min<long>(nspec, (long)(budget / ((size_t)K * frame_bytes)));
     sb = std::max<long>(1, std::min<long>(sb, 65535 / K));
     const size_t nf = (size_t)(sb * K);
-    rc = ensure_scratch(ctx, nf * frame_bytes);
+    // partial sums: groups(ns) * ns * c1 <= 2^18 + ns * c1 for every pass of ns <= sb spectra
+    const size_t part_bytes = ((size_t)262144 + (size_t)sb * (size_t)c1) * sizeof(float2);
+    rc = ensure_scratch(ctx, nf * frame_bytes + part_bytes);
     if (rc) return rc;
     float* fa = static_cast<float*>(ctx->scratch);
     float* fb = fa + nf * (size_t)n;
     float2* xa = reinterpret_cast<float2*>(fb + nf * (size_t)n);
     float2* xb = xa + nf * (size_t)c1;
+    float2* part = xb + nf * (size_t)c1;
     for (long s0 = 0; s0 < nspec; s0 += sb) {
         const long ns = std::min<long>(sb, nspec - s0);
         const long frames = ns * K;
@@ -580,7 +583,8 @@ extern "C" int fx_spectrometer_xpol(fx_ctx* ctx, const uint32_t* d_words_a, int 
                                         s0 * K, frames, win, fb));
         FX_CHECK_LAUNCH(fxk_fft_r2c(ctx->stream, ctx->tw, fa, xa, n, frames, 1));
         FX_CHECK_LAUNCH(fxk_fft_r2c(ctx->stream, ctx->tw, fb, xb, n, frames, 1));
-        FX_CHECK_LAUNCH(fxk_xpol_acc(ctx->stream, xa, xb, c1, (int)K, ns, 1.0f / (float)K,
+        const int gr = fxk_xpol_groups(c1, (int)K, ns);
+        FX_CHECK_LAUNCH(fxk_xpol_acc(ctx->stream, xa, xb, c1, (int)K, ns, 1.0f / (float)K, part, gr,
                                      reinterpret_cast<float2*>(d_out) + (size_t)s0 * c1));
     }
     return FX_OK;

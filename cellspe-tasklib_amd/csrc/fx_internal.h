@@ -407,9 +407,11 @@ int fxk_fill32(hipStream_t s, void* p, uint32_t value, size_t n);
 // windowed 2-bit FFT inputs of one channel, frames g0..g0+nframes-1 (see k_spec_frames)
 int fxk_spec_frames(hipStream_t s, const uint32_t* words, int nchan, int ch, long hop, long spec_stride, int K,
                     int n, long g0, long nframes, const float* win, float* out);
-// per spectrum and bin: scale * sum over K FFTs of A conj(B) (DC, Nyquist: re*re)
+// per spectrum and bin: scale * sum over K FFTs of A conj(B) (DC, Nyquist: re*re),
+// the FFTs split over `groups` partial sums (part: groups * nspec * c1 float2)
+int fxk_xpol_groups(int c1, int K, long nspec);
 int fxk_xpol_acc(hipStream_t s, const float2* A, const float2* B, int c1, int K, long nspec, float scale,
-                 float2* out);
+                 float2* part, int groups, float2* out);
 // phase-cal of nspec TaskCore spectra of L samples at d_words (1 channel),
 // spectrum indices counted from spec0; fold: grown device scratch
 int fx_spec_pcal(fx_ctx* ctx, const uint32_t* d_words, long nspec, long L, long spec0, int N, int ov, int K, int R,

@@ -649,18 +649,22 @@ __global__ __launch_bounds__(256) void k_spec_frames(const uint32_t* __restrict_
     }
 }
 
-// Mean over K FFTs of A conj(B) per bin of n/2+1; DC and Nyquist are real
-// products of the real parts (the IPP core keeps them apart).
-__global__ __launch_bounds__(256) void k_xpol_acc(const float2* __restrict__ A, const float2* __restrict__ B,
-                                                  int c1, int K, long nspec, float scale, float2* __restrict__ out)
+// Partial sums of A conj(B) per bin of n/2+1 over the FFTs j0..j1-1 of each
+// spectrum, j0 = blockIdx.y * per (the FFTs of a spectrum are split over
+// gridDim.y groups so that long integrations still fill the chip); DC and
+// Nyquist are real products of the real parts (the IPP core keeps them
+// apart).  part[g][sp][k].
+__global__ __launch_bounds__(256) void k_xpol_part(const float2* __restrict__ A, const float2* __restrict__ B,
+                                                   int c1, int K, int per, long nspec, float2* __restrict__ part)
 {
     const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= nspec * (long)c1) return;
     const long sp = idx / c1;
     const int k = (int)(idx % c1);
     const bool edge = k == 0 || k == c1 - 1;
+    const int j0 = blockIdx.y * per, j1 = min(K, j0 + per);
     float re = 0.f, im = 0.f;
-    for (int j = 0; j < K; j++) {
+    for (int j = j0; j < j1; j++) {
         const size_t at = ((size_t)(sp * K + j)) * (size_t)c1 + (size_t)k;
         const float2 a = A[at], b = B[at];
         if (edge) {
@@ -669,6 +673,21 @@ __global__ __launch_bounds__(256) void k_xpol_acc(const float2* __restrict__ A, 
             re += a.x * b.x + a.y * b.y;
             im += a.y * b.x - a.x * b.y;
         }
+    }
+    part[(size_t)blockIdx.y * (size_t)(nspec * c1) + (size_t)idx] = make_float2(re, im);
+}
+
+// out = scale * sum over the groups, in group order (deterministic)
+__global__ __launch_bounds__(256) void k_xpol_sum(const float2* __restrict__ part, int groups, long total,
+                                                  float scale, float2* __restrict__ out)
+{
+    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= total) return;
+    float re = 0.f, im = 0.f;
+    for (int g = 0; g < groups; g++) {
+        const float2 v = part[(size_t)g * (size_t)total + (size_t)idx];
+        re += v.x;
+        im += v.y;
     }
     out[idx] = make_float2(re * scale, im * scale);
 }
@@ -685,12 +704,23 @@ int fxk_spec_frames(hipStream_t s, const uint32_t* words, int nchan, int ch, lon
     return last_status();
 }
 
+int fxk_xpol_groups(int c1, int K, long nspec)
+{
+    // about 2^18 threads of partial sums, at least 8 FFTs per group
+    const long want = (262144 + nspec * (long)c1 - 1) / (nspec * (long)c1);
+    return (int)std::max<long>(1, std::min<long>({want, (long)(K + 7) / 8, 65535}));
+}
+
 int fxk_xpol_acc(hipStream_t s, const float2* A, const float2* B, int c1, int K, long nspec, float scale,
-                 float2* out)
+                 float2* part, int groups, float2* out)
 {
     const long total = nspec * (long)c1;
     if (total <= 0) return FX_OK;
-    hipLaunchKernelGGL(k_xpol_acc, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, A, B, c1, K, nspec,
-                       scale, out);
+    const int per = (K + groups - 1) / groups;
+    hipLaunchKernelGGL(k_xpol_part, dim3((unsigned)((total + 255) / 256), (unsigned)groups), dim3(256), 0, s, A, B,
+                       c1, K, per, nspec, part);
+    if (hipGetLastError() != hipSuccess) return FX_ERR_HIP;
+    hipLaunchKernelGGL(k_xpol_sum, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, part, groups, total, scale,
+                       out);
     return last_status();
 }

@@ -153,7 +153,8 @@ int fx_spectrometer_ex(fx_ctx* ctx, const uint32_t* d_words, size_t ntime, int n
  * the same windowed, overlapped FFT frames and spectrum boundaries as
  * fx_spectrometer_ex (ntime samples per channel in both streams), per
  * spectrum n/2+1 complex bins of (1/integrate_ffts) sum X_a conj(X_b) with
- * FP32 accumulation in FFT order; DC and Nyquist hold the product of the two
+ * FP32 accumulation (FFT order within groups of FFTs, then the groups in
+ * order: deterministic); DC and Nyquist hold the product of the two
  * real parts with a zero imaginary part, as the IPP core keeps them.  d_out:
  * nspec x (n/2+1) fx_cf32.  One pass holds at most 1 GiB of frames and
  * transforms of whole spectra (integrate_ffts <= 65535). */
