@@ -278,6 +278,10 @@ def front_end(fxlib, ctx, args):
                         "frac": alg / (ms * 1e-3) / 1e9 / PEAK["hbm"],
                         "algorithmic": "per frame: framebytes read + payload written + 8 B range"},
            "parity_probe_ok": bool(ok)}
+    tr = measured_traffic("k_frames_place<uint4, 8>")
+    out["roofline"]["traffic"] = None if tr is None else tr["bytes"]
+    if tr is not None:
+        out["roofline"]["traffic_source"] = tr["source"]
     d_raw.free()
     # the same bytes from a file through the double-buffered ingest (half the
     # stream, so the temporary file stays small)
