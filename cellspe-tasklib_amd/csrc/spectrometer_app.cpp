@@ -2,8 +2,9 @@
 // (TaskDispatcher.cpp:38-230, swspectrometer.cpp:150-420) on one GPU.
 //
 // One or two input files are streamed into HBM through the double-buffered
-// ingest (the dispatcher's prefill and double buffering); VDIF frames are
-// validated and header-stripped on the device by the frame placement kernel;
+// ingest (the dispatcher's prefill and double buffering); VDIF and Mark5B
+// frames are validated and header-stripped on the device by the frame
+// placement kernel (Mark5B samples recoded to this library's 2-bit codes);
 // each input keeps a device stream buffer that carries the samples of an
 // unfinished spectrum over to the next chunk, so spectra never depend on the
 // chunk size.  Per chunk: auto spectra of each input's selected channel
@@ -108,19 +109,29 @@ int pull(App& a, Input& in, const fx_swspec_settings* s, size_t payload_bytes, i
     }
     const void* src = d;
     size_t add = n;
-    if (s->format == FX_FMT_VDIF) {
+    if (s->format != FX_FMT_RAW) {
         const int fb = s->framebytes, hb = s->headerbytes;
         const size_t nfr = n / (size_t)fb;
-        FX_REQUIRE(nfr > 0 && n % (size_t)fb == 0, FX_ERR_ARG, "fx_swspectrometer_run: partial VDIF frame");
+        FX_REQUIRE(nfr > 0 && n % (size_t)fb == 0, FX_ERR_ARG, "fx_swspectrometer_run: partial frame");
         // slots counted from the chunk's first frame (frames arrive in order)
         unsigned char hdr[16];
         if ((rc = fx_memcpy_d2h(a.ctx, hdr, d, 16))) return rc;
-        fx_vdif_header h;
-        fx_vdif_decode(hdr, &h);
+        long sec0, frame0;
+        if (s->format == FX_FMT_VDIF) {
+            fx_vdif_header h;
+            fx_vdif_decode(hdr, &h);
+            sec0 = (long)h.seconds;
+            frame0 = (long)h.frame;
+        } else {
+            fx_mark5b_header h;
+            fx_mark5b_decode(hdr, &h);
+            sec0 = (long)h.day * 86400L + (long)h.second_of_day;
+            frame0 = (long)h.frame;
+        }
         add = nfr * payload_bytes;
         if ((rc = grow(a.ctx, in.place, in.place_cap, add))) return rc;
-        rc = fx_frames_place(a.ctx, FX_FMT_VDIF, d, nfr, fb, hb, in.map, 1024, 1, (long)h.seconds, (long)h.frame,
-                             fps, (long)nfr, (uint32_t*)in.place, add / 4, in.valid, in.counts, FX_PLACE_CLEAR);
+        rc = fx_frames_place(a.ctx, s->format, d, nfr, fb, hb, in.map, 1024, 1, sec0, frame0, fps, (long)nfr,
+                             (uint32_t*)in.place, add / 4, in.valid, in.counts, FX_PLACE_CLEAR);
         if (rc) return rc;
         src = in.place;
     }
@@ -144,8 +155,8 @@ extern "C" int fx_swspectrometer_run(const fx_swspec_settings* s, const char* in
     FX_REQUIRE(c.bits_per_sample == 2, FX_ERR_UNSUPPORTED, "fx_swspectrometer_run: %d-bit input", c.bits_per_sample);
     FX_REQUIRE(nch == 1 || nch == 2 || nch == 4 || nch == 8 || nch == 16, FX_ERR_ARG,
                "fx_swspectrometer_run: source_channels %d", nch);
-    FX_REQUIRE(s->format == FX_FMT_RAW || s->format == FX_FMT_VDIF, FX_ERR_ARG, "fx_swspectrometer_run: format %d",
-               s->format);
+    FX_REQUIRE(s->format == FX_FMT_RAW || s->format == FX_FMT_VDIF || s->format == FX_FMT_MARK5B, FX_ERR_ARG,
+               "fx_swspectrometer_run: format %d", s->format);
     FX_REQUIRE(s->spectra_per_chunk >= 1, FX_ERR_ARG, "fx_swspectrometer_run: spectra_per_chunk");
     const int ninp = input2 ? 2 : 1;
     for (int i = 0; i < ninp; i++)
@@ -174,17 +185,17 @@ extern "C" int fx_swspectrometer_run(const fx_swspec_settings* s, const char* in
     size_t chunk_bytes = spec_bytes * (size_t)s->spectra_per_chunk;
     size_t payload_bytes = 0;
     int fps = 1;
-    if (s->format == FX_FMT_VDIF) {
+    if (s->format != FX_FMT_RAW) {
         FX_REQUIRE(s->framebytes > s->headerbytes && (s->headerbytes == 16 || s->headerbytes == 32) &&
                        s->framebytes % 8 == 0 && s->frames_per_second > 0,
-                   FX_ERR_ARG, "fx_swspectrometer_run: VDIF frames %d/%d bytes, %d per second", s->framebytes,
+                   FX_ERR_ARG, "fx_swspectrometer_run: frames of %d/%d bytes, %d per second", s->framebytes,
                    s->headerbytes, s->frames_per_second);
         payload_bytes = (size_t)(s->framebytes - s->headerbytes);
         fps = s->frames_per_second;
         const size_t nfr = (chunk_bytes + payload_bytes - 1) / payload_bytes;
         chunk_bytes = nfr * (size_t)s->framebytes;
     }
-    const size_t new_max = s->format == FX_FMT_VDIF ? chunk_bytes / (size_t)s->framebytes * payload_bytes : chunk_bytes;
+    const size_t new_max = s->format != FX_FMT_RAW ? chunk_bytes / (size_t)s->framebytes * payload_bytes : chunk_bytes;
 
     App a;
     a.ninp = ninp;
@@ -198,7 +209,7 @@ extern "C" int fx_swspectrometer_run(const fx_swspec_settings* s, const char* in
         size_t c0 = 0, c1 = 0;
         rc = grow(a.ctx, in.buf[0], c0, in.cap);
         if (!rc) rc = grow(a.ctx, in.buf[1], c1, in.cap);
-        if (!rc && s->format == FX_FMT_VDIF) {
+        if (!rc && s->format != FX_FMT_RAW) {
             void *v = nullptr, *cn = nullptr, *m = nullptr;
             const size_t nfr = chunk_bytes / (size_t)s->framebytes;
             rc = fx_malloc(a.ctx, 8 * nfr, &v);

@@ -118,9 +118,10 @@ int fx_taskcore_destroy(fx_taskcore* core);
  * keys UseFile1Channel/UseFile2Channel, swspectrometer.cpp:140-160). */
 typedef struct {
     fx_spectrometer_settings core;  /* use_channel is ignored: see use_channel_file */
-    int format;                     /* FX_FMT_RAW or FX_FMT_VDIF (one thread per file, frames in order) */
-    int framebytes, headerbytes;    /* VDIF frame and header sizes */
-    int frames_per_second;          /* VDIF frames per second of the thread */
+    int format;                     /* FX_FMT_RAW, FX_FMT_VDIF (one thread per file) or FX_FMT_MARK5B;
+                                       frames in time order without gaps */
+    int framebytes, headerbytes;    /* frame and header sizes (Mark5B: 10016 and 16) */
+    int frames_per_second;          /* frames per second of the file */
     int use_channel_file[2];        /* 1-based channel of input 1 and input 2 */
     int calc_xpol;                  /* cross spectrum of the two selected channels */
     int spectra_per_chunk;          /* whole spectra per double-buffered input chunk */
@@ -145,8 +146,10 @@ typedef struct {
  * the binary sinks: fft_points/2+1 float32 per spectrum of each input's
  * selected channel, the cross spectrum of the two (fft_points/2+1 complex
  * float32, fx_spectrometer_xpol) and the phase-cal (tonebins complex
- * float32).  The run ends when either input ends.  Returns 0 or an FX_ERR_*
- * code (fx_last_error()). */
+ * float32).  The run ends when either input ends.  Frames that the blanker
+ * marks invalid are counted by the placement but their samples are used as
+ * they are: mark5access would decode them as zeros.  Returns 0 or an
+ * FX_ERR_* code (fx_last_error()). */
 int fx_swspectrometer_run(const fx_swspec_settings* settings, const char* input1, const char* input2,
                           fx_swspec_stats* stats);
 

@@ -143,12 +143,41 @@ def test_app_vdif_with_pcal(tmp_path):
         assert nrel(pc[sp], acc.astype(np.complex64)) < 1e-6
 
 
+def test_app_mark5b(tmp_path):
+    """A Mark5B file (10016-byte frames, sign/magnitude sample codes) crossing
+    a second boundary: the spectra equal the oracle's over the samples the
+    frames carry."""
+    import fxlib
+    n, ov, K = 1024, 2, 5
+    fps, nfr = 400, 40                         # 16 MHz, 2-bit, one channel: 400 frames/s
+    ntime = nfr * 10000 * 4
+    words = O.gen_multichannel(33, 1, [0.2], [0.1], 0.9816, ntime)
+    m5 = FR.mark5b_swap_pairs(words).astype("<u4").view(np.uint8)   # the swap is its own inverse
+    frames = []
+    for k in range(nfr):
+        f = fps - 25 + k                      # 25 frames of second 86398, the rest of 86399
+        frames.append(FR.mark5b_encode(417, 86398 + f // fps, f % fps) + m5[k * 10000:(k + 1) * 10000].tobytes())
+    path = str(tmp_path / "obs.m5b")
+    with open(path, "wb") as fh:
+        fh.write(b"".join(frames))
+    base = str(tmp_path / "m5")
+    s = _settings(n, ov, K, O.WIN_HANN, 1, fxlib.FX_FMT_MARK5B, base, chunk=5, framebytes=10016, fps=fps)
+    s.headerbytes = 16
+    st = TL.SwspecStats()
+    assert TL.load().fx_swspectrometer_run(s, path.encode(), None, st) == 0, fxlib.load().fx_last_error()
+    L = (K - 1) * (n // ov) + n
+    assert st.spectra == ntime // L
+    spec = _read(base + "_a_swspec.bin", np.float32, n // 2 + 1)
+    ref = O.spectrometer_ex_f32(words, ntime, 1, n, ov, K, O.WIN_HANN, False, True)[:, 0, :]
+    assert spec.shape == ref.shape and nrel(spec, ref) < 1e-5
+
+
 def test_app_rejects_bad_settings(tmp_path):
     import fxlib
     p = str(tmp_path / "x.raw")
     np.zeros(1024, np.uint32).tofile(p)
     lib = TL.load()
-    for kw in (dict(nch=3), dict(chans=(5, 1)), dict(fmt=fxlib.FX_FMT_MARK5B), dict(chunk=0)):
+    for kw in (dict(nch=3), dict(chans=(5, 1)), dict(fmt=fxlib.FX_FMT_MARK5B), dict(fmt=7), dict(chunk=0)):
         s = _settings(1024, 2, 4, O.WIN_NONE, kw.get("nch", 2), kw.get("fmt", fxlib.FX_FMT_RAW),
                       str(tmp_path / "o"), chunk=kw.get("chunk", 2), chans=kw.get("chans", (1, 1)))
         assert lib.fx_swspectrometer_run(s, p.encode(), None, None) != 0
