@@ -23,6 +23,8 @@
 
 #include <hip/hip_fp8.h>
 
+#include <cstdlib>
+
 typedef _Float16 fx_half8 __attribute__((ext_vector_type(8)));
 typedef float fx_float16 __attribute__((ext_vector_type(16)));
 typedef int fx_int4v __attribute__((ext_vector_type(4)));
@@ -237,6 +239,156 @@ __global__ __launch_bounds__(XNT, 2) void k_xengine(const typename XTraits<PREC>
 #pragma unroll
         for (int b = 0; b < NBW; b++) {
             if (diag && qi == qj && a > b) continue;
+            float* d = dst + (a * NBW + b) * (32 * 64);
+#pragma unroll
+            for (int r = 0; r < 16; r++) {
+                d[r * 64 + lane] += dre[a][b][r];
+                d[(16 + r) * 64 + lane] += dim[a][b][r];
+            }
+        }
+}
+
+// ---------------------------------------------------------------------------
+// One full 128 x 128 tile (ninp == 128, nt a multiple of KT): the stages go
+// through a ring of RING slabs in LDS filled by LDS-DMA loads
+// (global_load_lds_dwordx4: lane l's 16 bytes land at the wave's LDS base +
+// 16 l), so RING - 1 stages are in flight while one is multiplied and no
+// registers hold prefetched data.  Same products, order and engine image as
+// k_xengine<PREC, 128>.
+// ---------------------------------------------------------------------------
+template <int N>
+__device__ __forceinline__ void wait_vmcnt()
+{
+    // s_waitcnt simm16 on gfx9: vmcnt[3:0] | expcnt[6:4] | lgkmcnt[11:8] | vmcnt[5:4] << 14
+    static_assert(N >= 0 && N < 64, "vmcnt range");
+    __builtin_amdgcn_s_waitcnt((N & 0xF) | (0x7 << 4) | (0xF << 8) | (((N >> 4) & 3) << 14));
+}
+
+template <int PREC>
+__global__ __launch_bounds__(XNT, 2) void k_xengine_ring(const typename XTraits<PREC>::E* __restrict__ X,
+                                                         int nchan, long nt, float* __restrict__ acc)
+{
+    typedef typename XTraits<PREC>::E E;
+    constexpr int TSZ = 128, NBW = 2, NBT = 16;
+    constexpr int KT = XTraits<PREC>::KT;
+    constexpr int RING = 4;
+    constexpr int STAGE_BYTES = KT * TSZ * (int)sizeof(E);
+    constexpr int NPER = STAGE_BYTES / 1024 / (XNT / 64);   // 1 KB DMA loads per wave per stage
+    static_assert(NPER * 1024 * (XNT / 64) == STAGE_BYTES, "stage is not whole 1 KB loads per wave");
+    __shared__ E slab[RING][KT][TSZ];
+
+    const long lb = xcd_block((long)nchan);
+    if (lb >= (long)nchan) return;
+    const int c = (int)lb;
+    const int tid = threadIdx.x;
+    const int w = tid >> 6, lane = tid & 63;
+    const int qi = w >> 1, qj = w & 1;
+    const bool active = qi <= qj;
+    const char* base = reinterpret_cast<const char*>(X + (size_t)c * (size_t)nt * TSZ);
+    const long nstage = nt / KT;
+
+    auto issue = [&](long st) {
+        const char* src = base + (size_t)st * STAGE_BYTES;
+        char* dst = reinterpret_cast<char*>(&slab[st % RING][0][0]);
+#pragma unroll
+        for (int i = 0; i < NPER; i++) {
+            const int blk = w * NPER + i;
+            __builtin_amdgcn_global_load_lds(src + blk * 1024 + lane * 16,
+                                             (__attribute__((address_space(3))) void*)(dst + blk * 1024), 16, 0, 0);
+        }
+    };
+
+    fx_float16 dre[NBW][NBW], dim[NBW][NBW];
+#pragma unroll
+    for (int a = 0; a < NBW; a++)
+#pragma unroll
+        for (int b = 0; b < NBW; b++) {
+            dre[a][b] = fx_float16{};
+            dim[a][b] = fx_float16{};
+        }
+    const int r32 = lane & 31, h = lane >> 5;
+    for (long st = 0; st < RING - 1 && st < nstage; st++) issue(st);
+    for (long st = 0; st < nstage; st++) {
+        // this wave's loads of stage st are done once at most the later
+        // stages' loads are outstanding; the barrier makes every wave's
+        // stage visible and frees the slab of stage st - 1
+        const long ahead = nstage - 1 - st;
+        if (ahead >= RING - 2) wait_vmcnt<(RING - 2) * NPER>();
+        else if (ahead == 1) wait_vmcnt<NPER>();
+        else wait_vmcnt<0>();
+        // a plain barrier: __syncthreads() would also wait for the later
+        // stages' loads (its release fence drains vmcnt)
+        asm volatile("s_barrier" ::: "memory");
+        if (st + RING - 1 < nstage) issue(st + RING - 1);
+        if (!active) continue;
+        const E (*sl)[TSZ] = slab[st % RING];
+        if constexpr (PREC == FX_PREC_F16) {
+#pragma unroll
+            for (int ks = 0; ks < KT / 8; ks++) {
+                fx_half8 fa[NBW], fb[NBW], fr[NBW];
+#pragma unroll
+                for (int a = 0; a < NBW; a++) {
+                    fx_int4v ia;
+                    const int row = (NBW * qi + a) * 32 + r32;
+#pragma unroll
+                    for (int q = 0; q < 4; q++) ia[q] = (int)sl[ks * 8 + 4 * h + q][row];
+                    fa[a] = __builtin_bit_cast(fx_half8, ia);
+                }
+#pragma unroll
+                for (int b = 0; b < NBW; b++) {
+                    fx_int4v ib, ir;
+                    const int row = (NBW * qj + b) * 32 + r32;
+#pragma unroll
+                    for (int q = 0; q < 4; q++) {
+                        uint32_t v = sl[ks * 8 + 4 * h + q][row];
+                        ib[q] = (int)v;
+                        ir[q] = (int)rot_h2(v);
+                    }
+                    fb[b] = __builtin_bit_cast(fx_half8, ib);
+                    fr[b] = __builtin_bit_cast(fx_half8, ir);
+                }
+#pragma unroll
+                for (int a = 0; a < NBW; a++)
+#pragma unroll
+                    for (int b = 0; b < NBW; b++) {
+                        if (qi == qj && a > b) continue;
+                        dre[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_f16(fa[a], fb[b], dre[a][b], 0, 0, 0);
+                        dim[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_f16(fa[a], fr[b], dim[a][b], 0, 0, 0);
+                    }
+            }
+        } else {
+#pragma unroll
+            for (int ks = 0; ks < KT; ks++) {
+                float fa[NBW], fb[NBW], fr[NBW];
+#pragma unroll
+                for (int a = 0; a < NBW; a++) {
+                    const float2 v = sl[ks][(NBW * qi + a) * 32 + r32];
+                    fa[a] = h ? v.y : v.x;
+                }
+#pragma unroll
+                for (int b = 0; b < NBW; b++) {
+                    const float2 v = sl[ks][(NBW * qj + b) * 32 + r32];
+                    fb[b] = h ? v.y : v.x;
+                    fr[b] = h ? v.x : -v.y;
+                }
+#pragma unroll
+                for (int a = 0; a < NBW; a++)
+#pragma unroll
+                    for (int b = 0; b < NBW; b++) {
+                        if (qi == qj && a > b) continue;
+                        dre[a][b] = __builtin_amdgcn_mfma_f32_32x32x2f32(fa[a], fb[b], dre[a][b], 0, 0, 0);
+                        dim[a][b] = __builtin_amdgcn_mfma_f32_32x32x2f32(fa[a], fr[b], dim[a][b], 0, 0, 0);
+                    }
+            }
+        }
+    }
+    if (!active) return;
+    float* dst = acc + ((size_t)c * NBT + w * NBW * NBW) * (32 * 64);
+#pragma unroll
+    for (int a = 0; a < NBW; a++)
+#pragma unroll
+        for (int b = 0; b < NBW; b++) {
+            if (qi == qj && a > b) continue;
             float* d = dst + (a * NBW + b) * (32 * 64);
 #pragma unroll
             for (int r = 0; r < 16; r++) {
@@ -513,9 +665,32 @@ size_t fxk_xengine_acc_floats(int nchan, int ninp, int nsets, int ts)
     return (size_t)nsets * (size_t)nchan * (size_t)ntiles_ts(ninp, ts) * (size_t)((ts / 32) * (ts / 32)) * 32u * 64u;
 }
 
+// FX_XENGINE_RING=0 selects the register-prefetch engine for every shape
+static bool ring_enabled()
+{
+    static const bool on = [] {
+        const char* e = getenv("FX_XENGINE_RING");
+        return !(e && e[0] == '0');
+    }();
+    return on;
+}
+
 int fxk_xengine(hipStream_t s, int in_type, int ts, const void* X, int nchan, long nt, int ninp, float* acc)
 {
     long nblk = xcd_grid((long)ntiles_ts(ninp, ts) * nchan);
+    // one full tile and whole stages: the LDS-ring engine
+    if (ts == 128 && ninp == 128 && ((uintptr_t)X % 16) == 0 && ring_enabled()) {
+        if (in_type == FX_PREC_F16 && nt % XTraits<FX_PREC_F16>::KT == 0) {
+            hipLaunchKernelGGL(k_xengine_ring<FX_PREC_F16>, dim3((unsigned)nblk), dim3(XNT), 0, s,
+                               reinterpret_cast<const uint32_t*>(X), nchan, nt, acc);
+            return hipGetLastError() == hipSuccess ? FX_OK : FX_ERR_HIP;
+        }
+        if (in_type == FX_PREC_F32 && nt % XTraits<FX_PREC_F32>::KT == 0) {
+            hipLaunchKernelGGL(k_xengine_ring<FX_PREC_F32>, dim3((unsigned)nblk), dim3(XNT), 0, s,
+                               reinterpret_cast<const float2*>(X), nchan, nt, acc);
+            return hipGetLastError() == hipSuccess ? FX_OK : FX_ERR_HIP;
+        }
+    }
     if (in_type == FX_PREC_F32 && ts == 128) {
         hipLaunchKernelGGL((k_xengine<FX_PREC_F32, 128>), dim3((unsigned)nblk), dim3(XNT), 0, s,
                            reinterpret_cast<const float2*>(X), nchan, nt, ninp, acc);
