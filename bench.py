@@ -702,7 +702,7 @@ def main():
     dom = max(("fstage", "xengine"), key=lambda k: k16[k])
     if dom == "xengine":
         launch = k16["xengine"]
-        roof = {"kernel": "k_xengine<1>", "bound": "mfma",
+        roof = {"kernel": "k_xengine_ring<1>" if I == 128 else "k_xengine<1, 128>", "bound": "mfma",
                 "achieved": x_flop / (launch * 1e-3) / 1e12, "peak": PEAK["fp16"], "unit": "TFLOP/s",
                 "algorithmic": f"8*P*C*chunk = {x_flop:.3e} flop per launch"}
     else:
