@@ -186,6 +186,39 @@ def large_n(fxlib, ctx, args):
     return out
 
 
+def unpack(fxlib, ctx, args):
+    """2-bit unpack to FP32 through the level table (fx_unpack2b): 2^30
+    samples of one channel (256 MiB packed -> 4 GiB of floats), and 2^26 time
+    steps of 16 interleaved channels (same bytes, channel-major out).
+    Algorithmic bytes: 0.25 B read + 4 B written per sample."""
+    ns = 1 << 30
+    dw = ctx.alloc(ns // 4)
+    fxlib.check(fxlib.load().fx_synth_station(ctx.handle, args.seed + 8, 0, 0.0, 0, 0.0, 0.0, 0.9816, ns,
+                                              dw.ptr))
+    dout = ctx.alloc(ns * 4)
+    out = {"workload": "2^30 2-bit samples -> FP32: 1 channel, and 16 interleaved channels x 2^26 time steps",
+           "algorithmic": "0.25 B in + 4 B out per sample"}
+    e = [ctx.event() for _ in range(2)]
+    reps = max(3, args.steps)
+    for name, nch in (("1ch", 1), ("16ch", 16)):
+        ntime = ns // nch
+        ctx.unpack2b(dw, ntime, nch, dout)
+        ctx.sync()
+        ctx.record(e[0])
+        for _ in range(reps):
+            ctx.unpack2b(dw, ntime, nch, dout)
+        ctx.record(e[1])
+        ctx.sync()
+        ms = ctx.elapsed_ms(e[0], e[1]) / reps
+        gbps = ns * 4.25 / (ms * 1e-3) / 1e9
+        out[name] = {"ms": ms, "gsamples_per_s": ns / (ms * 1e-3) / 1e9, "gbps": gbps, "frac": gbps / PEAK["hbm"]}
+    log(f"[bench] unpack: 1ch {out['1ch']['ms']:.2f} ms ({out['1ch']['gbps']:.0f} GB/s), "
+        f"16ch {out['16ch']['ms']:.2f} ms ({out['16ch']['gbps']:.0f} GB/s)")
+    dw.free()
+    dout.free()
+    return out
+
+
 def products(fxlib, ctx, dvis, P, C, T, args):
     """Lag spectra and amplitude/phase of the dumped visibilities of the
     mid-size correlator (zerocorr.c:620-645 downstream products): one batched
@@ -550,7 +583,8 @@ def main():
         return
     if args.front_end_only:
         ctx = fxlib.Context(0)
-        print(json.dumps({"front_end": front_end(fxlib, ctx, args), "task_chains": task_chains(args),
+        print(json.dumps({"unpack": unpack(fxlib, ctx, args), "front_end": front_end(fxlib, ctx, args),
+                          "task_chains": task_chains(args),
                           "spectrometer_app": spectrometer_app(fxlib, args)}), flush=True)
         ctx.close()
         return
@@ -683,6 +717,7 @@ def main():
     dw.free()
     prod = products(fxlib, ctx, dvis, P, C, T, args)
     dvis.free()
+    unp = unpack(fxlib, ctx, args)
     fe = None if args.no_front_end else front_end(fxlib, ctx, args)
     chains = None if args.no_tasks else task_chains(args)
     app = None if args.no_front_end else spectrometer_app(fxlib, args)
@@ -766,6 +801,7 @@ def main():
         "large_n_xengine": large,
         "batched_subbands": batch,
         "visibility_products": prod,
+        "unpack": unp,
         "front_end": fe,
         "task_chains": chains,
         "spectrometer_app": app,

@@ -29,6 +29,23 @@ __global__ void k_unpack2b(const uint32_t* __restrict__ words, size_t ntime, int
     if (t0 >= ntime) return;
     const uint32_t v = words[w];
     if (nchan == 1) {
+        // a wave's 64 words are 1024 samples: store q puts lane l's float4
+        // at samples 4 (64 q + l) of that span, so every store is a contiguous
+        // 1 KB; the byte it needs is byte l % 4 of word 16 q + l / 4, taken
+        // from that lane with a cross-lane read
+        const int lane = threadIdx.x & 63;
+        const size_t w0 = w - (size_t)lane;
+        if ((w0 + 64) * 16 <= ntime) {
+            float4* o = reinterpret_cast<float4*>(out + w0 * 16);
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const uint32_t src = (uint32_t)__shfl((int)v, 16 * q + (lane >> 2), 64);
+                const uint32_t b = src >> (8 * (lane & 3));
+                o[64 * q + lane] = make_float4(lut2(b & 3u), lut2((b >> 2) & 3u), lut2((b >> 4) & 3u),
+                                               lut2((b >> 6) & 3u));
+            }
+            return;
+        }
         if (t0 + 16 <= ntime) {
             float4* o = reinterpret_cast<float4*>(out + t0);
 #pragma unroll
