@@ -223,6 +223,50 @@ __global__ __launch_bounds__(256) void k_pcal_fold(const uint32_t* __restrict__ 
     }
 }
 
+// nfold % 16 == 0: a word's 16 samples land in 16 consecutive bins, and with a
+// word stride T that is a multiple of nfold/16 every thread keeps the same 16
+// bins for all its words.  Each thread counts codes per bin in registers --
+// A = n(+H) - n(-H), B = n(+1) - n(-1), so the bin's sum is H A + B -- and
+// adds its 16 sums once.  The FP64 sums are the same exact values as the
+// sample-by-sample fold (multiples of 2^-22 below 2^31).
+__global__ __launch_bounds__(256) void k_pcal_fold16(const uint32_t* __restrict__ words, size_t nsamp,
+                                                     size_t position, int nfold, long T,
+                                                     double* __restrict__ acc, size_t in_stride)
+{
+    extern __shared__ double lacc[];
+    words += (size_t)blockIdx.y * in_stride;
+    acc += (size_t)blockIdx.y * (size_t)nfold;
+    for (int b = threadIdx.x; b < nfold; b += blockDim.x) lacc[b] = 0.0;
+    __syncthreads();
+    const long tid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t nw = (nsamp + 15) / 16;
+    int A[16], B[16];
+#pragma unroll
+    for (int j = 0; j < 16; j++) A[j] = B[j] = 0;
+    if (tid < T) {
+        for (size_t w = (size_t)tid; w < nw; w += (size_t)T) {
+            const uint32_t v = words[w];
+            const size_t rem = nsamp - w * 16;      // >= 1
+#pragma unroll
+            for (int j = 0; j < 16; j++) {
+                const uint32_t c = (v >> (2 * j)) & 3u;
+                const int live = (size_t)j < rem;
+                A[j] += live * ((int)(c == 3u) - (int)(c == 0u));
+                B[j] += live * ((int)(c == 2u) - (int)(c == 1u));
+            }
+        }
+        int bin = (int)((position + 16 * (size_t)tid) % (size_t)nfold);
+#pragma unroll
+        for (int j = 0; j < 16; j++) {
+            if (A[j] | B[j]) atomicAdd(&lacc[bin], (double)FX_LUT_HI * (double)A[j] + (double)B[j]);
+            if (++bin == nfold) bin = 0;
+        }
+    }
+    __syncthreads();
+    for (int b = threadIdx.x; b < nfold; b += blockDim.x)
+        if (lacc[b] != 0.0) atomicAdd(&acc[b], lacc[b]);
+}
+
 // tone t at DFT bin k = offset_bin + t*step of the nbins-point fold:
 //   A[b] = sum_{r = b mod nbins} R[r] e^{i dphi (r mod rotatorlen)}
 //          (Shifting extractor, nfold = lcm(rotatorlen, nbins))
@@ -537,6 +581,16 @@ int fxk_pcal_fold(hipStream_t s, const uint32_t* words, size_t nsamp, size_t pos
 {
     size_t nw = (nsamp + 15) / 16;
     if (nw == 0 || ninp <= 0) return FX_OK;
+    if (nfold % 16 == 0 && nfold <= 8192) {
+        // word stride: a multiple of nfold/16, about 8+ words per thread
+        const long nb = nfold / 16;
+        const long want = std::max<long>(256, std::min<long>(65536, (long)(nw / 8)));
+        const long T = std::max<long>(1, want / nb) * nb;
+        const unsigned blocks = (unsigned)((T + 255) / 256);
+        hipLaunchKernelGGL(k_pcal_fold16, dim3(blocks, (unsigned)ninp), dim3(256), (size_t)nfold * sizeof(double), s,
+                           words, nsamp, position, nfold, T, acc, in_stride_words);
+        return last_status();
+    }
     int use_lds = nfold <= 8192;
     size_t lds = use_lds ? (size_t)nfold * sizeof(double) : 0;
     // at least 16 words per thread: each workgroup's nfold global additions
