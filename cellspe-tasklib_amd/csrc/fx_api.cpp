@@ -527,6 +527,15 @@ extern "C" int fx_spectrometer_xpol(fx_ctx* ctx, const uint32_t* d_words_a, int 
                                     int overlap, int integrate_ffts, int window, int continuous, fx_cf32* d_out,
                                     long* nspec_out)
 {
+    return fx_spec_xpol_impl(ctx, d_words_a, nullptr, nchan_a, chan_a, d_words_b, nullptr, nchan_b, chan_b, ntime, n,
+                             overlap, integrate_ffts, window, continuous, d_out, nspec_out);
+}
+
+int fx_spec_xpol_impl(fx_ctx* ctx, const uint32_t* d_words_a, const uint8_t* d_valid_a, int nchan_a, int chan_a,
+                      const uint32_t* d_words_b, const uint8_t* d_valid_b, int nchan_b, int chan_b, size_t ntime,
+                      int n, int overlap, int integrate_ffts, int window, int continuous, fx_cf32* d_out,
+                      long* nspec_out)
+{
     int rc = ctx_device(ctx);
     if (rc) return rc;
     FX_REQUIRE(d_words_a && d_words_b && d_out, FX_ERR_ARG, "fx_spectrometer_xpol: null argument");
@@ -577,9 +586,9 @@ extern "C" int fx_spectrometer_xpol(fx_ctx* ctx, const uint32_t* d_words_a, int 
     for (long s0 = 0; s0 < nspec; s0 += sb) {
         const long ns = std::min<long>(sb, nspec - s0);
         const long frames = ns * K;
-        FX_CHECK_LAUNCH(fxk_spec_frames(ctx->stream, d_words_a, nchan_a, chan_a, hop, spec_stride, (int)K, n,
+        FX_CHECK_LAUNCH(fxk_spec_frames(ctx->stream, d_words_a, d_valid_a, nchan_a, chan_a, hop, spec_stride, (int)K, n,
                                         s0 * K, frames, win, fa));
-        FX_CHECK_LAUNCH(fxk_spec_frames(ctx->stream, d_words_b, nchan_b, chan_b, hop, spec_stride, (int)K, n,
+        FX_CHECK_LAUNCH(fxk_spec_frames(ctx->stream, d_words_b, d_valid_b, nchan_b, chan_b, hop, spec_stride, (int)K, n,
                                         s0 * K, frames, win, fb));
         FX_CHECK_LAUNCH(fxk_fft_r2c(ctx->stream, ctx->tw, fa, xa, n, frames, 1));
         FX_CHECK_LAUNCH(fxk_fft_r2c(ctx->stream, ctx->tw, fb, xb, n, frames, 1));

@@ -248,7 +248,42 @@ __global__ __launch_bounds__(256) void k_frames_place(const uint8_t* __restrict_
     }
 }
 
+__global__ __launch_bounds__(256) void k_valid_words(const uint32_t* __restrict__ valid, long nslots, int pwords,
+                                                     uint8_t* __restrict__ flags)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (size_t)nslots * (size_t)pwords) return;
+    const size_t slot = i / (size_t)pwords;
+    const uint32_t j = (uint32_t)(i - slot * (size_t)pwords);
+    flags[i] = (j >= valid[2 * slot] && j < valid[2 * slot + 1]) ? 1 : 0;
+}
+
+__global__ __launch_bounds__(256) void k_any_zero(const uint8_t* __restrict__ flags, size_t n, int* __restrict__ any)
+{
+    bool z = false;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        z = z || flags[i] == 0;
+    if (__any(z) && (threadIdx.x & 63) == 0) any[0] = 1;
+}
+
 } // namespace
+
+int fxk_valid_words(hipStream_t s, const uint32_t* valid, long nslots, int pwords, uint8_t* flags)
+{
+    const size_t n = (size_t)nslots * (size_t)pwords;
+    if (n == 0) return FX_OK;
+    hipLaunchKernelGGL(k_valid_words, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, valid, nslots, pwords,
+                       flags);
+    return hipGetLastError() == hipSuccess ? FX_OK : FX_ERR_HIP;
+}
+
+int fxk_any_zero(hipStream_t s, const uint8_t* flags, size_t n, int* any)
+{
+    if (n == 0) return FX_OK;
+    const size_t blocks = std::min<size_t>((n + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_any_zero, dim3((unsigned)blocks), dim3(256), 0, s, flags, n, any);
+    return hipGetLastError() == hipSuccess ? FX_OK : FX_ERR_HIP;
+}
 
 int fxk_frames_place(hipStream_t s, int format, const uint8_t* raw, long nframes, int framebytes,
                      int payload_offset, const int* thread_row, int map_size, int nrows, long sec0, long frame0,

@@ -387,8 +387,10 @@ int fxk_xengine_dump_pij(hipStream_t s, const float* acc, int nchan, int ninp, i
 int fxk_mfma_probe(hipStream_t s, int which, float* out);
 // phase-cal of ninp inputs in one launch: input y reads words + y*in_stride_words
 // and owns acc[y*nfold ..] and tones[y*2*ntones ..]
+// wvalid (may be null): one byte per packed word of input 0, 0 = blanked (skipped)
 int fxk_pcal_fold(hipStream_t s, const uint32_t* words, size_t nsamp, size_t position,
-                  int nfold, double* acc, int ninp = 1, size_t in_stride_words = 0);
+                  int nfold, double* acc, int ninp = 1, size_t in_stride_words = 0,
+                  const uint8_t* wvalid = nullptr);
 int fxk_pcal_final(hipStream_t s, const double* acc, int nfold, int nbins, int ntones,
                    int offset_bin, int step, int rotatorlen, double dphi, double* tones, int ninp = 1);
 int fxk_complex_mac(hipStream_t s, const float2* a, const float2* b, float2* acc, long n, int conj);
@@ -404,9 +406,20 @@ int fxk_fill32(hipStream_t s, void* p, uint32_t value, size_t n);
 // out rows by (second, frame number, thread); valid[(row*nslots + slot)*2]
 // gets the valid 32-bit word range of the slot's payload; counts: placed,
 // placed but blank, slot out of range, unmapped thread
-// windowed 2-bit FFT inputs of one channel, frames g0..g0+nframes-1 (see k_spec_frames)
-int fxk_spec_frames(hipStream_t s, const uint32_t* words, int nchan, int ch, long hop, long spec_stride, int K,
-                    int n, long g0, long nframes, const float* win, float* out);
+// windowed 2-bit FFT inputs of one channel, frames g0..g0+nframes-1 (see
+// k_spec_frames); wvalid (may be null): one byte per packed word, 0 = blanked
+int fxk_spec_frames(hipStream_t s, const uint32_t* words, const uint8_t* wvalid, int nchan, int ch, long hop,
+                    long spec_stride, int K, int n, long g0, long nframes, const float* win, float* out);
+// cross spectra with optional per-word validity of either stream (see fx_spectrometer_xpol)
+int fx_spec_xpol_impl(fx_ctx* ctx, const uint32_t* d_words_a, const uint8_t* d_valid_a, int nchan_a, int chan_a,
+                      const uint32_t* d_words_b, const uint8_t* d_valid_b, int nchan_b, int chan_b, size_t ntime,
+                      int n, int overlap, int integrate_ffts, int window, int continuous, fx_cf32* d_out,
+                      long* nspec_out);
+// per-word validity of nslots placed payloads of pwords words from their
+// [start, end) ranges (valid[2 slot], valid[2 slot + 1]); flags[slot * pwords + j]
+int fxk_valid_words(hipStream_t s, const uint32_t* valid, long nslots, int pwords, uint8_t* flags);
+// *any = 1 if one of the n flags is 0 (else left as is)
+int fxk_any_zero(hipStream_t s, const uint8_t* flags, size_t n, int* any);
 // per spectrum and bin: scale * sum over K FFTs of A conj(B) (DC, Nyquist: re*re),
 // the FFTs split over `groups` partial sums (part: groups * nspec * c1 float2)
 int fxk_xpol_groups(int c1, int K, long nspec);
@@ -415,7 +428,8 @@ int fxk_xpol_acc(hipStream_t s, const float2* A, const float2* B, int c1, int K,
 // phase-cal of nspec TaskCore spectra of L samples at d_words (1 channel),
 // spectrum indices counted from spec0; fold: grown device scratch
 int fx_spec_pcal(fx_ctx* ctx, const uint32_t* d_words, long nspec, long L, long spec0, int N, int ov, int K, int R,
-                 int T, double dphi, void*& fold, size_t& fold_cap, std::vector<fx_cf64>& out);
+                 int T, double dphi, void*& fold, size_t& fold_cap, std::vector<fx_cf64>& out,
+                 const uint8_t* d_wvalid = nullptr);
 int fxk_frames_place(hipStream_t s, int format, const uint8_t* raw, long nframes, int framebytes,
                      int payload_offset, const int* thread_row, int map_size, int nrows, long sec0, long frame0,
                      int fps,

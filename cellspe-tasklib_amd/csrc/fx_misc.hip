@@ -189,7 +189,8 @@ __global__ void k_synth_channelized(uint64_t seed, int nchan, long ntime, int ni
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_pcal_fold(const uint32_t* __restrict__ words, size_t nsamp,
                                                    size_t position, int nfold, int use_lds,
-                                                   double* __restrict__ acc, size_t in_stride)
+                                                   double* __restrict__ acc, size_t in_stride,
+                                                   const uint8_t* __restrict__ wvalid)
 {
     extern __shared__ double lacc[];
     // grid y = input: words of input y at words + y*in_stride, its fold at acc + y*nfold
@@ -203,6 +204,7 @@ __global__ __launch_bounds__(256) void k_pcal_fold(const uint32_t* __restrict__ 
     // LDS copy before the nfold global additions of its partial fold
     const size_t nw = (nsamp + 15) / 16;
     for (size_t w = (size_t)blockIdx.x * blockDim.x + threadIdx.x; w < nw; w += (size_t)gridDim.x * blockDim.x) {
+        if (wvalid && !wvalid[w]) continue;         // a blanked word decodes as zeros
         const uint32_t v = words[w];
         size_t n0 = w * 16;
         int bin = (int)((position + n0) % (size_t)nfold);
@@ -231,7 +233,8 @@ __global__ __launch_bounds__(256) void k_pcal_fold(const uint32_t* __restrict__ 
 // sample-by-sample fold (multiples of 2^-22 below 2^31).
 __global__ __launch_bounds__(256) void k_pcal_fold16(const uint32_t* __restrict__ words, size_t nsamp,
                                                      size_t position, int nfold, long T,
-                                                     double* __restrict__ acc, size_t in_stride)
+                                                     double* __restrict__ acc, size_t in_stride,
+                                                     const uint8_t* __restrict__ wvalid)
 {
     extern __shared__ double lacc[];
     words += (size_t)blockIdx.y * in_stride;
@@ -245,6 +248,7 @@ __global__ __launch_bounds__(256) void k_pcal_fold16(const uint32_t* __restrict_
     for (int j = 0; j < 16; j++) A[j] = B[j] = 0;
     if (tid < T) {
         for (size_t w = (size_t)tid; w < nw; w += (size_t)T) {
+            if (wvalid && !wvalid[w]) continue;     // a blanked word decodes as zeros
             const uint32_t v = words[w];
             const size_t rem = nsamp - w * 16;      // >= 1
 #pragma unroll
@@ -577,7 +581,7 @@ int fxk_codes_to_chan(hipStream_t s, const uint8_t* codes, int in_imajor, int nc
 }
 
 int fxk_pcal_fold(hipStream_t s, const uint32_t* words, size_t nsamp, size_t position, int nfold,
-                  double* acc, int ninp, size_t in_stride_words)
+                  double* acc, int ninp, size_t in_stride_words, const uint8_t* wvalid)
 {
     size_t nw = (nsamp + 15) / 16;
     if (nw == 0 || ninp <= 0) return FX_OK;
@@ -588,7 +592,7 @@ int fxk_pcal_fold(hipStream_t s, const uint32_t* words, size_t nsamp, size_t pos
         const long T = std::max<long>(1, want / nb) * nb;
         const unsigned blocks = (unsigned)((T + 255) / 256);
         hipLaunchKernelGGL(k_pcal_fold16, dim3(blocks, (unsigned)ninp), dim3(256), (size_t)nfold * sizeof(double), s,
-                           words, nsamp, position, nfold, T, acc, in_stride_words);
+                           words, nsamp, position, nfold, T, acc, in_stride_words, wvalid);
         return last_status();
     }
     int use_lds = nfold <= 8192;
@@ -597,7 +601,7 @@ int fxk_pcal_fold(hipStream_t s, const uint32_t* words, size_t nsamp, size_t pos
     // are amortized over >= 65536 samples
     const size_t blocks = std::min<size_t>(nblocks(nw, 256), std::max<size_t>(64, nw / 4096));
     hipLaunchKernelGGL(k_pcal_fold, dim3((unsigned)blocks, (unsigned)ninp), dim3(256), lds, s, words, nsamp,
-                       position, nfold, use_lds, acc, in_stride_words);
+                       position, nfold, use_lds, acc, in_stride_words, wvalid);
     return last_status();
 }
 
@@ -706,7 +710,8 @@ namespace {
 // Windowed real FFT inputs of one channel of a packed 2-bit stream: frame
 // g0 + blockIdx.y = (spectrum g / K, FFT g % K) starts at sample
 // (g / K) * spec_stride + (g % K) * hop.
-__global__ __launch_bounds__(256) void k_spec_frames(const uint32_t* __restrict__ words, int nchan, int ch,
+__global__ __launch_bounds__(256) void k_spec_frames(const uint32_t* __restrict__ words,
+                                                     const uint8_t* __restrict__ wvalid, int nchan, int ch,
                                                      long hop, long spec_stride, int K, int n, long g0,
                                                      const float* __restrict__ win, float* __restrict__ out)
 {
@@ -716,7 +721,8 @@ __global__ __launch_bounds__(256) void k_spec_frames(const uint32_t* __restrict_
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const size_t field = (size_t)(t0 + i) * (size_t)nchan + (size_t)ch;
         const uint32_t code = (words[field >> 4] >> (2u * (uint32_t)(field & 15u))) & 3u;
-        o[i] = lut2(code) * win[i];
+        // a word the blanker marked invalid decodes as 0 (mark5access zeros)
+        o[i] = (wvalid && !wvalid[field >> 4]) ? 0.f : lut2(code) * win[i];
     }
 }
 
@@ -765,12 +771,12 @@ __global__ __launch_bounds__(256) void k_xpol_sum(const float2* __restrict__ par
 
 } // namespace
 
-int fxk_spec_frames(hipStream_t s, const uint32_t* words, int nchan, int ch, long hop, long spec_stride, int K,
-                    int n, long g0, long nframes, const float* win, float* out)
+int fxk_spec_frames(hipStream_t s, const uint32_t* words, const uint8_t* wvalid, int nchan, int ch, long hop,
+                    long spec_stride, int K, int n, long g0, long nframes, const float* win, float* out)
 {
     if (nframes <= 0) return FX_OK;
     const int bx = std::min(8, (n + 255) / 256);
-    hipLaunchKernelGGL(k_spec_frames, dim3((unsigned)bx, (unsigned)nframes), dim3(256), 0, s, words, nchan, ch,
+    hipLaunchKernelGGL(k_spec_frames, dim3((unsigned)bx, (unsigned)nframes), dim3(256), 0, s, words, wvalid, nchan, ch,
                        hop, spec_stride, K, n, g0, win, out);
     return last_status();
 }

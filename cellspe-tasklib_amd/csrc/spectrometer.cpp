@@ -109,7 +109,8 @@ extern "C" int spectrometer_cb_run(const control_block_t* cb, spectrometer_mbox_
 // order-free); the rotator and the fold into tonebins (TaskCoreIPP.cpp:
 // 633-693) are then applied in FP64 on rotatorlen values per spectrum.
 int fx_spec_pcal(fx_ctx* ctx, const uint32_t* d_words, long nspec, long L, long spec0, int N, int ov, int K, int R,
-                 int T, double dphi, void*& fold, size_t& fold_cap, std::vector<fx_cf64>& out)
+                 int T, double dphi, void*& fold, size_t& fold_cap, std::vector<fx_cf64>& out,
+                 const uint8_t* d_wvalid)
 {
     const long hop = N / ov;
     const size_t bytes = (size_t)nspec * (size_t)R * sizeof(double);
@@ -128,8 +129,8 @@ int fx_spec_pcal(fx_ctx* ctx, const uint32_t* d_words, long nspec, long L, long 
         if (j0 >= K) continue;
         const long cnt = (K - 1 - j0) / ov + 1;
         const size_t start = (size_t)(sp * L + j0 * hop);
-        if (fxk_pcal_fold(st, d_words + start / 16, (size_t)(cnt * N), 0, R, (double*)fold + (size_t)sp * R) !=
-            FX_OK) {
+        if (fxk_pcal_fold(st, d_words + start / 16, (size_t)(cnt * N), 0, R, (double*)fold + (size_t)sp * R, 1, 0,
+                          d_wvalid ? d_wvalid + start / 16 : nullptr) != FX_OK) {
             fx_set_error("phase-cal fold launch failed");
             return -1;
         }

@@ -44,6 +44,7 @@ struct Input {
     int* map = nullptr;
     void* fold = nullptr;
     size_t fold_cap = 0;
+    uint8_t* wv[2] = {nullptr, nullptr};  // framed input: per packed word, 0 = blanked
     bool eof = false;
 };
 
@@ -56,6 +57,7 @@ struct App {
     size_t auto_cap = 0;
     void* d_x = nullptr;
     size_t x_cap = 0;
+    int* d_any = nullptr;
 };
 
 int grow(fx_ctx* ctx, void*& p, size_t& cap, size_t bytes)
@@ -76,7 +78,8 @@ void close_app(App& a)
         if (i.sink) fclose(i.sink);
         if (i.pcal_sink) fclose(i.pcal_sink);
         if (a.ctx) {
-            for (void* p : {i.buf[0], i.buf[1], i.place, (void*)i.valid, (void*)i.counts, (void*)i.map, i.fold})
+            for (void* p : {i.buf[0], i.buf[1], i.place, (void*)i.valid, (void*)i.counts, (void*)i.map, i.fold,
+                            (void*)i.wv[0], (void*)i.wv[1]})
                 if (p) fx_free(a.ctx, p);
         }
     }
@@ -84,6 +87,7 @@ void close_app(App& a)
     if (a.ctx) {
         if (a.d_auto) fx_free(a.ctx, a.d_auto);
         if (a.d_x) fx_free(a.ctx, a.d_x);
+        if (a.d_any) fx_free(a.ctx, a.d_any);
         fx_ctx_destroy(a.ctx);
     }
 }
@@ -137,6 +141,12 @@ int pull(App& a, Input& in, const fx_swspec_settings* s, size_t payload_bytes, i
     }
     FX_REQUIRE(in.have + add <= in.cap, FX_ERR_ARG, "fx_swspectrometer_run: chunk larger than the stream buffer");
     if ((rc = fx_memcpy_d2d(a.ctx, (char*)in.buf[in.cur] + in.have, src, add))) return rc;
+    if (s->format != FX_FMT_RAW &&
+        fxk_valid_words(fx_ctx_hip_stream(a.ctx), in.valid, (long)(n / (size_t)s->framebytes),
+                        (int)(payload_bytes / 4), in.wv[in.cur] + in.have / 4) != FX_OK) {
+        fx_set_error("fx_swspectrometer_run: validity launch failed");
+        return FX_ERR_HIP;
+    }
     in.have += add;
     return FX_OK;
 }
@@ -222,6 +232,16 @@ extern "C" int fx_swspectrometer_run(const fx_swspec_settings* s, const char* in
                 std::vector<int> map(1024, 0);       // every thread of the file is this input
                 rc = fx_memcpy_h2d(a.ctx, in.map, map.data(), map.size() * sizeof(int));
             }
+            void *w0 = nullptr, *w1 = nullptr;
+            if (!rc) rc = fx_malloc(a.ctx, in.cap / 4 + 16, &w0);
+            if (!rc) rc = fx_malloc(a.ctx, in.cap / 4 + 16, &w1);
+            in.wv[0] = (uint8_t*)w0;
+            in.wv[1] = (uint8_t*)w1;
+            if (!rc && !a.d_any) {
+                void* p = nullptr;
+                rc = fx_malloc(a.ctx, 16, &p);
+                a.d_any = (int*)p;
+            }
         }
         const char* base = s->basefilename[i] ? s->basefilename[i] : s->basefilename[0];
         if (!rc && !(in.sink = open_sink(base, "_swspec.bin"))) rc = FX_ERR_ARG;
@@ -259,14 +279,42 @@ extern "C" int fx_swspectrometer_run(const fx_swspec_settings* s, const char* in
             for (int i = 0; i < ninp && !rc; i++) {
                 Input& in = a.in[i];
                 const uint32_t* w = (const uint32_t*)in.buf[in.cur];
-                long got = 0;
-                rc = fx_spectrometer_ex(a.ctx, w, ntime, nch, N, ov, K, c.window, 0, 1, (float*)a.d_auto, &got);
-                if (!rc) rc = fx_memcpy_d2h(a.ctx, h_auto.data(), a.d_auto, h_auto.size() * sizeof(float));
-                for (long sp = 0; sp < ns && !rc; sp++)
-                    if (fwrite(&h_auto[((size_t)sp * nch + in.chan) * C1], sizeof(float), C1, in.sink) != (size_t)C1)
-                        rc = FX_ERR_STATE;
+                // a framed input whose processed words include blanked ones
+                // goes through the masked frame path (blanked words as zeros)
+                const uint8_t* wv = in.wv[in.cur];
+                bool blanked = false;
+                if (wv) {
+                    int any = 0;
+                    rc = fx_memset(a.ctx, a.d_any, 0, sizeof(int));
+                    if (!rc && fxk_any_zero(fx_ctx_hip_stream(a.ctx), wv, ntime * (size_t)nch / 16, a.d_any) != FX_OK)
+                        rc = FX_ERR_HIP;
+                    if (!rc) rc = fx_memcpy_d2h(a.ctx, &any, a.d_any, sizeof(int));
+                    blanked = any != 0;
+                }
+                if (!rc && blanked) {
+                    rc = grow(a.ctx, a.d_x, a.x_cap, (size_t)ns * C1 * sizeof(fx_cf32));
+                    if (!rc)
+                        rc = fx_spec_xpol_impl(a.ctx, w, wv, nch, in.chan, w, wv, nch, in.chan, ntime, N, ov, K,
+                                               c.window, 0, (fx_cf32*)a.d_x, nullptr);
+                    h_x.resize((size_t)ns * C1 * 2);
+                    if (!rc) rc = fx_memcpy_d2h(a.ctx, h_x.data(), a.d_x, h_x.size() * sizeof(float));
+                    std::vector<float> re((size_t)C1);
+                    for (long sp = 0; sp < ns && !rc; sp++) {
+                        for (int k = 0; k < C1; k++) re[(size_t)k] = h_x[2 * ((size_t)sp * C1 + k)];
+                        if (fwrite(re.data(), sizeof(float), C1, in.sink) != (size_t)C1) rc = FX_ERR_STATE;
+                    }
+                } else if (!rc) {
+                    long got = 0;
+                    rc = fx_spectrometer_ex(a.ctx, w, ntime, nch, N, ov, K, c.window, 0, 1, (float*)a.d_auto, &got);
+                    if (!rc) rc = fx_memcpy_d2h(a.ctx, h_auto.data(), a.d_auto, h_auto.size() * sizeof(float));
+                    for (long sp = 0; sp < ns && !rc; sp++)
+                        if (fwrite(&h_auto[((size_t)sp * nch + in.chan) * C1], sizeof(float), C1, in.sink) !=
+                            (size_t)C1)
+                            rc = FX_ERR_STATE;
+                }
                 if (!rc && T > 0) {
-                    rc = fx_spec_pcal(a.ctx, w, ns, L, total, N, ov, K, R, T, dphi, in.fold, in.fold_cap, pc);
+                    rc = fx_spec_pcal(a.ctx, w, ns, L, total, N, ov, K, R, T, dphi, in.fold, in.fold_cap, pc,
+                                      blanked ? wv : nullptr);
                     pc32.resize(pc.size() * 2);
                     for (size_t k = 0; k < pc.size(); k++) {
                         pc32[2 * k] = (float)pc[k].re;
@@ -279,9 +327,10 @@ extern "C" int fx_swspectrometer_run(const fx_swspec_settings* s, const char* in
             if (!rc && xpol) {
                 rc = grow(a.ctx, a.d_x, a.x_cap, (size_t)ns * C1 * sizeof(fx_cf32));
                 if (!rc)
-                    rc = fx_spectrometer_xpol(a.ctx, (const uint32_t*)a.in[0].buf[a.in[0].cur], nch, a.in[0].chan,
-                                              (const uint32_t*)a.in[1].buf[a.in[1].cur], nch, a.in[1].chan, ntime,
-                                              N, ov, K, c.window, 0, (fx_cf32*)a.d_x, nullptr);
+                    rc = fx_spec_xpol_impl(a.ctx, (const uint32_t*)a.in[0].buf[a.in[0].cur], a.in[0].wv[a.in[0].cur],
+                                           nch, a.in[0].chan, (const uint32_t*)a.in[1].buf[a.in[1].cur],
+                                           a.in[1].wv[a.in[1].cur], nch, a.in[1].chan, ntime, N, ov, K, c.window,
+                                           0, (fx_cf32*)a.d_x, nullptr);
                 h_x.resize((size_t)ns * C1 * 2);
                 if (!rc) rc = fx_memcpy_d2h(a.ctx, h_x.data(), a.d_x, h_x.size() * sizeof(float));
                 if (!rc && fwrite(h_x.data(), sizeof(float), h_x.size(), a.xsink) != h_x.size()) rc = FX_ERR_STATE;
@@ -293,6 +342,8 @@ extern "C" int fx_swspectrometer_run(const fx_swspec_settings* s, const char* in
                 Input& in = a.in[i];
                 const size_t used = (size_t)ns * spec_bytes, left = in.have - used;
                 if (left) rc = fx_memcpy_d2d(a.ctx, in.buf[1 - in.cur], (char*)in.buf[in.cur] + used, left);
+                if (left && !rc && in.wv[0])
+                    rc = fx_memcpy_d2d(a.ctx, in.wv[1 - in.cur], in.wv[in.cur] + used / 4, left / 4);
                 in.cur = 1 - in.cur;
                 in.have = left;
             }

@@ -146,10 +146,11 @@ typedef struct {
  * the binary sinks: fft_points/2+1 float32 per spectrum of each input's
  * selected channel, the cross spectrum of the two (fft_points/2+1 complex
  * float32, fx_spectrometer_xpol) and the phase-cal (tonebins complex
- * float32).  The run ends when either input ends.  Frames that the blanker
- * marks invalid are counted by the placement but their samples are used as
- * they are: mark5access would decode them as zeros.  Returns 0 or an
- * FX_ERR_* code (fx_last_error()). */
+ * float32).  The run ends when either input ends.  Samples of framed inputs
+ * that the blanker marks invalid (fx_frames_place ranges; a missing frame is
+ * invalid too) decode as zeros, as mark5access decodes them, in the spectra,
+ * the cross spectrum and the phase-cal.  Returns 0 or an FX_ERR_* code
+ * (fx_last_error()). */
 int fx_swspectrometer_run(const fx_swspec_settings* settings, const char* input1, const char* input2,
                           fx_swspec_stats* stats);
 

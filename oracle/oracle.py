@@ -416,11 +416,13 @@ def pcal_delay(tones, tone0_hz, tone_step_hz, band_center_hz, band_valid_hz):
 
 
 def spectrometer_xpol_f64(words_a, nchan_a, chan_a, words_b, nchan_b, chan_b, ntime, n, overlap,
-                          integrate_ffts, window_kind, continuous=False):
+                          integrate_ffts, window_kind, continuous=False, valid_a=None, valid_b=None):
     """Cross spectrum of two 2-bit channels (TaskCoreIPP.cpp:456-487) in FP64:
     the same windowed, overlapped frames as the spectrometer; per spectrum
     (1/K) sum X_a conj(X_b) over bins 0..n/2, with DC and Nyquist the product
-    of the real parts (imaginary part 0).  Returns [nspec][n/2+1] complex128."""
+    of the real parts (imaginary part 0).  valid_a/valid_b: optional per-word
+    flags; samples of a word flagged 0 are zero (mark5access decodes blanked
+    data as 0).  Returns [nspec][n/2+1] complex128."""
     hop = n // overlap
     K = integrate_ffts
     if continuous:
@@ -431,6 +433,10 @@ def spectrometer_xpol_f64(words_a, nchan_a, chan_a, words_b, nchan_b, chan_b, nt
         nspec = ntime // stride
     xa = unpack_nch(words_a, ntime, nchan_a)[chan_a].astype(np.float64)
     xb = unpack_nch(words_b, ntime, nchan_b)[chan_b].astype(np.float64)
+    for x, v, nch in ((xa, valid_a, nchan_a), (xb, valid_b, nchan_b)):
+        if v is not None:
+            word_of_t = (np.arange(ntime) * nch) // 16     # word of each time step (same for every channel)
+            x[~np.asarray(v, bool)[word_of_t]] = 0.0
     w = window(window_kind, n).astype(np.float64)
     out = np.zeros((nspec, n // 2 + 1), dtype=np.complex128)
     for s in range(nspec):

@@ -143,6 +143,61 @@ def test_app_vdif_with_pcal(tmp_path):
         assert nrel(pc[sp], acc.astype(np.complex64)) < 1e-6
 
 
+def test_app_vdif_blanked_frames(tmp_path):
+    """Invalid-bit and fill-pattern VDIF frames decode as zeros in the spectra
+    and the phase-cal (mark5access blanking), against the oracle with the
+    same words zeroed."""
+    import fxlib
+    n, ov, K = 1024, 2, 5
+    fs, off, harm = 16e6, 500e3, 1e6
+    T, R = 16, 32
+    pb, nfr, fps = 8000, 60, 4000
+    ntime = nfr * pb * 4
+    words = O.gen_multichannel(23, 1, [0.3], [(off + 3 * harm) / fs], 0.9816, ntime)
+    payload = bytearray(words.view(np.uint8).tobytes())
+    bad_bit, bad_fill = {7, 30}, {44}
+    for k in bad_fill:                          # fill at the end of the payload
+        payload[(k + 1) * pb - 8:(k + 1) * pb] = FR.FILL64.to_bytes(8, "little")
+    frames = []
+    for k in range(nfr):
+        hdr = FR.vdif_encode(1000, k, pb + 32, thread=0, epoch=40, invalid=k in bad_bit)
+        frames.append(hdr + bytes(payload[k * pb:(k + 1) * pb]))
+    path = str(tmp_path / "bad.vdif")
+    with open(path, "wb") as fh:
+        fh.write(b"".join(frames))
+    base = str(tmp_path / "bl")
+    s = _settings(n, ov, K, O.WIN_HANN, 1, fxlib.FX_FMT_VDIF, base, pcal=1, fs=fs, off=off, harm=harm, chunk=9,
+                  framebytes=pb + 32, fps=fps)
+    st = TL.SwspecStats()
+    assert TL.load().fx_swspectrometer_run(s, path.encode(), None, st) == 0, fxlib.load().fx_last_error()
+    placed = np.frombuffer(bytes(payload), dtype="<u4")
+    valid = np.ones(placed.size, bool)
+    for k in bad_bit | bad_fill:
+        valid[k * pb // 4:(k + 1) * pb // 4] = False
+    L = (K - 1) * (n // ov) + n
+    nspec = ntime // L
+    assert st.spectra == nspec
+    spec = _read(base + "_a_swspec.bin", np.float32, n // 2 + 1)
+    ref = O.spectrometer_xpol_f64(placed, 1, 0, placed, 1, 0, ntime, n, ov, K, O.WIN_HANN, valid_a=valid,
+                                  valid_b=valid).real
+    assert spec.shape == ref.shape and nrel(spec, ref) < 1e-5
+    # spectra that touch a blanked frame differ from the unblanked ones
+    plain = O.spectrometer_ex_f32(placed, ntime, 1, n, ov, K, O.WIN_HANN, False, True)[:, 0, :]
+    touched = [sp for sp in range(nspec) if not valid[sp * L // 16:(sp * L + L) // 16].all()]
+    assert touched and all(nrel(spec[sp], plain[sp]) > 1e-3 for sp in touched)
+    pc = _read(base + "_a_pcal.bin", np.complex64, T)
+    x = O.unpack_1ch(placed, ntime)
+    x[~valid[np.arange(ntime) // 16]] = 0.0
+    hop = n // ov
+    for sp in range(nspec):
+        acc = np.zeros(T, np.complex128)
+        for j in range(K):
+            if (sp * K + j) % ov == 0:
+                st0 = sp * L + j * hop
+                O.tcpcal_fold(x[st0: st0 + n], T, R, -2 * np.pi * off / fs, acc)
+        assert nrel(pc[sp], acc.astype(np.complex64)) < 1e-6
+
+
 def test_app_mark5b(tmp_path):
     """A Mark5B file (10016-byte frames, sign/magnitude sample codes) crossing
     a second boundary: the spectra equal the oracle's over the samples the
