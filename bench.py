@@ -14,7 +14,8 @@ chunk give the per-kernel durations inside the timed region.  The headline
 X-engine is FP16 MFMA; the plain FP32 path and the scaled FP8 path run the
 same step beside it.
 
-Further sections: the visibility products (lag spectra, amplitude/phase),
+Further sections: 2-bit unpack, the 16-channel spectrometer workload (2^28
+samples, 32768-point FFTs), the visibility products (lag spectra, amplitude/phase),
 the VDIF front end (frame placement of ~1 GB of frames in HBM, and the
 double-buffered file ingest), a task-runtime chain per task vs as one
 hipGraph, the spectrometer application over two files, and the batch of
@@ -183,6 +184,51 @@ def large_n(fxlib, ctx, args):
     out["err_vs_fp64"] = err
     out["note"] = ("fp8_2bit runs three e4m3 GEMMs (S, S+T, T of X = S + (H-1)T), so its MFMA work is "
                    "3x the algorithmic count; tflops_equiv counts 8*P*C*T")
+    return out
+
+
+def spectrometer(fxlib, ctx, args):
+    """The software spectrometer's workload: 16 interleaved 2-bit subband
+    channels, 2^28 samples in all (2^24 per channel, 64 MiB packed, resident
+    in HBM), 32768-point real FFT per channel, power accumulated over the
+    whole stream (512 FFTs per channel, one spectrum of 16384+1 bins each),
+    window off, no overlap (fx_spectrometer_ex)."""
+    nch, n = 16, 32768
+    ntime = (1 << 28) // nch
+    K = ntime // n
+    dw = ctx.alloc(ntime * nch // 4)
+    rng = np.random.default_rng(args.seed + 9)
+    amp = np.zeros(nch)
+    freq = np.zeros(nch)
+    for c in rng.choice(nch, 3, replace=False):
+        amp[c], freq[c] = 0.2, rng.uniform(0.05, 0.45)
+    ctx.synth_multichannel(args.seed + 9, nch, amp, freq, 0.9816, ntime, dw)
+    dout = ctx.alloc(nch * (n // 2 + 1) * 4)
+    lib = fxlib.load()
+    ns = fxlib.C.c_long()
+
+    def run():
+        fxlib.check(lib.fx_spectrometer_ex(ctx.handle, dw.ptr, ntime, nch, n, 1, K, fxlib.FX_WIN_NONE, 0, 1,
+                                           dout.ptr, fxlib.C.byref(ns)))
+
+    run()
+    ctx.sync()
+    e = [ctx.event() for _ in range(2)]
+    reps = max(3, args.steps)
+    ctx.record(e[0])
+    for _ in range(reps):
+        run()
+    ctx.record(e[1])
+    ctx.sync()
+    ms = ctx.elapsed_ms(e[0], e[1]) / reps
+    nfft = nch * K
+    out = {"workload": "16 channels x 2^24 2-bit samples, 32768-pt real FFT, power over the whole stream",
+           "ms": ms, "ffts_per_s": nfft / (ms * 1e-3), "gsamples_per_s": (1 << 28) / (ms * 1e-3) / 1e9,
+           "fft_tflops": 2.5 * n * np.log2(n) * nfft / (ms * 1e-3) / 1e12, "spectra": int(ns.value)}
+    log(f"[bench] spectrometer: {ms:.2f} ms for 2^28 samples ({out['gsamples_per_s']:.0f} GS/s, "
+        f"{out['ffts_per_s']:.3g} FFT/s)")
+    dw.free()
+    dout.free()
     return out
 
 
@@ -583,7 +629,8 @@ def main():
         return
     if args.front_end_only:
         ctx = fxlib.Context(0)
-        print(json.dumps({"unpack": unpack(fxlib, ctx, args), "front_end": front_end(fxlib, ctx, args),
+        print(json.dumps({"unpack": unpack(fxlib, ctx, args), "spectrometer": spectrometer(fxlib, ctx, args),
+                          "front_end": front_end(fxlib, ctx, args),
                           "task_chains": task_chains(args),
                           "spectrometer_app": spectrometer_app(fxlib, args)}), flush=True)
         ctx.close()
@@ -718,6 +765,7 @@ def main():
     prod = products(fxlib, ctx, dvis, P, C, T, args)
     dvis.free()
     unp = unpack(fxlib, ctx, args)
+    spec = spectrometer(fxlib, ctx, args)
     fe = None if args.no_front_end else front_end(fxlib, ctx, args)
     chains = None if args.no_tasks else task_chains(args)
     app = None if args.no_front_end else spectrometer_app(fxlib, args)
@@ -802,6 +850,7 @@ def main():
         "batched_subbands": batch,
         "visibility_products": prod,
         "unpack": unp,
+        "spectrometer": spec,
         "front_end": fe,
         "task_chains": chains,
         "spectrometer_app": app,
