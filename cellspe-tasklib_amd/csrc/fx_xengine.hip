@@ -399,6 +399,157 @@ __global__ __launch_bounds__(XNT, 2) void k_xengine_ring(const typename XTraits<
 }
 
 // ---------------------------------------------------------------------------
+// Full 128 x 128 tiles of more inputs (ninp a multiple of 128): a stage holds
+// the KT/2-sample slabs of the tile's row block and column block, ring of 4
+// in LDS, filled by LDS-DMA loads as in k_xengine_ring; a slab's time rows
+// are ninp apart in HBM, so each lane computes its own source address.  The
+// waves split the tile into quadrants as k_xengine<PREC, 128> does (same
+// products in the same order, same engine image).
+// ---------------------------------------------------------------------------
+template <int PREC>
+__global__ __launch_bounds__(XNT, 2) void k_xengine_ring2(const typename XTraits<PREC>::E* __restrict__ X,
+                                                          int nchan, long nt, int ninp, float* __restrict__ acc)
+{
+    typedef typename XTraits<PREC>::E E;
+    constexpr int TSZ = 128, NBW = 2, NBT = 16;
+    constexpr int KT = XTraits<PREC>::KT / 2;            // samples per stage and slab
+    constexpr int RING = 4;
+    constexpr int ROWB = TSZ * (int)sizeof(E);           // bytes of one slab time row
+    constexpr int RPB = 1024 / ROWB;                     // time rows per 1 KB load
+    constexpr int LPR = 64 / RPB;                        // lanes per time row
+    constexpr int CHE = 16 / (int)sizeof(E);             // samples per lane
+    constexpr int BPS = KT * ROWB / 1024;                // 1 KB loads per slab
+    constexpr int NPER = 2 * BPS / (XNT / 64);           // per wave per stage
+    static_assert(NPER * (XNT / 64) == 2 * BPS && RPB * ROWB == 1024, "stage is not whole 1 KB loads per wave");
+    __shared__ E slab[RING][2][KT][TSZ];
+
+    const int nb = ninp / TSZ;
+    const int ntl = nb * (nb + 1) / 2;
+    const long lb = xcd_block((long)ntl * nchan);
+    if (lb >= (long)ntl * nchan) return;
+    const int c = (int)(lb / ntl);
+    const int tile = (int)(lb - (long)c * ntl);
+    int ti, tj;
+    tile_coords(tile, nb, ti, tj);
+    const bool diag = ti == tj;
+    const int tid = threadIdx.x;
+    const int w = tid >> 6, lane = tid & 63;
+    const int qi = w >> 1, qj = w & 1;
+    const bool active = !(diag && qi > qj);
+    const E* base = X + (size_t)c * (size_t)nt * (size_t)ninp;
+    const long nstage = nt / KT;
+    // this lane's part of every 1 KB load: time row within the block, column
+    const int trow = lane / LPR, col = (lane % LPR) * CHE;
+
+    auto issue = [&](long st) {
+#pragma unroll
+        for (int i = 0; i < NPER; i++) {
+            const int blk = w * NPER + i;                // 0 .. 2 BPS - 1
+            const int sl = blk / BPS, b = blk - sl * BPS;
+            const long t = st * KT + b * RPB + trow;
+            const char* src = reinterpret_cast<const char*>(base + (size_t)t * ninp + (sl ? tj : ti) * TSZ + col);
+            char* dst = reinterpret_cast<char*>(&slab[st % RING][sl][0][0]) + b * 1024;
+            __builtin_amdgcn_global_load_lds(src, (__attribute__((address_space(3))) void*)dst, 16, 0, 0);
+        }
+    };
+
+    fx_float16 dre[NBW][NBW], dim[NBW][NBW];
+#pragma unroll
+    for (int a = 0; a < NBW; a++)
+#pragma unroll
+        for (int b = 0; b < NBW; b++) {
+            dre[a][b] = fx_float16{};
+            dim[a][b] = fx_float16{};
+        }
+    const int r32 = lane & 31, h = lane >> 5;
+    for (long st = 0; st < RING - 1 && st < nstage; st++) issue(st);
+    for (long st = 0; st < nstage; st++) {
+        const long ahead = nstage - 1 - st;
+        if (ahead >= RING - 2) wait_vmcnt<(RING - 2) * NPER>();
+        else if (ahead == 1) wait_vmcnt<NPER>();
+        else wait_vmcnt<0>();
+        asm volatile("s_barrier" ::: "memory");
+        if (st + RING - 1 < nstage) issue(st + RING - 1);
+        if (!active) continue;
+        const E (*sa)[TSZ] = slab[st % RING][0];
+        const E (*sb)[TSZ] = slab[st % RING][1];
+        if constexpr (PREC == FX_PREC_F16) {
+#pragma unroll
+            for (int ks = 0; ks < KT / 8; ks++) {
+                fx_half8 fa[NBW], fb[NBW], fr[NBW];
+#pragma unroll
+                for (int a = 0; a < NBW; a++) {
+                    fx_int4v ia;
+                    const int row = (NBW * qi + a) * 32 + r32;
+#pragma unroll
+                    for (int q = 0; q < 4; q++) ia[q] = (int)sa[ks * 8 + 4 * h + q][row];
+                    fa[a] = __builtin_bit_cast(fx_half8, ia);
+                }
+#pragma unroll
+                for (int b = 0; b < NBW; b++) {
+                    fx_int4v ib, ir;
+                    const int row = (NBW * qj + b) * 32 + r32;
+#pragma unroll
+                    for (int q = 0; q < 4; q++) {
+                        uint32_t v = sb[ks * 8 + 4 * h + q][row];
+                        ib[q] = (int)v;
+                        ir[q] = (int)rot_h2(v);
+                    }
+                    fb[b] = __builtin_bit_cast(fx_half8, ib);
+                    fr[b] = __builtin_bit_cast(fx_half8, ir);
+                }
+#pragma unroll
+                for (int a = 0; a < NBW; a++)
+#pragma unroll
+                    for (int b = 0; b < NBW; b++) {
+                        if (diag && qi == qj && a > b) continue;
+                        dre[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_f16(fa[a], fb[b], dre[a][b], 0, 0, 0);
+                        dim[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_f16(fa[a], fr[b], dim[a][b], 0, 0, 0);
+                    }
+            }
+        } else {
+#pragma unroll
+            for (int ks = 0; ks < KT; ks++) {
+                float fa[NBW], fb[NBW], fr[NBW];
+#pragma unroll
+                for (int a = 0; a < NBW; a++) {
+                    const float2 v = sa[ks][(NBW * qi + a) * 32 + r32];
+                    fa[a] = h ? v.y : v.x;
+                }
+#pragma unroll
+                for (int b = 0; b < NBW; b++) {
+                    const float2 v = sb[ks][(NBW * qj + b) * 32 + r32];
+                    fb[b] = h ? v.y : v.x;
+                    fr[b] = h ? v.x : -v.y;
+                }
+#pragma unroll
+                for (int a = 0; a < NBW; a++)
+#pragma unroll
+                    for (int b = 0; b < NBW; b++) {
+                        if (diag && qi == qj && a > b) continue;
+                        dre[a][b] = __builtin_amdgcn_mfma_f32_32x32x2f32(fa[a], fb[b], dre[a][b], 0, 0, 0);
+                        dim[a][b] = __builtin_amdgcn_mfma_f32_32x32x2f32(fa[a], fr[b], dim[a][b], 0, 0, 0);
+                    }
+            }
+        }
+    }
+    if (!active) return;
+    float* dst = acc + (((size_t)c * ntl + tile) * NBT + w * NBW * NBW) * (32 * 64);
+#pragma unroll
+    for (int a = 0; a < NBW; a++)
+#pragma unroll
+        for (int b = 0; b < NBW; b++) {
+            if (diag && qi == qj && a > b) continue;
+            float* d = dst + (a * NBW + b) * (32 * 64);
+#pragma unroll
+            for (int r = 0; r < 16; r++) {
+                d[r * 64 + lane] += dre[a][b][r];
+                d[(16 + r) * 64 + lane] += dim[a][b][r];
+            }
+        }
+}
+
+// ---------------------------------------------------------------------------
 // 8-bit engine: v_mfma_scale_f32_32x32x64_f8f6f4 with unit (E8M0 = 127)
 // scales, K = 64 e4m3 values = 32 complex samples per instruction.
 //
@@ -678,6 +829,19 @@ static bool ring_enabled()
 int fxk_xengine(hipStream_t s, int in_type, int ts, const void* X, int nchan, long nt, int ninp, float* acc)
 {
     long nblk = xcd_grid((long)ntiles_ts(ninp, ts) * nchan);
+    // whole 128-input tiles of more inputs and whole stages: the two-slab LDS ring
+    if (ts == 128 && ninp > 128 && ninp % 128 == 0 && ((uintptr_t)X % 16) == 0 && ring_enabled()) {
+        if (in_type == FX_PREC_F16 && nt % (XTraits<FX_PREC_F16>::KT / 2) == 0) {
+            hipLaunchKernelGGL(k_xengine_ring2<FX_PREC_F16>, dim3((unsigned)nblk), dim3(XNT), 0, s,
+                               reinterpret_cast<const uint32_t*>(X), nchan, nt, ninp, acc);
+            return hipGetLastError() == hipSuccess ? FX_OK : FX_ERR_HIP;
+        }
+        if (in_type == FX_PREC_F32 && nt % (XTraits<FX_PREC_F32>::KT / 2) == 0) {
+            hipLaunchKernelGGL(k_xengine_ring2<FX_PREC_F32>, dim3((unsigned)nblk), dim3(XNT), 0, s,
+                               reinterpret_cast<const float2*>(X), nchan, nt, ninp, acc);
+            return hipGetLastError() == hipSuccess ? FX_OK : FX_ERR_HIP;
+        }
+    }
     // one full tile and whole stages: the LDS-ring engine
     if (ts == 128 && ninp == 128 && ((uintptr_t)X % 16) == 0 && ring_enabled()) {
         if (in_type == FX_PREC_F16 && nt % XTraits<FX_PREC_F16>::KT == 0) {
