@@ -431,7 +431,8 @@ def spectrometer_app(fxlib, args):
         s.format = fxlib.FX_FMT_RAW
         s.use_channel_file[0] = s.use_channel_file[1] = 1
         s.calc_xpol, s.spectra_per_chunk = 1, 8
-        s.basefilename[0] = os.path.join(d, "out").encode()
+        s.basefilename[0] = os.path.join(d, "out1").encode()
+        s.basefilename[1] = os.path.join(d, "out2").encode()
         st = TL.SwspecStats()
         rc = TL.load().fx_swspectrometer_run(s, paths[0].encode(), paths[1].encode(), st)
         if rc != 0:

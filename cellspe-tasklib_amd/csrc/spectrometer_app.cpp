@@ -173,7 +173,9 @@ extern "C" int fx_swspectrometer_run(const fx_swspec_settings* s, const char* in
         FX_REQUIRE(s->use_channel_file[i] >= 1 && s->use_channel_file[i] <= nch, FX_ERR_ARG,
                    "fx_swspectrometer_run: use_channel_file[%d] = %d", i, s->use_channel_file[i]);
     const bool xpol = s->calc_xpol && ninp == 2;
-    FX_REQUIRE(!xpol || s->basefilename[0], FX_ERR_ARG, "fx_swspectrometer_run: cross spectrum needs a name");
+    // each input writes its own sinks (swspectrometer.cpp:250-254)
+    FX_REQUIRE(ninp == 1 || (s->basefilename[1] && strcmp(s->basefilename[0], s->basefilename[1]) != 0), FX_ERR_ARG,
+               "fx_swspectrometer_run: two inputs need two different sink names");
     // phase-cal as the settings derive it (swspectrometer.cpp:178-196)
     int T = 0, R = 0;
     if (c.extract_pcal) {
@@ -243,7 +245,7 @@ extern "C" int fx_swspectrometer_run(const fx_swspec_settings* s, const char* in
                 a.d_any = (int*)p;
             }
         }
-        const char* base = s->basefilename[i] ? s->basefilename[i] : s->basefilename[0];
+        const char* base = s->basefilename[i];
         if (!rc && !(in.sink = open_sink(base, "_swspec.bin"))) rc = FX_ERR_ARG;
         if (!rc && T > 0 && !(in.pcal_sink = open_sink(base, "_pcal.bin"))) rc = FX_ERR_ARG;
         if (!rc) rc = fx_ingest_open(a.ctx, names[i], 0, chunk_bytes, 2, &in.g);

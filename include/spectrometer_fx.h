@@ -126,7 +126,8 @@ typedef struct {
     int calc_xpol;                  /* cross spectrum of the two selected channels */
     int spectra_per_chunk;          /* whole spectra per double-buffered input chunk */
     const char* basefilename[2];    /* sink names: <base>_swspec.bin, <base>_pcal.bin,
-                                       <base 1>_xpol_swspec.bin; [1] NULL: [0] */
+                                       <base 1>_xpol_swspec.bin; with two inputs [1] must be
+                                       given and differ from [0] (else FX_ERR_ARG) */
 } fx_swspec_settings;
 
 typedef struct {

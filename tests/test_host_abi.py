@@ -174,6 +174,18 @@ def test_error_paths_without_device():
     cbs = (TL.MiniControlBlock * 1)()
     assert lib.minicorr_run(cbs, 1, None) == -1
     assert "stations" in TL.last_error()
+    # spectrometer application: two inputs need two sink names (both sinks
+    # of one name would be opened twice and overwrite each other)
+    sw = TL.SwspecSettings()
+    sw.core = TL.SpectrometerSettings(fft_points=1024, fft_overlap_factor=1, core_overlapped_ffts=4, window=0,
+                                      bits_per_sample=2, source_channels=1, use_channel=0, extract_pcal=0,
+                                      samplingfreq=16e6, pcaloffsethz=10e3, pcalharmonicshz=1e6)
+    sw.format, sw.spectra_per_chunk = fxlib.FX_FMT_RAW, 1
+    sw.use_channel_file[0] = sw.use_channel_file[1] = 1
+    for second in (None, b"same"):
+        sw.basefilename[0], sw.basefilename[1] = b"same", second
+        assert lib.fx_swspectrometer_run(C.byref(sw), b"in1", b"in2", None) == -1   # FX_ERR_ARG
+        assert "sink names" in TL.last_error()
 
 
 def test_product_path_fails_loudly_without_device():
