@@ -198,6 +198,16 @@ extern "C" int fx_memset(fx_ctx* ctx, void* dst, int value, size_t bytes)
     int rc = ctx_device(ctx);
     if (rc) return rc;
     if (!bytes) return FX_OK;
+    // whole aligned words through the fill kernel, so the clear can be
+    // captured into a hipGraph (memset nodes are kept out of graphs)
+    if ((uintptr_t)dst % 4 == 0 && bytes % 4 == 0) {
+        const uint32_t b = (uint32_t)(value & 0xff);
+        if (fxk_fill32(ctx->stream, dst, b * 0x01010101u, bytes / 4) != FX_OK) {
+            fx_set_error("fx_memset: fill launch failed");
+            return FX_ERR_HIP;
+        }
+        return FX_OK;
+    }
     FX_HIP_TRY(hipMemsetAsync(dst, value, bytes, ctx->stream));
     return FX_OK;
 }

@@ -84,7 +84,8 @@ int fx_free(fx_ctx* ctx, void* dptr);
 int fx_memcpy_h2d(fx_ctx* ctx, void* dst, const void* src, size_t bytes);   /* synchronous */
 int fx_memcpy_d2h(fx_ctx* ctx, void* dst, const void* src, size_t bytes);   /* synchronous */
 int fx_memcpy_d2d(fx_ctx* ctx, void* dst, const void* src, size_t bytes);   /* async */
-int fx_memset(fx_ctx* ctx, void* dst, int value, size_t bytes);             /* async */
+int fx_memset(fx_ctx* ctx, void* dst, int value, size_t bytes);             /* async; a fill kernel when dst
+                                                                               and bytes are 4-byte aligned */
 int fx_event_create(fx_ctx* ctx, void** ev);
 int fx_event_record(fx_ctx* ctx, void* ev);
 int fx_event_elapsed_ms(void* ev_start, void* ev_stop, float* ms);          /* syncs stop */

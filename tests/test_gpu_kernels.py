@@ -265,3 +265,25 @@ def test_pcal_2bit_inputs(ctx, bw, spacing, offset):
         ref = O.pcal_final(cfg, O.pcal_extract(cfg, O.unpack_1ch(rows[i], nsamp), 0))
         assert nrel(got[i], ref) < 1e-12, i
     pc.close()
+
+
+def test_memset_fill_and_bytes(ctx):
+    """fx_memset: aligned whole words go through the fill kernel, other
+    ranges through the byte memset; both give the byte pattern and touch
+    nothing outside the range."""
+    import ctypes as C
+    import fxlib
+    lib = fxlib.load()
+    buf = ctx.alloc(4096)
+    buf.upload(np.full(4096, 0x11, np.uint8))
+    base = buf.ptr.value
+    assert lib.fx_memset(ctx.handle, C.c_void_p(base + 1024), 0x5A, 2048) == 0      # fill kernel
+    assert lib.fx_memset(ctx.handle, C.c_void_p(base + 3), 0xC3, 7) == 0            # byte memset
+    assert lib.fx_memset(ctx.handle, C.c_void_p(base + 4000), 0x1FF, 8) == 0        # low byte only
+    got = buf.download(np.uint8)
+    want = np.full(4096, 0x11, np.uint8)
+    want[1024:3072] = 0x5A
+    want[3:10] = 0xC3
+    want[4000:4008] = 0xFF
+    assert np.array_equal(got, want)
+    buf.free()
