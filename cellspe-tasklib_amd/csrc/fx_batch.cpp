@@ -92,7 +92,9 @@ extern "C" int fx_autotune(fx_ctx* ctx, int nfft, int ninp, int prec, const char
     FX_REQUIRE(ctx && nfft >= 256 && ninp >= 1, FX_ERR_ARG, "fx_autotune: bad shape");
     FX_REQUIRE(prec == FX_PREC_F32 || prec == FX_PREC_F16 || prec == FX_PREC_FP8, FX_ERR_ARG,
                "fx_autotune: precision %d has no F-stage", prec);
-    const long T = 512;   // spectra per timed run
+    // spectra per timed run: enough for the deepest sub-chunk candidate of a
+    // short transform, whose many small launches otherwise go unmeasured
+    const long T = nfft <= 1024 ? 4096 : 512;
     const size_t wpi = (size_t)T * nfft / 16;
     void* words = nullptr;
     int rc = fx_malloc(ctx, (size_t)ninp * wpi * 4, &words);
@@ -106,7 +108,7 @@ extern "C" int fx_autotune(fx_ctx* ctx, int nfft, int ninp, int prec, const char
     void *e0 = nullptr, *e1 = nullptr;
     if (!rc) rc = fx_event_create(ctx, &e0);
     if (!rc) rc = fx_event_create(ctx, &e1);
-    const long subs[] = {64, 256, 1024};
+    const long subs[] = {64, 256, 1024, 4096};
     const int tiles[] = {128, 64};
     long best_sub = 0;
     int best_tile = 128;
