@@ -266,9 +266,12 @@ __device__ __forceinline__ uint16_t to_chan<uint16_t>(float2 v)
 // FP32), one complex multiply per channel.  The item's amplitude factor
 // amp = 1/2 (of the real split) x the uniform channel scale x the input scale
 // rides in ph_lo (or multiplies each channel when there is no phase model).
+// The split keeps PLO + PHI (the FP64 sincospi evaluations per item) near
+// its minimum 2 sqrt(M): PLO = 64 from M = 2048 up, 32 for M = 512 and 1024,
+// 16 below (M = 128: 24 entries per item rather than 66).
 template <int M>
 struct PhaseTab {
-    static constexpr int PLO = M < 64 ? M : 64;
+    static constexpr int PLO = M >= 2048 ? 64 : M >= 512 ? 32 : M >= 16 ? 16 : M;
     static constexpr int PHI = M / PLO;
 };
 
