@@ -499,6 +499,19 @@ def task_chains(args):
     return out
 
 
+def batch_draw(seed, nsub, nsamp=1 << 22):
+    """The uneven-subband draw: per subband an FFT length from
+    {256, 1024, 4096, 16384} and 8..48 stations (x2 polarizations); returns
+    [(nfft, ninp, nspectra)] for nsamp samples per input."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(nsub):
+        n = int(rng.choice([256, 1024, 4096, 16384]))
+        S = int(rng.integers(8, 49))
+        out.append((n, 2 * S, nsamp // n))
+    return out
+
+
 def batched(fxlib, ctx, args):
     """32 subbands with FFT lengths drawn from {256,1024,4096,16384}, station
     counts from 8..48 (x2 pol), rho = 0.03, 2^22 samples per input, FP64
@@ -506,18 +519,13 @@ def batched(fxlib, ctx, args):
     injected per subband): one hipGraph launch of the autotuned FP16 batch vs
     the same subbands one launch each in plain FP32."""
     import tempfile
-    rng = np.random.default_rng(args.seed + 4)
     nsub = args.batch_subbands
     nsamp = 1 << 22
     subs = []
     bufs = []
     tot_cmac = 0.0
     tot_samp = 0.0
-    for k in range(nsub):
-        n = int(rng.choice([256, 1024, 4096, 16384]))
-        S = int(rng.integers(8, 49))
-        I = 2 * S
-        T = nsamp // n
+    for k, (n, I, T) in enumerate(batch_draw(args.seed + 4, nsub, nsamp)):
         nw = nsamp // 16
         w = ctx.alloc(I * nw * 4)
         for i in range(I):
@@ -666,6 +674,7 @@ def main():
         corr.set_phase_model(models)
         dchan = ctx.alloc(C * chunk * I * esz)
         nf = corr.fstage_launches(chunk, prec)
+        xname = corr.xengine_name(chunk, dchan)
         evf = [ctx.event(), ctx.event()]
         evx = [ctx.event(), ctx.event()]
 
@@ -700,13 +709,13 @@ def main():
         x = ctx.elapsed_ms(evx[0], evx[1])
         dchan.free()
         corr.close()
-        return ms, {"fstage": f, "xengine": x, "fstage_launches": nf}
+        return ms, {"fstage": f, "xengine": x, "fstage_launches": nf, "xengine_kernel": xname}
 
     paths = {}
     ms16, k16 = run_path(fxlib.FX_PREC_F16, 4, args.steps, args.warmup)
     paths["f16"] = (ms16, k16)
     log(f"[bench] fp16 path: {ms16:.2f} ms/step; per chunk " +
-        ", ".join(f"{k} {v}" if k == "fstage_launches" else f"{k} {v:.2f} ms" for k, v in k16.items()))
+        ", ".join(f"{k} {v}" if isinstance(v, (int, str)) else f"{k} {v:.2f} ms" for k, v in k16.items()))
     if not args.no_fp32:
         paths["f32"] = run_path(fxlib.FX_PREC_F32, 8, max(1, args.steps - 1), 1)
         log(f"[bench] fp32 path: {paths['f32'][0]:.2f} ms/step")
@@ -723,12 +732,16 @@ def main():
     if not args.no_check:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle as O
-        te = 16
+        # a multiple of every engine's stage depth, so the error is measured on
+        # the same X-engine kernel the timed steps run
+        te = 32
         streams = [dw.download_at(i * nwords * 4, np.uint32, te * N // 16) for i in range(I)]
         got = {}
+        err_kernels = {}
         for name, prec in (("f16", fxlib.FX_PREC_F16), ("f32", fxlib.FX_PREC_F32), ("fp8", fxlib.FX_PREC_FP8)):
             c = fxlib.Correlator(ctx, I, N, te, prec, use_phase=True)
             c.set_phase_model(models)
+            err_kernels[name] = c.xengine_name(te)
             c.process(dw, nwords, 0, te)
             got[name] = c.visibilities()
             c.close()
@@ -738,8 +751,13 @@ def main():
         log(f"[bench] FP64 oracle on {te} spectra in {time.time() - t0:.1f}s")
         err = {k: nrel(v, ref64) for k, v in got.items()}
         err["sample"] = f"first {te} spectra of all {I} inputs, all {P} products x {C} channels"
+        err["xengine_kernels"] = err_kernels
+        for name, (_ms, kt) in paths.items():
+            if kt["xengine_kernel"] != err_kernels[name]:
+                raise SystemExit(f"bench: the {name} error sample ran {err_kernels[name]}, "
+                                 f"the timed steps {kt['xengine_kernel']}")
         log(f"[bench] normwise error vs FP64 oracle: " +
-            ", ".join(f"{k} {v:.2e}" for k, v in err.items() if k != "sample"))
+            ", ".join(f"{k} {v:.2e}" for k, v in err.items() if isinstance(v, float)))
         del ref64, got
 
     # ---- CPU baseline (oracle FP32 generic-C chain, all host threads) --------
@@ -786,7 +804,7 @@ def main():
     dom = max(("fstage", "xengine"), key=lambda k: k16[k])
     if dom == "xengine":
         launch = k16["xengine"]
-        roof = {"kernel": "k_xengine_ring<1>" if I == 128 else "k_xengine<1, 128>", "bound": "mfma",
+        roof = {"kernel": k16["xengine_kernel"], "bound": "mfma",
                 "achieved": x_flop / (launch * 1e-3) / 1e12, "peak": PEAK["fp16"], "unit": "TFLOP/s",
                 "algorithmic": f"8*P*C*chunk = {x_flop:.3e} flop per launch"}
     else:

@@ -44,6 +44,7 @@ struct ChainGraph {
     size_t pinned_cap = 0;
     std::vector<Region> regions;
     unsigned fft_n_after = 0;         // host FFT state after the chain
+    unsigned long slot_gen = 0;       // the unit's slot generation the graph was captured with
 };
 
 struct Unit {
@@ -62,6 +63,9 @@ struct Unit {
     size_t cap[2 * CELLDSP_MAX_NUM_IOVECTORS + 2] = {};
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     unsigned int fft_n = 0;   // continuable FFT size (0 = none)
+    // bumped whenever a slot is freed or reallocated: a captured graph holds
+    // the slot addresses, so it is valid only for the generation it saw
+    unsigned long slot_gen = 0;
     ChainGraph cg;
 };
 
@@ -102,6 +106,7 @@ bool unit_device(Unit& u)
 void* unit_slot(Unit& u, int s, size_t bytes)
 {
     if (u.cap[s] >= bytes && u.slot[s]) return u.slot[s];
+    u.slot_gen++;
     if (u.slot[s]) hipFree(u.slot[s]);
     u.slot[s] = nullptr;
     u.cap[s] = 0;
@@ -483,6 +488,7 @@ bool chain_capture(Unit& u, CellDSPTask_t* head)
     }
     hipStream_t s = fx_ctx_hip_stream(u.ctx);
     const unsigned fft_before = u.fft_n;
+    const unsigned long gen = u.slot_gen;
     // the slots were sized by the eager run of this chain, so nothing below allocates
     if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess) {
         hipGetLastError();
@@ -495,6 +501,8 @@ bool chain_capture(Unit& u, CellDSPTask_t* head)
     if (hipStreamEndCapture(s, &graph) != hipSuccess) ok = false;
     cg.fft_n_after = u.fft_n;
     u.fft_n = fft_before;
+    if (u.slot_gen != gen) ok = false;   // a slot moved under the capture
+    cg.slot_gen = u.slot_gen;
     if (ok && graph) ok = hipGraphInstantiate(&cg.exec, graph, nullptr, nullptr, 0) == hipSuccess;
     if (graph) hipGraphDestroy(graph);
     if (!ok) {
@@ -541,6 +549,13 @@ bool run_chain_graph(Unit& u, CellDSPTask_t* head, int* rc)
     if (sig != cg.sig) {
         chain_graph_reset(cg);
         cg.sig.swap(sig);
+    }
+    if (cg.exec && cg.slot_gen != u.slot_gen) {
+        // another task on this unit grew a slot the graph reads or writes:
+        // drop the graph, run this execution eagerly, capture again next time
+        hipGraphExecDestroy(cg.exec);
+        cg.exec = nullptr;
+        cg.seen = 0;
     }
     if (!cg.exec) {
         if (cg.broken || cg.seen++ == 0) return false;     // first run eager: sizes the slots
@@ -808,11 +823,23 @@ extern "C" int speNOP(void)
     return celldsp_executeTask(nop.getTask());
 }
 
-extern "C" int speIOCheck(float* out4)
+extern "C" int speIOCheck_out(float* out4)
 {
     CellDSPTaskClass io(CELLDSP_TASK_IOTEST);
     io.addOutVec((void*)out4, 4 * sizeof(float));
     return celldsp_executeTask(io.getTask());
+}
+
+// cellspe-tasklib-dsp.cpp:111-124: the IOTEST task into a static 8-float
+// buffer, then the values it wrote back are printed
+extern "C" int speIOCheck(void)
+{
+    alignas(128) static float output[8];
+    const int status = speIOCheck_out(output);
+    printf("speIOCheck wrote back : vect={%g,%g,%g,%g}   expected would be {0.1,2.3,4.5,6.7}\n", output[0],
+           output[1], output[2], output[3]);
+    fflush(stdout);
+    return status;
 }
 
 extern "C" int speAddProduct_32fc(fc32* src, fc32* src2, fc32* acc, int length)

@@ -834,6 +834,7 @@ struct fx_corr {
     float* coef = nullptr;  // FP8_2BIT: weights of the three accumulator sets
     int nsets = 1;
     int xtile = 128;        // X-engine input tile (128, or 64 for FP32/FP16 engines)
+    bool ring = true;       // the LDS-ring X-engines may run (FX_XENGINE_AUTO)
     long spectra_done = 0;
 };
 
@@ -841,7 +842,8 @@ struct fx_corr {
 static int corr_engine(fx_corr* c, const void* d_chan, long nt)
 {
     if (c->cfg.prec == FX_PREC_F32 || c->cfg.prec == FX_PREC_F16)
-        return fxk_xengine(c->ctx->stream, c->cfg.prec, c->xtile, d_chan, c->nchan, nt, c->cfg.ninp, c->acc);
+        return fxk_xengine(c->ctx->stream, c->cfg.prec, c->xtile, d_chan, c->nchan, nt, c->cfg.ninp, c->acc,
+                           c->ring);
     return fxk_xengine8(c->ctx->stream, c->cfg.prec, d_chan, c->nchan, nt, c->cfg.ninp, c->acc);
 }
 
@@ -891,6 +893,7 @@ extern "C" int fx_corr_create(fx_ctx* ctx, const fx_corr_config* cfg, fx_corr** 
     c->nat_bytes = twobit ? 0 : row_bytes * (size_t)c->sub;
     const size_t nat_bytes = c->nat_bytes;
     c->xtile = 128;
+    c->ring = fxk_xengine_ring_default();
     c->acc_floats = fxk_xengine_acc_floats(c->nchan, cfg->ninp, c->nsets, c->xtile);
     long P = (long)cfg->ninp * (cfg->ninp + 1) / 2;
     std::vector<int2> hp((size_t)P);
@@ -1201,6 +1204,27 @@ extern "C" int fx_corr_tune(fx_corr* c, long sub_spectra, int xtile)
         }
         c->sub = sub;
     }
+    return FX_OK;
+}
+
+extern "C" int fx_corr_set_xengine(fx_corr* c, int kind)
+{
+    FX_REQUIRE(c, FX_ERR_ARG, "null correlator");
+    FX_REQUIRE(kind == FX_XENGINE_AUTO || kind == FX_XENGINE_PREFETCH, FX_ERR_ARG, "fx_corr_set_xengine: kind %d",
+               kind);
+    c->ring = kind == FX_XENGINE_AUTO;
+    return FX_OK;
+}
+
+extern "C" int fx_corr_xengine_name(fx_corr* c, const void* d_chan, long nspectra, char* buf, int len)
+{
+    FX_REQUIRE(c && buf && len > 0, FX_ERR_ARG, "fx_corr_xengine_name: null argument");
+    const char* name = c->cfg.prec == FX_PREC_FP8        ? "k_xengine8<1>"
+                     : c->cfg.prec == FX_PREC_FP8_2BIT ? "k_xengine8<3>"
+                                                         : fxk_xengine_name(c->cfg.prec, c->xtile,
+                                                                            d_chan ? d_chan : c->chan, nspectra,
+                                                                            c->cfg.ninp, c->ring);
+    snprintf(buf, (size_t)len, "%s", name);
     return FX_OK;
 }
 

@@ -8,9 +8,10 @@
 // subband and the branches run concurrently; fx_batch_run replays it with one
 // hipGraphLaunch.  All clears inside it are fill kernels (no memset nodes).
 //
-// The autotuner times the F-stage + X-engine of a subband shape on synthetic
-// 2-bit input for each (sub-chunk depth, X tile) candidate with HIP events and
-// caches the fastest per (nfft, ninp, precision) in a text file.
+// The autotuner times the F-stage + X-engine of one chunk of a subband shape
+// on synthetic 2-bit input for each (sub-chunk depth, X tile) candidate with
+// HIP events and caches the fastest per (nfft, ninp, precision, chunk) in a
+// text file.
 #include "fx_internal.h"
 
 #include <cstdio>
@@ -34,7 +35,7 @@ long round8(long n) { return (n + 7) / 8 * 8; }
 
 long default_chunk(const fx_subband_desc& d) { return std::min<long>(4096, round8(std::max<long>(d.nspectra, 8))); }
 
-bool lookup(const char* file, int nfft, int ninp, int prec, long* sub, int* xtile)
+bool lookup(const char* file, int nfft, int ninp, int prec, long chunk, long* sub, int* xtile)
 {
     if (!file) return false;
     FILE* f = fopen(file, "r");
@@ -43,11 +44,11 @@ bool lookup(const char* file, int nfft, int ninp, int prec, long* sub, int* xtil
     bool found = false;
     while (fgets(line, sizeof(line), f)) {
         int n, i, p, x;
-        long s;
+        long ch, s;
         double ms;
         if (line[0] == '#') continue;
-        if (sscanf(line, "%d %d %d %ld %d %lf", &n, &i, &p, &s, &x, &ms) == 6 && n == nfft && i == ninp &&
-            p == prec) {
+        if (sscanf(line, "%d %d %d %ld %ld %d %lf", &n, &i, &p, &ch, &s, &x, &ms) == 7 && n == nfft &&
+            i == ninp && p == prec && ch == chunk) {
             *sub = s;
             *xtile = x;
             found = true;   // the last entry wins
@@ -86,15 +87,16 @@ struct fx_batch {
     hipGraphExec_t exec = nullptr;
 };
 
-extern "C" int fx_autotune(fx_ctx* ctx, int nfft, int ninp, int prec, const char* tune_file, long* sub_out,
-                           int* xtile_out)
+extern "C" int fx_autotune(fx_ctx* ctx, int nfft, int ninp, int prec, long chunk_spectra, const char* tune_file,
+                           long* sub_out, int* xtile_out)
 {
     FX_REQUIRE(ctx && nfft >= 256 && ninp >= 1, FX_ERR_ARG, "fx_autotune: bad shape");
     FX_REQUIRE(prec == FX_PREC_F32 || prec == FX_PREC_F16 || prec == FX_PREC_FP8, FX_ERR_ARG,
                "fx_autotune: precision %d has no F-stage", prec);
-    // spectra per timed run: enough for the deepest sub-chunk candidate of a
-    // short transform, whose many small launches otherwise go unmeasured
-    const long T = nfft <= 1024 ? 4096 : 512;
+    FX_REQUIRE(chunk_spectra >= 8 && chunk_spectra % 8 == 0, FX_ERR_ARG,
+               "fx_autotune: chunk of %ld spectra (a positive multiple of 8)", chunk_spectra);
+    // timed on one chunk of the shape that will run
+    const long T = chunk_spectra;
     const size_t wpi = (size_t)T * nfft / 16;
     void* words = nullptr;
     int rc = fx_malloc(ctx, (size_t)ninp * wpi * 4, &words);
@@ -142,7 +144,7 @@ extern "C" int fx_autotune(fx_ctx* ctx, int nfft, int ninp, int prec, const char
     if (tune_file) {
         FILE* f = fopen(tune_file, "a");
         if (f) {
-            fprintf(f, "%d %d %d %ld %d %.4f\n", nfft, ninp, prec, best_sub, best_tile, best / 2.0f);
+            fprintf(f, "%d %d %d %ld %ld %d %.4f\n", nfft, ninp, prec, T, best_sub, best_tile, best / 2.0f);
             fclose(f);
         }
     }
@@ -202,8 +204,8 @@ extern "C" int fx_batch_create(fx_ctx* ctx, const fx_subband_desc* sb, int nsub,
         if (!rc && d.pm) rc = fx_corr_set_phase_model(r.corr, d.pm);
         long sub = 0;
         int xtile = 128;
-        if (!rc && tune_file && !lookup(tune_file, d.nfft, d.ninp, prec, &sub, &xtile))
-            rc = fx_autotune(r.ctx, d.nfft, d.ninp, prec, tune_file, &sub, &xtile);
+        if (!rc && tune_file && !lookup(tune_file, d.nfft, d.ninp, prec, r.chunk, &sub, &xtile))
+            rc = fx_autotune(r.ctx, d.nfft, d.ninp, prec, r.chunk, tune_file, &sub, &xtile);
         if (!rc && tune_file) rc = fx_corr_tune(r.corr, sub, xtile);
         if (!rc) {
             r.sub = fx_corr_fstage_depth(r.corr);

@@ -369,9 +369,15 @@ bool fxk_fstage_direct_ok(int n, int ninp, long nt, int out_type, long ostride, 
 // out == nullptr stops after the F kernel)
 int fxk_cornerturn(hipStream_t s, int out_type, const void* nat, void* out, long nitems, int nchan,
                    long ostride, long col0, int ns, long ostride_i);
-// FP32/FP16 engine on [c][t][i], ts x ts input tiles (ts = 128 or 64)
+// FP32/FP16 engine on [c][t][i], ts x ts input tiles (ts = 128 or 64); ring:
+// the LDS-ring engines may run (whole 128-input tiles and whole stages),
+// else the register-prefetch engine runs for every shape
 int fxk_xengine(hipStream_t s, int in_type, int ts, const void* X, int nchan, long nt, int ninp,
-                float* acc);
+                float* acc, bool ring);
+// name of the kernel fxk_xengine launches for these arguments ("" if none)
+const char* fxk_xengine_name(int in_type, int ts, const void* X, long nt, int ninp, bool ring);
+// FX_XENGINE_RING=0 in the environment turns the ring engines off by default
+bool fxk_xengine_ring_default();
 // 8-bit X-engine on input-major channelized data X[c][i][t]: FX_PREC_FP8
 // (e4m3 re|im<<8 pairs, one accumulator set) or FX_PREC_FP8_2BIT (2-bit
 // codes re|im<<2, three accumulator sets of the exact decomposition)

@@ -817,7 +817,8 @@ size_t fxk_xengine_acc_floats(int nchan, int ninp, int nsets, int ts)
 }
 
 // FX_XENGINE_RING=0 selects the register-prefetch engine for every shape
-static bool ring_enabled()
+// (the default of a new correlator; fx_corr_set_xengine overrides it)
+bool fxk_xengine_ring_default()
 {
     static const bool on = [] {
         const char* e = getenv("FX_XENGINE_RING");
@@ -826,49 +827,65 @@ static bool ring_enabled()
     return on;
 }
 
-int fxk_xengine(hipStream_t s, int in_type, int ts, const void* X, int nchan, long nt, int ninp, float* acc)
+namespace {
+
+enum XKind { XK_NONE, XK_RING2_F16, XK_RING2_F32, XK_RING_F16, XK_RING_F32, XK_F32_128, XK_F32_64, XK_F16_128,
+             XK_F16_64 };
+
+XKind xengine_select(int in_type, int ts, const void* X, long nt, int ninp, bool ring)
 {
-    long nblk = xcd_grid((long)ntiles_ts(ninp, ts) * nchan);
+    const bool aligned = ((uintptr_t)X % 16) == 0;
     // whole 128-input tiles of more inputs and whole stages: the two-slab LDS ring
-    if (ts == 128 && ninp > 128 && ninp % 128 == 0 && ((uintptr_t)X % 16) == 0 && ring_enabled()) {
-        if (in_type == FX_PREC_F16 && nt % (XTraits<FX_PREC_F16>::KT / 2) == 0) {
-            hipLaunchKernelGGL(k_xengine_ring2<FX_PREC_F16>, dim3((unsigned)nblk), dim3(XNT), 0, s,
-                               reinterpret_cast<const uint32_t*>(X), nchan, nt, ninp, acc);
-            return hipGetLastError() == hipSuccess ? FX_OK : FX_ERR_HIP;
-        }
-        if (in_type == FX_PREC_F32 && nt % (XTraits<FX_PREC_F32>::KT / 2) == 0) {
-            hipLaunchKernelGGL(k_xengine_ring2<FX_PREC_F32>, dim3((unsigned)nblk), dim3(XNT), 0, s,
-                               reinterpret_cast<const float2*>(X), nchan, nt, ninp, acc);
-            return hipGetLastError() == hipSuccess ? FX_OK : FX_ERR_HIP;
-        }
+    if (ring && ts == 128 && ninp > 128 && ninp % 128 == 0 && aligned) {
+        if (in_type == FX_PREC_F16 && nt % (XTraits<FX_PREC_F16>::KT / 2) == 0) return XK_RING2_F16;
+        if (in_type == FX_PREC_F32 && nt % (XTraits<FX_PREC_F32>::KT / 2) == 0) return XK_RING2_F32;
     }
     // one full tile and whole stages: the LDS-ring engine
-    if (ts == 128 && ninp == 128 && ((uintptr_t)X % 16) == 0 && ring_enabled()) {
-        if (in_type == FX_PREC_F16 && nt % XTraits<FX_PREC_F16>::KT == 0) {
-            hipLaunchKernelGGL(k_xengine_ring<FX_PREC_F16>, dim3((unsigned)nblk), dim3(XNT), 0, s,
-                               reinterpret_cast<const uint32_t*>(X), nchan, nt, acc);
-            return hipGetLastError() == hipSuccess ? FX_OK : FX_ERR_HIP;
-        }
-        if (in_type == FX_PREC_F32 && nt % XTraits<FX_PREC_F32>::KT == 0) {
-            hipLaunchKernelGGL(k_xengine_ring<FX_PREC_F32>, dim3((unsigned)nblk), dim3(XNT), 0, s,
-                               reinterpret_cast<const float2*>(X), nchan, nt, acc);
-            return hipGetLastError() == hipSuccess ? FX_OK : FX_ERR_HIP;
-        }
+    if (ring && ts == 128 && ninp == 128 && aligned) {
+        if (in_type == FX_PREC_F16 && nt % XTraits<FX_PREC_F16>::KT == 0) return XK_RING_F16;
+        if (in_type == FX_PREC_F32 && nt % XTraits<FX_PREC_F32>::KT == 0) return XK_RING_F32;
     }
-    if (in_type == FX_PREC_F32 && ts == 128) {
-        hipLaunchKernelGGL((k_xengine<FX_PREC_F32, 128>), dim3((unsigned)nblk), dim3(XNT), 0, s,
-                           reinterpret_cast<const float2*>(X), nchan, nt, ninp, acc);
-    } else if (in_type == FX_PREC_F32 && ts == 64) {
-        hipLaunchKernelGGL((k_xengine<FX_PREC_F32, 64>), dim3((unsigned)nblk), dim3(XNT), 0, s,
-                           reinterpret_cast<const float2*>(X), nchan, nt, ninp, acc);
-    } else if (in_type == FX_PREC_F16 && ts == 128) {
-        hipLaunchKernelGGL((k_xengine<FX_PREC_F16, 128>), dim3((unsigned)nblk), dim3(XNT), 0, s,
-                           reinterpret_cast<const uint32_t*>(X), nchan, nt, ninp, acc);
-    } else if (in_type == FX_PREC_F16 && ts == 64) {
-        hipLaunchKernelGGL((k_xengine<FX_PREC_F16, 64>), dim3((unsigned)nblk), dim3(XNT), 0, s,
-                           reinterpret_cast<const uint32_t*>(X), nchan, nt, ninp, acc);
-    } else {
-        return FX_ERR_UNSUPPORTED;
+    if (in_type == FX_PREC_F32 && ts == 128) return XK_F32_128;
+    if (in_type == FX_PREC_F32 && ts == 64) return XK_F32_64;
+    if (in_type == FX_PREC_F16 && ts == 128) return XK_F16_128;
+    if (in_type == FX_PREC_F16 && ts == 64) return XK_F16_64;
+    return XK_NONE;
+}
+
+} // namespace
+
+const char* fxk_xengine_name(int in_type, int ts, const void* X, long nt, int ninp, bool ring)
+{
+    switch (xengine_select(in_type, ts, X, nt, ninp, ring)) {
+    case XK_RING2_F16: return "k_xengine_ring2<F16>";
+    case XK_RING2_F32: return "k_xengine_ring2<F32>";
+    case XK_RING_F16: return "k_xengine_ring<F16>";
+    case XK_RING_F32: return "k_xengine_ring<F32>";
+    case XK_F32_128: return "k_xengine<F32, 128>";
+    case XK_F32_64: return "k_xengine<F32, 64>";
+    case XK_F16_128: return "k_xengine<F16, 128>";
+    case XK_F16_64: return "k_xengine<F16, 64>";
+    default: return "";
+    }
+}
+
+int fxk_xengine(hipStream_t s, int in_type, int ts, const void* X, int nchan, long nt, int ninp, float* acc,
+                bool ring)
+{
+    const long nblk = xcd_grid((long)ntiles_ts(ninp, ts) * nchan);
+    const dim3 g((unsigned)nblk), b(XNT);
+    const uint32_t* xh = reinterpret_cast<const uint32_t*>(X);
+    const float2* xf = reinterpret_cast<const float2*>(X);
+    switch (xengine_select(in_type, ts, X, nt, ninp, ring)) {
+    case XK_RING2_F16: hipLaunchKernelGGL(k_xengine_ring2<FX_PREC_F16>, g, b, 0, s, xh, nchan, nt, ninp, acc); break;
+    case XK_RING2_F32: hipLaunchKernelGGL(k_xengine_ring2<FX_PREC_F32>, g, b, 0, s, xf, nchan, nt, ninp, acc); break;
+    case XK_RING_F16: hipLaunchKernelGGL(k_xengine_ring<FX_PREC_F16>, g, b, 0, s, xh, nchan, nt, acc); break;
+    case XK_RING_F32: hipLaunchKernelGGL(k_xengine_ring<FX_PREC_F32>, g, b, 0, s, xf, nchan, nt, acc); break;
+    case XK_F32_128: hipLaunchKernelGGL((k_xengine<FX_PREC_F32, 128>), g, b, 0, s, xf, nchan, nt, ninp, acc); break;
+    case XK_F32_64: hipLaunchKernelGGL((k_xengine<FX_PREC_F32, 64>), g, b, 0, s, xf, nchan, nt, ninp, acc); break;
+    case XK_F16_128: hipLaunchKernelGGL((k_xengine<FX_PREC_F16, 128>), g, b, 0, s, xh, nchan, nt, ninp, acc); break;
+    case XK_F16_64: hipLaunchKernelGGL((k_xengine<FX_PREC_F16, 64>), g, b, 0, s, xh, nchan, nt, ninp, acc); break;
+    default: return FX_ERR_UNSUPPORTED;
     }
     return hipGetLastError() == hipSuccess ? FX_OK : FX_ERR_HIP;
 }

@@ -18,6 +18,7 @@ LIB_PATH = os.path.join(HERE, "libcellspe_fx.so")
 
 FX_OK = 0
 FX_PREC_F32, FX_PREC_F16, FX_PREC_FP8, FX_PREC_FP8_2BIT = 0, 1, 2, 3
+FX_XENGINE_AUTO, FX_XENGINE_PREFETCH = 0, 1
 (FX_WIN_NONE, FX_WIN_SIN2, FX_WIN_COSINE, FX_WIN_COSINE2, FX_WIN_HAMMING, FX_WIN_BLACKMAN,
  FX_WIN_HANN) = range(7)
 
@@ -136,6 +137,8 @@ def load():
         "fx_corr_fstage_depth": (l, [vp]),
         "fx_corr_fstage_launches": (i, [vp, l, i]),
         "fx_corr_tune": (i, [vp, l, i]),
+        "fx_corr_set_xengine": (i, [vp, i]),
+        "fx_corr_xengine_name": (i, [vp, vp, l, C.c_char_p, i]),
         "fx_pcal_final_async": (i, [vp]),
         "fx_pcal_tones": (i, [vp, vp, C.POINTER(u64)]),
         "fx_pcal_create_inputs": (i, [vp, d, d, i, sz, i, C.POINTER(vp)]),
@@ -148,7 +151,7 @@ def load():
         "fx_batch_pcal": (i, [vp, i, i, vp, C.POINTER(i)]),
         "fx_batch_tuning": (i, [vp, i, C.POINTER(l), C.POINTER(i), C.POINTER(l)]),
         "fx_batch_destroy": (i, [vp]),
-        "fx_autotune": (i, [vp, i, i, i, C.c_char_p, C.POINTER(l), C.POINTER(i)]),
+        "fx_autotune": (i, [vp, i, i, i, l, C.c_char_p, C.POINTER(l), C.POINTER(i)]),
         "fx_corr_info": (i, [vp, C.POINTER(l), C.POINTER(sz), C.POINTER(sz)]),
         "fx_mfma_probe": (i, [vp, i, vp]),
     }
@@ -502,6 +505,18 @@ class Correlator:
 
     def tune(self, sub_spectra=0, xtile=0):
         check(load().fx_corr_tune(self.handle, sub_spectra, xtile))
+
+    def set_xengine(self, kind):
+        """FX_XENGINE_AUTO (LDS-ring engines where they apply) or
+        FX_XENGINE_PREFETCH (the register-prefetch engine for every shape)."""
+        check(load().fx_corr_set_xengine(self.handle, kind))
+
+    def xengine_name(self, nspectra, d_chan=None):
+        """Name of the X-engine kernel that runs on nspectra spectra."""
+        buf = C.create_string_buffer(64)
+        ptr = None if d_chan is None else d_chan.ptr
+        check(load().fx_corr_xengine_name(self.handle, ptr, nspectra, buf, 64))
+        return buf.value.decode()
 
     def visibilities(self):
         """Dump to a temporary device buffer and return [P][C] complex64."""

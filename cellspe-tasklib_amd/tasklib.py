@@ -196,7 +196,8 @@ def load():
         "celldsp_clientCount": (i, []),
         "celldsp_graphReplays": (C.c_long, []),
         "speNOP": (i, []),
-        "speIOCheck": (i, [vp]),
+        "speIOCheck": (i, []),
+        "speIOCheck_out": (i, [vp]),
         "speAddProduct_32fc": (i, [vp, vp, vp, i]),
         "speMul_32fc": (i, [vp, vp, vp, i]),
         "speAdd_32fc": (i, [vp, vp, vp, i]),

@@ -43,7 +43,11 @@ typedef struct { float re; float im; } fc32;
 typedef fc32 cf32;
 
 int speNOP(void);
-int speIOCheck(float* out4);
+/* cellspe-tasklib-dsp.cpp:111-124: IOTEST into a static buffer, prints the
+ * four values written back; vecNoErr when the task ran */
+int speIOCheck(void);
+/* not in the reference: IOTEST into the caller's 4 floats (no printing) */
+int speIOCheck_out(float* out4);
 int speAddProduct_32fc(fc32* src, fc32* src2, fc32* accumulator, int length);
 int speMul_32fc(fc32* src, fc32* src2, fc32* dest, int length);
 int speAdd_32fc(fc32* src, fc32* src2, fc32* dest, int length);

@@ -364,6 +364,18 @@ int fx_corr_info(fx_corr* corr, long* chunk_spectra, size_t* chan_bytes, size_t*
  * the FP32/FP16 engines; smaller tiles waste less MFMA work on small input
  * counts).  Changing the tile clears the accumulators. */
 int fx_corr_tune(fx_corr* corr, long sub_spectra, int xtile);
+/* X-engine choice of the FP32/FP16 correlators: FX_XENGINE_AUTO (default;
+ * FX_XENGINE_RING=0 in the environment at fx_corr_create makes the default
+ * FX_XENGINE_PREFETCH) runs the LDS-ring engines where they apply -- exactly
+ * 128 inputs with a multiple of KT spectra per call (32 FP16, 16 FP32), or a
+ * multiple of 128 inputs with a multiple of KT/2 -- and the register-prefetch
+ * engine otherwise; FX_XENGINE_PREFETCH runs the register-prefetch engine for
+ * every shape.  Both give bit-identical sums. */
+enum { FX_XENGINE_AUTO = 0, FX_XENGINE_PREFETCH = 1 };
+int fx_corr_set_xengine(fx_corr* corr, int kind);
+/* name of the X-engine kernel fx_corr_xengine / fx_corr_process runs on
+ * nspectra spectra of d_chan (NULL: the correlator's own channel buffer) */
+int fx_corr_xengine_name(fx_corr* corr, const void* d_chan, long nspectra, char* buf, int len);
 
 /* ---- batched subbands (uneven FFT lengths and input counts) -------------- */
 typedef struct {
@@ -397,10 +409,12 @@ int fx_batch_pcal(fx_batch* batch, int s, int i, fx_cf64* tones, int* ntones);
 int fx_batch_tuning(fx_batch* batch, int s, long* sub_spectra, int* xtile, long* chunk_spectra);
 int fx_batch_destroy(fx_batch* batch);
 
-/* Autotuner: times the F-stage + X-engine of an (nfft, ninp, prec) subband on
- * synthetic input for every candidate (sub-chunk depth x X tile) and appends
- * the fastest to tune_file ("nfft ninp prec sub xtile ms" lines). */
-int fx_autotune(fx_ctx* ctx, int nfft, int ninp, int prec, const char* tune_file, long* sub_spectra, int* xtile);
+/* Autotuner: times the F-stage + X-engine of one chunk of chunk_spectra
+ * spectra of an (nfft, ninp, prec) subband on synthetic input for every
+ * candidate (sub-chunk depth x X tile) and appends the fastest to tune_file
+ * ("nfft ninp prec chunk sub xtile ms" lines, looked up by the first four). */
+int fx_autotune(fx_ctx* ctx, int nfft, int ninp, int prec, long chunk_spectra, const char* tune_file,
+                long* sub_spectra, int* xtile);
 
 /* ---- diagnostics --------------------------------------------------------- */
 /* MFMA lane-layout probe: which = 0 (f32 32x32x2), 1 (f16 32x32x16),

@@ -12,6 +12,7 @@ import os
 import numpy as np
 import pytest
 
+import bench
 import fxlib
 import oracle as O
 
@@ -110,8 +111,56 @@ def test_autotune_cache(ctx, tmp_path):
     tune = str(tmp_path / "t.txt")
     import ctypes as C
     sub, xt = C.c_long(), C.c_int()
-    fxlib.check(fxlib.load().fx_autotune(ctx.handle, 1024, 40, fxlib.FX_PREC_F16, tune.encode(), C.byref(sub),
-                                         C.byref(xt)))
+    fxlib.check(fxlib.load().fx_autotune(ctx.handle, 1024, 40, fxlib.FX_PREC_F16, 2048, tune.encode(),
+                                         C.byref(sub), C.byref(xt)))
+    # the sub-chunk candidates that fit a 2048-spectrum chunk
     assert sub.value in (64, 256, 1024) and xt.value in (64, 128)
     f = open(tune).read().split()
-    assert f[:5] == ["1024", "40", str(fxlib.FX_PREC_F16), str(sub.value), str(xt.value)]
+    assert f[:6] == ["1024", "40", str(fxlib.FX_PREC_F16), "2048", str(sub.value), str(xt.value)]
+
+
+def test_batch_seeded_draw_all_subbands(ctx, tmp_path):
+    """The benchmark's seeded draw of 32 uneven subbands (FFT lengths from
+    {256, 1024, 4096, 16384}, 16-96 inputs, rho = 0.03, a phase-cal tone on
+    input 0) on a shortened integration of 2^17 samples per input: every
+    subband of the FP16 graph within 1e-3 of the FP64 oracle, of the FP32
+    batch within 1e-5 of the FP32 oracle, phase-cal tones within 1e-12."""
+    nsamp = 1 << 17
+    lib = fxlib.load()
+    subs, streams = [], []
+    for k, (n, I, T) in enumerate(bench.batch_draw(20240229 + 4, 32, nsamp)):
+        nw = nsamp // 16
+        w = ctx.alloc(I * nw * 4)
+        for i in range(I):
+            fxlib.check(lib.fx_synth_station(ctx.handle, 20240229 + 100 + k, i, 0.03, i % 7, 0.2 if i == 0 else 0.0,
+                                             (10e3 + 3e6) / FS, 0.9816, nsamp, w.ptr.value + i * nw * 4))
+        subs.append({"nfft": n, "ninp": I, "nspectra": T, "words": w, "stride_words": nw, "models": None,
+                     "pcal": (FS / 2, 1e6, 10e3)})
+    ctx.sync()
+    for s in subs:
+        streams.append([s["words"].download_at(i * s["stride_words"] * 4, np.uint32, s["stride_words"])
+                        for i in range(s["ninp"])])
+    g16 = fxlib.Batch(ctx, subs, fxlib.FX_PREC_F16, use_graph=True, tune_file=str(tmp_path / "tune.txt"))
+    b32 = fxlib.Batch(ctx, subs, fxlib.FX_PREC_F32, use_graph=False)
+    for b in (g16, b32):
+        b.run()
+        b.sync()
+    worst16 = worst32 = 0.0
+    for k, s in enumerate(subs):
+        n, I, T = s["nfft"], s["ninp"], s["nspectra"]
+        r64 = O.corr_f64(streams[k], n, T, None, nthreads=16)
+        r32 = O.corr_f32(streams[k], n, T, None, nthreads=16)
+        e16, e32 = nrel(g16.visibilities(k), r64), nrel(b32.visibilities(k), r32)
+        worst16, worst32 = max(worst16, e16), max(worst32, e32)
+        assert e16 < 1e-3, (k, n, I, e16)
+        assert e32 < TOL32, (k, n, I, e32)
+        cfg = O.pcal_config(*s["pcal"])
+        x = O.unpack_1ch(streams[k][0], n * T)
+        ref = O.pcal_final(cfg, O.pcal_extract(cfg, x))
+        for b in (g16, b32):
+            assert nrel(b.pcal(k, 0), ref) < 1e-12
+    print(f"32-subband draw: worst FP16 vs FP64 {worst16:.2e}, worst FP32 vs FP32 oracle {worst32:.2e}")
+    g16.close()
+    b32.close()
+    for s in subs:
+        s["words"].free()

@@ -14,7 +14,10 @@
   oracle, FP16 against the FP64 oracle (<= 1e-3);
 * large-N shape: 512 inputs, 8192 spectra of 2-bit channelized voltages
   (128 channels here), exact FP8 engine and FP16 engine against FP64 sums on
-  sampled channels.
+  sampled channels; and all 1024 channels of the large-N workload on a
+  64-spectrum time slice through the exact FP8, FP16 and plain FP32 engines,
+  every product and channel against FP64 sums (FP32 also against the FP32
+  restatement on sampled channels).
 """
 import numpy as np
 import pytest
@@ -153,3 +156,47 @@ def test_large_n_shape(ctx):
         print(f"large-N channel {ch}: fp8 exact {e8:.2e}, f16 {e16:.2e}")
         assert e8 < 1e-6
         assert e16 < 1e-3
+
+
+def test_large_n_all_channels_time_slice(ctx):
+    I, C, T = 512, 1024, 64
+    P = I * (I + 1) // 2
+    codes = ctx.alloc(C * I * T)
+    ctx.synth_channelized(2025, C, T, I, 0.01, 0.9816, codes, input_major=True)
+    cb = codes.download(np.uint8).reshape(C, I, T)
+    lut = O.LUT.astype(np.float64)
+    iu = np.triu_indices(I)
+    ref = np.empty((P, C), np.complex128)
+    for c in range(C):
+        x = lut[cb[c] & 3] + 1j * lut[(cb[c] >> 2) & 3]
+        ref[:, c] = (x @ x.conj().T)[iu]
+    vis = ctx.alloc(P * C * 8)
+    got = {}
+    c8 = fxlib.Correlator(ctx, I, 2 * C, T, fxlib.FX_PREC_FP8_2BIT)
+    c8.xengine(codes, T)
+    c8.dump(vis)
+    c8.close()
+    got["fp8_2bit"] = vis.download(np.complex64, shape=(P, C))
+    for name, prec, esz in (("f16", fxlib.FX_PREC_F16, 4), ("f32", fxlib.FX_PREC_F32, 8)):
+        dx = ctx.alloc(C * T * I * esz)
+        ctx.codes_to_chan(codes, True, C, T, I, prec, dx)
+        cc = fxlib.Correlator(ctx, I, 2 * C, 8, prec)
+        cc.set_scales(np.ones(C, np.float32), np.ones(I, np.float32))
+        assert cc.xengine_name(T, dx) == ("k_xengine_ring2<F16>" if name == "f16" else "k_xengine_ring2<F32>")
+        cc.xengine(dx, T)
+        cc.dump(vis)
+        cc.close()
+        dx.free()
+        got[name] = vis.download(np.complex64, shape=(P, C))
+    codes.free()
+    vis.free()
+    err = {k: nrel(v, ref) for k, v in got.items()}
+    print("large-N, all 1024 channels x 64 spectra vs FP64: " + ", ".join(f"{k} {v:.2e}" for k, v in err.items()))
+    assert err["fp8_2bit"] < 1e-6
+    assert err["f32"] < TOL32
+    assert err["f16"] < 1e-3
+    # the FP32 engine against the FP32 restatement of the reference MAC
+    for c in (0, 511, 1023):
+        x = (lut[cb[c] & 3] + 1j * lut[(cb[c] >> 2) & 3]).T.astype(np.complex64)   # [t][i]
+        r32 = O.xengine_f32(x[None])[:, 0]
+        assert nrel(got["f32"][:, c], r32) < TOL32

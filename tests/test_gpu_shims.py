@@ -42,9 +42,12 @@ def rt():
 
 
 # ---------------------------------------------------------------- spe* wrappers
-def test_iocheck(rt):
+def test_iocheck(rt, capfd):
+    # the reference form: no argument, a static buffer, the values printed
+    assert rt.speIOCheck() == 0
+    assert "wrote back : vect={0.1,2.3,4.5,6.7}" in capfd.readouterr().out
     out = np.zeros(4, np.float32)
-    assert rt.speIOCheck(P(out)) == 0
+    assert rt.speIOCheck_out(P(out)) == 0
     assert np.array_equal(out, np.array([0.1, 2.3, 4.5, 6.7], np.float32))
 
 
@@ -229,6 +232,62 @@ def test_task_chain_graph_replays(rt):
         assert rt.celldsp_executeTask(tasks[0].getTask()) == 0
     assert rt.celldsp_graphReplays() - r0 == 5
     rt.celldsp_setBehaviour(TL.CDSP_BEHAV_DEFAULT)
+
+
+def test_chain_graph_after_slot_growth(rt):
+    """A captured chain's graph holds the unit's device staging addresses.  A
+    larger single task on the same unit regrows one of those buffers; the
+    chain's next graph-mode execution must not replay the stale graph: it
+    runs eagerly, is captured again on the one after, and every result equals
+    the eager chain's bit for bit."""
+    tasks, v = _graph_chain()
+    rng = np.random.default_rng(34)
+    outs = ("X", "Y", "acc", "f", "fo", "cfo", "acc2")
+    unit = rt.celldsp_getFreeSPE()
+    assert unit >= 0
+
+    def run(head):
+        assert rt.celldsp_startTask(unit, head) == 0
+        return rt.celldsp_waitTask(unit)
+
+    def chain(behaviour, x, words, acc0, acc20):
+        v["x"][:] = x; v["words"][:] = words; v["acc"][:] = acc0; v["acc2"][:] = acc20
+        for k in ("X", "Y", "f", "fo", "cfo"):
+            v[k][:] = 0
+        rt.celldsp_setBehaviour(behaviour)
+        assert run(tasks[0].getTask()) == TL.CELLDSP_TASK_OK
+        return {k: v[k].copy() for k in outs}
+
+    def inputs():
+        return ((rng.standard_normal(1024) + 1j * rng.standard_normal(1024)).astype(np.complex64),
+                rng.integers(0, 2**32, 512, dtype=np.uint64).astype(np.uint32),
+                (rng.standard_normal(1024) + 0j).astype(np.complex64),
+                (rng.standard_normal(257) + 0j).astype(np.complex64))
+
+    try:
+        for _ in range(3):              # eager (sizes the buffers), capture + replay, replay
+            chain(TL.CDSP_BEHAV_GRAPH, *inputs())
+        r0 = rt.celldsp_graphReplays()
+        # a 32 MiB output round trip through HBM: larger than anything the
+        # tests before put in that output buffer, so it is reallocated
+        big = TL.CellDSPTaskClass(TL.CELLDSP_TASK_IOBENCH)
+        vb = rng.integers(0, 2**32, 8 << 20, dtype=np.uint64).astype(np.uint32)
+        sent = vb.copy()
+        big.addOutVec(vb)
+        assert run(big.getTask()) == TL.CELLDSP_TASK_OK
+        assert np.array_equal(vb, sent)
+        for rep in range(3):
+            args = inputs()
+            got = chain(TL.CDSP_BEHAV_GRAPH, *args)
+            ref = chain(TL.CDSP_BEHAV_DEFAULT, *args)
+            for k in outs:
+                assert np.array_equal(got[k].view(np.uint32), ref[k].view(np.uint32)), (rep, k)
+            assert nrel(ref["X"], O.fft_r2(args[0])) < TOL32
+        # after the growth: one eager run, then a new capture and two replays
+        assert rt.celldsp_graphReplays() - r0 == 2
+    finally:
+        rt.celldsp_setBehaviour(TL.CDSP_BEHAV_DEFAULT)
+        rt.celldsp_setFreeSPE(unit)
 
 
 def test_task_chain_on_device(rt):

@@ -298,7 +298,8 @@ def test_device_commands_report_generic_error_without_device(runtime):
     if fxlib.load().fx_device_count() > 0:
         pytest.skip("a HIP device is present")
     out = np.zeros(4, np.float32)
-    assert lib.speIOCheck(TL._p(out)) == 1
+    assert lib.speIOCheck_out(TL._p(out)) == 1
+    assert lib.speIOCheck() == 1
     t = TL.CellDSPTaskClass(TL.CELLDSP_TASK_IOTEST)
     t.addOutVec(out)
     assert lib.celldsp_executeTask(t.getTask()) == 1
