@@ -425,38 +425,18 @@ extern "C" int fx_sincos_generator(fx_ctx* ctx, double start, double inc, fx_cf3
 // ---------------------------------------------------------------------------
 // spectrometer
 // ---------------------------------------------------------------------------
+// the window table of (kind, n), computed on the device once per context
 static float* get_window(fx_ctx* ctx, int kind, int n)
 {
     auto key = std::make_pair(kind, n);
     auto it = ctx->windows.find(key);
     if (it != ctx->windows.end()) return it->second;
-    std::vector<float> w((size_t)n);
-    for (int i = 0; i < n; i++) {
-        double v = 1.0;
-        if (kind == FX_WIN_SIN2) {
-            double s = sin(M_PI * (double)i / (double)(n - 1));
-            v = s * s;
-        } else if (kind == FX_WIN_COSINE) {
-            v = cos(M_PI * ((double)i - 0.5 * (double)n + 0.5) / (double)n);
-        } else if (kind == FX_WIN_COSINE2) {
-            double c = cos(M_PI * ((double)i - 0.5 * (double)n + 0.5) / (double)n);
-            v = c * c;
-        } else if (kind >= FX_WIN_HAMMING) {
-            // generalized cosine windows over n-1 intervals (TaskCoreIPP.cpp:
-            // 589-602: Hamming, Blackman with alpha = 0.5, Hann)
-            const double x = 2.0 * M_PI * (double)i / (double)(n - 1);
-            if (kind == FX_WIN_HAMMING)
-                v = 0.54 - 0.46 * cos(x);
-            else if (kind == FX_WIN_BLACKMAN)
-                v = 0.75 - 0.5 * cos(x) - 0.25 * cos(2.0 * x);
-            else
-                v = 0.5 - 0.5 * cos(x);
-        }
-        w[(size_t)i] = (float)v;
-    }
     float* d = nullptr;
-    if (hipMalloc(&d, sizeof(float) * (size_t)n) != hipSuccess) return nullptr;
-    if (hipMemcpy(d, w.data(), sizeof(float) * (size_t)n, hipMemcpyHostToDevice) != hipSuccess) {
+    if (hipMalloc(&d, sizeof(float) * (size_t)n) != hipSuccess) {
+        hipGetLastError();
+        return nullptr;
+    }
+    if (fxk_window(ctx->stream, kind, n, d) != FX_OK) {
         hipFree(d);
         return nullptr;
     }
@@ -486,12 +466,9 @@ extern "C" int fx_spectrometer(fx_ctx* ctx, const uint32_t* d_words, size_t ntim
     return fx_spectrometer_ex(ctx, d_words, ntime, nchan, n, 1, integrate_ffts, window, 0, 0, d_out, nullptr);
 }
 
-extern "C" int fx_spectrometer_ex(fx_ctx* ctx, const uint32_t* d_words, size_t ntime, int nchan, int n,
-                                  int overlap, int integrate_ffts, int window, int continuous,
-                                  int keep_nyquist, float* d_out, long* nspec_out)
+int fx_spec_make_plan(size_t ntime, int nchan, int n, int overlap, int integrate_ffts, int window, int continuous,
+                      fx_spec_plan* plan)
 {
-    int rc = ctx_device(ctx);
-    if (rc) return rc;
     FX_REQUIRE(valid_nchan(nchan), FX_ERR_ARG, "fx_spectrometer: nchan %d", nchan);
     FX_REQUIRE(is_pow2(n) && n >= 256 && n <= FX_MAX_N, FX_ERR_ARG, "fx_spectrometer: n=%d", n);
     FX_REQUIRE(integrate_ffts >= 1, FX_ERR_ARG, "fx_spectrometer: integrate_ffts=%d", integrate_ffts);
@@ -510,24 +487,39 @@ extern "C" int fx_spectrometer_ex(fx_ctx* ctx, const uint32_t* d_words, size_t n
         spec_stride = (long)(integrate_ffts - 1) * hop + n;
         nspec = (long)(ntime / (size_t)spec_stride);
     }
-    if (nspec_out) *nspec_out = nspec;
-    if (nspec == 0) return FX_OK;
-    std::lock_guard<std::mutex> lk(ctx->mu);
-    float* win = get_window(ctx, window, n);
-    FX_REQUIRE(win != nullptr, FX_ERR_NOMEM, "fx_spectrometer: window table");
     // split each spectrum's FFTs over `groups` workgroups so the grid holds
     // at least ~512 workgroups; each writes one partial spectrum tile
     const int M = n / 2;
     const int FM = (M >= 16384) ? 16384 : 8192;
     const int F = FM / M;
-    long per_spec_units = (integrate_ffts + F - 1) / F;
-    long want = 512 / (nspec * nchan) + 1;
-    int groups = (int)std::min<long>(std::max<long>(want, 1), per_spec_units);
-    size_t part_bytes = (size_t)nspec * nchan * groups * (size_t)(FM + F) * sizeof(float);
-    rc = ensure_scratch(ctx, part_bytes);
+    const long per_spec_units = (integrate_ffts + F - 1) / F;
+    const long want = nspec > 0 ? 512 / (nspec * nchan) + 1 : 1;
+    plan->nspec = nspec;
+    plan->spec_stride = spec_stride;
+    plan->hop = hop;
+    plan->groups = (int)std::min<long>(std::max<long>(want, 1), per_spec_units);
+    plan->part_bytes = (size_t)nspec * nchan * plan->groups * (size_t)(FM + F) * sizeof(float);
+    return FX_OK;
+}
+
+extern "C" int fx_spectrometer_ex(fx_ctx* ctx, const uint32_t* d_words, size_t ntime, int nchan, int n,
+                                  int overlap, int integrate_ffts, int window, int continuous,
+                                  int keep_nyquist, float* d_out, long* nspec_out)
+{
+    int rc = ctx_device(ctx);
     if (rc) return rc;
-    FX_CHECK_LAUNCH(fxk_spectrometer(ctx->stream, ctx->tw, win, d_words, hop, spec_stride, nchan, n,
-                                     integrate_ffts, (int)nspec, ctx->scratch, groups, keep_nyquist ? 1 : 0,
+    fx_spec_plan pl;
+    rc = fx_spec_make_plan(ntime, nchan, n, overlap, integrate_ffts, window, continuous, &pl);
+    if (rc) return rc;
+    if (nspec_out) *nspec_out = pl.nspec;
+    if (pl.nspec == 0) return FX_OK;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    float* win = get_window(ctx, window, n);
+    FX_REQUIRE(win != nullptr, FX_ERR_NOMEM, "fx_spectrometer: window table");
+    rc = ensure_scratch(ctx, pl.part_bytes);
+    if (rc) return rc;
+    FX_CHECK_LAUNCH(fxk_spectrometer(ctx->stream, ctx->tw, win, d_words, pl.hop, pl.spec_stride, nchan, n,
+                                     integrate_ffts, (int)pl.nspec, ctx->scratch, pl.groups, keep_nyquist ? 1 : 0,
                                      d_out));
     return FX_OK;
 }

@@ -345,6 +345,19 @@ int fxk_fft_c2c(hipStream_t s, const float2* tw, const float2* in, float2* out, 
                 int inverse = 0, float scale = 1.0f);
 int fxk_fft_r2c(hipStream_t s, const float2* tw, const float* in, float2* out, int n, long batch,
                 int keep_nyquist);
+// window table of kind FX_WIN_* and n points, FP64 closed forms rounded to FP32
+int fxk_window(hipStream_t s, int kind, int n, float* w);
+// Launch shape of fx_spectrometer_ex: spectra, their sample stride, the FFT
+// hop, the partial-sum groups per (spectrum, channel) and the bytes of
+// partial spectra the launch needs (fxk_spectrometer's partial buffer).
+struct fx_spec_plan {
+    long nspec, spec_stride, hop;
+    int groups;
+    size_t part_bytes;
+};
+// validates the arguments of fx_spectrometer_ex and fills the plan
+int fx_spec_make_plan(size_t ntime, int nchan, int n, int overlap, int integrate_ffts, int window, int continuous,
+                      fx_spec_plan* plan);
 int fxk_spectrometer(hipStream_t s, const float2* tw, const float* window, const uint32_t* words,
                      long hop, long spec_stride, int nchan, int n, int integrate_ffts, int nspec, float* partial,
                      int groups, int nyq, float* out);

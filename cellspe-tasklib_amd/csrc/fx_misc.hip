@@ -476,6 +476,37 @@ __global__ void k_fill32(uint32_t* __restrict__ p, uint32_t value, size_t n)
         p[i] = value;
 }
 
+// Spectrometer window table w[i], i < n, evaluated in FP64 and rounded once:
+// SIN2 sin^2(pi i/(n-1)) (spectrometer_spu.c:297-315); COSINE and COSINE2
+// over n points centred between samples, and the generalized cosine
+// windows Hamming, Blackman (alpha 0.5) and Hann over n-1 intervals
+// (TaskCoreIPP.cpp:569-605); NONE all ones.
+__global__ void k_window(int kind, int n, float* __restrict__ w)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double pi = 3.14159265358979323846;
+    double v = 1.0;
+    if (kind == FX_WIN_SIN2) {
+        const double s = sin(pi * (double)i / (double)(n - 1));
+        v = s * s;
+    } else if (kind == FX_WIN_COSINE) {
+        v = cos(pi * ((double)i - 0.5 * (double)n + 0.5) / (double)n);
+    } else if (kind == FX_WIN_COSINE2) {
+        const double c = cos(pi * ((double)i - 0.5 * (double)n + 0.5) / (double)n);
+        v = c * c;
+    } else if (kind >= FX_WIN_HAMMING) {
+        const double x = 2.0 * pi * (double)i / (double)(n - 1);
+        if (kind == FX_WIN_HAMMING)
+            v = 0.54 - 0.46 * cos(x);
+        else if (kind == FX_WIN_BLACKMAN)
+            v = 0.75 - 0.5 * cos(x) - 0.25 * cos(2.0 * x);
+        else
+            v = 0.5 - 0.5 * cos(x);
+    }
+    w[i] = (float)v;
+}
+
 __global__ void k_iotest(float* out)
 {
     if (threadIdx.x == 0) {
@@ -693,6 +724,13 @@ int fxk_fill32(hipStream_t s, void* p, uint32_t value, size_t n)
     if (n == 0) return FX_OK;
     hipLaunchKernelGGL(k_fill32, dim3(nblocks_gs(n, 256)), dim3(256), 0, s, reinterpret_cast<uint32_t*>(p), value,
                        n);
+    return last_status();
+}
+
+int fxk_window(hipStream_t s, int kind, int n, float* w)
+{
+    if (n <= 0) return FX_OK;
+    hipLaunchKernelGGL(k_window, dim3(nblocks((size_t)n, 256)), dim3(256), 0, s, kind, n, w);
     return last_status();
 }
 

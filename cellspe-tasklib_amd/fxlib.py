@@ -115,6 +115,7 @@ def load():
         "fx_sincos_generator": (i, [vp, d, d, vp, l]),
         "fx_spectrometer": (i, [vp, vp, sz, i, i, i, i, vp]),
         "fx_spectrometer_ex": (i, [vp, vp, sz, i, i, i, i, i, i, i, vp, C.POINTER(l)]),
+        "fx_spectrometer_ex_noalloc": (i, [vp, vp, sz, i, i, i, i, i, i, i, vp, C.POINTER(l)]),
         "fx_pcal_create": (i, [vp, d, d, i, sz, C.POINTER(vp)]),
         "fx_pcal_destroy": (i, [vp]),
         "fx_pcal_clear": (i, [vp]),

@@ -149,6 +149,20 @@ int fx_spectrometer(fx_ctx* ctx, const uint32_t* d_words, size_t ntime, int ncha
 int fx_spectrometer_ex(fx_ctx* ctx, const uint32_t* d_words, size_t ntime, int nchan, int n,
                        int overlap, int integrate_ffts, int window, int continuous, int keep_nyquist,
                        float* d_out, long* nspec_out);
+/* No counterpart in the reference: fx_spectrometer_ex for the callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops).  Same arguments, same kernels and launch shape, identical spectra;
+ * it calls no hipMalloc/hipFree: the window table and the partial spectra
+ * live in one static device array of FX_SPEC_NOALLOC_BYTES per GPU (held until
+ * the process ends).  Every call on a GPU shares that GPU's array, so calls
+ * that overlap on one GPU race.  A shape whose partial spectra need more than
+ * FX_SPEC_NOALLOC_BYTES - 128 KiB (nspec x nchan x groups x (8192 or 16384 +
+ * 1) floats) is refused with FX_ERR_ARG; the 16-channel, 32768-point
+ * whole-stream spectrum needs 34.6 MB. */
+#define FX_SPEC_NOALLOC_BYTES (40u << 20)
+int fx_spectrometer_ex_noalloc(fx_ctx* ctx, const uint32_t* d_words, size_t ntime, int nchan, int n,
+                               int overlap, int integrate_ffts, int window, int continuous, int keep_nyquist,
+                               float* d_out, long* nspec_out);
 /* Cross spectrum of channel chan_a of stream a and channel chan_b of stream b
  * (TaskCoreIPP.cpp:456-487, the cross-polarization product of two sources):
  * the same windowed, overlapped FFT frames and spectrum boundaries as
