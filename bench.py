@@ -69,6 +69,40 @@ def nrel(a, b):
     return float(np.linalg.norm((a - b).ravel()) / np.linalg.norm(b.ravel()))
 
 
+def cpu_threads():
+    """Threads of the CPU baseline: the CPUs this process may run on, at most
+    16 (one GPU's share of the host on the benchmark boxes).  Returns
+    (threads, CPUs in the affinity mask, online CPUs)."""
+    try:
+        aff = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        aff = os.cpu_count() or 1
+    online = os.sysconf("SC_NPROCESSORS_ONLN")
+    return min(16, aff), aff, online
+
+
+def cpu_median(fn, reps=3):
+    """Median wall time of reps calls of fn after one warm-up call."""
+    fn()
+    ts = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        fn()
+        ts.append(time.perf_counter() - t0)
+    return float(np.median(ts))
+
+
+def cpu_obj(value, unit, sample, extra=None):
+    nth, aff, online = cpu_threads()
+    o = {"value": value, "unit": unit, "cores": nth, "kind": "port", "sample": sample,
+         "affinity_cpus": aff, "online_cpus": online,
+         "build": "oracle/fx_oracle.c -O3 -march=native -fopenmp on this host; project radix-2 FFT, not FFTW",
+         "timing": "median of 3 runs after one warm-up"}
+    if extra:
+        o.update(extra)
+    return o
+
+
 def measured_traffic(kernel):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC
     summaries (lines 'bytes_per_launch <kernel> <bytes>'), or None."""
@@ -104,6 +138,64 @@ def copy_probe(fxlib, args):
     print(json.dumps({"copy_probe_bytes": nbytes, "ms_per_copy": ms,
                       "gbps_read_plus_write": 2 * nbytes / (ms * 1e-3) / 1e9}), flush=True)
     ctx.close()
+
+
+def mini_correlator(fxlib, ctx, args, O):
+    """The plain mini FX correlator (FP32): 4 stations, 2^20 samples each,
+    0.05 correlated fraction with seeded integer delays 0-7, a 10 kHz
+    phase-cal tone at 0.03 sigma (fs = 16 MHz), 1024-point real FFT (512
+    channels), all 10 products over all 1024 spectra; inputs resident in HBM.
+    GPU: reset + F + X + dump per step (fx_corr_process, FP32 MFMA engine);
+    CPU: the FP32 restatement of the same chain on the same words."""
+    nst, nsamp, n = 4, 1 << 20, 1024
+    T, C = nsamp // n, n // 2
+    P = nst * (nst + 1) // 2
+    delays = np.random.default_rng(args.seed + 1).integers(0, 8, nst)
+    nw = nsamp // 16
+    dw = ctx.alloc(nst * nw * 4)
+    lib = fxlib.load()
+    for s in range(nst):
+        fxlib.check(lib.fx_synth_station(ctx.handle, args.seed + 1, s, 0.05, int(delays[s]), 0.03, 10e3 / 16e6,
+                                         0.9816, nsamp, dw.ptr.value + s * nw * 4))
+    corr = fxlib.Correlator(ctx, nst, n, T, fxlib.FX_PREC_F32)
+    dvis = ctx.alloc(P * C * 8)
+
+    def step():
+        corr.reset()
+        corr.process(dw, nw, 0, T)
+        corr.dump(dvis)
+
+    step()
+    ctx.sync()
+    reps = max(20, args.steps)
+    e = [ctx.event() for _ in range(2)]
+    ctx.record(e[0])
+    for _ in range(reps):
+        step()
+    ctx.record(e[1])
+    ctx.sync()
+    ms = ctx.elapsed_ms(e[0], e[1]) / reps
+    cmac = float(P) * C * T
+    out = {"workload": "4 stations x 2^20 2-bit samples, rho=0.05, delays 0-7, 10 kHz pcal tone, 1024-pt FFT, "
+                       "512 channels, 10 products over 1024 spectra, FP32",
+           "ms": ms, "complex_mac_per_s": cmac / (ms * 1e-3), "input_gsps": nst * nsamp / (ms * 1e-3) / 1e9,
+           "xengine_kernel": corr.xengine_name(T)}
+    got = dvis.download(np.complex64, shape=(P, C))
+    corr.close()
+    dvis.free()
+    if not args.no_cpu:
+        nth = cpu_threads()[0]
+        streams = [dw.download_at(s * nw * 4, np.uint32, nw) for s in range(nst)]
+        ref = O.corr_f32(streams, n, T, nthreads=nth)
+        out["err_vs_cpu_fp32"] = nrel(got, ref)
+        dt = cpu_median(lambda: O.corr_f32(streams, n, T, nthreads=nth))
+        out["cpu_baseline"] = cpu_obj(cmac / dt, "complex MAC/s", "the whole workload", {
+            "ms": dt * 1e3, "input_gsps": nst * nsamp / dt / 1e9})
+        out["gpu_over_cpu"] = dt * 1e3 / ms
+    log(f"[bench] mini correlator: {ms:.3f} ms ({out['input_gsps']:.1f} GS/s)" +
+        (f", CPU {out['cpu_baseline']['ms']:.1f} ms" if "cpu_baseline" in out else ""))
+    dw.free()
+    return out
 
 
 def large_n(fxlib, ctx, args):
@@ -173,6 +265,22 @@ def large_n(fxlib, ctx, args):
         err = {k: nrel(v, ref) for k, v in vis_sample.items()}
         err["sample"] = f"channels {sample_ch}, all {P} products, all {T} spectra, vs FP64"
         log(f"[bench] large-N errors: " + ", ".join(f"{k} {v:.2e}" for k, v in err.items() if k != "sample"))
+    if not args.no_cpu:
+        # FP32 restatement of the X-engine on a sample of channels, all inputs and spectra
+        O = args.oracle
+        nth = cpu_threads()[0]
+        cs, tsmp = 128, min(T, 512)
+        X = np.empty((tsmp, I, cs), np.complex64)          # [t][i][c]: the generic core's loop order
+        for k in range(cs):
+            cb = codes.download_at(k * I * T, np.uint8, I * T).reshape(I, T)[:, :tsmp].T
+            X[:, :, k] = lut[cb & 3] + 1j * lut[(cb >> 2) & 3]
+        dt = cpu_median(lambda: O.xcorr_tic_f32(X, nth))
+        cm = float(P) * cs * tsmp
+        out["cpu_baseline"] = cpu_obj(cm / dt, "complex MAC/s",
+                                      f"{cs} of {C} channels x {tsmp} of {T} spectra x {I} inputs (FP32 X-engine), "
+                                      f"{dt:.2f} s per run", {"input_gsps": I * cs * tsmp / dt / 1e9})
+        log(f"[bench] large-N CPU baseline: {cm / dt:.3e} CMAC/s on {nth} threads")
+        del X
     codes.free()
     vis.free()
     flop = 8.0 * cmac
@@ -227,6 +335,18 @@ def spectrometer(fxlib, ctx, args):
            "fft_tflops": 2.5 * n * np.log2(n) * nfft / (ms * 1e-3) / 1e12, "spectra": int(ns.value)}
     log(f"[bench] spectrometer: {ms:.2f} ms for 2^28 samples ({out['gsamples_per_s']:.0f} GS/s, "
         f"{out['ffts_per_s']:.3g} FFT/s)")
+    if not args.no_cpu:
+        O = args.oracle
+        nth = cpu_threads()[0]
+        words = dw.download(np.uint32)
+        got = dout.download(np.float32, shape=(1, nch, n // 2 + 1))
+        ref = O.spectrometer_ex_f32(words, ntime, nch, n, 1, K, O.WIN_NONE, 0, 1, nthreads=nth)
+        out["err_vs_cpu_fp32"] = max(nrel(got[0, c], ref[0, c]) for c in range(nch))
+        dt = cpu_median(lambda: O.spectrometer_ex_f32(words, ntime, nch, n, 1, K, O.WIN_NONE, 0, 1, nthreads=nth))
+        out["cpu_baseline"] = cpu_obj((1 << 28) / dt / 1e9, "input GS/s", "the whole workload",
+                                      {"ms": dt * 1e3, "ffts_per_s": nfft / dt})
+        out["gpu_over_cpu"] = dt * 1e3 / ms
+        log(f"[bench] spectrometer CPU baseline: {dt * 1e3:.0f} ms on {nth} threads")
     dw.free()
     dout.free()
     return out
@@ -560,20 +680,54 @@ def batched(fxlib, ctx, args):
     ms32 = timed(e32, max(1, args.steps - 1))
     log(f"[bench] batch: fp16 graph {ms16:.2f} ms, fp32 one launch each {ms32:.2f} ms")
     err = None
+    nth = cpu_threads()[0]
+    frac = 64
     if not args.no_check:
-        sys.path.insert(0, os.path.join(ROOT, "oracle"))
-        import oracle as O
-        cost = [s["ninp"] ** 2 * s["nfft"] * s["nspectra"] for s in subs]
-        err = {}
-        for k in sorted(range(nsub), key=lambda q: cost[q])[:2]:
-            s = subs[k]
-            st = [s["words"].download_at(i * s["stride_words"] * 4, np.uint32, s["stride_words"])
+        # every subband against the oracle on its first 1/64 of the integration
+        # (at least 8 spectra): the same F -> X chain, a shorter sum
+        O = args.oracle
+        err = {"sample": f"every subband, its first 1/{frac} of the spectra (>= 8)"}
+        sub16 = fxlib.Batch(ctx, [dict(s, nspectra=max(8, s["nspectra"] // frac)) for s in subs],
+                            fxlib.FX_PREC_F16, use_graph=True, tune_file=tune)
+        sub32 = fxlib.Batch(ctx, [dict(s, nspectra=max(8, s["nspectra"] // frac)) for s in subs],
+                            fxlib.FX_PREC_F32, use_graph=False)
+        for b in (sub16, sub32):
+            b.run()
+            b.sync()
+        worst16 = worst32 = 0.0
+        for k, s in enumerate(subs):
+            ts = max(8, s["nspectra"] // frac)
+            st = [s["words"].download_at(i * s["stride_words"] * 4, np.uint32, ts * s["nfft"] // 16)
                   for i in range(s["ninp"])]
-            r64 = O.corr_f64(st, s["nfft"], s["nspectra"], None, nthreads=min(16, os.cpu_count() or 8))
-            r32 = O.corr_f32(st, s["nfft"], s["nspectra"], None, nthreads=min(16, os.cpu_count() or 8))
-            err[f"subband{k}_N{s['nfft']}_I{s['ninp']}"] = {"f16_vs_fp64": nrel(g16.visibilities(k), r64),
-                                                            "f32_vs_fp32_oracle": nrel(e32.visibilities(k), r32)}
-        log(f"[bench] batch errors: {err}")
+            r64 = O.corr_f64(st, s["nfft"], ts, None, nthreads=nth)
+            r32 = O.corr_f32(st, s["nfft"], ts, None, nthreads=nth)
+            worst16 = max(worst16, nrel(sub16.visibilities(k), r64))
+            worst32 = max(worst32, nrel(sub32.visibilities(k), r32))
+        err["f16_vs_fp64_worst"] = worst16
+        err["f32_vs_fp32_oracle_worst"] = worst32
+        sub16.close()
+        sub32.close()
+        log(f"[bench] batch errors (all {nsub} subbands): f16 vs FP64 worst {worst16:.2e}, "
+            f"f32 vs FP32 oracle worst {worst32:.2e}")
+    cpu = None
+    if not args.no_cpu:
+        O = args.oracle
+        streams = []
+        for s in subs:
+            ts = max(8, s["nspectra"] // frac)
+            streams.append([s["words"].download_at(i * s["stride_words"] * 4, np.uint32, ts * s["nfft"] // 16)
+                            for i in range(s["ninp"])])
+
+        def cpu_all():
+            for s, st in zip(subs, streams):
+                O.corr_f32(st, s["nfft"], max(8, s["nspectra"] // frac), None, nthreads=nth)
+
+        dt = cpu_median(cpu_all)
+        cm = sum((s["ninp"] * (s["ninp"] + 1) / 2) * (s["nfft"] / 2) * max(8, s["nspectra"] // frac) for s in subs)
+        smp = sum(s["ninp"] * s["nfft"] * max(8, s["nspectra"] // frac) for s in subs)
+        cpu = cpu_obj(cm / dt, "complex MAC/s", f"all {nsub} subbands, 1/{frac} of each integration (>= 8 spectra), "
+                      f"FP32 F -> X, no phase-cal, {dt:.2f} s per run", {"input_gsps": smp / dt / 1e9})
+        log(f"[bench] batch CPU baseline: {cm / dt:.3e} CMAC/s on {nth} threads")
     tuning = [g16.tuning(k) for k in range(nsub)]
     g16.close()
     e32.close()
@@ -591,7 +745,7 @@ def batched(fxlib, ctx, args):
             "complex_mac_per_s": {"f16_graph": tot_cmac / (ms16 * 1e-3), "f32_one_launch_each": tot_cmac / (ms32 * 1e-3)},
             "input_gsps": {"f16_graph": tot_samp / (ms16 * 1e-3) / 1e9,
                            "f32_one_launch_each": tot_samp / (ms32 * 1e-3) / 1e9},
-            "speedup": ms32 / ms16, "complex_macs": tot_cmac, "err": err}
+            "speedup": ms32 / ms16, "complex_macs": tot_cmac, "err": err, "cpu_baseline": cpu}
 
 
 def main():
@@ -617,7 +771,8 @@ def main():
     ap.add_argument("--batch-subbands", type=int, default=32)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-check", action="store_true")
-    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0,
+                    help="CPU budget of the headline workload's CPU baseline")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="instead of the benchmark, run the plain copy kernel over MIB MiB "
                          "(HBM bandwidth check and traffic-counter calibration)")
@@ -632,6 +787,17 @@ def main():
         sys.exit(2)
 
     import fxlib
+
+    # the CPU restatement, built -O3 -march=native for this host: the CPU
+    # baseline and the error checks' oracle
+    args.oracle = None
+    if not (args.no_cpu and args.no_check):
+        import tempfile
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+        ndir = tempfile.mkdtemp(prefix="fx_oracle_")
+        O.build_native(os.path.join(ndir, "liboracle_fx_native.so"))
+        args.oracle = O
 
     if args.copy_probe:
         copy_probe(fxlib, args)
@@ -729,9 +895,8 @@ def main():
 
     # ---- error vs the FP64 oracle on a bounded sub-run ------------------------
     err = None
+    O = args.oracle
     if not args.no_check:
-        sys.path.insert(0, os.path.join(ROOT, "oracle"))
-        import oracle as O
         # a multiple of every engine's stage depth, so the error is measured on
         # the same X-engine kernel the timed steps run
         te = 32
@@ -745,7 +910,7 @@ def main():
             c.process(dw, nwords, 0, te)
             got[name] = c.visibilities()
             c.close()
-        nth = min(16, os.cpu_count() or 8)
+        nth = cpu_threads()[0]
         t0 = time.time()
         ref64 = O.corr_f64(streams, N, te, models, nthreads=nth)
         log(f"[bench] FP64 oracle on {te} spectra in {time.time() - t0:.1f}s")
@@ -760,29 +925,30 @@ def main():
             ", ".join(f"{k} {v:.2e}" for k, v in err.items() if isinstance(v, float)))
         del ref64, got
 
-    # ---- CPU baseline (oracle FP32 generic-C chain, all host threads) --------
+    # ---- CPU baseline (oracle FP32 generic-C chain, the host's CPU share) ----
     cpu = None
     if not args.no_cpu:
-        sys.path.insert(0, os.path.join(ROOT, "oracle"))
-        import oracle as O
-        nth = min(16, os.cpu_count() or 8)
-        ts = max(2, nth)
-        while True:
-            streams = [dw.download_at(i * nwords * 4, np.uint32, ts * N // 16) for i in range(I)]
-            t0 = time.time()
-            O.corr_f32(streams, N, ts, models, nthreads=nth)
-            dt = time.time() - t0
-            if dt > args.cpu_seconds / 3 or ts * 2 > T:
-                break
+        nth = cpu_threads()[0]
+        # a bounded sample: a probe run on 64 spectra sizes the sample so one
+        # run takes about a quarter of the budget (four runs in all)
+        ts = min(T, 64)
+        streams = [dw.download_at(i * nwords * 4, np.uint32, ts * N // 16) for i in range(I)]
+        t0 = time.time()
+        O.corr_f32(streams, N, ts, models, nthreads=nth)
+        dt = time.time() - t0
+        while ts * 2 <= T and dt * 2 < args.cpu_seconds / 4:
             ts *= 2
-        cpu = {"value": P * C * ts / dt, "unit": "complex MAC/s", "cores": nth, "kind": "port",
-               "input_gsps": I * N * ts / dt / 1e9,
-               "sample": f"{ts} spectra x {I} inputs x {C} channels (FP32 generic-C restatement, "
-                         f"project radix-2 FFT, not FFTW), {dt:.1f}s"}
-        log(f"[bench] CPU baseline: {cpu['value']:.3e} CMAC/s on {nth} threads ({ts} spectra, {dt:.1f}s)")
+            dt *= 2
+        streams = [dw.download_at(i * nwords * 4, np.uint32, ts * N // 16) for i in range(I)]
+        dt = cpu_median(lambda: O.corr_f32(streams, N, ts, models, nthreads=nth))
+        cpu = cpu_obj(P * C * ts / dt, "complex MAC/s",
+                      f"{ts} of {T} spectra x {I} inputs x {C} channels, FP32 F -> X with the FP64 phase model, "
+                      f"{dt:.2f} s per run", {"input_gsps": I * N * ts / dt / 1e9})
+        log(f"[bench] CPU baseline: {cpu['value']:.3e} CMAC/s on {nth} threads ({ts} spectra, {dt:.2f}s)")
     dw.free()
     prod = products(fxlib, ctx, dvis, P, C, T, args)
     dvis.free()
+    mini = mini_correlator(fxlib, ctx, args, O)
     unp = unpack(fxlib, ctx, args)
     spec = spectrometer(fxlib, ctx, args)
     fe = None if args.no_front_end else front_end(fxlib, ctx, args)
@@ -865,6 +1031,7 @@ def main():
         "speedup_fp16_vs_fp32": None if "f32" not in paths else paths["f32"][0] / ms16,
         "kernels_ms_per_chunk": kern,
         "err_vs_fp64": err,
+        "mini_correlator": mini,
         "large_n_xengine": large,
         "batched_subbands": batch,
         "visibility_products": prod,

@@ -422,6 +422,28 @@ void fxo_xengine_f32(const fxo_cf32* X, int nchan, long ntime, int ninp, fxo_cf3
     }
 }
 
+/* fxo_xengine_f32 with the channels shared among nthreads workers: every
+ * (product, channel) sum runs over t in the same order, so the results equal
+ * the single-threaded ones bit for bit. */
+void fxo_xengine_f32_mt(const fxo_cf32* X, int nchan, long ntime, int ninp, fxo_cf32* vis, int nthreads)
+{
+    if (nthreads < 1) nthreads = 1;
+#ifdef _OPENMP
+#pragma omp parallel for num_threads(nthreads) schedule(dynamic, 1)
+#endif
+    for (int c = 0; c < nchan; c++) {
+        for (long t = 0; t < ntime; t++) {
+            const fxo_cf32* row = X + ((size_t)c * (size_t)ntime + (size_t)t) * (size_t)ninp;
+            for (int i = 0; i < ninp; i++)
+                for (int j = i; j < ninp; j++) {
+                    fxo_cf32* acc = vis + fxo_prod_index(i, j, ninp) * (size_t)nchan + (size_t)c;
+                    acc->re += (row[i].re * row[j].re) + (row[i].im * row[j].im);
+                    acc->im += (row[i].im * row[j].re) - (row[i].re * row[j].im);
+                }
+        }
+    }
+}
+
 void fxo_xengine_f64(const fxo_cf32* X, int nchan, long ntime, int ninp, fxo_cf64* vis)
 {
     for (int c = 0; c < nchan; c++) {
@@ -518,11 +540,61 @@ void fxo_corr_f32(const uint32_t* const* words, int ninp, int n, long ntime,
         }
         free(W); free(xs); free(spec); free(work);
     }
-    for (size_t e = 0; e < P * (size_t)C; e++) {
+    /* workers' sums added in worker order; the elements are independent */
+    const long ne = (long)(P * (size_t)C);
+#ifdef _OPENMP
+#pragma omp parallel for num_threads(nthreads) schedule(static)
+#endif
+    for (long e = 0; e < ne; e++) {
         fxo_cf32 s = part[e];
         for (int w = 1; w < nthreads; w++) {
-            s.re += part[(size_t)w * P * (size_t)C + e].re;
-            s.im += part[(size_t)w * P * (size_t)C + e].im;
+            s.re += part[(size_t)w * P * (size_t)C + (size_t)e].re;
+            s.im += part[(size_t)w * P * (size_t)C + (size_t)e].im;
+        }
+        vis[e].re += s.re;
+        vis[e].im += s.im;
+    }
+    free(part);
+}
+
+/* The X-engine of channelized data X[t][i][c] (c < nchan) the way the
+ * generic core loops (TaskCoreIPP.cpp:456-487 per source pair,
+ * genericAddProduct_32fc_conj over the channels, architecture.h:427-434),
+ * time-sliced over nthreads workers with their own accumulators summed in
+ * worker order (TaskDispatcher.cpp:116-170).  vis[p][c] += . */
+void fxo_xcorr_tic_f32(const fxo_cf32* X, int ninp, int nchan, long ntime, fxo_cf32* vis, int nthreads)
+{
+    size_t P = fxo_nprod(ninp);
+    if (nthreads < 1) nthreads = 1;
+    if (nthreads > ntime) nthreads = (int)ntime;
+    fxo_cf32* part = (fxo_cf32*)calloc((size_t)nthreads * P * (size_t)nchan, sizeof(fxo_cf32));
+#ifdef _OPENMP
+#pragma omp parallel num_threads(nthreads)
+#endif
+    {
+        int tid = 0;
+#ifdef _OPENMP
+        tid = omp_get_thread_num();
+#endif
+        long ta = ntime * tid / nthreads, tb = ntime * (tid + 1) / nthreads;
+        fxo_cf32* acc = part + (size_t)tid * P * (size_t)nchan;
+        for (long t = ta; t < tb; t++) {
+            const fxo_cf32* row = X + (size_t)t * (size_t)ninp * (size_t)nchan;
+            for (int i = 0; i < ninp; i++)
+                for (int j = i; j < ninp; j++)
+                    fxo_addproduct_conj(row + (size_t)i * (size_t)nchan, row + (size_t)j * (size_t)nchan,
+                                        acc + fxo_prod_index(i, j, ninp) * (size_t)nchan, nchan);
+        }
+    }
+    const long ne = (long)(P * (size_t)nchan);
+#ifdef _OPENMP
+#pragma omp parallel for num_threads(nthreads) schedule(static)
+#endif
+    for (long e = 0; e < ne; e++) {
+        fxo_cf32 s = part[e];
+        for (int w = 1; w < nthreads; w++) {
+            s.re += part[(size_t)w * P * (size_t)nchan + (size_t)e].re;
+            s.im += part[(size_t)w * P * (size_t)nchan + (size_t)e].im;
         }
         vis[e].re += s.re;
         vis[e].im += s.im;
@@ -565,10 +637,14 @@ void fxo_corr_f64(const uint32_t* const* words, int ninp, int n, long ntime,
         }
         free(spec);
     }
-    for (size_t e = 0; e < P * (size_t)C; e++)
+    const long ne = (long)(P * (size_t)C);
+#ifdef _OPENMP
+#pragma omp parallel for num_threads(nthreads) schedule(static)
+#endif
+    for (long e = 0; e < ne; e++)
         for (int w = 0; w < nthreads; w++) {
-            vis[e].re += part[(size_t)w * P * (size_t)C + e].re;
-            vis[e].im += part[(size_t)w * P * (size_t)C + e].im;
+            vis[e].re += part[(size_t)w * P * (size_t)C + (size_t)e].re;
+            vis[e].im += part[(size_t)w * P * (size_t)C + (size_t)e].im;
         }
     free(part);
 }

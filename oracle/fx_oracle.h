@@ -96,7 +96,10 @@ size_t fxo_nprod(int ninp);
 
 /* X-engine over channelized FP32 data X[c][t][i]; vis[p][c] += sum_t X_i conj(X_j). */
 void fxo_xengine_f32(const fxo_cf32* X, int nchan, long ntime, int ninp, fxo_cf32* vis);
+void fxo_xengine_f32_mt(const fxo_cf32* X, int nchan, long ntime, int ninp, fxo_cf32* vis, int nthreads);
 void fxo_xengine_f64(const fxo_cf32* X, int nchan, long ntime, int ninp, fxo_cf64* vis);
+/* X[t][i][c] channelized data, reference loop order, time-sliced workers */
+void fxo_xcorr_tic_f32(const fxo_cf32* X, int ninp, int nchan, long ntime, fxo_cf32* vis, int nthreads);
 
 /* Full F-X chain, FP32 (generic-C semantics): words[i] = packed stream of
  * input i (ntime*n samples), vis[p][c] with c = n/2 channels.  pm may be

@@ -38,13 +38,24 @@ def build():
     subprocess.check_call(["make", "-s", "-C", HERE, "liboracle_fx.so"])
 
 
-def load():
+def build_native(out_path):
+    """Build the restatement -O3 -march=native for this host at out_path (the
+    benchmark's CPU baseline) and make it the library every call below uses."""
+    global _lib
+    subprocess.check_call(["make", "-s", "-C", HERE, "native", f"NATIVE_OUT={out_path}"])
+    _lib = None
+    return load(out_path)
+
+
+def load(path=None):
     global _lib
     if _lib is not None:
         return _lib
-    if not os.path.exists(LIB_PATH):
-        build()
-    lib = C.CDLL(LIB_PATH)
+    if path is None:
+        path = LIB_PATH
+        if not os.path.exists(path):
+            build()
+    lib = C.CDLL(path)
     vp, sz, i, l, d, u64 = C.c_void_p, C.c_size_t, C.c_int, C.c_long, C.c_double, C.c_uint64
     sigs = {
         "fxo_mix64": (u64, [u64]),
@@ -71,6 +82,8 @@ def load():
         "fxo_prod_index": (sz, [i, i, i]),
         "fxo_nprod": (sz, [i]),
         "fxo_xengine_f32": (None, [vp, i, l, i, vp]),
+        "fxo_xengine_f32_mt": (None, [vp, i, l, i, vp, i]),
+        "fxo_xcorr_tic_f32": (None, [vp, i, i, l, vp, i]),
         "fxo_xengine_f64": (None, [vp, i, l, i, vp]),
         "fxo_corr_f32": (None, [vp, i, i, l, vp, vp, i]),
         "fxo_corr_f64": (None, [vp, i, i, l, vp, vp, i]),
@@ -227,6 +240,26 @@ def xengine_f32(X):
     C_, T, I = X.shape
     vis = np.zeros((I * (I + 1) // 2, C_), dtype=np.complex64)
     load().fxo_xengine_f32(_p(X), C_, T, I, _p(vis))
+    return vis
+
+
+def xengine_f32_mt(X, nthreads):
+    """xengine_f32 with the channels shared among nthreads threads (same sums)."""
+    X = np.ascontiguousarray(X, np.complex64)
+    C_, T, I = X.shape
+    vis = np.zeros((I * (I + 1) // 2, C_), dtype=np.complex64)
+    load().fxo_xengine_f32_mt(_p(X), C_, T, I, _p(vis), nthreads)
+    return vis
+
+
+def xcorr_tic_f32(X, nthreads):
+    """X[t][i][c] complex64 -> vis[p][c]: the reference's loop order
+    (genericAddProduct_32fc_conj over the channels per input pair),
+    time-sliced over nthreads workers summed in worker order."""
+    X = np.ascontiguousarray(X, np.complex64)
+    T, I, C_ = X.shape
+    vis = np.zeros((I * (I + 1) // 2, C_), dtype=np.complex64)
+    load().fxo_xcorr_tic_f32(_p(X), I, C_, T, _p(vis), nthreads)
     return vis
 
 

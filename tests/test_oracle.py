@@ -471,3 +471,19 @@ def test_xpol_oracle_pins(window):
     k = np.arange(20, n // 2 - 20)
     err = np.angle(xd[:, k].sum(axis=0) * np.exp(-2j * np.pi * k * d / n))
     assert np.median(np.abs(err)) < 0.05
+
+
+def test_threaded_and_channelized_xengines():
+    """The oracle's threaded X-engine equals the serial one bit for bit, and
+    the channelized one in the generic core's loop order (input pairs outer,
+    genericAddProduct_32fc_conj over the channels) gives the FP64 sums."""
+    rng = np.random.default_rng(77)
+    C_, T, I = 7, 33, 11
+    X = (rng.standard_normal((C_, T, I)) + 1j * rng.standard_normal((C_, T, I))).astype(np.complex64)
+    a = O.xengine_f32(X)
+    assert np.array_equal(O.xengine_f32_mt(X, 3).view(np.uint32), a.view(np.uint32))
+    ref = np.stack([(X[c].T.astype(np.complex128) @ X[c].conj().astype(np.complex128))[np.triu_indices(I)]
+                    for c in range(C_)], 1)
+    for nth in (1, 4):
+        got = O.xcorr_tic_f32(X.transpose(1, 2, 0), nth)
+        assert np.abs(got - ref).max() < 1e-5 * np.abs(ref).max()
