@@ -445,10 +445,12 @@ int fxk_xpol_groups(int c1, int K, long nspec);
 int fxk_xpol_acc(hipStream_t s, const float2* A, const float2* B, int c1, int K, long nspec, float scale,
                  float2* part, int groups, float2* out);
 // phase-cal of nspec TaskCore spectra of L samples at d_words (1 channel),
-// spectrum indices counted from spec0; fold: grown device scratch
+// spectrum indices counted from spec0; fold: grown device scratch; kuse
+// (< 0: K): FFTs of each spectrum taken (a trailing partial spectrum has
+// fewer than K)
 int fx_spec_pcal(fx_ctx* ctx, const uint32_t* d_words, long nspec, long L, long spec0, int N, int ov, int K, int R,
                  int T, double dphi, void*& fold, size_t& fold_cap, std::vector<fx_cf64>& out,
-                 const uint8_t* d_wvalid = nullptr);
+                 const uint8_t* d_wvalid = nullptr, int kuse = -1);
 int fxk_frames_place(hipStream_t s, int format, const uint8_t* raw, long nframes, int framebytes,
                      int payload_offset, const int* thread_row, int map_size, int nrows, long sec0, long frame0,
                      int fps,

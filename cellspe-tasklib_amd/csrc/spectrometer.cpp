@@ -13,7 +13,9 @@
 #include "celldsp_tasklib.h"
 #include "spectrometer_fx.h"
 
+#include <climits>
 #include <cmath>
+#include <cstdint>
 #include <cstring>
 #include <numeric>
 #include <string>
@@ -110,8 +112,9 @@ extern "C" int spectrometer_cb_run(const control_block_t* cb, spectrometer_mbox_
 // 633-693) are then applied in FP64 on rotatorlen values per spectrum.
 int fx_spec_pcal(fx_ctx* ctx, const uint32_t* d_words, long nspec, long L, long spec0, int N, int ov, int K, int R,
                  int T, double dphi, void*& fold, size_t& fold_cap, std::vector<fx_cf64>& out,
-                 const uint8_t* d_wvalid)
+                 const uint8_t* d_wvalid, int kuse)
 {
+    if (kuse < 0) kuse = K;
     const long hop = N / ov;
     const size_t bytes = (size_t)nspec * (size_t)R * sizeof(double);
     if (fold_cap < bytes) {
@@ -126,8 +129,8 @@ int fx_spec_pcal(fx_ctx* ctx, const uint32_t* d_words, long nspec, long L, long 
     for (long sp = 0; sp < nspec; sp++) {
         const long g0 = (spec0 + sp) * (long)K;      // index of the spectrum's first FFT
         const long j0 = (ov - g0 % ov) % ov;         // first selected FFT of this spectrum
-        if (j0 >= K) continue;
-        const long cnt = (K - 1 - j0) / ov + 1;
+        if (j0 >= kuse) continue;
+        const long cnt = (kuse - 1 - j0) / ov + 1;
         const size_t start = (size_t)(sp * L + j0 * hop);
         if (fxk_pcal_fold(st, d_words + start / 16, (size_t)(cnt * N), 0, R, (double*)fold + (size_t)sp * R, 1, 0,
                           d_wvalid ? d_wvalid + start / 16 : nullptr) != FX_OK) {
@@ -179,6 +182,16 @@ struct fx_taskcore {
     std::vector<float> last;
     std::vector<fx_cf64> last_pcal;
     std::string err;
+    // fx_taskcore_run_buffers: device copies of the sources, the last run's
+    // spectra per source / pair and phase-cal per source
+    int nsrc = 0;
+    std::vector<void*> srcw;
+    std::vector<size_t> srcw_cap;
+    void* xout = nullptr;
+    size_t xout_cap = 0;
+    std::vector<std::vector<float>> m_auto;
+    std::vector<std::vector<fx_cf32>> m_xpol;
+    std::vector<std::vector<fx_cf64>> m_pcal;
 };
 
 namespace {
@@ -256,6 +269,169 @@ int core_work(fx_taskcore* c, const void* raw, size_t rawbytes, float* spectra_o
     return (int)nspec;
 }
 
+// The IPP core's doMaths over nsources sources (TaskCoreIPP.cpp:356-560);
+// returns the spectra written (full + partial) or -1.
+int core_work_buffers(fx_taskcore* c, int ns, const std::vector<fx_buffer*>& in, const std::vector<fx_buffer*>& outb,
+                      const std::vector<fx_buffer*>& xpb, const std::vector<fx_buffer*>& pcb)
+{
+    const fx_spectrometer_settings& s = c->s;
+    const int N = s.fft_points, ov = s.fft_overlap_factor, K = s.core_overlapped_ffts;
+    const int nch = s.source_channels;
+    const int C1 = N / 2 + 1;
+    const long hop = N / ov;
+    const int ch0 = (s.use_channel ? s.use_channel : 1) - 1;
+    const int ch1 = s.use_channel_file2 ? s.use_channel_file2 - 1 : ch0;
+    auto chan_of = [&](int k) { return k == 0 ? ch0 : ch1; };
+    const int np = ns * (ns - 1) / 2;
+    c->m_auto.assign((size_t)ns, {});
+    c->m_xpol.assign((size_t)(xpb.empty() ? 0 : np), {});
+    c->m_pcal.assign((size_t)ns, {});
+    // samples per channel every source still has
+    size_t ntime = SIZE_MAX;
+    for (int k = 0; k < ns; k++) ntime = std::min(ntime, in[(size_t)k]->length / 4 * 16 / (size_t)nch);
+    const long L = (long)(K - 1) * hop + N;
+    const long nfull = (long)(ntime / (size_t)L);
+    const long rest = (long)ntime - nfull * L;
+    // whole FFTs left after the last full spectrum (at most K-1)
+    const long f = rest >= N ? std::min<long>(K - 1, (rest - N) / hop + 1) : 0;
+    const long total = nfull + (f > 0 ? 1 : 0);
+    const bool pcal = c->tonebins > 0 && !pcb.empty();
+    const int T = c->tonebins;
+    for (int k = 0; k < ns; k++) {
+        if (outb[(size_t)k]->allocated < (size_t)total * C1 * sizeof(float)) {
+            c->err = "outbuf too small for " + std::to_string(total) + " spectra";
+            return -1;
+        }
+        if (pcal && pcb[(size_t)k]->allocated < (size_t)total * T * sizeof(fx_cf32)) {
+            c->err = "pcalbuf too small";
+            return -1;
+        }
+    }
+    for (int x = 0; x < (int)c->m_xpol.size(); x++)
+        if (xpb[(size_t)x]->allocated < (size_t)total * C1 * sizeof(fx_cf32)) {
+            c->err = "xpolbuf too small";
+            return -1;
+        }
+    if (total == 0) {
+        for (int k = 0; k < ns; k++) {
+            outb[(size_t)k]->length = 0;
+            if (pcal) pcb[(size_t)k]->length = 0;
+        }
+        for (size_t x = 0; x < c->m_xpol.size(); x++) xpb[x]->length = 0;
+        return 0;
+    }
+    // stage the samples the spectra use (whole words; every spectrum and FFT
+    // start is a multiple of 16 samples, so of nch/16 words)
+    const size_t used = (size_t)(nfull * L + (f ? (f - 1) * hop + N : 0));
+    const size_t nwords = (used * (size_t)nch + 15) / 16;
+    c->srcw.resize((size_t)ns, nullptr);
+    c->srcw_cap.resize((size_t)ns, 0);
+    for (int k = 0; k < ns; k++) {
+        if (ensure(c->ctx, c->srcw[(size_t)k], c->srcw_cap[(size_t)k], nwords * 4 + 16)) return -1;
+        if (fx_memcpy_h2d(c->ctx, c->srcw[(size_t)k], in[(size_t)k]->data, nwords * 4)) return -1;
+    }
+    const size_t tail_off = (size_t)nfull * L * nch / 16;   // words to the partial spectrum
+    const size_t tail_t = f ? (size_t)((f - 1) * hop + N) : 0;
+    // autos: every channel of each source through the device spectrometer,
+    // then its selected channel
+    if (ensure(c->ctx, c->out, c->out_cap, (size_t)std::max<long>(nfull, 1) * nch * C1 * sizeof(float))) return -1;
+    std::vector<float> h;
+    for (int k = 0; k < ns; k++) {
+        std::vector<float>& a = c->m_auto[(size_t)k];
+        a.assign((size_t)total * C1, 0.f);
+        const uint32_t* w = (const uint32_t*)c->srcw[(size_t)k];
+        const int ch = chan_of(k);
+        if (nfull) {
+            long got = 0;
+            if (fx_spectrometer_ex(c->ctx, w, (size_t)nfull * L, nch, N, ov, K, s.window, 0, 1, (float*)c->out,
+                                   &got))
+                return -1;
+            h.resize((size_t)nfull * nch * C1);
+            if (fx_memcpy_d2h(c->ctx, h.data(), c->out, h.size() * sizeof(float))) return -1;
+            for (long sp = 0; sp < nfull; sp++)
+                memcpy(&a[(size_t)sp * C1], &h[((size_t)sp * nch + ch) * C1], C1 * sizeof(float));
+        }
+        if (f) {
+            // the partial spectrum holds sums: the mean of f FFTs times f
+            long got = 0;
+            if (fx_spectrometer_ex(c->ctx, w + tail_off, tail_t, nch, N, ov, (int)f, s.window, 0, 1,
+                                   (float*)c->out, &got))
+                return -1;
+            h.resize((size_t)nch * C1);
+            if (fx_memcpy_d2h(c->ctx, h.data(), c->out, h.size() * sizeof(float))) return -1;
+            for (int b = 0; b < C1; b++) a[(size_t)nfull * C1 + b] = h[(size_t)ch * C1 + b] * (float)f;
+        }
+    }
+    // cross spectra of every source pair
+    if (!c->m_xpol.empty()) {
+        if (ensure(c->ctx, c->xout, c->xout_cap, (size_t)std::max<long>(nfull, 1) * C1 * sizeof(fx_cf32))) return -1;
+        int x = 0;
+        for (int a = 0; a < ns; a++)
+            for (int b = a + 1; b < ns; b++, x++) {
+                std::vector<fx_cf32>& v = c->m_xpol[(size_t)x];
+                v.assign((size_t)total * C1, fx_cf32{0.f, 0.f});
+                const uint32_t* wa = (const uint32_t*)c->srcw[(size_t)a];
+                const uint32_t* wb = (const uint32_t*)c->srcw[(size_t)b];
+                long got = 0;
+                if (nfull) {
+                    if (fx_spectrometer_xpol(c->ctx, wa, nch, chan_of(a), wb, nch, chan_of(b), (size_t)nfull * L, N, ov,
+                                             K, s.window, 0, (fx_cf32*)c->xout, &got))
+                        return -1;
+                    if (fx_memcpy_d2h(c->ctx, v.data(), c->xout, (size_t)nfull * C1 * sizeof(fx_cf32))) return -1;
+                }
+                if (f) {
+                    if (fx_spectrometer_xpol(c->ctx, wa + tail_off, nch, chan_of(a), wb + tail_off, nch, chan_of(b),
+                                             tail_t, N, ov, (int)f, s.window, 0, (fx_cf32*)c->xout, &got))
+                        return -1;
+                    fx_cf32* t = &v[(size_t)nfull * C1];
+                    if (fx_memcpy_d2h(c->ctx, t, c->xout, (size_t)C1 * sizeof(fx_cf32))) return -1;
+                    for (int b2 = 0; b2 < C1; b2++) {
+                        t[b2].re *= (float)f;
+                        t[b2].im *= (float)f;
+                    }
+                }
+            }
+    }
+    // phase-cal of each source (FP64 accumulation)
+    if (pcal) {
+        std::vector<fx_cf64> part;
+        for (int k = 0; k < ns; k++) {
+            const uint32_t* w = (const uint32_t*)c->srcw[(size_t)k];
+            std::vector<fx_cf64>& pv = c->m_pcal[(size_t)k];
+            pv.assign((size_t)total * T, fx_cf64{0.0, 0.0});
+            if (nfull) {
+                if (fx_spec_pcal(c->ctx, w, nfull, L, 0, N, ov, K, c->rotatorlen, T, c->dphi, c->fold, c->fold_cap,
+                                 part))
+                    return -1;
+                std::copy(part.begin(), part.end(), pv.begin());
+            }
+            if (f) {
+                if (fx_spec_pcal(c->ctx, w + tail_off, 1, L, nfull, N, ov, K, c->rotatorlen, T, c->dphi, c->fold,
+                                 c->fold_cap, part, nullptr, (int)f))
+                    return -1;
+                std::copy(part.begin(), part.end(), pv.begin() + (size_t)nfull * T);
+            }
+        }
+    }
+    // into the caller's buffers
+    for (int k = 0; k < ns; k++) {
+        memcpy(outb[(size_t)k]->data, c->m_auto[(size_t)k].data(), (size_t)total * C1 * sizeof(float));
+        outb[(size_t)k]->length = (size_t)total * C1 * sizeof(float);
+        if (pcal) {
+            fx_cf32* o = (fx_cf32*)pcb[(size_t)k]->data;
+            for (size_t e = 0; e < (size_t)total * T; e++)
+                o[e] = fx_cf32{(float)c->m_pcal[(size_t)k][e].re, (float)c->m_pcal[(size_t)k][e].im};
+            pcb[(size_t)k]->length = (size_t)total * T * sizeof(fx_cf32);
+        }
+    }
+    for (size_t x = 0; x < c->m_xpol.size(); x++) {
+        memcpy(xpb[x]->data, c->m_xpol[x].data(), (size_t)total * C1 * sizeof(fx_cf32));
+        xpb[x]->length = (size_t)total * C1 * sizeof(fx_cf32);
+    }
+    c->nsrc = ns;
+    return (int)total;
+}
+
 } // namespace
 
 extern "C" int fx_taskcore_create(int rank, const fx_spectrometer_settings* s, fx_taskcore** out)
@@ -275,7 +451,9 @@ extern "C" int fx_taskcore_create(int rank, const fx_spectrometer_settings* s, f
     const int nc = s->source_channels;
     FX_REQUIRE(nc == 1 || nc == 2 || nc == 4 || nc == 8 || nc == 16, FX_ERR_ARG,
                "fx_taskcore_create: source_channels %d", nc);
-    FX_REQUIRE(s->use_channel >= 0 && s->use_channel <= nc, FX_ERR_ARG, "fx_taskcore_create: use_channel");
+    FX_REQUIRE(s->use_channel >= 0 && s->use_channel <= nc && s->use_channel_file2 >= 0 &&
+                   s->use_channel_file2 <= nc,
+               FX_ERR_ARG, "fx_taskcore_create: use_channel");
     int T = 0, R = 0;
     if (s->extract_pcal) {
         FX_REQUIRE(nc == 1, FX_ERR_UNSUPPORTED, "fx_taskcore_create: phase-cal on multi-channel sources");
@@ -324,6 +502,84 @@ extern "C" int fx_taskcore_run(fx_taskcore* c, const void* raw, size_t rawbytes,
     return FX_OK;
 }
 
+extern "C" int fx_taskcore_run_buffers(fx_taskcore* c, int ns, fx_buffer* const* inbuf, fx_buffer* const* outbuf,
+                                       fx_buffer* const* xpolbuf, fx_buffer* const* pcalbuf)
+{
+    FX_REQUIRE(c && inbuf && outbuf, FX_ERR_ARG, "fx_taskcore_run_buffers: null argument");
+    FX_REQUIRE(ns >= 1 && ns <= FX_TC_MAX_SOURCES, FX_ERR_ARG, "fx_taskcore_run_buffers: %d sources", ns);
+    FX_REQUIRE(!c->th.joinable(), FX_ERR_STATE, "fx_taskcore_run_buffers: previous run not joined");
+    const int np = ns * (ns - 1) / 2;
+    std::vector<fx_buffer*> in((size_t)ns), out((size_t)ns), xp, pc;
+    for (int k = 0; k < ns; k++) {
+        FX_REQUIRE(inbuf[k] && outbuf[k] && (inbuf[k]->data || !inbuf[k]->length) && outbuf[k]->data, FX_ERR_ARG,
+                   "fx_taskcore_run_buffers: source %d buffers", k);
+        in[(size_t)k] = inbuf[k];
+        out[(size_t)k] = outbuf[k];
+    }
+    if (xpolbuf && np) {
+        for (int x = 0; x < np; x++) {
+            FX_REQUIRE(xpolbuf[x] && xpolbuf[x]->data, FX_ERR_ARG, "fx_taskcore_run_buffers: xpol buffer %d", x);
+            xp.push_back(xpolbuf[x]);
+        }
+    }
+    if (pcalbuf && c->tonebins > 0) {
+        for (int k = 0; k < ns; k++) {
+            FX_REQUIRE(pcalbuf[k] && pcalbuf[k]->data, FX_ERR_ARG, "fx_taskcore_run_buffers: pcal buffer %d", k);
+            pc.push_back(pcalbuf[k]);
+        }
+    }
+    c->err.clear();
+    c->result = 0;
+    c->th = std::thread([=]() {
+        c->result = core_work_buffers(c, ns, in, out, xp, pc);
+        if (c->result < 0 && c->err.empty()) c->err = fx_last_error();
+    });
+    return FX_OK;
+}
+
+extern "C" int fx_taskcore_combine_buffers(fx_taskcore* c, fx_buffer* const* outbuf, fx_buffer* const* pcalbuf)
+{
+    FX_REQUIRE(c && outbuf, FX_ERR_ARG, "fx_taskcore_combine_buffers: null argument");
+    FX_REQUIRE(!c->th.joinable(), FX_ERR_STATE, "fx_taskcore_combine_buffers: run not joined");
+    const int C1 = c->s.fft_points / 2 + 1;
+    for (int k = 0; k < c->nsrc; k++) {
+        const std::vector<float>& a = c->m_auto[(size_t)k];
+        FX_REQUIRE(outbuf[k] && outbuf[k]->allocated >= C1 * sizeof(float), FX_ERR_ARG,
+                   "fx_taskcore_combine_buffers: outbuf %d", k);
+        float* o = (float*)outbuf[k]->data;
+        for (int b = 0; b < C1 && (size_t)b < a.size(); b++) o[b] += a[(size_t)b];
+        outbuf[k]->length = C1 * sizeof(float);
+    }
+    for (size_t x = 0; x < c->m_xpol.size(); x++) {
+        fx_buffer* ob = outbuf[(size_t)c->nsrc + x];
+        FX_REQUIRE(ob && ob->allocated >= C1 * sizeof(fx_cf32), FX_ERR_ARG,
+                   "fx_taskcore_combine_buffers: cross-spectrum outbuf %zu", x);
+        fx_cf32* o = (fx_cf32*)ob->data;
+        const std::vector<fx_cf32>& v = c->m_xpol[x];
+        for (int b = 0; b < C1 && (size_t)b < v.size(); b++) {
+            o[b].re += v[(size_t)b].re;
+            o[b].im += v[(size_t)b].im;
+        }
+        ob->length = C1 * sizeof(fx_cf32);
+    }
+    if (pcalbuf && c->tonebins > 0) {
+        const int T = c->tonebins;
+        for (int k = 0; k < c->nsrc; k++) {
+            const std::vector<fx_cf64>& v = c->m_pcal[(size_t)k];
+            if (v.empty()) continue;
+            FX_REQUIRE(pcalbuf[k] && pcalbuf[k]->allocated >= T * sizeof(fx_cf32), FX_ERR_ARG,
+                       "fx_taskcore_combine_buffers: pcalbuf %d", k);
+            fx_cf32* o = (fx_cf32*)pcalbuf[k]->data;
+            for (int b = 0; b < T; b++) {
+                o[b].re += (float)v[(size_t)b].re;
+                o[b].im += (float)v[(size_t)b].im;
+            }
+            pcalbuf[k]->length = T * sizeof(fx_cf32);
+        }
+    }
+    return FX_OK;
+}
+
 extern "C" int fx_taskcore_join(fx_taskcore* c)
 {
     FX_REQUIRE(c, -1, "fx_taskcore_join: null core");
@@ -368,6 +624,9 @@ extern "C" int fx_taskcore_destroy(fx_taskcore* c)
         if (c->words) fx_free(c->ctx, c->words);
         if (c->out) fx_free(c->ctx, c->out);
         if (c->fold) fx_free(c->ctx, c->fold);
+        if (c->xout) fx_free(c->ctx, c->xout);
+        for (void* w : c->srcw)
+            if (w) fx_free(c->ctx, w);
         fx_ctx_destroy(c->ctx);
     }
     delete c;

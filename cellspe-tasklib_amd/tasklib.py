@@ -146,7 +146,13 @@ class SpectrometerSettings(C.Structure):
         ("samplingfreq", C.c_double),
         ("pcaloffsethz", C.c_double),
         ("pcalharmonicshz", C.c_double),
+        ("use_channel_file2", C.c_int),
     ]
+
+
+class FxBuffer(C.Structure):
+    """fx_buffer: the reference Buffer (data, allocated bytes, length)."""
+    _fields_ = [("data", C.c_void_p), ("allocated", C.c_size_t), ("length", C.c_size_t)]
 
 
 class SwspecSettings(C.Structure):
@@ -215,6 +221,8 @@ def load():
         "fx_taskcore_create": (i, [i, C.POINTER(SpectrometerSettings), C.POINTER(vp)]),
         "fx_taskcore_run": (i, [vp, vp, sz, vp, sz, vp, sz]),
         "fx_taskcore_join": (i, [vp]),
+        "fx_taskcore_run_buffers": (i, [vp, i, vp, vp, vp, vp]),
+        "fx_taskcore_combine_buffers": (i, [vp, vp, vp]),
         "fx_taskcore_combine": (i, [vp, vp, sz, vp, sz]),
         "fx_taskcore_pcal_length": (i, [vp]),
         "fx_taskcore_finalize": (i, [vp]),
@@ -389,6 +397,60 @@ class TaskCoreHIP:
 
     def results(self):
         return self._spec[: self.nspec], self._pcal[: self.nspec, : self.pcal_len]
+
+    @staticmethod
+    def _bufs(arrays):
+        bufs = [FxBuffer(a.ctypes.data, a.nbytes, 0) for a in arrays]
+        ptrs = (C.POINTER(FxBuffer) * len(bufs))(*[C.pointer(b) for b in bufs])
+        return bufs, ptrs
+
+    def run_buffers(self, raws, max_spectra, xpol=True, pcal=True):
+        """TaskCore::run(inbuf, outbuf, xpolbuf, pcalbuf) over len(raws)
+        sources (fx_taskcore_run_buffers); the outputs are numpy arrays sized
+        for max_spectra spectra."""
+        s = self.settings
+        C1 = s.fft_points // 2 + 1
+        ns = len(raws)
+        npair = ns * (ns - 1) // 2
+        m = max(max_spectra, 1)
+        self._in = [np.ascontiguousarray(r, np.uint8) for r in raws]
+        self._auto = [np.zeros((m, C1), np.float32) for _ in range(ns)]
+        self._xpol = [np.zeros((m, C1), np.complex64) for _ in range(npair)] if xpol else []
+        self._pcb = [np.zeros((m, max(self.pcal_len, 1)), np.complex64) for _ in range(ns)] \
+            if pcal and self.pcal_len else []
+        self._ib, ip = self._bufs(self._in)
+        for b, a in zip(self._ib, self._in):
+            b.length = a.nbytes
+        self._ob, op = self._bufs(self._auto)
+        self._xb, xp = self._bufs(self._xpol) if self._xpol else ([], None)
+        self._pb, pp = self._bufs(self._pcb) if self._pcb else ([], None)
+        self._ptrs = (ip, op, xp, pp)
+        rc = load().fx_taskcore_run_buffers(self.handle, ns, C.cast(ip, C.c_void_p), C.cast(op, C.c_void_p),
+                                            None if xp is None else C.cast(xp, C.c_void_p),
+                                            None if pp is None else C.cast(pp, C.c_void_p))
+        if rc != 0:
+            raise FxError(f"TaskCore run: {last_error()}")
+        return 0
+
+    def results_buffers(self):
+        """(autos per source, cross spectra per pair, phase-cal per source),
+        each cut to the length the core set."""
+        C1 = self.settings.fft_points // 2 + 1
+        autos = [a[: b.length // (4 * C1)] for a, b in zip(self._auto, self._ob)]
+        xp = [a[: b.length // (8 * C1)] for a, b in zip(self._xpol, self._xb)]
+        pc = [a[: b.length // (8 * self.pcal_len), : self.pcal_len] for a, b in zip(self._pcb, self._pb)]
+        return autos, xp, pc
+
+    def combine_buffers(self, outs, pcals=None):
+        """combineResults(outbuf, pcalbuf): outs = autos of each source then
+        the cross spectra; returns the lengths the core set."""
+        ob, op = self._bufs(outs)
+        pb, pp = self._bufs(pcals) if pcals else ([], None)
+        rc = load().fx_taskcore_combine_buffers(self.handle, C.cast(op, C.c_void_p),
+                                                None if pp is None else C.cast(pp, C.c_void_p))
+        if rc != 0:
+            raise FxError(f"TaskCore combineResults: {last_error()}")
+        return [b.length for b in ob], [b.length for b in pb]
 
     def combineResults(self, out, pcal=None):
         rc = load().fx_taskcore_combine(self.handle, _p(out), out.size,

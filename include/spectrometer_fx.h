@@ -82,6 +82,8 @@ typedef struct {
     double samplingfreq;        /* Hz */
     double pcaloffsethz;
     double pcalharmonicshz;
+    int use_channel_file2;      /* 1-based channel of sources 2.. in fx_taskcore_run_buffers
+                                   (use_channel_file2, Settings.h:68); 0: as use_channel */
 } fx_spectrometer_settings;
 
 typedef struct fx_taskcore fx_taskcore;
@@ -102,7 +104,47 @@ int fx_taskcore_create(int rank, const fx_spectrometer_settings* settings, fx_ta
  * join().  Returns 0 when the work was started. */
 int fx_taskcore_run(fx_taskcore* core, const void* raw, size_t rawbytes, float* spectra_out,
                     size_t spectra_capacity_floats, fx_cf64* pcal_out, size_t pcal_capacity);
-/* TaskCore::join: -1 on failure, else the number of complete spectra */
+/* Buffer (Buffer.h): a caller-owned byte array; the core reads `length`
+ * bytes of an input and sets `length` of an output (Buffer::setLength) */
+typedef struct {
+    void* data;        /* Buffer::getData()      */
+    size_t allocated;  /* Buffer::getAllocated() */
+    size_t length;     /* Buffer::getLength()    */
+} fx_buffer;
+/* TaskCore::run(Buffer** inbuf, Buffer** outbuf, Buffer** xpolbuf,
+ * Buffer** pcalbuf) (TaskCore.h:58), the IPP core's doMaths
+ * (TaskCoreIPP.cpp:356-560), asynchronous until join().  nsources raw
+ * inputs (1..FX_TC_MAX_SOURCES); source 0 uses channel use_channel (1 if 0)
+ * of its source_channels, the others use_channel_file2.  The sources are
+ * processed while every one of them has samples left: spectra of
+ * core_overlapped_ffts FFTs hopping fft_points/overlap samples, the next
+ * spectrum after the last sample of the previous one.  Per spectrum:
+ *   outbuf[s]  fft_points/2+1 float32, the mean power (1/K) sum |X|^2;
+ *   xpolbuf[x] (NULL: none) fft_points/2+1 complex float32, (1/K) sum
+ *              X_a conj(X_b) for pair x of (a, b), a < b, in the order
+ *              (0,1), (0,2), ..., (1,2), ...; DC and Nyquist the product of
+ *              the real parts, imaginary part 0;
+ *   pcalbuf[s] (NULL: none; single-channel sources) tonebins complex
+ *              float32, the folded phase-cal pulse of the spectrum's
+ *              non-overlapped FFT inputs (FFT index a multiple of the
+ *              overlap factor), accumulated in FP64.
+ * When the shortest source ends with 1..K-1 whole FFTs past the last full
+ * spectrum, those are written as one more, partial spectrum of unscaled sums
+ * (TaskCoreIPP.cpp:519-524).  Each output's length is set to its spectra's
+ * bytes after the work; allocated bytes that cannot hold them fail the run
+ * (join() returns -1).  The buffers stay the caller's until join(). */
+#define FX_TC_MAX_SOURCES 8
+int fx_taskcore_run_buffers(fx_taskcore* core, int nsources, fx_buffer* const* inbuf, fx_buffer* const* outbuf,
+                            fx_buffer* const* xpolbuf, fx_buffer* const* pcalbuf);
+/* TaskCore::combineResults(Buffer** outbuf, Buffer** pcalbuf) after
+ * fx_taskcore_run_buffers (TaskCoreIPP.cpp:311-345): outbuf[s] += the first
+ * spectrum of source s, outbuf[nsources + x] += the first cross spectrum of
+ * pair x (complex float32), pcalbuf[s] += the first phase-cal (may be NULL);
+ * each length set to one spectrum's bytes. */
+int fx_taskcore_combine_buffers(fx_taskcore* core, fx_buffer* const* outbuf, fx_buffer* const* pcalbuf);
+/* TaskCore::join: -1 on failure, else the number of spectra written
+ * (fx_taskcore_run: complete spectra; fx_taskcore_run_buffers: including a
+ * trailing partial spectrum) */
 int fx_taskcore_join(fx_taskcore* core);
 /* TaskCore::combineResults (TaskCoreIPP.cpp:311-345): out[k] += this core's
  * last spectra, pcal[k] += its last phase-cal (pcal may be NULL) */

@@ -373,11 +373,11 @@ def test_spectrometer_control_block(N, ov, K):
     assert np.array_equal(p[:, N // 2 + 1:], p[:, 1: N // 2][:, ::-1])
 
 
-def _settings(N, ov, K, win, nch, use, pcal=0, fs=16e6, off=500e3, harm=1e6):
+def _settings(N, ov, K, win, nch, use, pcal=0, fs=16e6, off=500e3, harm=1e6, use2=0):
     return TL.SpectrometerSettings(fft_points=N, fft_overlap_factor=ov, core_overlapped_ffts=K,
                                    window=win, bits_per_sample=2, source_channels=nch,
                                    use_channel=use, extract_pcal=pcal, samplingfreq=fs,
-                                   pcaloffsethz=off, pcalharmonicshz=harm)
+                                   pcaloffsethz=off, pcalharmonicshz=harm, use_channel_file2=use2)
 
 
 @pytest.mark.parametrize("nch,use,win", [(1, 0, O.WIN_HANN), (4, 0, O.WIN_COSINE), (4, 3, O.WIN_BLACKMAN)])
@@ -442,3 +442,97 @@ def test_taskcore_pcal_and_two_cores():
     assert nrel(total, cores[0].results()[0] + cores[1].results()[0]) < 1e-7
     for c in cores:
         c.close()
+
+
+def _power_sum_f64(x, N, hop, nfft, win):
+    """FP64 sum over nfft windowed FFTs of x (hop apart) of |X|^2, bins 0..N/2."""
+    w = O.window(win, N).astype(np.float64)
+    return sum(np.abs(np.fft.rfft(x[j * hop: j * hop + N] * w)) ** 2 for j in range(nfft))
+
+
+def _cross_sum_f64(xa, xb, N, hop, nfft, win):
+    """FP64 sum of X_a conj(X_b) with DC and Nyquist the real-part products."""
+    w = O.window(win, N).astype(np.float64)
+    acc = np.zeros(N // 2 + 1, np.complex128)
+    for j in range(nfft):
+        A = np.fft.rfft(xa[j * hop: j * hop + N] * w)
+        B = np.fft.rfft(xb[j * hop: j * hop + N] * w)
+        p = A * np.conj(B)
+        p[0] = A[0].real * B[0].real
+        p[-1] = A[-1].real * B[-1].real
+        acc += p
+    return acc
+
+
+def test_taskcore_buffers_two_sources_xpol_partial():
+    """TaskCore::run(inbuf, outbuf, xpolbuf, pcalbuf) with two 4-channel
+    sources (channel 2 of the first, channel 3 of the second): the mean
+    power of every full spectrum, the cross spectrum of the pair, and at the
+    shorter source's end a partial spectrum of 3 FFTs holding unscaled sums
+    (TaskCoreIPP.cpp:456-487,519-524); combineResults adds each source's and
+    the pair's first spectrum."""
+    N, ov, K, nch, win = 1024, 2, 5, 4, O.WIN_HANN
+    hop = N // ov
+    L = (K - 1) * hop + N
+    f = 3
+    ntime0 = 3 * L + (f - 1) * hop + N + 48          # 3 full spectra, then 3 whole FFTs and a bit
+    ntime1 = ntime0 + 5000
+    w0 = O.gen_multichannel(41, nch, [0.2, 0.1, 0.0, 0.3], [0.11, 0.2, 0.0, 0.31], 0.9816, ntime0)
+    w1 = O.gen_multichannel(42, nch, [0.0, 0.1, 0.2, 0.3], [0.0, 0.2, 0.11, 0.31], 0.9816, ntime1)
+    core = TL.TaskCoreHIP(rank=0)
+    core.prepare(_settings(N, ov, K, win, nch, 2, use2=3))
+    core.run_buffers([w0.view(np.uint8), w1.view(np.uint8)], max_spectra=8)
+    n = core.join()
+    assert n == 4
+    autos, xps, _ = core.results_buffers()
+    assert [a.shape[0] for a in autos] == [4, 4] and len(xps) == 1 and xps[0].shape[0] == 4
+    ch = (1, 2)
+    for k, w in enumerate((w0, w1)):
+        ref = O.spectrometer_ex_f32(w, 3 * L, nch, N, ov, K, win, False, True)[:, ch[k]]
+        assert nrel(autos[k][:3], ref) < TOL32
+        x = O.unpack_nch(w, ntime0, nch)[ch[k]].astype(np.float64)[3 * L:]
+        assert nrel(autos[k][3], _power_sum_f64(x, N, hop, f, win)) < TOL32
+    xref = O.spectrometer_xpol_f64(w0, nch, ch[0], w1, nch, ch[1], 3 * L, N, ov, K, win)
+    assert nrel(xps[0][:3], xref) < TOL32
+    xa = O.unpack_nch(w0, ntime0, nch)[ch[0]].astype(np.float64)[3 * L:]
+    xb = O.unpack_nch(w1, ntime0, nch)[ch[1]].astype(np.float64)[3 * L:]
+    assert nrel(xps[0][3], _cross_sum_f64(xa, xb, N, hop, f, win)) < TOL32
+    outs = [np.zeros(N // 2 + 1, np.float32), np.full(N // 2 + 1, 1.0, np.float32), np.zeros(N // 2 + 1, np.complex64)]
+    lens, _ = core.combine_buffers(outs)
+    assert lens == [4 * (N // 2 + 1), 4 * (N // 2 + 1), 8 * (N // 2 + 1)]
+    assert np.array_equal(outs[0], autos[0][0]) and np.array_equal(outs[1], autos[1][0] + 1)
+    assert np.array_equal(outs[2], xps[0][0])
+    core.finalize()
+    core.close()
+
+
+def test_taskcore_buffers_pcal_partial():
+    """Phase-cal per source through the Buffer form: complex float32 tone bins
+    per spectrum (FP64 accumulation), the partial spectrum folding only its
+    non-overlapped FFTs; no cross spectrum requested."""
+    N, ov, K = 1024, 2, 5
+    fs, off, harm = 16e6, 500e3, 1e6
+    T, R = 16, 32
+    hop = N // ov
+    L = (K - 1) * hop + N
+    f = 4
+    ntime = 2 * L + (f - 1) * hop + N
+    ws = [O.gen_multichannel(50 + k, 1, [0.3], [(off + (3 + k) * harm) / fs], 0.9816, ntime) for k in range(2)]
+    core = TL.TaskCoreHIP(rank=1)
+    core.prepare(_settings(N, ov, K, O.WIN_SIN2, 1, 0, pcal=1, fs=fs, off=off, harm=harm))
+    core.run_buffers([w.view(np.uint8) for w in ws], max_spectra=4, xpol=False)
+    assert core.join() == 3
+    autos, xps, pcs = core.results_buffers()
+    assert xps == [] and [p.shape for p in pcs] == [(3, T), (3, T)]
+    for k, w in enumerate(ws):
+        x = O.unpack_1ch(w, ntime)
+        for s in range(3):
+            acc = np.zeros(T, np.complex128)
+            for j in range(K if s < 2 else f):
+                if (s * K + j) % ov == 0:
+                    st = s * L + j * hop
+                    O.tcpcal_fold(x[st: st + N], T, R, -2 * np.pi * off / fs, acc)
+            assert nrel(pcs[k][s], acc) < 1e-6
+        ref = O.spectrometer_ex_f32(w, 2 * L, 1, N, ov, K, O.WIN_SIN2, False, True)[:, 0]
+        assert nrel(autos[k][:2], ref) < TOL32
+    core.close()

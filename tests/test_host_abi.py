@@ -304,3 +304,69 @@ def test_device_commands_report_generic_error_without_device(runtime):
     t.addOutVec(out)
     assert lib.celldsp_executeTask(t.getTask()) == 1
     assert t.completed_flag == TL.CELLDSP_TASK_GENERICERROR
+
+
+# ---------------------------------------------------------------- reference declarations
+REF = "/root/reference"
+
+
+def _struct_fields(path, name):
+    """Member declarations of `typedef struct ... { ... } name` in a header, as
+    normalized (type, declarator) strings: comments, alignment attributes and
+    qualifiers that do not change the layout removed."""
+    text = open(path).read()
+    text = re.sub(r"/\*.*?\*/", " ", text, flags=re.S)
+    text = re.sub(r"//[^\n]*", " ", text)
+    m = re.search(r"\}\s*" + name + r"\b[^;]*;", text)
+    assert m, (path, name)
+    end = m.start()
+    depth, i = 0, end
+    while True:           # back to the brace that opens this body
+        i -= 1
+        if text[i] == "}":
+            depth += 1
+        elif text[i] == "{":
+            if depth == 0:
+                break
+            depth -= 1
+    out = []
+    for decl in text[i + 1:end].split(";"):
+        d = re.sub(r"__attribute__\s*\(\((?:[^()]|\([^()]*\))*\)\)|_QUAD_ALIGN|_CLINE_ALIGN|\bvolatile\b", " ",
+                   decl)
+        d = " ".join(d.split())
+        if d:
+            out.append(d.replace(" [", "["))
+    return out
+
+
+@pytest.mark.skipif(not os.path.isdir(REF), reason="the reference tree is not present")
+@pytest.mark.parametrize("ref_file,name,ours", [
+    ("cellspe-tasklib.h", "CellDSPTask_t", "celldsp_tasklib.h"),
+    ("minicorrelator/minicorrelator.h", "control_block", "minicorrelator_fx.h"),
+    ("spectrometer/src/Cell/cell_spectrometer.h", "control_block_t", "spectrometer_fx.h"),
+])
+def test_abi_structs_match_reference_declarations(ref_file, name, ours):
+    """The drop-in structs declare the reference's members, in order, with the
+    same types and array sizes (the reference headers need the Cell SDK, so
+    they are compared as declarations rather than compiled)."""
+    ref = _struct_fields(os.path.join(REF, ref_file), name)
+    got = _struct_fields(os.path.join(INC, ours), name)
+    assert got == ref
+
+
+@pytest.mark.skipif(not os.path.isdir(os.path.join(REF, "spectrometer/src")),
+                    reason="the reference tree is not present")
+def test_integration_taskcore_stub_compiles(tmp_path):
+    """The TaskCoreHIP class a maintainer adds compiles against
+    the reference's own TaskCore.h, Settings.h and Buffer.h and implements
+    every pure virtual of TaskCore (it is instantiated through the base)."""
+    doc = open(os.path.join(os.path.dirname(INC), "INTEGRATION.md")).read()
+    sec = doc[doc.index("## 3. Spectrometer"):]
+    code = sec[sec.index("```cpp") + 6: sec.index("```", sec.index("```cpp") + 6)]
+    (tmp_path / "TaskCoreHIP.h").write_text("#include <cstring>\n" + code)
+    (tmp_path / "main.cpp").write_text('#include "TaskCoreHIP.h"\n'
+                                       "int main() { TaskCore* t = new TaskCoreHIP(0); delete t; return 0; }\n")
+    r = subprocess.run(["g++", "-std=c++11", "-fsyntax-only", "-Wall", "-I", str(tmp_path), "-I",
+                        os.path.join(REF, "spectrometer/src"), "-I", INC, str(tmp_path / "main.cpp")],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr[-3000:]
