@@ -661,8 +661,12 @@ def batched(fxlib, ctx, args):
     ctx.sync()
     tune = os.path.join(tempfile.gettempdir(), f"fx_tune_{os.getpid()}.txt")
     t0 = time.time()
-    g16 = fxlib.Batch(ctx, subs, fxlib.FX_PREC_F16, use_graph=True, tune_file=tune)
-    log(f"[bench] batch: {nsub} subbands, autotuned + captured in {time.time() - t0:.1f}s")
+    p16 = fxlib.Batch(ctx, subs, fxlib.FX_PREC_F16, use_graph=fxlib.FX_BATCH_PERSISTENT, tune_file=tune)
+    log(f"[bench] batch: {nsub} subbands, persistent table autotuned + captured in {time.time() - t0:.1f}s")
+    ph16 = fxlib.Batch(ctx, subs, fxlib.FX_PREC_F16, use_graph=fxlib.FX_BATCH_PHASED, tune_file=tune)
+    t0 = time.time()
+    g16 = fxlib.Batch(ctx, subs, fxlib.FX_PREC_F16, use_graph=fxlib.FX_BATCH_GRAPH, tune_file=tune)
+    log(f"[bench] batch: per-subband graph autotuned + captured in {time.time() - t0:.1f}s")
     e32 = fxlib.Batch(ctx, subs, fxlib.FX_PREC_F32, use_graph=False)
 
     def timed(b, reps):
@@ -676,9 +680,14 @@ def batched(fxlib, ctx, args):
         ctx.sync()
         return ctx.elapsed_ms(e0, e1) / reps
 
+    msp = timed(p16, args.steps)
+    msph = timed(ph16, args.steps)
     ms16 = timed(g16, args.steps)
     ms32 = timed(e32, max(1, args.steps - 1))
-    log(f"[bench] batch: fp16 graph {ms16:.2f} ms, fp32 one launch each {ms32:.2f} ms")
+    p16.sync()
+    nitems, nblocks, lds = p16.persist_info()
+    log(f"[bench] batch: fp16 persistent {msp:.2f} ms ({nitems} items on {nblocks} workgroups), phased "
+        f"{msph:.2f} ms, per-subband graph {ms16:.2f} ms, fp32 one launch each {ms32:.2f} ms")
     err = None
     nth = cpu_threads()[0]
     frac = 64
@@ -688,7 +697,7 @@ def batched(fxlib, ctx, args):
         O = args.oracle
         err = {"sample": f"every subband, its first 1/{frac} of the spectra (>= 8)"}
         sub16 = fxlib.Batch(ctx, [dict(s, nspectra=max(8, s["nspectra"] // frac)) for s in subs],
-                            fxlib.FX_PREC_F16, use_graph=True, tune_file=tune)
+                            fxlib.FX_PREC_F16, use_graph=fxlib.FX_BATCH_PERSISTENT, tune_file=tune)
         sub32 = fxlib.Batch(ctx, [dict(s, nspectra=max(8, s["nspectra"] // frac)) for s in subs],
                             fxlib.FX_PREC_F32, use_graph=False)
         for b in (sub16, sub32):
@@ -729,8 +738,9 @@ def batched(fxlib, ctx, args):
                       f"FP32 F -> X, no phase-cal, {dt:.2f} s per run", {"input_gsps": smp / dt / 1e9})
         log(f"[bench] batch CPU baseline: {cm / dt:.3e} CMAC/s on {nth} threads")
     tuning = [g16.tuning(k) for k in range(nsub)]
-    g16.close()
-    e32.close()
+    ptuning = [p16.persist_tuning(k) for k in range(nsub)]
+    for b in (p16, ph16, g16, e32):
+        b.close()
     for w in bufs:
         w.free()
     try:
@@ -740,12 +750,16 @@ def batched(fxlib, ctx, args):
     return {"workload": f"{nsub} subbands, N in {{256,1024,4096,16384}}, 16-96 inputs, rho=0.03, "
                         f"2^22 samples per input, FP64 phase-cal of every input",
             "subbands": [{"nfft": s["nfft"], "inputs": s["ninp"], "spectra": s["nspectra"],
-                          "sub_spectra": t[0], "xtile": t[1]} for s, t in zip(subs, tuning)],
-            "ms": {"f16_graph": ms16, "f32_one_launch_each": ms32},
-            "complex_mac_per_s": {"f16_graph": tot_cmac / (ms16 * 1e-3), "f32_one_launch_each": tot_cmac / (ms32 * 1e-3)},
-            "input_gsps": {"f16_graph": tot_samp / (ms16 * 1e-3) / 1e9,
+                          "graph_sub_spectra": t[0], "graph_xtile": t[1], "persistent_xtile": q[0],
+                          "persistent_channels_per_x_item": q[1], "persistent_f_groups_per_item": q[2]}
+                         for s, t, q in zip(subs, tuning, ptuning)],
+            "persistent_launch": {"items": nitems, "workgroups": nblocks, "lds_bytes": lds},
+            "ms": {"f16_persistent": msp, "f16_phased": msph, "f16_graph": ms16, "f32_one_launch_each": ms32},
+            "complex_mac_per_s": {"f16_persistent": tot_cmac / (msp * 1e-3), "f16_graph": tot_cmac / (ms16 * 1e-3),
+                                  "f32_one_launch_each": tot_cmac / (ms32 * 1e-3)},
+            "input_gsps": {"f16_persistent": tot_samp / (msp * 1e-3) / 1e9, "f16_graph": tot_samp / (ms16 * 1e-3) / 1e9,
                            "f32_one_launch_each": tot_samp / (ms32 * 1e-3) / 1e9},
-            "speedup": ms32 / ms16, "complex_macs": tot_cmac, "err": err, "cpu_baseline": cpu}
+            "speedup": ms32 / msp, "complex_macs": tot_cmac, "err": err, "cpu_baseline": cpu}
 
 
 def main():
@@ -776,6 +790,8 @@ def main():
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="instead of the benchmark, run the plain copy kernel over MIB MiB "
                          "(HBM bandwidth check and traffic-counter calibration)")
+    ap.add_argument("--batch-only", action="store_true",
+                    help="instead of the benchmark, run only the batch of uneven subbands")
     ap.add_argument("--front-end-only", action="store_true",
                     help="instead of the benchmark, run only the front-end, task-chain and spectrometer-application sections")
     args = ap.parse_args()
@@ -801,6 +817,11 @@ def main():
 
     if args.copy_probe:
         copy_probe(fxlib, args)
+        return
+    if args.batch_only:
+        ctx = fxlib.Context(0)
+        print(json.dumps({"batched_subbands": batched(fxlib, ctx, args)}), flush=True)
+        ctx.close()
         return
     if args.front_end_only:
         ctx = fxlib.Context(0)

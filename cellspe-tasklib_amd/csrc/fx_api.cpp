@@ -725,6 +725,23 @@ extern "C" int fx_pcal_final_async(fx_pcal* p)
     return FX_OK;
 }
 
+int fx_pcal_device_state(fx_pcal* p, fx_psub* S, size_t* position)
+{
+    FX_REQUIRE(p && S, FX_ERR_ARG, "fx_pcal_device_state: null argument");
+    S->pacc = p->acc;
+    S->tones = p->tones;
+    S->nfold = p->nfold;
+    S->nbins = p->nbins;
+    S->ntones = p->ntones;
+    S->offset_bin = p->offset_bin;
+    S->step = p->step;
+    S->rotatorlen = p->rotatorlen;
+    S->dphi = p->dphi;
+    if (position) *position = p->position;
+    p->finalized = true;   // the caller's kernels fold and extract into acc and tones
+    return FX_OK;
+}
+
 extern "C" int fx_pcal_tones_input(fx_pcal* p, int input, fx_cf64* tones, uint64_t* nsamples)
 {
     FX_REQUIRE(p && tones && input >= 0 && input < p->ninp, FX_ERR_ARG, "fx_pcal_tones: bad argument");

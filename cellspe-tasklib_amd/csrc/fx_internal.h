@@ -465,6 +465,59 @@ int fxk_pcal_delay(hipStream_t s, const double* tones, int ninp, int ntones, dou
 int fxk_difx_post(hipStream_t s, const float2* fft, int nfft, int band0, int n, const float2* mult,
                   float2* out, float2* conj_out, float2* acc);
 
+// ---------------------------------------------------------------------------
+// Persistent batch kernel (fx_persist.hip): one launch walks a table of work
+// items over many subbands -- phase-cal folds, F groups, X tiles, phase-cal
+// tones -- taking them in table order from a queue counter; an item may wait
+// for a counter of the items it depends on, all of which come earlier in the
+// table.
+// ---------------------------------------------------------------------------
+struct fx_psub {
+    const uint32_t* words;      // input 0; input i at words + i*stride
+    size_t stride;
+    int nfft, ninp;
+    long nt;                    // spectra integrated
+    const fx_phase_model* pm;   // device array of ninp models, or null
+    float sc;                   // uniform amplitude scale before the FP16 cast
+    uint32_t* chan;             // [c][t][i] half2 samples, nt deep
+    float2* vis;                // [p][c] visibilities (scale removed)
+    double* pacc;               // phase-cal fold, ninp x nfold (null: no phase-cal)
+    double* tones;              // ninp x ntones complex
+    int nfold, nbins, ntones, offset_bin, step, rotatorlen;
+    double dphi;
+};
+enum { FX_PI_PCAL = 0, FX_PI_F = 1, FX_PI_X = 2, FX_PI_TONE = 3 };
+struct fx_pitem {
+    int type, sb;
+    long a, b;      // F: first group, groups; X: first channel, channels; PCAL: input, first word; TONE: input
+    int c;          // X: tile index; PCAL: words per item
+    int xtile;      // X: 64 or 128
+    int wait_ctr;   // counter this item waits for (-1: none)
+    int wait_val;   // value that counter must reach
+    int done_ctr;   // counter incremented when the item is done (-1: none)
+    int pad;
+};
+// counters: [0..3] queue heads (one per launch of a phased run), [4] error
+// flag, then the items' wait/done counters
+#define FX_PCTR_ERR 4
+#define FX_PCTR_FIRST 8
+// F groups of an nfft-point subband: 4096/(nfft/2) items per group, 256 threads
+int fx_persist_group_items(int nfft);
+// LDS bytes an item of this kind needs (F: nfft; X: tile; PCAL: nfold; TONE)
+size_t fx_persist_lds(int type, int arg);
+// items [i0, i1) of the table on nblocks workgroups of 256 threads; wait: honour
+// the items' waits (one fused launch), else the caller orders the launches
+int fxk_batch_persistent(hipStream_t s, const float2* tw, const fx_psub* subs, const fx_pitem* items, int i0, int i1,
+                         int* ctrs, int head, int nblocks, int wait, size_t lds_bytes);
+// workgroups that fill the device with this much dynamic LDS (0 on error)
+int fxk_batch_persistent_blocks(size_t lds_bytes);
+
+// the device fold and tone buffers and the extraction parameters of a
+// phase-cal object, for kernels that fold and extract themselves (marks it
+// finalized so fx_pcal_tones* read what they wrote); *position: its sample
+// position modulo the fold length
+int fx_pcal_device_state(fx_pcal* p, fx_psub* S, size_t* position);
+
 // context internals shared with the task runtime and the application shims
 const float2* fx_ctx_twiddles(fx_ctx* ctx);
 hipStream_t fx_ctx_hip_stream(fx_ctx* ctx);

@@ -20,55 +20,13 @@
 // acc[c][tile][wave][block][reg][lane] (coalesced), converted to the
 // canonical [p][c] complex64 visibilities by k_xengine_dump.
 #include "fx_internal.h"
+#include "fx_xtile.h"
 
 #include <hip/hip_fp8.h>
 
 #include <cstdlib>
 
-typedef _Float16 fx_half8 __attribute__((ext_vector_type(8)));
-typedef float fx_float16 __attribute__((ext_vector_type(16)));
-typedef int fx_int4v __attribute__((ext_vector_type(4)));
-typedef int fx_int8 __attribute__((ext_vector_type(8)));
-
 namespace {
-
-constexpr int TS = 128;     // tile rows of the 8-bit engine (and default of the others)
-constexpr int XNT = 256;    // threads per workgroup
-
-__host__ __device__ inline int ntiles_ts(int ninp, int ts)
-{
-    int nb = (ninp + ts - 1) / ts;
-    return nb * (nb + 1) / 2;
-}
-__host__ __device__ inline int ntiles128(int ninp) { return ntiles_ts(ninp, TS); }
-
-// tile index -> (ti, tj), row-major over the upper triangle
-__device__ inline void tile_coords(int tile, int nb, int& bi, int& bj)
-{
-    int r = 0, rem = tile;
-    while (rem >= nb - r) {
-        rem -= nb - r;
-        r++;
-    }
-    bi = r;
-    bj = r + rem;
-}
-
-__device__ __forceinline__ uint32_t rot_h2(uint32_t v)
-{
-    // (re, im) -> (-im, re) on a packed half2: swap halves, flip the new low sign
-    return ((v >> 16) | (v << 16)) ^ 0x00008000u;
-}
-
-template <int PREC> struct XTraits;
-template <> struct XTraits<FX_PREC_F32> {
-    typedef float2 E;                // one complex sample
-    static constexpr int KT = 16;    // samples per LDS stage (16 KiB per slab at TSZ = 128)
-};
-template <> struct XTraits<FX_PREC_F16> {
-    typedef uint32_t E;
-    static constexpr int KT = 32;
-};
 
 // TSZ x TSZ input tiles (TSZ = 128 or 64); wave w = (qi, qj) owns the
 // (TSZ/2) x (TSZ/2) quadrant as NBW x NBW blocks of 32 x 32, NBW = TSZ/64.
@@ -82,10 +40,6 @@ __global__ __launch_bounds__(XNT, 2) void k_xengine(const typename XTraits<PREC>
     constexpr int KT = XTraits<PREC>::KT;
     constexpr int NBW = TSZ / 64;                         // blocks per wave per dimension
     constexpr int NBT = (TSZ / 32) * (TSZ / 32);          // blocks per tile
-    constexpr int CH = 16 / sizeof(E);                    // samples per 16-byte chunk
-    constexpr int SLAB_CHUNKS = KT * TSZ / CH;            // chunks per slab
-    constexpr int LPT = SLAB_CHUNKS / XNT;                // chunks per thread per slab
-    static_assert(LPT >= 1, "slab smaller than the workgroup");
     __shared__ E slab[2][KT][TSZ];
 
     const int nb = (ninp + TSZ - 1) / TSZ;
@@ -96,156 +50,22 @@ __global__ __launch_bounds__(XNT, 2) void k_xengine(const typename XTraits<PREC>
     const int tile = (int)(lb - (long)c * ntl);
     int ti, tj;
     tile_coords(tile, nb, ti, tj);
-    const bool diag = ti == tj;
-    const int nslab = diag ? 1 : 2;
-    const int tid = threadIdx.x;
-    const int w = tid >> 6, lane = tid & 63;
-    const int qi = w >> 1, qj = w & 1;
-    const bool active = !(diag && qi > qj);
-    const E* base = X + (size_t)c * (size_t)nt * (size_t)ninp;
-    const int row0[2] = {ti * TSZ, tj * TSZ};
-
-    fx_float16 dre[NBW][NBW], dim[NBW][NBW];
+    xengine_tile<PREC, TSZ>(X + (size_t)c * (size_t)nt * (size_t)ninp, nt, ninp, ti, tj, slab,
+                            [&](const auto& dre, const auto& dim, int w, int qi, int qj, bool diag, int lane) {
+        float* dst = acc + (((size_t)c * ntl + tile) * NBT + w * NBW * NBW) * (32 * 64);
 #pragma unroll
-    for (int a = 0; a < NBW; a++)
+        for (int a = 0; a < NBW; a++)
 #pragma unroll
-        for (int b = 0; b < NBW; b++) {
-            dre[a][b] = fx_float16{};
-            dim[a][b] = fx_float16{};
-        }
-
-    // global -> register prefetch of one stage: chunk q of slab s covers
-    // sample t0 + q / (TSZ/CH), rows (q % (TSZ/CH)) * CH .. + CH
-    uint4 pre[2][LPT];
-    auto gload = [&](long t0) {
+            for (int b = 0; b < NBW; b++) {
+                if (diag && qi == qj && a > b) continue;
+                float* d = dst + (a * NBW + b) * (32 * 64);
 #pragma unroll
-        for (int s = 0; s < 2; s++) {
-            if (s >= nslab) break;
-#pragma unroll
-            for (int k = 0; k < LPT; k++) {
-                const int q = tid + k * XNT;
-                const int tt = q / (TSZ / CH);
-                const int rr = (q - tt * (TSZ / CH)) * CH;
-                const long t = t0 + tt;
-                const int row = row0[s] + rr;
-                uint4 v = make_uint4(0u, 0u, 0u, 0u);
-                if (t < nt) {
-                    if (row + CH <= ninp && (ninp % CH) == 0) {
-                        v = *reinterpret_cast<const uint4*>(base + (size_t)t * ninp + row);
-                    } else {
-                        E tmp[CH];
-#pragma unroll
-                        for (int e = 0; e < CH; e++) {
-                            if (row + e < ninp) tmp[e] = base[(size_t)t * ninp + row + e];
-                            else __builtin_memset(&tmp[e], 0, sizeof(E));
-                        }
-                        v = *reinterpret_cast<uint4*>(tmp);
-                    }
+                for (int r = 0; r < 16; r++) {
+                    d[r * 64 + lane] += dre[a][b][r];
+                    d[(16 + r) * 64 + lane] += dim[a][b][r];
                 }
-                pre[s][k] = v;
             }
-        }
-    };
-    auto lstore = [&]() {
-#pragma unroll
-        for (int s = 0; s < 2; s++) {
-            if (s >= nslab) break;
-#pragma unroll
-            for (int k = 0; k < LPT; k++) {
-                const int q = tid + k * XNT;
-                const int tt = q / (TSZ / CH);
-                const int rr = (q - tt * (TSZ / CH)) * CH;
-                *reinterpret_cast<uint4*>(&slab[s][tt][rr]) = pre[s][k];
-            }
-        }
-    };
-
-    const int sb = diag ? 0 : 1;          // LDS slab holding the j rows
-    const int r32 = lane & 31, h = lane >> 5;
-    const long nstage = (nt + KT - 1) / KT;
-    gload(0);
-    for (long st = 0; st < nstage; st++) {
-        __syncthreads();
-        lstore();
-        __syncthreads();
-        if (st + 1 < nstage) gload((st + 1) * KT);
-        if (!active) continue;
-        if constexpr (PREC == FX_PREC_F16) {
-#pragma unroll
-            for (int ks = 0; ks < KT / 8; ks++) {
-                fx_half8 fa[NBW], fb[NBW], fr[NBW];
-#pragma unroll
-                for (int a = 0; a < NBW; a++) {
-                    fx_int4v ia;
-                    const int row = (NBW * qi + a) * 32 + r32;
-#pragma unroll
-                    for (int q = 0; q < 4; q++) ia[q] = (int)slab[0][ks * 8 + 4 * h + q][row];
-                    fa[a] = __builtin_bit_cast(fx_half8, ia);
-                }
-#pragma unroll
-                for (int b = 0; b < NBW; b++) {
-                    fx_int4v ib, ir;
-                    const int row = (NBW * qj + b) * 32 + r32;
-#pragma unroll
-                    for (int q = 0; q < 4; q++) {
-                        uint32_t v = slab[sb][ks * 8 + 4 * h + q][row];
-                        ib[q] = (int)v;
-                        ir[q] = (int)rot_h2(v);
-                    }
-                    fb[b] = __builtin_bit_cast(fx_half8, ib);
-                    fr[b] = __builtin_bit_cast(fx_half8, ir);
-                }
-#pragma unroll
-                for (int a = 0; a < NBW; a++)
-#pragma unroll
-                    for (int b = 0; b < NBW; b++) {
-                        if (diag && qi == qj && a > b) continue;
-                        dre[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_f16(fa[a], fb[b], dre[a][b], 0, 0, 0);
-                        dim[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_f16(fa[a], fr[b], dim[a][b], 0, 0, 0);
-                    }
-            }
-        } else {
-            // FP32: K = 2 reals = one complex sample per MFMA; lanes 0-31
-            // carry the real parts, lanes 32-63 the imaginary parts
-#pragma unroll
-            for (int ks = 0; ks < KT; ks++) {
-                float fa[NBW], fb[NBW], fr[NBW];
-#pragma unroll
-                for (int a = 0; a < NBW; a++) {
-                    const float2 v = slab[0][ks][(NBW * qi + a) * 32 + r32];
-                    fa[a] = h ? v.y : v.x;
-                }
-#pragma unroll
-                for (int b = 0; b < NBW; b++) {
-                    const float2 v = slab[sb][ks][(NBW * qj + b) * 32 + r32];
-                    fb[b] = h ? v.y : v.x;
-                    fr[b] = h ? v.x : -v.y;
-                }
-#pragma unroll
-                for (int a = 0; a < NBW; a++)
-#pragma unroll
-                    for (int b = 0; b < NBW; b++) {
-                        if (diag && qi == qj && a > b) continue;
-                        dre[a][b] = __builtin_amdgcn_mfma_f32_32x32x2f32(fa[a], fb[b], dre[a][b], 0, 0, 0);
-                        dim[a][b] = __builtin_amdgcn_mfma_f32_32x32x2f32(fa[a], fr[b], dim[a][b], 0, 0, 0);
-                    }
-            }
-        }
-    }
-    if (!active) return;
-    float* dst = acc + (((size_t)c * ntl + tile) * NBT + w * NBW * NBW) * (32 * 64);
-#pragma unroll
-    for (int a = 0; a < NBW; a++)
-#pragma unroll
-        for (int b = 0; b < NBW; b++) {
-            if (diag && qi == qj && a > b) continue;
-            float* d = dst + (a * NBW + b) * (32 * 64);
-#pragma unroll
-            for (int r = 0; r < 16; r++) {
-                d[r * 64 + lane] += dre[a][b][r];
-                d[(16 + r) * 64 + lane] += dim[a][b][r];
-            }
-        }
+    });
 }
 
 // ---------------------------------------------------------------------------

@@ -19,6 +19,7 @@ LIB_PATH = os.path.join(HERE, "libcellspe_fx.so")
 FX_OK = 0
 FX_PREC_F32, FX_PREC_F16, FX_PREC_FP8, FX_PREC_FP8_2BIT = 0, 1, 2, 3
 FX_XENGINE_AUTO, FX_XENGINE_PREFETCH = 0, 1
+FX_BATCH_EAGER, FX_BATCH_GRAPH, FX_BATCH_PERSISTENT, FX_BATCH_PHASED = 0, 1, 2, 3
 (FX_WIN_NONE, FX_WIN_SIN2, FX_WIN_COSINE, FX_WIN_COSINE2, FX_WIN_HAMMING, FX_WIN_BLACKMAN,
  FX_WIN_HANN) = range(7)
 
@@ -152,6 +153,8 @@ def load():
         "fx_batch_pcal": (i, [vp, i, i, vp, C.POINTER(i)]),
         "fx_batch_tuning": (i, [vp, i, C.POINTER(l), C.POINTER(i), C.POINTER(l)]),
         "fx_batch_destroy": (i, [vp]),
+        "fx_batch_persist_tuning": (i, [vp, i, C.POINTER(i), C.POINTER(i), C.POINTER(i)]),
+        "fx_batch_persist_info": (i, [vp, C.POINTER(i), C.POINTER(i), C.POINTER(sz)]),
         "fx_autotune": (i, [vp, i, i, i, l, C.c_char_p, C.POINTER(l), C.POINTER(i)]),
         "fx_corr_info": (i, [vp, C.POINTER(l), C.POINTER(sz), C.POINTER(sz)]),
         "fx_mfma_probe": (i, [vp, i, vp]),
@@ -602,6 +605,19 @@ class Batch:
         sub, x, ch = C.c_long(), C.c_int(), C.c_long()
         check(load().fx_batch_tuning(self.handle, s, C.byref(sub), C.byref(x), C.byref(ch)))
         return sub.value, x.value, ch.value
+
+    def persist_tuning(self, s):
+        """(X tile, channels per X item, F groups per item) of subband s in
+        the persistent modes."""
+        x, cb, g = C.c_int(), C.c_int(), C.c_int()
+        check(load().fx_batch_persist_tuning(self.handle, s, C.byref(x), C.byref(cb), C.byref(g)))
+        return x.value, cb.value, g.value
+
+    def persist_info(self):
+        """(work items, workgroups, dynamic LDS bytes) of the persistent launch."""
+        n, b, l = C.c_int(), C.c_int(), C.c_size_t()
+        check(load().fx_batch_persist_info(self.handle, C.byref(n), C.byref(b), C.byref(l)))
+        return n.value, b.value, l.value
 
     def close(self):
         if self.handle is not None:

@@ -164,3 +164,51 @@ def test_batch_seeded_draw_all_subbands(ctx, tmp_path):
     b32.close()
     for s in subs:
         s["words"].free()
+
+
+@pytest.mark.parametrize("mode", [fxlib.FX_BATCH_PERSISTENT, fxlib.FX_BATCH_PHASED])
+def test_batch_persistent_vs_oracle(ctx, tmp_path, mode):
+    """The persistent batch kernel (one launch walking the work table of every
+    subband, or the same table as three phased launches): every subband's
+    FP16 visibilities within 1e-3 of the FP64 oracle, its FP64 phase-cal
+    tones equal to the oracle's, replays bit-identical (with eager work in
+    between), and the fused and phased runs bit-identical to each other."""
+    subs, streams = _make_batch_input(ctx)
+    tune = str(tmp_path / "tune.txt")
+    b = fxlib.Batch(ctx, subs, fxlib.FX_PREC_F16, use_graph=mode, tune_file=tune)
+    assert any(ln.startswith("P ") for ln in open(tune))
+    nitems, nblocks, lds = b.persist_info()
+    assert nitems > 0 and nblocks >= 256 and lds <= 160 * 1024
+    b.run()
+    b.sync()
+    first = [b.visibilities(k) for k in range(len(subs))]
+    e32 = fxlib.Batch(ctx, subs, fxlib.FX_PREC_F32, use_graph=False)
+    for rep in range(2):
+        e32.run()
+        e32.sync()
+        b.run()
+        b.sync()
+        for k in range(len(subs)):
+            assert np.array_equal(b.visibilities(k).view(np.uint32), first[k].view(np.uint32)), (rep, k)
+    for k, s in enumerate(subs):
+        n, I, T = s["nfft"], s["ninp"], s["nspectra"]
+        r64 = O.corr_f64(streams[k], n, T, s["models"], nthreads=8)
+        e = nrel(first[k], r64)
+        assert e < 1e-3, (k, e)
+        xt, cb, g = b.persist_tuning(k)
+        assert xt == 64 and cb >= 1 and g >= 1
+        if s["pcal"] is not None:
+            cfg = O.pcal_config(*s["pcal"])
+            for i in (0, I - 1):
+                x = O.unpack_1ch(streams[k][i], n * T)
+                ref = O.pcal_final(cfg, O.pcal_extract(cfg, x))
+                assert nrel(b.pcal(k, i), ref) < 1e-12
+    other = fxlib.Batch(ctx, subs, fxlib.FX_PREC_F16,
+                        use_graph=fxlib.FX_BATCH_PHASED if mode == fxlib.FX_BATCH_PERSISTENT
+                        else fxlib.FX_BATCH_PERSISTENT, tune_file=tune)
+    other.run()
+    other.sync()
+    for k in range(len(subs)):
+        assert np.array_equal(other.visibilities(k).view(np.uint32), first[k].view(np.uint32)), k
+    for bb in (b, e32, other):
+        bb.close()
