@@ -31,11 +31,18 @@ struct Subband {
     fx_pcal* pcal = nullptr;   // all inputs of the subband, one fold + one final launch
     long chunk = 0, sub = 0;
     int xtile = 128;
-    // persistent modes: the whole integration's channelized samples, the
-    // device phase models, channels per X item and F groups per item
+    // persistent modes: the whole integration's channelized samples (and
+    // natural rows, slice sums), the device phase models, and the tuning:
+    // natural rows + corner turns, channels per X item, F groups per item,
+    // time slices of the X items
     uint32_t* chan = nullptr;
+    uint32_t* nat = nullptr;
+    float2* part = nullptr;
+    size_t part_bytes = 0;
     fx_phase_model* pm_dev = nullptr;
-    int cb = 1, gpi = 4;
+    int ipad = 0;
+    long ntp = 0;
+    int pnat = 0, cb = 2, gpi = 4, nts = 1;
 };
 
 long round8(long n) { return (n + 7) / 8 * 8; }
@@ -98,33 +105,82 @@ struct fx_batch {
     fx_pitem* d_items = nullptr;
     int* d_ctrs = nullptr;
     int nctrs = 0, nitems = 0;
-    int phase_end[3] = {0, 0, 0};   // phased: launches over items [0,e0), [e0,e1), [e1,e2)
+    int phase_end[4] = {0, 0, 0, 0};   // phased: launches over items [0,e0), [e0,e1), [e1,e2), [e2,e3)
     int nblocks = 0;
     size_t lds = 0;
 };
 
 namespace {
 
-// Work table of the persistent modes.  Counters: FX_PCTR_FIRST + 2 s counts
-// the finished F items of subband s, + 1 its finished phase-cal folds.
-// fused (FX_BATCH_PERSISTENT): the folds, then the F and X items of the
-// subbands interleaved so that a subband's X items follow the next subband's
-// F items (its own F items are then mostly done when they are taken), then
-// the tone items; phased: folds and F items, X items, tones as three ranges.
+// ---------------------------------------------------------------------------
+// Persistent modes
+// ---------------------------------------------------------------------------
+constexpr long kPartMax = 2L << 30;   // bytes of slice sums a subband may hold
+
+long round_up(long n, long m) { return (n + m - 1) / m * m; }
+
+// spectra per time slice (a multiple of the X stage) and the slices that gives
+long slice_len(long ntp, int nts) { return round_up((ntp + nts - 1) / nts, FX_PXKT); }
+int slices_of(long ntp, int nts) { return (int)((ntp + slice_len(ntp, nts) - 1) / slice_len(ntp, nts)); }
+
+size_t part_bytes_of(const fx_subband_desc& d, int nts)
+{
+    const size_t P = (size_t)d.ninp * (d.ninp + 1) / 2, C = (size_t)d.nfft / 2;
+    return nts > 1 ? (size_t)nts * P * C * sizeof(float2) : 0;
+}
+
+// the device view of subband r with its current tuning
+void fill_psub(const Subband& r, fx_psub& S)
+{
+    const fx_subband_desc& d = r.d;
+    S = fx_psub{};
+    S.words = d.d_words;
+    S.stride = d.stride_words;
+    S.nfft = d.nfft;
+    S.ninp = d.ninp;
+    S.nt = d.nspectra;
+    S.pm = r.pm_dev;
+    // a 2-bit stream through the level table has E[x^2] ~ 4.316: the FFT
+    // output's sigma sqrt(N * 4.316) maps to 1 before the FP16 cast
+    S.sc = (float)(1.0 / sqrt((double)d.nfft * 4.316));
+    S.ipad = r.ipad;
+    S.chan = r.chan;
+    S.nat = r.pnat ? r.nat : nullptr;
+    S.vis = (float2*)r.vis;
+    S.nts = r.nts;
+    S.part = r.nts > 1 ? r.part : nullptr;
+    S.ntp = r.ntp;
+    S.tslice = slice_len(r.ntp, r.nts);
+    if (r.pcal) fx_pcal_device_state(r.pcal, &S, nullptr);
+}
+
+int upload_psub(fx_batch* b, int s)
+{
+    fx_psub S;
+    fill_psub(b->sb[(size_t)s], S);
+    return fx_memcpy_h2d(b->ctx, b->d_subs + s, &S, sizeof(S));
+}
+
+// Work table of the persistent modes.  Counters FX_PCTR_FIRST +
+// FX_PCTR_PER_SUB s + {0, 1, 2, 3} count the finished F, CT, X and PCAL items
+// of subband s.  fused (FX_BATCH_PERSISTENT): the folds, then the subbands'
+// F, CT and X items staggered (subband k's F items, k-1's corner turns,
+// k-2's X items, ...) so that an item's inputs are mostly done when it is
+// taken, then the slice reductions and the tones; phased: PCAL + F, CT, X,
+// RED + TONE as four ranges.
 int build_table(fx_batch* b, bool fused, std::vector<fx_pitem>& items, int* phase_end)
 {
     const int ns = (int)b->sb.size();
-    std::vector<int> nF((size_t)ns, 0), nP((size_t)ns, 0);
-    auto fctr = [](int s) { return FX_PCTR_FIRST + 2 * s; };
-    auto pctr = [](int s) { return FX_PCTR_FIRST + 2 * s + 1; };
-    auto item = [](int type, int sb, long a, long bb, int c, int xt, int wc, int wv, int dc) {
+    std::vector<int> nF((size_t)ns, 0), nC((size_t)ns, 0), nX((size_t)ns, 0), nP((size_t)ns, 0);
+    auto ctr = [](int s, int k) { return FX_PCTR_FIRST + FX_PCTR_PER_SUB * s + k; };
+    auto item = [](int type, int sb, long a, long bb, int c, int d, int wc, int wv, int dc) {
         fx_pitem it;
         it.type = type;
         it.sb = sb;
         it.a = a;
         it.b = bb;
         it.c = c;
-        it.xtile = xt;
+        it.d = d;
         it.wait_ctr = wc;
         it.wait_val = wv;
         it.done_ctr = dc;
@@ -141,7 +197,7 @@ int build_table(fx_batch* b, bool fused, std::vector<fx_pitem>& items, int* phas
         const long nw = r.d.nspectra * r.d.nfft / 16;
         for (int y = 0; y < r.d.ninp; y++)
             for (long w0 = 0; w0 < nw; w0 += wpi) {
-                items.push_back(item(FX_PI_PCAL, s, y, w0, (int)wpi, 0, -1, 0, pctr(s)));
+                items.push_back(item(FX_PI_PCAL, s, y, w0, (int)wpi, 0, -1, 0, ctr(s, 3)));
                 nP[(size_t)s]++;
             }
     };
@@ -150,23 +206,46 @@ int build_table(fx_batch* b, bool fused, std::vector<fx_pitem>& items, int* phas
         const long F = fx_persist_group_items(r.d.nfft);
         const long G = (r.d.nspectra * r.d.ninp + F - 1) / F;
         for (long g0 = 0; g0 < G; g0 += r.gpi) {
-            items.push_back(item(FX_PI_F, s, g0, std::min<long>(r.gpi, G - g0), 0, 0, -1, 0, fctr(s)));
+            items.push_back(item(FX_PI_F, s, g0, std::min<long>(r.gpi, G - g0), 0, 0, -1, 0, ctr(s, 0)));
             nF[(size_t)s]++;
         }
     };
+    auto ct_items = [&](int s) {
+        const Subband& r = b->sb[(size_t)s];
+        if (!r.pnat) return;
+        const int nblk = (r.d.nfft / 2 / 64) * ((r.d.ninp + 63) / 64);
+        for (long t0 = 0; t0 < r.d.nspectra; t0 += FX_PCT_T)
+            for (int k = 0; k < nblk; k++) {
+                items.push_back(item(FX_PI_CT, s, t0, FX_PCT_T, k, 0, ctr(s, 0), nF[(size_t)s], ctr(s, 1)));
+                nC[(size_t)s]++;
+            }
+    };
     auto x_items = [&](int s) {
         const Subband& r = b->sb[(size_t)s];
-        const int nb = (r.d.ninp + r.xtile - 1) / r.xtile;
+        const int nb = (r.d.ninp + 63) / 64;
         const long C = r.d.nfft / 2;
-        for (int t = 0; t < nb * (nb + 1) / 2; t++)
-            for (long c0 = 0; c0 < C; c0 += r.cb)
-                items.push_back(
-                    item(FX_PI_X, s, c0, std::min<long>(r.cb, C - c0), t, r.xtile, fctr(s), nF[(size_t)s], -1));
+        const int wc = r.pnat ? ctr(s, 1) : ctr(s, 0);
+        const int wv = r.pnat ? nC[(size_t)s] : nF[(size_t)s];
+        for (int sl = 0; sl < r.nts; sl++)
+            for (int t = 0; t < nb * (nb + 1) / 2; t++)
+                for (long c0 = 0; c0 < C; c0 += r.cb) {
+                    items.push_back(item(FX_PI_X, s, c0, std::min<long>(r.cb, C - c0), t, sl, wc, wv, ctr(s, 2)));
+                    nX[(size_t)s]++;
+                }
+    };
+    auto red_items = [&](int s) {
+        const Subband& r = b->sb[(size_t)s];
+        if (r.nts < 2) return;
+        const long P = (long)r.d.ninp * (r.d.ninp + 1) / 2, C = r.d.nfft / 2;
+        const long pch = std::max<long>(1, 262144 / C);   // about 256K visibilities per item
+        for (long p0 = 0; p0 < P; p0 += pch)
+            items.push_back(item(FX_PI_RED, s, p0, std::min<long>(pch, P - p0), 0, 0, ctr(s, 2), nX[(size_t)s], -1));
     };
     auto t_items = [&](int s) {
         const Subband& r = b->sb[(size_t)s];
         if (!r.pcal) return;
-        for (int y = 0; y < r.d.ninp; y++) items.push_back(item(FX_PI_TONE, s, y, 0, 0, 0, pctr(s), nP[(size_t)s], -1));
+        for (int y = 0; y < r.d.ninp; y++)
+            items.push_back(item(FX_PI_TONE, s, y, 0, 0, 0, ctr(s, 3), nP[(size_t)s], -1));
     };
     // the largest subbands first (their X items take longest)
     std::vector<int> order((size_t)ns);
@@ -179,23 +258,26 @@ int build_table(fx_batch* b, bool fused, std::vector<fx_pitem>& items, int* phas
     items.clear();
     for (int s : order) pcal_items(s);
     if (fused) {
-        for (int k = 0; k < ns; k++) {
-            f_items(order[(size_t)k]);
-            if (k) x_items(order[(size_t)k - 1]);
+        for (int k = 0; k < ns + 2; k++) {
+            if (k < ns) f_items(order[(size_t)k]);
+            if (k >= 1 && k <= ns) ct_items(order[(size_t)k - 1]);
+            if (k >= 2) x_items(order[(size_t)k - 2]);
         }
-        x_items(order[(size_t)ns - 1]);
-        phase_end[0] = phase_end[1] = (int)items.size();
+        phase_end[0] = phase_end[1] = phase_end[2] = (int)items.size();
     } else {
         for (int s : order) f_items(s);
         phase_end[0] = (int)items.size();
-        for (int s : order) x_items(s);
+        for (int s : order) ct_items(s);
         phase_end[1] = (int)items.size();
+        for (int s : order) x_items(s);
+        phase_end[2] = (int)items.size();
     }
+    for (int s : order) red_items(s);
     for (int s : order) t_items(s);
-    phase_end[2] = (int)items.size();
+    phase_end[3] = (int)items.size();
     // every wait names items that come earlier in the table: the workgroups
     // that took them are running when the wait starts, so it ends
-    std::vector<int> seen((size_t)(FX_PCTR_FIRST + 2 * ns), 0);
+    std::vector<int> seen((size_t)(FX_PCTR_FIRST + FX_PCTR_PER_SUB * ns), 0);
     for (const fx_pitem& it : items) {
         if (it.wait_ctr >= 0 && seen[(size_t)it.wait_ctr] != it.wait_val) {
             fx_set_error("fx_batch_create: work table waits for items that do not precede it");
@@ -207,8 +289,9 @@ int build_table(fx_batch* b, bool fused, std::vector<fx_pitem>& items, int* phas
 }
 
 // the persistent run's launches on stream s: clear the counters and folds, then
-// one fused launch or three phased ones
-int enqueue_persist(fx_batch* b, hipStream_t s)
+// one fused launch or four phased ones (blocks: workgroups, 0 for the batch's)
+int enqueue_persist(fx_batch* b, hipStream_t s, const fx_pitem* d_items, const int* phase_end, bool fused,
+                    int blocks)
 {
     if (fxk_fill32(s, b->d_ctrs, 0u, (size_t)b->nctrs) != FX_OK) return FX_ERR_HIP;
     for (const Subband& r : b->sb) {
@@ -218,19 +301,19 @@ int enqueue_persist(fx_batch* b, hipStream_t s)
         if (fxk_fill32(s, S.pacc, 0u, (size_t)S.nfold * r.d.ninp * 2) != FX_OK) return FX_ERR_HIP;
     }
     const float2* tw = fx_ctx_twiddles(b->ctx);
-    if (b->mode == FX_BATCH_PERSISTENT)
-        return fxk_batch_persistent(s, tw, b->d_subs, b->d_items, 0, b->nitems, b->d_ctrs, 0, b->nblocks, 1, b->lds);
+    const int nb = blocks > 0 ? blocks : b->nblocks;
+    if (fused)
+        return fxk_batch_persistent(s, tw, b->d_subs, d_items, 0, phase_end[3], b->d_ctrs, 0, nb, 1, b->lds);
     int i0 = 0;
-    for (int ph = 0; ph < 3; ph++) {
-        int rc = fxk_batch_persistent(s, tw, b->d_subs, b->d_items, i0, b->phase_end[ph], b->d_ctrs, ph, b->nblocks, 0,
-                                      b->lds);
+    for (int ph = 0; ph < 4; ph++) {
+        int rc = fxk_batch_persistent(s, tw, b->d_subs, d_items, i0, phase_end[ph], b->d_ctrs, ph, nb, 0, b->lds);
         if (rc) return rc;
-        i0 = b->phase_end[ph];
+        i0 = phase_end[ph];
     }
     return FX_OK;
 }
 
-bool lookup_persist(const char* file, int nfft, int ninp, long nspectra, int* xtile, int* cb, int* gpi)
+bool lookup_persist(const char* file, const fx_subband_desc& d, int blocks, Subband& r)
 {
     if (!file) return false;
     FILE* f = fopen(file, "r");
@@ -238,14 +321,15 @@ bool lookup_persist(const char* file, int nfft, int ninp, long nspectra, int* xt
     char line[256];
     bool found = false;
     while (fgets(line, sizeof(line), f)) {
-        int n, i, x, c, g;
+        int n, i, bl, na, c, g, ts;
         long t;
         double ms;
-        if (sscanf(line, "P %d %d %ld %d %d %d %lf", &n, &i, &t, &x, &c, &g, &ms) == 7 && n == nfft && i == ninp &&
-            t == nspectra) {
-            *xtile = x;
-            *cb = c;
-            *gpi = g;
+        if (sscanf(line, "P %d %d %ld %d %d %d %d %d %lf", &n, &i, &t, &bl, &na, &c, &g, &ts, &ms) == 9 &&
+            n == d.nfft && i == d.ninp && t == d.nspectra && bl == blocks) {
+            r.pnat = na;
+            r.cb = c;
+            r.gpi = g;
+            r.nts = ts;
             found = true;
         }
     }
@@ -253,15 +337,18 @@ bool lookup_persist(const char* file, int nfft, int ninp, long nspectra, int* xt
     return found;
 }
 
-// defaults: 64-input tiles, X items of about 2048 spectra of channels, 4 F groups per item
-void persist_defaults(const fx_subband_desc& d, int* xtile, int* cb, int* gpi)
+// defaults: natural rows from 4096-point transforms, channel pairs of X
+// items of about 2048 spectra of channels, 4 F groups per item, one slice
+void persist_defaults(Subband& r)
 {
-    *xtile = 64;
-    *cb = (int)std::max<long>(1, std::min<long>(d.nfft / 2, 2048 / std::max<long>(1, d.nspectra)));
-    *gpi = 4;
+    const fx_subband_desc& d = r.d;
+    r.pnat = d.nfft >= 4096 ? 1 : 0;
+    r.cb = (int)std::max<long>(2, std::min<long>(d.nfft / 2, 2048 / std::max<long>(1, d.nspectra)) / 2 * 2);
+    r.gpi = 4;
+    r.nts = 1;
 }
 
-// table of one subband only (the autotuner's timing runs)
+// table of one subband only (the autotuner's timing runs), phased
 int build_table_one(fx_batch* b, int s, std::vector<fx_pitem>& items, int* phase_end)
 {
     fx_batch one;
@@ -269,20 +356,21 @@ int build_table_one(fx_batch* b, int s, std::vector<fx_pitem>& items, int* phase
     int rc = build_table(&one, false, items, phase_end);
     for (fx_pitem& it : items) {
         it.sb = s;
-        if (it.done_ctr >= 0) it.done_ctr += 2 * s;
-        if (it.wait_ctr >= 0) it.wait_ctr += 2 * s;
+        if (it.done_ctr >= 0) it.done_ctr += FX_PCTR_PER_SUB * s;
+        if (it.wait_ctr >= 0) it.wait_ctr += FX_PCTR_PER_SUB * s;
     }
     one.sb.clear();   // owned by b
     return rc;
 }
 
-// Time one subband's F items (ph = 0) or X items (ph = 1) with its current
-// tuning, phased, in ms (the second of two runs).
-int time_phase(fx_batch* b, int s, int ph, float* ms)
+// Time launches [ph0, ph1] of subband s's phased table with its current
+// tuning on `blocks` workgroups, in ms (the second of two runs).
+int time_phases(fx_batch* b, int s, int ph0, int ph1, int blocks, float* ms)
 {
     std::vector<fx_pitem> items;
-    int pe[3];
+    int pe[4];
     int rc = build_table_one(b, s, items, pe);
+    if (!rc) rc = upload_psub(b, s);
     if (rc) return rc;
     fx_pitem* d = nullptr;
     rc = fx_malloc(b->ctx, items.size() * sizeof(fx_pitem), (void**)&d);
@@ -293,11 +381,19 @@ int time_phase(fx_batch* b, int s, int ph, float* ms)
     if (!rc) rc = fx_event_create(b->ctx, &e1);
     hipStream_t st = fx_ctx_hip_stream(b->ctx);
     const float2* tw = fx_ctx_twiddles(b->ctx);
-    const int i0 = ph == 0 ? 0 : pe[0], i1 = ph == 0 ? pe[0] : pe[1];
     for (int rep = 0; rep < 2 && !rc; rep++) {
         rc = fxk_fill32(st, b->d_ctrs, 0u, (size_t)b->nctrs);
+        // the earlier launches once untimed (their outputs feed the timed ones)
+        int i0 = 0;
+        for (int ph = 0; ph < ph0 && !rc; ph++) {
+            rc = fxk_batch_persistent(st, tw, b->d_subs, d, i0, pe[ph], b->d_ctrs, ph, blocks, 0, b->lds);
+            i0 = pe[ph];
+        }
         if (!rc) rc = fx_event_record(b->ctx, e0);
-        if (!rc) rc = fxk_batch_persistent(st, tw, b->d_subs, d, i0, i1, b->d_ctrs, ph, b->nblocks, 0, b->lds);
+        for (int ph = ph0; ph <= ph1 && !rc; ph++) {
+            rc = fxk_batch_persistent(st, tw, b->d_subs, d, i0, pe[ph], b->d_ctrs, ph, blocks, 0, b->lds);
+            i0 = pe[ph];
+        }
         if (!rc) rc = fx_event_record(b->ctx, e1);
         if (!rc) rc = fx_event_elapsed_ms(e0, e1, ms);
     }
@@ -308,50 +404,66 @@ int time_phase(fx_batch* b, int s, int ph, float* ms)
     return rc;
 }
 
-// Autotuner of the persistent modes for subband s (every later subband of the
-// same shape takes its result): F groups per item from {1, 2, 4, 8, 16}, then
-// channels per X item (64 x 64-input tiles) from {1, 2, ..., 64},
-// each by timing that subband's items alone on the device (its real input);
-// the choice is appended to tune_file as "P nfft ninp nspectra xtile cb gpi ms".
-int autotune_persist(fx_batch* b, int s, const char* tune_file)
+// Autotuner of the persistent modes for subband s, timing that subband's
+// items alone on `blocks` workgroups (its share of the device in the batch):
+// natural rows + corner turns or direct stores (transforms from 2048 points)
+// and F groups per item from {1, 2, 4, 8, 16} on the F (+ CT) launches, then
+// channels per X item from {2, 4, ..., 64} on the X launch, then time slices
+// from {2, 4, 8, 16} on the X + reduction launches.  The choice is appended to
+// tune_file as "P nfft ninp nspectra blocks nat cb gpi nts ms".
+int autotune_persist(fx_batch* b, int s, int blocks, const char* tune_file)
 {
     Subband& r = b->sb[(size_t)s];
+    const fx_subband_desc& d = r.d;
     float best = 1e30f, ms = 0.f, fms = 0.f;
-    int best_g = r.gpi;
-    for (int g : {1, 2, 4, 8, 16}) {
-        r.gpi = g;
-        int rc = time_phase(b, s, 0, &ms);
-        if (rc) return rc;
-        if (ms < best) {
-            best = ms;
-            best_g = g;
-        }
-    }
-    r.gpi = best_g;
-    fms = best;
-    best = 1e30f;
-    int best_x = r.xtile, best_c = r.cb;
-    const long C = r.d.nfft / 2;
-    for (int xt : {64}) {   // the persistent kernel's X items are 64 x 64 input tiles
-        for (int cb = 1; cb <= 64 && cb <= C; cb *= 2) {
-            r.xtile = xt;
-            r.cb = cb;
-            int rc = time_phase(b, s, 1, &ms);
+    int best_n = r.pnat, best_g = r.gpi;
+    for (int na = 0; na <= (r.nat ? 1 : 0); na++)
+        for (int g : {1, 2, 4, 8, 16}) {
+            r.pnat = na;
+            r.gpi = g;
+            int rc = time_phases(b, s, 0, 1, blocks, &ms);
             if (rc) return rc;
             if (ms < best) {
                 best = ms;
-                best_x = xt;
-                best_c = cb;
+                best_n = na;
+                best_g = g;
             }
         }
+    r.pnat = best_n;
+    r.gpi = best_g;
+    fms = best;
+    best = 1e30f;
+    r.nts = 1;
+    int best_c = r.cb;
+    const long C = d.nfft / 2;
+    for (int cb = 2; cb <= 64 && cb <= C; cb *= 2) {
+        r.cb = cb;
+        int rc = time_phases(b, s, 2, 3, blocks, &ms);
+        if (rc) return rc;
+        if (ms < best) {
+            best = ms;
+            best_c = cb;
+        }
     }
-    r.xtile = best_x;
     r.cb = best_c;
+    int best_t = 1;
+    for (int ts : {2, 4, 8, 16}) {
+        const int eff = slices_of(r.ntp, ts);
+        if (eff <= best_t || eff > ts || part_bytes_of(d, eff) > r.part_bytes) continue;
+        r.nts = eff;
+        int rc = time_phases(b, s, 2, 3, blocks, &ms);
+        if (rc) return rc;
+        if (ms < best) {
+            best = ms;
+            best_t = eff;
+        }
+    }
+    r.nts = best_t;
     if (tune_file) {
         FILE* f = fopen(tune_file, "a");
         if (f) {
-            fprintf(f, "P %d %d %ld %d %d %d %.4f\n", r.d.nfft, r.d.ninp, r.d.nspectra, r.xtile, r.cb, r.gpi,
-                    fms + best);
+            fprintf(f, "P %d %d %ld %d %d %d %d %d %.4f\n", d.nfft, d.ninp, d.nspectra, blocks, r.pnat, r.cb, r.gpi,
+                    r.nts, fms + best);
             fclose(f);
         }
     }
@@ -370,44 +482,40 @@ int batch_create_persist(fx_ctx* ctx, const fx_subband_desc* sb, int nsub, int p
     b->use_graph = true;
     b->sb.resize((size_t)nsub);
     int rc = FX_OK;
-    size_t lds = fx_persist_lds(FX_PI_TONE, 0);
-    std::vector<fx_psub> hs((size_t)nsub);
-    std::vector<bool> tuned((size_t)nsub, false);
+    size_t lds = std::max({fx_persist_lds(FX_PI_TONE, 0, false), fx_persist_lds(FX_PI_CT, 0, false),
+                           fx_persist_lds(FX_PI_X, 0, false), fx_persist_lds(FX_PI_X, 1, false)});
     for (int s = 0; s < nsub && !rc; s++) {
         Subband& r = b->sb[(size_t)s];
         r.d = sb[s];
         const fx_subband_desc& d = r.d;
         const bool pow2 = d.nfft >= 256 && d.nfft <= 16384 && (d.nfft & (d.nfft - 1)) == 0;
         if (!pow2 || d.ninp < 1 || d.nspectra < 1 || !d.d_words || d.stride_words % 2 ||
-            ((uintptr_t)d.d_words % 8) || d.stride_words < (size_t)d.nspectra * d.nfft / 16) {
+            ((uintptr_t)d.d_words % 8) || d.stride_words < (size_t)d.nspectra * d.nfft / 16 ||
+            (double)d.nspectra * d.ninp >= 2147483647.0 || d.nspectra >= (1L << 30)) {
             fx_set_error("fx_batch_create: subband %d: nfft %d, %d inputs, %ld spectra, word stride %zu", s, d.nfft,
                          d.ninp, d.nspectra, d.stride_words);
             rc = FX_ERR_ARG;
             break;
         }
         r.chunk = d.nspectra;
-        persist_defaults(d, &r.xtile, &r.cb, &r.gpi);
-        if (tune_file) tuned[(size_t)s] = lookup_persist(tune_file, d.nfft, d.ninp, d.nspectra, &r.xtile, &r.cb, &r.gpi);
+        persist_defaults(r);
+        r.ipad = (int)round_up(d.ninp, 4);
+        r.ntp = round_up(d.nspectra, FX_PXKT);
         const size_t C = (size_t)d.nfft / 2, P = (size_t)d.ninp * (d.ninp + 1) / 2;
-        rc = fx_malloc(ctx, C * (size_t)d.nspectra * (size_t)d.ninp * 4, (void**)&r.chan);
+        // chan: the pad inputs and spectra stay zero; 1 KiB of slack for the
+        // X items' 64-input slab rows past the last channel's last row
+        const size_t chan_bytes = C * (size_t)r.ntp * (size_t)r.ipad * 4 + 1024;
+        rc = fx_malloc(ctx, chan_bytes, (void**)&r.chan);
+        if (!rc) rc = fx_memset(ctx, r.chan, 0, chan_bytes);
         if (!rc) rc = fx_malloc(ctx, P * C * sizeof(fx_cf32), (void**)&r.vis);
+        if (!rc && d.nfft >= 2048)
+            rc = fx_malloc(ctx, C * (size_t)d.nspectra * (size_t)r.ipad * 4, (void**)&r.nat);
         if (!rc && d.pm) {
             rc = fx_malloc(ctx, sizeof(fx_phase_model) * (size_t)d.ninp, (void**)&r.pm_dev);
             if (!rc) rc = fx_memcpy_h2d(ctx, r.pm_dev, d.pm, sizeof(fx_phase_model) * (size_t)d.ninp);
         }
-        fx_psub& S = hs[(size_t)s];
-        S.words = d.d_words;
-        S.stride = d.stride_words;
-        S.nfft = d.nfft;
-        S.ninp = d.ninp;
-        S.nt = d.nspectra;
-        S.pm = r.pm_dev;
-        // a 2-bit stream through the level table has E[x^2] ~ 4.316: the FFT
-        // output's sigma sqrt(N * 4.316) maps to 1 before the FP16 cast
-        S.sc = (float)(1.0 / sqrt((double)d.nfft * 4.316));
-        S.chan = r.chan;
-        S.vis = (float2*)r.vis;
         if (!rc && d.pcal) {
+            fx_psub S{};
             rc = fx_pcal_create_inputs(ctx, d.bandwidth_hz, d.pcal_spacing_hz, d.pcal_offset_hz, 0, d.ninp, &r.pcal);
             if (!rc) rc = fx_pcal_device_state(r.pcal, &S, nullptr);
             if (!rc && (S.nfold % 16 || S.nfold > 4096)) {
@@ -415,33 +523,82 @@ int batch_create_persist(fx_ctx* ctx, const fx_subband_desc* sb, int nsub, int p
                              "multiples of 16 up to 4096)", s, S.nfold);
                 rc = FX_ERR_UNSUPPORTED;
             }
-            lds = std::max(lds, fx_persist_lds(FX_PI_PCAL, S.nfold));
+            lds = std::max(lds, fx_persist_lds(FX_PI_PCAL, S.nfold, false));
         }
-        lds = std::max(lds, std::max(fx_persist_lds(FX_PI_F, d.nfft), fx_persist_lds(FX_PI_X, 128)));
+        lds = std::max(lds, fx_persist_lds(FX_PI_F, d.nfft, d.pm != nullptr));
     }
-    b->nctrs = FX_PCTR_FIRST + 2 * nsub;
-    if (!rc) rc = fx_malloc(ctx, sizeof(fx_psub) * (size_t)nsub, (void**)&b->d_subs);
-    if (!rc) rc = fx_memcpy_h2d(ctx, b->d_subs, hs.data(), sizeof(fx_psub) * (size_t)nsub);
+    const int ns = nsub;
+    b->nctrs = FX_PCTR_FIRST + FX_PCTR_PER_SUB * ns;
+    if (!rc) rc = fx_malloc(ctx, sizeof(fx_psub) * (size_t)ns, (void**)&b->d_subs);
     if (!rc) rc = fx_malloc(ctx, sizeof(int) * (size_t)b->nctrs, (void**)&b->d_ctrs);
     if (!rc) {
         b->lds = lds;
         b->nblocks = fxk_batch_persistent_blocks(lds);
         if (b->nblocks < 1) rc = FX_ERR_HIP;   // fx_last_error() says why
     }
-    // tune each shape once (the tune file already holds the others)
-    for (int s = 0; s < nsub && !rc; s++) {
+    // each subband's share of the device: its part of the batch's work, at
+    // least one XCD's worth (an eighth), in whole XCD rounds
+    std::vector<int> tblocks((size_t)ns, 0);
+    if (!rc) {
+        double tot = 0;
+        std::vector<double> w((size_t)ns);
+        for (int s = 0; s < ns; s++) {
+            const fx_subband_desc& d = b->sb[(size_t)s].d;
+            w[(size_t)s] = (double)d.nspectra * d.ninp * (2.5 * d.nfft * std::log2((double)d.nfft) + 4.0 * d.nfft * (d.ninp + 1));
+            tot += w[(size_t)s];
+        }
+        for (int s = 0; s < ns; s++) {
+            const long share = std::lround(b->nblocks * w[(size_t)s] / std::max(tot, 1.0));
+            tblocks[(size_t)s] = (int)std::min<long>(b->nblocks, std::max<long>(b->nblocks / 8, round_up(share, 8)));
+        }
+    }
+    std::vector<bool> tuned((size_t)ns, false);
+    for (int s = 0; s < ns && !rc; s++) {
+        Subband& r = b->sb[(size_t)s];
+        if (tune_file) tuned[(size_t)s] = lookup_persist(tune_file, r.d, tblocks[(size_t)s], r);
+        if (!r.nat) r.pnat = 0;
+    }
+    // tune each shape once (the tune file already holds the others); the
+    // slice sums get a temporary buffer of the largest candidate that fits
+    for (int s = 0; s < ns && !rc; s++) {
         if (!tune_file || tuned[(size_t)s]) continue;
-        rc = autotune_persist(b, s, tune_file);
-        const fx_subband_desc& d = b->sb[(size_t)s].d;
-        for (int q = s + 1; q < nsub; q++) {
+        Subband& r = b->sb[(size_t)s];
+        size_t pb = 0;
+        for (int ts : {16, 8, 4, 2})
+            if (!pb && part_bytes_of(r.d, slices_of(r.ntp, ts)) <= (size_t)kPartMax)
+                pb = part_bytes_of(r.d, slices_of(r.ntp, ts));
+        if (pb) rc = fx_malloc(ctx, pb, (void**)&r.part);
+        r.part_bytes = rc ? 0 : pb;
+        if (!rc) rc = autotune_persist(b, s, tblocks[(size_t)s], tune_file);
+        if (r.part) fx_free(ctx, r.part);
+        r.part = nullptr;
+        r.part_bytes = 0;
+        for (int q = s + 1; q < ns; q++) {
             Subband& o = b->sb[(size_t)q];
-            if (!tuned[(size_t)q] && o.d.nfft == d.nfft && o.d.ninp == d.ninp && o.d.nspectra == d.nspectra) {
-                o.xtile = b->sb[(size_t)s].xtile;
-                o.cb = b->sb[(size_t)s].cb;
-                o.gpi = b->sb[(size_t)s].gpi;
+            if (!tuned[(size_t)q] && o.d.nfft == r.d.nfft && o.d.ninp == r.d.ninp && o.d.nspectra == r.d.nspectra &&
+                tblocks[(size_t)q] == tblocks[(size_t)s]) {
+                o.pnat = r.pnat;
+                o.cb = r.cb;
+                o.gpi = r.gpi;
+                o.nts = r.nts;
                 tuned[(size_t)q] = true;
             }
         }
+    }
+    // the chosen layouts: slice sums where the X items are sliced, natural
+    // rows only where they are used
+    for (int s = 0; s < ns && !rc; s++) {
+        Subband& r = b->sb[(size_t)s];
+        r.nts = slices_of(r.ntp, std::max(1, r.nts));
+        if (r.nts > 1) {
+            r.part_bytes = part_bytes_of(r.d, r.nts);
+            rc = fx_malloc(ctx, r.part_bytes, (void**)&r.part);
+        }
+        if (!rc && !r.pnat && r.nat) {
+            fx_free(ctx, r.nat);
+            r.nat = nullptr;
+        }
+        if (!rc) rc = upload_psub(b, s);
     }
     std::vector<fx_pitem> items;
     if (!rc) rc = build_table(b, mode == FX_BATCH_PERSISTENT, items, b->phase_end);
@@ -450,14 +607,14 @@ int batch_create_persist(fx_ctx* ctx, const fx_subband_desc* sb, int nsub, int p
         rc = fx_malloc(ctx, sizeof(fx_pitem) * items.size(), (void**)&b->d_items);
     }
     if (!rc) rc = fx_memcpy_h2d(ctx, b->d_items, items.data(), sizeof(fx_pitem) * items.size());
+    if (!rc) rc = fx_ctx_sync(ctx);
     if (!rc) {
         // one graph: the clears and the launch(es)
         hipStream_t s0 = fx_ctx_hip_stream(ctx);
-        if (hipStreamSynchronize(s0) != hipSuccess ||
-            hipStreamBeginCapture(s0, hipStreamCaptureModeRelaxed) != hipSuccess) {
+        if (hipStreamBeginCapture(s0, hipStreamCaptureModeRelaxed) != hipSuccess) {
             rc = FX_ERR_HIP;
         } else {
-            int crc = enqueue_persist(b, s0);
+            int crc = enqueue_persist(b, s0, b->d_items, b->phase_end, mode == FX_BATCH_PERSISTENT, 0);
             hipGraph_t g = nullptr;
             const bool ended = hipStreamEndCapture(s0, &g) == hipSuccess;
             if (crc || !ended) {
@@ -565,6 +722,8 @@ extern "C" int fx_batch_destroy(fx_batch* b)
         fx_ctx* own = s.ctx ? s.ctx : b->ctx;
         if (s.vis) fx_free(own, s.vis);
         if (s.chan) fx_free(own, s.chan);
+        if (s.nat) fx_free(own, s.nat);
+        if (s.part) fx_free(own, s.part);
         if (s.pm_dev) fx_free(own, s.pm_dev);
         if (s.ctx) fx_ctx_destroy(s.ctx);
     }
@@ -704,14 +863,16 @@ extern "C" int fx_batch_sync(fx_batch* b)
     return FX_OK;
 }
 
-extern "C" int fx_batch_persist_tuning(fx_batch* b, int s, int* xtile, int* chan_per_item, int* groups_per_item)
+extern "C" int fx_batch_persist_tuning(fx_batch* b, int s, int* natural_rows, int* chan_per_item,
+                                       int* groups_per_item, int* time_slices)
 {
     FX_REQUIRE(b && s >= 0 && s < (int)b->sb.size(), FX_ERR_ARG, "fx_batch_persist_tuning: subband %d", s);
     FX_REQUIRE(b->mode >= FX_BATCH_PERSISTENT, FX_ERR_STATE, "fx_batch_persist_tuning: not a persistent batch");
     const Subband& r = b->sb[(size_t)s];
-    if (xtile) *xtile = r.xtile;
+    if (natural_rows) *natural_rows = r.pnat;
     if (chan_per_item) *chan_per_item = r.cb;
     if (groups_per_item) *groups_per_item = r.gpi;
+    if (time_slices) *time_slices = r.nts;
     return FX_OK;
 }
 

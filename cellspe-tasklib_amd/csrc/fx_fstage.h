@@ -9,6 +9,7 @@
 
 #include <hip/hip_fp8.h>
 
+#include <cstddef>
 #include <type_traits>
 
 namespace {
@@ -58,13 +59,21 @@ struct PhaseTab {
     static constexpr int PHI = M / PLO;
 };
 
+// The item bookkeeping comes first: a kernel without a phase model may give
+// an FItems only the LDS up to ph_lo (fitems_head_bytes).
 template <int M, int F>
 struct FItems {
+    int t[F], i[F];   // (spectrum, input) of item f; t = -1 past the end
+    float amp[F];
     float2 ph_lo[F][PhaseTab<M>::PLO];
     float2 ph_hi[F][PhaseTab<M>::PHI];
-    float amp[F];
-    int t[F], i[F];   // (spectrum, input) of item f; t = -1 past the end
 };
+template <int M, int F>
+__host__ __device__ constexpr size_t fitems_head_bytes()
+{
+    using T = FItems<M, F>;
+    return offsetof(T, ph_lo);
+}
 
 // Tables of the F kernels in LDS: the 16-entry unpack table, the second-pass
 // twiddles tw2[r*16 + k] = W_256^{r k}, for M = 4096 the third-pass tables
@@ -106,12 +115,15 @@ __device__ __forceinline__ void fstage_consts(const float2* __restrict__ twg, FT
 // them: complex point e of an item is the 4-bit field e%8 of word e/8, one
 // lookup in the 16-entry table lut (both samples of the point).
 // hook(std::integral_constant<int, p>) runs after FFT pass p
-// (p < fft_npasses<M, 1>()), between the passes' barriers.
-template <int M, int F, int NT, bool IM, typename Hook>
+// (p < fft_npasses<M, 1>()), between the passes' barriers.  stage(raw)
+// writes the group's packed words to raw (thread tid the WTH words from
+// WTH*tid on); the overload without it loads them from words.
+template <int M, int F, int NT, bool IM, typename Hook, typename Stage>
 __device__ __forceinline__ void fstage_group(const uint32_t* __restrict__ words, size_t in_stride, int ninp,
                                              long t_first, long nt, const fx_phase_model* __restrict__ pm,
                                              float sc_uni, const float* __restrict__ sc_inp, long g0, float2* sh,
-                                             const FTabs<M>& tb, FItems<M, F>& it, int tid, Hook&& hook)
+                                             const FTabs<M>& tb, FItems<M, F>& it, int tid, Hook&& hook,
+                                             Stage&& stage)
 {
     constexpr int N = 2 * M;
     constexpr int WPT = N / 16;             // packed words per transform
@@ -156,17 +168,7 @@ __device__ __forceinline__ void fstage_group(const uint32_t* __restrict__ words,
         }
     }
     uint32_t* raw = reinterpret_cast<uint32_t*>(sh);
-    {
-        const int wl = WTH * tid;
-        const int f = wl / WPT;
-        const int wof = wl - f * WPT;
-        const bool valid = it.t[f] >= 0;
-        const uint32_t* src = words + (size_t)it.i[f] * in_stride + (size_t)(t_first + it.t[f]) * WPT + wof;
-#pragma unroll
-        for (int q = 0; q < WTH; q += 2)
-            *reinterpret_cast<uint2*>(raw + wl + q) =
-                valid ? *reinterpret_cast<const uint2*>(src + q) : make_uint2(0u, 0u);
-    }
+    stage(raw);
     __syncthreads();
     {
         constexpr int S1 = M / 16;
@@ -198,6 +200,28 @@ __device__ __forceinline__ void fstage_group(const uint32_t* __restrict__ words,
     }
     hook(std::integral_constant<int, 0>{});
     fft_passes_hook<M, 16, NT, F, true, FTabs<M>::T3, 1>(sh, tb.twl, tid, tb.tw2, tb.t3, hook);
+}
+
+template <int M, int F, int NT, bool IM, typename Hook>
+__device__ __forceinline__ void fstage_group(const uint32_t* __restrict__ words, size_t in_stride, int ninp,
+                                             long t_first, long nt, const fx_phase_model* __restrict__ pm,
+                                             float sc_uni, const float* __restrict__ sc_inp, long g0, float2* sh,
+                                             const FTabs<M>& tb, FItems<M, F>& it, int tid, Hook&& hook)
+{
+    constexpr int WPT = M / 8;                  // packed words per transform
+    constexpr int WTH = (F * M / 8) / NT;       // packed words per thread
+    fstage_group<M, F, NT, IM>(words, in_stride, ninp, t_first, nt, pm, sc_uni, sc_inp, g0, sh, tb, it, tid, hook,
+                               [&](uint32_t* raw) {
+        const int wl = WTH * tid;
+        const int f = wl / WPT;
+        const int wof = wl - f * WPT;
+        const bool valid = it.t[f] >= 0;
+        const uint32_t* src = words + (size_t)it.i[f] * in_stride + (size_t)(t_first + it.t[f]) * WPT + wof;
+#pragma unroll
+        for (int q = 0; q < WTH; q += 2)
+            *reinterpret_cast<uint2*>(raw + wl + q) =
+                valid ? *reinterpret_cast<const uint2*>(src + q) : make_uint2(0u, 0u);
+    });
 }
 
 template <int M, int F, int NT, bool IM>

@@ -467,11 +467,14 @@ int fxk_difx_post(hipStream_t s, const float2* fft, int nfft, int band0, int n, 
 
 // ---------------------------------------------------------------------------
 // Persistent batch kernel (fx_persist.hip): one launch walks a table of work
-// items over many subbands -- phase-cal folds, F groups, X tiles, phase-cal
-// tones -- taking them in table order from a queue counter; an item may wait
-// for a counter of the items it depends on, all of which come earlier in the
-// table.
-// ---------------------------------------------------------------------------
+// items over many subbands -- phase-cal folds, F groups, corner turns, X
+// tiles, slice reductions, phase-cal tones -- with 512-thread workgroups.
+//
+// Subband layouts: chan[c][t][i] half2, ntp rows of ipad inputs per channel
+// (ipad = ninp rounded up to 4, ntp = nt rounded up to 32, the pad never
+// written and zero); nat[t][i][c] (rows t*ipad + i of nfft/2 channels) when
+// the F items write natural rows and corner-turn items build chan; part[s][p][c]
+// the raw sums of time slice s when the X items are split over nts slices.
 struct fx_psub {
     const uint32_t* words;      // input 0; input i at words + i*stride
     size_t stride;
@@ -479,34 +482,48 @@ struct fx_psub {
     long nt;                    // spectra integrated
     const fx_phase_model* pm;   // device array of ninp models, or null
     float sc;                   // uniform amplitude scale before the FP16 cast
-    uint32_t* chan;             // [c][t][i] half2 samples, nt deep
+    int ipad;                   // inputs per chan / nat row
+    uint32_t* chan;             // [c][ntp][ipad] half2 samples
+    uint32_t* nat;              // [nt][ipad][nfft/2] natural rows, or null (F items write chan)
     float2* vis;                // [p][c] visibilities (scale removed)
+    float2* part;               // [nts][p][c] raw slice sums (nts > 1), else null
+    long ntp;                   // spectra rows of chan (multiple of 32)
+    long tslice;                // spectra per time slice (multiple of 32)
+    int nts;                    // time slices
+    int pad0;
     double* pacc;               // phase-cal fold, ninp x nfold (null: no phase-cal)
     double* tones;              // ninp x ntones complex
     int nfold, nbins, ntones, offset_bin, step, rotatorlen;
     double dphi;
 };
-enum { FX_PI_PCAL = 0, FX_PI_F = 1, FX_PI_X = 2, FX_PI_TONE = 3 };
+enum { FX_PI_PCAL = 0, FX_PI_F = 1, FX_PI_X = 2, FX_PI_TONE = 3, FX_PI_CT = 4, FX_PI_RED = 5 };
 struct fx_pitem {
     int type, sb;
-    long a, b;      // F: first group, groups; X: first channel, channels; PCAL: input, first word; TONE: input
-    int c;          // X: tile index; PCAL: words per item
-    int xtile;      // X: 64 or 128
+    long a, b;      // F: first group, groups; X: first channel, channels; CT: first spectrum, spectra;
+                    // PCAL: input, first word; TONE: input; RED: first product, products
+    int c;          // X: tile index; CT: channel block * (input blocks) + input block; PCAL: words per item
+    int d;          // X: time slice
     int wait_ctr;   // counter this item waits for (-1: none)
     int wait_val;   // value that counter must reach
     int done_ctr;   // counter incremented when the item is done (-1: none)
     int pad;
 };
 // counters: [0..3] queue heads (one per launch of a phased run), [4] error
-// flag, then the items' wait/done counters
+// flag, then four per subband: F, CT, X and PCAL items done
 #define FX_PCTR_ERR 4
 #define FX_PCTR_FIRST 8
-// F groups of an nfft-point subband: 4096/(nfft/2) items per group, 256 threads
+#define FX_PCTR_PER_SUB 4
+#define FX_PNT 512         // threads per workgroup
+#define FX_PXKT 32         // spectra per X stage (chan rows are padded to a multiple)
+#define FX_PCT_T 32        // spectra per corner-turn item
+// F groups of an nfft-point subband: 16384/nfft transforms per group
 int fx_persist_group_items(int nfft);
-// LDS bytes an item of this kind needs (F: nfft; X: tile; PCAL: nfold; TONE)
-size_t fx_persist_lds(int type, int arg);
-// items [i0, i1) of the table on nblocks workgroups of 256 threads; wait: honour
-// the items' waits (one fused launch), else the caller orders the launches
+// LDS bytes an item of this kind needs (F: nfft, with phase tables when
+// pm; X: diagonal-only or any tile; PCAL: nfold; the others: none extra)
+size_t fx_persist_lds(int type, int arg, bool pm);
+// items [i0, i1) of the table on nblocks workgroups; wait: honour the items'
+// waits and count their completions (one fused launch), else the caller
+// orders the launches
 int fxk_batch_persistent(hipStream_t s, const float2* tw, const fx_psub* subs, const fx_pitem* items, int i0, int i1,
                          int* ctrs, int head, int nblocks, int wait, size_t lds_bytes);
 // workgroups that fill the device with this much dynamic LDS (0 on error)

@@ -1,17 +1,26 @@
 // fx_persist.hip -- the batch of uneven subbands in one persistent launch.
 //
 // k_batch_persistent walks a host-built table of work items with a
-// workgroup-wide queue counter: phase-cal folds, F groups (2-bit unpack ->
-// real FFT -> FP64-model rotation -> FP16 [c][t][i] store), X tiles (the
-// FP16 MFMA engine over a subband's whole integration, writing its
-// visibilities) and phase-cal tone extractions, for subbands of any FFT length
-// and input count in one table.  An X item waits until every F group of its
-// subband is done and a tone item until every fold of its subband is done;
-// the items it waits for come earlier in the table, so the workgroups that
-// took them are running and the wait ends (it is also bounded: a wait that
-// outlasts PSPIN polls raises the error counter and skips the item).  The
-// same table can run as phased launches instead (folds and F groups, then X
-// tiles, then tones), which order the items by launch and skip the waits.
+// workgroup-wide queue counter; 512-thread workgroups, two per CU:
+//   PCAL  exact FP64 phase-cal fold of a range of one input's words;
+//   F     F groups (2-bit unpack -> real FFT -> FP64-model rotation -> FP16),
+//         8192 complex points per group, stored either straight into the
+//         X-engine layout chan[c][t][i] or as natural rows nat[t][i][c];
+//   CT    corner turn of natural rows into chan, 64 x 64 tiles through LDS
+//         (long transforms: a group holds only 1-4 of them, so their direct
+//         stores would be 4-16 byte fragments);
+//   X     channels of one 64 x 64 input tile over one time slice, FP16 MFMA,
+//         the two halves of the workgroup on the two channels of a pair,
+//         each streaming its spectra through an LDS ring by LDS-DMA loads;
+//   RED   the time slices' partial sums of a product range, in slice order;
+//   TONE  the tones of one input from its fold.
+// In the fused form (one launch) an item may wait for a counter of items it
+// depends on (CT for the subband's F items, X for its F or CT items, RED for
+// its X items, TONE for its folds); those items come earlier in the table, so
+// the workgroups that took them are running and the wait ends (it is also
+// bounded: a wait that outlasts PSPIN polls raises the error counter and
+// skips the item).  The phased form runs the same items as four ordered
+// launches (PCAL+F, CT, X, RED+TONE) without waits.
 //
 // The time-sliced per-core loop of the reference's dispatcher
 // (TaskDispatcher.cpp:116-137) becomes this single queue of uneven work.
@@ -23,14 +32,17 @@
 
 namespace {
 
-constexpr int PNT = XNT;          // threads per workgroup
-constexpr int PFM = 4096;         // complex points per F group (one transform from M = 8192 up)
+constexpr int PNT = FX_PNT;       // threads per workgroup
+constexpr int PFM = 8192;         // complex points per F group
+constexpr int XKT = FX_PXKT;      // spectra per X stage
+constexpr int XTS = 64;           // inputs per X tile side
+constexpr int CTP = 65;           // corner-turn tile row pitch (words): conflict-free both ways
 constexpr long PSPIN = 1L << 24;  // polls of a wait before it is abandoned (seconds of GPU time)
+static_assert(PNT == 512, "the item layouts below assume 8 waves");
 
 template <int M>
 struct PGroup {
-    static constexpr int FM = M > PFM ? M : PFM;
-    static constexpr int F = FM / M;   // transforms per group
+    static constexpr int F = PFM / M;   // transforms per group
 };
 
 // A pointer read from memory is a generic pointer to the compiler, and a
@@ -39,6 +51,11 @@ struct PGroup {
 // whose pointers are declared global, so their accesses stay global loads and
 // stores.
 #define FX_GLOBAL __attribute__((address_space(1)))
+// plain vector types (HIP's vector classes cannot be assigned in another
+// address space)
+typedef unsigned int pu32x2 __attribute__((ext_vector_type(2)));
+typedef unsigned int pu32x4 __attribute__((ext_vector_type(4)));
+typedef float pf32x2 __attribute__((ext_vector_type(2)));
 struct PSubView {
     const uint32_t FX_GLOBAL* words;
     size_t stride;
@@ -46,54 +63,129 @@ struct PSubView {
     long nt;
     const fx_phase_model FX_GLOBAL* pm;
     float sc;
+    int ipad;
     uint32_t FX_GLOBAL* chan;
+    uint32_t FX_GLOBAL* nat;
     float2 FX_GLOBAL* vis;
+    float2 FX_GLOBAL* part;
+    long ntp;
+    long tslice;
+    int nts;
+    int pad0;
     double FX_GLOBAL* pacc;
     double FX_GLOBAL* tones;
     int nfold, nbins, ntones, offset_bin, step, rotatorlen;
     double dphi;
 };
 static_assert(sizeof(PSubView) == sizeof(fx_psub) && offsetof(PSubView, dphi) == offsetof(fx_psub, dphi) &&
-                  offsetof(PSubView, chan) == offsetof(fx_psub, chan) && offsetof(PSubView, pacc) == offsetof(fx_psub, pacc),
+                  offsetof(PSubView, chan) == offsetof(fx_psub, chan) &&
+                  offsetof(PSubView, part) == offsetof(fx_psub, part) &&
+                  offsetof(PSubView, nts) == offsetof(fx_psub, nts) && offsetof(PSubView, pacc) == offsetof(fx_psub, pacc),
               "PSubView mirrors fx_psub");
+
+#define FX_CONST __attribute__((address_space(4)))
+
+// The item functions are separate (not inlined) functions, so each gets the
+// whole register budget of the 16 waves per CU for itself.  Their arguments
+// arrive in per-lane registers: these make them uniform again, and the
+// subband's view a scalar-loaded constant.
+__device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ long uni(long v)
+{
+    const uint64_t u = (uint64_t)v;
+    return (long)(((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(u >> 32)) << 32) |
+                  (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)u));
+}
+__device__ __forceinline__ const PSubView FX_CONST& psub(const fx_psub* subs, int sb)
+{
+    const long a = uni((long)(uintptr_t)subs);
+    return reinterpret_cast<const PSubView FX_CONST*>((uintptr_t)a)[uni(sb)];
+}
 
 __host__ __device__ constexpr size_t al16(size_t n) { return (n + 15) / 16 * 16; }
 
-// LDS of an F item: the transform tile, the tables, the items' phasor tables
+// LDS of an F item: the transform tile, the tables, the items (their phasor
+// tables only with a phase model)
 template <int M>
-__host__ __device__ constexpr size_t f_sh_bytes() { return al16((size_t)lds_complex(PGroup<M>::FM) * sizeof(float2)); }
+__host__ __device__ constexpr size_t f_sh_bytes() { return al16((size_t)lds_complex(PFM) * sizeof(float2)); }
 template <int M>
 __host__ __device__ constexpr size_t f_tab_bytes() { return al16(sizeof(FTabs<M>)); }
 template <int M>
-__host__ __device__ constexpr size_t f_lds_bytes()
+__host__ __device__ constexpr size_t f_lds_bytes(bool pm)
 {
-    return f_sh_bytes<M>() + f_tab_bytes<M>() + sizeof(FItems<M, PGroup<M>::F>);
+    return f_sh_bytes<M>() + f_tab_bytes<M>() +
+           (pm ? sizeof(FItems<M, PGroup<M>::F>) : fitems_head_bytes<M, PGroup<M>::F>());
+}
+
+// LDS of an X item: two halves x RING stages x (one or two 8 KiB slabs)
+constexpr int XSLAB = XKT * XTS * 4;
+template <bool DIAG>
+__host__ __device__ constexpr int x_ring() { return DIAG ? 4 : 2; }
+template <bool DIAG>
+__host__ __device__ constexpr size_t x_lds_bytes() { return (size_t)2 * x_ring<DIAG>() * (DIAG ? 1 : 2) * XSLAB; }
+
+template <int N>
+__device__ __forceinline__ void p_wait_vmcnt()
+{
+    // s_waitcnt simm16 on gfx9: vmcnt[3:0] | expcnt[6:4] | lgkmcnt[11:8] | vmcnt[5:4] << 14
+    static_assert(N >= 0 && N < 64, "vmcnt range");
+    __builtin_amdgcn_s_waitcnt((N & 0xF) | (0x7 << 4) | (0xF << 8) | (((N >> 4) & 3) << 14));
 }
 
 // F groups [g_first, g_first + ngroups) of subband S: group g holds items
 // g*F .. g*F+F-1 in the order item = t*I + i, each an unpack -> FFT ->
-// rotate of one (spectrum, input); every channel goes to chan[c][t][i].
+// rotate of one (spectrum, input).  Each thread's packed words of the next
+// group load while the current group transforms.
 template <int M>
-__device__ void f_item(const float2* __restrict__ twg, const PSubView& S, long g_first, long ngroups,
-                       unsigned char* lds)
+__device__ __noinline__ void f_item(const float2* __restrict__ twg_, const fx_psub* subs, int sb, long g_first_,
+                                    long ngroups_, unsigned char* lds)
 {
     constexpr int F = PGroup<M>::F;
     constexpr int N = 2 * M;
     constexpr int PLO = PhaseTab<M>::PLO;
+    constexpr int WPT = M / 8;               // packed words per transform
+    constexpr int WTH = (F * M / 8) / PNT;   // packed words per thread
+    static_assert(WTH == 2, "one uint2 of packed words per thread and group");
     float2* sh = reinterpret_cast<float2*>(lds);
     FTabs<M>& tb = *reinterpret_cast<FTabs<M>*>(lds + f_sh_bytes<M>());
     FItems<M, F>& it = *reinterpret_cast<FItems<M, F>*>(lds + f_sh_bytes<M>() + f_tab_bytes<M>());
     const int tid = threadIdx.x;
+    const PSubView FX_CONST& S = psub(subs, sb);
+    const long g_first = uni(g_first_), ngroups = uni(ngroups_);
+    const float2* twg = reinterpret_cast<const float2*>(uni((long)(uintptr_t)twg_));
+    const uint32_t FX_GLOBAL* const words = S.words;
+    const size_t stride = S.stride;
     const int I = S.ninp;
-    const size_t row = (size_t)S.nt * (size_t)I;   // one channel of the whole integration
-    const bool pm = (const void*)S.pm != nullptr;
+    const long nt = S.nt;
+    const fx_phase_model* const pmp = (const fx_phase_model*)S.pm;
+    const float sc = S.sc;
+    const int ipad = S.ipad;
+    uint32_t FX_GLOBAL* const chan = S.chan;
+    uint32_t FX_GLOBAL* const natr = S.nat;
+    const long nitems = nt * (long)I;
+    const bool pm = pmp != nullptr;
+    const bool nat = (const void*)natr != nullptr;
+    const size_t crow = (size_t)S.ntp * (size_t)ipad;   // one channel of chan
     auto tw_of = [&](int k) {
-        if constexpr (FTabs<M>::TWL)
+        if constexpr (FTabs<M>::TWL) {
             return twl_get(tb.twl, k * (FX_TW_N / N));
-        else
-            return twg[k * (FX_TW_N / N)];
+        } else {
+            const pf32x2 v = ((const pf32x2 FX_GLOBAL*)(uintptr_t)twg)[k * (FX_TW_N / N)];
+            return make_float2(v.x, v.y);
+        }
+    };
+    auto wload = [&](long grp) -> pu32x2 {
+        const int wl = WTH * tid;
+        const int f = wl / WPT;
+        const long g = grp * F + f;
+        if (g >= nitems) return pu32x2{0u, 0u};
+        const long t = g / I;
+        const int i = (int)(g - t * I);
+        return *reinterpret_cast<const pu32x2 FX_GLOBAL*>(words + (size_t)i * stride + (size_t)t * WPT +
+                                                          (wl - f * WPT));
     };
     fstage_consts<M, PNT>(twg, tb, tid);
+    pu32x2 pre = wload(g_first);
     int tl = tid;
     for (long gg = 0; gg < ngroups; gg++) {
         if (gg) {
@@ -103,73 +195,274 @@ __device__ void f_item(const float2* __restrict__ twg, const PSubView& S, long g
             // the loop into registers held across all groups
             asm volatile("" : "+v"(tl));
         }
-        fstage_group<M, F, PNT, false>((const uint32_t*)S.words, S.stride, I, 0, S.nt, (const fx_phase_model*)S.pm,
-                                       S.sc, nullptr, (g_first + gg) * F, sh, tb,
-                                       it, tl, [](auto) {});
-        // channel pairs (kp, M - kp), items fastest: neighbouring lanes store
-        // neighbouring inputs of one spectrum
+        const long grp = g_first + gg;
+        fstage_group<M, F, PNT, false>((const uint32_t*)words, stride, I, 0, nt, pmp, sc, nullptr, grp * F, sh, tb,
+                                       it, tl, [](auto) {},
+                                       [&](uint32_t* raw) {
+            *reinterpret_cast<pu32x2*>(raw + WTH * tl) = pre;
+            if (gg + 1 < ngroups) pre = wload(grp + 1);
+        });
+        auto fin = [&](float2 x, int f, int k) {
+            return fstage_finish<M, __half2, F, false>(x, it, f, k, pm ? it.ph_lo[f][k % PLO] : make_float2(1.f, 0.f),
+                                                      pm, nullptr, nullptr);
+        };
+        if (!nat) {
+            // channel pairs (kp, M - kp), items fastest: neighbouring lanes
+            // store neighbouring inputs of one spectrum
 #pragma unroll 1
-        for (int q = tl; q < F * (M / 2); q += PNT) {
-            const int f = q % F;
-            const int kp = q / F;
-            if (it.t[f] < 0) continue;
-            float2 xk, xm;
-            rsplit2_tw<M>(sh, f * M, kp, tw_of(kp), xk, xm);
-            uint32_t FX_GLOBAL* col = S.chan + (size_t)it.t[f] * I + it.i[f];
-            col[(size_t)kp * row] = fstage_finish<M, __half2, F, false>(
-                xk, it, f, kp, pm ? it.ph_lo[f][kp % PLO] : make_float2(1.f, 0.f), pm, nullptr, nullptr);
-            if (kp) {   // kp = 0: M - kp is the dropped Nyquist bin
-                const int k2 = M - kp;
-                col[(size_t)k2 * row] = fstage_finish<M, __half2, F, false>(
-                    xm, it, f, k2, pm ? it.ph_lo[f][k2 % PLO] : make_float2(1.f, 0.f), pm, nullptr, nullptr);
+            for (int q = tl; q < F * (M / 2); q += PNT) {
+                const int f = q % F;
+                const int kp = q / F;
+                if (it.t[f] < 0) continue;
+                float2 xk, xm;
+                rsplit2_tw<M>(sh, f * M, kp, tw_of(kp), xk, xm);
+                uint32_t FX_GLOBAL* col = chan + (size_t)it.t[f] * ipad + it.i[f];
+                col[(size_t)kp * crow] = fin(xk, f, kp);
+                if (kp) col[(size_t)(M - kp) * crow] = fin(xm, f, M - kp);   // kp = 0: M - kp is the dropped Nyquist bin
             }
-        }
-        if (tl < F && it.t[tl] >= 0) {
-            float2 xk, xm;
-            rsplit2_tw<M>(sh, tl * M, M / 2, tw_of(M / 2), xk, xm);
-            S.chan[(size_t)(M / 2) * row + (size_t)it.t[tl] * I + it.i[tl]] = fstage_finish<M, __half2, F, false>(
-                xk, it, tl, M / 2, pm ? it.ph_lo[tl][(M / 2) % PLO] : make_float2(1.f, 0.f), pm, nullptr, nullptr);
+            if (tl < F && it.t[tl] >= 0) {
+                float2 xk, xm;
+                rsplit2_tw<M>(sh, tl * M, M / 2, tw_of(M / 2), xk, xm);
+                chan[(size_t)(M / 2) * crow + (size_t)it.t[tl] * ipad + it.i[tl]] = fin(xk, tl, M / 2);
+            }
+        } else {
+            // natural rows: neighbouring lanes store neighbouring channels
+#pragma unroll 1
+            for (int q = tl; q < F * (M / 2); q += PNT) {
+                const int f = q / (M / 2);
+                const int kp = q - f * (M / 2);
+                if (it.t[f] < 0) continue;
+                float2 xk, xm;
+                rsplit2_tw<M>(sh, f * M, kp, tw_of(kp), xk, xm);
+                uint32_t FX_GLOBAL* row = natr + ((size_t)it.t[f] * ipad + it.i[f]) * (size_t)M;
+                row[kp] = fin(xk, f, kp);
+                if (kp) row[M - kp] = fin(xm, f, M - kp);
+            }
+            if (tl < F && it.t[tl] >= 0) {
+                float2 xk, xm;
+                rsplit2_tw<M>(sh, tl * M, M / 2, tw_of(M / 2), xk, xm);
+                natr[((size_t)it.t[tl] * ipad + it.i[tl]) * (size_t)M + M / 2] = fin(xk, tl, M / 2);
+            }
         }
     }
 }
 
-// X tile `tile` (TSZ x TSZ inputs) of channels [c0, c0 + nc) over the
-// subband's whole integration; visibilities vis[p][c] with the F stage's
-// amplitude scale removed (p = i*I - i(i-1)/2 + (j-i), i <= j)
-template <int TSZ>
-__device__ void x_item(const PSubView& S, long c0, long nc, int tile, unsigned char* lds)
+// Corner turn of the natural rows of spectra [t0, t0 + ntt) into chan, for
+// channel block cb (64 channels) and input block ib (64 inputs): per spectrum
+// a 64 x 64 tile read as rows of 64 channels, written as rows of 64 inputs,
+// the next spectrum's rows loading while the current tile turns.
+__device__ __noinline__ void ct_item(const fx_psub* subs, int sb, long t0_, long ntt_, int blk_, unsigned char* lds)
 {
-    constexpr int KT = XTraits<FX_PREC_F16>::KT;
-    constexpr int NBW = TSZ / 64;
-    uint32_t(*slab)[KT][TSZ] = reinterpret_cast<uint32_t(*)[KT][TSZ]>(lds);
+    uint32_t* tile = reinterpret_cast<uint32_t*>(lds);   // [64 channels][CTP]
+    const int tid = threadIdx.x;
+    const PSubView FX_CONST& S = psub(subs, sb);
+    const long t0 = uni(t0_), ntt = uni(ntt_);
+    const int blk = uni(blk_);
     const int I = S.ninp;
-    const int nb = (I + TSZ - 1) / TSZ;
-    int ti, tj;
-    tile_coords(tile, nb, ti, tj);
+    const int ipad = S.ipad;
+    const uint32_t FX_GLOBAL* const natr = S.nat;
+    uint32_t FX_GLOBAL* const chan = S.chan;
+    const int nib = (I + 63) / 64;
+    const int cb = blk / nib, ib = blk - cb * nib;
     const long C = S.nfft / 2;
+    const int i0 = ib * 64;
+    const long cc0 = (long)cb * 64;
+    const size_t crow = (size_t)S.ntp * (size_t)ipad;
+    const long t1 = t0 + ntt < S.nt ? t0 + ntt : S.nt;
+    // read map: input rows rr and rr + 32, channels rs .. rs + 3
+    const int rr = tid >> 4, rs = (tid & 15) * 4;
+    // write map: channel wc, inputs wi .. wi + 7
+    const int wc = tid >> 3, wi = (tid & 7) * 8;
+    pu32x4 v[2];
+    auto load = [&](long t) {
+#pragma unroll
+        for (int k = 0; k < 2; k++) {
+            const int i = i0 + rr + 32 * k;
+            v[k] = (t < t1 && i < I)
+                       ? *reinterpret_cast<const pu32x4 FX_GLOBAL*>(natr + ((size_t)t * ipad + i) * (size_t)C + cc0 + rs)
+                       : pu32x4{0u, 0u, 0u, 0u};
+        }
+    };
+    load(t0);
+    for (long t = t0; t < t1; t++) {
+        if (t > t0) __syncthreads();   // the previous spectrum's tile has been read
+#pragma unroll
+        for (int k = 0; k < 2; k++) {
+            const int r = rr + 32 * k;
+            tile[(rs + 0) * CTP + r] = v[k].x;
+            tile[(rs + 1) * CTP + r] = v[k].y;
+            tile[(rs + 2) * CTP + r] = v[k].z;
+            tile[(rs + 3) * CTP + r] = v[k].w;
+        }
+        __syncthreads();
+        load(t + 1);
+        uint32_t o[8];
+#pragma unroll
+        for (int e = 0; e < 8; e++) o[e] = tile[wc * CTP + wi + e];
+        uint32_t FX_GLOBAL* dst = chan + (size_t)(cc0 + wc) * crow + (size_t)t * ipad + i0 + wi;
+        if (i0 + wi + 8 <= I) {
+            reinterpret_cast<pu32x4 FX_GLOBAL*>(dst)[0] = pu32x4{o[0], o[1], o[2], o[3]};
+            reinterpret_cast<pu32x4 FX_GLOBAL*>(dst)[1] = pu32x4{o[4], o[5], o[6], o[7]};
+        } else {
+#pragma unroll
+            for (int e = 0; e < 8; e++)
+                if (i0 + wi + e < I) dst[e] = o[e];
+        }
+    }
+}
+
+// X work of channels [c0, c0 + nc) of tile (ti, tj) over time slice `slice`.
+// Waves 0-3 take the even channels of the range, waves 4-7 the odd ones;
+// each half streams its KT-spectrum slabs (one for a diagonal tile, the row
+// and column blocks otherwise) through its own ring of LDS stages filled by
+// LDS-DMA loads (global_load_lds_dwordx4: lane l's 16 bytes land at the
+// wave's LDS base + 16 l), RING - 1 stages in flight, continuing from one
+// channel to the next.  Wave (qi, qj) of a half accumulates the 32 x 32
+// block (qi, qj) of the tile (the strictly-lower block of a diagonal tile
+// idles).  A slab row holds 64 inputs from row0: rows past the subband's
+// inputs read padding or the next spectrum's inputs and only reach products
+// that are not stored.  With one slice the visibilities go to vis with the F
+// stage's amplitude scale removed, else the raw sums to part[slice].
+template <bool DIAG>
+__device__ __noinline__ void x_item(const fx_psub* subs, int sb, int c0_, int nc_, int ti_, int tj_, int slice_,
+                                    unsigned char* lds)
+{
+    constexpr int RING = x_ring<DIAG>();
+    constexpr int NSLAB = DIAG ? 1 : 2;
+    constexpr int STAGE = NSLAB * XSLAB;
+    constexpr int NPER = STAGE / 1024 / 4;   // 1 KiB loads per wave per stage
+    const int tid = threadIdx.x;
+    const int w = tid >> 6, lane = tid & 63;
+    const int half = w >> 2, hw = w & 3;
+    const int qi = hw >> 1, qj = hw & 1;
+    const bool active = !(DIAG && qi > qj);
+    unsigned char* ring = lds + half * (RING * STAGE);
+    const PSubView FX_CONST& S = psub(subs, sb);
+    const int c0 = uni(c0_), nc = uni(nc_), ti = uni(ti_), tj = uni(tj_), slice = uni(slice_);
+    const int I = S.ninp;
+    const int C = S.nfft / 2;
+    const int ipad = S.ipad;
+    const uint32_t FX_GLOBAL* const chan = S.chan;
+    const int nts = S.nts;
     const float d = S.sc * S.sc;
-    for (long c = c0; c < c0 + nc; c++) {
-        if (c > c0) __syncthreads();   // the previous channel's last stage has been read
-        xengine_tile<FX_PREC_F16, TSZ>((const uint32_t*)(S.chan + (size_t)c * (size_t)S.nt * (size_t)I), S.nt, I, ti,
-                                       tj, slab,
-                                       [&](const auto& dre, const auto& dim, int w, int qi, int qj, bool diag,
-                                           int lane) {
+    pf32x2 FX_GLOBAL* const outb = reinterpret_cast<pf32x2 FX_GLOBAL*>(nts == 1 ? S.vis : S.part) +
+                                   (nts == 1 ? 0 : (size_t)slice * ((size_t)I * (I + 1) / 2) * C);
+    const int t0 = slice * (int)S.tslice;
+    const int t1 = t0 + (int)S.tslice < (int)S.ntp ? t0 + (int)S.tslice : (int)S.ntp;
+    const int nst = (t1 - t0) / XKT;   // >= 1: the host sizes the slices
+    const int G = ((nc + 1) / 2) * nst;
+    const size_t crow = (size_t)S.ntp * (size_t)ipad;
+    // this lane's part of every 1 KiB load: spectrum row 4 b8 + lrow of the
+    // stage, inputs lcol .. lcol + 3 of the slab's 64
+    const int loff = (lane >> 4) * ipad + (lane & 15) * 4;
+    // the next stage to load: channel pair ip, stage ist
+    int ip = 0, ist = 0, nissued = 0;
+    auto issue = [&]() {
+        int c = c0 + 2 * ip + half;
+        if (c >= c0 + nc) c = c0 + 2 * ip;   // odd count: the idle half reloads its partner's channel
+        const uint32_t FX_GLOBAL* src = chan + (size_t)c * crow + (size_t)(t0 + ist * XKT) * ipad + loff;
+        unsigned char* dst = ring + (nissued % RING) * STAGE;
 #pragma unroll
-            for (int a = 0; a < NBW; a++)
+        for (int k = 0; k < NPER; k++) {
+            const int blk = hw * NPER + k;   // 1 KiB block of the stage
+            const int s = blk >> 3, b8 = blk & 7;
+            const uint32_t FX_GLOBAL* a = src + (4 * b8) * ipad + (s ? tj : ti) * XTS;
+            __builtin_amdgcn_global_load_lds((FX_GLOBAL void*)a,
+                                             (__attribute__((address_space(3))) void*)(dst + blk * 1024), 16, 0, 0);
+        }
+        nissued++;
+        if (++ist == nst) {
+            ist = 0;
+            ip++;
+        }
+    };
+    fx_float16 dre = fx_float16{}, dim = fx_float16{};
+    const int r32 = lane & 31, h = lane >> 5;
+    p_wait_vmcnt<0>();   // nothing of an earlier item is outstanding in the counts below
+    for (int g = 0; g < RING - 1 && g < G; g++) issue();
+    bool drain = false;
+    int p = 0, st = 0;   // the stage consumed: channel pair p, stage st
+    for (int g = 0; g < G; g++) {
+        // this wave's loads of stage g are done once at most the later
+        // stages' loads are outstanding (after a channel's stores: none);
+        // the barrier makes every wave's part of the stage visible and frees
+        // the slot of stage g - 1
+        const int ahead = G - 1 - g;
+        if (drain) p_wait_vmcnt<0>();
+        else if (ahead >= RING - 2) p_wait_vmcnt<(RING - 2) * NPER>();
+        else if (ahead == 1) p_wait_vmcnt<NPER>();
+        else p_wait_vmcnt<0>();
+        drain = false;
+        // a plain barrier: __syncthreads() would also wait for the later
+        // stages' loads (its release fence drains vmcnt)
+        asm volatile("s_barrier" ::: "memory");
+        if (nissued < G) issue();
+        if (active) {
+            const uint32_t* sa = reinterpret_cast<const uint32_t*>(ring + (g % RING) * STAGE) + 4 * h * XTS +
+                                 qi * 32 + r32;
+            const uint32_t* sb = reinterpret_cast<const uint32_t*>(ring + (g % RING) * STAGE + (DIAG ? 0 : XSLAB)) +
+                                 4 * h * XTS + qj * 32 + r32;
+#pragma unroll 2
+            for (int ks = 0; ks < XKT / 8; ks++) {
+                fx_int4v ia, ib, ir;
 #pragma unroll
-                for (int b = 0; b < NBW; b++) {
-                    if (diag && qi == qj && a > b) continue;
+                for (int q = 0; q < 4; q++) {
+                    ia[q] = (int)sa[(ks * 8 + q) * XTS];
+                    const uint32_t v = sb[(ks * 8 + q) * XTS];
+                    ib[q] = (int)v;
+                    ir[q] = (int)rot_h2(v);
+                }
+                const fx_half8 fa = __builtin_bit_cast(fx_half8, ia);
+                dre = __builtin_amdgcn_mfma_f32_32x32x16_f16(fa, __builtin_bit_cast(fx_half8, ib), dre, 0, 0, 0);
+                dim = __builtin_amdgcn_mfma_f32_32x32x16_f16(fa, __builtin_bit_cast(fx_half8, ir), dim, 0, 0, 0);
+            }
+        }
+        if (++st == nst) {
+            const int c = c0 + 2 * p + half;
+            if (active && c < c0 + nc) {
+                const int j = tj * XTS + qj * 32 + r32;
+                const int ib0 = ti * XTS + qi * 32 + 4 * h;
+                pf32x2 FX_GLOBAL* out = outb + c;
+                const float dd = nts == 1 ? d : 1.f;
 #pragma unroll
-                    for (int r = 0; r < 16; r++) {
-                        const int i = ti * TSZ + (NBW * qi + a) * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-                        const int j = tj * TSZ + (NBW * qj + b) * 32 + (lane & 31);
-                        if (i <= j && j < I) {
-                            const long p = (long)i * I - (long)i * (i - 1) / 2 + (j - i);
-                            ((float2*)S.vis)[(size_t)p * C + c] = make_float2(dre[a][b][r] / d, dim[a][b][r] / d);
-                        }
+                for (int r = 0; r < 16; r++) {
+                    const int i = ib0 + 8 * (r >> 2) + (r & 3);
+                    if (i <= j && j < I) {
+                        const int pp = i * I - i * (i - 1) / 2 + (j - i);
+                        out[(size_t)pp * C] = pf32x2{dre[r] / dd, dim[r] / dd};
                     }
                 }
-        });
+            }
+            dre = fx_float16{};
+            dim = fx_float16{};
+            drain = true;
+            st = 0;
+            p++;
+        }
+    }
+}
+
+// vis[p][c] of products [p0, p0 + np): the slices' raw sums added in slice
+// order, the F stage's amplitude scale removed
+__device__ __noinline__ void red_item(const fx_psub* subs, int sb, long p0_, long np_)
+{
+    const PSubView FX_CONST& S = psub(subs, sb);
+    const long p0 = uni(p0_), np = uni(np_);
+    const long C = S.nfft / 2;
+    const long P = (long)S.ninp * (S.ninp + 1) / 2;
+    const float d = S.sc * S.sc;
+    const int nts = S.nts;
+    const pf32x2 FX_GLOBAL* const part = reinterpret_cast<const pf32x2 FX_GLOBAL*>(S.part);
+    pf32x2 FX_GLOBAL* const vis = reinterpret_cast<pf32x2 FX_GLOBAL*>(S.vis);
+    const size_t n = (size_t)np * C, base = (size_t)p0 * C, sstr = (size_t)P * C;
+    for (size_t e = threadIdx.x; e < n; e += PNT) {
+        float re = 0.f, im = 0.f;
+        for (int s = 0; s < nts; s++) {
+            const pf32x2 v = part[s * sstr + base + e];
+            re += v.x;
+            im += v.y;
+        }
+        vis[base + e] = pf32x2{re / d, im / d};
     }
 }
 
@@ -179,10 +472,14 @@ __device__ void x_item(const PSubView& S, long c0, long nc, int tile, unsigned c
 // the same and it counts codes per bin in registers (A = n(+H) - n(-H),
 // B = n(+1) - n(-1), bin sum H A + B); the FP64 sums are exact, so their
 // order does not matter.
-__device__ void pcal_item(const PSubView& S, int y, long w_first, long nwords, unsigned char* lds)
+__device__ __noinline__ void pcal_item(const fx_psub* subs, int sb, int y_, long w_first_, long nwords_,
+                                       unsigned char* lds)
 {
     double* lacc = reinterpret_cast<double*>(lds);
     const int tid = threadIdx.x;
+    const PSubView FX_CONST& S = psub(subs, sb);
+    const int y = uni(y_);
+    const long w_first = uni(w_first_), nwords = uni(nwords_);
     const int nfold = S.nfold;
     const int nb = nfold / 16;
     for (int b = tid; b < nfold; b += PNT) lacc[b] = 0.0;
@@ -219,10 +516,12 @@ __device__ void pcal_item(const PSubView& S, int y, long w_first, long nwords, u
 // tone t at DFT bin offset_bin + t*step of the nbins-point fold, the
 // rotator folded in for the Shifting extractor), partial sums reduced in a
 // fixed order
-__device__ void tone_item(const PSubView& S, int y, unsigned char* lds)
+__device__ __noinline__ void tone_item(const fx_psub* subs, int sb, int y_, unsigned char* lds)
 {
     double(*red)[PNT] = reinterpret_cast<double(*)[PNT]>(lds);
     const int tid = threadIdx.x;
+    const PSubView FX_CONST& S = psub(subs, sb);
+    const int y = uni(y_);
     const double FX_GLOBAL* acc = S.pacc + (size_t)y * (size_t)S.nfold;
     double FX_GLOBAL* tones = S.tones + (size_t)y * 2 * (size_t)S.ntones;
     for (int t = 0; t < S.ntones; t++) {
@@ -245,10 +544,10 @@ __device__ void tone_item(const PSubView& S, int y, unsigned char* lds)
         red[0][tid] = sr;
         red[1][tid] = si;
         __syncthreads();
-        for (int h = PNT / 2; h > 0; h >>= 1) {
-            if (tid < h) {
-                red[0][tid] += red[0][tid + h];
-                red[1][tid] += red[1][tid + h];
+        for (int hh = PNT / 2; hh > 0; hh >>= 1) {
+            if (tid < hh) {
+                red[0][tid] += red[0][tid + hh];
+                red[1][tid] += red[1][tid + hh];
             }
             __syncthreads();
         }
@@ -274,7 +573,7 @@ __device__ __forceinline__ void with_m(int nfft, Fn&& fn)
     }
 }
 
-__global__ __launch_bounds__(PNT, 2) void k_batch_persistent(const float2* __restrict__ twg,
+__global__ __launch_bounds__(PNT, 4) void k_batch_persistent(const float2* __restrict__ twg,
                                                           const fx_psub* __restrict__ subs,
                                                           const fx_pitem* __restrict__ items, int i0, int i1,
                                                           int* __restrict__ ctrs, int head, int wait)
@@ -285,15 +584,19 @@ __global__ __launch_bounds__(PNT, 2) void k_batch_persistent(const float2* __res
     for (;;) {
         if (tid == 0) s_it = i0 + atomicAdd(&ctrs[head], 1);
         __syncthreads();
-        const int k = s_it;
+        const int k = __builtin_amdgcn_readfirstlane(s_it);
         if (k >= i1) break;   // the queue is empty: every workgroup gets here
-        const fx_pitem item = items[k];
-        const PSubView& S = reinterpret_cast<const PSubView*>(subs)[item.sb];
-        if (wait && item.wait_ctr >= 0) {
+        const fx_pitem FX_CONST& item = reinterpret_cast<const fx_pitem FX_CONST*>((uintptr_t)items)[k];
+        const int type = item.type, sb = item.sb;
+        const long ia = item.a, ib = item.b;
+        const int ic = item.c, id = item.d;
+        const int wait_ctr = item.wait_ctr, wait_val = item.wait_val, done_ctr = item.done_ctr;
+        if (wait && wait_ctr >= 0) {
             if (tid == 0) {
+                // poll without ordering; one acquire once the count is reached
                 int ok = 1;
-                for (long spin = 0; __hip_atomic_load(&ctrs[item.wait_ctr], __ATOMIC_ACQUIRE,
-                                                      __HIP_MEMORY_SCOPE_AGENT) < item.wait_val;
+                for (long spin = 0;
+                     __hip_atomic_load(&ctrs[wait_ctr], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < wait_val;
                      spin++) {
                     if (spin >= PSPIN ||
                         __hip_atomic_load(&ctrs[FX_PCTR_ERR], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
@@ -303,63 +606,74 @@ __global__ __launch_bounds__(PNT, 2) void k_batch_persistent(const float2* __res
                     }
                     __builtin_amdgcn_s_sleep(16);
                 }
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
                 s_ok = ok;
             }
-            __syncthreads();
+            __syncthreads();       // the other waves read after thread 0's acquire
             if (!s_ok) continue;   // abandoned (the error counter is set)
-            __threadfence();       // every wave sees what the awaited items stored
         }
-        switch (item.type) {
+        const PSubView FX_CONST& S = reinterpret_cast<const PSubView FX_CONST*>((uintptr_t)subs)[sb];
+        switch (type) {
         case FX_PI_F:
-            with_m(S.nfft, [&](auto m) { f_item<decltype(m)::value>(twg, S, item.a, item.b, lds); });
+            with_m(S.nfft, [&](auto m) { f_item<decltype(m)::value>(twg, subs, sb, ia, ib, lds); });
             break;
-        case FX_PI_X:
-            x_item<64>(S, item.a, item.b, item.c, lds);
+        case FX_PI_CT:
+            ct_item(subs, sb, ia, ib, ic, lds);
+            break;
+        case FX_PI_X: {
+            int ti, tj;
+            tile_coords(ic, (S.ninp + XTS - 1) / XTS, ti, tj);
+            if (ti == tj)
+                x_item<true>(subs, sb, (int)ia, (int)ib, ti, tj, id, lds);
+            else
+                x_item<false>(subs, sb, (int)ia, (int)ib, ti, tj, id, lds);
+            break;
+        }
+        case FX_PI_RED:
+            red_item(subs, sb, ia, ib);
             break;
         case FX_PI_PCAL:
-            pcal_item(S, (int)item.a, item.b, item.c, lds);
+            pcal_item(subs, sb, (int)ia, ib, ic, lds);
             break;
         case FX_PI_TONE:
-            tone_item(S, (int)item.a, lds);
+            tone_item(subs, sb, (int)ia, lds);
             break;
         default:
             break;
         }
-        if (item.done_ctr >= 0) {
-            __threadfence();       // this wave's stores reach device scope
+        if (wait && done_ctr >= 0) {
+            // every wave's stores are done (the barrier's workgroup release);
+            // thread 0's agent-scope release makes them visible device-wide
             __syncthreads();
-            if (tid == 0) __hip_atomic_fetch_add(&ctrs[item.done_ctr], 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            if (tid == 0) __hip_atomic_fetch_add(&ctrs[done_ctr], 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
         }
         __syncthreads();           // LDS and s_it are free for the next item
     }
 }
 
 template <int M>
-size_t f_lds_of() { return f_lds_bytes<M>(); }
+size_t f_lds_of(bool pm) { return f_lds_bytes<M>(pm); }
 
 } // namespace
 
-int fx_persist_group_items(int nfft)
-{
-    const int M = nfft / 2;
-    return (M > PFM ? M : PFM) / M;
-}
+int fx_persist_group_items(int nfft) { return PFM / (nfft / 2); }
 
-size_t fx_persist_lds(int type, int arg)
+size_t fx_persist_lds(int type, int arg, bool pm)
 {
     switch (type) {
     case FX_PI_F:
         switch (arg) {
-        case 256: return f_lds_of<128>();
-        case 512: return f_lds_of<256>();
-        case 1024: return f_lds_of<512>();
-        case 2048: return f_lds_of<1024>();
-        case 4096: return f_lds_of<2048>();
-        case 8192: return f_lds_of<4096>();
-        case 16384: return f_lds_of<8192>();
+        case 256: return f_lds_of<128>(pm);
+        case 512: return f_lds_of<256>(pm);
+        case 1024: return f_lds_of<512>(pm);
+        case 2048: return f_lds_of<1024>(pm);
+        case 4096: return f_lds_of<2048>(pm);
+        case 8192: return f_lds_of<4096>(pm);
+        case 16384: return f_lds_of<8192>(pm);
         default: return 0;
         }
-    case FX_PI_X: return (size_t)2 * XTraits<FX_PREC_F16>::KT * (size_t)arg * sizeof(uint32_t);
+    case FX_PI_X: return arg ? x_lds_bytes<true>() : x_lds_bytes<false>();
+    case FX_PI_CT: return (size_t)64 * CTP * sizeof(uint32_t);
     case FX_PI_PCAL: return (size_t)arg * sizeof(double);
     case FX_PI_TONE: return (size_t)2 * PNT * sizeof(double);
     default: return 0;
@@ -399,7 +713,7 @@ int fxk_batch_persistent_blocks(size_t lds_bytes)
         // from the LDS alone: 160 KiB per CU
         occ = (int)std::max<size_t>(1, (160u * 1024u) / std::max<size_t>(lds_bytes + 64, 1));
     }
-    return occ * std::max(1, ncu);
+    return std::min(occ, 2) * std::max(1, ncu);
 }
 
 int fxk_batch_persistent(hipStream_t s, const float2* tw, const fx_psub* subs, const fx_pitem* items, int i0, int i1,

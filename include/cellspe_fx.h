@@ -411,22 +411,27 @@ typedef struct fx_batch fx_batch;
  *                   hipGraph with one concurrent branch per subband;
  *  FX_BATCH_PERSISTENT  (FP16) one persistent launch, in a hipGraph, walks a
  *                   work table of every subband: phase-cal folds, F groups
- *                   (unpack -> FFT -> rotate -> FP16 [c][t][i] over the
- *                   whole integration), X tiles over the whole integration
- *                   writing the visibilities, tone extractions; an X tile waits
- *                   in the kernel for its subband's F groups (items it waits for
- *                   come earlier in the table), a bounded wait that fails the
- *                   run (fx_batch_sync) if it ever expires;
- *  FX_BATCH_PHASED  (FP16) the same table as three launches (folds and F
- *                   groups, X tiles, tones), no waits.
+ *                   (unpack -> FFT -> rotate -> FP16, stored in the X-engine
+ *                   layout [c][t][i] over the whole integration, or as
+ *                   natural rows that corner-turn items re-tile), X items
+ *                   (channels of a 64 x 64 input tile over the integration or
+ *                   one time slice of it), slice reductions, tone
+ *                   extractions; an item waits in the kernel for the items
+ *                   whose output it reads (they come earlier in the table), a
+ *                   bounded wait that fails the run (fx_batch_sync) if it ever
+ *                   expires;
+ *  FX_BATCH_PHASED  (FP16) the same table as four launches (folds and F
+ *                   groups, corner turns, X items, reductions and tones), no
+ *                   waits.
  * The persistent modes keep each subband's channelized samples for its whole
- * integration in HBM (nfft/2 x nspectra x ninp x 4 bytes) and fold phase-cal
- * of up to 4096 bins (multiples of 16).  fx_batch_run replays the graph (or
- * issues the eager launches) with one call.  tune_file: autotuner cache (NULL:
- * defaults); entries missing from it are tuned on the spot and appended --
- * per-subband F sub-chunk depth and X tile for the first two modes
- * (fx_autotune), X tile, channels per X item and F groups per item for the
- * persistent modes ("P" lines). */
+ * integration in HBM (nfft/2 x nspectra x ninp x 4 bytes, and as much again
+ * for natural rows) and fold phase-cal of up to 4096 bins (multiples of 16).
+ * fx_batch_run replays the graph (or issues the eager launches) with one call.
+ * tune_file: autotuner cache (NULL: defaults); entries missing from it are
+ * tuned on the spot and appended -- per-subband F sub-chunk depth and X tile
+ * for the first two modes (fx_autotune); natural rows or direct stores, F
+ * groups per item, channels per X item and time slices for the persistent
+ * modes ("P" lines, timed on the subband's share of the device). */
 enum { FX_BATCH_EAGER = 0, FX_BATCH_GRAPH = 1, FX_BATCH_PERSISTENT = 2, FX_BATCH_PHASED = 3 };
 int fx_batch_create(fx_ctx* ctx, const fx_subband_desc* sb, int nsub, int prec, int use_graph,
                     const char* tune_file, fx_batch** batch);
@@ -438,9 +443,11 @@ const fx_cf32* fx_batch_vis(fx_batch* batch, int s);
 int fx_batch_pcal(fx_batch* batch, int s, int i, fx_cf64* tones, int* ntones);
 /* chosen tuning of subband s */
 int fx_batch_tuning(fx_batch* batch, int s, long* sub_spectra, int* xtile, long* chunk_spectra);
-/* persistent modes: X tile, channels per X item and F groups per item of
- * subband s; the work items, workgroups and dynamic LDS bytes of the launch */
-int fx_batch_persist_tuning(fx_batch* batch, int s, int* xtile, int* chan_per_item, int* groups_per_item);
+/* persistent modes: natural rows + corner turns (1) or direct stores (0),
+ * channels per X item, F groups per item and X time slices of subband s; the
+ * work items, workgroups and dynamic LDS bytes of the launch */
+int fx_batch_persist_tuning(fx_batch* batch, int s, int* natural_rows, int* chan_per_item, int* groups_per_item,
+                            int* time_slices);
 int fx_batch_persist_info(fx_batch* batch, int* nitems, int* nblocks, size_t* lds_bytes);
 int fx_batch_destroy(fx_batch* batch);
 
