@@ -752,7 +752,7 @@ def batched(fxlib, ctx, args):
             "subbands": [{"nfft": s["nfft"], "inputs": s["ninp"], "spectra": s["nspectra"],
                           "graph_sub_spectra": t[0], "graph_xtile": t[1], "persistent_natural_rows": q[0],
                           "persistent_channels_per_x_item": q[1], "persistent_f_groups_per_item": q[2],
-                          "persistent_time_slices": q[3]}
+                          "persistent_time_slices": q[3], "persistent_xtile": q[4]}
                          for s, t, q in zip(subs, tuning, ptuning)],
             "persistent_launch": {"items": nitems, "workgroups": nblocks, "lds_bytes": lds},
             "ms": {"f16_persistent": msp, "f16_phased": msph, "f16_graph": ms16, "f32_one_launch_each": ms32},

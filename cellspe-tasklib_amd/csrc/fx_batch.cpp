@@ -42,7 +42,10 @@ struct Subband {
     fx_phase_model* pm_dev = nullptr;
     int ipad = 0;
     long ntp = 0;
-    int pnat = 0, cb = 2, gpi = 4, nts = 1;
+    int pnat = 0, cb = 2, gpi = 4, nts = 1, xt = 64;
+    // tuned launch times of the subband's F, CT and X items (ms, as one
+    // subband among many): the fused table's interleaving weights
+    float f_ms = 1.f, ct_ms = 0.f, x_ms = 1.f;
 };
 
 long round8(long n) { return (n + 7) / 8 * 8; }
@@ -184,7 +187,7 @@ int build_table(fx_batch* b, bool fused, std::vector<fx_pitem>& items, int* phas
         it.wait_ctr = wc;
         it.wait_val = wv;
         it.done_ctr = dc;
-        it.pad = 0;
+        it.xt = 64;
         return it;
     };
     auto pcal_items = [&](int s) {
@@ -193,7 +196,7 @@ int build_table(fx_batch* b, bool fused, std::vector<fx_pitem>& items, int* phas
         fx_psub S{};
         fx_pcal_device_state(r.pcal, &S, nullptr);
         const long nbin = S.nfold / 16;
-        const long wpi = nbin * std::max<long>(1, 65536 / nbin);   // about 1M samples per fold item
+        const long wpi = nbin * std::max<long>(1, 262144 / nbin);   // about 4M samples per fold item
         const long nw = r.d.nspectra * r.d.nfft / 16;
         for (int y = 0; y < r.d.ninp; y++)
             for (long w0 = 0; w0 < nw; w0 += wpi) {
@@ -222,14 +225,18 @@ int build_table(fx_batch* b, bool fused, std::vector<fx_pitem>& items, int* phas
     };
     auto x_items = [&](int s) {
         const Subband& r = b->sb[(size_t)s];
-        const int nb = (r.d.ninp + 63) / 64;
+        const int nb = r.xt == 128 ? 1 : (r.d.ninp + 63) / 64;
         const long C = r.d.nfft / 2;
         const int wc = r.pnat ? ctr(s, 1) : ctr(s, 0);
         const int wv = r.pnat ? nC[(size_t)s] : nF[(size_t)s];
+        // X items are counted (a release each) only for the slice reductions
+        const int dc = r.nts > 1 ? ctr(s, 2) : -1;
         for (int sl = 0; sl < r.nts; sl++)
             for (int t = 0; t < nb * (nb + 1) / 2; t++)
                 for (long c0 = 0; c0 < C; c0 += r.cb) {
-                    items.push_back(item(FX_PI_X, s, c0, std::min<long>(r.cb, C - c0), t, sl, wc, wv, ctr(s, 2)));
+                    fx_pitem it = item(FX_PI_X, s, c0, std::min<long>(r.cb, C - c0), t, sl, wc, wv, dc);
+                    it.xt = r.xt;
+                    items.push_back(it);
                     nX[(size_t)s]++;
                 }
     };
@@ -256,15 +263,61 @@ int build_table(fx_batch* b, bool fused, std::vector<fx_pitem>& items, int* phas
     };
     std::stable_sort(order.begin(), order.end(), [&](int a, int c) { return work(a) > work(c); });
     items.clear();
-    for (int s : order) pcal_items(s);
     if (fused) {
+        // round k: the folds and F items of subband k, the corner turns of
+        // k - 1 and the X items of k - 2, merged so that every stream is the
+        // same fraction done at each point of the round (by the tuned times),
+        // so the workgroups run VALU-bound F work beside HBM-bound X work
         for (int k = 0; k < ns + 2; k++) {
-            if (k < ns) f_items(order[(size_t)k]);
-            if (k >= 1 && k <= ns) ct_items(order[(size_t)k - 1]);
-            if (k >= 2) x_items(order[(size_t)k - 2]);
+            std::vector<fx_pitem> st[4];
+            double cost[4] = {0, 0, 0, 0};
+            const size_t base = items.size();
+            if (k < ns) {
+                const int s = order[(size_t)k];
+                pcal_items(s);
+                st[0].assign(items.begin() + (long)base, items.end());
+                items.resize(base);
+                f_items(s);
+                st[1].assign(items.begin() + (long)base, items.end());
+                items.resize(base);
+                cost[0] = 0.1 * b->sb[(size_t)s].f_ms;
+                cost[1] = b->sb[(size_t)s].f_ms;
+            }
+            if (k >= 1 && k <= ns) {
+                const int s = order[(size_t)k - 1];
+                ct_items(s);
+                st[2].assign(items.begin() + (long)base, items.end());
+                items.resize(base);
+                cost[2] = b->sb[(size_t)s].ct_ms;
+            }
+            if (k >= 2) {
+                const int s = order[(size_t)k - 2];
+                x_items(s);
+                st[3].assign(items.begin() + (long)base, items.end());
+                items.resize(base);
+                cost[3] = b->sb[(size_t)s].x_ms;
+            }
+            size_t pos[4] = {0, 0, 0, 0};
+            for (;;) {
+                int pick = -1;
+                double best = 2.0;
+                for (int q = 0; q < 4; q++) {
+                    if (pos[q] >= st[q].size()) continue;
+                    // fraction of the stream's time already placed (streams
+                    // without a time estimate go first)
+                    const double frac = cost[q] > 0 ? (double)pos[q] / (double)st[q].size() : -1.0;
+                    if (frac < best) {
+                        best = frac;
+                        pick = q;
+                    }
+                }
+                if (pick < 0) break;
+                items.push_back(st[pick][pos[pick]++]);
+            }
         }
         phase_end[0] = phase_end[1] = phase_end[2] = (int)items.size();
     } else {
+        for (int s : order) pcal_items(s);
         for (int s : order) f_items(s);
         phase_end[0] = (int)items.size();
         for (int s : order) ct_items(s);
@@ -289,9 +342,8 @@ int build_table(fx_batch* b, bool fused, std::vector<fx_pitem>& items, int* phas
 }
 
 // the persistent run's launches on stream s: clear the counters and folds, then
-// one fused launch or four phased ones (blocks: workgroups, 0 for the batch's)
-int enqueue_persist(fx_batch* b, hipStream_t s, const fx_pitem* d_items, const int* phase_end, bool fused,
-                    int blocks)
+// one fused launch or four phased ones
+int enqueue_persist(fx_batch* b, hipStream_t s, const fx_pitem* d_items, const int* phase_end, bool fused)
 {
     if (fxk_fill32(s, b->d_ctrs, 0u, (size_t)b->nctrs) != FX_OK) return FX_ERR_HIP;
     for (const Subband& r : b->sb) {
@@ -301,7 +353,7 @@ int enqueue_persist(fx_batch* b, hipStream_t s, const fx_pitem* d_items, const i
         if (fxk_fill32(s, S.pacc, 0u, (size_t)S.nfold * r.d.ninp * 2) != FX_OK) return FX_ERR_HIP;
     }
     const float2* tw = fx_ctx_twiddles(b->ctx);
-    const int nb = blocks > 0 ? blocks : b->nblocks;
+    const int nb = b->nblocks;
     if (fused)
         return fxk_batch_persistent(s, tw, b->d_subs, d_items, 0, phase_end[3], b->d_ctrs, 0, nb, 1, b->lds);
     int i0 = 0;
@@ -313,7 +365,7 @@ int enqueue_persist(fx_batch* b, hipStream_t s, const fx_pitem* d_items, const i
     return FX_OK;
 }
 
-bool lookup_persist(const char* file, const fx_subband_desc& d, int blocks, Subband& r)
+bool lookup_persist(const char* file, const fx_subband_desc& d, int copies, Subband& r)
 {
     if (!file) return false;
     FILE* f = fopen(file, "r");
@@ -321,15 +373,20 @@ bool lookup_persist(const char* file, const fx_subband_desc& d, int blocks, Subb
     char line[256];
     bool found = false;
     while (fgets(line, sizeof(line), f)) {
-        int n, i, bl, na, c, g, ts;
+        int n, i, bl, na, c, g, ts, xt;
         long t;
-        double ms;
-        if (sscanf(line, "P %d %d %ld %d %d %d %d %d %lf", &n, &i, &t, &bl, &na, &c, &g, &ts, &ms) == 9 &&
-            n == d.nfft && i == d.ninp && t == d.nspectra && bl == blocks) {
+        float fms, cms, xms;
+        if (sscanf(line, "P %d %d %ld %d %d %d %d %d %d %f %f %f", &n, &i, &t, &bl, &na, &c, &g, &ts, &xt, &fms, &cms,
+                   &xms) == 12 &&
+            n == d.nfft && i == d.ninp && t == d.nspectra && bl == copies) {
             r.pnat = na;
             r.cb = c;
             r.gpi = g;
             r.nts = ts;
+            r.xt = xt;
+            r.f_ms = fms;
+            r.ct_ms = cms;
+            r.x_ms = xms;
             found = true;
         }
     }
@@ -346,6 +403,8 @@ void persist_defaults(Subband& r)
     r.cb = (int)std::max<long>(2, std::min<long>(d.nfft / 2, 2048 / std::max<long>(1, d.nspectra)) / 2 * 2);
     r.gpi = 4;
     r.nts = 1;
+    r.xt = d.ninp > 64 && d.ninp <= 128 ? 128 : 64;
+    if (r.xt == 128) r.cb = 1;
 }
 
 // table of one subband only (the autotuner's timing runs), phased
@@ -364,14 +423,22 @@ int build_table_one(fx_batch* b, int s, std::vector<fx_pitem>& items, int* phase
 }
 
 // Time launches [ph0, ph1] of subband s's phased table with its current
-// tuning on `blocks` workgroups, in ms (the second of two runs).
-int time_phases(fx_batch* b, int s, int ph0, int ph1, int blocks, float* ms)
+// tuning, in ms per copy (the second of two runs): every launch holds `rep`
+// copies of the subband's items on all the batch's workgroups, so the
+// subband runs on about its share of a loaded device (the copies write the
+// same values; the real runs clear and rewrite everything).
+int time_phases(fx_batch* b, int s, int ph0, int ph1, int rep, float* ms)
 {
-    std::vector<fx_pitem> items;
-    int pe[4];
-    int rc = build_table_one(b, s, items, pe);
+    std::vector<fx_pitem> one, items;
+    int pe1[4], pe[4];
+    int rc = build_table_one(b, s, one, pe1);
     if (!rc) rc = upload_psub(b, s);
     if (rc) return rc;
+    for (int ph = 0, i0 = 0; ph < 4; ph++) {
+        for (int r = 0; r < rep; r++) items.insert(items.end(), one.begin() + i0, one.begin() + pe1[ph]);
+        pe[ph] = (int)items.size();
+        i0 = pe1[ph];
+    }
     fx_pitem* d = nullptr;
     rc = fx_malloc(b->ctx, items.size() * sizeof(fx_pitem), (void**)&d);
     if (rc) return rc;
@@ -381,22 +448,23 @@ int time_phases(fx_batch* b, int s, int ph0, int ph1, int blocks, float* ms)
     if (!rc) rc = fx_event_create(b->ctx, &e1);
     hipStream_t st = fx_ctx_hip_stream(b->ctx);
     const float2* tw = fx_ctx_twiddles(b->ctx);
-    for (int rep = 0; rep < 2 && !rc; rep++) {
+    for (int run = 0; run < 2 && !rc; run++) {
         rc = fxk_fill32(st, b->d_ctrs, 0u, (size_t)b->nctrs);
         // the earlier launches once untimed (their outputs feed the timed ones)
         int i0 = 0;
         for (int ph = 0; ph < ph0 && !rc; ph++) {
-            rc = fxk_batch_persistent(st, tw, b->d_subs, d, i0, pe[ph], b->d_ctrs, ph, blocks, 0, b->lds);
+            rc = fxk_batch_persistent(st, tw, b->d_subs, d, i0, pe[ph], b->d_ctrs, ph, b->nblocks, 0, b->lds);
             i0 = pe[ph];
         }
         if (!rc) rc = fx_event_record(b->ctx, e0);
         for (int ph = ph0; ph <= ph1 && !rc; ph++) {
-            rc = fxk_batch_persistent(st, tw, b->d_subs, d, i0, pe[ph], b->d_ctrs, ph, blocks, 0, b->lds);
+            rc = fxk_batch_persistent(st, tw, b->d_subs, d, i0, pe[ph], b->d_ctrs, ph, b->nblocks, 0, b->lds);
             i0 = pe[ph];
         }
         if (!rc) rc = fx_event_record(b->ctx, e1);
         if (!rc) rc = fx_event_elapsed_ms(e0, e1, ms);
     }
+    *ms /= (float)rep;
     if (e0) fx_event_destroy(e0);
     if (e1) fx_event_destroy(e1);
     fx_ctx_sync(b->ctx);
@@ -405,24 +473,27 @@ int time_phases(fx_batch* b, int s, int ph0, int ph1, int blocks, float* ms)
 }
 
 // Autotuner of the persistent modes for subband s, timing that subband's
-// items alone on `blocks` workgroups (its share of the device in the batch):
+// items in `rep` concurrent copies (its share of the device in the batch):
 // natural rows + corner turns or direct stores (transforms from 2048 points)
-// and F groups per item from {1, 2, 4, 8, 16} on the F (+ CT) launches, then
-// channels per X item from {2, 4, ..., 64} on the X launch, then time slices
-// from {2, 4, 8, 16} on the X + reduction launches.  The choice is appended to
-// tune_file as "P nfft ninp nspectra blocks nat cb gpi nts ms".
-int autotune_persist(fx_batch* b, int s, int blocks, const char* tune_file)
+// and F groups per item from {4, 8, 16, 32} on the F (+ CT) launches, then
+// the X tile width (one 128-input tile for 65 to 128 inputs, or 64-input
+// tiles) and channels per X item from {1 or 2, ..., 64} on the X launch, then
+// time slices from {2, 4, 8, 16} on the X + reduction launches.  The choice
+// and the F, CT and X launch times are appended to tune_file as
+// "P nfft ninp nspectra copies nat cb gpi nts xt f_ms ct_ms x_ms".
+int autotune_persist(fx_batch* b, int s, int rep, const char* tune_file)
 {
     Subband& r = b->sb[(size_t)s];
     const fx_subband_desc& d = r.d;
-    float best = 1e30f, ms = 0.f, fms = 0.f;
+    float best = 1e30f, ms = 0.f;
+    int rc = FX_OK;
+    // F: natural rows + corner turns or direct stores, groups per item
     int best_n = r.pnat, best_g = r.gpi;
     for (int na = 0; na <= (r.nat ? 1 : 0); na++)
-        for (int g : {1, 2, 4, 8, 16}) {
+        for (int g : {4, 8, 16, 32}) {
             r.pnat = na;
             r.gpi = g;
-            int rc = time_phases(b, s, 0, 1, blocks, &ms);
-            if (rc) return rc;
+            if ((rc = time_phases(b, s, 0, 1, rep, &ms))) return rc;
             if (ms < best) {
                 best = ms;
                 best_n = na;
@@ -431,39 +502,49 @@ int autotune_persist(fx_batch* b, int s, int blocks, const char* tune_file)
         }
     r.pnat = best_n;
     r.gpi = best_g;
-    fms = best;
+    if ((rc = time_phases(b, s, 0, 0, rep, &r.f_ms))) return rc;
+    r.ct_ms = 0.f;
+    if (r.pnat && (rc = time_phases(b, s, 1, 1, rep, &r.ct_ms))) return rc;
+    // X: one 128-input tile or 64-input tiles (65 to 128 inputs), channels
+    // per item, then time slices
     best = 1e30f;
     r.nts = 1;
-    int best_c = r.cb;
+    int best_x = r.xt, best_c = r.cb;
     const long C = d.nfft / 2;
-    for (int cb = 2; cb <= 64 && cb <= C; cb *= 2) {
-        r.cb = cb;
-        int rc = time_phases(b, s, 2, 3, blocks, &ms);
-        if (rc) return rc;
-        if (ms < best) {
-            best = ms;
-            best_c = cb;
+    for (int xt : {64, 128}) {
+        if (xt == 128 && !(d.ninp > 64 && d.ninp <= 128)) continue;
+        for (int cb = xt == 128 ? 1 : 2; cb <= 64 && cb <= C; cb *= 2) {
+            r.xt = xt;
+            r.cb = cb;
+            if ((rc = time_phases(b, s, 2, 3, rep, &ms))) return rc;
+            if (ms < best) {
+                best = ms;
+                best_x = xt;
+                best_c = cb;
+            }
         }
     }
+    r.xt = best_x;
     r.cb = best_c;
-    int best_t = 1;
+    int best_t = 1, last = 1;
     for (int ts : {2, 4, 8, 16}) {
         const int eff = slices_of(r.ntp, ts);
-        if (eff <= best_t || eff > ts || part_bytes_of(d, eff) > r.part_bytes) continue;
+        if (eff <= last || part_bytes_of(d, eff) > r.part_bytes) continue;
+        last = eff;
         r.nts = eff;
-        int rc = time_phases(b, s, 2, 3, blocks, &ms);
-        if (rc) return rc;
+        if ((rc = time_phases(b, s, 2, 3, rep, &ms))) return rc;
         if (ms < best) {
             best = ms;
             best_t = eff;
         }
     }
     r.nts = best_t;
+    r.x_ms = best;
     if (tune_file) {
         FILE* f = fopen(tune_file, "a");
         if (f) {
-            fprintf(f, "P %d %d %ld %d %d %d %d %d %.4f\n", d.nfft, d.ninp, d.nspectra, blocks, r.pnat, r.cb, r.gpi,
-                    r.nts, fms + best);
+            fprintf(f, "P %d %d %ld %d %d %d %d %d %d %.4f %.4f %.4f\n", d.nfft, d.ninp, d.nspectra, rep, r.pnat, r.cb,
+                    r.gpi, r.nts, r.xt, r.f_ms, r.ct_ms, r.x_ms);
             fclose(f);
         }
     }
@@ -537,8 +618,8 @@ int batch_create_persist(fx_ctx* ctx, const fx_subband_desc* sb, int nsub, int p
         if (b->nblocks < 1) rc = FX_ERR_HIP;   // fx_last_error() says why
     }
     // each subband's share of the device: its part of the batch's work, at
-    // least one XCD's worth (an eighth), in whole XCD rounds
-    std::vector<int> tblocks((size_t)ns, 0);
+    // least an eighth; tuned as that many concurrent copies of it
+    std::vector<int> tcopies((size_t)ns, 1);
     if (!rc) {
         double tot = 0;
         std::vector<double> w((size_t)ns);
@@ -548,14 +629,14 @@ int batch_create_persist(fx_ctx* ctx, const fx_subband_desc* sb, int nsub, int p
             tot += w[(size_t)s];
         }
         for (int s = 0; s < ns; s++) {
-            const long share = std::lround(b->nblocks * w[(size_t)s] / std::max(tot, 1.0));
-            tblocks[(size_t)s] = (int)std::min<long>(b->nblocks, std::max<long>(b->nblocks / 8, round_up(share, 8)));
+            const double share = std::max(0.125, std::min(1.0, w[(size_t)s] / std::max(tot, 1.0)));
+            tcopies[(size_t)s] = std::max(1, (int)std::lround(1.0 / share));
         }
     }
     std::vector<bool> tuned((size_t)ns, false);
     for (int s = 0; s < ns && !rc; s++) {
         Subband& r = b->sb[(size_t)s];
-        if (tune_file) tuned[(size_t)s] = lookup_persist(tune_file, r.d, tblocks[(size_t)s], r);
+        if (tune_file) tuned[(size_t)s] = lookup_persist(tune_file, r.d, tcopies[(size_t)s], r);
         if (!r.nat) r.pnat = 0;
     }
     // tune each shape once (the tune file already holds the others); the
@@ -569,18 +650,22 @@ int batch_create_persist(fx_ctx* ctx, const fx_subband_desc* sb, int nsub, int p
                 pb = part_bytes_of(r.d, slices_of(r.ntp, ts));
         if (pb) rc = fx_malloc(ctx, pb, (void**)&r.part);
         r.part_bytes = rc ? 0 : pb;
-        if (!rc) rc = autotune_persist(b, s, tblocks[(size_t)s], tune_file);
+        if (!rc) rc = autotune_persist(b, s, tcopies[(size_t)s], tune_file);
         if (r.part) fx_free(ctx, r.part);
         r.part = nullptr;
         r.part_bytes = 0;
         for (int q = s + 1; q < ns; q++) {
             Subband& o = b->sb[(size_t)q];
             if (!tuned[(size_t)q] && o.d.nfft == r.d.nfft && o.d.ninp == r.d.ninp && o.d.nspectra == r.d.nspectra &&
-                tblocks[(size_t)q] == tblocks[(size_t)s]) {
+                tcopies[(size_t)q] == tcopies[(size_t)s]) {
                 o.pnat = r.pnat;
                 o.cb = r.cb;
                 o.gpi = r.gpi;
                 o.nts = r.nts;
+                o.xt = r.xt;
+                o.f_ms = r.f_ms;
+                o.ct_ms = r.ct_ms;
+                o.x_ms = r.x_ms;
                 tuned[(size_t)q] = true;
             }
         }
@@ -614,7 +699,7 @@ int batch_create_persist(fx_ctx* ctx, const fx_subband_desc* sb, int nsub, int p
         if (hipStreamBeginCapture(s0, hipStreamCaptureModeRelaxed) != hipSuccess) {
             rc = FX_ERR_HIP;
         } else {
-            int crc = enqueue_persist(b, s0, b->d_items, b->phase_end, mode == FX_BATCH_PERSISTENT, 0);
+            int crc = enqueue_persist(b, s0, b->d_items, b->phase_end, mode == FX_BATCH_PERSISTENT);
             hipGraph_t g = nullptr;
             const bool ended = hipStreamEndCapture(s0, &g) == hipSuccess;
             if (crc || !ended) {
@@ -864,7 +949,7 @@ extern "C" int fx_batch_sync(fx_batch* b)
 }
 
 extern "C" int fx_batch_persist_tuning(fx_batch* b, int s, int* natural_rows, int* chan_per_item,
-                                       int* groups_per_item, int* time_slices)
+                                       int* groups_per_item, int* time_slices, int* xtile)
 {
     FX_REQUIRE(b && s >= 0 && s < (int)b->sb.size(), FX_ERR_ARG, "fx_batch_persist_tuning: subband %d", s);
     FX_REQUIRE(b->mode >= FX_BATCH_PERSISTENT, FX_ERR_STATE, "fx_batch_persist_tuning: not a persistent batch");
@@ -873,6 +958,7 @@ extern "C" int fx_batch_persist_tuning(fx_batch* b, int s, int* natural_rows, in
     if (chan_per_item) *chan_per_item = r.cb;
     if (groups_per_item) *groups_per_item = r.gpi;
     if (time_slices) *time_slices = r.nts;
+    if (xtile) *xtile = r.xt;
     return FX_OK;
 }
 

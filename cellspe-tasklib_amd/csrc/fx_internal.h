@@ -506,7 +506,7 @@ struct fx_pitem {
     int wait_ctr;   // counter this item waits for (-1: none)
     int wait_val;   // value that counter must reach
     int done_ctr;   // counter incremented when the item is done (-1: none)
-    int pad;
+    int xt;         // X: tile width, 64 or 128 (one tile of up to 128 inputs)
 };
 // counters: [0..3] queue heads (one per launch of a phased run), [4] error
 // flag, then four per subband: F, CT, X and PCAL items done
@@ -515,7 +515,7 @@ struct fx_pitem {
 #define FX_PCTR_PER_SUB 4
 #define FX_PNT 512         // threads per workgroup
 #define FX_PXKT 32         // spectra per X stage (chan rows are padded to a multiple)
-#define FX_PCT_T 32        // spectra per corner-turn item
+#define FX_PCT_T 128       // spectra per corner-turn item
 // F groups of an nfft-point subband: 16384/nfft transforms per group
 int fx_persist_group_items(int nfft);
 // LDS bytes an item of this kind needs (F: nfft, with phase tables when

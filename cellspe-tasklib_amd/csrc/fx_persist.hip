@@ -442,6 +442,122 @@ __device__ __noinline__ void x_item(const fx_psub* subs, int sb, int c0_, int nc
     }
 }
 
+// X work of channels [c0, c0 + nc) of a subband of 65 to 128 inputs over time
+// slice `slice`, as one 128 x 128 tile: the workgroup streams one channel's
+// 128-input slabs through a ring of LDS stages (LDS-DMA loads, as x_item),
+// and its 8 waves share the tile's 10 upper 32 x 32 blocks (waves 0 and 1 take
+// two).  Reading each spectrum's inputs once instead of as the three 64-input
+// tiles' four slabs halves the X item's HBM traffic.
+__device__ __noinline__ void x_item128(const fx_psub* subs, int sb, int c0_, int nc_, int slice_, unsigned char* lds)
+{
+    constexpr int RING = 4;
+    constexpr int STAGE = XKT * 128 * 4;      // 16 KiB
+    constexpr int NPER = STAGE / 1024 / 8;    // 1 KiB loads per wave per stage
+    const int tid = threadIdx.x;
+    const int w = tid >> 6, lane = tid & 63;
+    const PSubView FX_CONST& S = psub(subs, sb);
+    const int c0 = uni(c0_), nc = uni(nc_), slice = uni(slice_);
+    const int I = S.ninp;
+    const int C = S.nfft / 2;
+    const int ipad = S.ipad;
+    const uint32_t FX_GLOBAL* const chan = S.chan;
+    const int nts = S.nts;
+    const float d = S.sc * S.sc;
+    pf32x2 FX_GLOBAL* const outb = reinterpret_cast<pf32x2 FX_GLOBAL*>(nts == 1 ? S.vis : S.part) +
+                                   (nts == 1 ? 0 : (size_t)slice * ((size_t)I * (I + 1) / 2) * C);
+    const int t0 = slice * (int)S.tslice;
+    const int t1 = t0 + (int)S.tslice < (int)S.ntp ? t0 + (int)S.tslice : (int)S.ntp;
+    const int nst = (t1 - t0) / XKT;   // >= 1: the host sizes the slices
+    const int G = nc * nst;
+    const size_t crow = (size_t)S.ntp * (size_t)ipad;
+    // blocks (a, b), a <= b, of the 4 x 4 grid: wave w takes block w and
+    // waves 0, 1 also blocks 8, 9
+    constexpr int BA[10] = {0, 0, 0, 0, 1, 1, 1, 2, 2, 3};
+    constexpr int BB[10] = {0, 1, 2, 3, 1, 2, 3, 2, 3, 3};
+    const int a0 = BA[w], b0 = BB[w];
+    const bool two = w < 2;
+    const int a1 = two ? BA[8 + w] : 0, b1 = two ? BB[8 + w] : 0;
+    // this lane's part of every 1 KiB load: spectrum row 2 blk + (lane >> 5),
+    // inputs (lane & 31) * 4 .. + 3
+    const int loff = (lane >> 5) * ipad + (lane & 31) * 4;
+    int ic = 0, ist = 0, nissued = 0;
+    auto issue = [&]() {
+        const uint32_t FX_GLOBAL* src = chan + (size_t)(c0 + ic) * crow + (size_t)(t0 + ist * XKT) * ipad + loff;
+        unsigned char* dst = lds + (nissued % RING) * STAGE;
+#pragma unroll
+        for (int k = 0; k < NPER; k++) {
+            const int blk = w * NPER + k;
+            __builtin_amdgcn_global_load_lds((FX_GLOBAL void*)(src + (2 * blk) * ipad),
+                                             (__attribute__((address_space(3))) void*)(dst + blk * 1024), 16, 0, 0);
+        }
+        nissued++;
+        if (++ist == nst) {
+            ist = 0;
+            ic++;
+        }
+    };
+    fx_float16 re0 = fx_float16{}, im0 = fx_float16{}, re1 = fx_float16{}, im1 = fx_float16{};
+    const int r32 = lane & 31, h = lane >> 5;
+    p_wait_vmcnt<0>();   // nothing of an earlier item is outstanding in the counts below
+    for (int g = 0; g < RING - 1 && g < G; g++) issue();
+    bool drain = false;
+    int c = c0, st = 0;
+    for (int g = 0; g < G; g++) {
+        const int ahead = G - 1 - g;
+        if (drain) p_wait_vmcnt<0>();
+        else if (ahead >= RING - 2) p_wait_vmcnt<(RING - 2) * NPER>();
+        else if (ahead == 1) p_wait_vmcnt<NPER>();
+        else p_wait_vmcnt<0>();
+        drain = false;
+        asm volatile("s_barrier" ::: "memory");
+        if (nissued < G) issue();
+        const uint32_t* sl = reinterpret_cast<const uint32_t*>(lds + (g % RING) * STAGE) + 4 * h * 128 + r32;
+        auto mac = [&](int a, int b, fx_float16& dre, fx_float16& dim) {
+#pragma unroll 2
+            for (int ks = 0; ks < XKT / 8; ks++) {
+                fx_int4v ia, ib, ir;
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    ia[q] = (int)sl[(ks * 8 + q) * 128 + a * 32];
+                    const uint32_t v = sl[(ks * 8 + q) * 128 + b * 32];
+                    ib[q] = (int)v;
+                    ir[q] = (int)rot_h2(v);
+                }
+                const fx_half8 fa = __builtin_bit_cast(fx_half8, ia);
+                dre = __builtin_amdgcn_mfma_f32_32x32x16_f16(fa, __builtin_bit_cast(fx_half8, ib), dre, 0, 0, 0);
+                dim = __builtin_amdgcn_mfma_f32_32x32x16_f16(fa, __builtin_bit_cast(fx_half8, ir), dim, 0, 0, 0);
+            }
+        };
+        mac(a0, b0, re0, im0);
+        if (two) mac(a1, b1, re1, im1);
+        if (++st == nst) {
+            const float dd = nts == 1 ? d : 1.f;
+            auto put = [&](int a, int b, const fx_float16& dre, const fx_float16& dim) {
+                // the lane re-derived (an empty asm): the 16 product offsets
+                // are computed here, not hoisted out of the stage loop and
+                // held (or spilled) across it
+                int ln = lane;
+                asm volatile("" : "+v"(ln));
+                const int j = b * 32 + (ln & 31);
+#pragma unroll
+                for (int r = 0; r < 16; r++) {
+                    const int i = a * 32 + 8 * (r >> 2) + 4 * (ln >> 5) + (r & 3);
+                    if (i <= j && j < I) {
+                        const int pp = i * I - i * (i - 1) / 2 + (j - i);
+                        outb[(size_t)pp * C + c] = pf32x2{dre[r] / dd, dim[r] / dd};
+                    }
+                }
+            };
+            put(a0, b0, re0, im0);
+            if (two) put(a1, b1, re1, im1);
+            re0 = im0 = re1 = im1 = fx_float16{};
+            drain = true;
+            st = 0;
+            c++;
+        }
+    }
+}
+
 // vis[p][c] of products [p0, p0 + np): the slices' raw sums added in slice
 // order, the F stage's amplitude scale removed
 __device__ __noinline__ void red_item(const fx_psub* subs, int sb, long p0_, long np_)
@@ -581,6 +697,10 @@ __global__ __launch_bounds__(PNT, 4) void k_batch_persistent(const float2* __res
     extern __shared__ __align__(16) unsigned char lds[];
     __shared__ int s_it, s_ok;
     const int tid = threadIdx.x;
+    // the last wait this workgroup completed: a later item waiting for the
+    // same counter and at most that count needs neither the polls nor another
+    // acquire (the data it reads was already visible after the first)
+    int seen_ctr = -1, seen_val = 0;
     for (;;) {
         if (tid == 0) s_it = i0 + atomicAdd(&ctrs[head], 1);
         __syncthreads();
@@ -591,7 +711,7 @@ __global__ __launch_bounds__(PNT, 4) void k_batch_persistent(const float2* __res
         const long ia = item.a, ib = item.b;
         const int ic = item.c, id = item.d;
         const int wait_ctr = item.wait_ctr, wait_val = item.wait_val, done_ctr = item.done_ctr;
-        if (wait && wait_ctr >= 0) {
+        if (wait && wait_ctr >= 0 && !(wait_ctr == seen_ctr && wait_val <= seen_val)) {
             if (tid == 0) {
                 // poll without ordering; one acquire once the count is reached
                 int ok = 1;
@@ -611,6 +731,8 @@ __global__ __launch_bounds__(PNT, 4) void k_batch_persistent(const float2* __res
             }
             __syncthreads();       // the other waves read after thread 0's acquire
             if (!s_ok) continue;   // abandoned (the error counter is set)
+            seen_ctr = wait_ctr;
+            seen_val = wait_val;
         }
         const PSubView FX_CONST& S = reinterpret_cast<const PSubView FX_CONST*>((uintptr_t)subs)[sb];
         switch (type) {
@@ -621,6 +743,10 @@ __global__ __launch_bounds__(PNT, 4) void k_batch_persistent(const float2* __res
             ct_item(subs, sb, ia, ib, ic, lds);
             break;
         case FX_PI_X: {
+            if (item.xt == 128) {
+                x_item128(subs, sb, (int)ia, (int)ib, id, lds);
+                break;
+            }
             int ti, tj;
             tile_coords(ic, (S.ninp + XTS - 1) / XTS, ti, tj);
             if (ti == tj)
@@ -672,7 +798,7 @@ size_t fx_persist_lds(int type, int arg, bool pm)
         case 16384: return f_lds_of<8192>(pm);
         default: return 0;
         }
-    case FX_PI_X: return arg ? x_lds_bytes<true>() : x_lds_bytes<false>();
+    case FX_PI_X: return arg ? x_lds_bytes<true>() : x_lds_bytes<false>();   // x_item128: 4 x 16 KiB, the same
     case FX_PI_CT: return (size_t)64 * CTP * sizeof(uint32_t);
     case FX_PI_PCAL: return (size_t)arg * sizeof(double);
     case FX_PI_TONE: return (size_t)2 * PNT * sizeof(double);

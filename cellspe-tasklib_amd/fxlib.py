@@ -153,7 +153,7 @@ def load():
         "fx_batch_pcal": (i, [vp, i, i, vp, C.POINTER(i)]),
         "fx_batch_tuning": (i, [vp, i, C.POINTER(l), C.POINTER(i), C.POINTER(l)]),
         "fx_batch_destroy": (i, [vp]),
-        "fx_batch_persist_tuning": (i, [vp, i, C.POINTER(i), C.POINTER(i), C.POINTER(i), C.POINTER(i)]),
+        "fx_batch_persist_tuning": (i, [vp, i, C.POINTER(i), C.POINTER(i), C.POINTER(i), C.POINTER(i), C.POINTER(i)]),
         "fx_batch_persist_info": (i, [vp, C.POINTER(i), C.POINTER(i), C.POINTER(sz)]),
         "fx_autotune": (i, [vp, i, i, i, l, C.c_char_p, C.POINTER(l), C.POINTER(i)]),
         "fx_corr_info": (i, [vp, C.POINTER(l), C.POINTER(sz), C.POINTER(sz)]),
@@ -608,10 +608,12 @@ class Batch:
 
     def persist_tuning(self, s):
         """(natural rows + corner turns, channels per X item, F groups per
-        item, X time slices) of subband s in the persistent modes."""
-        na, cb, g, ts = C.c_int(), C.c_int(), C.c_int(), C.c_int()
-        check(load().fx_batch_persist_tuning(self.handle, s, C.byref(na), C.byref(cb), C.byref(g), C.byref(ts)))
-        return na.value, cb.value, g.value, ts.value
+        item, X time slices, X tile width) of subband s in the persistent
+        modes."""
+        na, cb, g, ts, xt = C.c_int(), C.c_int(), C.c_int(), C.c_int(), C.c_int()
+        check(load().fx_batch_persist_tuning(self.handle, s, C.byref(na), C.byref(cb), C.byref(g), C.byref(ts),
+                                             C.byref(xt)))
+        return na.value, cb.value, g.value, ts.value, xt.value
 
     def persist_info(self):
         """(work items, workgroups, dynamic LDS bytes) of the persistent launch."""

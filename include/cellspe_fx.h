@@ -444,10 +444,11 @@ int fx_batch_pcal(fx_batch* batch, int s, int i, fx_cf64* tones, int* ntones);
 /* chosen tuning of subband s */
 int fx_batch_tuning(fx_batch* batch, int s, long* sub_spectra, int* xtile, long* chunk_spectra);
 /* persistent modes: natural rows + corner turns (1) or direct stores (0),
- * channels per X item, F groups per item and X time slices of subband s; the
- * work items, workgroups and dynamic LDS bytes of the launch */
+ * channels per X item, F groups per item, X time slices and X tile width (64,
+ * or 128: one tile of 65 to 128 inputs) of subband s; the work items,
+ * workgroups and dynamic LDS bytes of the launch */
 int fx_batch_persist_tuning(fx_batch* batch, int s, int* natural_rows, int* chan_per_item, int* groups_per_item,
-                            int* time_slices);
+                            int* time_slices, int* xtile);
 int fx_batch_persist_info(fx_batch* batch, int* nitems, int* nblocks, size_t* lds_bytes);
 int fx_batch_destroy(fx_batch* batch);
 
