@@ -78,6 +78,65 @@ __device__ __forceinline__ float lut2(uint32_t code)
     return (code & 2u) ? m : -m;
 }
 
+// Exact per-position code counts of packed 2-bit words, for the phase-cal
+// folds: the 16 codes of a word sit at positions j = 0..15 (bits 2j, 2j+1).
+// Byte b of the lane counters n3[s], n2[s], n1[s] counts codes 3, 2, 1 at
+// position 4b + s, so one 32-bit add counts four positions at once; every
+// 255 words flush() folds them into A[j] = n(+H) - n(-H) = 2 n3 + n2 + n1 - n
+// and B[j] = n(+1) - n(-1) = n2 - n1 (n: words counted).  A level-table sum
+// is then H A[j] + B[j], the same integers as counting code by code.
+struct Pcal16 {
+    int A[16], B[16];
+    uint32_t n3[4], n2[4], n1[4];
+    int batch;
+    __device__ __forceinline__ void init()
+    {
+#pragma unroll
+        for (int j = 0; j < 16; j++) A[j] = B[j] = 0;
+#pragma unroll
+        for (int s = 0; s < 4; s++) n3[s] = n2[s] = n1[s] = 0u;
+        batch = 0;
+    }
+    __device__ __forceinline__ void flush()
+    {
+#pragma unroll
+        for (int s = 0; s < 4; s++) {
+#pragma unroll
+            for (int b = 0; b < 4; b++) {
+                const int c3 = (int)((n3[s] >> (8 * b)) & 255u), c2 = (int)((n2[s] >> (8 * b)) & 255u),
+                          c1 = (int)((n1[s] >> (8 * b)) & 255u);
+                A[4 * b + s] += 2 * c3 + c2 + c1 - batch;
+                B[4 * b + s] += c2 - c1;
+            }
+            n3[s] = n2[s] = n1[s] = 0u;
+        }
+        batch = 0;
+    }
+    __device__ __forceinline__ void add(uint32_t v)
+    {
+#pragma unroll
+        for (int s = 0; s < 4; s++) {
+            const uint32_t x = (v >> (2 * s)) & 0x03030303u;
+            const uint32_t lo = x & 0x01010101u, hi = (x >> 1) & 0x01010101u, p3 = hi & lo;
+            n3[s] += p3;
+            n2[s] += hi ^ p3;
+            n1[s] += lo ^ p3;
+        }
+        if (++batch == 255) flush();
+    }
+    // the first `live` codes of a word only (the stream's last, partial word)
+    __device__ __forceinline__ void add_partial(uint32_t v, int live)
+    {
+#pragma unroll
+        for (int j = 0; j < 16; j++) {
+            const uint32_t c = (v >> (2 * j)) & 3u;
+            const int on = j < live;
+            A[j] += on * ((int)(c == 3u) - (int)(c == 0u));
+            B[j] += on * ((int)(c == 2u) - (int)(c == 1u));
+        }
+    }
+};
+
 // cos/sin(2 pi k/16), k = 0..7 (forward sign: -sin)
 __device__ constexpr float kW16re[8] = {1.0f, 0.9238795325f, 0.7071067812f, 0.3826834324f,
                                        0.0f, -0.3826834324f, -0.7071067812f, -0.9238795325f};

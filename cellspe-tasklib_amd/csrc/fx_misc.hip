@@ -228,8 +228,8 @@ __global__ __launch_bounds__(256) void k_pcal_fold(const uint32_t* __restrict__ 
 // nfold % 16 == 0: a word's 16 samples land in 16 consecutive bins, and with a
 // word stride T that is a multiple of nfold/16 every thread keeps the same 16
 // bins for all its words.  Each thread counts codes per bin in registers --
-// A = n(+H) - n(-H), B = n(+1) - n(-1), so the bin's sum is H A + B -- and
-// adds its 16 sums once.  The FP64 sums are the same exact values as the
+// A = n(+H) - n(-H), B = n(+1) - n(-1), so the bin's sum is H A + B, four
+// bins per 32-bit add in byte lanes (Pcal16) -- and adds its 16 sums once.  The FP64 sums are the same exact values as the
 // sample-by-sample fold (multiples of 2^-22 below 2^31).
 __global__ __launch_bounds__(256) void k_pcal_fold16(const uint32_t* __restrict__ words, size_t nsamp,
                                                      size_t position, int nfold, long T,
@@ -243,26 +243,24 @@ __global__ __launch_bounds__(256) void k_pcal_fold16(const uint32_t* __restrict_
     __syncthreads();
     const long tid = (long)blockIdx.x * blockDim.x + threadIdx.x;
     const size_t nw = (nsamp + 15) / 16;
-    int A[16], B[16];
-#pragma unroll
-    for (int j = 0; j < 16; j++) A[j] = B[j] = 0;
+    const size_t nfull = nsamp / 16;   // words whose 16 samples all count
+    Pcal16 cnt;
+    cnt.init();
     if (tid < T) {
         for (size_t w = (size_t)tid; w < nw; w += (size_t)T) {
             if (wvalid && !wvalid[w]) continue;     // a blanked word decodes as zeros
             const uint32_t v = words[w];
-            const size_t rem = nsamp - w * 16;      // >= 1
-#pragma unroll
-            for (int j = 0; j < 16; j++) {
-                const uint32_t c = (v >> (2 * j)) & 3u;
-                const int live = (size_t)j < rem;
-                A[j] += live * ((int)(c == 3u) - (int)(c == 0u));
-                B[j] += live * ((int)(c == 2u) - (int)(c == 1u));
-            }
+            if (w < nfull)
+                cnt.add(v);
+            else
+                cnt.add_partial(v, (int)(nsamp - w * 16));
         }
+        cnt.flush();
         int bin = (int)((position + 16 * (size_t)tid) % (size_t)nfold);
 #pragma unroll
         for (int j = 0; j < 16; j++) {
-            if (A[j] | B[j]) atomicAdd(&lacc[bin], (double)FX_LUT_HI * (double)A[j] + (double)B[j]);
+            if (cnt.A[j] | cnt.B[j])
+                atomicAdd(&lacc[bin], (double)FX_LUT_HI * (double)cnt.A[j] + (double)cnt.B[j]);
             if (++bin == nfold) bin = 0;
         }
     }

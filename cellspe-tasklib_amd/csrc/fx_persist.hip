@@ -207,36 +207,57 @@ __device__ __noinline__ void f_item(const float2* __restrict__ twg_, const fx_ps
                                                       pm, nullptr, nullptr);
         };
         if (!nat) {
-            // channel pairs (kp, M - kp), items fastest: neighbouring lanes
-            // store neighbouring inputs of one spectrum
-#pragma unroll 1
-            for (int q = tl; q < F * (M / 2); q += PNT) {
-                const int f = q % F;
-                const int kp = q / F;
-                if (it.t[f] < 0) continue;
-                float2 xk, xm;
-                rsplit2_tw<M>(sh, f * M, kp, tw_of(kp), xk, xm);
-                uint32_t FX_GLOBAL* col = chan + (size_t)it.t[f] * ipad + it.i[f];
-                col[(size_t)kp * crow] = fin(xk, f, kp);
-                if (kp) col[(size_t)(M - kp) * crow] = fin(xm, f, M - kp);   // kp = 0: M - kp is the dropped Nyquist bin
+            // Channel pairs (kp, M - kp), items fastest: neighbouring lanes
+            // store neighbouring inputs of one spectrum.  F divides PNT, so a
+            // thread keeps one item f and steps through its channels by
+            // STEP = PNT / F (8 pairs per thread for every M): the item's
+            // column, the store addresses and the split twiddle W_N^kp
+            // advance by constant steps (the twiddle by one complex multiply).
+            constexpr int STEP = PNT / F;
+            const int f = tl & (F - 1);
+            const int k0 = tl / F;
+            if (it.t[f] >= 0) {
+                const size_t col = (size_t)it.t[f] * ipad + it.i[f];
+                uint32_t FX_GLOBAL* pk = chan + col + (size_t)k0 * crow;
+                uint32_t FX_GLOBAL* pm2 = chan + col + (size_t)(M - k0) * crow;
+                const size_t dstep = (size_t)STEP * crow;
+                float2 w = tw_of(k0);
+                const float2 ws = tw_of(STEP);
+#pragma unroll 2
+                for (int kp = k0; kp < M / 2; kp += STEP) {
+                    float2 xk, xm;
+                    rsplit2_tw<M>(sh, f * M, kp, w, xk, xm);
+                    *pk = fin(xk, f, kp);
+                    if (kp) *pm2 = fin(xm, f, M - kp);   // kp = 0: M - kp is the dropped Nyquist bin
+                    w = cmul(w, ws);
+                    pk += dstep;
+                    pm2 -= dstep;
+                }
             }
             if (tl < F && it.t[tl] >= 0) {
                 float2 xk, xm;
                 rsplit2_tw<M>(sh, tl * M, M / 2, tw_of(M / 2), xk, xm);
                 chan[(size_t)(M / 2) * crow + (size_t)it.t[tl] * ipad + it.i[tl]] = fin(xk, tl, M / 2);
             }
-        } else {
-            // natural rows: neighbouring lanes store neighbouring channels
-#pragma unroll 1
-            for (int q = tl; q < F * (M / 2); q += PNT) {
-                const int f = q / (M / 2);
-                const int kp = q - f * (M / 2);
+        } else if constexpr (M >= 1024) {
+            // natural rows (the host sets them from 2048-point transforms):
+            // neighbouring lanes store neighbouring channels, KPT pairs per
+            // thread and item, the split twiddle advancing by W_N^PNT
+            constexpr int KPT = (M / 2) / PNT;
+            const float2 ws = tw_of(PNT);
+            for (int f = 0; f < F; f++) {
                 if (it.t[f] < 0) continue;
-                float2 xk, xm;
-                rsplit2_tw<M>(sh, f * M, kp, tw_of(kp), xk, xm);
                 uint32_t FX_GLOBAL* row = natr + ((size_t)it.t[f] * ipad + it.i[f]) * (size_t)M;
-                row[kp] = fin(xk, f, kp);
-                if (kp) row[M - kp] = fin(xm, f, M - kp);
+                float2 w = tw_of(tl);
+#pragma unroll
+                for (int kk = 0; kk < KPT; kk++) {
+                    const int kp = tl + kk * PNT;
+                    float2 xk, xm;
+                    rsplit2_tw<M>(sh, f * M, kp, w, xk, xm);
+                    row[kp] = fin(xk, f, kp);
+                    if (kp) row[M - kp] = fin(xm, f, M - kp);
+                    w = cmul(w, ws);
+                }
             }
             if (tl < F && it.t[tl] >= 0) {
                 float2 xk, xm;
@@ -585,9 +606,9 @@ __device__ __noinline__ void red_item(const fx_psub* subs, int sb, long p0_, lon
 // Phase-cal fold of words [w_first, w_first + nwords) of input y (w_first a
 // multiple of nfold/16): pacc[y][(16 w + j) mod nfold] += LUT(code).  A
 // thread takes the words of one residue w mod nfold/16, so its 16 bins stay
-// the same and it counts codes per bin in registers (A = n(+H) - n(-H),
-// B = n(+1) - n(-1), bin sum H A + B); the FP64 sums are exact, so their
-// order does not matter.
+// the same and it counts codes per bin in registers (Pcal16: four positions
+// per 32-bit add; bin sum H A + B); the FP64 sums are exact, so their order
+// does not matter.
 __device__ __noinline__ void pcal_item(const fx_psub* subs, int sb, int y_, long w_first_, long nwords_,
                                        unsigned char* lds)
 {
@@ -606,21 +627,14 @@ __device__ __noinline__ void pcal_item(const fx_psub* subs, int sb, int y_, long
     const int lanes = PNT / nb;   // threads per residue
     if (tid < lanes * nb) {
         const int r = tid % nb;
-        int A[16], B[16];
-#pragma unroll
-        for (int j = 0; j < 16; j++) A[j] = B[j] = 0;
-        for (long w = w_first + r + (long)nb * (tid / nb); w < w_end; w += (long)nb * lanes) {
-            const uint32_t v = words[w];
-#pragma unroll
-            for (int j = 0; j < 16; j++) {
-                const uint32_t c = (v >> (2 * j)) & 3u;
-                A[j] += (int)(c == 3u) - (int)(c == 0u);
-                B[j] += (int)(c == 2u) - (int)(c == 1u);
-            }
-        }
+        Pcal16 cnt;
+        cnt.init();
+        for (long w = w_first + r + (long)nb * (tid / nb); w < w_end; w += (long)nb * lanes) cnt.add(words[w]);
+        cnt.flush();
 #pragma unroll
         for (int j = 0; j < 16; j++)
-            if (A[j] | B[j]) atomicAdd(&lacc[16 * r + j], (double)FX_LUT_HI * (double)A[j] + (double)B[j]);
+            if (cnt.A[j] | cnt.B[j])
+                atomicAdd(&lacc[16 * r + j], (double)FX_LUT_HI * (double)cnt.A[j] + (double)cnt.B[j]);
     }
     __syncthreads();
     double FX_GLOBAL* acc = S.pacc + (size_t)y * (size_t)nfold;
