@@ -265,46 +265,47 @@ int build_table(fx_batch* b, bool fused, std::vector<fx_pitem>& items, int* phas
     items.clear();
     if (fused) {
         // round k: the folds and F items of subband k, the corner turns of
-        // k - 1 and the X items of k - 2, merged so that every stream is the
-        // same fraction done at each point of the round (by the tuned times),
-        // so the workgroups run VALU-bound F work beside HBM-bound X work
-        for (int k = 0; k < ns + 2; k++) {
-            std::vector<fx_pitem> st[4];
-            double cost[4] = {0, 0, 0, 0};
+        // k - 2, the X items of k - 2 (direct stores) or k - 4 (after corner
+        // turns): an item's inputs were made a whole round earlier, so it
+        // rarely waits.  The streams of a round are merged so that each is the
+        // same fraction done at every point of the round (by the tuned times),
+        // so the workgroups run VALU-bound F work beside HBM-bound X work.
+        auto xround = [&](int k) { return k + (b->sb[(size_t)order[(size_t)k]].pnat ? 4 : 2); };
+        for (int k = 0; k < ns + 4; k++) {
+            std::vector<fx_pitem> st[5];
+            double cost[5] = {0, 0, 0, 0, 0};
             const size_t base = items.size();
+            auto take = [&](int q, double c) {
+                st[q].assign(items.begin() + (long)base, items.end());
+                items.resize(base);
+                cost[q] = c;
+            };
             if (k < ns) {
                 const int s = order[(size_t)k];
                 pcal_items(s);
-                st[0].assign(items.begin() + (long)base, items.end());
-                items.resize(base);
+                take(0, 0.1 * b->sb[(size_t)s].f_ms);
                 f_items(s);
-                st[1].assign(items.begin() + (long)base, items.end());
-                items.resize(base);
-                cost[0] = 0.1 * b->sb[(size_t)s].f_ms;
-                cost[1] = b->sb[(size_t)s].f_ms;
+                take(1, b->sb[(size_t)s].f_ms);
             }
-            if (k >= 1 && k <= ns) {
-                const int s = order[(size_t)k - 1];
-                ct_items(s);
-                st[2].assign(items.begin() + (long)base, items.end());
-                items.resize(base);
-                cost[2] = b->sb[(size_t)s].ct_ms;
-            }
-            if (k >= 2) {
+            if (k >= 2 && k - 2 < ns) {
                 const int s = order[(size_t)k - 2];
-                x_items(s);
-                st[3].assign(items.begin() + (long)base, items.end());
-                items.resize(base);
-                cost[3] = b->sb[(size_t)s].x_ms;
+                ct_items(s);
+                take(2, b->sb[(size_t)s].ct_ms);
             }
-            size_t pos[4] = {0, 0, 0, 0};
+            for (int j = std::max(0, k - 4); j <= k - 2 && j < ns; j++) {
+                if (xround(j) != k) continue;
+                const int s = order[(size_t)j];
+                x_items(s);
+                take(j == k - 2 ? 3 : 4, b->sb[(size_t)s].x_ms);
+            }
+            size_t pos[5] = {0, 0, 0, 0, 0};
             for (;;) {
                 int pick = -1;
                 double best = 2.0;
-                for (int q = 0; q < 4; q++) {
+                for (int q = 0; q < 5; q++) {
                     if (pos[q] >= st[q].size()) continue;
-                    // fraction of the stream's time already placed (streams
-                    // without a time estimate go first)
+                    // fraction of the stream already placed (streams without
+                    // a time estimate go first)
                     const double frac = cost[q] > 0 ? (double)pos[q] / (double)st[q].size() : -1.0;
                     if (frac < best) {
                         best = frac;
