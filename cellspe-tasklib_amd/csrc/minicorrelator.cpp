@@ -47,11 +47,18 @@ int baseline_index(int i, int j, int nst)
 
 } // namespace
 
-extern "C" int minicorr_run(const control_block* cb, int nst, minicorr_stats* stats)
+extern "C" int minicorr_run(const control_block* cb_in, int nst, minicorr_stats* stats)
 {
-    FX_REQUIRE(cb != nullptr, -1, "minicorr_run: null control blocks");
+    FX_REQUIRE(cb_in != nullptr, -1, "minicorr_run: null control blocks");
     FX_REQUIRE(nst >= 2 && nst <= NUM_SPE_THREADS, -1, "minicorr_run: %d stations (2..%d)", nst,
                NUM_SPE_THREADS);
+    // the type is 128-byte aligned (the reference's SPU DMA needs it), but a
+    // host caller -- an FFI binding -- need not align the array: read the
+    // blocks as bytes into an aligned copy
+    std::vector<control_block> cbv((size_t)nst);
+    memcpy(cbv.data(), reinterpret_cast<const void*>(reinterpret_cast<uintptr_t>(cb_in)),
+           sizeof(control_block) * (size_t)nst);
+    const control_block* cb = cbv.data();
     const unsigned T = cb[0].ffts_total;
     const unsigned K = cb[0].ffts_to_integrate;
     FX_REQUIRE(K >= 1 && T >= K, -1, "minicorr_run: ffts_total=%u ffts_to_integrate=%u", T, K);

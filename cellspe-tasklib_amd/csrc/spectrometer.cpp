@@ -42,9 +42,15 @@ struct Dev {
 
 } // namespace
 
-extern "C" int spectrometer_cb_run(const control_block_t* cb, spectrometer_mbox_t* mbox)
+extern "C" int spectrometer_cb_run(const control_block_t* cb_in, spectrometer_mbox_t* mbox)
 {
-    FX_REQUIRE(cb && mbox, -1, "spectrometer_cb_run: null argument");
+    FX_REQUIRE(cb_in && mbox, -1, "spectrometer_cb_run: null argument");
+    // the type is 128-byte aligned (the reference's SPU DMA needs it), but a
+    // host caller -- an FFI binding -- need not align the block: read it as
+    // bytes into an aligned copy
+    control_block_t cbv;
+    memcpy(&cbv, reinterpret_cast<const void*>(reinterpret_cast<uintptr_t>(cb_in)), sizeof(cbv));
+    const control_block_t* cb = &cbv;
     memset(mbox, 0, sizeof(*mbox));
     if (cb->terminate_flag) return 0;
     const unsigned N = cb->fft_points;

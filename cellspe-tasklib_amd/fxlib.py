@@ -14,7 +14,8 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libcellspe_fx.so")
+# CELLSPE_FX_LIB names another build of the library (the sanitizer build)
+LIB_PATH = os.environ.get("CELLSPE_FX_LIB") or os.path.join(HERE, "libcellspe_fx.so")
 
 FX_OK = 0
 FX_PREC_F32, FX_PREC_F16, FX_PREC_FP8, FX_PREC_FP8_2BIT = 0, 1, 2, 3

@@ -245,6 +245,16 @@ def _p(a):
     return a.ctypes.data_as(C.c_void_p)
 
 
+def _aligned_structs(cls, n=1):
+    """n zeroed structures of cls at a 128-byte aligned address (the drop-in's
+    control blocks are aligned(128) C types); returns (array, buffer to keep
+    alive)."""
+    raw = C.create_string_buffer(C.sizeof(cls) * n + 128)
+    addr = (C.addressof(raw) + 127) & ~127
+    C.memset(addr, 0, C.sizeof(cls) * n)
+    return (cls * n).from_address(addr), raw
+
+
 class CellDSPTaskClass:
     """Task builder (cellspe-tasklib.h:175-222): the struct sits in a
     128-byte aligned slot; vectors are numpy arrays kept alive here."""
@@ -308,7 +318,7 @@ def minicorr_run(raw_stations, ffts_to_integrate, fmultipliers=None):
     bls = [np.zeros((max(ndump, 1), 2, N_FFT), np.float32) for _ in range(nbl)]
     raws = [np.ascontiguousarray(r, np.uint32) for r in raw_stations]
     fm = None if fmultipliers is None else np.ascontiguousarray(fmultipliers, np.float32)
-    cbs = (MiniControlBlock * nst)()
+    cbs, cbs_buf = _aligned_structs(MiniControlBlock, nst)
     for s in range(nst):
         cb = cbs[s]
         cb.rawdata_src.p = raws[s].ctypes.data
@@ -338,7 +348,8 @@ def spectrometer_cb_run(raw_bytes, fft_points, overlap_factor, integrate_ffts, t
     nfft = (ntime - N) // hop + 1 if ntime >= N else 0
     nspec = nfft // integrate_ffts
     dst = np.zeros((max(nspec, 1), N, 2), np.float32)
-    cb = SpecControlBlock()
+    cbs, cbs_buf = _aligned_structs(SpecControlBlock)
+    cb = cbs[0]
     cb.rank = 0
     cb.terminate_flag = int(terminate)
     cb.fft_points = N

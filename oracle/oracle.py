@@ -13,7 +13,8 @@ import subprocess
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "liboracle_fx.so")
+# FX_ORACLE_LIB names another build of the restatement (the sanitizer build)
+LIB_PATH = os.environ.get("FX_ORACLE_LIB") or os.path.join(HERE, "liboracle_fx.so")
 
 LUT = np.array([-3.3359, -1.0, 1.0, 3.3359], dtype=np.float32)
 WIN_NONE, WIN_SIN2, WIN_COSINE, WIN_COSINE2, WIN_HAMMING, WIN_BLACKMAN, WIN_HANN = range(7)
