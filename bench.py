@@ -233,11 +233,12 @@ def large_n(fxlib, ctx, args):
         return ms, v
 
     res = {}
-    c8 = fxlib.Correlator(ctx, I, 2 * C, T, fxlib.FX_PREC_FP8_2BIT)
-    ms, v = timed(c8, codes, args.steps)
-    c8.close()
-    res["fp8_2bit"], vis_sample["fp8_2bit"] = ms, v
-    log(f"[bench] large-N fp8 (exact 2-bit decomposition): {ms:.2f} ms")
+    for name, prec in (("fp4_2bit", fxlib.FX_PREC_FP4_2BIT), ("fp8_2bit", fxlib.FX_PREC_FP8_2BIT)):
+        c8 = fxlib.Correlator(ctx, I, 2 * C, T, prec)
+        ms, v = timed(c8, codes, args.steps)
+        c8.close()
+        res[name], vis_sample[name] = ms, v
+        log(f"[bench] large-N {name} (exact 2-bit decomposition): {ms:.2f} ms")
     for name, prec, esz, reps in (("f16", fxlib.FX_PREC_F16, 4, args.steps),
                                   ("f32", fxlib.FX_PREC_F32, 8, 1)):
         if name == "f32" and args.no_fp32:
@@ -263,8 +264,15 @@ def large_n(fxlib, ctx, args):
             x = lut[cb & 3] + 1j * lut[(cb >> 2) & 3]
             ref[:, k] = (x @ x.conj().T)[iu]
         err = {k: nrel(v, ref) for k, v in vis_sample.items()}
+        # the exact engines against the FP64 brute force rounded to FP32: units in the last place
+        r32 = ref.astype(np.complex64).view(np.int32).astype(np.int64)
+        ulp = {k: np.abs(vis_sample[k].view(np.int32).astype(np.int64) - r32) for k in ("fp4_2bit", "fp8_2bit")}
+        err["exact_vs_fp64_rounded"] = {k: {"max_ulp": int(u.max()), "equal_frac": float(np.mean(u == 0))}
+                                        for k, u in ulp.items()}
         err["sample"] = f"channels {sample_ch}, all {P} products, all {T} spectra, vs FP64"
-        log(f"[bench] large-N errors: " + ", ".join(f"{k} {v:.2e}" for k, v in err.items() if k != "sample"))
+        log(f"[bench] large-N errors: " + ", ".join(f"{k} {v:.2e}" for k, v in err.items()
+                                                    if k not in ("sample", "exact_vs_fp64_rounded"))
+            + f"; exact engines vs FP64 rounded to FP32: {err['exact_vs_fp64_rounded']}")
     if not args.no_cpu:
         # FP32 restatement of the X-engine on a sample of channels, all inputs and spectra
         O = args.oracle
@@ -290,8 +298,8 @@ def large_n(fxlib, ctx, args):
     out["speedup_vs_fp32"] = ({k: res["f32"] / v for k, v in res.items() if k != "f32"}
                               if "f32" in res else None)
     out["err_vs_fp64"] = err
-    out["note"] = ("fp8_2bit runs three e4m3 GEMMs (S, S+T, T of X = S + (H-1)T), so its MFMA work is "
-                   "3x the algorithmic count; tflops_equiv counts 8*P*C*T")
+    out["note"] = ("fp8_2bit / fp4_2bit run three e4m3 / e2m1 GEMMs (S, S+T, T of X = S + (H-1)T), so "
+                   "their MFMA work is 3x the algorithmic count; tflops_equiv counts 8*P*C*T")
     return out
 
 
@@ -791,6 +799,8 @@ def main():
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="instead of the benchmark, run the plain copy kernel over MIB MiB "
                          "(HBM bandwidth check and traffic-counter calibration)")
+    ap.add_argument("--large-only", action="store_true",
+                    help="instead of the benchmark, run only the large-N X-engine section")
     ap.add_argument("--batch-only", action="store_true",
                     help="instead of the benchmark, run only the batch of uneven subbands")
     ap.add_argument("--front-end-only", action="store_true",
@@ -818,6 +828,11 @@ def main():
 
     if args.copy_probe:
         copy_probe(fxlib, args)
+        return
+    if args.large_only:
+        ctx = fxlib.Context(0)
+        print(json.dumps({"large_n_xengine": large_n(fxlib, ctx, args)}), flush=True)
+        ctx.close()
         return
     if args.batch_only:
         ctx = fxlib.Context(0)

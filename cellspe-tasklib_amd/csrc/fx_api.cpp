@@ -840,12 +840,14 @@ struct fx_corr {
     bool sc_uniform = true;  // every channel scale equals sc_uni (the F kernel folds it per item)
     float sc_uni = 1.0f;
     int2* pij = nullptr;
-    float* coef = nullptr;  // FP8_2BIT: weights of the three accumulator sets
+    long long* coef = nullptr;  // FP8_2BIT / FP4_2BIT: integer coefficients of the exact combination
     int nsets = 1;
     int xtile = 128;        // X-engine input tile (128, or 64 for FP32/FP16 engines)
     bool ring = true;       // the LDS-ring X-engines may run (FX_XENGINE_AUTO)
     long spectra_done = 0;
 };
+
+static bool is_twobit(int prec) { return prec == FX_PREC_FP8_2BIT || prec == FX_PREC_FP4_2BIT; }
 
 // engine dispatch: FP32/FP16 read [c][t][i], the 8-bit engines [c][i][t]
 static int corr_engine(fx_corr* c, const void* d_chan, long nt)
@@ -863,6 +865,7 @@ static size_t prec_elem(int prec)
     case FX_PREC_F16: return 4;
     case FX_PREC_FP8: return 2;
     case FX_PREC_FP8_2BIT: return 1;
+    case FX_PREC_FP4_2BIT: return 1;
     default: return 0;
     }
 }
@@ -875,9 +878,9 @@ extern "C" int fx_corr_create(fx_ctx* ctx, const fx_corr_config* cfg, fx_corr** 
     FX_REQUIRE(cfg->ninp >= 1 && cfg->ninp <= 4096, FX_ERR_ARG, "fx_corr_create: ninp=%d", cfg->ninp);
     FX_REQUIRE(is_pow2(cfg->nfft) && cfg->nfft >= 256 && cfg->nfft <= 16384, FX_ERR_ARG,
                "fx_corr_create: nfft=%d not a power of two in [256,16384]", cfg->nfft);
-    FX_REQUIRE(cfg->prec >= FX_PREC_F32 && cfg->prec <= FX_PREC_FP8_2BIT, FX_ERR_UNSUPPORTED,
+    FX_REQUIRE(cfg->prec >= FX_PREC_F32 && cfg->prec <= FX_PREC_FP4_2BIT, FX_ERR_UNSUPPORTED,
                "fx_corr_create: precision %d", cfg->prec);
-    const bool twobit = cfg->prec == FX_PREC_FP8_2BIT;
+    const bool twobit = is_twobit(cfg->prec);
     FX_REQUIRE(cfg->chunk_spectra >= 8 && cfg->chunk_spectra % (twobit ? 16 : 8) == 0, FX_ERR_ARG,
                "fx_corr_create: chunk_spectra=%ld must be a positive multiple of %d", cfg->chunk_spectra,
                twobit ? 16 : 8);
@@ -917,10 +920,18 @@ extern "C" int fx_corr_create(fx_ctx* ctx, const fx_corr_config* cfg, fx_corr** 
     if (ok) ok = hipMemsetAsync(c->acc, 0, c->acc_floats * sizeof(float), ctx->stream) == hipSuccess;
     if (ok) ok = hipMemsetAsync(c->pm, 0, sizeof(fx_phase_model) * (size_t)cfg->ninp, ctx->stream) == hipSuccess;
     if (ok && twobit) {
-        // X X^H = (2-H) S S^H + (H-1) U U^H + (H-1)(H-2) T T^H
-        const double h = (double)FX_LUT_HI;
-        const float w[3] = {(float)(2.0 - h), (float)(h - 1.0), (float)((h - 1.0) * (h - 2.0))};
-        ok = hipMalloc(&c->coef, sizeof(w)) == hipSuccess &&
+        // X X^H = (2-H) S S^H + (H-1) U U^H + (H-1)(H-2) T T^H with H = h 2^-22
+        // (h an integer: the FP32 level value has 22 fraction bits): 2^44 X X^H
+        // = 2^22 [(2^23 - h) SS + (h - 2^22) UU] + (h - 2^22)(h - 2^23) TT
+        const double hd = ldexp((double)FX_LUT_HI, 22);
+        const long long h = (long long)hd;
+        if ((double)h != hd) {
+            fx_set_error("fx_corr_create: the level value is not a multiple of 2^-22");
+            ok = false;
+        }
+        const long long c2 = (h - (1LL << 22)) * (h - (1LL << 23));
+        const long long w[4] = {(1LL << 23) - h, h - (1LL << 22), c2 >> 22, c2 & ((1LL << 22) - 1)};
+        ok = ok && hipMalloc(&c->coef, sizeof(w)) == hipSuccess &&
              hipMemcpy(c->coef, w, sizeof(w), hipMemcpyHostToDevice) == hipSuccess;
     }
     if (ok && (cfg->prec == FX_PREC_F16 || cfg->prec == FX_PREC_FP8)) {
@@ -1045,7 +1056,7 @@ extern "C" int fx_corr_fstage(fx_corr* c, const uint32_t* d_words, size_t in_str
     FX_REQUIRE(t_first >= 0 && nspectra >= 0, FX_ERR_ARG, "fx_corr_fstage: negative range");
     FX_REQUIRE(nspectra <= c->cfg.chunk_spectra, FX_ERR_ARG,
                "fx_corr_fstage: %ld spectra exceed the %ld-spectrum chunk", nspectra, c->cfg.chunk_spectra);
-    FX_REQUIRE(c->nat != nullptr && out_prec != FX_PREC_FP8_2BIT, FX_ERR_UNSUPPORTED,
+    FX_REQUIRE(c->nat != nullptr && !is_twobit(out_prec), FX_ERR_UNSUPPORTED,
                "fx_corr_fstage: no F-stage for 2-bit channelized engines");
     FX_REQUIRE(prec_elem(out_prec) > 0 && prec_elem(out_prec) <= c->elem, FX_ERR_ARG,
                "fx_corr_fstage: output precision %d wider than the engine's", out_prec);
@@ -1091,7 +1102,7 @@ extern "C" int fx_corr_fstage_rows(fx_corr* c, const uint32_t* d_words, size_t i
     int rc = ctx_device(c->ctx);
     if (rc) return rc;
     FX_REQUIRE(c->nat != nullptr && prec_elem(out_prec) > 0 && prec_elem(out_prec) <= c->elem &&
-                   out_prec != FX_PREC_FP8_2BIT,
+                   !is_twobit(out_prec),
                FX_ERR_ARG, "fx_corr_fstage_rows: output precision %d", out_prec);
     FX_REQUIRE(in_stride_words % 2 == 0 && ((uintptr_t)d_words % 8) == 0 && t_first >= 0 && nspectra >= 0 &&
                    nspectra <= c->sub,
@@ -1119,7 +1130,7 @@ extern "C" int fx_corr_process(fx_corr* c, const uint32_t* d_words, size_t in_st
     FX_REQUIRE(c && d_words, FX_ERR_ARG, "fx_corr_process: null argument");
     int rc = ctx_device(c->ctx);
     if (rc) return rc;
-    FX_REQUIRE(c->cfg.prec != FX_PREC_FP8_2BIT, FX_ERR_UNSUPPORTED,
+    FX_REQUIRE(!is_twobit(c->cfg.prec), FX_ERR_UNSUPPORTED,
                "fx_corr_process: a 2-bit channelized engine takes fx_corr_xengine input");
     FX_REQUIRE(c->cfg.prec != FX_PREC_FP8 || nspectra % 8 == 0, FX_ERR_ARG,
                "fx_corr_process: the FP8 engine needs a multiple of 8 spectra");
@@ -1140,10 +1151,10 @@ extern "C" int fx_corr_xengine(fx_corr* c, const void* d_chan, long nspectra)
     int rc = ctx_device(c->ctx);
     if (rc) return rc;
     if (nspectra <= 0) return nspectra == 0 ? FX_OK : FX_ERR_ARG;
-    const int mult = c->cfg.prec == FX_PREC_FP8_2BIT ? 16 : c->cfg.prec == FX_PREC_FP8 ? 8 : 1;
+    const int mult = is_twobit(c->cfg.prec) ? 16 : c->cfg.prec == FX_PREC_FP8 ? 8 : 1;
     FX_REQUIRE(nspectra % mult == 0, FX_ERR_ARG, "fx_corr_xengine: %ld spectra, need a multiple of %d",
                nspectra, mult);
-    FX_REQUIRE(c->cfg.prec != FX_PREC_FP8_2BIT || c->spectra_done + nspectra <= (1L << 21), FX_ERR_ARG,
+    FX_REQUIRE(!is_twobit(c->cfg.prec) || c->spectra_done + nspectra <= (1L << 21), FX_ERR_ARG,
                "fx_corr_xengine: more than 2^21 spectra per dump leave the exact FP32 range");
     FX_CHECK_LAUNCH(corr_engine(c, d_chan, nspectra));
     c->spectra_done += nspectra;
@@ -1230,6 +1241,7 @@ extern "C" int fx_corr_xengine_name(fx_corr* c, const void* d_chan, long nspectr
     FX_REQUIRE(c && buf && len > 0, FX_ERR_ARG, "fx_corr_xengine_name: null argument");
     const char* name = c->cfg.prec == FX_PREC_FP8        ? "k_xengine8<1>"
                      : c->cfg.prec == FX_PREC_FP8_2BIT ? "k_xengine8<3>"
+                     : c->cfg.prec == FX_PREC_FP4_2BIT ? "k_xengine4"
                                                          : fxk_xengine_name(c->cfg.prec, c->xtile,
                                                                             d_chan ? d_chan : c->chan, nspectra,
                                                                             c->cfg.ninp, c->ring);

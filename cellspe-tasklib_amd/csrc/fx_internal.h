@@ -452,15 +452,18 @@ const char* fxk_xengine_name(int in_type, int ts, const void* X, long nt, int ni
 bool fxk_xengine_ring_default();
 // 8-bit X-engine on input-major channelized data X[c][i][t]: FX_PREC_FP8
 // (e4m3 re|im<<8 pairs, one accumulator set) or FX_PREC_FP8_2BIT (2-bit
-// codes re|im<<2, three accumulator sets of the exact decomposition)
+// codes re|im<<2, three accumulator sets of the exact decomposition) or
+// FX_PREC_FP4_2BIT (the same decomposition with e2m1 operands)
 int fxk_xengine8(hipStream_t s, int prec, const void* X, int nchan, long nt, int ninp, float* acc);
 // 2-bit channelized codes [c][t][i] or [c][i][t] -> [c][t][i] FP32/FP16 samples
 int fxk_codes_to_chan(hipStream_t s, const uint8_t* codes, int in_imajor, int nchan, long nt, int ninp,
                       int out_prec, void* out);
 // engine image size for ts x ts tiles (the 8-bit engines use ts = 128)
 size_t fxk_xengine_acc_floats(int nchan, int ninp, int nsets, int ts);
+// exact_coef: the four integer coefficients of the exact 2-bit combination
+// (three accumulator sets), or null for a plain sum of the sets
 int fxk_xengine_dump_pij(hipStream_t s, const float* acc, int nchan, int ninp, int ts, int nsets,
-                         const float* set_coef, const float* scale_chan, const float* scale_inp,
+                         const long long* exact_coef, const float* scale_chan, const float* scale_inp,
                          const int2* pij, int accumulate, float2* vis);
 int fxk_mfma_probe(hipStream_t s, int which, float* out);
 // phase-cal of ninp inputs in one launch: input y reads words + y*in_stride_words

@@ -392,7 +392,6 @@ __global__ __launch_bounds__(XNT, 2) void k_xengine_ring2(const typename XTraits
 // ---------------------------------------------------------------------------
 constexpr int X8NT = 512;                // threads (8 waves)
 constexpr int X8KT = 64;                 // samples per stage
-constexpr int X8RS = 2 * X8KT + 16;      // LDS row stride in bytes (padded)
 
 // e4m3 images indexed by the 2-bit code of {-H, -1, +1, +H} (byte c = code c)
 constexpr uint32_t kTblS = 0x3838B8B8u;  // S:  -1 -1 +1 +1
@@ -424,7 +423,8 @@ __global__ __launch_bounds__(X8NT, (NS == 1 ? 2 : 1)) void k_xengine8(const uint
     constexpr int RCH = TS * CPR;                 // chunks per row range
     constexpr int LPT = 2 * RCH / X8NT;           // chunks per thread (4 or 2)
     constexpr int SPC = 16 / BPS;                 // samples per chunk (8 or 16)
-    __shared__ __attribute__((aligned(16))) uint8_t op[2][NS][TS][X8RS];
+    constexpr int RS = X8KT * 2 + 16;             // LDS row stride in bytes (padded)
+    __shared__ __attribute__((aligned(16))) uint8_t op[2][NS][TS][RS];
 
     const int nb = (ninp + TS - 1) / TS;
     const int ntl = nb * (nb + 1) / 2;
@@ -548,12 +548,228 @@ __global__ __launch_bounds__(X8NT, (NS == 1 ? 2 : 1)) void k_xengine8(const uint
 }
 
 // ---------------------------------------------------------------------------
-// dump: engine layout -> vis[p][c] = sum_s coef[s] * acc_s / (sc_c^2 s_i s_j)
+// Exact 2-bit engine on FP4 MFMA: v_mfma_scale_f32_32x32x64_f8f6f4 with
+// cbsz = blgp = 4 (e2m1: A and B are 32 values = four VGPRs a lane, twice the
+// e4m3 rate) and unit scales.  The operands are the same three exact parts of
+// the 2-bit levels as the FP8 form -- S (sign), U = S + T, T (sign on the
+// outer levels, 0 inside), all of them e2m1 values (0, +-1, +-2) -- so every
+// product and every FP32 partial sum is the same exact integer.
+//
+// Operand layout, re-planar: a lane's 16 bytes are 16 complex samples, words
+// 0 and 1 their re parts as nibbles and words 2 and 3 their im parts in the
+// same sample order, so the rotated operand (im, -re) of Im V is words
+// (2, 3, 0 ^ sign, 1 ^ sign): two XORs a 16-sample operand.  Within a word
+// nibble 2k is sample k and nibble 2k+1 sample k+4 of an 8-sample group
+// (A and B are packed alike, so the K order is free).
+//
+// The code bits map to e2m1 nibbles bitwise, eight samples a 32-bit word:
+// code (hi, lo) = (0,0) -H, (0,1) -1, (1,0) +1, (1,1) +H; sign = !hi,
+// eq = (hi == lo) marks |x| = H; S = 0x2 | sign, U = S + (eq << 1) (0x4 = 2),
+// T = eq ? S : 0.
+//
+// Workgroup and tile as k_xengine8 (8 waves on one 128 x 128 tile of one
+// channel, wave w on rows 32 (w/2) and columns 64 (w%2), three accumulator
+// sets), with the stage loop double-buffered in LDS and one barrier a stage:
+// while the MFMAs read stage st from one buffer, the same waves expand stage
+// st+1 from registers into the other and fetch stage st+2 from HBM.
+// ---------------------------------------------------------------------------
+constexpr int X4KT = 64;                 // samples per stage
+constexpr int X4RS = X4KT + 16;          // LDS row stride in bytes: 64 samples x 2 nibbles + pad
+
+// 16 code bytes (re | im << 2 each) -> the re-planar e2m1 operands of S, U, T
+__device__ __forceinline__ void expand16_e2m1(uint4 w, uint4& vs, uint4& vu, uint4& vt)
+{
+    constexpr uint32_t N = 0x0F0F0F0Fu;
+    // P: byte k = code of sample k | code of sample k + 4 << 4 (upper code bits ignored)
+    const uint32_t P[2] = {(w.x & N) | ((w.y << 4) & ~N), (w.z & N) | ((w.w << 4) & ~N)};
+    uint32_t sr[2], ur[2], tr[2], si[2], ui[2], ti[2];
+#pragma unroll
+    for (int g = 0; g < 2; g++) {
+        const uint32_t p = P[g];
+        const uint32_t eq = ~(p ^ (p >> 1)) & 0x55555555u;       // bit 0: re lo == hi, bit 2: im lo == hi
+        const uint32_t er = eq & 0x11111111u, ei = (eq >> 2) & 0x11111111u;
+        sr[g] = ((~p & 0x22222222u) << 2) | 0x22222222u;         // sign = !re hi (bit 1 -> bit 3)
+        si[g] = (~p & 0x88888888u) | 0x22222222u;                // sign = !im hi (bit 3)
+        ur[g] = sr[g] + (er << 1);
+        ui[g] = si[g] + (ei << 1);
+        tr[g] = (er << 1) | ((er << 3) & sr[g]);
+        ti[g] = (ei << 1) | ((ei << 3) & si[g]);
+    }
+    vs = make_uint4(sr[0], sr[1], si[0], si[1]);
+    vu = make_uint4(ur[0], ur[1], ui[0], ui[1]);
+    vt = make_uint4(tr[0], tr[1], ti[0], ti[1]);
+}
+
+__global__ __launch_bounds__(X8NT, 1) void k_xengine4(const uint8_t* __restrict__ X, int nchan, long nt, int ninp,
+                                                      float* __restrict__ acc, size_t set_stride)
+{
+    __shared__ __attribute__((aligned(16))) uint8_t op[2][2][3][TS][X4RS];   // [buffer][slab][set][row][byte]
+
+    const int nb = (ninp + TS - 1) / TS;
+    const int ntl = nb * (nb + 1) / 2;
+    const long lb = xcd_block((long)ntl * nchan);
+    if (lb >= (long)ntl * nchan) return;
+    const int c = (int)(lb / ntl);
+    const int tile = (int)(lb - (long)c * ntl);
+    int ti, tj;
+    tile_coords(tile, nb, ti, tj);
+    const bool diag = ti == tj;
+    const int nslab = diag ? 1 : 2;
+    const int tid = threadIdx.x;
+    const int w = tid >> 6, lane = tid & 63;
+    const int qa = w >> 1, qb = w & 1;
+    // blocks (qa, 2 qb) and (qa, 2 qb + 1); in a diagonal tile the strictly lower ones are not stored,
+    // and a wave with both strictly lower idles.  A wave with one lower block still computes both:
+    // it shares its SIMD with an idle or a one-block wave, so the stage time is unchanged.
+    const int mode = __builtin_amdgcn_readfirstlane(!(diag && qa > 2 * qb) ? 2 : !(diag && qa > 2 * qb + 1) ? 1 : 0);
+
+    // staging: thread tid moves the 16-sample chunk (row tid / 4, part tid % 4) of each slab
+    const int row = tid >> 2, part = tid & 3;
+    const uint8_t* src[2];
+    bool rok[2];
+#pragma unroll
+    for (int s = 0; s < 2; s++) {
+        const int r = (s == 0 ? ti : tj) * TS + row;
+        rok[s] = s < nslab && r < ninp;
+        src[s] = X + ((size_t)c * (size_t)ninp + (size_t)(rok[s] ? r : 0)) * (size_t)nt + part * 16;
+    }
+
+    fx_float16 are[3][2], aim[3][2];
+#pragma unroll
+    for (int set = 0; set < 3; set++)
+#pragma unroll
+        for (int b = 0; b < 2; b++) {
+            are[set][b] = fx_float16{};
+            aim[set][b] = fx_float16{};
+        }
+
+    uint4 pre[2];
+    bool val[2];
+    auto gload = [&](long t0) {
+#pragma unroll
+        for (int s = 0; s < 2; s++) {
+            val[s] = rok[s] && t0 + part * 16 < nt;   // nt is a multiple of 16: chunks are whole
+            pre[s] = val[s] ? *reinterpret_cast<const uint4*>(src[s] + t0) : make_uint4(0u, 0u, 0u, 0u);
+        }
+    };
+    auto lstore = [&](int buf) {
+#pragma unroll
+        for (int s = 0; s < 2; s++) {
+            if (s >= nslab) break;
+            uint4* dst[3] = {reinterpret_cast<uint4*>(&op[buf][s][0][row][part * 16]),
+                             reinterpret_cast<uint4*>(&op[buf][s][1][row][part * 16]),
+                             reinterpret_cast<uint4*>(&op[buf][s][2][row][part * 16])};
+            if (val[s]) {
+                uint4 vs, vu, vt;
+                expand16_e2m1(pre[s], vs, vu, vt);
+                *dst[0] = vs;
+                *dst[1] = vu;
+                *dst[2] = vt;
+            } else {
+                // rows past the last input and samples past the end contribute zeros
+                *dst[0] = *dst[1] = *dst[2] = make_uint4(0u, 0u, 0u, 0u);
+            }
+        }
+    };
+
+    const int sb = diag ? 0 : 1;
+    const int r32 = lane & 31, h = lane >> 5;
+    // the MFMAs of one stage
+    auto stage = [&](int buf) {
+#pragma unroll
+        for (int ks = 0; ks < X4KT / 32; ks++) {
+            const int kb = ks * 32 + h * 16;      // byte offset of this lane's 16 samples
+#pragma unroll
+            for (int set = 0; set < 3; set++) {
+                const uint4 a = *reinterpret_cast<const uint4*>(&op[buf][0][set][qa * 32 + r32][kb]);
+                const fx_int8 fa = {(int)a.x, (int)a.y, (int)a.z, (int)a.w, 0, 0, 0, 0};
+                const fx_int8 fr = {(int)a.z, (int)a.w, (int)(a.x ^ 0x88888888u), (int)(a.y ^ 0x88888888u), 0, 0, 0, 0};
+#pragma unroll
+                for (int b = 0; b < 2; b++) {
+                    const uint4 v =
+                        *reinterpret_cast<const uint4*>(&op[buf][sb][set][(2 * qb + b) * 32 + r32][kb]);
+                    const fx_int8 fb = {(int)v.x, (int)v.y, (int)v.z, (int)v.w, 0, 0, 0, 0};
+                    are[set][b] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(fa, fb, are[set][b], 4, 4, 0, 127, 0,
+                                                                                  127);
+                    aim[set][b] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(fr, fb, aim[set][b], 4, 4, 0, 127, 0,
+                                                                                  127);
+                }
+            }
+        }
+    };
+
+    const long nstage = (nt + X4KT - 1) / X4KT;
+    gload(0);
+    lstore(0);
+    if (nstage > 1) gload(X4KT);
+    __syncthreads();
+    for (long st = 0; st < nstage; st++) {
+        const int buf = (int)(st & 1);
+        if (mode) stage(buf);
+        if (st + 1 < nstage) {
+            // the other buffer was last read in stage st - 1, before the barrier that ended it
+            lstore(buf ^ 1);
+            if (st + 2 < nstage) gload((st + 2) * X4KT);
+        }
+        __syncthreads();
+    }
+    if (mode == 0) return;
+#pragma unroll
+    for (int b = 0; b < 2; b++) {
+        if (b == 0 && mode == 1) continue;
+        const int bi = qa, bj = 2 * qb + b;
+        const int w4 = (bi >> 1) * 2 + (bj >> 1), s4 = (bi & 1) * 2 + (bj & 1);
+#pragma unroll
+        for (int set = 0; set < 3; set++) {
+            float* d = acc + set * set_stride + (((size_t)c * ntl + tile) * 16 + w4 * 4 + s4) * (32 * 64);
+#pragma unroll
+            for (int r = 0; r < 16; r++) {
+                d[r * 64 + lane] += are[set][b][r];
+                d[(16 + r) * 64 + lane] += aim[set][b][r];
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// dump: engine layout -> vis[p][c] = sum_s acc_s / (sc_c^2 s_i s_j), or for
+// the exact 2-bit decomposition (three sets of exact integer sums A0, A1, A2)
+// the visibility (2-H) A0 + (H-1) A1 + (H-1)(H-2) A2 rounded once to FP32.
+// With H = h 2^-22 (h an integer: the FP32 level value) it is
+//   K 2^-22 + L 2^-44,  K = e0 A0 + e1 A1 + e2 A2,  L = e3 A2,
+// e = {2^23 - h, h - 2^22, hi and lo parts (c2 = e2 2^22 + e3) of
+// (h - 2^22)(h - 2^23)}: K and L are exact 64-bit integers and exact doubles,
+// their two-sum is the exact value as an unevaluated pair, and the FP32
+// rounding of the pair is corrected where the high part sits on a midpoint.
+__device__ __forceinline__ float round_pair_f32(double hi, double lo)
+{
+    float f = (float)hi;
+    const double fd = (double)f;
+    if (lo != 0.0 && fd != hi) {
+        // g: the float on the other side of hi; a pair on the midpoint of f
+        // and g goes to the side lo points to
+        const float g = nextafterf(f, hi > fd ? INFINITY : -INFINITY);
+        if (hi == 0.5 * (fd + (double)g) && ((lo > 0.0) == ((double)g > fd))) f = g;
+    }
+    return f;
+}
+
+__device__ __forceinline__ float exact2bit(const long long* __restrict__ e, float a0, float a1, float a2)
+{
+    const long long A0 = (long long)a0, A1 = (long long)a1, A2 = (long long)a2;
+    const long long K = e[0] * A0 + e[1] * A1 + e[2] * A2;
+    const long long L = e[3] * A2;
+    const double x = ldexp((double)K, -22), y = ldexp((double)L, -44);
+    const double s = x + y;                      // two-sum: s + err = x + y exactly
+    const double bb = s - x;
+    const double err = (x - (s - bb)) + (y - bb);
+    return round_pair_f32(s, err);
+}
 // MFMA 32x32 output layout: lane l, register r holds row 8(r/4) + 4(l/32) + r%4,
 // column l%32.
 // ---------------------------------------------------------------------------
 __global__ void k_xengine_dump(const float* __restrict__ acc, int nchan, int ninp, int ts, int nsets,
-                               size_t set_stride, const float* __restrict__ coef,
+                               size_t set_stride, const long long* __restrict__ ecoef,
                                const float* __restrict__ sc_chan, const float* __restrict__ sc_inp,
                                const int2* __restrict__ pij, long nprod, int accumulate,
                                float2* __restrict__ vis)
@@ -578,11 +794,18 @@ __global__ void k_xengine_dump(const float* __restrict__ acc, int nchan, int nin
     const int l = jj + 32 * ((ii >> 2) & 1);
     const size_t off = (((size_t)c * ntl + tile) * (size_t)((ts / 32) * (ts / 32)) + blk) * (32 * 64);
     float re = 0.f, im = 0.f;
-    for (int q = 0; q < nsets; q++) {
-        const float* a = acc + q * set_stride + off;
-        float k = coef ? coef[q] : 1.0f;
-        re += k * a[r * 64 + l];
-        im += k * a[(16 + r) * 64 + l];
+    if (ecoef) {
+        const float* a0 = acc + off;
+        const float* a1 = a0 + set_stride;
+        const float* a2 = a1 + set_stride;
+        re = exact2bit(ecoef, a0[r * 64 + l], a1[r * 64 + l], a2[r * 64 + l]);
+        im = exact2bit(ecoef, a0[(16 + r) * 64 + l], a1[(16 + r) * 64 + l], a2[(16 + r) * 64 + l]);
+    } else {
+        for (int q = 0; q < nsets; q++) {
+            const float* a = acc + q * set_stride + off;
+            re += a[r * 64 + l];
+            im += a[(16 + r) * 64 + l];
+        }
     }
     if (sc_chan) {
         float d = sc_chan[c] * sc_chan[c] * sc_inp[i] * sc_inp[j];
@@ -721,6 +944,9 @@ int fxk_xengine8(hipStream_t s, int prec, const void* X, int nchan, long nt, int
     } else if (prec == FX_PREC_FP8_2BIT) {
         if (nt % 16) return FX_ERR_ARG;
         hipLaunchKernelGGL(k_xengine8<3>, dim3((unsigned)nblk), dim3(X8NT), 0, s, x, nchan, nt, ninp, acc, stride);
+    } else if (prec == FX_PREC_FP4_2BIT) {
+        if (nt % 16) return FX_ERR_ARG;
+        hipLaunchKernelGGL(k_xengine4, dim3((unsigned)nblk), dim3(X8NT), 0, s, x, nchan, nt, ninp, acc, stride);
     } else {
         return FX_ERR_UNSUPPORTED;
     }
@@ -728,14 +954,14 @@ int fxk_xengine8(hipStream_t s, int prec, const void* X, int nchan, long nt, int
 }
 
 int fxk_xengine_dump_pij(hipStream_t s, const float* acc, int nchan, int ninp, int ts, int nsets,
-                         const float* set_coef, const float* scale_chan, const float* scale_inp,
+                         const long long* exact_coef, const float* scale_chan, const float* scale_inp,
                          const int2* pij, int accumulate, float2* vis)
 {
     long nprod = (long)ninp * (ninp + 1) / 2;
     long total = nprod * nchan;
     size_t set_stride = fxk_xengine_acc_floats(nchan, ninp, 1, ts);
     hipLaunchKernelGGL(k_xengine_dump, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, acc,
-                       nchan, ninp, ts, nsets, set_stride, set_coef, scale_chan, scale_inp, pij, nprod,
+                       nchan, ninp, ts, nsets, set_stride, exact_coef, scale_chan, scale_inp, pij, nprod,
                        accumulate, vis);
     return hipGetLastError() == hipSuccess ? FX_OK : FX_ERR_HIP;
 }

@@ -18,7 +18,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.environ.get("CELLSPE_FX_LIB") or os.path.join(HERE, "libcellspe_fx.so")
 
 FX_OK = 0
-FX_PREC_F32, FX_PREC_F16, FX_PREC_FP8, FX_PREC_FP8_2BIT = 0, 1, 2, 3
+FX_PREC_F32, FX_PREC_F16, FX_PREC_FP8, FX_PREC_FP8_2BIT, FX_PREC_FP4_2BIT = 0, 1, 2, 3, 4
 FX_XENGINE_AUTO, FX_XENGINE_PREFETCH = 0, 1
 FX_BATCH_EAGER, FX_BATCH_GRAPH, FX_BATCH_PERSISTENT, FX_BATCH_PHASED = 0, 1, 2, 3
 (FX_WIN_NONE, FX_WIN_SIN2, FX_WIN_COSINE, FX_WIN_COSINE2, FX_WIN_HAMMING, FX_WIN_BLACKMAN,

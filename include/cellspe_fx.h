@@ -50,8 +50,9 @@ typedef struct {
 enum {
     FX_PREC_F32 = 0,     /* plain FP32 MFMA (v_mfma_f32_32x32x2_f32)              */
     FX_PREC_F16 = 1,     /* FP16 in, FP32 accumulate (v_mfma_f32_32x32x16_f16)     */
-    FX_PREC_FP8 = 2,     /* FP8-e4m3 in, FP32 accumulate, per-channel/input scales */
-    FX_PREC_FP8_2BIT = 3 /* exact FP8 decomposition of 2-bit {+-1,+-H} voltages    */
+    FX_PREC_FP8 = 2,      /* FP8-e4m3 in, FP32 accumulate, per-channel/input scales */
+    FX_PREC_FP8_2BIT = 3, /* exact FP8 decomposition of 2-bit {+-1,+-H} voltages    */
+    FX_PREC_FP4_2BIT = 4  /* the same exact decomposition on FP4-e2m1 MFMA          */
 };
 
 /* recorded-data frame formats of the front end */
@@ -341,8 +342,12 @@ int fx_corr_process(fx_corr* corr, const uint32_t* d_words, size_t in_stride_wor
  *                      and T the signs of the |x| = H samples, XX^H =
  *                      (2-H) SS^H + (H-1) UU^H + (H-1)(H-2) TT^H, U = S + T,
  *                      three FP8 GEMMs on values in {0, +-1, +-2} with exact
- *                      FP32 sums (up to 2^21 spectra per dump) combined in
- *                      FP32 by the dump. */
+ *                      FP32 sums (up to 2^21 spectra per dump) combined
+ *                      exactly by the dump (64-bit integer coefficients) and
+ *                      rounded once to FP32 -- the exact sums to nearest-even;
+ *   FX_PREC_FP4_2BIT:  the same input and decomposition with the operands in
+ *                      FP4-e2m1 (0, +-1, +-2 are e2m1 values): the same
+ *                      integer sums, the same visibilities. */
 int fx_corr_xengine(fx_corr* corr, const void* d_chan, long nspectra);
 /* write visibilities [p][c] complex64, p = i*I - i(i-1)/2 + (j-i), i <= j */
 int fx_corr_dump(fx_corr* corr, fx_cf32* d_vis);

@@ -272,6 +272,67 @@ def xengine_f64(X):
     return vis
 
 
+def round_int_f32(n, e):
+    """n * 2^e for an integer n of any size, rounded to the nearest FP32 with
+    ties to even (normal range; test-only)."""
+    if n == 0:
+        return 0.0
+    a, sgn = abs(n), -1 if n < 0 else 1
+    sh = a.bit_length() - 24
+    if sh > 0:
+        q, rem = a >> sh, a & ((1 << sh) - 1)
+        half = 1 << (sh - 1)
+        if rem > half or (rem == half and (q & 1)):
+            q += 1
+        a, e = q, e + sh
+    return sgn * float(a) * 2.0 ** e
+
+
+def xengine_2bit_exact_f32(codes):
+    """Visibilities of 2-bit channelized codes (codes[c][t][i], byte re | im << 2,
+    levels LUT) as the exact sums rounded once to FP32.  Returns [p][c]
+    complex64 (p over i <= j): the exact counterpart of xengine_f64 on the
+    level values (test-only).
+
+    Each level times 2^22 is an integer, 2^22 s + d m with s its sign, m = s
+    on the outer levels (|x| = H = h 2^-22, FP32 H has 22 fraction bits) and
+    0 on the inner ones, and d = h - 2^22.  A product sum is then
+    2^-44 (2^44 SS + 2^22 d (SM + MS) + d^2 MM) with SS, SM, MS, MM integer
+    sums of +-1 and 0 (FP64 matrix products, exact below 2^53), combined in
+    int64 (|value| < 2^62 for up to 8192 spectra) and rounded once by the
+    int64 -> FP32 conversion (round to nearest, ties to even)."""
+    codes = np.asarray(codes, np.uint8)
+    C, T, I = codes.shape
+    assert T <= 8192, "int64 combination holds up to 8192 spectra"
+    h = int(np.ldexp(np.float64(LUT[3]), 22))
+    assert float(h) == np.ldexp(np.float64(LUT[3]), 22) and LUT[2] == 1.0
+    d = np.int64(h - (1 << 22))
+    iu = np.triu_indices(I)
+    out = np.empty((len(iu[0]), C), np.complex64)
+    scale = np.float32(2.0 ** -44)
+
+    def parts(c):
+        s = np.where(c >= 2, 1.0, -1.0)
+        return s, np.where((c == 0) | (c == 3), s, 0.0)
+
+    def combine(ss, sm, ms, mm):
+        # all four are integer-valued FP64 arrays
+        n = (ss.astype(np.int64) << 44) + (sm + ms).astype(np.int64) * d * (1 << 22) + mm.astype(np.int64) * (d * d)
+        return n.astype(np.float32) * scale
+
+    for c in range(C):
+        sr, mr = parts((codes[c] & 3).T)           # [i][t]
+        si, mi = parts(((codes[c] >> 2) & 3).T)
+        A = np.concatenate([sr, si], 1)           # re part first, then im along t
+        B = np.concatenate([mr, mi], 1)
+        Ax = np.concatenate([si, -sr], 1)         # (im, -re): x_i conj(x_j) imaginary part
+        Bx = np.concatenate([mi, -mr], 1)
+        re = combine(A @ A.T, A @ B.T, B @ A.T, B @ B.T)
+        im = combine(Ax @ A.T, Ax @ B.T, Bx @ A.T, Bx @ B.T)
+        out[:, c] = re[iu] + 1j * im[iu]
+    return out
+
+
 def _word_ptrs(streams):
     arrs = [np.ascontiguousarray(s, np.uint32) for s in streams]
     ptrs = (C.c_void_p * len(arrs))(*[a.ctypes.data for a in arrs])

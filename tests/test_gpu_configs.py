@@ -14,9 +14,11 @@
   oracle, FP16 against the FP64 oracle (<= 1e-3);
 * large-N shape: 512 inputs, 8192 spectra of 2-bit channelized voltages
   (128 channels here), exact FP8 engine and FP16 engine against FP64 sums on
-  sampled channels; and all 1024 channels of the large-N workload on a
-  64-spectrum time slice through the exact FP8, FP16 and plain FP32 engines,
-  every product and channel against FP64 sums (FP32 also against the FP32
+  sampled channels (the exact engine bit for bit against the oracle's exact
+  integer sums rounded once); and all 1024 channels of the large-N workload
+  on a 64-spectrum time slice through the exact FP8 and FP4, FP16 and plain
+  FP32 engines, every product and channel against FP64 sums and the exact
+  engines bit for bit against the exact sums (FP32 also against the FP32
   restatement on sampled channels).
 """
 import numpy as np
@@ -155,6 +157,9 @@ def test_large_n_shape(ctx):
         e8, e16 = nrel(v8[:, ch], ref), nrel(v16[:, ch], ref)
         print(f"large-N channel {ch}: fp8 exact {e8:.2e}, f16 {e16:.2e}")
         assert e8 < 1e-6
+        # 8192-spectrum sums: the exact integer sums rounded once, bit for bit
+        exact = O.xengine_2bit_exact_f32(cb.T[None])[:, 0]
+        assert np.array_equal(np.ascontiguousarray(v8[:, ch]).view(np.uint32), exact.view(np.uint32))
         assert e16 < 1e-3
 
 
@@ -172,11 +177,12 @@ def test_large_n_all_channels_time_slice(ctx):
         ref[:, c] = (x @ x.conj().T)[iu]
     vis = ctx.alloc(P * C * 8)
     got = {}
-    c8 = fxlib.Correlator(ctx, I, 2 * C, T, fxlib.FX_PREC_FP8_2BIT)
-    c8.xengine(codes, T)
-    c8.dump(vis)
-    c8.close()
-    got["fp8_2bit"] = vis.download(np.complex64, shape=(P, C))
+    for name, prec in (("fp8_2bit", fxlib.FX_PREC_FP8_2BIT), ("fp4_2bit", fxlib.FX_PREC_FP4_2BIT)):
+        c8 = fxlib.Correlator(ctx, I, 2 * C, T, prec)
+        c8.xengine(codes, T)
+        c8.dump(vis)
+        c8.close()
+        got[name] = vis.download(np.complex64, shape=(P, C))
     for name, prec, esz in (("f16", fxlib.FX_PREC_F16, 4), ("f32", fxlib.FX_PREC_F32, 8)):
         dx = ctx.alloc(C * T * I * esz)
         ctx.codes_to_chan(codes, True, C, T, I, prec, dx)
@@ -193,6 +199,10 @@ def test_large_n_all_channels_time_slice(ctx):
     err = {k: nrel(v, ref) for k, v in got.items()}
     print("large-N, all 1024 channels x 64 spectra vs FP64: " + ", ".join(f"{k} {v:.2e}" for k, v in err.items()))
     assert err["fp8_2bit"] < 1e-6
+    # the exact engines: the exact sums rounded once to FP32, bit for bit, on every channel
+    exact = O.xengine_2bit_exact_f32(cb.transpose(0, 2, 1))
+    assert np.array_equal(got["fp8_2bit"].view(np.uint32), exact.view(np.uint32))
+    assert np.array_equal(got["fp4_2bit"].view(np.uint32), exact.view(np.uint32))
     assert err["f32"] < TOL32
     assert err["f16"] < 1e-3
     # the FP32 engine against the FP32 restatement of the reference MAC

@@ -1,8 +1,11 @@
 """GPU parity of the 8-bit X-engines (v_mfma_scale_f32_32x32x64_f8f6f4).
 
-* FX_PREC_FP8_2BIT: 2-bit channelized voltages through the exact
-  decomposition X = S + (H-1) T; visibilities must match the FP64 oracle of
-  the level-table values to FP32 rounding of the final combination (<= 1e-6).
+* FX_PREC_FP8_2BIT / FX_PREC_FP4_2BIT: 2-bit channelized voltages through the
+  exact decomposition of the level products into three integer sums on e4m3
+  or e2m1 operands; the dump combines the sums exactly and rounds once, so the
+  visibilities equal the oracle's exact sums rounded to FP32
+  (xengine_2bit_exact_f32) bit for bit, the two operand forms are
+  bit-identical, and both are within one ulp of the FP64 brute force.
 * FX_PREC_FP8: e4m3 inputs; the engine must reproduce the FP64 sums of the
   decoded e4m3 values (products are exact, FP32 accumulation, <= 1e-6), and
   the full F->X chain in FP8 is reported against the FP64 oracle.
@@ -51,24 +54,33 @@ def test_synth_channelized_layouts(ctx, nchan, nt, ninp):
 
 
 @pytest.mark.parametrize("ninp,nt", [(40, 80), (200, 64), (256, 48)])
-def test_fp8_2bit_exact(ctx, ninp, nt):
+def test_2bit_exact(ctx, ninp, nt):
     nchan = 128
     codes = O.gen_channelized_codes(7 + ninp, nchan, nt, ninp, 0.05, 0.9816)   # [c][t][i]
     d = ctx.to_device(np.ascontiguousarray(codes.transpose(0, 2, 1)))
-    corr = fxlib.Correlator(ctx, ninp, 2 * nchan, 64, fxlib.FX_PREC_FP8_2BIT)
-    corr.xengine(d, nt)
-    got = corr.visibilities()
-    corr.close()
+    got = {}
+    for prec in (fxlib.FX_PREC_FP8_2BIT, fxlib.FX_PREC_FP4_2BIT):
+        corr = fxlib.Correlator(ctx, ninp, 2 * nchan, 64, prec)
+        corr.xengine(d, nt)
+        got[prec] = corr.visibilities()
+        corr.close()
     ref = O.xengine_f64(codes_to_values(codes).astype(np.complex64))
-    err = nrel(got, ref)
-    print(f"FP8_2BIT ninp={ninp} nt={nt}: normwise error vs FP64 {err:.2e}")
-    assert err < 1e-6
+    g8, g4 = got[fxlib.FX_PREC_FP8_2BIT], got[fxlib.FX_PREC_FP4_2BIT]
+    assert np.array_equal(g8.view(np.uint32), g4.view(np.uint32))
+    exact = O.xengine_2bit_exact_f32(codes)
+    assert np.array_equal(g4.view(np.uint32), exact.view(np.uint32))
+    r32 = ref.astype(np.complex64)
+    ulp = np.abs(g4.view(np.int32).astype(np.int64) - r32.view(np.int32).astype(np.int64))
+    print(f"2-bit ninp={ninp} nt={nt}: normwise error vs FP64 {nrel(g4, ref):.2e}, "
+          f"{100 * np.mean(ulp == 0):.4f}% of components equal to the FP64 brute force rounded to FP32")
+    assert ulp.max() <= 1
 
 
-def test_fp8_2bit_accumulates_over_calls(ctx):
+@pytest.mark.parametrize("prec", [fxlib.FX_PREC_FP8_2BIT, fxlib.FX_PREC_FP4_2BIT])
+def test_2bit_accumulates_over_calls(ctx, prec):
     nchan, ninp, nt = 128, 64, 32
     codes = O.gen_channelized_codes(3, nchan, 2 * nt, ninp, 0.1, 0.9816)
-    corr = fxlib.Correlator(ctx, ninp, 2 * nchan, 64, fxlib.FX_PREC_FP8_2BIT)
+    corr = fxlib.Correlator(ctx, ninp, 2 * nchan, 64, prec)
     for h in range(2):
         part = np.ascontiguousarray(codes[:, h * nt:(h + 1) * nt, :].transpose(0, 2, 1))
         corr.xengine(ctx.to_device(part), nt)
@@ -76,8 +88,7 @@ def test_fp8_2bit_accumulates_over_calls(ctx):
     corr.reset()
     assert np.all(corr.visibilities() == 0)
     corr.close()
-    ref = O.xengine_f64(codes_to_values(codes).astype(np.complex64))
-    assert nrel(got, ref) < 1e-6
+    assert np.array_equal(got.view(np.uint32), O.xengine_2bit_exact_f32(codes).view(np.uint32))
 
 
 @pytest.mark.parametrize("ninp,nt", [(48, 64), (160, 40)])

@@ -487,3 +487,68 @@ def test_threaded_and_channelized_xengines():
     for nth in (1, 4):
         got = O.xcorr_tic_f32(X.transpose(1, 2, 0), nth)
         assert np.abs(got - ref).max() < 1e-5 * np.abs(ref).max()
+
+
+def test_round_int_f32():
+    """The integer-to-FP32 rounding of the exact 2-bit reference: numpy's
+    FP64 -> FP32 conversion (round to nearest, ties to even) on integers FP64
+    holds exactly, and the tie cases by hand (both neighbours, odd and even)."""
+    rng = np.random.default_rng(3)
+    for n in list(rng.integers(-(1 << 52), 1 << 52, 2000)) + [1, -1, (1 << 24) + 1, (1 << 25) + 3]:
+        n = int(n)
+        for e in (-44, -22, 0):
+            assert O.round_int_f32(n, e) == float(np.float32(np.ldexp(np.float64(n), e)))
+    # ties: 2^24 + 1 lies half way between 2^24 and 2^24 + 2 (even mantissa below),
+    # 2^24 + 3 half way between 2^24 + 2 and 2^24 + 4 (even mantissa above)
+    assert O.round_int_f32((1 << 24) + 1, 0) == float(1 << 24)
+    assert O.round_int_f32((1 << 24) + 3, 0) == float((1 << 24) + 4)
+    assert O.round_int_f32(-((1 << 24) + 3), 0) == -float((1 << 24) + 4)
+    # beyond FP64: one more unit below the tie point rounds down, at or above up
+    big = (1 << 80) + (1 << 56)            # exactly half an FP32 ulp above 2^80 (ulp 2^57)
+    assert O.round_int_f32(big, -44) == 2.0 ** 36
+    assert O.round_int_f32(big + 1, -44) == (2.0 ** 80 + 2.0 ** 57) * 2.0 ** -44
+
+
+def _direct_exact_2bit(codes):
+    """Brute force: the level values times 2^22 as Python integers, products
+    summed without rounding, each sum rounded once by round_int_f32."""
+    C, T, I = codes.shape
+    lv = {k: int(np.ldexp(np.float64(O.LUT[k]), 22)) for k in range(4)}
+    out = np.empty((I * (I + 1) // 2, C), np.complex64)
+    for c in range(C):
+        re = [[lv[int(codes[c, t, i]) & 3] for t in range(T)] for i in range(I)]
+        im = [[lv[(int(codes[c, t, i]) >> 2) & 3] for t in range(T)] for i in range(I)]
+        p = 0
+        for i in range(I):
+            for j in range(i, I):
+                vr = sum(re[i][t] * re[j][t] + im[i][t] * im[j][t] for t in range(T))
+                vi = sum(im[i][t] * re[j][t] - re[i][t] * im[j][t] for t in range(T))
+                out[p, c] = O.round_int_f32(vr, -44) + 1j * O.round_int_f32(vi, -44)
+                p += 1
+    return out
+
+
+def test_exact_2bit_reference():
+    """xengine_2bit_exact_f32: one input and one spectrum gives |x|^2 = H^2 + 1
+    (the FP32 level H squared exactly, then rounded once); on random codes it
+    equals the integer brute force bit for bit (including 8192-spectrum sums
+    whose exact values need more than FP64's 53 bits), agrees with the FP64 sums of the level
+    values rounded to FP32 to within one unit in the last place, and gives
+    real, positive autocorrelations."""
+    Hf = np.float64(np.float32(H))
+    one = np.array([[[3 | (1 << 2)]]], np.uint8)          # re = +H, im = -1
+    v = O.xengine_2bit_exact_f32(one)
+    assert v.shape == (1, 1) and v[0, 0] == np.complex64(np.float32(Hf * Hf + 1.0))
+    codes = O.gen_channelized_codes(7, 3, 200, 9, 0.05, 0.9816)
+    ex = O.xengine_2bit_exact_f32(codes)
+    assert np.array_equal(ex.view(np.uint32), _direct_exact_2bit(codes).view(np.uint32))
+    long_codes = O.gen_channelized_codes(9, 1, 8192, 3, 0.3, 0.9816)
+    assert np.array_equal(O.xengine_2bit_exact_f32(long_codes).view(np.uint32),
+                          _direct_exact_2bit(long_codes).view(np.uint32))
+    X = O.LUT[codes & 3].astype(np.float64) + 1j * O.LUT[(codes >> 2) & 3].astype(np.float64)
+    iu = np.triu_indices(9)
+    ref = np.stack([(X[c].T @ X[c].conj())[iu] for c in range(3)], 1).astype(np.complex64)
+    d = np.abs(ex.view(np.int32).astype(np.int64) - ref.view(np.int32).astype(np.int64))
+    assert d.max() <= 1
+    autos = [p for p, (i, j) in enumerate(zip(*iu)) if i == j]
+    assert np.all(ex[autos].imag == 0) and np.all(ex[autos].real > 0)
