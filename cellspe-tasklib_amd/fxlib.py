@@ -9,6 +9,7 @@ been built.
 from __future__ import annotations
 
 import ctypes as C
+import weakref
 import os
 
 import numpy as np
@@ -209,11 +210,14 @@ class DeviceBuffer:
         p = C.c_void_p()
         check(load().fx_malloc(ctx.handle, self.nbytes, C.byref(p)))
         self.ptr = p
+        ctx._adopt(self)
 
     def free(self):
-        if self.ptr is not None and self.ptr.value:
+        if self.ptr is not None and self.ptr.value and self.ctx.handle is not None:
             load().fx_free(self.ctx.handle, self.ptr)
         self.ptr = None
+
+    close = free
 
     def __del__(self):
         try:
@@ -253,9 +257,19 @@ class Context:
         check(lib.fx_ctx_create(device, C.byref(h)))
         self.handle = h
         self.lib = lib
+        self._children = weakref.WeakSet()
+
+    def _adopt(self, obj):
+        """Objects holding device resources of this context: closed before it."""
+        self._children.add(obj)
 
     def close(self):
         if self.handle is not None:
+            # a buffer, correlator or batch still alive (a caller that never
+            # closed it, a traceback holding it) is released while the
+            # context's device and stream still exist
+            for obj in list(self._children):
+                obj.close()
             self.lib.fx_ctx_destroy(self.handle)
             self.handle = None
 
@@ -383,6 +397,7 @@ class Ingest:
         h = C.c_void_p()
         check(load().fx_ingest_open(ctx.handle, os.fsencode(path), offset, chunk_bytes, nbuf, C.byref(h)))
         self.handle = h
+        ctx._adopt(self)
 
     def next(self):
         p, n = C.c_void_p(), C.c_size_t()
@@ -418,6 +433,7 @@ class PCal:
         check(load().fx_pcal_create_inputs(ctx.handle, bandwidth_hz, spacing_hz, int(offset_hz),
                                            sampleoffset, ninp, C.byref(h)))
         self.handle = h
+        ctx._adopt(self)
         k, nb, nt = C.c_int(), C.c_int(), C.c_int()
         check(load().fx_pcal_info(h, C.byref(k), C.byref(nb), C.byref(nt)))
         self.kind, self.nbins, self.ntones = k.value, nb.value, nt.value
@@ -463,6 +479,7 @@ class Correlator:
         h = C.c_void_p()
         check(load().fx_corr_create(ctx.handle, C.byref(self.cfg), C.byref(h)))
         self.handle = h
+        ctx._adopt(self)
         self.ninp, self.nfft, self.nchan = ninp, nfft, nfft // 2
         self.nprod = ninp * (ninp + 1) // 2
 
@@ -578,6 +595,7 @@ class Batch:
         check(load().fx_batch_create(ctx.handle, C.cast(descs, C.c_void_p), n, prec, int(use_graph), tf,
                                      C.byref(h)))
         self.handle = h
+        ctx._adopt(self)
         self.subbands = subbands
 
     def run(self):

@@ -145,6 +145,38 @@ def test_fstage_f32(ctx, n, I, nt, phase):
     corr.close()
 
 
+@pytest.mark.parametrize("n,I,nt,phase", [(4096, 8, 16, True), (8192, 16, 8, True), (8192, 8, 16, False),
+                                           (16384, 8, 8, True)])
+def test_fstage_f16_direct(ctx, n, I, nt, phase):
+    """The FP16-output F kernels with the corner turn folded in (one launch,
+    FFT lengths 4096, 8192 and 16384): every channel of every item within 1e-3
+    (normwise) of the FP32 oracle with the default FP16 power scale -- the
+    FP16 budget of the visibilities -- and no channel's error beyond 1e-2 of
+    that spectrum's RMS (the FP16 rounding of the output is about 3e-4)."""
+    import fxlib
+    t0 = 5
+    streams = _streams(I, n * (t0 + nt), seed=11)
+    models = [(0.37 * i + 0.1, 1e-3 * i, 0.5 * i, 0.01 * i) for i in range(I)] if phase else None
+    dw, stride = _upload_streams(ctx, streams)
+    corr = fxlib.Correlator(ctx, I, n, nt, fxlib.FX_PREC_F16, use_phase=phase)
+    if phase:
+        corr.set_phase_model(models)
+    assert corr.fstage_launches(nt, fxlib.FX_PREC_F16) == 1
+    dX = ctx.alloc((n // 2) * nt * I * 4)
+    corr.fstage(dw, stride, t0, nt, dX, fxlib.FX_PREC_F16)
+    h = dX.download(np.float16, shape=(n // 2, nt, I, 2)).astype(np.float32)
+    got = h[..., 0] + 1j * h[..., 1]
+    # FP16 channels carry the correlator's default power scale 1/sqrt(N E[x^2]), E[x^2] = 4.316
+    ref = O.fstage_f32(streams, n, t0, nt, models) * np.float32(1.0 / np.sqrt(n * 4.316))
+    err = nrel(got, ref)
+    rms = np.sqrt(np.mean(np.abs(ref) ** 2, axis=0))          # per (t, i)
+    worst = float(np.max(np.abs(got - ref) / rms[None]))
+    print(f"FP16 F-stage n={n} I={I} nt={nt}: normwise {err:.2e}, worst channel {worst:.2e} of the spectrum RMS")
+    assert err < 1e-3
+    assert worst < 1e-2
+    corr.close()
+
+
 @pytest.mark.parametrize("I,C,T", [(4, 16, 24), (70, 8, 32), (128, 4, 16)])
 def test_xengine_f32(ctx, I, C, T):
     import fxlib
