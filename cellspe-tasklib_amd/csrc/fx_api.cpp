@@ -831,6 +831,8 @@ struct fx_corr {
     long sub = 0;           // spectra per F-stage sub-chunk (nat depth)
     size_t nat_bytes = 0;   // bytes of the row buffer
     bool direct = true;     // F kernel writes the engine layout itself (no corner turn)
+    int nwg = 0;            // workgroups of the persistent direct F kernel (one per CU; 0: one block per workgroup)
+    unsigned* qctr = nullptr;  // its per-XCD block counters
     float* acc = nullptr;   // engine-layout accumulators
     size_t acc_floats = 0;
     fx_phase_model* pm = nullptr;
@@ -902,6 +904,11 @@ extern "C" int fx_corr_create(fx_ctx* ctx, const fx_corr_config* cfg, fx_corr** 
     c->sub = std::max<long>(8, std::min<long>(cfg->chunk_spectra, (long)(sub_target / row_bytes) / 8 * 8));
     const char* direct_env = getenv("FX_FSTAGE_DIRECT");
     c->direct = !direct_env || atoi(direct_env) != 0;
+    // FX_FSTAGE_PERSIST=0 launches the direct F kernel one block per workgroup
+    const char* persist_env = getenv("FX_FSTAGE_PERSIST");
+    const bool persist = !persist_env || atoi(persist_env) != 0;
+    if (!persist || hipDeviceGetAttribute(&c->nwg, hipDeviceAttributeMultiprocessorCount, ctx->device) != hipSuccess)
+        c->nwg = 0;
     c->nat_bytes = twobit ? 0 : row_bytes * (size_t)c->sub;
     const size_t nat_bytes = c->nat_bytes;
     c->xtile = 128;
@@ -914,6 +921,7 @@ extern "C" int fx_corr_create(fx_ctx* ctx, const fx_corr_config* cfg, fx_corr** 
     bool ok = hipMalloc(&c->acc, c->acc_floats * sizeof(float)) == hipSuccess &&
               hipMalloc(&c->pm, sizeof(fx_phase_model) * (size_t)cfg->ninp) == hipSuccess &&
               hipMalloc(&c->pij, sizeof(int2) * (size_t)P) == hipSuccess;
+    if (ok && c->nwg >= 8) ok = hipMalloc(&c->qctr, 8 * sizeof(unsigned)) == hipSuccess;
     if (ok && chan_bytes)
         ok = hipMalloc(&c->chan, chan_bytes) == hipSuccess && hipMalloc(&c->nat, nat_bytes) == hipSuccess;
     if (ok) ok = hipMemcpy(c->pij, hp.data(), sizeof(int2) * (size_t)P, hipMemcpyHostToDevice) == hipSuccess;
@@ -973,6 +981,7 @@ extern "C" int fx_corr_destroy(fx_corr* c)
     if (c->sc_chan) hipFree(c->sc_chan);
     if (c->sc_inp) hipFree(c->sc_inp);
     if (c->coef) hipFree(c->coef);
+    if (c->qctr) hipFree(c->qctr);
     delete c;
     return FX_OK;
 }
@@ -1071,7 +1080,8 @@ extern "C" int fx_corr_fstage(fx_corr* c, const uint32_t* d_words, size_t in_str
                                           c->cfg.nfft, t_first, nspectra, c->cfg.use_phase ? c->pm : nullptr,
                                           c->mult, scaled && !c->sc_uniform ? c->sc_chan : nullptr,
                                           scaled && c->sc_uniform ? c->sc_uni : 1.0f, scaled ? c->sc_inp : nullptr,
-                                          out_prec, d_chan, nspectra * I, 0, imajor ? nspectra : 0));
+                                          out_prec, d_chan, nspectra * I, 0, imajor ? nspectra : 0,
+                                          imajor ? nullptr : c->qctr, c->nwg));
         return FX_OK;
     }
     // sub-chunks: F kernel into the natural-row buffer, corner turn into d_chan

@@ -21,6 +21,7 @@
 
 #include <hip/hip_fp8.h>
 
+#include <algorithm>
 #include <type_traits>
 
 namespace {
@@ -380,7 +381,6 @@ __global__ __launch_bounds__(FMX / 16) void k_fstage_direct(
     const long lb = xcd_block(nblk);
     if (lb >= nblk) return;
     const long gb = lb * FC;
-    fstage_consts<M, NT>(twg, tb, tid);
     const bool has_pm = PMO || pm != nullptr;
     const long col = IM ? (gb / nt) * ostride_i + col0 + (gb % nt) : col0 + gb;
     const int lo_k = tid % PLO, lo_m = (PLO - lo_k) % PLO;
@@ -395,8 +395,9 @@ __global__ __launch_bounds__(FMX / 16) void k_fstage_direct(
             __syncthreads();   // the previous round's epilogue has read sh and it
             asm volatile("" : "+v"(tl));
         }
-        fstage_group<M, F, NT, IM>(words, in_stride, ninp, t_first, nt, pm, sc_uni, sc_inp, gb + rd * F, sh, tb, it,
-                                   tl, [](auto) {});
+        // the first round also fills the LDS tables
+        fstage_group<M, F, NT, IM>(rd == 0 ? twg : nullptr, words, in_stride, ninp, t_first, nt, pm, sc_uni, sc_inp,
+                                   gb + rd * F, sh, tb, it, tl, [](auto) {});
         const float2 wstep = twg[NT * (FX_TW_N / N)];
         float2 w = twg[tl * (FX_TW_N / N)];
 #pragma unroll
@@ -428,6 +429,166 @@ __global__ __launch_bounds__(FMX / 16) void k_fstage_direct(
         const int kp = q * NT + tid;
         store_run<V, FC>(out + (size_t)kp * ostride + col, vk[q]);
         if (kp != 0) store_run<V, FC>(out + (size_t)(M - kp) * ostride + col, vm[q]);
+    }
+}
+
+// Inputs of the persistent F kernel: their phase models and scales are held
+// in LDS (more inputs take the one-block-per-workgroup kernel).
+#define FX_PDIR_PM 256
+
+// Persistent form of k_fstage_direct (one round: F = FMX/M items, the F
+// inputs of one spectrum, per block).  Workgroup j of XCD x (blockIdx = 8 j
+// + x, one 1024-thread workgroup per CU) takes blocks lo + j, lo + j + W,
+// lo + j + 2W, ... of the XCD's contiguous range [lo, hi) (W workgroups per
+// XCD), so the XCD's workgroups advance through its range side by side and
+// the blocks whose 16-byte runs share the 128-byte lines of a spectrum run
+// together on one XCD (their partial lines merge in its L2), as with one block
+// per workgroup.  What it removes: a workgroup's epilogue ends in line stores
+// that take longer to issue than to compute, and with one block per
+// workgroup the CU idles until they have drained and the next workgroup has
+// started (the LDS holds one workgroup per CU).  Here the next block's packed
+// words are read during the current block's epilogue, and that epilogue's
+// stores drain while the next block transforms: inside the loop every
+// barrier waits for LDS operations only, and nothing else waits for a global
+// access (phase models and input scales come from LDS, the epilogue's
+// twiddles from registers; the first block is peeled off the loop so the wait
+// for the prefetched words never covers the previous block's stores).
+// HASPM: a phase model is applied; PMO: phase model only (no multiplier, no
+// per-channel scales).
+template <int M, typename T, bool HASPM, bool PMO, int FMX>
+__global__ __launch_bounds__(FMX / 16) void k_fstage_direct_p(
+    const float2* __restrict__ twg, const uint32_t* __restrict__ words, size_t in_stride, int ninp,
+    long t_first, long nt, const fx_phase_model* __restrict__ pm, const float2* __restrict__ mult,
+    const float* __restrict__ sc_chan, float sc_uni, const float* __restrict__ sc_inp,
+    typename ChanT<T>::V* __restrict__ out, long ostride, long col0, unsigned* __restrict__ ctr)
+{
+    constexpr int NT = FMX / 16;
+    constexpr int F = FMX / M;
+    constexpr int N = 2 * M;
+    constexpr int H = M / 2;
+    constexpr int QN = H / NT;
+    constexpr int WPT = M / 8;               // packed words per transform
+    constexpr int WTH = (F * M / 8) / NT;    // packed words per thread
+    constexpr int PLO = PhaseTab<M>::PLO;
+    static_assert(H % NT == 0 && NT % 64 == 0 && WTH % 2 == 0, "persistent direct F kernel: M in 2048 .. 8192");
+    static_assert(!PMO || HASPM, "phase-model-only needs a phase model");
+    typedef typename ChanT<T>::V V;
+    __shared__ float2 sh[lds_complex(F * M)];
+    __shared__ FTabs<M> tb;
+    __shared__ FItems<M, F> it;
+    __shared__ fx_phase_model pms[HASPM ? FX_PDIR_PM : 1];
+    __shared__ float sis[FX_PDIR_PM];
+    __shared__ long nextb;
+    const int tid = threadIdx.x;
+    const int x = (int)(blockIdx.x & 7);
+    const long nblk = nt * ninp / F;
+    const long per = (nblk + 7) / 8;
+    const long lo = x * per;
+    const long hi = lo + per < nblk ? lo + per : nblk;
+    for (int i = tid; i < ninp; i += NT) {
+        if constexpr (HASPM) pms[i] = pm[i];
+        sis[i] = sc_inp ? sc_inp[i] : 1.0f;
+    }
+    if (tid == 0) nextb = lo + (long)atomicAdd(&ctr[x], 1u);
+    lds_barrier();   // every thread reads other inputs' models and scales, and the first block
+    long b = nextb;
+    if (b >= hi) return;
+    // this thread's staged words: item wf of a block, words wof .. wof+WTH-1
+    const int wl = WTH * tid;
+    const int wf = wl / WPT;
+    const int wof = wl - wf * WPT;
+    uint2 wpre[WTH / 2];
+    auto wload = [&](long blk) {
+        const long g = blk * F + wf;
+        const long t = g / ninp;
+        const int i = (int)(g - t * ninp);
+        const uint32_t* src = words + (size_t)i * in_stride + (size_t)(t_first + t) * WPT + wof;
+#pragma unroll
+        for (int q = 0; q < WTH; q += 2) wpre[q / 2] = *reinterpret_cast<const uint2*>(src + q);
+    };
+    wload(b);
+    FConstsPre<M, NT> cpre;
+    cpre.load(twg, tid);
+    // the epilogue's split twiddles W_N^kp, kp = q NT + tid: the first, the
+    // step per q and (threads < F) that of kp = M/2
+    const float2 w0 = twg[tid * (FX_TW_N / N)];
+    const float2 wstep = twg[NT * (FX_TW_N / N)];
+    const float2 wmid = twg[(M / 2) * (FX_TW_N / N)];
+    const int lo_k = tid % PLO, lo_m = (PLO - lo_k) % PLO;
+    const fx_phase_model* pml = HASPM ? pms : nullptr;
+    int tl = tid;
+    // thread 0: the next block's offset in the XCD's range.  Left undefined
+    // in the other threads (they never read it), so no value merges with the
+    // atomic's result where its branch ends and the wait for the result sits
+    // at its one use, two passes later.
+    unsigned nb;
+    auto block = [&](auto first) -> long {
+        const long gb = b * F;
+        // an increment (wrapping at 2^32 - 1), which the compiler issues as
+        // one atomic and does not rewrite into a wave-combined add whose
+        // result is needed at once
+        if (tid == 0) nb = atomicInc(&ctr[x], 0xffffffffu);
+        fstage_group<M, F, NT, false>(words, in_stride, ninp, t_first, nt, pml, sc_uni, sis, gb, sh, tb, it, tl,
+                                      [&](auto p) {
+            // published after the second pass, visible after the third's barrier
+            if constexpr (decltype(p)::value == 1) {
+                if (tid == 0) nextb = lo + (long)nb;
+            }
+        },
+                                      []() {},
+                                      [&](uint32_t* raw) {
+#pragma unroll
+            for (int q = 0; q < WTH; q += 2) *reinterpret_cast<uint2*>(raw + wl + q) = wpre[q / 2];
+            if constexpr (decltype(first)::value) cpre.put(tb, tl);
+        });
+        // the next block's words (past the range: this block's again, unused)
+        // arrive during this epilogue
+        const long bn = nextb;
+        wload(bn < hi ? bn : b);
+        const long col = col0 + gb;
+        // kp = M/2 of each item (first: the stores every lane issues come
+        // last, so the next block's wait for its words leaves them in flight)
+        if (tl < F) {
+            float2 xk, xm;
+            rsplit2_tw<M>(sh, tl * M, M / 2, wmid, xk, xm);
+            out[(size_t)(M / 2) * ostride + col + tl] = fstage_finish<M, T, F, PMO>(
+                xk, it, tl, M / 2, HASPM ? it.ph_lo[tl][(M / 2) % PLO] : make_float2(1.f, 0.f), HASPM, mult, sc_chan);
+        }
+        float2 w = w0;
+#pragma unroll
+        for (int q = 0; q < QN; q++) {
+            const int kp = q * NT + tl;
+            if (q) w = cmul(w, wstep);
+            const int k2m = (M - kp) & (M - 1);   // kp = 0: the dropped Nyquist bin
+            V vk[F], vm[F];
+#pragma unroll
+            for (int f = 0; f < F; f++) {
+                float2 xk, xm;
+                rsplit2_tw<M>(sh, f * M, kp, w, xk, xm);
+                vk[f] = fstage_finish<M, T, F, PMO>(xk, it, f, kp, HASPM ? it.ph_lo[f][lo_k] : make_float2(1.f, 0.f),
+                                                   HASPM, mult, sc_chan);
+                vm[f] = fstage_finish<M, T, F, PMO>(xm, it, f, k2m, HASPM ? it.ph_lo[f][lo_m] : make_float2(1.f, 0.f),
+                                                   HASPM, mult, sc_chan);
+            }
+            store_run<V, F>(out + (size_t)kp * ostride + col, vk);
+            // kp = 0 has no channel M - kp: that lane stores its channel 0 a
+            // second time (same address, same values), so every lane issues
+            // the store
+#pragma unroll
+            for (int f = 0; f < F; f++)
+                if (kp == 0) vm[f] = vk[f];
+            store_run<V, F>(out + (size_t)(kp ? M - kp : 0) * ostride + col, vm);
+        }
+        return bn;
+    };
+    b = block(std::true_type{});
+    while (b < hi) {
+        lds_barrier();   // this epilogue's reads of sh and it precede the next block's staging
+        // tl is tid re-derived every block (an empty asm the compiler cannot
+        // see through): the block's LDS address arithmetic is not hoisted into
+        // registers held across blocks
+        asm volatile("" : "+v"(tl));
+        b = block(std::false_type{});
     }
 }
 
@@ -610,16 +771,49 @@ int launch_fstage_direct_t(hipStream_t s, const float2* tw, const uint32_t* word
     return hipGetLastError() == hipSuccess ? FX_OK : FX_ERR_HIP;
 }
 
+// the persistent form: nwg = workgroups to launch (one per CU; at least 8)
+template <int M, typename T>
+int launch_fstage_direct_p(hipStream_t s, const float2* tw, const uint32_t* words, size_t stride, int ninp,
+                           long t_first, long nt, const fx_phase_model* pm, const float2* mult, const float* sc,
+                           float su, const float* si, void* out, long ostride, long col0, unsigned* ctr, int nwg)
+{
+    constexpr int FMX = direct_fm<T>();
+    constexpr int F = FMX / M;
+    if (ninp % F || ostride % F || col0 % F || nwg < 8 || ninp > FX_PDIR_PM) return FX_ERR_ARG;
+    const long nblk = nt * ninp / F;
+    if (nblk == 0) return FX_OK;
+    const long grid = std::min((long)(nwg / 8) * 8, xcd_grid(nblk));
+    if (fxk_fill32(s, ctr, 0u, 8) != FX_OK) return FX_ERR_HIP;
+    auto o = reinterpret_cast<typename ChanT<T>::V*>(out);
+    if (pm && !mult && !sc)
+        hipLaunchKernelGGL((k_fstage_direct_p<M, T, true, true, FMX>), dim3((unsigned)grid), dim3(FMX / 16), 0, s,
+                           tw, words, stride, ninp, t_first, nt, pm, mult, sc, su, si, o, ostride, col0, ctr);
+    else if (pm)
+        hipLaunchKernelGGL((k_fstage_direct_p<M, T, true, false, FMX>), dim3((unsigned)grid), dim3(FMX / 16), 0, s,
+                           tw, words, stride, ninp, t_first, nt, pm, mult, sc, su, si, o, ostride, col0, ctr);
+    else
+        hipLaunchKernelGGL((k_fstage_direct_p<M, T, false, false, FMX>), dim3((unsigned)grid), dim3(FMX / 16), 0, s,
+                           tw, words, stride, ninp, t_first, nt, pm, mult, sc, su, si, o, ostride, col0, ctr);
+    return hipGetLastError() == hipSuccess ? FX_OK : FX_ERR_HIP;
+}
+
 template <int M>
 int launch_fstage_direct(hipStream_t s, const float2* tw, const uint32_t* words, size_t stride, int ninp,
                          long t_first, long nt, const fx_phase_model* pm, const float2* mult, const float* sc, float su,
-                         const float* si, int out_type, void* out, long ostride, long col0, long ostride_i)
+                         const float* si, int out_type, void* out, long ostride, long col0, long ostride_i,
+                         unsigned* ctr, int nwg)
 {
     switch (out_type) {
     case FX_PREC_F32:
+        if (ctr && nwg >= 8 && ninp <= FX_PDIR_PM)
+            return launch_fstage_direct_p<M, float2>(s, tw, words, stride, ninp, t_first, nt, pm, mult, sc, su, si,
+                                                     out, ostride, col0, ctr, nwg);
         return launch_fstage_direct_t<M, float2>(s, tw, words, stride, ninp, t_first, nt, pm, mult, sc, su, si, out,
                                                  ostride, col0, ostride_i);
     case FX_PREC_F16:
+        if (ctr && nwg >= 8 && ninp <= FX_PDIR_PM)
+            return launch_fstage_direct_p<M, __half2>(s, tw, words, stride, ninp, t_first, nt, pm, mult, sc, su, si,
+                                                      out, ostride, col0, ctr, nwg);
         return launch_fstage_direct_t<M, __half2>(s, tw, words, stride, ninp, t_first, nt, pm, mult, sc, su, si, out,
                                                   ostride, col0, ostride_i);
     case FX_PREC_FP8:
@@ -729,12 +923,13 @@ int fxk_cornerturn(hipStream_t s, int out_type, const void* nat, void* out, long
 
 int fxk_fstage_direct(hipStream_t s, const float2* tw, const uint32_t* words, size_t stride, int ninp, int n,
                       long t_first, long nt, const fx_phase_model* pm, const float2* mult, const float* sc, float su,
-                      const float* si, int out_type, void* out, long ostride, long col0, long ostride_i)
+                      const float* si, int out_type, void* out, long ostride, long col0, long ostride_i,
+                      unsigned* ctr, int nwg)
 {
     switch (n) {
-    case 4096: return launch_fstage_direct<2048>(s, tw, words, stride, ninp, t_first, nt, pm, mult, sc, su, si, out_type, out, ostride, col0, ostride_i);
-    case 8192: return launch_fstage_direct<4096>(s, tw, words, stride, ninp, t_first, nt, pm, mult, sc, su, si, out_type, out, ostride, col0, ostride_i);
-    case 16384: return launch_fstage_direct<8192>(s, tw, words, stride, ninp, t_first, nt, pm, mult, sc, su, si, out_type, out, ostride, col0, ostride_i);
+    case 4096: return launch_fstage_direct<2048>(s, tw, words, stride, ninp, t_first, nt, pm, mult, sc, su, si, out_type, out, ostride, col0, ostride_i, ctr, nwg);
+    case 8192: return launch_fstage_direct<4096>(s, tw, words, stride, ninp, t_first, nt, pm, mult, sc, su, si, out_type, out, ostride, col0, ostride_i, ctr, nwg);
+    case 16384: return launch_fstage_direct<8192>(s, tw, words, stride, ninp, t_first, nt, pm, mult, sc, su, si, out_type, out, ostride, col0, ostride_i, ctr, nwg);
     default: return FX_ERR_ARG;
     }
 }

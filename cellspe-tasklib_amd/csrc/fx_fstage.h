@@ -108,6 +108,66 @@ __device__ __forceinline__ void fstage_consts(const float2* __restrict__ twg, FT
     if constexpr (FTabs<M>::TWL) twl_load(tb.twl, twg, tid, NT);
 }
 
+// The same tables as register-held entries: load() issues the global reads
+// of thread tid's entries, put() writes them (and the unpack table) to LDS,
+// so a kernel's first group can have these reads in flight together with its
+// packed words and phase models (one memory round trip before the first
+// barrier instead of three).
+template <int M, int NT>
+struct FConstsPre {
+    static constexpr int N2 = M >= 256 ? 256 : 0;
+    static constexpr int N3 = FTabs<M>::T3 ? 768 : 0;
+    static constexpr int NL = FTabs<M>::TWL ? FX_TWL : 0;
+    static constexpr int TOT = N2 + N3 + NL;
+    static constexpr int PER = (TOT + NT - 1) / NT;
+    float2 v[PER > 0 ? PER : 1];
+    // index into the master table of entry e (tw2, then t3, then twl)
+    __device__ static __forceinline__ int src(int e)
+    {
+        if (e < N2) return (e >> 4) * (e & 15) * (FX_TW_N / 256);
+        e -= N2;
+        if (e < N3) {
+            const int blk = e >> 8, k = e & 255;
+            return (k << (blk == 0 ? 3 : blk == 1 ? 5 : 6)) & (FX_TW_N - 1);
+        }
+        e -= N3;
+        return e < 256 ? e : (e - 256) << 8;
+    }
+    __device__ __forceinline__ void load(const float2* __restrict__ twg, int tid)
+    {
+#pragma unroll
+        for (int p = 0; p < PER; p++) {
+            const int e = tid + p * NT;
+            if (e < TOT) v[p] = twg[src(e)];
+        }
+    }
+    __device__ __forceinline__ void put(FTabs<M>& tb, int tid) const
+    {
+        if (tid < 16) tb.lut[tid] = make_float2(lut2((uint32_t)tid & 3u), lut2((uint32_t)tid >> 2));
+#pragma unroll
+        for (int p = 0; p < PER; p++) {
+            const int e = tid + p * NT;
+            if (e < N2) {
+                tb.tw2[e] = v[p];
+            } else if (e < N2 + N3) {
+                tb.t3[e - N2] = v[p];
+            } else if (e < TOT) {
+                tb.twl[e - N2 - N3] = v[p];
+            }
+        }
+    }
+};
+
+// (spectrum t, input i) of item g of a chunk of nt spectra x ninp inputs
+// (g = t*I + i, or g = i*nt + t when IM); false past the chunk's last item.
+template <bool IM>
+__device__ __forceinline__ bool fitem(long g, long nt, int ninp, long nitems, long& t, int& i)
+{
+    t = IM ? g % nt : g / ninp;
+    i = (int)(IM ? g / nt : g - t * ninp);
+    return g < nitems;
+}
+
 // Front of the F kernels for the F items g0 .. g0+F-1: item bookkeeping, the
 // FP64 phasor tables, the LDS FFT.  Item order: g = t*I + i, or g = i*nt + t
 // when IM.  The packed words of the items are staged at the head of sh and
@@ -115,15 +175,19 @@ __device__ __forceinline__ void fstage_consts(const float2* __restrict__ twg, FT
 // them: complex point e of an item is the 4-bit field e%8 of word e/8, one
 // lookup in the 16-entry table lut (both samples of the point).
 // hook(std::integral_constant<int, p>) runs after FFT pass p
-// (p < fft_npasses<M, 1>()), between the passes' barriers.  stage(raw)
-// writes the group's packed words to raw (thread tid the WTH words from
-// WTH*tid on); the overload without it loads them from words.
-template <int M, int F, int NT, bool IM, typename Hook, typename Stage>
+// (p < fft_npasses<M, 1>()), between the passes' barriers.  pre() runs first
+// and issues the group's global reads into registers; stage(raw) then writes
+// the group's packed words to raw (thread tid the WTH words from WTH*tid on)
+// after the phasor tables are built, so the reads of words, phase models and
+// tables are in flight together.  Every thread derives the (spectrum, input)
+// of the items it needs itself: no barrier before stage.  The overload
+// without pre/stage loads the words from `words`.
+template <int M, int F, int NT, bool IM, typename Hook, typename Pre, typename Stage>
 __device__ __forceinline__ void fstage_group(const uint32_t* __restrict__ words, size_t in_stride, int ninp,
                                              long t_first, long nt, const fx_phase_model* __restrict__ pm,
                                              float sc_uni, const float* __restrict__ sc_inp, long g0, float2* sh,
                                              const FTabs<M>& tb, FItems<M, F>& it, int tid, Hook&& hook,
-                                             Stage&& stage)
+                                             Pre&& pre, Stage&& stage)
 {
     constexpr int N = 2 * M;
     constexpr int WPT = N / 16;             // packed words per transform
@@ -132,22 +196,24 @@ __device__ __forceinline__ void fstage_group(const uint32_t* __restrict__ words,
     constexpr int PHI = PhaseTab<M>::PHI;
     static_assert(WTH % 2 == 0 && M >= 128, "F kernel shape");
     const long nitems = nt * ninp;
+    pre();
     if (tid < F) {
-        const long g = g0 + tid;
-        const long t = IM ? g % nt : g / ninp;
-        const int i = (int)(IM ? g / nt : g - t * ninp);
-        it.t[tid] = g < nitems ? (int)t : -1;
+        long t;
+        int i;
+        const bool v = fitem<IM>(g0 + tid, nt, ninp, nitems, t, i);
+        it.t[tid] = v ? (int)t : -1;
         it.i[tid] = i;
-        it.amp[tid] = 0.5f * sc_uni * ((sc_inp && g < nitems) ? sc_inp[i] : 1.0f);
+        it.amp[tid] = 0.5f * sc_uni * ((sc_inp && v) ? sc_inp[i] : 1.0f);
     }
-    __syncthreads();
     if (pm) {
         for (int e = tid; e < F * (PLO + PHI); e += NT) {
             const int f = e / (PLO + PHI);
             const int j = e - f * (PLO + PHI);
-            if (it.t[f] < 0) continue;
-            const fx_phase_model pmi = pm[it.i[f]];
-            const double tg = (double)(t_first + it.t[f]);
+            long t;
+            int i;
+            if (!fitem<IM>(g0 + f, nt, ninp, nitems, t, i)) continue;
+            const fx_phase_model pmi = pm[i];
+            const double tg = (double)(t_first + t);
             const double tau = pmi.delay0 + pmi.delay_rate * tg;
             double cyc;
             if (j < PLO) {
@@ -160,7 +226,8 @@ __device__ __forceinline__ void fstage_group(const uint32_t* __restrict__ words,
             double sn, cs;
             sincospi(2.0 * cyc, &sn, &cs);
             if (j < PLO) {
-                const double a = (double)it.amp[f];
+                // the item amplitude, as it.amp[f] holds it
+                const double a = (double)(0.5f * sc_uni * (sc_inp ? sc_inp[i] : 1.0f));
                 it.ph_lo[f][j] = make_float2((float)(cs * a), (float)(sn * a));
             } else {
                 it.ph_hi[f][j - PLO] = make_float2((float)cs, (float)sn);
@@ -169,7 +236,7 @@ __device__ __forceinline__ void fstage_group(const uint32_t* __restrict__ words,
     }
     uint32_t* raw = reinterpret_cast<uint32_t*>(sh);
     stage(raw);
-    __syncthreads();
+    lds_barrier();
     {
         constexpr int S1 = M / 16;
         constexpr int BPT = (F * M / 16) / NT;
@@ -185,7 +252,7 @@ __device__ __forceinline__ void fstage_group(const uint32_t* __restrict__ words,
 #pragma unroll
             for (int r = 0; r < 16; r++) u[b][r] = tb.lut[(rw[r * (S1 >> 3)] >> s4) & 15u];
         }
-        __syncthreads();
+        lds_barrier();
 #pragma unroll
         for (int b = 0; b < BPT; b++) {
             const int g = tid + b * NT;
@@ -196,31 +263,45 @@ __device__ __forceinline__ void fstage_group(const uint32_t* __restrict__ words,
 #pragma unroll
             for (int r = 0; r < 16; r++) o[r] = u[b][r];
         }
-        __syncthreads();
+        lds_barrier();
     }
     hook(std::integral_constant<int, 0>{});
     fft_passes_hook<M, 16, NT, F, true, FTabs<M>::T3, 1>(sh, tb.twl, tid, tb.tw2, tb.t3, hook);
 }
 
 template <int M, int F, int NT, bool IM, typename Hook>
-__device__ __forceinline__ void fstage_group(const uint32_t* __restrict__ words, size_t in_stride, int ninp,
-                                             long t_first, long nt, const fx_phase_model* __restrict__ pm,
-                                             float sc_uni, const float* __restrict__ sc_inp, long g0, float2* sh,
-                                             const FTabs<M>& tb, FItems<M, F>& it, int tid, Hook&& hook)
+__device__ __forceinline__ void fstage_group(const float2* __restrict__ twg_tabs, const uint32_t* __restrict__ words,
+                                             size_t in_stride, int ninp, long t_first, long nt,
+                                             const fx_phase_model* __restrict__ pm, float sc_uni,
+                                             const float* __restrict__ sc_inp, long g0, float2* sh, FTabs<M>& tb,
+                                             FItems<M, F>& it, int tid, Hook&& hook)
 {
     constexpr int WPT = M / 8;                  // packed words per transform
     constexpr int WTH = (F * M / 8) / NT;       // packed words per thread
+    const int wl = WTH * tid;
+    const int f = wl / WPT;
+    const int wof = wl - f * WPT;
+    uint2 wpre[WTH / 2];
+    bool valid;
+    FConstsPre<M, NT> cpre;
+    // twg_tabs: also fill the LDS tables (the kernel's first group)
     fstage_group<M, F, NT, IM>(words, in_stride, ninp, t_first, nt, pm, sc_uni, sc_inp, g0, sh, tb, it, tid, hook,
+                               [&]() {
+        long t;
+        int i;
+        valid = fitem<IM>(g0 + f, nt, ninp, nt * ninp, t, i);
+        // unconditional reads (an item past the chunk's end reads its input's
+        // first spectrum and stages zeros), so no wait sits inside a branch
+        const uint32_t* src = words + (size_t)i * in_stride + (size_t)(t_first + (valid ? t : 0)) * WPT + wof;
+#pragma unroll
+        for (int q = 0; q < WTH; q += 2) wpre[q / 2] = *reinterpret_cast<const uint2*>(src + q);
+        if (twg_tabs) cpre.load(twg_tabs, tid);
+    },
                                [&](uint32_t* raw) {
-        const int wl = WTH * tid;
-        const int f = wl / WPT;
-        const int wof = wl - f * WPT;
-        const bool valid = it.t[f] >= 0;
-        const uint32_t* src = words + (size_t)it.i[f] * in_stride + (size_t)(t_first + it.t[f]) * WPT + wof;
 #pragma unroll
         for (int q = 0; q < WTH; q += 2)
-            *reinterpret_cast<uint2*>(raw + wl + q) =
-                valid ? *reinterpret_cast<const uint2*>(src + q) : make_uint2(0u, 0u);
+            *reinterpret_cast<uint2*>(raw + wl + q) = valid ? wpre[q / 2] : make_uint2(0u, 0u);
+        if (twg_tabs) cpre.put(tb, tid);
     });
 }
 
@@ -231,8 +312,7 @@ __device__ __forceinline__ void fstage_front(const float2* __restrict__ twg, con
                                              const float* __restrict__ sc_inp, long g0, float2* sh, FTabs<M>& tb,
                                              FItems<M, F>& it, int tid)
 {
-    fstage_consts<M, NT>(twg, tb, tid);
-    fstage_group<M, F, NT, IM>(words, in_stride, ninp, t_first, nt, pm, sc_uni, sc_inp, g0, sh, tb, it, tid,
+    fstage_group<M, F, NT, IM>(twg, words, in_stride, ninp, t_first, nt, pm, sc_uni, sc_inp, g0, sh, tb, it, tid,
                                [](auto) {});
 }
 

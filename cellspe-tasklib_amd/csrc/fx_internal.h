@@ -143,6 +143,15 @@ __device__ constexpr float kW16re[8] = {1.0f, 0.9238795325f, 0.7071067812f, 0.38
 __device__ constexpr float kW16im[8] = {-0.0f, -0.3826834324f, -0.7071067812f, -0.9238795325f,
                                        -1.0f, -0.9238795325f, -0.7071067812f, -0.3826834324f};
 
+// Workgroup barrier for data exchanged through LDS only: the wave waits for
+// its own LDS operations, not for its outstanding global loads and stores (a
+// __syncthreads() also drains those, so global stores issued before it could
+// not overlap the work after it).
+__device__ __forceinline__ void lds_barrier()
+{
+    asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+}
+
 // In-register forward DFT of R points (R = 1,2,4,8,16), decimation in time.
 template <int R>
 __device__ __forceinline__ void dft_reg(float2* u)
@@ -270,7 +279,7 @@ __device__ __forceinline__ void stockham_pass(float2* sh, const float2* __restri
             for (int r = 0; r < R; r++) u[b][r] = sh[lpad(f * M + j + r * S)];
         }
     }
-    __syncthreads();
+    lds_barrier();
 #pragma unroll
     for (int b = 0; b < BPT; b++) {
         int g = tid + b * NT;
@@ -307,7 +316,7 @@ __device__ __forceinline__ void stockham_pass(float2* sh, const float2* __restri
             for (int r = 0; r < R; r++) sh[lpad(base + r * P)] = u[b][r];
         }
     }
-    __syncthreads();
+    lds_barrier();
 }
 
 template <int M, int P, int NT, int F, bool T2 = false>
@@ -430,10 +439,12 @@ int fxk_fstage(hipStream_t s, const float2* tw, const uint32_t* words, size_t in
 // ostride columns; FP8 out[c][i*ostride_i + col0 + t']) in one pass, for real
 // lengths n = 4096, 8192, 16384 with F = 32768/n items per workgroup;
 // fxk_fstage_direct_ok says whether a launch of these shapes is supported
+// nwg: workgroups of the persistent form (one per CU; FP32/FP16 out, at most
+// 256 inputs), or 0 for one block per workgroup
 int fxk_fstage_direct(hipStream_t s, const float2* tw, const uint32_t* words, size_t in_stride_words,
                       int ninp, int n, long t_first, long nt, const fx_phase_model* pm,
                       const float2* mult, const float* scale_chan, float scale_uni, const float* scale_inp,
-                      int out_type, void* out, long ostride, long col0, long ostride_i);
+                      int out_type, void* out, long ostride, long col0, long ostride_i, unsigned* ctr, int nwg);
 bool fxk_fstage_direct_ok(int n, int ninp, long nt, int out_type, long ostride, long col0, long ostride_i);
 // corner turn of one sub-chunk of F-stage rows nat[g][c] (nitems items of ns
 // spectra) into the chunk layout: FP32/FP16 out[c][col0 + g] with rows of

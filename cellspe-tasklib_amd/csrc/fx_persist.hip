@@ -197,7 +197,7 @@ __device__ __noinline__ void f_item(const float2* __restrict__ twg_, const fx_ps
         }
         const long grp = g_first + gg;
         fstage_group<M, F, PNT, false>((const uint32_t*)words, stride, I, 0, nt, pmp, sc, nullptr, grp * F, sh, tb,
-                                       it, tl, [](auto) {},
+                                       it, tl, [](auto) {}, []() {},
                                        [&](uint32_t* raw) {
             *reinterpret_cast<pu32x2*>(raw + WTH * tl) = pre;
             if (gg + 1 < ngroups) pre = wload(grp + 1);
