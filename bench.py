@@ -103,6 +103,21 @@ def cpu_obj(value, unit, sample, extra=None):
     return o
 
 
+def measured_valu(kernel):
+    """VALU busy fraction of `kernel` from the committed rocprofv3 counter
+    summaries (lines 'valu_busy <kernel> <fraction>'), or None."""
+    pdir = os.path.join(ROOT, "profiles")
+    if not os.path.isdir(pdir):
+        return None
+    for fn in sorted(os.listdir(pdir)):
+        with open(os.path.join(pdir, fn)) as f:
+            for line in f:
+                m = re.match(r"^valu_busy\s+(\S.*\S)\s+([0-9.eE+]+)\s*$", line)
+                if m and m.group(1) == kernel:
+                    return {"frac": float(m.group(2)), "source": f"profiles/{fn}"}
+    return None
+
+
 def measured_traffic(kernel):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC
     summaries (lines 'bytes_per_launch <kernel> <bytes>'), or None."""
@@ -877,6 +892,7 @@ def main():
         corr.set_phase_model(models)
         dchan = ctx.alloc(C * chunk * I * esz)
         nf = corr.fstage_launches(chunk, prec)
+        fname = corr.fstage_name(chunk, prec)
         xname = corr.xengine_name(chunk, dchan)
         evf = [ctx.event(), ctx.event()]
         evx = [ctx.event(), ctx.event()]
@@ -912,7 +928,8 @@ def main():
         x = ctx.elapsed_ms(evx[0], evx[1])
         dchan.free()
         corr.close()
-        return ms, {"fstage": f, "xengine": x, "fstage_launches": nf, "xengine_kernel": xname}
+        return ms, {"fstage": f, "xengine": x, "fstage_launches": nf, "fstage_kernel": fname,
+                    "xengine_kernel": xname}
 
     paths = {}
     ms16, k16 = run_path(fxlib.FX_PREC_F16, 4, args.steps, args.warmup)
@@ -1011,14 +1028,18 @@ def main():
                 "achieved": x_flop / (launch * 1e-3) / 1e12, "peak": PEAK["fp16"], "unit": "TFLOP/s",
                 "algorithmic": f"8*P*C*chunk = {x_flop:.3e} flop per launch"}
     else:
-        if k16["fstage_launches"] != 1:
-            raise SystemExit("bench: the F stage of this shape is not the single direct launch")
+        if not k16["fstage_kernel"].startswith("k_fstage_direct"):
+            raise SystemExit("bench: the F stage of this shape is not the direct engine-layout kernel")
         launch = k16["fstage"]
-        roof = {"kernel": "k_fstage_direct<4096, __half2, true>", "bound": "hbm",
+        fft_tflops = 2.5 * N * np.log2(N) * I * chunk / (launch * 1e-3) / 1e12
+        roof = {"kernel": k16["fstage_kernel"], "bound": "hbm",
                 "achieved": f_bytes / (launch * 1e-3) / 1e9, "peak": PEAK["hbm"], "unit": "GB/s",
                 "algorithmic": f"per input-spectrum N/4 B packed in + C*4 B FP16 out; {chunk} spectra x {I} "
                                f"inputs = {f_bytes:.3e} B per launch",
-                "valu_tflops": 2.5 * N * np.log2(N) * I * chunk / (launch * 1e-3) / 1e12}
+                "launches_timed": k16["fstage_launches"],
+                # the roof that binds: VALU issue (counters) and the FFT's flop rate
+                "valu": {"fft_tflops": fft_tflops, "fft_frac_of_fp32_peak": fft_tflops / PEAK["fp32"],
+                         "busy": measured_valu(k16["fstage_kernel"])}}
     roof["frac"] = roof["achieved"] / roof["peak"]
     tr = measured_traffic(roof["kernel"])
     roof["traffic"] = None if tr is None else tr["bytes"]

@@ -1096,13 +1096,35 @@ extern "C" int fx_corr_fstage(fx_corr* c, const uint32_t* d_words, size_t in_str
 
 extern "C" long fx_corr_fstage_depth(fx_corr* c) { return c ? c->sub : 0; }
 
+// the direct F kernel runs in its persistent form (a counter clear, then the
+// kernel) for FP32/FP16 output and at most FX_PDIR_PM = 256 inputs
+static bool fstage_persistent(const fx_corr* c, int out_prec)
+{
+    return c->qctr && c->nwg >= 8 && c->cfg.ninp <= 256 && out_prec != FX_PREC_FP8;
+}
+
 extern "C" int fx_corr_fstage_launches(fx_corr* c, long nspectra, int out_prec)
 {
     if (!c || nspectra <= 0) return 0;
     const long I = c->cfg.ninp;
     if (c->direct && fxk_fstage_direct_ok(c->cfg.nfft, (int)I, nspectra, out_prec, nspectra * I, 0, nspectra))
-        return 1;
+        return fstage_persistent(c, out_prec) ? 2 : 1;
     return (int)(2 * ((nspectra + c->sub - 1) / c->sub));
+}
+
+extern "C" int fx_corr_fstage_name(fx_corr* c, long nspectra, int out_prec, char* buf, int len)
+{
+    FX_REQUIRE(c && buf && len > 0, FX_ERR_ARG, "fx_corr_fstage_name: null argument");
+    const long I = c->cfg.ninp;
+    const int M = c->cfg.nfft / 2;
+    const char* t = out_prec == FX_PREC_F32 ? "float2" : out_prec == FX_PREC_F16 ? "__half2" : "fp8x2";
+    if (c->direct && nspectra > 0 &&
+        fxk_fstage_direct_ok(c->cfg.nfft, (int)I, nspectra, out_prec, nspectra * I, 0, nspectra))
+        snprintf(buf, (size_t)len, "%s<%d, %s>", fstage_persistent(c, out_prec) ? "k_fstage_direct_p" : "k_fstage_direct",
+                 M, t);
+    else
+        snprintf(buf, (size_t)len, "k_fstage<%d, %s> + k_cornerturn", M, t);
+    return FX_OK;
 }
 
 extern "C" int fx_corr_fstage_rows(fx_corr* c, const uint32_t* d_words, size_t in_stride_words,

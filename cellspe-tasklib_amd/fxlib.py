@@ -143,6 +143,7 @@ def load():
         "fx_corr_tune": (i, [vp, l, i]),
         "fx_corr_set_xengine": (i, [vp, i]),
         "fx_corr_xengine_name": (i, [vp, vp, l, C.c_char_p, i]),
+        "fx_corr_fstage_name": (i, [vp, l, i, C.c_char_p, i]),
         "fx_pcal_final_async": (i, [vp]),
         "fx_pcal_tones": (i, [vp, vp, C.POINTER(u64)]),
         "fx_pcal_create_inputs": (i, [vp, d, d, i, sz, i, C.POINTER(vp)]),
@@ -513,8 +514,15 @@ class Correlator:
 
     def fstage_launches(self, nspectra, out_prec):
         """Kernel launches of fstage() for nspectra spectra (1: the direct
-        engine-layout transform kernel)."""
+        engine-layout transform kernel; 2: its persistent form after the clear
+        of its block counters)."""
         return load().fx_corr_fstage_launches(self.handle, nspectra, out_prec)
+
+    def fstage_name(self, nspectra, out_prec):
+        """Name of the F transform kernel fstage() runs on nspectra spectra."""
+        buf = C.create_string_buffer(64)
+        check(load().fx_corr_fstage_name(self.handle, nspectra, out_prec, buf, 64))
+        return buf.value.decode()
 
     def xengine(self, d_chan, nspectra):
         check(load().fx_corr_xengine(self.handle, d_chan.ptr, nspectra))

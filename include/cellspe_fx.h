@@ -372,6 +372,10 @@ int fx_corr_fstage(fx_corr* corr, const uint32_t* d_words, size_t in_stride_word
  * turn to spectra [s0, s0 + nspectra) of a chunk of chunk_spectra spectra in
  * d_chan. */
 int fx_corr_fstage_launches(fx_corr* corr, long nspectra, int out_prec);
+/* name of the F transform kernel fx_corr_fstage runs on nspectra spectra
+ * (k_fstage_direct_p: the persistent direct kernel, launched after a clear of
+ * its eight block counters) */
+int fx_corr_fstage_name(fx_corr* corr, long nspectra, int out_prec, char* buf, int len);
 long fx_corr_fstage_depth(fx_corr* corr);
 int fx_corr_fstage_rows(fx_corr* corr, const uint32_t* d_words, size_t in_stride_words,
                         long t_first, long nspectra, int out_prec);
