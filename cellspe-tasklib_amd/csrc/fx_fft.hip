@@ -22,6 +22,7 @@
 #include <hip/hip_fp8.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <type_traits>
 
 namespace {
@@ -432,6 +433,20 @@ __global__ __launch_bounds__(FMX / 16) void k_fstage_direct(
     }
 }
 
+// The value of the other lane of each lane pair (2m, 2m+1): a DPP quad
+// permutation [1, 0, 3, 2].  One-word channel values only (FP16 complex);
+// the generic form is never called.
+template <typename V>
+__device__ __forceinline__ V swap_lane_pairs(V v)
+{
+    return v;
+}
+template <>
+__device__ __forceinline__ uint32_t swap_lane_pairs<uint32_t>(uint32_t v)
+{
+    return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0xB1, 0xF, 0xF, true);
+}
+
 // Inputs of the persistent F kernel: their phase models and scales are held
 // in LDS (more inputs take the one-block-per-workgroup kernel).
 #define FX_PDIR_PM 256
@@ -455,7 +470,7 @@ __global__ __launch_bounds__(FMX / 16) void k_fstage_direct(
 // for the prefetched words never covers the previous block's stores).
 // HASPM: a phase model is applied; PMO: phase model only (no multiplier, no
 // per-channel scales).
-template <int M, typename T, bool HASPM, bool PMO, int FMX>
+template <int M, typename T, bool HASPM, bool PMO, bool PAIR, int FMX>
 __global__ __launch_bounds__(FMX / 16) void k_fstage_direct_p(
     const float2* __restrict__ twg, const uint32_t* __restrict__ words, size_t in_stride, int ninp,
     long t_first, long nt, const fx_phase_model* __restrict__ pm, const float2* __restrict__ mult,
@@ -472,6 +487,8 @@ __global__ __launch_bounds__(FMX / 16) void k_fstage_direct_p(
     constexpr int PLO = PhaseTab<M>::PLO;
     static_assert(H % NT == 0 && NT % 64 == 0 && WTH % 2 == 0, "persistent direct F kernel: M in 2048 .. 8192");
     static_assert(!PMO || HASPM, "phase-model-only needs a phase model");
+    static_assert(!PAIR || std::is_same<T, __half2>::value, "paired stores: FP16 channels (one word a sample)");
+    constexpr int U = PAIR ? 2 : 1;          // blocks per unit of the queue
     typedef typename ChanT<T>::V V;
     __shared__ float2 sh[lds_complex(F * M)];
     __shared__ FTabs<M> tb;
@@ -481,7 +498,7 @@ __global__ __launch_bounds__(FMX / 16) void k_fstage_direct_p(
     __shared__ long nextb;
     const int tid = threadIdx.x;
     const int x = (int)(blockIdx.x & 7);
-    const long nblk = nt * ninp / F;
+    const long nblk = nt * ninp / (F * U);   // units
     const long per = (nblk + 7) / 8;
     const long lo = x * per;
     const long hi = lo + per < nblk ? lo + per : nblk;
@@ -491,7 +508,7 @@ __global__ __launch_bounds__(FMX / 16) void k_fstage_direct_p(
     }
     if (tid == 0) nextb = lo + (long)atomicAdd(&ctr[x], 1u);
     lds_barrier();   // every thread reads other inputs' models and scales, and the first block
-    long b = nextb;
+    long b = nextb;   // the unit: blocks U b .. U b + U - 1
     if (b >= hi) return;
     // this thread's staged words: item wf of a block, words wof .. wof+WTH-1
     const int wl = WTH * tid;
@@ -506,15 +523,21 @@ __global__ __launch_bounds__(FMX / 16) void k_fstage_direct_p(
 #pragma unroll
         for (int q = 0; q < WTH; q += 2) wpre[q / 2] = *reinterpret_cast<const uint2*>(src + q);
     };
-    wload(b);
+    wload(U * b);
     FConstsPre<M, NT> cpre;
     cpre.load(twg, tid);
     // the epilogue's split twiddles W_N^kp, kp = q NT + tid: the first, the
-    // step per q and (threads < F) that of kp = M/2
-    const float2 w0 = twg[tid * (FX_TW_N / N)];
-    const float2 wstep = twg[NT * (FX_TW_N / N)];
-    const float2 wmid = twg[(M / 2) * (FX_TW_N / N)];
-    const int lo_k = tid % PLO, lo_m = (PLO - lo_k) % PLO;
+    // step per q and (threads < F) that of kp = M/2; held in registers, or
+    // (PAIR, where the held channel values need the registers) read from
+    // the table in each epilogue, before its word reads so the wait for them
+    // leaves those in flight
+    float2 w0r, wstepr, wmidr;
+    auto twiddles = [&]() {
+        w0r = twg[tid * (FX_TW_N / N)];
+        wstepr = twg[NT * (FX_TW_N / N)];
+        wmidr = twg[(M / 2) * (FX_TW_N / N)];
+    };
+    if constexpr (!PAIR) twiddles();
     const fx_phase_model* pml = HASPM ? pms : nullptr;
     int tl = tid;
     // thread 0: the next block's offset in the XCD's range.  Left undefined
@@ -522,16 +545,24 @@ __global__ __launch_bounds__(FMX / 16) void k_fstage_direct_p(
     // atomic's result where its branch ends and the wait for the result sits
     // at its one use, two passes later.
     unsigned nb;
-    auto block = [&](auto first) -> long {
-        const long gb = b * F;
+    // PAIR: the first block's channel values, held until the second block's
+    // (one word each: [q][channel k, M - k][item])
+    V hold[PAIR ? QN : 1][2][F];
+    // sub = 0: a lone block, stored at once; 1: a pair's first block, held;
+    // 2: a pair's second block, stored with the first as 32-byte runs
+    // (both lambdas inlined: an out-of-line call would put the arrays they
+    // capture by reference in scratch memory)
+    auto block = [&](auto first, auto sub) __attribute__((always_inline)) -> long {
+        constexpr int SUB = decltype(sub)::value;
+        const long gb = (U * b + (SUB == 2 ? 1 : 0)) * F;
         // an increment (wrapping at 2^32 - 1), which the compiler issues as
         // one atomic and does not rewrite into a wave-combined add whose
         // result is needed at once
-        if (tid == 0) nb = atomicInc(&ctr[x], 0xffffffffu);
+        if (SUB != 2 && tid == 0) nb = atomicInc(&ctr[x], 0xffffffffu);
         fstage_group<M, F, NT, false>(words, in_stride, ninp, t_first, nt, pml, sc_uni, sis, gb, sh, tb, it, tl,
                                       [&](auto p) {
             // published after the second pass, visible after the third's barrier
-            if constexpr (decltype(p)::value == 1) {
+            if constexpr (decltype(p)::value == 1 && SUB != 2) {
                 if (tid == 0) nextb = lo + (long)nb;
             }
         },
@@ -544,7 +575,12 @@ __global__ __launch_bounds__(FMX / 16) void k_fstage_direct_p(
         // the next block's words (past the range: this block's again, unused)
         // arrive during this epilogue
         const long bn = nextb;
-        wload(bn < hi ? bn : b);
+        if constexpr (PAIR) twiddles();
+        if constexpr (SUB == 1)
+            wload(U * b + 1);
+        else
+            wload(U * (bn < hi ? bn : b));
+        const float2 w0 = w0r, wstep = wstepr, wmid = wmidr;
         const long col = col0 + gb;
         // kp = M/2 of each item (first: the stores every lane issues come
         // last, so the next block's wait for its words leaves them in flight)
@@ -557,6 +593,12 @@ __global__ __launch_bounds__(FMX / 16) void k_fstage_direct_p(
         float2 w = w0;
 #pragma unroll
         for (int q = 0; q < QN; q++) {
+            // PAIR: the phasor factors are read again for every q (tq hides
+            // that they are the same ones), so they are not held across q
+            // beside the held channel values
+            int tq = tl;
+            if constexpr (PAIR) asm volatile("" : "+v"(tq));
+            const int lo_k = tq % PLO, lo_m = (PLO - lo_k) % PLO;
             const int kp = q * NT + tl;
             if (q) w = cmul(w, wstep);
             const int k2m = (M - kp) & (M - 1);   // kp = 0: the dropped Nyquist bin
@@ -570,25 +612,69 @@ __global__ __launch_bounds__(FMX / 16) void k_fstage_direct_p(
                 vm[f] = fstage_finish<M, T, F, PMO>(xm, it, f, k2m, HASPM ? it.ph_lo[f][lo_m] : make_float2(1.f, 0.f),
                                                    HASPM, mult, sc_chan);
             }
-            store_run<V, F>(out + (size_t)kp * ostride + col, vk);
             // kp = 0 has no channel M - kp: that lane stores its channel 0 a
             // second time (same address, same values), so every lane issues
             // the store
 #pragma unroll
             for (int f = 0; f < F; f++)
                 if (kp == 0) vm[f] = vk[f];
-            store_run<V, F>(out + (size_t)(kp ? M - kp : 0) * ostride + col, vm);
+            if constexpr (SUB == 0) {
+                store_run<V, F>(out + (size_t)kp * ostride + col, vk);
+                store_run<V, F>(out + (size_t)(kp ? M - kp : 0) * ostride + col, vm);
+            } else if constexpr (SUB == 1) {
+#pragma unroll
+                for (int f = 0; f < F; f++) {
+                    hold[q][0][f] = vk[f];
+                    hold[q][1][f] = vm[f];
+                }
+            } else {
+                // Lanes 2m and 2m+1 hold channels kp and kp ^ 1 of both
+                // blocks (the first block's F columns, then the second's):
+                // they swap the first block's value of the odd channel for
+                // the second block's value of the even one, then each store
+                // writes one channel's 2F columns as two neighbouring 16-byte
+                // halves, one per lane (one 32-byte piece of the line).
+                const bool odd = (tl & 1) != 0;
+                const int kpe = kp & ~1, kpo = kp | 1;
+#pragma unroll
+                for (int s2 = 0; s2 < 2; s2++) {
+                    V st1[F], st2[F];
+#pragma unroll
+                    for (int f = 0; f < F; f++) {
+                        const V a = hold[q][s2][f];
+                        const V bv = s2 ? vm[f] : vk[f];
+                        const V recv = swap_lane_pairs(odd ? a : bv);
+                        st1[f] = odd ? recv : a;
+                        st2[f] = odd ? bv : recv;
+                    }
+                    const int ce = s2 ? (kpe ? M - kpe : 0) : kpe;
+                    const int co = s2 ? M - kpo : kpo;
+                    const long cc = col - F + (odd ? F : 0);   // the pair's first column, + F for the odd lane
+                    store_run<V, F>(out + (size_t)ce * ostride + cc, st1);
+                    store_run<V, F>(out + (size_t)co * ostride + cc, st2);
+                }
+            }
         }
         return bn;
     };
-    b = block(std::true_type{});
+    auto unit = [&](auto first) __attribute__((always_inline)) -> long {
+        if constexpr (PAIR) {
+            block(first, std::integral_constant<int, 1>{});
+            lds_barrier();   // the first block's epilogue has read sh and it
+            asm volatile("" : "+v"(tl));
+            return block(std::false_type{}, std::integral_constant<int, 2>{});
+        } else {
+            return block(first, std::integral_constant<int, 0>{});
+        }
+    };
+    b = unit(std::true_type{});
     while (b < hi) {
         lds_barrier();   // this epilogue's reads of sh and it precede the next block's staging
         // tl is tid re-derived every block (an empty asm the compiler cannot
         // see through): the block's LDS address arithmetic is not hoisted into
         // registers held across blocks
         asm volatile("" : "+v"(tl));
-        b = block(std::false_type{});
+        b = unit(std::false_type{});
     }
 }
 
@@ -771,6 +857,16 @@ int launch_fstage_direct_t(hipStream_t s, const float2* tw, const uint32_t* word
     return hipGetLastError() == hipSuccess ? FX_OK : FX_ERR_HIP;
 }
 
+// FX_FSTAGE_PAIR=0 keeps the persistent kernel's blocks single (16-byte runs)
+static bool direct_pair()
+{
+    static const bool on = [] {
+        const char* e = getenv("FX_FSTAGE_PAIR");
+        return !e || atoi(e) != 0;
+    }();
+    return on;
+}
+
 // the persistent form: nwg = workgroups to launch (one per CU; at least 8)
 template <int M, typename T>
 int launch_fstage_direct_p(hipStream_t s, const float2* tw, const uint32_t* words, size_t stride, int ninp,
@@ -785,14 +881,25 @@ int launch_fstage_direct_p(hipStream_t s, const float2* tw, const uint32_t* word
     const long grid = std::min((long)(nwg / 8) * 8, xcd_grid(nblk));
     if (fxk_fill32(s, ctr, 0u, 8) != FX_OK) return FX_ERR_HIP;
     auto o = reinterpret_cast<typename ChanT<T>::V*>(out);
-    if (pm && !mult && !sc)
-        hipLaunchKernelGGL((k_fstage_direct_p<M, T, true, true, FMX>), dim3((unsigned)grid), dim3(FMX / 16), 0, s,
+    // FP16 with the blocks of a spectrum in pairs: 32-byte store runs
+    constexpr bool P2 = std::is_same<T, __half2>::value;
+    if (P2 && ninp % (2 * F) == 0 && direct_pair()) {
+        // the general epilogue also for a phase model alone: the straight-line
+        // one needs more registers than the held values leave (it spills)
+        if (pm)
+            hipLaunchKernelGGL((k_fstage_direct_p<M, T, true, false, P2, FMX>), dim3((unsigned)grid), dim3(FMX / 16),
+                               0, s, tw, words, stride, ninp, t_first, nt, pm, mult, sc, su, si, o, ostride, col0, ctr);
+        else
+            hipLaunchKernelGGL((k_fstage_direct_p<M, T, false, false, P2, FMX>), dim3((unsigned)grid), dim3(FMX / 16),
+                               0, s, tw, words, stride, ninp, t_first, nt, pm, mult, sc, su, si, o, ostride, col0, ctr);
+    } else if (pm && !mult && !sc)
+        hipLaunchKernelGGL((k_fstage_direct_p<M, T, true, true, false, FMX>), dim3((unsigned)grid), dim3(FMX / 16), 0, s,
                            tw, words, stride, ninp, t_first, nt, pm, mult, sc, su, si, o, ostride, col0, ctr);
     else if (pm)
-        hipLaunchKernelGGL((k_fstage_direct_p<M, T, true, false, FMX>), dim3((unsigned)grid), dim3(FMX / 16), 0, s,
+        hipLaunchKernelGGL((k_fstage_direct_p<M, T, true, false, false, FMX>), dim3((unsigned)grid), dim3(FMX / 16), 0, s,
                            tw, words, stride, ninp, t_first, nt, pm, mult, sc, su, si, o, ostride, col0, ctr);
     else
-        hipLaunchKernelGGL((k_fstage_direct_p<M, T, false, false, FMX>), dim3((unsigned)grid), dim3(FMX / 16), 0, s,
+        hipLaunchKernelGGL((k_fstage_direct_p<M, T, false, false, false, FMX>), dim3((unsigned)grid), dim3(FMX / 16), 0, s,
                            tw, words, stride, ninp, t_first, nt, pm, mult, sc, su, si, o, ostride, col0, ctr);
     return hipGetLastError() == hipSuccess ? FX_OK : FX_ERR_HIP;
 }
