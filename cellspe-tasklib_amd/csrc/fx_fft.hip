@@ -129,17 +129,39 @@ __global__ __launch_bounds__(FftShape<M>::NT) void k_fft_r2c(const float2* __res
 // of channel c, F transforms at a time, and writes its partial power
 // partial[wg][f][k].
 // ---------------------------------------------------------------------------
+// The spectrometer's tile: FftShape's, with 1024 threads for the 16384-point
+// tile (one transform of M = 16384; its LDS allows one workgroup per CU, and
+// 16 waves hide the load and LDS latency that 8 leave exposed).
 template <int M>
-__global__ __launch_bounds__(FftShape<M>::NT) void k_spectrometer(
+struct SpecShape {
+    static constexpr int FM = FftShape<M>::FM;
+    static constexpr int F = FftShape<M>::F;
+    static constexpr int NT = FM >= 16384 ? 1024 : FftShape<M>::NT;
+    static constexpr int QPT = FM / NT;
+    static constexpr int SLT = FM / (2 * NT);
+    static constexpr int LDS = FftShape<M>::LDS;
+};
+
+template <int M>
+__global__ __launch_bounds__(SpecShape<M>::NT) void k_spectrometer(
     const float2* __restrict__ twg, const float* __restrict__ win, const uint32_t* __restrict__ words,
     int nchan, int integrate_ffts, int groups, int per_group, long hop, long spec_stride,
     float* __restrict__ partial)
 {
-    using S = FftShape<M>;
+    using S = SpecShape<M>;
     __shared__ float2 sh[S::LDS];
     __shared__ float2 tw[FX_TWL];
+    // the radix-16 passes' twiddle tables: W_256^{rk} (span 16) and
+    // W_4096^{k}, W_4096^{4k}, W_4096^{8k} (span 256), as the F kernels use
+    __shared__ float2 tw2[256];
+    __shared__ float2 t3[768];
     const int tid = threadIdx.x;
     twl_load(tw, twg, tid, S::NT);
+    for (int e = tid; e < 256; e += S::NT) tw2[e] = twg[(e >> 4) * (e & 15) * (FX_TW_N / 256)];
+    for (int e = tid; e < 768; e += S::NT) {
+        const int blk = e >> 8, k = e & 255;
+        t3[e] = twg[(k << (blk == 0 ? 3 : blk == 1 ? 5 : 6)) & (FX_TW_N - 1)];
+    }
     const int wg = blockIdx.x;
     const int g = wg % groups;
     const int cs = wg / groups;
@@ -158,7 +180,12 @@ __global__ __launch_bounds__(FftShape<M>::NT) void k_spectrometer(
     int fft_count = per_group;
     if ((long)g * per_group + fft_count > integrate_ffts) fft_count = integrate_ffts - g * per_group;
 
+    int tl = tid;
     for (int it = 0; it < fft_count; it += S::F) {
+        // tl is tid re-derived every iteration (an empty asm the compiler
+        // cannot see through): the split twiddles and addresses are
+        // recomputed, not held across iterations in registers (or scratch)
+        asm volatile("" : "+v"(tl));
         // load F transforms: F*N samples of channel c; thread handles 2*16 = 32
         // consecutive-in-pairs samples: sample pair e -> z[e] = (x[2e], x[2e+1])
         // (partial unroll bounds the loads in flight to the 1024-thread budget)
@@ -171,8 +198,16 @@ __global__ __launch_bounds__(FftShape<M>::NT) void k_spectrometer(
             if (it + f < fft_count) {
                 // FFT j of spectrum s starts at s*spec_stride + j*hop (hop = N/overlap)
                 long t = s * spec_stride + ((long)g * per_group + it + f) * hop + n2;
-                uint32_t w0 = words[t >> lspw];
-                uint32_t w1 = words[(t + 1) >> lspw];
+                uint32_t w0, w1;
+                if (spw == 1 && ((t & 1) == 0) && (((uintptr_t)words & 7) == 0)) {
+                    // 16 channels: one word a time sample, both words at once
+                    const uint2 ww = *reinterpret_cast<const uint2*>(words + t);
+                    w0 = ww.x;
+                    w1 = ww.y;
+                } else {
+                    w0 = words[t >> lspw];
+                    w1 = words[(t + 1) >> lspw];
+                }
                 int p0 = (int)(t & (spw - 1)) * bits_per_t + shift0;
                 int p1 = (int)((t + 1) & (spw - 1)) * bits_per_t + shift0;
                 z.x = lut2((w0 >> p0) & 3u) * win[n2];
@@ -181,12 +216,12 @@ __global__ __launch_bounds__(FftShape<M>::NT) void k_spectrometer(
             sh[lpad(e)] = z;
         }
         __syncthreads();
-        fft_passes<M, 1, S::NT, S::F>(sh, tw, tid);
+        fft_passes_hook<M, 1, S::NT, S::F, true, true>(sh, tw, tid, tw2, t3, [](auto) {});
         // power of the thread's 8 (kp, f) items, kp-major so the same thread
         // owns the same bins on every iteration
 #pragma unroll
         for (int i = 0; i < S::SLT; i++) {
-            int q = tid * S::SLT + i;
+            int q = tl * S::SLT + i;
             int kp = q / S::F;
             int f = q - kp * S::F;
             if (it + f < fft_count) {
@@ -634,8 +669,17 @@ __global__ __launch_bounds__(FMX / 16) void k_fstage_direct_p(
                 // the second block's value of the even one, then each store
                 // writes one channel's 2F columns as two neighbouring 16-byte
                 // halves, one per lane (one 32-byte piece of the line).
+                // Row offsets: the even channel kpe = q NT + (tl & ~1) from
+                // one per-block product and a uniform step per q; the odd
+                // one a row further (channel kpe + 1) or nearer (M - kpe - 1);
+                // kpe = 0 has no mirror: its lanes store channel 0 again.
                 const bool odd = (tl & 1) != 0;
-                const int kpe = kp & ~1, kpo = kp | 1;
+                const size_t os = (size_t)ostride;
+                const size_t ke = (size_t)q * NT * os + (size_t)(tl & ~1) * os;
+                V* const pb = out + (col - F) + (odd ? F : 0);   // the pair's first column, + F for the odd lane
+                V* const pke = pb + ke;
+                V* const pmo = pb + (size_t)M * os - ke - os;
+                V* const pme = (q == 0 && tl < 2) ? pb : pb + (size_t)M * os - ke;
 #pragma unroll
                 for (int s2 = 0; s2 < 2; s2++) {
                     V st1[F], st2[F];
@@ -647,11 +691,8 @@ __global__ __launch_bounds__(FMX / 16) void k_fstage_direct_p(
                         st1[f] = odd ? recv : a;
                         st2[f] = odd ? bv : recv;
                     }
-                    const int ce = s2 ? (kpe ? M - kpe : 0) : kpe;
-                    const int co = s2 ? M - kpo : kpo;
-                    const long cc = col - F + (odd ? F : 0);   // the pair's first column, + F for the odd lane
-                    store_run<V, F>(out + (size_t)ce * ostride + cc, st1);
-                    store_run<V, F>(out + (size_t)co * ostride + cc, st2);
+                    store_run<V, F>(s2 ? pme : pke, st1);
+                    store_run<V, F>(s2 ? pmo : pke + os, st2);
                 }
             }
         }
@@ -746,7 +787,7 @@ int launch_spec(hipStream_t s, const float2* tw, const float* win, const uint32_
                 long spec_stride, int nchan, int integrate_ffts, int nspec, int groups, int nyq, float* partial,
                 float* out)
 {
-    using S = FftShape<M>;
+    using S = SpecShape<M>;
     int per_group = (integrate_ffts + groups - 1) / groups;
     per_group = ((per_group + S::F - 1) / S::F) * S::F;
     long nblk = (long)nspec * nchan * groups;
