@@ -831,8 +831,9 @@ struct fx_corr {
     long sub = 0;           // spectra per F-stage sub-chunk (nat depth)
     size_t nat_bytes = 0;   // bytes of the row buffer
     bool direct = true;     // F kernel writes the engine layout itself (no corner turn)
-    int nwg = 0;            // workgroups of the persistent direct F kernel (one per CU; 0: one block per workgroup)
+    int nwg = 0;            // workgroups of the persistent direct F kernel (one per CU)
     unsigned* qctr = nullptr;  // its per-XCD block counters
+    int fmode = FX_FSTAGE_AUTO;  // direct F kernel form (fx_corr_set_fstage)
     float* acc = nullptr;   // engine-layout accumulators
     size_t acc_floats = 0;
     fx_phase_model* pm = nullptr;
@@ -904,10 +905,14 @@ extern "C" int fx_corr_create(fx_ctx* ctx, const fx_corr_config* cfg, fx_corr** 
     c->sub = std::max<long>(8, std::min<long>(cfg->chunk_spectra, (long)(sub_target / row_bytes) / 8 * 8));
     const char* direct_env = getenv("FX_FSTAGE_DIRECT");
     c->direct = !direct_env || atoi(direct_env) != 0;
-    // FX_FSTAGE_PERSIST=0 launches the direct F kernel one block per workgroup
+    // the direct F kernel's default form: FX_FSTAGE_PERSIST=0 launches it one
+    // block per workgroup, FX_FSTAGE_PAIR=0 keeps its persistent blocks single
     const char* persist_env = getenv("FX_FSTAGE_PERSIST");
-    const bool persist = !persist_env || atoi(persist_env) != 0;
-    if (!persist || hipDeviceGetAttribute(&c->nwg, hipDeviceAttributeMultiprocessorCount, ctx->device) != hipSuccess)
+    const char* pair_env = getenv("FX_FSTAGE_PAIR");
+    c->fmode = (persist_env && atoi(persist_env) == 0) ? FX_FSTAGE_BLOCKS
+             : (pair_env && atoi(pair_env) == 0)       ? FX_FSTAGE_PERSISTENT
+                                                       : FX_FSTAGE_AUTO;
+    if (hipDeviceGetAttribute(&c->nwg, hipDeviceAttributeMultiprocessorCount, ctx->device) != hipSuccess)
         c->nwg = 0;
     c->nat_bytes = twobit ? 0 : row_bytes * (size_t)c->sub;
     const size_t nat_bytes = c->nat_bytes;
@@ -1054,6 +1059,8 @@ static int cornerturn_from(fx_corr* c, long nspectra, long s0, long chunk_spectr
     return FX_OK;
 }
 
+static bool fstage_persistent(const fx_corr* c, int out_prec);
+
 extern "C" int fx_corr_fstage(fx_corr* c, const uint32_t* d_words, size_t in_stride_words,
                               long t_first, long nspectra, void* d_chan, int out_prec)
 {
@@ -1081,7 +1088,8 @@ extern "C" int fx_corr_fstage(fx_corr* c, const uint32_t* d_words, size_t in_str
                                           c->mult, scaled && !c->sc_uniform ? c->sc_chan : nullptr,
                                           scaled && c->sc_uniform ? c->sc_uni : 1.0f, scaled ? c->sc_inp : nullptr,
                                           out_prec, d_chan, nspectra * I, 0, imajor ? nspectra : 0,
-                                          imajor ? nullptr : c->qctr, c->nwg));
+                                          fstage_persistent(c, out_prec) ? c->qctr : nullptr, c->nwg,
+                                          c->fmode == FX_FSTAGE_AUTO));
         return FX_OK;
     }
     // sub-chunks: F kernel into the natural-row buffer, corner turn into d_chan
@@ -1100,7 +1108,16 @@ extern "C" long fx_corr_fstage_depth(fx_corr* c) { return c ? c->sub : 0; }
 // kernel) for FP32/FP16 output and at most FX_PDIR_PM = 256 inputs
 static bool fstage_persistent(const fx_corr* c, int out_prec)
 {
-    return c->qctr && c->nwg >= 8 && c->cfg.ninp <= 256 && out_prec != FX_PREC_FP8;
+    return c->fmode != FX_FSTAGE_BLOCKS && c->qctr && c->nwg >= 8 && c->cfg.ninp <= 256 && out_prec != FX_PREC_FP8;
+}
+
+extern "C" int fx_corr_set_fstage(fx_corr* c, int kind)
+{
+    FX_REQUIRE(c, FX_ERR_ARG, "null correlator");
+    FX_REQUIRE(kind == FX_FSTAGE_AUTO || kind == FX_FSTAGE_PERSISTENT || kind == FX_FSTAGE_BLOCKS, FX_ERR_ARG,
+               "fx_corr_set_fstage: kind %d", kind);
+    c->fmode = kind;
+    return FX_OK;
 }
 
 extern "C" int fx_corr_fstage_launches(fx_corr* c, long nspectra, int out_prec)
@@ -1120,8 +1137,12 @@ extern "C" int fx_corr_fstage_name(fx_corr* c, long nspectra, int out_prec, char
     const char* t = out_prec == FX_PREC_F32 ? "float2" : out_prec == FX_PREC_F16 ? "__half2" : "fp8x2";
     if (c->direct && nspectra > 0 &&
         fxk_fstage_direct_ok(c->cfg.nfft, (int)I, nspectra, out_prec, nspectra * I, 0, nspectra))
-        snprintf(buf, (size_t)len, "%s<%d, %s>", fstage_persistent(c, out_prec) ? "k_fstage_direct_p" : "k_fstage_direct",
-                 M, t);
+        snprintf(buf, (size_t)len, "%s<%d, %s>%s",
+                 fstage_persistent(c, out_prec) ? "k_fstage_direct_p" : "k_fstage_direct", M, t,
+                 fstage_persistent(c, out_prec) && c->fmode == FX_FSTAGE_AUTO && out_prec == FX_PREC_F16 &&
+                         I % (2 * (16384 / M)) == 0
+                     ? " (paired)"
+                     : "");
     else
         snprintf(buf, (size_t)len, "k_fstage<%d, %s> + k_cornerturn", M, t);
     return FX_OK;

@@ -898,21 +898,13 @@ int launch_fstage_direct_t(hipStream_t s, const float2* tw, const uint32_t* word
     return hipGetLastError() == hipSuccess ? FX_OK : FX_ERR_HIP;
 }
 
-// FX_FSTAGE_PAIR=0 keeps the persistent kernel's blocks single (16-byte runs)
-static bool direct_pair()
-{
-    static const bool on = [] {
-        const char* e = getenv("FX_FSTAGE_PAIR");
-        return !e || atoi(e) != 0;
-    }();
-    return on;
-}
-
-// the persistent form: nwg = workgroups to launch (one per CU; at least 8)
+// the persistent form: nwg = workgroups to launch (one per CU; at least 8);
+// pair: FP16 blocks in pairs where the inputs allow (32-byte store runs)
 template <int M, typename T>
 int launch_fstage_direct_p(hipStream_t s, const float2* tw, const uint32_t* words, size_t stride, int ninp,
                            long t_first, long nt, const fx_phase_model* pm, const float2* mult, const float* sc,
-                           float su, const float* si, void* out, long ostride, long col0, unsigned* ctr, int nwg)
+                           float su, const float* si, void* out, long ostride, long col0, unsigned* ctr, int nwg,
+                           bool pair)
 {
     constexpr int FMX = direct_fm<T>();
     constexpr int F = FMX / M;
@@ -924,7 +916,7 @@ int launch_fstage_direct_p(hipStream_t s, const float2* tw, const uint32_t* word
     auto o = reinterpret_cast<typename ChanT<T>::V*>(out);
     // FP16 with the blocks of a spectrum in pairs: 32-byte store runs
     constexpr bool P2 = std::is_same<T, __half2>::value;
-    if (P2 && ninp % (2 * F) == 0 && direct_pair()) {
+    if (P2 && pair && ninp % (2 * F) == 0) {
         // the general epilogue also for a phase model alone: the straight-line
         // one needs more registers than the held values leave (it spills)
         if (pm)
@@ -949,19 +941,19 @@ template <int M>
 int launch_fstage_direct(hipStream_t s, const float2* tw, const uint32_t* words, size_t stride, int ninp,
                          long t_first, long nt, const fx_phase_model* pm, const float2* mult, const float* sc, float su,
                          const float* si, int out_type, void* out, long ostride, long col0, long ostride_i,
-                         unsigned* ctr, int nwg)
+                         unsigned* ctr, int nwg, bool pair)
 {
     switch (out_type) {
     case FX_PREC_F32:
         if (ctr && nwg >= 8 && ninp <= FX_PDIR_PM)
             return launch_fstage_direct_p<M, float2>(s, tw, words, stride, ninp, t_first, nt, pm, mult, sc, su, si,
-                                                     out, ostride, col0, ctr, nwg);
+                                                     out, ostride, col0, ctr, nwg, pair);
         return launch_fstage_direct_t<M, float2>(s, tw, words, stride, ninp, t_first, nt, pm, mult, sc, su, si, out,
                                                  ostride, col0, ostride_i);
     case FX_PREC_F16:
         if (ctr && nwg >= 8 && ninp <= FX_PDIR_PM)
             return launch_fstage_direct_p<M, __half2>(s, tw, words, stride, ninp, t_first, nt, pm, mult, sc, su, si,
-                                                      out, ostride, col0, ctr, nwg);
+                                                      out, ostride, col0, ctr, nwg, pair);
         return launch_fstage_direct_t<M, __half2>(s, tw, words, stride, ninp, t_first, nt, pm, mult, sc, su, si, out,
                                                   ostride, col0, ostride_i);
     case FX_PREC_FP8:
@@ -1072,12 +1064,12 @@ int fxk_cornerturn(hipStream_t s, int out_type, const void* nat, void* out, long
 int fxk_fstage_direct(hipStream_t s, const float2* tw, const uint32_t* words, size_t stride, int ninp, int n,
                       long t_first, long nt, const fx_phase_model* pm, const float2* mult, const float* sc, float su,
                       const float* si, int out_type, void* out, long ostride, long col0, long ostride_i,
-                      unsigned* ctr, int nwg)
+                      unsigned* ctr, int nwg, bool pair)
 {
     switch (n) {
-    case 4096: return launch_fstage_direct<2048>(s, tw, words, stride, ninp, t_first, nt, pm, mult, sc, su, si, out_type, out, ostride, col0, ostride_i, ctr, nwg);
-    case 8192: return launch_fstage_direct<4096>(s, tw, words, stride, ninp, t_first, nt, pm, mult, sc, su, si, out_type, out, ostride, col0, ostride_i, ctr, nwg);
-    case 16384: return launch_fstage_direct<8192>(s, tw, words, stride, ninp, t_first, nt, pm, mult, sc, su, si, out_type, out, ostride, col0, ostride_i, ctr, nwg);
+    case 4096: return launch_fstage_direct<2048>(s, tw, words, stride, ninp, t_first, nt, pm, mult, sc, su, si, out_type, out, ostride, col0, ostride_i, ctr, nwg, pair);
+    case 8192: return launch_fstage_direct<4096>(s, tw, words, stride, ninp, t_first, nt, pm, mult, sc, su, si, out_type, out, ostride, col0, ostride_i, ctr, nwg, pair);
+    case 16384: return launch_fstage_direct<8192>(s, tw, words, stride, ninp, t_first, nt, pm, mult, sc, su, si, out_type, out, ostride, col0, ostride_i, ctr, nwg, pair);
     default: return FX_ERR_ARG;
     }
 }

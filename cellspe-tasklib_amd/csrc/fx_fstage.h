@@ -34,6 +34,10 @@ __device__ __forceinline__ float2 to_chan<float2>(float2 v) { return v; }
 template <>
 __device__ __forceinline__ uint32_t to_chan<__half2>(float2 v)
 {
+    // the FP32 value first (an empty asm the compiler cannot see through), then
+    // one rounding to FP16: no fused multiply-add-to-FP16 in some epilogues
+    // and not in others
+    asm volatile("" : "+v"(v.x), "+v"(v.y));
     __half2 h = __floats2half2_rn(v.x, v.y);
     return *reinterpret_cast<uint32_t*>(&h);
 }

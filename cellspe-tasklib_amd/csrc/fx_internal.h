@@ -52,9 +52,12 @@ void fx_set_error(const char* fmt, ...);
 // ---------------------------------------------------------------------------
 // Device helpers
 // ---------------------------------------------------------------------------
+// Complex product with its two roundings fixed (one product, then one fused
+// multiply-add per component), so every kernel that inlines it rounds the
+// same way whatever the surrounding code lets the compiler contract.
 __device__ __forceinline__ float2 cmul(float2 a, float2 b)
 {
-    return make_float2(a.x * b.x - a.y * b.y, a.y * b.x + a.x * b.y);
+    return make_float2(__builtin_fmaf(a.x, b.x, -(a.y * b.y)), __builtin_fmaf(a.y, b.x, a.x * b.y));
 }
 __device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
 __device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
@@ -440,11 +443,13 @@ int fxk_fstage(hipStream_t s, const float2* tw, const uint32_t* words, size_t in
 // lengths n = 4096, 8192, 16384 with F = 32768/n items per workgroup;
 // fxk_fstage_direct_ok says whether a launch of these shapes is supported
 // nwg: workgroups of the persistent form (one per CU; FP32/FP16 out, at most
-// 256 inputs), or 0 for one block per workgroup
+// 256 inputs; ctr its 8 block counters), or 0 for one block per workgroup;
+// pair: the persistent form takes FP16 blocks in pairs where the inputs allow
 int fxk_fstage_direct(hipStream_t s, const float2* tw, const uint32_t* words, size_t in_stride_words,
                       int ninp, int n, long t_first, long nt, const fx_phase_model* pm,
                       const float2* mult, const float* scale_chan, float scale_uni, const float* scale_inp,
-                      int out_type, void* out, long ostride, long col0, long ostride_i, unsigned* ctr, int nwg);
+                      int out_type, void* out, long ostride, long col0, long ostride_i, unsigned* ctr, int nwg,
+                      bool pair);
 bool fxk_fstage_direct_ok(int n, int ninp, long nt, int out_type, long ostride, long col0, long ostride_i);
 // corner turn of one sub-chunk of F-stage rows nat[g][c] (nitems items of ns
 // spectra) into the chunk layout: FP32/FP16 out[c][col0 + g] with rows of

@@ -21,6 +21,7 @@ LIB_PATH = os.environ.get("CELLSPE_FX_LIB") or os.path.join(HERE, "libcellspe_fx
 FX_OK = 0
 FX_PREC_F32, FX_PREC_F16, FX_PREC_FP8, FX_PREC_FP8_2BIT, FX_PREC_FP4_2BIT = 0, 1, 2, 3, 4
 FX_XENGINE_AUTO, FX_XENGINE_PREFETCH = 0, 1
+FX_FSTAGE_AUTO, FX_FSTAGE_PERSISTENT, FX_FSTAGE_BLOCKS = 0, 1, 2
 FX_BATCH_EAGER, FX_BATCH_GRAPH, FX_BATCH_PERSISTENT, FX_BATCH_PHASED = 0, 1, 2, 3
 (FX_WIN_NONE, FX_WIN_SIN2, FX_WIN_COSINE, FX_WIN_COSINE2, FX_WIN_HAMMING, FX_WIN_BLACKMAN,
  FX_WIN_HANN) = range(7)
@@ -142,6 +143,7 @@ def load():
         "fx_corr_fstage_launches": (i, [vp, l, i]),
         "fx_corr_tune": (i, [vp, l, i]),
         "fx_corr_set_xengine": (i, [vp, i]),
+        "fx_corr_set_fstage": (i, [vp, i]),
         "fx_corr_xengine_name": (i, [vp, vp, l, C.c_char_p, i]),
         "fx_corr_fstage_name": (i, [vp, l, i, C.c_char_p, i]),
         "fx_pcal_final_async": (i, [vp]),
@@ -540,6 +542,12 @@ class Correlator:
         """FX_XENGINE_AUTO (LDS-ring engines where they apply) or
         FX_XENGINE_PREFETCH (the register-prefetch engine for every shape)."""
         check(load().fx_corr_set_xengine(self.handle, kind))
+
+    def set_fstage(self, kind):
+        """FX_FSTAGE_AUTO (persistent, FP16 blocks paired where the inputs
+        allow), FX_FSTAGE_PERSISTENT (persistent, single blocks) or
+        FX_FSTAGE_BLOCKS (one block per workgroup) for the direct F kernel."""
+        check(load().fx_corr_set_fstage(self.handle, kind))
 
     def xengine_name(self, nspectra, d_chan=None):
         """Name of the X-engine kernel that runs on nspectra spectra."""

@@ -396,6 +396,15 @@ int fx_corr_tune(fx_corr* corr, long sub_spectra, int xtile);
  * every shape.  Both give bit-identical sums. */
 enum { FX_XENGINE_AUTO = 0, FX_XENGINE_PREFETCH = 1 };
 int fx_corr_set_xengine(fx_corr* corr, int kind);
+/* Form of the direct F kernel (FFT lengths 4096-16384): FX_FSTAGE_AUTO
+ * (default) runs its persistent form, one workgroup per CU walking its XCD's
+ * blocks from a counter, with FP16 blocks in pairs where the inputs are a
+ * multiple of 2F (32-byte store runs); FX_FSTAGE_PERSISTENT the persistent
+ * form with single blocks; FX_FSTAGE_BLOCKS one block per workgroup.  The
+ * same values in every form (the environment variables FX_FSTAGE_PERSIST=0
+ * and FX_FSTAGE_PAIR=0 set the default at creation). */
+enum { FX_FSTAGE_AUTO = 0, FX_FSTAGE_PERSISTENT = 1, FX_FSTAGE_BLOCKS = 2 };
+int fx_corr_set_fstage(fx_corr* corr, int kind);
 /* name of the X-engine kernel fx_corr_xengine / fx_corr_process runs on
  * nspectra spectra of d_chan (NULL: the correlator's own channel buffer) */
 int fx_corr_xengine_name(fx_corr* corr, const void* d_chan, long nspectra, char* buf, int len);
