@@ -103,19 +103,23 @@ def cpu_obj(value, unit, sample, extra=None):
     return o
 
 
-def measured_valu(kernel):
-    """VALU busy fraction of `kernel` from the committed rocprofv3 counter
-    summaries (lines 'valu_busy <kernel> <fraction>'), or None."""
+def measured_busy(kernel, unit="valu"):
+    """VALU (or MFMA) busy fraction of `kernel` from the committed rocprofv3
+    counter summaries (lines '<unit>_busy <kernel> <fraction>'), or None."""
     pdir = os.path.join(ROOT, "profiles")
     if not os.path.isdir(pdir):
         return None
     for fn in sorted(os.listdir(pdir)):
         with open(os.path.join(pdir, fn)) as f:
             for line in f:
-                m = re.match(r"^valu_busy\s+(\S.*\S)\s+([0-9.eE+]+)\s*$", line)
+                m = re.match(r"^" + unit + r"_busy\s+(\S.*\S)\s+([0-9.eE+]+)\s*$", line)
                 if m and m.group(1) == kernel:
                     return {"frac": float(m.group(2)), "source": f"profiles/{fn}"}
     return None
+
+
+def measured_valu(kernel):
+    return measured_busy(kernel, "valu")
 
 
 def measured_traffic(kernel):
@@ -308,6 +312,9 @@ def large_n(fxlib, ctx, args):
     vis.free()
     flop = 8.0 * cmac
     out["ms"] = res
+    out["mfma_busy"] = {k: measured_busy(n, "mfma") for k, n in
+                        (("fp4_2bit", "k_xengine4 (FP4 2-bit)"), ("fp8_2bit", "k_xengine8<3> (FP8 2-bit)"),
+                         ("f16", "k_xengine_ring2<F16>"), ("f32", "k_xengine_ring2<F32>"))}
     out["complex_mac_per_s"] = {k: cmac / (v * 1e-3) for k, v in res.items()}
     out["tflops_equiv"] = {k: flop / (v * 1e-3) / 1e12 for k, v in res.items()}
     out["speedup_vs_fp32"] = ({k: res["f32"] / v for k, v in res.items() if k != "f32"}
@@ -1052,6 +1059,7 @@ def main():
         for k, v in kt.items():
             kern[f"{k}_{name}"] = v
     kern["xengine_f16_tflops"] = x_flop / (k16["xengine"] * 1e-3) / 1e12
+    kern["xengine_f16_mfma_busy"] = measured_busy(k16["xengine_kernel"], "mfma")
     if "f32" in paths:
         kern["xengine_f32_tflops"] = x_flop / (paths["f32"][1]["xengine"] * 1e-3) / 1e12
     if "fp8" in paths:
