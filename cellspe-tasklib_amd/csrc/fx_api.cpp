@@ -916,7 +916,9 @@ extern "C" int fx_corr_create(fx_ctx* ctx, const fx_corr_config* cfg, fx_corr** 
         c->nwg = 0;
     c->nat_bytes = twobit ? 0 : row_bytes * (size_t)c->sub;
     const size_t nat_bytes = c->nat_bytes;
-    c->xtile = 128;
+    // FP32/FP16 engines: 64 x 64 tiles for at most 64 inputs (a 128 x 128 tile
+    // would be at least three quarters padding there: 4x the MFMA work)
+    c->xtile = (!twobit && cfg->prec != FX_PREC_FP8 && cfg->ninp <= 64) ? 64 : 128;
     c->ring = fxk_xengine_ring_default();
     c->acc_floats = fxk_xengine_acc_floats(c->nchan, cfg->ninp, c->nsets, c->xtile);
     long P = (long)cfg->ninp * (cfg->ninp + 1) / 2;
