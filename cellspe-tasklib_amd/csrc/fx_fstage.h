@@ -219,22 +219,29 @@ __device__ __forceinline__ void fstage_group(const uint32_t* __restrict__ words,
             const fx_phase_model pmi = pm[i];
             const double tg = (double)(t_first + t);
             const double tau = pmi.delay0 + pmi.delay_rate * tg;
+            // the phase in cycles, accumulated in FP64 (1/N is exact: N is a
+            // power of two) and reduced to [-1/2, 1/2]
             double cyc;
             if (j < PLO) {
-                cyc = (double)j * tau / (double)N;
+                cyc = (double)j * tau * (1.0 / (double)N);
             } else {
-                cyc = (double)((j - PLO) * PLO) * tau / (double)N +
+                cyc = (double)((j - PLO) * PLO) * tau * (1.0 / (double)N) +
                       (pmi.fringe0 + pmi.fringe_rate * tg) * 0.15915494309189535;
             }
             cyc -= rint(cyc);
-            double sn, cs;
-            sincospi(2.0 * cyc, &sn, &cs);
+            // then its sine and cosine in FP32 on the transcendental unit
+            // (v_sin_f32 / v_cos_f32 take the angle in cycles): the reduced phase
+            // in FP32 and its sincos are exact to a few 1e-8, the reference's
+            // FP64-phase / FP32-sincos convention, at a tenth of an FP64
+            // sincospi per table entry
+            const float cf = (float)cyc;
+            const float sn = __builtin_amdgcn_sinf(cf), cs = __builtin_amdgcn_cosf(cf);
             if (j < PLO) {
                 // the item amplitude, as it.amp[f] holds it
-                const double a = (double)(0.5f * sc_uni * (sc_inp ? sc_inp[i] : 1.0f));
-                it.ph_lo[f][j] = make_float2((float)(cs * a), (float)(sn * a));
+                const float a = 0.5f * sc_uni * (sc_inp ? sc_inp[i] : 1.0f);
+                it.ph_lo[f][j] = make_float2(cs * a, sn * a);
             } else {
-                it.ph_hi[f][j - PLO] = make_float2((float)cs, (float)sn);
+                it.ph_hi[f][j - PLO] = make_float2(cs, sn);
             }
         }
     }
