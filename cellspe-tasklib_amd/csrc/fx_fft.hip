@@ -550,10 +550,12 @@ __global__ __launch_bounds__(FMX / 16) void k_fstage_direct_p(
     const int wf = wl / WPT;
     const int wof = wl - wf * WPT;
     uint2 wpre[WTH / 2];
+    // (blocks are the same in every thread: the block's spectrum and first
+    // input come from one scalar division; its items are inputs i0 .. i0+F-1)
     auto wload = [&](long blk) {
-        const long g = blk * F + wf;
+        const long g = uniform_long(blk) * F;
         const long t = g / ninp;
-        const int i = (int)(g - t * ninp);
+        const int i = (int)(g - t * ninp) + wf;
         const uint32_t* src = words + (size_t)i * in_stride + (size_t)(t_first + t) * WPT + wof;
 #pragma unroll
         for (int q = 0; q < WTH; q += 2) wpre[q / 2] = *reinterpret_cast<const uint2*>(src + q);
@@ -594,7 +596,7 @@ __global__ __launch_bounds__(FMX / 16) void k_fstage_direct_p(
         // one atomic and does not rewrite into a wave-combined add whose
         // result is needed at once
         if (SUB != 2 && tid == 0) nb = atomicInc(&ctr[x], 0xffffffffu);
-        fstage_group<M, F, NT, false>(words, in_stride, ninp, t_first, nt, pml, sc_uni, sis, gb, sh, tb, it, tl,
+        fstage_group<M, F, NT, false, true>(words, in_stride, ninp, t_first, nt, pml, sc_uni, sis, gb, sh, tb, it, tl,
                                       [&](auto p) {
             // published after the second pass, visible after the third's barrier
             if constexpr (decltype(p)::value == 1 && SUB != 2) {

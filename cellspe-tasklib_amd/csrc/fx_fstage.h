@@ -162,6 +162,15 @@ struct FConstsPre {
     }
 };
 
+// A value the caller knows to be the same in every lane, moved to scalar
+// registers (the integer arithmetic on it then runs on the scalar unit).
+__device__ __forceinline__ long uniform_long(long v)
+{
+    const int lo = __builtin_amdgcn_readfirstlane((int)(v & 0xffffffffL));
+    const int hi = __builtin_amdgcn_readfirstlane((int)(v >> 32));
+    return (long)(((unsigned long)(unsigned)hi << 32) | (unsigned long)(unsigned)lo);
+}
+
 // (spectrum t, input i) of item g of a chunk of nt spectra x ninp inputs
 // (g = t*I + i, or g = i*nt + t when IM); false past the chunk's last item.
 template <bool IM>
@@ -186,7 +195,7 @@ __device__ __forceinline__ bool fitem(long g, long nt, int ninp, long nitems, lo
 // tables are in flight together.  Every thread derives the (spectrum, input)
 // of the items it needs itself: no barrier before stage.  The overload
 // without pre/stage loads the words from `words`.
-template <int M, int F, int NT, bool IM, typename Hook, typename Pre, typename Stage>
+template <int M, int F, int NT, bool IM, bool ONEROW = false, typename Hook, typename Pre, typename Stage>
 __device__ __forceinline__ void fstage_group(const uint32_t* __restrict__ words, size_t in_stride, int ninp,
                                              long t_first, long nt, const fx_phase_model* __restrict__ pm,
                                              float sc_uni, const float* __restrict__ sc_inp, long g0, float2* sh,
@@ -199,12 +208,33 @@ __device__ __forceinline__ void fstage_group(const uint32_t* __restrict__ words,
     constexpr int PLO = PhaseTab<M>::PLO;
     constexpr int PHI = PhaseTab<M>::PHI;
     static_assert(WTH % 2 == 0 && M >= 128, "F kernel shape");
+    static_assert(!(ONEROW && IM), "one-row groups are input-fastest");
     const long nitems = nt * ninp;
     pre();
+    // ONEROW: the caller guarantees that g0 is the same in every thread and
+    // that the group's items are inputs i0 .. i0 + F - 1 of one spectrum t0 (F
+    // divides the input count): one division for the group, on the scalar
+    // unit, instead of one per thread and use
+    long t0r = 0;
+    int i0r = 0;
+    if constexpr (ONEROW) {
+        const long gu = uniform_long(g0);
+        t0r = gu / ninp;
+        i0r = (int)(gu - t0r * ninp);
+    }
+    auto item = [&](int f, long& t, int& i) -> bool {
+        if constexpr (ONEROW) {
+            t = t0r;
+            i = i0r + f;
+            return g0 + f < nitems;
+        } else {
+            return fitem<IM>(g0 + f, nt, ninp, nitems, t, i);
+        }
+    };
     if (tid < F) {
         long t;
         int i;
-        const bool v = fitem<IM>(g0 + tid, nt, ninp, nitems, t, i);
+        const bool v = item(tid, t, i);
         it.t[tid] = v ? (int)t : -1;
         it.i[tid] = i;
         it.amp[tid] = 0.5f * sc_uni * ((sc_inp && v) ? sc_inp[i] : 1.0f);
@@ -215,7 +245,7 @@ __device__ __forceinline__ void fstage_group(const uint32_t* __restrict__ words,
             const int j = e - f * (PLO + PHI);
             long t;
             int i;
-            if (!fitem<IM>(g0 + f, nt, ninp, nitems, t, i)) continue;
+            if (!item(f, t, i)) continue;
             const fx_phase_model pmi = pm[i];
             const double tg = (double)(t_first + t);
             const double tau = pmi.delay0 + pmi.delay_rate * tg;
