@@ -258,8 +258,13 @@ def large_n(fxlib, ctx, args):
         c8.close()
         res[name], vis_sample[name] = ms, v
         log(f"[bench] large-N {name} (exact 2-bit decomposition): {ms:.2f} ms")
-    for name, prec, esz, reps in (("f16", fxlib.FX_PREC_F16, 4, args.steps),
-                                  ("f32", fxlib.FX_PREC_F32, 8, 1)):
+    knames = {"fp4_2bit": "k_xengine4 (FP4 2-bit)", "fp8_2bit": "k_xengine8<3> (FP8 2-bit)"}
+    # f16_lockstep: the FP16 engine in lockstep groups (HBM fetches of the
+    # shared input blocks once instead of about three times; bit-identical)
+    for name, prec, esz, reps, kind in (("f16", fxlib.FX_PREC_F16, 4, args.steps, fxlib.FX_XENGINE_AUTO),
+                                        ("f16_lockstep", fxlib.FX_PREC_F16, 4, args.steps,
+                                         fxlib.FX_XENGINE_LOCKSTEP),
+                                        ("f32", fxlib.FX_PREC_F32, 8, 1, fxlib.FX_XENGINE_AUTO)):
         if name == "f32" and args.no_fp32:
             continue
         dx = ctx.alloc(C * T * I * esz)
@@ -267,11 +272,13 @@ def large_n(fxlib, ctx, args):
         cc = fxlib.Correlator(ctx, I, 2 * C, 8, prec)
         # the level values go in unscaled: unit power scales in the dump
         cc.set_scales(np.ones(C, np.float32), np.ones(I, np.float32))
+        cc.set_xengine(kind)
+        knames[name] = cc.xengine_name(T, dx)
         ms, v = timed(cc, dx, reps)
         cc.close()
         dx.free()
         res[name], vis_sample[name] = ms, v
-        log(f"[bench] large-N {name}: {ms:.2f} ms")
+        log(f"[bench] large-N {name}: {ms:.2f} ms ({knames[name]})")
     # FP64 reference on the sample channels (numpy, level-table values)
     lut = np.array([-3.3359, -1.0, 1.0, 3.3359], np.float32).astype(np.float64)
     iu = np.triu_indices(I)
@@ -312,9 +319,8 @@ def large_n(fxlib, ctx, args):
     vis.free()
     flop = 8.0 * cmac
     out["ms"] = res
-    out["mfma_busy"] = {k: measured_busy(n, "mfma") for k, n in
-                        (("fp4_2bit", "k_xengine4 (FP4 2-bit)"), ("fp8_2bit", "k_xengine8<3> (FP8 2-bit)"),
-                         ("f16", "k_xengine_ring2<F16>"), ("f32", "k_xengine_ring2<F32>"))}
+    out["kernels"] = knames
+    out["mfma_busy"] = {k: measured_busy(n, "mfma") for k, n in knames.items()}
     out["complex_mac_per_s"] = {k: cmac / (v * 1e-3) for k, v in res.items()}
     out["tflops_equiv"] = {k: flop / (v * 1e-3) / 1e12 for k, v in res.items()}
     out["speedup_vs_fp32"] = ({k: res["f32"] / v for k, v in res.items() if k != "f32"}

@@ -832,7 +832,7 @@ struct fx_corr {
     size_t nat_bytes = 0;   // bytes of the row buffer
     bool direct = true;     // F kernel writes the engine layout itself (no corner turn)
     int nwg = 0;            // workgroups of the persistent direct F kernel (one per CU)
-    unsigned* qctr = nullptr;  // its per-XCD block counters
+    unsigned* qctr = nullptr;  // its per-XCD block counters, then the X-engine's lockstep words
     int fmode = FX_FSTAGE_AUTO;  // direct F kernel form (fx_corr_set_fstage)
     float* acc = nullptr;   // engine-layout accumulators
     size_t acc_floats = 0;
@@ -846,7 +846,8 @@ struct fx_corr {
     long long* coef = nullptr;  // FP8_2BIT / FP4_2BIT: integer coefficients of the exact combination
     int nsets = 1;
     int xtile = 128;        // X-engine input tile (128, or 64 for FP32/FP16 engines)
-    bool ring = true;       // the LDS-ring X-engines may run (FX_XENGINE_AUTO)
+    bool ring = true;       // the LDS-ring X-engines may run (FX_XENGINE_AUTO, FX_XENGINE_LOCKSTEP)
+    bool xlock = false;     // the large-input ring engine may run in lockstep groups (FX_XENGINE_LOCKSTEP)
     long spectra_done = 0;
 };
 
@@ -857,7 +858,7 @@ static int corr_engine(fx_corr* c, const void* d_chan, long nt)
 {
     if (c->cfg.prec == FX_PREC_F32 || c->cfg.prec == FX_PREC_F16)
         return fxk_xengine(c->ctx->stream, c->cfg.prec, c->xtile, d_chan, c->nchan, nt, c->cfg.ninp, c->acc,
-                           c->ring);
+                           c->ring, c->xlock && c->qctr ? c->qctr + 8 : nullptr);
     return fxk_xengine8(c->ctx->stream, c->cfg.prec, d_chan, c->nchan, nt, c->cfg.ninp, c->acc);
 }
 
@@ -920,6 +921,7 @@ extern "C" int fx_corr_create(fx_ctx* ctx, const fx_corr_config* cfg, fx_corr** 
     // would be at least three quarters padding there: 4x the MFMA work)
     c->xtile = (!twobit && cfg->prec != FX_PREC_FP8 && cfg->ninp <= 64) ? 64 : 128;
     c->ring = fxk_xengine_ring_default();
+    c->xlock = fxk_xengine_lock_default();
     c->acc_floats = fxk_xengine_acc_floats(c->nchan, cfg->ninp, c->nsets, c->xtile);
     long P = (long)cfg->ninp * (cfg->ninp + 1) / 2;
     std::vector<int2> hp((size_t)P);
@@ -928,7 +930,8 @@ extern "C" int fx_corr_create(fx_ctx* ctx, const fx_corr_config* cfg, fx_corr** 
     bool ok = hipMalloc(&c->acc, c->acc_floats * sizeof(float)) == hipSuccess &&
               hipMalloc(&c->pm, sizeof(fx_phase_model) * (size_t)cfg->ninp) == hipSuccess &&
               hipMalloc(&c->pij, sizeof(int2) * (size_t)P) == hipSuccess;
-    if (ok && c->nwg >= 8) ok = hipMalloc(&c->qctr, 8 * sizeof(unsigned)) == hipSuccess;
+    if (ok && c->nwg >= 8)
+        ok = hipMalloc(&c->qctr, (8 + (size_t)fxk_xengine_prog_words()) * sizeof(unsigned)) == hipSuccess;
     if (ok && chan_bytes)
         ok = hipMalloc(&c->chan, chan_bytes) == hipSuccess && hipMalloc(&c->nat, nat_bytes) == hipSuccess;
     if (ok) ok = hipMemcpy(c->pij, hp.data(), sizeof(int2) * (size_t)P, hipMemcpyHostToDevice) == hipSuccess;
@@ -1285,9 +1288,10 @@ extern "C" int fx_corr_tune(fx_corr* c, long sub_spectra, int xtile)
 extern "C" int fx_corr_set_xengine(fx_corr* c, int kind)
 {
     FX_REQUIRE(c, FX_ERR_ARG, "null correlator");
-    FX_REQUIRE(kind == FX_XENGINE_AUTO || kind == FX_XENGINE_PREFETCH, FX_ERR_ARG, "fx_corr_set_xengine: kind %d",
-               kind);
-    c->ring = kind == FX_XENGINE_AUTO;
+    FX_REQUIRE(kind == FX_XENGINE_AUTO || kind == FX_XENGINE_PREFETCH || kind == FX_XENGINE_LOCKSTEP, FX_ERR_ARG,
+               "fx_corr_set_xengine: kind %d", kind);
+    c->ring = kind != FX_XENGINE_PREFETCH;
+    c->xlock = kind == FX_XENGINE_LOCKSTEP;
     return FX_OK;
 }
 
@@ -1298,8 +1302,9 @@ extern "C" int fx_corr_xengine_name(fx_corr* c, const void* d_chan, long nspectr
                      : c->cfg.prec == FX_PREC_FP8_2BIT ? "k_xengine8<3>"
                      : c->cfg.prec == FX_PREC_FP4_2BIT ? "k_xengine4"
                                                          : fxk_xengine_name(c->cfg.prec, c->xtile,
-                                                                            d_chan ? d_chan : c->chan, nspectra,
-                                                                            c->cfg.ninp, c->ring);
+                                                                            d_chan ? d_chan : c->chan, c->nchan,
+                                                                            nspectra, c->cfg.ninp, c->ring,
+                                                                            c->xlock && c->qctr);
     snprintf(buf, (size_t)len, "%s", name);
     return FX_OK;
 }

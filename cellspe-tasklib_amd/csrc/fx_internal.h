@@ -459,11 +459,19 @@ int fxk_cornerturn(hipStream_t s, int out_type, const void* nat, void* out, long
                    long ostride, long col0, int ns, long ostride_i);
 // FP32/FP16 engine on [c][t][i], ts x ts input tiles (ts = 128 or 64); ring:
 // the LDS-ring engines may run (whole 128-input tiles and whole stages),
-// else the register-prefetch engine runs for every shape
+// else the register-prefetch engine runs for every shape; prog (null: none)
+// holds fxk_xengine_prog_words() words for the lockstep groups of the
+// large-input ring engine (cleared by a fill kernel before each launch)
 int fxk_xengine(hipStream_t s, int in_type, int ts, const void* X, int nchan, long nt, int ninp,
-                float* acc, bool ring);
+                float* acc, bool ring, unsigned* prog);
+int fxk_xengine_prog_words();
 // name of the kernel fxk_xengine launches for these arguments ("" if none)
-const char* fxk_xengine_name(int in_type, int ts, const void* X, long nt, int ninp, bool ring);
+// (lock: the large-input ring engine may run in lockstep groups, as with a
+// non-null prog above)
+const char* fxk_xengine_name(int in_type, int ts, const void* X, int nchan, long nt, int ninp, bool ring,
+                             bool lock);
+// FX_XENGINE_LOCK=1 in the environment turns the lockstep groups on by default
+bool fxk_xengine_lock_default();
 // FX_XENGINE_RING=0 in the environment turns the ring engines off by default
 bool fxk_xengine_ring_default();
 // 8-bit X-engine on input-major channelized data X[c][i][t]: FX_PREC_FP8

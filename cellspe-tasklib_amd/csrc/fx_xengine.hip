@@ -225,10 +225,28 @@ __global__ __launch_bounds__(XNT, 2) void k_xengine_ring(const typename XTraits<
 // are ninp apart in HBM, so each lane computes its own source address.  The
 // waves split the tile into quadrants as k_xengine<PREC, 128> does (same
 // products in the same order, same engine image).
+//
+// LOCK: the workgroups form groups of ntl (the tiles of one channel), all of
+// a group on one XCD (blocks b = x + 8 (g ntl + tile) of XCD x); a group
+// walks the channels grp, grp + 8 ngx, ... together.  Each 128-input block of
+// a channel is read by 5 tile slabs; kept within a few stages of each other,
+// the group's workgroups find the blocks their siblings fetched still in the
+// XCD's L2, where tiles scheduled independently drift apart by more than the
+// L2 holds.  Every XLP stages wave 0 publishes the stages it has consumed in
+// prog[grp ntl + tile] and, if a sibling was more than lag stages behind at
+// the previous round, polls until it has caught up; the other waves wait at
+// the stage barrier.  The waits only order the fetches -- results do not
+// depend on them -- and are bounded: after XSPIN polls the workgroup stops
+// waiting for the rest of the launch.  The host sizes the grid so that every
+// group is resident at once (its tiles' workgroups fit one XCD's CUs).
 // ---------------------------------------------------------------------------
-template <int PREC>
+constexpr int XLP = 4;             // stages per progress round
+constexpr int XSPIN = 1 << 14;     // polls of a wait before lockstep is dropped
+
+template <int PREC, bool LOCK>
 __global__ __launch_bounds__(XNT, 2) void k_xengine_ring2(const typename XTraits<PREC>::E* __restrict__ X,
-                                                          int nchan, long nt, int ninp, float* __restrict__ acc)
+                                                          int nchan, long nt, int ninp, float* __restrict__ acc,
+                                                          unsigned* __restrict__ prog, int ngx, unsigned lag)
 {
     typedef typename XTraits<PREC>::E E;
     constexpr int TSZ = 128, NBW = 2, NBT = 16;
@@ -245,10 +263,21 @@ __global__ __launch_bounds__(XNT, 2) void k_xengine_ring2(const typename XTraits
 
     const int nb = ninp / TSZ;
     const int ntl = nb * (nb + 1) / 2;
-    const long lb = xcd_block((long)ntl * nchan);
-    if (lb >= (long)ntl * nchan) return;
-    const int c = (int)(lb / ntl);
-    const int tile = (int)(lb - (long)c * ntl);
+    int c0, tile, cstep, grp = 0;
+    if constexpr (LOCK) {
+        const int b = blockIdx.x, l = b >> 3;
+        const int gx = l / ntl;
+        tile = l - gx * ntl;
+        grp = (b & 7) + 8 * gx;
+        c0 = grp;
+        cstep = 8 * ngx;
+    } else {
+        const long lb = xcd_block((long)ntl * nchan);
+        if (lb >= (long)ntl * nchan) return;
+        c0 = (int)(lb / ntl);
+        tile = (int)(lb - (long)c0 * ntl);
+        cstep = nchan;
+    }
     int ti, tj;
     tile_coords(tile, nb, ti, tj);
     const bool diag = ti == tj;
@@ -256,117 +285,159 @@ __global__ __launch_bounds__(XNT, 2) void k_xengine_ring2(const typename XTraits
     const int w = tid >> 6, lane = tid & 63;
     const int qi = w >> 1, qj = w & 1;
     const bool active = !(diag && qi > qj);
-    const E* base = X + (size_t)c * (size_t)nt * (size_t)ninp;
     const long nstage = nt / KT;
     // this lane's part of every 1 KB load: time row within the block, column
     const int trow = lane / LPR, col = (lane % LPR) * CHE;
+    unsigned* const sib = prog + (size_t)grp * ntl;
+    bool lock = LOCK;
+    unsigned seen = 0;      // (wave 0, lanes < ntl) the siblings' progress at the last round
+    auto peek = [&]() { return __hip_atomic_load(sib + lane, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); };
+    unsigned sg = 0;        // stages this workgroup has consumed, all channels
+    for (int c = c0; c < nchan; c += cstep) {
+        if (c != c0) __syncthreads();   // every wave is done with the previous channel's ring
+        const E* base = X + (size_t)c * (size_t)nt * (size_t)ninp;
 
-    auto issue = [&](long st) {
+        // the loads of stage st into its ring slot; past the last stage the
+        // last stage's samples again (every stage issues NPER loads per wave,
+        // so one wait count fits every stage, and the compiler can count them)
+        auto issue = [&](long st) {
+            const long ss = st < nstage ? st : nstage - 1;
 #pragma unroll
-        for (int i = 0; i < NPER; i++) {
-            const int blk = w * NPER + i;                // 0 .. 2 BPS - 1
-            const int sl = blk / BPS, b = blk - sl * BPS;
-            const long t = st * KT + b * RPB + trow;
-            const char* src = reinterpret_cast<const char*>(base + (size_t)t * ninp + (sl ? tj : ti) * TSZ + col);
-            char* dst = reinterpret_cast<char*>(&slab[st % RING][sl][0][0]) + b * 1024;
-            __builtin_amdgcn_global_load_lds(src, (__attribute__((address_space(3))) void*)dst, 16, 0, 0);
-        }
-    };
+            for (int i = 0; i < NPER; i++) {
+                const int blk = w * NPER + i;                // 0 .. 2 BPS - 1
+                const int sl = blk / BPS, b = blk - sl * BPS;
+                const long t = ss * KT + b * RPB + trow;
+                const char* src =
+                    reinterpret_cast<const char*>(base + (size_t)t * ninp + (sl ? tj : ti) * TSZ + col);
+                char* dst = reinterpret_cast<char*>(&slab[st % RING][sl][0][0]) + b * 1024;
+                __builtin_amdgcn_global_load_lds(src, (__attribute__((address_space(3))) void*)dst, 16, 0, 0);
+            }
+        };
 
-    fx_float16 dre[NBW][NBW], dim[NBW][NBW];
+        fx_float16 dre[NBW][NBW], dim[NBW][NBW];
 #pragma unroll
-    for (int a = 0; a < NBW; a++)
+        for (int a = 0; a < NBW; a++)
 #pragma unroll
-        for (int b = 0; b < NBW; b++) {
-            dre[a][b] = fx_float16{};
-            dim[a][b] = fx_float16{};
+            for (int b = 0; b < NBW; b++) {
+                dre[a][b] = fx_float16{};
+                dim[a][b] = fx_float16{};
+            }
+        const int r32 = lane & 31, h = lane >> 5;
+        for (long st = 0; st < RING - 1; st++) issue(st);
+        // LOCK: rounds of XLP stages unrolled (the host ensures nstage % XLP
+        // == 0), so the round's progress load sits at one place in straight
+        // code and the compiler can count the loads issued after it
+        constexpr int SPR = LOCK ? XLP : 1;
+        for (long st0 = 0; st0 < nstage; st0 += SPR) {
+#pragma unroll
+            for (int k = 0; k < SPR; k++, sg++) {
+                const long st = st0 + k;
+                // this wave's loads of stage st are done once at most the two
+                // later stages' loads are outstanding
+                wait_vmcnt<(RING - 2) * NPER>();
+                if constexpr (LOCK) {
+                    if (k == 0 && lock && w == 0 && sg) {
+                        const unsigned need = sg > lag ? sg - lag : 0u;
+                        // seen was loaded a round ago (the wait above has retired it:
+                        // three stages' loads followed it); if a sibling trails, poll
+                        // until it catches up
+                        if (!__all(lane >= ntl || seen >= need)) {
+                            for (int n = 0;; n++) {
+                                if (n == XSPIN) {
+                                    lock = false;
+                                    break;
+                                }
+                                __builtin_amdgcn_s_sleep(1);
+                                const unsigned v = lane < ntl ? peek() : ~0u;
+                                if (__all(v >= need)) break;
+                            }
+                        }
+                        if (lane == 0) __hip_atomic_store(sib + tile, sg, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        if (lane < ntl) seen = peek();
+                    }
+                }
+                asm volatile("s_barrier" ::: "memory");
+                issue(st + RING - 1);
+                if (!active) continue;
+                const E (*sa)[TSZ] = slab[st % RING][0];
+                const E (*sb)[TSZ] = slab[st % RING][1];
+                if constexpr (PREC == FX_PREC_F16) {
+#pragma unroll
+                    for (int ks = 0; ks < KT / 8; ks++) {
+                        fx_half8 fa[NBW], fb[NBW], fr[NBW];
+#pragma unroll
+                        for (int a = 0; a < NBW; a++) {
+                            fx_int4v ia;
+                            const int row = (NBW * qi + a) * 32 + r32;
+#pragma unroll
+                            for (int q = 0; q < 4; q++) ia[q] = (int)sa[ks * 8 + 4 * h + q][row];
+                            fa[a] = __builtin_bit_cast(fx_half8, ia);
+                        }
+#pragma unroll
+                        for (int b = 0; b < NBW; b++) {
+                            fx_int4v ib, ir;
+                            const int row = (NBW * qj + b) * 32 + r32;
+#pragma unroll
+                            for (int q = 0; q < 4; q++) {
+                                uint32_t v = sb[ks * 8 + 4 * h + q][row];
+                                ib[q] = (int)v;
+                                ir[q] = (int)rot_h2(v);
+                            }
+                            fb[b] = __builtin_bit_cast(fx_half8, ib);
+                            fr[b] = __builtin_bit_cast(fx_half8, ir);
+                        }
+#pragma unroll
+                        for (int a = 0; a < NBW; a++)
+#pragma unroll
+                            for (int b = 0; b < NBW; b++) {
+                                if (diag && qi == qj && a > b) continue;
+                                dre[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_f16(fa[a], fb[b], dre[a][b], 0, 0, 0);
+                                dim[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_f16(fa[a], fr[b], dim[a][b], 0, 0, 0);
+                            }
+                    }
+                } else {
+#pragma unroll
+                    for (int ks = 0; ks < KT; ks++) {
+                        float fa[NBW], fb[NBW], fr[NBW];
+#pragma unroll
+                        for (int a = 0; a < NBW; a++) {
+                            const float2 v = sa[ks][(NBW * qi + a) * 32 + r32];
+                            fa[a] = h ? v.y : v.x;
+                        }
+#pragma unroll
+                        for (int b = 0; b < NBW; b++) {
+                            const float2 v = sb[ks][(NBW * qj + b) * 32 + r32];
+                            fb[b] = h ? v.y : v.x;
+                            fr[b] = h ? v.x : -v.y;
+                        }
+#pragma unroll
+                        for (int a = 0; a < NBW; a++)
+#pragma unroll
+                            for (int b = 0; b < NBW; b++) {
+                                if (diag && qi == qj && a > b) continue;
+                                dre[a][b] = __builtin_amdgcn_mfma_f32_32x32x2f32(fa[a], fb[b], dre[a][b], 0, 0, 0);
+                                dim[a][b] = __builtin_amdgcn_mfma_f32_32x32x2f32(fa[a], fr[b], dim[a][b], 0, 0, 0);
+                            }
+                    }
+                }
+            }
         }
-    const int r32 = lane & 31, h = lane >> 5;
-    for (long st = 0; st < RING - 1 && st < nstage; st++) issue(st);
-    for (long st = 0; st < nstage; st++) {
-        const long ahead = nstage - 1 - st;
-        if (ahead >= RING - 2) wait_vmcnt<(RING - 2) * NPER>();
-        else if (ahead == 1) wait_vmcnt<NPER>();
-        else wait_vmcnt<0>();
-        asm volatile("s_barrier" ::: "memory");
-        if (st + RING - 1 < nstage) issue(st + RING - 1);
+        wait_vmcnt<0>();   // the loads past the last stage have landed before the ring is reused or freed
         if (!active) continue;
-        const E (*sa)[TSZ] = slab[st % RING][0];
-        const E (*sb)[TSZ] = slab[st % RING][1];
-        if constexpr (PREC == FX_PREC_F16) {
+        float* dst = acc + (((size_t)c * ntl + tile) * NBT + w * NBW * NBW) * (32 * 64);
 #pragma unroll
-            for (int ks = 0; ks < KT / 8; ks++) {
-                fx_half8 fa[NBW], fb[NBW], fr[NBW];
+        for (int a = 0; a < NBW; a++)
 #pragma unroll
-                for (int a = 0; a < NBW; a++) {
-                    fx_int4v ia;
-                    const int row = (NBW * qi + a) * 32 + r32;
+            for (int b = 0; b < NBW; b++) {
+                if (diag && qi == qj && a > b) continue;
+                float* d = dst + (a * NBW + b) * (32 * 64);
 #pragma unroll
-                    for (int q = 0; q < 4; q++) ia[q] = (int)sa[ks * 8 + 4 * h + q][row];
-                    fa[a] = __builtin_bit_cast(fx_half8, ia);
+                for (int r = 0; r < 16; r++) {
+                    d[r * 64 + lane] += dre[a][b][r];
+                    d[(16 + r) * 64 + lane] += dim[a][b][r];
                 }
-#pragma unroll
-                for (int b = 0; b < NBW; b++) {
-                    fx_int4v ib, ir;
-                    const int row = (NBW * qj + b) * 32 + r32;
-#pragma unroll
-                    for (int q = 0; q < 4; q++) {
-                        uint32_t v = sb[ks * 8 + 4 * h + q][row];
-                        ib[q] = (int)v;
-                        ir[q] = (int)rot_h2(v);
-                    }
-                    fb[b] = __builtin_bit_cast(fx_half8, ib);
-                    fr[b] = __builtin_bit_cast(fx_half8, ir);
-                }
-#pragma unroll
-                for (int a = 0; a < NBW; a++)
-#pragma unroll
-                    for (int b = 0; b < NBW; b++) {
-                        if (diag && qi == qj && a > b) continue;
-                        dre[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_f16(fa[a], fb[b], dre[a][b], 0, 0, 0);
-                        dim[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_f16(fa[a], fr[b], dim[a][b], 0, 0, 0);
-                    }
             }
-        } else {
-#pragma unroll
-            for (int ks = 0; ks < KT; ks++) {
-                float fa[NBW], fb[NBW], fr[NBW];
-#pragma unroll
-                for (int a = 0; a < NBW; a++) {
-                    const float2 v = sa[ks][(NBW * qi + a) * 32 + r32];
-                    fa[a] = h ? v.y : v.x;
-                }
-#pragma unroll
-                for (int b = 0; b < NBW; b++) {
-                    const float2 v = sb[ks][(NBW * qj + b) * 32 + r32];
-                    fb[b] = h ? v.y : v.x;
-                    fr[b] = h ? v.x : -v.y;
-                }
-#pragma unroll
-                for (int a = 0; a < NBW; a++)
-#pragma unroll
-                    for (int b = 0; b < NBW; b++) {
-                        if (diag && qi == qj && a > b) continue;
-                        dre[a][b] = __builtin_amdgcn_mfma_f32_32x32x2f32(fa[a], fb[b], dre[a][b], 0, 0, 0);
-                        dim[a][b] = __builtin_amdgcn_mfma_f32_32x32x2f32(fa[a], fr[b], dim[a][b], 0, 0, 0);
-                    }
-            }
-        }
     }
-    if (!active) return;
-    float* dst = acc + (((size_t)c * ntl + tile) * NBT + w * NBW * NBW) * (32 * 64);
-#pragma unroll
-    for (int a = 0; a < NBW; a++)
-#pragma unroll
-        for (int b = 0; b < NBW; b++) {
-            if (diag && qi == qj && a > b) continue;
-            float* d = dst + (a * NBW + b) * (32 * 64);
-#pragma unroll
-            for (int r = 0; r < 16; r++) {
-                d[r * 64 + lane] += dre[a][b][r];
-                d[(16 + r) * 64 + lane] += dim[a][b][r];
-            }
-        }
 }
 
 // ---------------------------------------------------------------------------
@@ -895,13 +966,68 @@ XKind xengine_select(int in_type, int ts, const void* X, long nt, int ninp, bool
     return XK_NONE;
 }
 
+// Lockstep groups per XCD for the large-input ring engine (0: run it
+// ungrouped): every group of ntl workgroups must be resident at once, so the
+// groups of an XCD fill at most its CUs times the engine's workgroups per CU;
+// a round is XLP whole stages.
+template <int PREC>
+int ring2_groups_per_xcd(int ntl, int nchan, long nt)
+{
+    static const int slots = [] {
+        int dev = 0, ncu = 0, occ = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_xengine_ring2<PREC, true>, XNT, 0) != hipSuccess) {
+            (void)hipGetLastError();
+            return 0;
+        }
+        return ncu / 8 * occ;
+    }();
+    if (ntl < 2 || ntl > 64 || nchan < 8 || nt % ((long)XTraits<PREC>::KT / 2 * XLP) != 0) return 0;
+    return std::min(slots / ntl, nchan / 8);
+}
+
+int ring2_groups(XKind kind, int ninp, int ts, int nchan, long nt, bool lock)
+{
+    if (!lock || (kind != XK_RING2_F16 && kind != XK_RING2_F32)) return 0;
+    const int ntl = ntiles_ts(ninp, ts);
+    return kind == XK_RING2_F16 ? ring2_groups_per_xcd<FX_PREC_F16>(ntl, nchan, nt)
+                                : ring2_groups_per_xcd<FX_PREC_F32>(ntl, nchan, nt);
+}
+
 } // namespace
 
-const char* fxk_xengine_name(int in_type, int ts, const void* X, long nt, int ninp, bool ring)
+// FX_XENGINE_LOCK=1 runs the large-input ring engine in lockstep groups (the
+// default of a new correlator; fx_corr_set_xengine overrides it)
+bool fxk_xengine_lock_default()
 {
-    switch (xengine_select(in_type, ts, X, nt, ninp, ring)) {
-    case XK_RING2_F16: return "k_xengine_ring2<F16>";
-    case XK_RING2_F32: return "k_xengine_ring2<F32>";
+    static const bool on = [] {
+        const char* e = getenv("FX_XENGINE_LOCK");
+        return e && e[0] == '1';
+    }();
+    return on;
+}
+
+int fxk_xengine_prog_words() { return 8 * 64; }
+
+// stages a lockstep sibling may trail (FX_XENGINE_LAG overrides the 8)
+static unsigned xlag()
+{
+    static const unsigned v = [] {
+        const char* e = getenv("FX_XENGINE_LAG");
+        return e && atoi(e) > 0 ? (unsigned)atoi(e) : 8u;
+    }();
+    return v;
+}
+
+const char* fxk_xengine_name(int in_type, int ts, const void* X, int nchan, long nt, int ninp, bool ring,
+                             bool lock)
+{
+    const XKind kind = xengine_select(in_type, ts, X, nt, ninp, ring);
+    const bool grouped = ring2_groups(kind, ninp, ts, nchan, nt, lock) > 0;
+    switch (kind) {
+    case XK_RING2_F16: return grouped ? "k_xengine_ring2<F16> (lockstep)" : "k_xengine_ring2<F16>";
+    case XK_RING2_F32: return grouped ? "k_xengine_ring2<F32> (lockstep)" : "k_xengine_ring2<F32>";
     case XK_RING_F16: return "k_xengine_ring<F16>";
     case XK_RING_F32: return "k_xengine_ring<F32>";
     case XK_F32_128: return "k_xengine<F32, 128>";
@@ -913,15 +1039,36 @@ const char* fxk_xengine_name(int in_type, int ts, const void* X, long nt, int ni
 }
 
 int fxk_xengine(hipStream_t s, int in_type, int ts, const void* X, int nchan, long nt, int ninp, float* acc,
-                bool ring)
+                bool ring, unsigned* prog)
 {
     const long nblk = xcd_grid((long)ntiles_ts(ninp, ts) * nchan);
     const dim3 g((unsigned)nblk), b(XNT);
     const uint32_t* xh = reinterpret_cast<const uint32_t*>(X);
     const float2* xf = reinterpret_cast<const float2*>(X);
-    switch (xengine_select(in_type, ts, X, nt, ninp, ring)) {
-    case XK_RING2_F16: hipLaunchKernelGGL(k_xengine_ring2<FX_PREC_F16>, g, b, 0, s, xh, nchan, nt, ninp, acc); break;
-    case XK_RING2_F32: hipLaunchKernelGGL(k_xengine_ring2<FX_PREC_F32>, g, b, 0, s, xf, nchan, nt, ninp, acc); break;
+    const XKind kind = xengine_select(in_type, ts, X, nt, ninp, ring);
+    const int ngx = ring2_groups(kind, ninp, ts, nchan, nt, prog != nullptr);
+    if (ngx > 0) {
+        const int ntl = ntiles_ts(ninp, ts);
+        const int rc = fxk_fill32(s, prog, 0u, (size_t)8 * ngx * ntl);
+        if (rc) return rc;
+        const dim3 gl((unsigned)(8 * ngx * ntl));
+        if (kind == XK_RING2_F16)
+            hipLaunchKernelGGL((k_xengine_ring2<FX_PREC_F16, true>), gl, b, 0, s, xh, nchan, nt, ninp, acc, prog, ngx,
+                               xlag());
+        else
+            hipLaunchKernelGGL((k_xengine_ring2<FX_PREC_F32, true>), gl, b, 0, s, xf, nchan, nt, ninp, acc, prog, ngx,
+                               xlag());
+        return hipGetLastError() == hipSuccess ? FX_OK : FX_ERR_HIP;
+    }
+    switch (kind) {
+    case XK_RING2_F16:
+        hipLaunchKernelGGL((k_xengine_ring2<FX_PREC_F16, false>), g, b, 0, s, xh, nchan, nt, ninp, acc, nullptr, 0,
+                           0u);
+        break;
+    case XK_RING2_F32:
+        hipLaunchKernelGGL((k_xengine_ring2<FX_PREC_F32, false>), g, b, 0, s, xf, nchan, nt, ninp, acc, nullptr, 0,
+                           0u);
+        break;
     case XK_RING_F16: hipLaunchKernelGGL(k_xengine_ring<FX_PREC_F16>, g, b, 0, s, xh, nchan, nt, acc); break;
     case XK_RING_F32: hipLaunchKernelGGL(k_xengine_ring<FX_PREC_F32>, g, b, 0, s, xf, nchan, nt, acc); break;
     case XK_F32_128: hipLaunchKernelGGL((k_xengine<FX_PREC_F32, 128>), g, b, 0, s, xf, nchan, nt, ninp, acc); break;

@@ -20,7 +20,7 @@ LIB_PATH = os.environ.get("CELLSPE_FX_LIB") or os.path.join(HERE, "libcellspe_fx
 
 FX_OK = 0
 FX_PREC_F32, FX_PREC_F16, FX_PREC_FP8, FX_PREC_FP8_2BIT, FX_PREC_FP4_2BIT = 0, 1, 2, 3, 4
-FX_XENGINE_AUTO, FX_XENGINE_PREFETCH = 0, 1
+FX_XENGINE_AUTO, FX_XENGINE_PREFETCH, FX_XENGINE_LOCKSTEP = 0, 1, 2
 FX_FSTAGE_AUTO, FX_FSTAGE_PERSISTENT, FX_FSTAGE_BLOCKS = 0, 1, 2
 FX_BATCH_EAGER, FX_BATCH_GRAPH, FX_BATCH_PERSISTENT, FX_BATCH_PHASED = 0, 1, 2, 3
 (FX_WIN_NONE, FX_WIN_SIN2, FX_WIN_COSINE, FX_WIN_COSINE2, FX_WIN_HAMMING, FX_WIN_BLACKMAN,
@@ -539,8 +539,10 @@ class Correlator:
         check(load().fx_corr_tune(self.handle, sub_spectra, xtile))
 
     def set_xengine(self, kind):
-        """FX_XENGINE_AUTO (LDS-ring engines where they apply) or
-        FX_XENGINE_PREFETCH (the register-prefetch engine for every shape)."""
+        """FX_XENGINE_AUTO (LDS-ring engines where they apply),
+        FX_XENGINE_LOCKSTEP (the same, the large-input one in lockstep groups)
+        or FX_XENGINE_PREFETCH (the register-prefetch engine for every
+        shape)."""
         check(load().fx_corr_set_xengine(self.handle, kind))
 
     def set_fstage(self, kind):

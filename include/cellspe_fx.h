@@ -393,8 +393,15 @@ int fx_corr_tune(fx_corr* corr, long sub_spectra, int xtile);
  * 128 inputs with a multiple of KT spectra per call (32 FP16, 16 FP32), or a
  * multiple of 128 inputs with a multiple of KT/2 -- and the register-prefetch
  * engine otherwise; FX_XENGINE_PREFETCH runs the register-prefetch engine for
- * every shape.  Both give bit-identical sums. */
-enum { FX_XENGINE_AUTO = 0, FX_XENGINE_PREFETCH = 1 };
+ * every shape.  FX_XENGINE_LOCKSTEP (or FX_XENGINE_LOCK=1 in the environment
+ * at fx_corr_create) runs the ring engines as AUTO does, the large-input one
+ * -- a multiple of 128 inputs (2 to 64 tiles per channel), a multiple of
+ * 4 KT/2 spectra -- in lockstep groups: the workgroups of one channel's tiles
+ * on one XCD, kept within a few stages of each other, so the input blocks
+ * they share are fetched from HBM once instead of about three times (slower
+ * on MI355X all the same: the engine is not bound by those fetches).  All give
+ * bit-identical sums. */
+enum { FX_XENGINE_AUTO = 0, FX_XENGINE_PREFETCH = 1, FX_XENGINE_LOCKSTEP = 2 };
 int fx_corr_set_xengine(fx_corr* corr, int kind);
 /* Form of the direct F kernel (FFT lengths 4096-16384): FX_FSTAGE_AUTO
  * (default) runs its persistent form, one workgroup per CU walking its XCD's

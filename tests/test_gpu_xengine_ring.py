@@ -7,7 +7,9 @@
   64 spectra;
 * k_xengine_ring2<F16> / <F32>: a multiple of 128 inputs (several tiles per
   channel) -- the large-N engine, on 2-bit channelized voltages of 256 and 512
-  inputs.
+  inputs, ungrouped (the default) and in lockstep groups (FX_XENGINE_LOCKSTEP:
+  the tiles of a channel advance together on one XCD), with channels spread
+  unevenly over the groups and rounds that straddle channels.
 
 Each is checked against the oracle (FP32 within 1e-5 of the FP32 restatement
 of genericAddProduct_32fc_conj, architecture.h:427-434, and of the FP64 sums;
@@ -89,28 +91,30 @@ def test_ring_engine_mid_shape(ctx, mid64, prec, name):
     assert nrel(two, ref64) < (TOL32 if prec == fxlib.FX_PREC_F32 else 1e-3)
 
 
-@pytest.mark.parametrize("I", [256, 512])
+@pytest.mark.parametrize("I,C,T", [(256, 128, 64), (512, 128, 64), (512, 128, 256)])
 @pytest.mark.parametrize("prec", [fxlib.FX_PREC_F32, fxlib.FX_PREC_F16])
-def test_ring2_engine(ctx, I, prec):
-    C, T = 128, 64
+def test_ring2_engine(ctx, I, C, T, prec):
     codes = ctx.alloc(C * I * T)
-    ctx.synth_channelized(7 + I, C, T, I, 0.01, 0.9816, codes, input_major=False)
+    ctx.synth_channelized(7 + I + C, C, T, I, 0.01, 0.9816, codes, input_major=False)
     esz = 8 if prec == fxlib.FX_PREC_F32 else 4
     dx = ctx.alloc(C * T * I * esz)
     ctx.codes_to_chan(codes, False, C, T, I, prec, dx)
+    p = "F32" if prec == fxlib.FX_PREC_F32 else "F16"
+    names = {fxlib.FX_XENGINE_AUTO: f"k_xengine_ring2<{p}>",
+             fxlib.FX_XENGINE_LOCKSTEP: f"k_xengine_ring2<{p}> (lockstep)",
+             fxlib.FX_XENGINE_PREFETCH: f"k_xengine<{p}, 128>"}
     got = {}
-    for kind in (fxlib.FX_XENGINE_AUTO, fxlib.FX_XENGINE_PREFETCH):
+    for kind, want in names.items():
         corr = fxlib.Correlator(ctx, I, 2 * C, 8, prec)
         corr.set_scales(np.ones(C, np.float32), np.ones(I, np.float32))
         corr.set_xengine(kind)
-        want = ("k_xengine_ring2<%s>" if kind == fxlib.FX_XENGINE_AUTO else "k_xengine<%s, 128>") % (
-            "F32" if prec == fxlib.FX_PREC_F32 else "F16")
         assert corr.xengine_name(T, dx) == want
         corr.xengine(dx, T)
         got[kind] = corr.visibilities()
         corr.close()
     ring, pre = got[fxlib.FX_XENGINE_AUTO], got[fxlib.FX_XENGINE_PREFETCH]
     assert np.array_equal(ring.view(np.uint32), pre.view(np.uint32))
+    assert np.array_equal(got[fxlib.FX_XENGINE_LOCKSTEP].view(np.uint32), pre.view(np.uint32))
     cb = codes.download(np.uint8).reshape(C, T, I)
     lut = O.LUT.astype(np.float64)
     X = lut[cb & 3] + 1j * lut[(cb >> 2) & 3]              # [c][t][i], level-table values
