@@ -101,7 +101,7 @@ __global__ __launch_bounds__(XNT, 2) void k_xengine_ring(const typename XTraits<
     if (lb >= (long)nchan) return;
     const int c = (int)lb;
     const int tid = threadIdx.x;
-    const int w = tid >> 6, lane = tid & 63;
+    const int w = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
     const int qi = w >> 1, qj = w & 1;
     const bool active = qi <= qj;
     const char* base = reinterpret_cast<const char*>(X + (size_t)c * (size_t)nt * TSZ);
@@ -282,7 +282,7 @@ __global__ __launch_bounds__(XNT, 2) void k_xengine_ring2(const typename XTraits
     tile_coords(tile, nb, ti, tj);
     const bool diag = ti == tj;
     const int tid = threadIdx.x;
-    const int w = tid >> 6, lane = tid & 63;
+    const int w = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
     const int qi = w >> 1, qj = w & 1;
     const bool active = !(diag && qi > qj);
     const long nstage = nt / KT;
@@ -295,21 +295,29 @@ __global__ __launch_bounds__(XNT, 2) void k_xengine_ring2(const typename XTraits
     unsigned sg = 0;        // stages this workgroup has consumed, all channels
     for (int c = c0; c < nchan; c += cstep) {
         if (c != c0) __syncthreads();   // every wave is done with the previous channel's ring
-        const E* base = X + (size_t)c * (size_t)nt * (size_t)ninp;
+        // this lane's source of each of its NPER loads in stage 0; stage s is
+        // s KT time rows further
+        const E* lsrc[NPER];
+#pragma unroll
+        for (int i = 0; i < NPER; i++) {
+            const int blk = w * NPER + i;                    // 0 .. 2 BPS - 1
+            const int sl = blk / BPS, b = blk - sl * BPS;
+            lsrc[i] = X + ((size_t)c * (size_t)nt + (size_t)(b * RPB + trow)) * (size_t)ninp + (sl ? tj : ti) * TSZ +
+                      col;
+        }
+        const size_t sstep = (size_t)KT * (size_t)ninp;     // samples per stage of the channel
 
         // the loads of stage st into its ring slot; past the last stage the
         // last stage's samples again (every stage issues NPER loads per wave,
         // so one wait count fits every stage, and the compiler can count them)
         auto issue = [&](long st) {
-            const long ss = st < nstage ? st : nstage - 1;
+            const size_t so = (size_t)(st < nstage ? st : nstage - 1) * sstep;
 #pragma unroll
             for (int i = 0; i < NPER; i++) {
-                const int blk = w * NPER + i;                // 0 .. 2 BPS - 1
+                const int blk = w * NPER + i;
                 const int sl = blk / BPS, b = blk - sl * BPS;
-                const long t = ss * KT + b * RPB + trow;
-                const char* src =
-                    reinterpret_cast<const char*>(base + (size_t)t * ninp + (sl ? tj : ti) * TSZ + col);
                 char* dst = reinterpret_cast<char*>(&slab[st % RING][sl][0][0]) + b * 1024;
+                const char* src = reinterpret_cast<const char*>(lsrc[i] + so);
                 __builtin_amdgcn_global_load_lds(src, (__attribute__((address_space(3))) void*)dst, 16, 0, 0);
             }
         };

@@ -47,8 +47,13 @@ __device__ inline void tile_coords(int tile, int nb, int& bi, int& bj)
 
 __device__ __forceinline__ uint32_t rot_h2(uint32_t v)
 {
-    // (re, im) -> (-im, re) on a packed half2: swap halves, flip the new low sign
-    return ((v >> 16) | (v << 16)) ^ 0x00008000u;
+    // (re, im) -> (-im, re) on a packed half2 in one packed multiply: the low
+    // result is the high half times -1, the high result the low half times +1
+    // (op_sel picks the halves) -- exact for every finite value and signed
+    // zero, the same bits as swapping the halves and flipping the new low sign
+    uint32_t r;
+    asm("v_pk_mul_f16 %0, %1, %2 op_sel:[1,0] op_sel_hi:[0,1]" : "=v"(r) : "v"(v), "s"(0x3C00BC00u));
+    return r;
 }
 
 template <int PREC> struct XTraits;
