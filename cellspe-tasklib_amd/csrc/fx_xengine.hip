@@ -702,8 +702,11 @@ __global__ __launch_bounds__(X8NT, 1) void k_xengine4(const uint8_t* __restrict_
     // it shares its SIMD with an idle or a one-block wave, so the stage time is unchanged.
     const int mode = __builtin_amdgcn_readfirstlane(!(diag && qa > 2 * qb) ? 2 : !(diag && qa > 2 * qb + 1) ? 1 : 0);
 
-    // staging: thread tid moves the 16-sample chunk (row tid / 4, part tid % 4) of each slab
-    const int row = tid >> 2, part = tid & 3;
+    // staging: lane l of wave w moves the 16-sample chunk (row 16 w + l % 16, part l / 16) of each slab:
+    // a quarter-wave stores one part of 16 rows (80-byte stride: on distinct LDS banks), and a wave
+    // still loads 16 whole 64-byte row segments from HBM
+    static_assert(X8NT == 512 && TS == 128, "8 waves x 16 rows cover the tile's rows");
+    const int row = ((tid >> 6) << 4) | (tid & 15), part = (tid >> 4) & 3;
     const uint8_t* src[2];
     bool rok[2];
 #pragma unroll
