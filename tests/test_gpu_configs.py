@@ -11,7 +11,9 @@
   against the FP32 oracle (<= 1e-5) and the tone peaks at the seeded bins;
 * mid-size correlator shape: 128 inputs, 8192-point FFT, FP64 phase model
   with fractional delays 0-50 samples, 16 spectra -- FP32 against the FP32
-  oracle, FP16 against the FP64 oracle (<= 1e-3);
+  oracle, FP16 against the FP64 oracle (<= 1e-3); and the whole 65536-spectrum
+  integration: FP16 against FP32, autos, split-run additivity, and the last
+  64 spectra against the oracle;
 * large-N shape: 512 inputs, 8192 spectra of 2-bit channelized voltages
   (128 channels here), exact FP8 engine and FP16 engine against FP64 sums on
   sampled channels (the exact engine bit for bit against the oracle's exact
@@ -129,6 +131,75 @@ def test_mid_correlator_shape(ctx):
     ref64 = O.corr_f64(streams, n, T, models, nthreads=16)
     e32, e16 = nrel(out[0], ref32), nrel(out[1], ref64)
     print(f"mid shape: f32 vs FP32 oracle {e32:.2e}, f16 vs FP64 {e16:.2e}")
+    assert e32 < TOL32
+    assert e16 < 1e-3
+
+
+def test_mid_correlator_full_run(ctx):
+    """The mid-size correlator over its whole integration, as the benchmark
+    runs it: 128 inputs x 2^29 samples (16 GiB of 2-bit words generated on the
+    device), 8192-point FFT, 65536 spectra in 4096-spectrum chunks, seeded
+    delays 0-50 samples with delay and fringe rates.  Size-independent checks
+    at full size: the FP16 chain against the FP32 chain over all 8256 products
+    x 4096 channels (<= 1e-3); autos real and positive; the run split at
+    spectrum 32768 into two runs sums to the whole run; and a 64-spectrum
+    window at the end of the integration (t = 65472 .., where the FP64 phase
+    model's rate terms are largest) against the oracle on the same words, with
+    the model's constant terms moved to the window's start (FP32 <= 1e-5
+    against the FP32 oracle, FP16 <= 1e-3 against FP64)."""
+    stations, n, T, chunk = 64, 8192, 65536, 4096
+    I, C = 2 * stations, n // 2
+    P = I * (I + 1) // 2
+    rng = np.random.default_rng(20240229)
+    tau = rng.uniform(0.0, 50.0, stations)
+    fr0 = rng.uniform(-np.pi, np.pi, stations)
+    fr1 = rng.uniform(-1e-3, 1e-3, stations)
+    models = [(float(tau[s]), 1e-7 * s, float(fr0[s]), float(fr1[s])) for s in range(stations) for _ in range(2)]
+    nw = T * n // 16
+    dw = ctx.alloc(I * nw * 4)
+    lib = fxlib.load()
+    for i in range(I):
+        fxlib.check(lib.fx_synth_station(ctx.handle, 20240229, i, 0.02, int(tau[i // 2]), 0.0, 0.0, 0.9816, T * n,
+                                         dw.ptr.value + i * nw * 4))
+    ctx.sync()
+
+    def run(prec, t0, nt, ch):
+        corr = fxlib.Correlator(ctx, I, n, ch, prec, use_phase=True)
+        corr.set_phase_model(models)
+        corr.process(dw, nw, t0, nt)
+        v = corr.visibilities()
+        corr.close()
+        return v
+
+    v16 = run(fxlib.FX_PREC_F16, 0, T, chunk)
+    v32 = run(fxlib.FX_PREC_F32, 0, T, chunk)
+    e = nrel(v16, v32)
+    ia = np.arange(I)
+    autos = ia * I - ia * (ia - 1) // 2
+    a16, a32 = v16[autos].astype(np.complex128), v32[autos].astype(np.complex128)
+    va = run(fxlib.FX_PREC_F16, 0, T // 2, chunk)
+    vb = run(fxlib.FX_PREC_F16, T // 2, T // 2, chunk)
+    esplit = nrel(va.astype(np.complex128) + vb, v16)
+    del va, vb
+    print(f"mid full run: f16 vs f32 {e:.2e}, split vs whole {esplit:.2e}")
+    assert e < 1e-3
+    assert esplit < 1e-6
+    for a in (a16, a32):
+        assert np.all(a.real > 0)
+        assert np.max(np.abs(a.imag) / a.real) < 1e-5
+    del v16, v32
+    # the window at the end of the integration
+    tw, nt = T - 64, 64
+    wpt = n // 16
+    streams = [dw.download_at((i * nw + tw * wpt) * 4, np.uint32, nt * wpt) for i in range(I)]
+    shifted = [(d0 + d1 * tw, d1, f0 + f1 * tw, f1) for (d0, d1, f0, f1) in models]
+    g32 = run(fxlib.FX_PREC_F32, tw, nt, nt)
+    g16 = run(fxlib.FX_PREC_F16, tw, nt, nt)
+    dw.free()
+    r32 = O.corr_f32(streams, n, nt, shifted, nthreads=16)
+    r64 = O.corr_f64(streams, n, nt, shifted, nthreads=16)
+    e32, e16 = nrel(g32, r32), nrel(g16, r64)
+    print(f"mid full run, last 64 spectra: f32 vs FP32 oracle {e32:.2e}, f16 vs FP64 {e16:.2e}")
     assert e32 < TOL32
     assert e16 < 1e-3
 
