@@ -16,8 +16,8 @@
   64 spectra against the oracle;
 * large-N shape: 512 inputs, 8192 spectra of 2-bit channelized voltages
   (128 channels here), exact FP8 engine and FP16 engine against FP64 sums on
-  sampled channels (the exact engine bit for bit against the oracle's exact
-  integer sums rounded once); and all 1024 channels of the large-N workload
+  sampled channels (the exact FP8 and FP4 engines bit for bit against the
+  oracle's exact integer sums rounded once); and all 1024 channels of the large-N workload
   on a 64-spectrum time slice through the exact FP8 and FP4, FP16 and plain
   FP32 engines, every product and channel against FP64 sums and the exact
   engines bit for bit against the exact sums (FP32 also against the FP32
@@ -212,6 +212,10 @@ def test_large_n_shape(ctx):
     c8.xengine(codes, T)
     v8 = c8.visibilities()
     c8.close()
+    c4 = fxlib.Correlator(ctx, I, 2 * C, T, fxlib.FX_PREC_FP4_2BIT)
+    c4.xengine(codes, T)
+    v4 = c4.visibilities()
+    c4.close()
     x16 = ctx.alloc(C * T * I * 4)
     ctx.codes_to_chan(codes, True, C, T, I, fxlib.FX_PREC_F16, x16)
     c16 = fxlib.Correlator(ctx, I, 2 * C, 8, fxlib.FX_PREC_F16)
@@ -231,6 +235,7 @@ def test_large_n_shape(ctx):
         # 8192-spectrum sums: the exact integer sums rounded once, bit for bit
         exact = O.xengine_2bit_exact_f32(cb.T[None])[:, 0]
         assert np.array_equal(np.ascontiguousarray(v8[:, ch]).view(np.uint32), exact.view(np.uint32))
+        assert np.array_equal(np.ascontiguousarray(v4[:, ch]).view(np.uint32), exact.view(np.uint32))
         assert e16 < 1e-3
 
 
