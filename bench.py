@@ -7,7 +7,7 @@ per-station delays of 0-50 samples, 8192-point real FFT (4096 channels), FP64
 delay/fringe phase model applied after the FFT, 65536 spectra (2^29 samples
 per input, 16 GiB of packed input resident in HBM), all 8256 products.
 
-One step = reset accumulators -> per 4096-spectrum chunk: F kernel (unpack,
+One step = reset accumulators -> per 16384-spectrum chunk: F kernel (unpack,
 FFT, rotate, scale, convert, store in the engine layout) -> X-engine -> dump
 [p][c] complex64 visibilities.  HIP events around the launches of the first
 chunk give the per-kernel durations inside the timed region.  The headline
@@ -807,7 +807,8 @@ def main():
     ap.add_argument("--stations", type=int, default=64)
     ap.add_argument("--nfft", type=int, default=8192)
     ap.add_argument("--spectra", type=int, default=65536)
-    ap.add_argument("--chunk", type=int, default=4096)
+    ap.add_argument("--chunk", type=int, default=16384,
+                    help="spectra per chunk of the mid-size correlator (16384: 34 GB of FP16 channels, 69 GB FP32)")
     ap.add_argument("--rho", type=float, default=0.02)
     ap.add_argument("--seed", type=int, default=20240229)
     ap.add_argument("--large-inputs", type=int, default=512)
