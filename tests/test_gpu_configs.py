@@ -138,7 +138,7 @@ def test_mid_correlator_shape(ctx):
 def test_mid_correlator_full_run(ctx):
     """The mid-size correlator over its whole integration, as the benchmark
     runs it: 128 inputs x 2^29 samples (16 GiB of 2-bit words generated on the
-    device), 8192-point FFT, 65536 spectra in 4096-spectrum chunks, seeded
+    device), 8192-point FFT, 65536 spectra in 16384-spectrum chunks, seeded
     delays 0-50 samples with delay and fringe rates.  Size-independent checks
     at full size: the FP16 chain against the FP32 chain over all 8256 products
     x 4096 channels (<= 1e-3); autos real and positive; the run split at
@@ -147,7 +147,7 @@ def test_mid_correlator_full_run(ctx):
     model's rate terms are largest) against the oracle on the same words, with
     the model's constant terms moved to the window's start (FP32 <= 1e-5
     against the FP32 oracle, FP16 <= 1e-3 against FP64)."""
-    stations, n, T, chunk = 64, 8192, 65536, 4096
+    stations, n, T, chunk = 64, 8192, 65536, 16384
     I, C = 2 * stations, n // 2
     P = I * (I + 1) // 2
     rng = np.random.default_rng(20240229)
